@@ -1,0 +1,136 @@
+#!/usr/bin/env python3
+"""Benchmark of the mapping hot path on one MI355X.
+
+Workload (default): 256 Mbp uniform-random genome with 5% inserted repeat
+families (300 bp units x 500 copies at 10% divergence), 12-mer index; 1M
+reads of 150 bp sampled from both strands with 2% substitutions and 0.5%
+1-3 bp indels; e = 5.  A step maps the whole 1M-read batch (reads resident
+in HBM) through seed lookup, candidate batches, banded Myers and record
+emission.  Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+MYERS_OPS_PER_STEP = 20          # int32 ops per band step (X 1, D0 4, HN 1, HP 3, >>1 1, VN 1, VP 3, tally 2, Peq 4)
+VALU_PEAK_OPS = 256 * 4 * 16 * 2.4e9   # CUs x SIMDs x lanes x clock = 39.3e12 int32 ops/s
+
+CONFIGS = {
+    "repeat5": dict(n=256_000_000, kind=1, frac=0.05, reads=1_000_000, lmin=150, lmax=150, err=(0.025, 0.025), e=5,
+                    desc="256 Mbp uniform + 5% repeat families (300 bp x 500 copies, 10% div), 12-mer index; "
+                         "1M x 150 bp reads, 2% sub + 0.5% 1-3 bp indel; e=5"),
+    "small": dict(n=4_000_000, kind=0, frac=0.0, reads=100_000, lmin=150, lmax=150, err=(0.025, 0.025), e=5,
+                  desc="4 Mbp uniform, 12-mer index; 100k x 150 bp reads, 2% sub + 0.5% indel; e=5"),
+}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="repeat5", choices=sorted(CONFIGS))
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target length of the CPU baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(mapping shards by reads across GPUs; see the sharding notes)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import swmapper_amd as swm
+    cfg = CONFIGS[args.config]
+    log = lambda *a: print("[bench]", *a, file=sys.stderr, flush=True)
+    assert torch.cuda.is_available(), "bench needs a HIP device"
+    dev = torch.device("cuda:0")
+
+    t0 = time.time()
+    g = swm.synth_genome(cfg["n"], seed=20240601, kind=cfg["kind"], frac=cfg["frac"])
+    log(f"genome {cfg['n']} bp in {time.time() - t0:.1f}s")
+    ix = swm.build_index(g, 12)
+    log(f"index: {ix.positions.size} positions, GPU build {ix.build_ms:.1f} ms")
+    reads, lens, _, _ = swm.synth_reads(g, cfg["reads"], len_min=cfg["lmin"], len_max=cfg["lmax"],
+                                        err_min=cfg["err"][0], err_max=cfg["err"][1], indel_share=0.2,
+                                        seed=777, stride=256)
+    log(f"reads {cfg['reads']} in {time.time() - t0:.1f}s")
+    m = swm.Mapper.from_index(ix)
+    n = cfg["reads"]
+    d_reads = torch.from_numpy(reads).to(dev)
+    d_lens = torch.from_numpy(lens.view(np.int32)).to(dev)
+    d_out = torch.zeros((n, 3), dtype=torch.int32, device=dev)   # 12-byte swm_read_result_t
+    e = cfg["e"]
+
+    def step():
+        m.map_device(d_reads.data_ptr(), 256, d_lens.data_ptr(), n, e, d_out.data_ptr())
+        return m.stats()
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    lat, st_all = [], []
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        ts = time.perf_counter()
+        st_all.append(step())
+        lat.append((time.perf_counter() - ts) * 1e3)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t_start
+    nv = st_all[-1]["n_verifications"]
+    sum_l_ver = nv * cfg["lmin"] if cfg["lmin"] == cfg["lmax"] else None
+    verif_per_s = nv * args.steps / elapsed
+    reads_per_s = n * args.steps / elapsed
+    ms_myers = statistics.mean(s["ms_myers"] for s in st_all)
+    for s in st_all:
+        log(" ".join(f"{k}={s[k]:.2f}" for k in ("ms_seed", "ms_plan", "ms_compact", "ms_myers", "ms_emit", "ms_total")))
+    roof = None
+    if sum_l_ver is not None and ms_myers > 0:
+        ach = MYERS_OPS_PER_STEP * sum_l_ver / (ms_myers / 1e3)
+        roof = {"bound": "valu", "kernel": "k_myers", "achieved": round(ach / 1e12, 3),
+                "peak": round(VALU_PEAK_OPS / 1e12, 2), "unit": "Tops/s (int32)",
+                "frac": round(ach / VALU_PEAK_OPS, 4), "traffic": None,
+                "algorithmic_ops_per_launch": MYERS_OPS_PER_STEP * sum_l_ver, "avg_launch_ms": round(ms_myers, 3)}
+
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib as orc
+        ref = orc.OracleRef(12, ix.hash_count, ix.positions, ix.pos_lseeds, ix.base_2bits)
+        threads = min(16, os.cpu_count() or 1)
+        probe = min(n, 4000)
+        tp = time.perf_counter()
+        ref.map(reads[:probe], lens[:probe], e, threads)
+        dtp = time.perf_counter() - tp
+        ns = int(min(n, max(probe, probe * args.cpu_seconds / max(dtp, 1e-3))))
+        tc = time.perf_counter()
+        _, nv_cpu = ref.map(reads[:ns], lens[:ns], e, threads)
+        dtc = time.perf_counter() - tc
+        cpu = {"value": round(nv_cpu / dtc, 1), "unit": "verifications/s", "cores": threads, "kind": "port",
+               "reads_per_s": round(ns / dtc, 1),
+               "sample": f"first {ns} reads of the same batch, oracle (C restatement, OpenMP dynamic,1), "
+                         f"{dtc:.1f}s"}
+    res = {
+        "metric": "candidate verifications/sec + reads/sec at e=5; p50/p99 batch latency",
+        "value": round(verif_per_s, 1), "unit": "verifications/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(elapsed * 1e3 / args.steps, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "u32", "data": "synthetic",
+        "reads_per_s": round(reads_per_s, 1), "verifications_per_step": nv,
+        "batch_latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
+                             "p99": round(float(np.percentile(lat, 99)), 3)},
+        "config": {"workload": cfg["desc"], "mode": "plain (one launch per stage)", "reads": n, "e": e},
+        "roofline": roof, "cpu_baseline": cpu,
+    }
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

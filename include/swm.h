@@ -1,0 +1,139 @@
+/*
+ * swm.h -- C ABI of the MI355X mapping hot path (SWMapper drop-in).
+ *
+ * The reference's boundary for this path is
+ *   - the index files written by serial_index.c (serial_index.c:73-138,393-408)
+ *     and read by load_ref_info (mapper_mpe.c:198-314);
+ *   - athread_spawn(find_positions, &ref_info_t) / athread_join()
+ *     (mapper_mpe.c:972,1036,1056,1088; kernel mapper_cpe.c:2592-2663),
+ *     with the parameter block ref_info_t (fm_define.h:22-55);
+ *   - the mapper options -l -e -s -p (mapper_mpe.c:717-760) and the index
+ *     builder options -i -p -s (serial_index.c:819-846).
+ * Each entry point below names the reference interface it replaces.
+ *
+ * All functions return SWM_OK (0) or a negative SWM_E* status.  No function
+ * prints or aborts.  A context is safe to use from one host thread at a time;
+ * batches submitted on one context run on that context's stream in order.
+ */
+#ifndef SWM_H
+#define SWM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SWM_OK 0
+#define SWM_EINVAL -1      /* bad argument (option out of range, NULL pointer) */
+#define SWM_EIO -2         /* index or sequence file missing, short or mis-sized */
+#define SWM_ENOMEM -3      /* host or device allocation failed */
+#define SWM_EDEVICE -4     /* HIP runtime error or no device */
+#define SWM_EBUSY -5       /* serving loop: no admission possible right now */
+
+/* Per-read result: the four per-read outputs of the mapping path.
+ * count = records emitted over both strands (the reference's cigar records,
+ * duplicates included); best = minimum edit distance, first in emission order
+ * (forward strand first); pos = 1-based genome coordinate of that record
+ * (cigar->location, mapper_cpe.c:286,484).  Unmapped: count 0, pos
+ * UINT32_MAX, strand 0xFF, dist 0xFF. */
+typedef struct swm_read_result {
+    uint32_t pos;
+    uint8_t strand;
+    uint8_t dist;
+    uint16_t pad;
+    uint32_t count;
+} swm_read_result_t;
+
+/* In-memory index, the exact content of the reference's index files
+ * (.base_2bits, .base_3bits, .hash_count, .hash_offset, .positions,
+ * .pos_lseeds).  Arrays are owned by the library when produced by it. */
+typedef struct swm_index_host {
+    uint32_t k;
+    uint64_t n_bases;
+    uint64_t n_words;          /* ceil(n_bases/32) */
+    uint32_t *base_2bits;      /* 2*n_words */
+    uint32_t *base_3bits;      /* n_words */
+    uint32_t *hash_count;      /* 4^k */
+    uint32_t *hash_offset_end; /* 4^k + 1 (END offsets, as serial_index.c writes) */
+    uint32_t *positions;       /* n_pos */
+    uint16_t *pos_lseeds;      /* n_pos */
+    uint64_t n_pos;
+} swm_index_host_t;
+
+/* Timing/counters of the last batch on a context. */
+typedef struct swm_stats {
+    uint64_t n_reads;
+    uint64_t n_verifications;  /* (read, strand, candidate) windows run through banded Myers */
+    uint64_t n_records;
+    double ms_seed, ms_plan, ms_compact, ms_myers, ms_emit, ms_total;
+} swm_stats_t;
+
+typedef struct swm_ctx swm_ctx_t;
+
+/* ---- index build (replaces serial_index.c main :818-865,
+ *      construct_hash_index :359-445) ---- */
+
+/* FASTA -> index files <prefix>.{ref_info,base_2bits,base_3bits,hash_count,
+ * hash_offset,positions,pos_lseeds}, byte-identical to serial_index.c under
+ * HOST_BUILD.  The heavy stages (encode, histogram, scan, stable scatter)
+ * run on the GPU.  k is the -s option (2..14). */
+int swm_index_build(const char *fasta_path, const char *prefix, uint32_t k);
+
+/* In-memory variant: seq = contigs concatenated in FASTA order.  Fills *out
+ * (free with swm_index_host_free).  The GPU stages report their time in *ms
+ * when ms != NULL. */
+int swm_index_build_mem(const char *seq, uint64_t n, uint32_t k, swm_index_host_t *out, double *ms);
+void swm_index_host_free(swm_index_host_t *ix);
+
+/* Write/read the reference's index files for an in-memory index.  ref_info
+ * is written with one contig of the given name when contigs == NULL. */
+int swm_index_write(const swm_index_host_t *ix, const char *prefix, const char *contig_name);
+
+/* ---- device-resident index (replaces load_ref_info, mapper_mpe.c:198-314) ---- */
+
+/* Loads <prefix>.* with seed length k (-s) to the current HIP device. */
+int swm_index_load(const char *prefix, uint32_t k, swm_ctx_t **ctx);
+/* Same from memory: hash_count (4^k), positions/pos_lseeds (n_pos),
+ * base_2bits (n_2bits_words as in the file). */
+int swm_ctx_create(uint32_t k, const uint32_t *hash_count, const uint32_t *positions,
+                   const uint16_t *pos_lseeds, uint64_t n_pos, const uint32_t *base_2bits,
+                   uint64_t n_2bits_words, swm_ctx_t **ctx);
+void swm_ctx_destroy(swm_ctx_t *ctx);
+
+/* ---- mapping (replaces athread_spawn(find_positions)+athread_join,
+ *      mapper_mpe.c:972-1036; kernel mapper_cpe.c:2592-2663) ---- */
+
+/* Host buffers: reads[i*stride .. +lens[i]) for i < n (the reference's
+ * reads[cur_block] / reads_seq_len[cur_block] with stride = -l), errors e
+ * (-e, 0..15).  Requires lens[i] + 2 <= stride <= 256 (kseq stride rule).
+ * out[n] receives one result per read. */
+int swm_map_batch(swm_ctx_t *ctx, const char *reads, uint64_t stride, const uint32_t *lens,
+                  uint64_t n, uint32_t e, swm_read_result_t *out);
+
+/* Device buffers (HIP device pointers), enqueued on the context's stream;
+ * returns when the results are in d_out.  For callers that keep reads
+ * resident in HBM. */
+int swm_map_batch_device(swm_ctx_t *ctx, const char *d_reads, uint64_t stride,
+                         const uint32_t *d_lens, uint64_t n, uint32_t e,
+                         swm_read_result_t *d_out);
+
+/* Statistics of the last batch run on ctx. */
+int swm_last_stats(const swm_ctx_t *ctx, swm_stats_t *st);
+
+/* ---- stage entry points (per-kernel parity) ---- */
+
+/* Banded Myers over explicit candidates, device pointers:
+ * d_read_words[n_reads][16] interleaved 2-bit read words, d_lens[n_reads],
+ * d_cand_read[n] (read row), d_win_start[n] (window start, genome bases),
+ * d_ref_len[n]; outputs d_err[n], d_loc[n] (MAXERROR 10000 / -1 if > e). */
+int swm_myers_device(swm_ctx_t *ctx, const uint32_t *d_read_words, const uint32_t *d_lens,
+                     const uint32_t *d_cand_read, const uint32_t *d_win_start,
+                     const uint32_t *d_ref_len, uint64_t n, uint32_t e,
+                     int32_t *d_err, int32_t *d_loc);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,230 @@
+"""swmapper_amd -- MI355X-native mapping hot path of SWMapper behind a C ABI.
+
+Python is a thin host mirror of ``include/swm.h`` (ctypes); all compute runs
+in ``libswm.so`` (HIP kernels for gfx950).  Loading fails loudly when the
+library has not been built: there is no CPU fallback.
+
+Mirrors of the reference's interface:
+  * ``build_index(seq, k)``           -- serial_index.c construct_hash_index (:359-445)
+  * ``index_build_files(fa, p, k)``   -- serial_index.c main (:818-865)
+  * ``Mapper.from_files(prefix, k)``  -- load_ref_info (mapper_mpe.c:198-314)
+  * ``Mapper.map(reads, lens, e)``    -- athread_spawn(find_positions) + join
+                                         (mapper_mpe.c:972-1036)
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libswm.so")
+
+SWM_OK = 0
+_ERRORS = {-1: "invalid argument", -2: "index/sequence file error", -3: "out of memory",
+           -4: "HIP device error", -5: "busy"}
+
+RESULT_DTYPE = np.dtype([("pos", "<u4"), ("strand", "u1"), ("dist", "u1"), ("pad", "<u2"), ("count", "<u4")])
+UNMAPPED_POS = 0xFFFFFFFF
+
+GENOME_UNIFORM, GENOME_REPEAT_FAMILIES, GENOME_TANDEM = 0, 1, 2
+
+
+class SwmError(RuntimeError):
+    def __init__(self, fn, rc):
+        super().__init__(f"{fn} failed: {_ERRORS.get(rc, 'status')} ({rc})")
+        self.rc = rc
+
+
+class IndexHost(C.Structure):
+    _fields_ = [("k", C.c_uint32), ("n_bases", C.c_uint64), ("n_words", C.c_uint64),
+                ("base_2bits", C.POINTER(C.c_uint32)), ("base_3bits", C.POINTER(C.c_uint32)),
+                ("hash_count", C.POINTER(C.c_uint32)), ("hash_offset_end", C.POINTER(C.c_uint32)),
+                ("positions", C.POINTER(C.c_uint32)), ("pos_lseeds", C.POINTER(C.c_uint16)),
+                ("n_pos", C.c_uint64)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("n_reads", C.c_uint64), ("n_verifications", C.c_uint64), ("n_records", C.c_uint64),
+                ("ms_seed", C.c_double), ("ms_plan", C.c_double), ("ms_compact", C.c_double),
+                ("ms_myers", C.c_double), ("ms_emit", C.c_double), ("ms_total", C.c_double)]
+
+
+_lib_handle = None
+
+
+def lib():
+    """The product library; raises if it has not been built."""
+    global _lib_handle
+    if _lib_handle is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: run `python -m swmapper_amd.build` (no CPU fallback)")
+        L = C.CDLL(LIB_PATH)
+        vp, u32, u64, i32 = C.c_void_p, C.c_uint32, C.c_uint64, C.c_int32
+        sig = {
+            "swm_index_build": ([C.c_char_p, C.c_char_p, u32], C.c_int),
+            "swm_index_build_mem": ([vp, u64, u32, C.POINTER(IndexHost), C.POINTER(C.c_double)], C.c_int),
+            "swm_index_host_free": ([C.POINTER(IndexHost)], None),
+            "swm_index_write": ([C.POINTER(IndexHost), C.c_char_p, C.c_char_p], C.c_int),
+            "swm_index_load": ([C.c_char_p, u32, C.POINTER(vp)], C.c_int),
+            "swm_ctx_create": ([u32, vp, vp, vp, u64, vp, u64, C.POINTER(vp)], C.c_int),
+            "swm_ctx_destroy": ([vp], None),
+            "swm_map_batch": ([vp, vp, u64, vp, u64, u32, vp], C.c_int),
+            "swm_map_batch_device": ([vp, vp, u64, vp, u64, u32, vp], C.c_int),
+            "swm_last_stats": ([vp, C.POINTER(Stats)], C.c_int),
+            "swm_myers_device": ([vp, vp, vp, vp, vp, vp, u64, u32, vp, vp], C.c_int),
+            "swm_synth_genome": ([vp, u64, u64, C.c_int, C.c_double], C.c_int),
+            "swm_synth_reads": ([vp, u64, u64, u32, u32, C.c_double, C.c_double, C.c_double, u64,
+                                 vp, u64, vp, vp, vp], C.c_int),
+        }
+        for name, (args, res) in sig.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        _lib_handle = L
+    return _lib_handle
+
+
+def _check(fn, rc):
+    if rc != SWM_OK:
+        raise SwmError(fn, rc)
+
+
+def _ptr(a):
+    return C.c_void_p(a.ctypes.data) if a is not None else None
+
+
+# ---------------------------------------------------------------- synthetic data
+
+def synth_genome(n, seed=1, kind=GENOME_UNIFORM, frac=0.0):
+    g = np.empty(n, dtype=np.uint8)
+    _check("swm_synth_genome", lib().swm_synth_genome(_ptr(g), n, seed, kind, frac))
+    return g
+
+
+def synth_reads(genome, n_reads, len_min=150, len_max=150, err_min=0.025, err_max=0.025, indel_share=0.2,
+                seed=7, stride=256):
+    reads = np.zeros((n_reads, stride), dtype=np.uint8)
+    lens = np.zeros(n_reads, dtype=np.uint32)
+    tpos = np.zeros(n_reads, dtype=np.uint32)
+    tstr = np.zeros(n_reads, dtype=np.uint8)
+    _check("swm_synth_reads", lib().swm_synth_reads(_ptr(genome), genome.size, n_reads, len_min, len_max, err_min,
+                                                    err_max, indel_share, seed, _ptr(reads), stride, _ptr(lens),
+                                                    _ptr(tpos), _ptr(tstr)))
+    return reads, lens, tpos, tstr
+
+
+# ---------------------------------------------------------------- index
+
+class Index:
+    """In-memory index: the exact content of the reference's index files."""
+
+    def __init__(self, k, n_bases, base_2bits, base_3bits, hash_count, hash_offset_end, positions, pos_lseeds):
+        self.k, self.n_bases = k, n_bases
+        self.base_2bits, self.base_3bits = base_2bits, base_3bits
+        self.hash_count, self.hash_offset_end = hash_count, hash_offset_end
+        self.positions, self.pos_lseeds = positions, pos_lseeds
+        self.build_ms = None
+
+    def files(self):
+        return {".base_2bits": self.base_2bits, ".base_3bits": self.base_3bits, ".hash_count": self.hash_count,
+                ".hash_offset": self.hash_offset_end, ".positions": self.positions, ".pos_lseeds": self.pos_lseeds}
+
+    def write(self, prefix, contig_name="chr1"):
+        ih = IndexHost(self.k, self.n_bases, (self.base_2bits.size + 1) // 2,
+                       self.base_2bits.ctypes.data_as(C.POINTER(C.c_uint32)),
+                       self.base_3bits.ctypes.data_as(C.POINTER(C.c_uint32)),
+                       self.hash_count.ctypes.data_as(C.POINTER(C.c_uint32)),
+                       self.hash_offset_end.ctypes.data_as(C.POINTER(C.c_uint32)),
+                       self.positions.ctypes.data_as(C.POINTER(C.c_uint32)),
+                       self.pos_lseeds.ctypes.data_as(C.POINTER(C.c_uint16)), self.positions.size)
+        _check("swm_index_write", lib().swm_index_write(C.byref(ih), prefix.encode(), contig_name.encode()))
+
+
+def build_index(seq, k=12):
+    """GPU index build of a concatenated sequence (uint8 array or bytes)."""
+    s = np.frombuffer(seq, dtype=np.uint8) if isinstance(seq, (bytes, bytearray)) else np.ascontiguousarray(seq)
+    ih = IndexHost()
+    ms = C.c_double(0)
+    _check("swm_index_build_mem", lib().swm_index_build_mem(_ptr(s), s.size, k, C.byref(ih), C.byref(ms)))
+    try:
+        nw, nb, m = ih.n_words, 1 << (2 * k), ih.n_pos
+
+        def cp(p, n):
+            return np.ctypeslib.as_array(p, (n,)).copy() if n else np.zeros(0, dtype=np.dtype(p._type_))
+        ix = Index(k, ih.n_bases, cp(ih.base_2bits, 2 * nw), cp(ih.base_3bits, nw), cp(ih.hash_count, nb),
+                   cp(ih.hash_offset_end, nb + 1), cp(ih.positions, m), cp(ih.pos_lseeds, m))
+        ix.build_ms = ms.value
+        return ix
+    finally:
+        lib().swm_index_host_free(C.byref(ih))
+
+
+def index_build_files(fasta_path, prefix, k=12):
+    _check("swm_index_build", lib().swm_index_build(fasta_path.encode(), prefix.encode(), k))
+
+
+# ---------------------------------------------------------------- mapping
+
+class Mapper:
+    """Device-resident index context (one HIP stream)."""
+
+    def __init__(self, handle, k):
+        self._h = handle
+        self.k = k
+
+    @classmethod
+    def from_index(cls, ix):
+        h = C.c_void_p()
+        _check("swm_ctx_create", lib().swm_ctx_create(ix.k, _ptr(ix.hash_count), _ptr(ix.positions),
+                                                      _ptr(ix.pos_lseeds), ix.positions.size, _ptr(ix.base_2bits),
+                                                      ix.base_2bits.size, C.byref(h)))
+        return cls(h, ix.k)
+
+    @classmethod
+    def from_files(cls, prefix, k=12):
+        h = C.c_void_p()
+        _check("swm_index_load", lib().swm_index_load(prefix.encode(), k, C.byref(h)))
+        return cls(h, k)
+
+    def close(self):
+        if self._h:
+            lib().swm_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def map(self, reads, lens, e=5):
+        """reads: (n, stride) uint8; lens: (n,) uint32.  Returns RESULT_DTYPE[n]."""
+        reads = np.ascontiguousarray(reads, dtype=np.uint8)
+        lens = np.ascontiguousarray(lens, dtype=np.uint32)
+        n, stride = reads.shape
+        out = np.zeros(n, dtype=RESULT_DTYPE)
+        _check("swm_map_batch", lib().swm_map_batch(self._h, _ptr(reads), stride, _ptr(lens), n, e, _ptr(out)))
+        return out
+
+    def map_device(self, d_reads, stride, d_lens, n, e, d_out):
+        """Device pointers (ints), e.g. torch tensors' data_ptr()."""
+        _check("swm_map_batch_device", lib().swm_map_batch_device(self._h, C.c_void_p(d_reads), stride,
+                                                                  C.c_void_p(d_lens), n, e, C.c_void_p(d_out)))
+
+    def myers_device(self, d_read_words, d_lens, d_cread, d_wstart, d_rlen, n, e, d_err, d_loc):
+        _check("swm_myers_device", lib().swm_myers_device(self._h, *(C.c_void_p(x) for x in (
+            d_read_words, d_lens, d_cread, d_wstart, d_rlen)), n, e, C.c_void_p(d_err), C.c_void_p(d_loc)))
+
+    def stats(self):
+        s = Stats()
+        _check("swm_last_stats", lib().swm_last_stats(self._h, C.byref(s)))
+        return {f: getattr(s, f) for f, _ in Stats._fields_}
+
+
+def device_count():
+    """HIP devices visible to the library (0 on a CPU-only host)."""
+    try:
+        import torch
+        return torch.cuda.device_count()
+    except Exception:
+        return 0
