@@ -133,6 +133,15 @@ int swm_myers_device(swm_ctx_t *ctx, const uint32_t *d_read_words, const uint32_
                      const uint32_t *d_ref_len, uint64_t n, uint32_t e,
                      int32_t *d_err, int32_t *d_loc);
 
+/* ---- device memory plumbing for callers without their own HIP setup
+ *      (no counterpart in the reference, whose host/CPE memory is shared) ---- */
+int swm_device_count(int *n);
+int swm_dev_alloc(void **p, uint64_t bytes);
+int swm_dev_free(void *p);
+int swm_copy_h2d(void *dst, const void *src, uint64_t bytes);
+int swm_copy_d2h(void *dst, const void *src, uint64_t bytes);
+int swm_dev_memset(void *dst, int value, uint64_t bytes);
+
 #ifdef __cplusplus
 }
 #endif

@@ -73,6 +73,12 @@ def lib():
             "swm_last_stats": ([vp, C.POINTER(Stats)], C.c_int),
             "swm_myers_device": ([vp, vp, vp, vp, vp, vp, u64, u32, vp, vp], C.c_int),
             "swm_synth_genome": ([vp, u64, u64, C.c_int, C.c_double], C.c_int),
+            "swm_device_count": ([C.POINTER(C.c_int)], C.c_int),
+            "swm_dev_alloc": ([C.POINTER(vp), u64], C.c_int),
+            "swm_dev_free": ([vp], C.c_int),
+            "swm_copy_h2d": ([vp, vp, u64], C.c_int),
+            "swm_copy_d2h": ([vp, vp, u64], C.c_int),
+            "swm_dev_memset": ([vp, C.c_int, u64], C.c_int),
             "swm_synth_reads": ([vp, u64, u64, u32, u32, C.c_double, C.c_double, C.c_double, u64,
                                  vp, u64, vp, vp, vp], C.c_int),
         }
@@ -223,8 +229,45 @@ class Mapper:
 
 def device_count():
     """HIP devices visible to the library (0 on a CPU-only host)."""
-    try:
-        import torch
-        return torch.cuda.device_count()
-    except Exception:
-        return 0
+    n = C.c_int(0)
+    lib().swm_device_count(C.byref(n))
+    return n.value
+
+
+class DeviceBuffer:
+    """HBM buffer owned through the library's HIP runtime (no torch in the
+    process: torch ships its own HIP runtime build)."""
+
+    def __init__(self, nbytes):
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        _check("swm_dev_alloc", lib().swm_dev_alloc(C.byref(p), self.nbytes))
+        self.ptr = p.value
+
+    @classmethod
+    def from_array(cls, a):
+        a = np.ascontiguousarray(a)
+        b = cls(a.nbytes)
+        _check("swm_copy_h2d", lib().swm_copy_h2d(C.c_void_p(b.ptr), C.c_void_p(a.ctypes.data), a.nbytes))
+        return b
+
+    def zero(self):
+        _check("swm_dev_memset", lib().swm_dev_memset(C.c_void_p(self.ptr), 0, self.nbytes))
+        return self
+
+    def to_array(self, dtype, shape=None):
+        dt = np.dtype(dtype)
+        out = np.empty(self.nbytes // dt.itemsize, dtype=dt)
+        _check("swm_copy_d2h", lib().swm_copy_d2h(C.c_void_p(out.ctypes.data), C.c_void_p(self.ptr), out.nbytes))
+        return out.reshape(shape) if shape is not None else out
+
+    def free(self):
+        if self.ptr:
+            lib().swm_dev_free(C.c_void_p(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass

@@ -1012,3 +1012,45 @@ extern "C" int swm_myers_device(swm_ctx_t *ctx, const uint32_t *d_read_words, co
     SWM_HIP_CHECK(hipStreamSynchronize(ctx->stream));
     return SWM_OK;
 }
+
+/* ------------------------------------------------------- device plumbing */
+
+extern "C" int swm_device_count(int *n)
+{
+    if (!n) return SWM_EINVAL;
+    *n = 0;
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) return SWM_OK;
+    *n = c;
+    return SWM_OK;
+}
+
+extern "C" int swm_dev_alloc(void **p, uint64_t bytes)
+{
+    if (!p) return SWM_EINVAL;
+    *p = nullptr;
+    return hipMalloc(p, bytes ? bytes : 1) == hipSuccess ? SWM_OK : SWM_ENOMEM;
+}
+
+extern "C" int swm_dev_free(void *p)
+{
+    return (!p || hipFree(p) == hipSuccess) ? SWM_OK : SWM_EDEVICE;
+}
+
+extern "C" int swm_copy_h2d(void *dst, const void *src, uint64_t bytes)
+{
+    if (bytes && (!dst || !src)) return SWM_EINVAL;
+    return (!bytes || hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess) ? SWM_OK : SWM_EDEVICE;
+}
+
+extern "C" int swm_copy_d2h(void *dst, const void *src, uint64_t bytes)
+{
+    if (bytes && (!dst || !src)) return SWM_EINVAL;
+    return (!bytes || hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost) == hipSuccess) ? SWM_OK : SWM_EDEVICE;
+}
+
+extern "C" int swm_dev_memset(void *dst, int value, uint64_t bytes)
+{
+    if (bytes && !dst) return SWM_EINVAL;
+    return (!bytes || hipMemset(dst, value, bytes) == hipSuccess) ? SWM_OK : SWM_EDEVICE;
+}

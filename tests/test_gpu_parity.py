@@ -120,7 +120,6 @@ def test_map_genome_ends_and_n():
 
 
 def test_myers_stage_matches_oracle():
-    import torch
     rng = np.random.default_rng(3)
     g = wl.genome(20_000, seed=61)
     ix = orc.OracleIndex(g, 12)
@@ -153,15 +152,13 @@ def test_myers_stage_matches_oracle():
             lo[j] = ((a0 >> sh) | ((a1 << (32 - sh)) if sh else 0)) & 0xFFFFFFFF
             hi[j] = ((h0 >> sh) | ((h1 << (32 - sh)) if sh else 0)) & 0xFFFFFFFF
         exp_err[i], exp_loc[i] = orc.myers(words[i], lo, hi, int(L[i]), int(rlen[i]), e)
-    dev = torch.device("cuda")
-    t = lambda a: torch.from_numpy(a).to(dev)
+    t = swm.DeviceBuffer.from_array
     dw, dl, dc, dws, drl = t(words), t(L), t(np.arange(n, dtype=np.uint32)), t(ws), t(rlen)
-    de = torch.zeros(n, dtype=torch.int32, device=dev)
-    dlo = torch.zeros(n, dtype=torch.int32, device=dev)
+    de = swm.DeviceBuffer(4 * n).zero()
+    dlo = swm.DeviceBuffer(4 * n).zero()
     _say("myers stage", n)
-    m.myers_device(dw.data_ptr(), dl.data_ptr(), dc.data_ptr(), dws.data_ptr(), drl.data_ptr(), n, e,
-                   de.data_ptr(), dlo.data_ptr())
-    got_e, got_l = de.cpu().numpy(), dlo.cpu().numpy()
+    m.myers_device(dw.ptr, dl.ptr, dc.ptr, dws.ptr, drl.ptr, n, e, de.ptr, dlo.ptr)
+    got_e, got_l = de.to_array(np.int32), dlo.to_array(np.int32)
     assert np.array_equal(got_e, exp_err)
     assert np.array_equal(got_l, exp_loc)
     assert (exp_err <= e).mean() > 0.3
