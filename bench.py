@@ -40,6 +40,8 @@ def main():
     ap.add_argument("--config", default="repeat5", choices=sorted(CONFIGS))
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target length of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="one 256 MiB device-to-device copy first (calibrates HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -73,6 +75,12 @@ def main():
         m.map_device(d_reads.ptr, 256, d_lens.ptr, n, e, d_out.ptr)
         return m.stats()
 
+    if args.copy_probe:
+        a = swm.DeviceBuffer(256 << 20).zero()
+        b = swm.DeviceBuffer(256 << 20)
+        swm.lib().swm_copy_d2d(swm.C.c_void_p(b.ptr), swm.C.c_void_p(a.ptr), 256 << 20)
+        a.free()
+        b.free()
     for _ in range(args.warmup):
         step()
     lat, st_all = [], []

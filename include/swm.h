@@ -67,6 +67,8 @@ typedef struct swm_stats {
     uint64_t n_reads;
     uint64_t n_verifications;  /* (read, strand, candidate) windows run through banded Myers */
     uint64_t n_records;
+    /* stage times (HIP events on the context stream): seed lookup; candidate
+     * batches; compaction; banded Myers; Hamming + traceback + emission */
     double ms_seed, ms_plan, ms_compact, ms_myers, ms_emit, ms_total;
 } swm_stats_t;
 
@@ -140,6 +142,7 @@ int swm_dev_alloc(void **p, uint64_t bytes);
 int swm_dev_free(void *p);
 int swm_copy_h2d(void *dst, const void *src, uint64_t bytes);
 int swm_copy_d2h(void *dst, const void *src, uint64_t bytes);
+int swm_copy_d2d(void *dst, const void *src, uint64_t bytes);
 int swm_dev_memset(void *dst, int value, uint64_t bytes);
 
 #ifdef __cplusplus

@@ -350,6 +350,7 @@ void orc_reverse_read(char *out, const char *read, uint32_t L)      /* mapper_cp
 int orc_encode_read(const char *read, uint32_t L, uint32_t k, uint32_t *words32, uint32_t *seeds)
 {
     memset(words32, 0, 32 * sizeof(uint32_t));
+    if (L > ORC_READ_LEN - 2 || L / (k + ORC_SEED_STEP - 1) > 32) return -1;
     uint64_t seeds_size = (L / (k + ORC_SEED_STEP - 1)) * ORC_SEED_STEP;
     uint32_t seed = 0, enc = 0;
     for (uint64_t j = 0; j < seeds_size; j++) {
@@ -980,6 +981,10 @@ static void map_one(const orc_ref_t *ref, const char *read, uint32_t L, uint32_t
     if (keep) { em.recs = *recs; em.n_recs = *n_recs; em.cap_recs = *cap_recs; }
     hit->pos = 0xffffffffu; hit->strand = 0xff; hit->dist = 0xff; hit->pad = 0; hit->count = 0;
 
+    /* more than 16 long seeds overflow the reference's per-read tables
+     * (hash_seed_info[16], hash_count[64], mapper_cpe.c:2503-2515): such
+     * reads are defined unmapped */
+    if (L / (ref->k + ORC_SEED_STEP - 1) > 16) L = 0;
     for (int strand = 0; strand < 2; strand++) {
         const char *rs = read;
         if (strand == 1) { orc_reverse_read(rev, read, L); rs = rev; }

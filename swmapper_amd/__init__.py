@@ -78,6 +78,7 @@ def lib():
             "swm_dev_free": ([vp], C.c_int),
             "swm_copy_h2d": ([vp, vp, u64], C.c_int),
             "swm_copy_d2h": ([vp, vp, u64], C.c_int),
+            "swm_copy_d2d": ([vp, vp, u64], C.c_int),
             "swm_dev_memset": ([vp, C.c_int, u64], C.c_int),
             "swm_synth_reads": ([vp, u64, u64, u32, u32, C.c_double, C.c_double, C.c_double, u64,
                                  vp, u64, vp, vp, vp], C.c_int),

@@ -8,6 +8,7 @@
 #include <stdint.h>
 
 #include "swm.h"
+#include "swm_recipe.h"
 
 #define SWM_SEED_STEP 4        /* fm_define.h:13 */
 #define SWM_MAXERROR 10000     /* mapper_cpe.c:17 */
@@ -37,12 +38,12 @@ struct swm_ctx {
     uint32_t k;
     uint64_t n_buckets;
     uint32_t *d_hash_offset;    /* exclusive scan of hash_count, 4^k + 1 */
-    uint32_t *d_positions;
-    uint16_t *d_lseeds;
+    uint2 *d_pl;                /* bucket entries: (position, 12-bit residual) */
     uint64_t n_pos;
     uint32_t *d_base_2bits;     /* file words + SWM_WIN_PAIRS_PAD zero pairs */
     uint64_t n_2bits_words;
     uint64_t ref_length;        /* 16 * bytes(.base_2bits) (mapper_mpe.c:288) */
+    swm_recipe_t *d_recipe;     /* read-encoding recipe per read length 0..254 */
     swm_stats_t stats;
     swm_ws *ws;
 };

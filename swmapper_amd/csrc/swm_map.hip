@@ -5,25 +5,35 @@
  * The path replaces find_positions (mapper_cpe.c:2592-2663) behind
  * swm_map_batch*.  One launch per stage on the context's stream:
  *
- *   k_seed     one thread per (read, strand): read encoding, 12-mer hash
- *              offset gathers, long-seed ranking, residuals
- *              (mapper_cpe.c:1246-1401, 2451-2499)
+ *   k_seed     64 reads per workgroup staged in LDS, one thread per
+ *              (read, strand): read encoding (applied from a per-length
+ *              recipe), 12-mer hash-offset gathers, long-seed ranking,
+ *              residuals (mapper_cpe.c:1246-1401, 2451-2499)
  *   k_plan     one thread per (read, strand): the batch/heap state machine of
- *              alignment_rmdup (mapper_cpe.c:2124-2426) with the 15-mer filter
- *              (:2090-2123) applied as positions stream in, then the per-batch
- *              sorted unique list (block_heap_sort :1999-2089)
+ *              alignment_rmdup (mapper_cpe.c:2124-2426); each fetched chunk
+ *              of <= 32 positions is loaded at once, filtered by the 15-mer
+ *              test (:2090-2123) and shifted to read starts (:1893-1917);
+ *              small batches are sorted/uniqued in place (block_heap_sort's
+ *              result, :1999-2089), large ones are queued for
+ *   k_bigsort  one workgroup per large batch: LDS bitonic sort + unique
  *   k_compact  one thread per (read, strand): dense candidate list
- *   k_myers    one candidate per lane: window fetch + banded Myers
- *              (semantics mapper_cpe.c:519-660 == :668-976)
- *   k_emit     one thread per read: Hamming check / traceback, duplicate
- *              rule and flush state machine, per-read reduction
- *              (mapper_cpe.c:229-517, 1437-1507, 1790, 1858)
+ *   k_myers    one candidate per lane: window fetch, banded Myers (semantics
+ *              mapper_cpe.c:519-660 == :668-976)
+ *   k_hamming  one lane per passing candidate: the Hamming check of
+ *              align_reads (:252-287)
+ *   k_trace    one lane per passing candidate that failed the Hamming check:
+ *              the traceback of align_reads (:288-509)
+ *   k_emit     one thread per read: duplicate rule and flush state machine
+ *              (:475-477, 1790, 1858, 1437-1507), per-read reduction
  */
 #include "swm_internal.h"
 
 #include <hipcub/hipcub.hpp>
+#include <new>
 #include <stdlib.h>
 #include <string.h>
+
+#include "swm_recipe.h"
 
 namespace {
 
@@ -43,19 +53,6 @@ __device__ __forceinline__ uint32_t genome_word(const uint32_t *__restrict__ b2,
     uint32_t sh = (uint32_t)g & 31u;
     uint32_t a = b2[2 * wi + plane], b = b2[2 * wi + 2 + plane];
     return __builtin_amdgcn_alignbit(b, a, sh);
-}
-
-__device__ __forceinline__ char strand_char(const char *__restrict__ rd, uint32_t L, int strand, uint32_t t)
-{
-    if (!strand) return rd[t];
-    char c = rd[L - 1 - t];
-    switch (c) {                       /* reverse_read, mapper_cpe.c:2429-2450 */
-    case 'A': return 'T';
-    case 'C': return 'G';
-    case 'G': return 'C';
-    case 'T': return 'A';
-    default: return c;
-    }
 }
 
 /* pos_start / adjust_ref_len (mapper_cpe.c:1403-1435). */
@@ -82,198 +79,242 @@ __device__ __forceinline__ uint32_t pos_start(uint64_t ref_length, uint64_t e, u
     return (uint32_t)((uint64_t)base_pos - shift);
 }
 
+__device__ __forceinline__ int ndigits(uint32_t v)
+{
+    int n = 0;
+    while (v) { n++; v /= 10; }
+    return n;
+}
+
+/* Block-wide append: one global atomic per workgroup (same-address atomics
+ * from every wave serialise at one L2 channel).  Every thread of the block
+ * must reach the call; s_tmp is __shared__ uint32_t[2]. */
+__device__ __forceinline__ void block_append(bool pred, uint32_t value, uint32_t *list, uint32_t *count,
+                                             uint32_t *s_tmp)
+{
+    const uint64_t m = __ballot(pred);
+    const uint32_t lane = __lane_id();
+    if (threadIdx.x == 0) s_tmp[0] = 0;
+    __syncthreads();
+    uint32_t woff = 0;
+    if (lane == 0 && m) woff = atomicAdd(&s_tmp[0], (uint32_t)__popcll(m));
+    woff = __shfl(woff, 0);
+    __syncthreads();
+    if (threadIdx.x == 0) s_tmp[1] = s_tmp[0] ? atomicAdd(count, s_tmp[0]) : 0u;
+    __syncthreads();
+    if (pred) list[s_tmp[1] + woff + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = value;
+}
+
 /* ------------------------------------------------------------------ k_seed */
+
+#define SEED_READS 64        /* reads per workgroup (2 threads each) */
 
 struct SeedOut {
     uint32_t *words;    /* [nrs][32]: 16 interleaved 2-bit words, 8 plane pairs */
     swm_sel_t *sel;     /* [nrs][seln] */
     uint32_t *nsel;     /* [nrs] selected 12-mers (4 per selected long seed) */
     uint64_t *npos;     /* [nrs + 1] positions over the selected seeds */
-    uint32_t *nchunk;   /* [nrs + 1] sum of ceil(cnt / 32) */
+    uint32_t *nchunk;   /* [nrs + 1] sum of ceil(cnt / 32), + 1 */
 };
 
-/* One base of get_seed_from_reads' encoder (mapper_cpe.c:1258-1305).  cp is
- * the interleaved/plane code. */
-__device__ __forceinline__ void enc_base(uint32_t (&w)[32], uint32_t &enc, uint64_t rp, uint32_t cp)
+/* A read base as the mapper sees it: A,C,G,T -> 0..3 (complemented on the
+ * reverse strand, reverse_read :2429-2450); anything else keeps its byte and
+ * is "invalid" (code C in the planes, Q15). */
+struct Base {
+    uint32_t code;
+    bool valid;
+};
+
+__device__ __forceinline__ Base read_base(const uint8_t *rd, uint32_t L, int strand, uint32_t t)
 {
-    enc |= cp << ((rp % 16) * 2);
-    w[16 + (rp >> 5) * 2] |= (cp & 1u) << (rp & 0x1f);
-    w[16 + (rp >> 5) * 2 + 1] |= (cp >> 1) << (rp & 0x1f);
-    if (rp % 16 == 15) {
-        w[rp / 16] = enc;
-        enc = 0;
-    }
+    uint32_t ch = strand ? rd[L - 1 - t] : rd[t];
+    Base b;
+    b.valid = (ch == 'A') | (ch == 'C') | (ch == 'G') | (ch == 'T');
+    uint32_t c = ((ch >> 1) ^ (ch >> 2)) & 3u;   /* A 0, C 1, G 2, T 3 */
+    if (strand) c = 3u - c;
+    b.code = b.valid ? c : 1u;
+    return b;
 }
 
-__device__ __forceinline__ uint32_t plane_code(char b)
+/* Seed value of base 15m+u of long seed m (get_seed_from_reads :1260-1300):
+ * a non-ACGT base counts as u % 4 when it enters through the k-base loop of
+ * the long seed's first 12-mer, and as 3 when it enters as a rolling base. */
+__device__ __forceinline__ uint32_t seed_val(const Base &b, uint32_t u, uint32_t k)
 {
-    switch (b) {
-    case 'A': return 0;
-    case 'C': return 1;
-    case 'G': return 2;
-    case 'T': return 3;
-    default: return 1;      /* non-ACGT encodes as C (mapper_cpe.c:1295-1299, Q15) */
-    }
+    return b.valid ? b.code : (u < k ? (u & 3u) : 3u);
 }
 
 __global__ void __launch_bounds__(128) k_seed(const char *__restrict__ reads, uint64_t stride,
-                                              const uint32_t *__restrict__ lens, uint32_t nrs, uint32_t k,
+                                              const uint32_t *__restrict__ lens, uint32_t nreads, uint32_t k,
                                               uint32_t e, uint32_t seln, const uint32_t *__restrict__ hash_offset,
-                                              SeedOut o)
+                                              const swm_recipe_t *__restrict__ recipes, SeedOut o)
 {
-    uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= nrs) return;
-    const uint32_t read = r >> 1;
-    const int strand = (int)(r & 1u);
-    const uint32_t L = lens[read];
-    const char *rd = reads + (uint64_t)read * stride;
-
-    uint32_t w[32];
-#pragma unroll
-    for (int j = 0; j < 32; j++) w[j] = 0;
-    uint32_t enc = 0, seed = 0;
-    const uint32_t seed_len = k + SWM_SEED_STEP - 1;
-    const uint32_t n_long = L / seed_len;
-    const uint32_t seeds_size = n_long * SWM_SEED_STEP;
-    uint32_t off[64], cnt[64];
-
-    /* choose_seed (mapper_cpe.c:1365-1384) driving get_seed_from_reads
-     * (:1246-1364) seed by seed, including its tail branch (Q3). */
-    for (uint32_t sn = 0; sn < seeds_size; sn++) {
-        uint64_t i = (sn % SWM_SEED_STEP == 0) ? 0 : k - 1;
-        uint64_t roff = (uint64_t)(sn / SWM_SEED_STEP) * seed_len;
-        uint64_t soff = sn % SWM_SEED_STEP;
-        uint64_t rp = roff + soff + i;
-        for (; i < k; i++) {
-            rp = roff + soff + i;
-            char b = strand_char(rd, L, strand, (uint32_t)rp);
-            uint32_t cs;
-            switch (b) {
-            case 'A': cs = 0; break;
-            case 'C': cs = 1; break;
-            case 'G': cs = 2; break;
-            case 'T': cs = 3; break;
-            default: cs = (uint32_t)(i % 4); break;
-            }
-            seed = (seed >> 2) | (cs << ((k - 1) * 2));
-            enc_base(w, enc, rp, plane_code(b));
+    __shared__ uint32_t s_rd32[SEED_READS * 256 / 4];
+    /* offset and count of every 12-mer of the thread's strand, kept from the
+     * gathers so that the selected ones are not fetched twice */
+    __shared__ uint32_t s_off[64 * 2 * SEED_READS], s_cnt[64 * 2 * SEED_READS];
+    uint8_t *s_rd = reinterpret_cast<uint8_t *>(s_rd32);
+    const uint32_t r0 = blockIdx.x * SEED_READS;
+    const uint32_t nr = min((uint32_t)SEED_READS, nreads - r0);
+    {
+        const char *src = reads + (uint64_t)r0 * stride;
+        const uint32_t nbytes = nr * (uint32_t)stride;
+        if ((((uintptr_t)src) & 3u) == 0 && (nbytes & 3u) == 0) {
+            const uint32_t *s32 = reinterpret_cast<const uint32_t *>(src);
+            for (uint32_t i = threadIdx.x; i < nbytes / 4; i += blockDim.x) s_rd32[i] = s32[i];
+        } else {
+            for (uint32_t i = threadIdx.x; i < nbytes; i += blockDim.x) s_rd[i] = (uint8_t)src[i];
         }
-        if ((uint64_t)L - rp < k) {
-            for (; rp < L; rp++) enc_base(w, enc, rp, plane_code(strand_char(rd, L, strand, (uint32_t)rp)));
-            w[(rp - 1) / 16] = enc;
-        }
-        uint32_t a = hash_offset[seed], b2 = hash_offset[seed + 1];
-        off[sn] = a;
-        cnt[sn] = b2 - a;
     }
-    /* long-seed totals and the exchange sort (mapper_cpe.c:1385-1399, Q4) */
-    uint32_t icount[16], iindex[16];
-#pragma unroll
-    for (int j = 0; j < 16; j++) { icount[j] = 0; iindex[j] = (uint32_t)j; }
-    for (uint32_t j = 0; j < seeds_size; j++) icount[j / SWM_SEED_STEP] += cnt[j];
-    for (uint32_t j = 0; j < n_long; j++)
-        for (uint32_t q = j + 1; q < n_long; q++)
-            if (icount[j] > icount[q]) {
-                uint32_t tc = icount[j]; icount[j] = icount[q]; icount[q] = tc;
-                uint32_t ti = iindex[j]; iindex[j] = iindex[q]; iindex[q] = ti;
-            }
+    __syncthreads();
+    const uint32_t lr = threadIdx.x >> 1;
+    if (lr >= nr) return;
+    const int strand = (int)(threadIdx.x & 1u);
+    const uint32_t read = r0 + lr;
+    const uint32_t r = 2 * read + (uint32_t)strand;
+    const uint8_t *rd = s_rd + lr * (uint32_t)stride;
+    uint32_t L = lens[read];
+    const uint32_t seed_len = k + SWM_SEED_STEP - 1;
+    /* reads that do not fit the reference's fixed tables map to nothing:
+     * L + 2 > -l (kseq stride), L > 254 or more than 16 long seeds */
+    if ((uint64_t)L + 2 > stride || L > SWM_READ_MAX || L / seed_len > 16) L = 0;
+    const uint32_t n_long = L / seed_len;
 
-    /* selected 12-mers: the e+1 rarest long seeds, fewer if the read has
-     * fewer (Q5); residuals as set_read_long_seeds (mapper_cpe.c:2451-2499) */
-    uint32_t n_sel_long = (e + 1 < n_long) ? e + 1 : n_long;
-    uint32_t nsel = n_sel_long * SWM_SEED_STEP;
-    const uint32_t seed_mask = (1u << seed_len) - 1;
-    uint64_t npos = 0;
-    uint32_t nchunk = 0;
-    swm_sel_t *so = o.sel + (uint64_t)r * seln;
-    for (uint32_t i = 0; i < nsel; i++) {
-        uint32_t m = iindex[i / SWM_SEED_STEP];
-        uint32_t q = i % SWM_SEED_STEP;
-        uint32_t si = m * SWM_SEED_STEP + q;
-        uint32_t so_ = m * seed_len;
-        uint32_t j = so_ >> 5, sh = so_ & 31u;
-        uint32_t lo_next = (j + 1 < 8) ? w[16 + (j + 1) * 2] : 0u;
-        uint32_t hi_next = (j + 1 < 8) ? w[16 + (j + 1) * 2 + 1] : 0u;
-        uint32_t l_read = (shr32(w[16 + j * 2], (int)sh) | shl32(lo_next, 32 - (int)sh)) & seed_mask;
-        uint32_t h_read = (shr32(w[16 + j * 2 + 1], (int)sh) | shl32(hi_next, 32 - (int)sh)) & seed_mask;
-        uint32_t fmask = (1u << q) - 1;
-        uint32_t rl = ((l_read & fmask) | ((l_read >> (k + q)) << q)) & 7u;
-        uint32_t rh = ((h_read & fmask) | ((h_read >> (k + q)) << q)) & 7u;
-        swm_sel_t s;
-        s.off = off[si];
-        s.cnt = cnt[si];
-        s.meta = (m * seed_len + q) | (rl << 16) | (rh << 19) | (q << 22);
-        so[i] = s;
-        npos += s.cnt;
-        nchunk += (s.cnt + SWM_CHUNK - 1) / SWM_CHUNK;
+    /* plain packing + bit planes, then the length's store recipe (Q3) */
+    const swm_recipe_t *rec = recipes + L;
+    const uint32_t n_planes = rec->n_planes;
+    uint32_t pw[18], pl[8], ph[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) { pl[j] = 0; ph[j] = 0; }
+    pw[16] = pw[17] = 0;
+#pragma unroll
+    for (int j = 0; j < 16; j++) {
+        uint32_t v = 0;
+        if ((uint32_t)(16 * j) < L) {
+#pragma unroll
+            for (int s = 0; s < 16; s++) {
+                const uint32_t t = 16 * j + s;
+                if (t < L) {
+                    Base b = read_base(rd, L, strand, t);
+                    v |= b.code << (2 * s);
+                    if (t < n_planes) {
+                        pl[j >> 1] |= (b.code & 1u) << (t & 31u);
+                        ph[j >> 1] |= (b.code >> 1) << (t & 31u);
+                    }
+                }
+            }
+        }
+        pw[j] = v;
     }
     uint32_t *wo = o.words + (uint64_t)r * 32;
 #pragma unroll
-    for (int j = 0; j < 32; j++) wo[j] = w[j];
-    o.nsel[r] = nsel;
+    for (int j = 0; j < 16; j++)
+        wo[j] = (pw[j] & rec->mask[j][0]) | (pw[j + 1] & rec->mask[j][1]) | (pw[j + 2] & rec->mask[j][2]);
+#pragma unroll
+    for (int j = 0; j < 8; j++) { wo[16 + 2 * j] = pl[j]; wo[17 + 2 * j] = ph[j]; }
+
+    /* 12-mer codes and hash-offset gathers, long seed by long seed
+     * (choose_seed :1372-1390) */
+    uint32_t tot[16], idx[16];
+#pragma unroll
+    for (int m = 0; m < 16; m++) {
+        tot[m] = 0;
+        idx[m] = (uint32_t)m;
+        if ((uint32_t)m < n_long) {
+            const uint32_t base = (uint32_t)m * seed_len;
+            uint32_t code = 0;
+            for (uint32_t t = 0; t < k; t++) code |= seed_val(read_base(rd, L, strand, base + t), t, k) << (2 * t);
+            uint32_t cq[4];
+            cq[0] = code;
+#pragma unroll
+            for (int q = 1; q < 4; q++) {
+                const uint32_t u = k - 1 + (uint32_t)q;
+                code = (code >> 2) | (seed_val(read_base(rd, L, strand, base + u), u, k) << (2 * (k - 1)));
+                cq[q] = code;
+            }
+            uint32_t a[4], b[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) { a[q] = hash_offset[cq[q]]; b[q] = hash_offset[cq[q] + 1]; }
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                tot[m] += b[q] - a[q];
+                s_off[(4 * m + q) * (2 * SEED_READS) + threadIdx.x] = a[q];
+                s_cnt[(4 * m + q) * (2 * SEED_READS) + threadIdx.x] = b[q] - a[q];
+            }
+        }
+    }
+    /* exchange sort of the long seeds by total hits (:1391-1399, Q4) */
+#pragma unroll
+    for (int j = 0; j < 16; j++) {
+#pragma unroll
+        for (int q = j + 1; q < 16; q++) {
+            if ((uint32_t)q < n_long && tot[j] > tot[q]) {
+                uint32_t t0 = tot[j]; tot[j] = tot[q]; tot[q] = t0;
+                uint32_t t1 = idx[j]; idx[j] = idx[q]; idx[q] = t1;
+            }
+        }
+    }
+    /* the e+1 rarest long seeds (fewer if the read has fewer, Q5): offsets
+     * (cached from the gathers above) and the 3-base residuals of
+     * set_read_long_seeds (:2467-2496) */
+    const uint32_t n_sel_long = (e + 1 < n_long) ? e + 1 : n_long;
+    uint64_t npos = 0;
+    uint32_t nchunk = 0;
+    swm_sel_t *so = o.sel + (uint64_t)r * seln;
+#pragma unroll
+    for (int jm = 0; jm < 16; jm++) {
+        if ((uint32_t)jm < n_sel_long) {
+            const uint32_t m = idx[jm];
+            const uint32_t base = m * seed_len;
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                uint32_t rl = 0, rh = 0;
+#pragma unroll
+                for (int bb = 0; bb < 3; bb++) {
+                    const uint32_t u = (bb < q) ? (uint32_t)bb : k + (uint32_t)bb;
+                    const uint32_t c = read_base(rd, L, strand, base + u).code;
+                    rl |= (c & 1u) << bb;
+                    rh |= (c >> 1) << bb;
+                }
+                swm_sel_t s;
+                s.off = s_off[(4 * m + q) * (2 * SEED_READS) + threadIdx.x];
+                s.cnt = s_cnt[(4 * m + q) * (2 * SEED_READS) + threadIdx.x];
+                s.meta = (base + (uint32_t)q) | (rl << 16) | (rh << 19) | ((uint32_t)q << 22);
+                so[4 * jm + q] = s;
+                npos += s.cnt;
+                nchunk += (s.cnt + SWM_CHUNK - 1) / SWM_CHUNK;
+            }
+        }
+    }
+    o.nsel[r] = n_sel_long * SWM_SEED_STEP;
     o.npos[r] = npos;
     o.nchunk[r] = nchunk + 1;   /* batches <= chunks + 1 (an empty first batch) */
 }
 
 /* ------------------------------------------------------------------ k_plan */
 
+#define SMALL_BATCH 32
+
 struct PlanIO {
     const swm_sel_t *sel;
+    uint32_t seln;
     const uint32_t *nsel;
     const uint64_t *surv_off;   /* exclusive scan of npos */
     const uint32_t *bd_off;     /* exclusive scan of nchunk */
     uint32_t *surv;             /* survivor / candidate values */
-    uint2 *bdesc;               /* per batch: (start relative to surv_off, unique count) */
+    uint2 *bdesc;               /* per batch: (start relative to surv_off, count) */
     uint32_t *nbatch;           /* [nrs] */
     uint32_t *nuniq;            /* [nrs + 1] */
+    uint2 *big;                 /* (rs, batch) of batches left for k_bigsort */
+    uint32_t *big_count;
 };
 
-__device__ __forceinline__ void sort_unique(uint32_t *a, uint32_t n, uint32_t *out_n)
-{
-    if (n <= 1) { *out_n = n; return; }
-    if (n <= 48) {
-        for (uint32_t i = 1; i < n; i++) {
-            uint32_t v = a[i];
-            uint32_t j = i;
-            while (j > 0 && a[j - 1] > v) { a[j] = a[j - 1]; j--; }
-            a[j] = v;
-        }
-    } else {                           /* heapsort in place */
-        for (uint32_t start = n / 2; start-- > 0;) {
-            uint32_t root = start;
-            for (;;) {
-                uint32_t c = 2 * root + 1;
-                if (c >= n) break;
-                if (c + 1 < n && a[c + 1] > a[c]) c++;
-                if (a[root] < a[c]) { uint32_t t = a[root]; a[root] = a[c]; a[c] = t; root = c; }
-                else break;
-            }
-        }
-        for (uint32_t end = n - 1; end > 0; end--) {
-            uint32_t t = a[0]; a[0] = a[end]; a[end] = t;
-            uint32_t root = 0;
-            for (;;) {
-                uint32_t c = 2 * root + 1;
-                if (c >= end) break;
-                if (c + 1 < end && a[c + 1] > a[c]) c++;
-                if (a[root] < a[c]) { uint32_t t2 = a[root]; a[root] = a[c]; a[c] = t2; root = c; }
-                else break;
-            }
-        }
-    }
-    uint32_t u = 1;
-    for (uint32_t i = 1; i < n; i++)
-        if (a[i] != a[u - 1]) a[u++] = a[i];
-    *out_n = u;
-}
-
-__global__ void __launch_bounds__(64) k_plan(uint32_t nrs, const uint32_t *__restrict__ positions,
-                                             const uint16_t *__restrict__ lseeds, PlanIO io)
+__global__ void __launch_bounds__(64) k_plan(uint32_t nrs, const uint2 *__restrict__ pl, PlanIO io)
 {
     uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= nrs) return;
     const uint32_t nsel = io.nsel[r];
-    const swm_sel_t *sel = io.sel + (uint64_t)r * 64;
+    const swm_sel_t *sel = io.sel + (uint64_t)r * io.seln;
     uint32_t *surv = io.surv + io.surv_off[r];
     uint2 *bd = io.bdesc + io.bd_off[r];
 
@@ -283,18 +324,25 @@ __global__ void __launch_bounds__(64) k_plan(uint32_t nrs, const uint32_t *__res
     uint32_t nsurv = 0, bstart = 0, batch = 0, pos_size = 0, blocks = 0;
     bool first_pending = true;        /* Q7: the batch's first position is never kept */
 
-    /* positions [g, g+len) of selected seed i enter the batch: 15-mer filter
-     * (long_seeds_filter, :2106-2117) and read-start adjustment
-     * (adjust_ref_pos_rmdup, :1903-1905) */
+    /* positions [g, g+len) of one selected 12-mer enter the batch: 15-mer
+     * filter (long_seeds_filter :2106-2117) and read-start adjustment
+     * (adjust_ref_pos_rmdup :1903-1905).  The <= 32 loads go out together. */
     auto take = [&](uint32_t g, uint32_t len, uint32_t meta) {
         const uint32_t bo = meta & 0xffffu, rl = (meta >> 16) & 7u, rh = (meta >> 19) & 7u, q = (meta >> 22) & 3u;
-        for (uint32_t j = 0; j < len; j++) {
-            uint32_t ls = lseeds[g + j];
-            if (first_pending) { first_pending = false; continue; }
-            uint32_t res = (((ls >> (3 - q)) ^ rl) | ((ls >> (9 - q)) ^ rh)) & 7u;
-            if (res == 0) {
-                uint32_t p = positions[g + j];
-                surv[nsurv++] = p > bo ? p - bo : 0u;
+        uint32_t lv[SWM_CHUNK], pv[SWM_CHUNK];
+#pragma unroll
+        for (int j = 0; j < SWM_CHUNK; j++) {
+            if ((uint32_t)j < len) { const uint2 v = pl[g + j]; pv[j] = v.x; lv[j] = v.y; }
+        }
+#pragma unroll
+        for (int j = 0; j < SWM_CHUNK; j++) {
+            if ((uint32_t)j < len) {
+                if (first_pending) {
+                    first_pending = false;
+                } else {
+                    const uint32_t res = (((lv[j] >> (3 - q)) ^ rl) | ((lv[j] >> (9 - q)) ^ rh)) & 7u;
+                    if (res == 0) surv[nsurv++] = pv[j] > bo ? pv[j] - bo : 0u;
+                }
             }
         }
     };
@@ -326,17 +374,24 @@ __global__ void __launch_bounds__(64) k_plan(uint32_t nrs, const uint32_t *__res
         }
     };
 
-    /* phase 1 (:2194-2236) */
+    /* phase 1 (:2194-2236); the next selection entry is in flight while
+     * the current chunk is filtered */
+    bool heavy = false;
+    swm_sel_t s_next = nsel ? sel[0] : swm_sel_t{0, 0, 0};
     for (uint32_t i = 0; i < nsel; i++) {
-        swm_sel_t s = sel[i];
+        const swm_sel_t s = s_next;
+        if (i + 1 < nsel) s_next = sel[i + 1];
         uint32_t get = s.cnt > SWM_CHUNK ? SWM_CHUNK : s.cnt;
         if (get > 0) { take(s.off, get, s.meta); blocks++; }
         pos_size += get;
         pcount[i] = get;
+        heavy |= get != s.cnt;
     }
-    for (uint32_t i = 0; i < nsel; i++) {                     /* :2244-2252 */
-        swm_sel_t s = sel[i];
-        if (pcount[i] != s.cnt) heap_insert(positions[s.off + pcount[i] - 1], i);
+    if (heavy) {
+        for (uint32_t i = 0; i < nsel; i++) {                 /* :2244-2252 */
+            swm_sel_t s = sel[i];
+            if (pcount[i] != s.cnt) heap_insert(pl[s.off + pcount[i] - 1].x, i);
+        }
     }
     for (;;) {                                                /* :2261-2419 */
         if (hsize > 0 && pos_size + SWM_CHUNK <= SWM_BATCH_POS && blocks < SWM_MAX_BLOCKS) {
@@ -348,13 +403,28 @@ __global__ void __launch_bounds__(64) k_plan(uint32_t nrs, const uint32_t *__res
             pos_size += get;
             pcount[i] += get;
             blocks++;
-            heap_pop_insert(positions[s.off + pcount[i] - 1], pcount[i] == s.cnt);
+            heap_pop_insert(pl[s.off + pcount[i] - 1].x, pcount[i] == s.cnt);
         }
         if (pos_size + SWM_CHUNK > SWM_BATCH_POS || hsize == 0 || blocks >= SWM_MAX_BLOCKS) {
-            uint32_t u = 0;
-            sort_unique(surv + bstart, nsurv - bstart, &u);
-            bd[batch] = make_uint2(bstart, u);
-            nsurv = bstart + u;
+            /* end of batch: the sorted unique list (block_heap_sort's result) */
+            uint32_t n = nsurv - bstart;
+            if (n <= SMALL_BATCH) {
+                uint32_t *a = surv + bstart;
+                for (uint32_t i = 1; i < n; i++) {
+                    uint32_t v = a[i], j = i;
+                    while (j > 0 && a[j - 1] > v) { a[j] = a[j - 1]; j--; }
+                    a[j] = v;
+                }
+                uint32_t u = n ? 1u : 0u;
+                for (uint32_t i = 1; i < n; i++)
+                    if (a[i] != a[u - 1]) a[u++] = a[i];
+                bd[batch] = make_uint2(bstart, u);
+                nsurv = bstart + u;
+            } else {
+                bd[batch] = make_uint2(bstart, n);
+                uint32_t slot = atomicAdd(io.big_count, 1u);
+                io.big[slot] = make_uint2(r, batch);
+            }
             bstart = nsurv;
             batch++;
             pos_size = 0;
@@ -367,6 +437,66 @@ __global__ void __launch_bounds__(64) k_plan(uint32_t nrs, const uint32_t *__res
     io.nuniq[r] = nsurv;
 }
 
+/* One workgroup per large batch (<= 2048 values): bitonic sort in LDS, then
+ * unique by a block-wide prefix over "differs from the left neighbour". */
+__global__ void __launch_bounds__(256) k_bigsort(const uint2 *__restrict__ big, const uint32_t *__restrict__ big_count,
+                                                 const uint64_t *__restrict__ surv_off,
+                                                 const uint32_t *__restrict__ bd_off, uint32_t *__restrict__ surv,
+                                                 uint2 *__restrict__ bdesc, uint32_t *__restrict__ nuniq)
+{
+    __shared__ uint32_t a[SWM_BATCH_POS];
+    __shared__ uint32_t part[256];
+    const uint32_t nb = *big_count;
+    const uint32_t tid = threadIdx.x;
+    for (uint32_t b = blockIdx.x; b < nb; b += gridDim.x) {
+        const uint2 rb = big[b];
+        uint2 *dp = bdesc + bd_off[rb.x] + rb.y;
+        const uint2 d = *dp;
+        const uint32_t n = d.y;
+        uint32_t *base = surv + surv_off[rb.x] + d.x;
+        uint32_t P = 64;
+        while (P < n) P <<= 1;
+        for (uint32_t i = tid; i < P; i += 256) a[i] = i < n ? base[i] : 0xffffffffu;
+        __syncthreads();
+        for (uint32_t kk = 2; kk <= P; kk <<= 1) {
+            for (uint32_t j = kk >> 1; j > 0; j >>= 1) {
+                for (uint32_t i = tid; i < P; i += 256) {
+                    const uint32_t ixj = i ^ j;
+                    if (ixj > i) {
+                        const bool up = (i & kk) == 0;
+                        const uint32_t x = a[i], y = a[ixj];
+                        if ((x > y) == up) { a[i] = y; a[ixj] = x; }
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        /* thread tid owns elements [tid*per, tid*per + per) */
+        const uint32_t per = (P + 255) / 256;
+        const uint32_t lo = tid * per;
+        uint32_t cnt = 0;
+        for (uint32_t i = lo; i < lo + per && i < n; i++) cnt += (i == 0 || a[i] != a[i - 1]) ? 1u : 0u;
+        part[tid] = cnt;
+        __syncthreads();
+        for (uint32_t off = 1; off < 256; off <<= 1) {
+            uint32_t v = tid >= off ? part[tid - off] : 0u;
+            __syncthreads();
+            part[tid] += v;
+            __syncthreads();
+        }
+        uint32_t w = part[tid] - cnt;
+        const uint32_t total = part[255];
+        for (uint32_t i = lo; i < lo + per && i < n; i++)
+            if (i == 0 || a[i] != a[i - 1]) base[w++] = a[i];
+        __syncthreads();
+        if (tid == 0) {
+            dp->y = total;
+            atomicSub(&nuniq[rb.x], n - total);
+        }
+        __syncthreads();
+    }
+}
+
 /* --------------------------------------------------------------- k_compact */
 
 struct CandSet {
@@ -375,7 +505,12 @@ struct CandSet {
     uint32_t *bi;       /* batch << 12 | index within batch */
     int32_t *err;       /* -2: not verified (pos_start rejected) */
     int32_t *loc;
+    uint32_t *rec;      /* record location (cigar->location) for passing lanes */
+    uint32_t *info;     /* CIGAR bytes | kind bit 16 (1 = traceback) | bit 17: record location < 0 */
 };
+
+#define INFO_TRACE (1u << 16)
+#define INFO_NEG (1u << 17)
 
 __global__ void __launch_bounds__(128) k_compact(uint32_t nrs, const uint64_t *__restrict__ surv_off,
                                                  const uint32_t *__restrict__ surv, const uint32_t *__restrict__ bd_off,
@@ -404,19 +539,26 @@ __global__ void __launch_bounds__(128) k_compact(uint32_t nrs, const uint64_t *_
 /* Banded Myers for one candidate (the recurrence of mapper_cpe.c:570-631 with
  * the end scan of :634-650).  The window planes are aligned at the window
  * start; at read base t the match mask over band bits j = window base t + j
- * is built from the plane bits (the compiled kernel's form, :845-857). */
+ * is built from the plane bits (the compiled kernel's form, :845-857).  With
+ * STORE the per-step D0 and HP words are kept for the traceback. */
 #define SWM_NW 9   /* window words: ((254 - 1) >> 5) + 2 */
 
+template <bool STORE>
 __device__ __forceinline__ void myers_lane(const uint32_t *__restrict__ rw, uint32_t L, uint32_t e,
                                            uint32_t ref_len, uint32_t (&wl)[SWM_NW], uint32_t (&wh)[SWM_NW],
-                                           int32_t *err_out, int32_t *loc_out)
+                                           int32_t *err_out, int32_t *loc_out, uint32_t *d0a, uint32_t *hpa)
 {
+    /* the read's 2-bit words are loaded once and rotate with the window, so
+     * the step loop touches no memory */
+    uint32_t rv[16];
+#pragma unroll
+    for (int j = 0; j < 16; j++) rv[j] = rw[j];
     const uint32_t B = 2 * e + 1;
     const uint32_t mask = (1u << B) - 1u;
     uint32_t vp = 0, vn = 0, ones = 0;
     const uint32_t nfull = L >> 5, rem = L & 31u;
 
-#define SWM_STEP(S, R)                                                                     \
+#define SWM_STEP(S, R, T)                                                                  \
     {                                                                                      \
         const int sb = 2 * ((S) & 15);                                                     \
         uint32_t RL = (uint32_t)(((int32_t)((R) << (31 - sb))) >> 31);                     \
@@ -432,28 +574,31 @@ __device__ __forceinline__ void myers_lane(const uint32_t *__restrict__ rw, uint
         vn = x & hp;                                                                       \
         vp = hn | ~(x | hp);                                                               \
         ones += d0 & 1u;                                                                   \
+        if (STORE) { d0a[(T)] = d0; hpa[(T)] = hp; }                                       \
     }
 
     for (uint32_t c = 0; c < nfull; c++) {
-        const uint32_t r0 = rw[2 * c], r1 = rw[2 * c + 1];
+        const uint32_t r0 = rv[0], r1 = rv[1];
         const uint32_t l0 = wl[0], l1 = wl[1], h0 = wh[0], h1 = wh[1];
 #pragma unroll
-        for (int s = 0; s < 16; s++) SWM_STEP(s, r0)
+        for (int s = 0; s < 16; s++) SWM_STEP(s, r0, 32 * c + s)
 #pragma unroll
-        for (int s = 16; s < 32; s++) SWM_STEP(s, r1)
+        for (int s = 16; s < 32; s++) SWM_STEP(s, r1, 32 * c + s)
 #pragma unroll
         for (int j = 0; j < SWM_NW - 1; j++) { wl[j] = wl[j + 1]; wh[j] = wh[j + 1]; }
+#pragma unroll
+        for (int j = 0; j < 14; j++) rv[j] = rv[j + 2];
     }
     if (rem) {
-        const uint32_t r0 = rw[2 * nfull], r1 = (rem > 16) ? rw[2 * nfull + 1] : 0u;
+        const uint32_t r0 = rv[0], r1 = rv[1];
         const uint32_t l0 = wl[0], l1 = wl[1], h0 = wh[0], h1 = wh[1];
 #pragma unroll
         for (int s = 0; s < 16; s++) {
-            if ((uint32_t)s < rem) SWM_STEP(s, r0)
+            if ((uint32_t)s < rem) SWM_STEP(s, r0, 32 * nfull + s)
         }
 #pragma unroll
         for (int s = 16; s < 32; s++) {
-            if ((uint32_t)s < rem) SWM_STEP(s, r1)
+            if ((uint32_t)s < rem) SWM_STEP(s, r1, 32 * nfull + s)
         }
     }
 #undef SWM_STEP
@@ -474,12 +619,15 @@ __device__ __forceinline__ void myers_lane(const uint32_t *__restrict__ rw, uint
 __device__ __forceinline__ void load_window(const uint32_t *__restrict__ b2, uint32_t win_start, uint32_t L,
                                             uint32_t (&wl)[SWM_NW], uint32_t (&wh)[SWM_NW])
 {
+    /* all SWM_NW + 1 pairs, unconditionally: a window starts at a genome
+     * position, and the device genome has SWM_WIN_PAIRS_PAD zero pairs past
+     * its end, so the loads stay in bounds and go out together */
     const uint2 *P = reinterpret_cast<const uint2 *>(b2) + (win_start >> 5);
     const uint32_t sh = win_start & 31u;
-    const uint32_t nneed = ((L - 1) >> 5) + 2;
+    (void)L;
     uint2 pr[SWM_NW + 1];
 #pragma unroll
-    for (int j = 0; j < SWM_NW + 1; j++) pr[j] = ((uint32_t)j <= nneed) ? P[j] : make_uint2(0u, 0u);
+    for (int j = 0; j < SWM_NW + 1; j++) pr[j] = P[j];
 #pragma unroll
     for (int j = 0; j < SWM_NW; j++) {
         wl[j] = __builtin_amdgcn_alignbit(pr[j + 1].x, pr[j].x, sh);
@@ -489,11 +637,9 @@ __device__ __forceinline__ void load_window(const uint32_t *__restrict__ b2, uin
 
 __global__ void __launch_bounds__(256) k_myers(uint32_t ncand, const uint32_t *__restrict__ lens,
                                                const uint32_t *__restrict__ words, const uint32_t *__restrict__ b2,
-                                               uint64_t ref_length, uint32_t e, CandSet c,
-                                               unsigned long long *n_verif)
+                                               uint64_t ref_length, uint32_t e, CandSet c)
 {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    bool valid = false;
     if (i < ncand) {
         const uint32_t rs = c.rs[i];
         const uint32_t L = lens[rs >> 1];
@@ -505,15 +651,63 @@ __global__ void __launch_bounds__(256) k_myers(uint32_t ncand, const uint32_t *_
         } else {
             uint32_t wl[SWM_NW], wh[SWM_NW];
             load_window(b2, st, L, wl, wh);
+            const uint32_t *w = words + (uint64_t)rs * 32;
             int32_t er, lc;
-            myers_lane(words + (uint64_t)rs * 32, L, e, ref_len, wl, wh, &er, &lc);
+            myers_lane<false>(w, L, e, ref_len, wl, wh, &er, &lc, nullptr, nullptr);
             c.err[i] = er;
             c.loc[i] = lc;
-            valid = true;
         }
     }
-    unsigned long long bal = __ballot(valid);
-    if ((threadIdx.x & 63) == 0 && bal) atomicAdd(n_verif, (unsigned long long)__popcll(bal));
+}
+
+/* Hamming check of align_reads (:252-287) for the lanes with err <= e: the
+ * read against the window at the implied start loc - L + 1.  Equal counts
+ * give a "<L>M" record; the rest go to the traceback. */
+__global__ void __launch_bounds__(256) k_hamming(uint32_t ncand, const uint32_t *__restrict__ lens,
+                                                 const uint32_t *__restrict__ words, const uint32_t *__restrict__ b2,
+                                                 uint64_t ref_length, uint32_t e, CandSet c,
+                                                 uint32_t *__restrict__ trace, uint32_t *__restrict__ trace_count,
+                                                 uint32_t *__restrict__ n_rejected)
+{
+    __shared__ uint32_t s_tmp[2];
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool to_trace = false;
+    if (i < ncand) {
+        const int32_t er = c.err[i];
+        if (er == -2) {
+            atomicAdd(n_rejected, 1u);         /* pos_start rejected the window: not a lane */
+        } else if (er <= (int32_t)e) {
+            const int32_t lc = c.loc[i];
+            const uint32_t rs = c.rs[i];
+            const uint32_t L = lens[rs >> 1];
+            uint32_t ref_len;
+            const uint32_t st = pos_start(ref_length, e, c.val[i], L, &ref_len);
+            const uint32_t *w = words + (uint64_t)rs * 32;
+            uint32_t wl[SWM_NW], wh[SWM_NW];
+            load_window(b2, st, L, wl, wh);
+            const int sl = lc - (int)L + 1;
+            int tmp = 0;
+#pragma unroll
+            for (int ii = 0; ii < SWM_NW - 1; ii++) {
+                if ((uint32_t)(32 * ii) < L) {
+                    uint32_t mask = 0xffffffffu;
+                    if ((uint32_t)(32 * (ii + 1)) > L) mask >>= (32 - (L - 32 * ii));
+                    const uint32_t lref = __builtin_amdgcn_alignbit(wl[ii + 1], wl[ii], (uint32_t)sl);
+                    const uint32_t href = __builtin_amdgcn_alignbit(wh[ii + 1], wh[ii], (uint32_t)sl);
+                    tmp += __popc(((w[17 + 2 * ii] ^ href) | (w[16 + 2 * ii] ^ lref)) & mask);
+                }
+            }
+            if (tmp == er) {
+                c.rec[i] = st + (uint32_t)sl + 1u;
+                c.info[i] = 2u + (uint32_t)ndigits(L) + 1u;
+            } else {
+                c.rec[i] = 0;
+                c.info[i] = INFO_TRACE;
+                to_trace = true;
+            }
+        }
+    }
+    block_append(to_trace, i, trace, trace_count, s_tmp);
 }
 
 /* Stage entry: explicit candidates. */
@@ -532,182 +726,179 @@ __global__ void __launch_bounds__(256) k_myers_explicit(uint32_t n, const uint32
     uint32_t wl[SWM_NW], wh[SWM_NW];
     load_window(b2, wstart[i], L, wl, wh);
     int32_t er, lc;
-    myers_lane(read_words + (uint64_t)rr * 16, L, e, rlen[i], wl, wh, &er, &lc);
+    myers_lane<false>(read_words + (uint64_t)rr * 16, L, e, rlen[i], wl, wh, &er, &lc, nullptr, nullptr);
     err[i] = er;
     loc[i] = lc;
 }
 
+/* ----------------------------------------------------------------- k_trace */
+
+/* The traceback of align_reads (:288-509) for one passing candidate whose
+ * Hamming count differs from its edit distance: the record's start location
+ * and the bytes its CIGAR adds (the latter drives the flush test).
+ *
+ * The reference keeps D0 and HP of every read base (d0_arry_64/hp_arry_64)
+ * and walks them backwards.  Here the forward pass keeps (VP, VN) at every
+ * 32-base chunk start; the walk recomputes one chunk at a time into
+ * registers (static indices, no scratch) and walks it from its last base.
+ * The CIGAR byte count is accumulated as the runs close: adjacent M/S runs
+ * form one "<n>M" group, every I or D run its own "<n>I"/"<n>D". */
+__device__ __forceinline__ void close_run(char c, int n, uint32_t &ms_acc, uint32_t &text)
+{
+    if (c == 'M' || c == 'S') {
+        ms_acc += (uint32_t)n;
+    } else {
+        if (ms_acc) { text += (uint32_t)ndigits(ms_acc) + 1u; ms_acc = 0; }
+        text += (uint32_t)ndigits((uint32_t)n) + 1u;
+    }
+}
+
+__global__ void __launch_bounds__(64) k_trace(const uint32_t *__restrict__ trace, const uint32_t *__restrict__ trace_count,
+                                              const uint32_t *__restrict__ lens, const uint32_t *__restrict__ words,
+                                              const uint32_t *__restrict__ b2, uint64_t ref_length, uint32_t e,
+                                              CandSet c)
+{
+    __shared__ uint32_t s_ck[64 * 16];
+    uint32_t *ck = s_ck + threadIdx.x * 16;
+    const uint32_t nt = *trace_count;
+    for (uint32_t ti = blockIdx.x * blockDim.x + threadIdx.x; ti < nt; ti += gridDim.x * blockDim.x) {
+        const uint32_t ci = trace[ti];
+        const uint32_t rs = c.rs[ci];
+        const uint32_t L = lens[rs >> 1];
+        uint32_t ref_len;
+        const uint32_t st = pos_start(ref_length, e, c.val[ci], L, &ref_len);
+        const uint32_t *w = words + (uint64_t)rs * 32;
+        const uint32_t mask = (1u << (2 * e + 1)) - 1u;
+        const uint32_t nfull = L >> 5, rem = L & 31u, nch = nfull + (rem ? 1u : 0u);
+
+#define TR_STEP(S, R)                                                                      \
+        {                                                                                  \
+            const int sb = 2 * ((S) & 15);                                                 \
+            uint32_t RL = (uint32_t)(((int32_t)((R) << (31 - sb))) >> 31);                 \
+            uint32_t RH = (uint32_t)(((int32_t)((R) << (30 - sb))) >> 31);                 \
+            uint32_t bl = __builtin_amdgcn_alignbit(l1, l0, (S));                          \
+            uint32_t bh = __builtin_amdgcn_alignbit(h1, h0, (S));                          \
+            uint32_t eq = ~((bl ^ RL) | (bh ^ RH)) & mask;                                 \
+            uint32_t x = eq | vn;                                                          \
+            uint32_t d0 = (((x & vp) + vp) ^ vp) | x;                                      \
+            uint32_t hn = vp & d0;                                                         \
+            uint32_t hp = vn | ~(vp | d0);                                                 \
+            x = d0 >> 1;                                                                   \
+            vn = x & hp;                                                                   \
+            vp = hn | ~(x | hp);                                                           \
+            d0r[(S)] = d0;                                                                 \
+            hpr[(S)] = hp;                                                                 \
+        }
+
+        /* forward: checkpoints at chunk starts */
+        uint32_t vp = 0, vn = 0;
+        uint32_t d0r[32], hpr[32];
+        for (uint32_t cc = 0; cc < nfull; cc++) {
+            ck[2 * cc] = vp;
+            ck[2 * cc + 1] = vn;
+            const uint32_t r0 = w[2 * cc], r1 = w[2 * cc + 1];
+            const uint32_t l0 = genome_word(b2, (int64_t)st + 32 * cc, 0), l1 = genome_word(b2, (int64_t)st + 32 * cc + 32, 0);
+            const uint32_t h0 = genome_word(b2, (int64_t)st + 32 * cc, 1), h1 = genome_word(b2, (int64_t)st + 32 * cc + 32, 1);
+#pragma unroll
+            for (int s = 0; s < 16; s++) TR_STEP(s, r0)
+#pragma unroll
+            for (int s = 16; s < 32; s++) TR_STEP(s, r1)
+        }
+        if (rem) { ck[2 * nfull] = vp; ck[2 * nfull + 1] = vn; }
+
+        /* backward walk (:363-466) */
+        const int error = c.err[ci], location = c.loc[ci];
+        int start_location = location - (int)L + 1;
+        int match_site = location;
+        int search_site = match_site - ((int)L - 1);
+        int i = (int)L - 1;
+        int sum_err = 0;
+        bool first = true, done = false;
+        char pre_char = 'N';
+        int pre_size = 0;
+        uint32_t ms_acc = 0, text = 2;
+        for (int cc = (int)nch - 1; cc >= 0 && !done && i >= 0; cc--) {
+            const uint32_t steps = ((uint32_t)cc == nfull) ? rem : 32u;
+            vp = ck[2 * cc];
+            vn = ck[2 * cc + 1];
+            const uint32_t r0 = w[2 * cc], r1 = w[2 * cc + 1];
+            const int64_t g0 = (int64_t)st + 32 * (int64_t)cc;
+            const uint32_t wml = genome_word(b2, g0 - 32, 0), wmh = genome_word(b2, g0 - 32, 1);
+            const uint32_t l0 = genome_word(b2, g0, 0), l1 = genome_word(b2, g0 + 32, 0);
+            const uint32_t h0 = genome_word(b2, g0, 1), h1 = genome_word(b2, g0 + 32, 1);
+#pragma unroll
+            for (int s = 0; s < 32; s++) {
+                if ((uint32_t)s < steps) {
+                    if (s < 16) TR_STEP(s, r0)
+                    else TR_STEP(s, r1)
+                }
+            }
+            /* window base at match_site (REF_LBIT/REF_HBIT, :374-378) */
+            auto refb = [&](int ms) -> uint32_t {
+                const int wi = ms >> 5, sh = ms & 31;
+                uint32_t lo, hi;
+                if (wi == cc) { lo = l0; hi = h0; }
+                else if (wi == cc + 1) { lo = l1; hi = h1; }
+                else if (wi == cc - 1) { lo = wml; hi = wmh; }
+                else { lo = genome_word(b2, (int64_t)st + 32 * (int64_t)wi, 0); hi = genome_word(b2, (int64_t)st + 32 * (int64_t)wi, 1); }
+                return ((lo >> sh) & 1u) | (((hi >> sh) & 1u) << 1);
+            };
+#pragma unroll
+            for (int s = 31; s >= 0; s--) {
+                const int t = 32 * cc + s;
+                const uint32_t qb = ((s < 16 ? r0 : r1) >> (2 * (s & 15))) & 3u;
+                while (!done && i == t) {
+                    if (!first && sum_err == error) { done = true; break; }
+                    const uint32_t bd = (search_site < 0 || search_site >= 32) ? 0u : ((d0r[s] >> search_site) & 1u);
+                    const uint32_t bh = (search_site < 0 || search_site >= 32) ? 0u : ((hpr[s] >> search_site) & 1u);
+                    const uint32_t rb = refb(match_site);
+                    char nc;
+                    if (bd && rb == qb) { i--; match_site--; nc = 'M'; }
+                    else if (!bd && rb != qb) { i--; match_site--; sum_err++; nc = 'S'; }
+                    else if (bh) { i--; search_site++; sum_err++; start_location++; nc = first ? 'S' : 'I'; }
+                    else { search_site--; match_site--; sum_err++; start_location--; nc = 'D'; }
+                    if (first) { pre_char = nc; pre_size = 1; first = false; }
+                    else if (nc != pre_char) { close_run(pre_char, pre_size, ms_acc, text); pre_char = nc; pre_size = 1; }
+                    else pre_size++;
+                }
+            }
+        }
+#undef TR_STEP
+        close_run(pre_char, pre_size, ms_acc, text);                                 /* :468-473 */
+        if (i >= 0) close_run('M', i + 1, ms_acc, text);
+        if (ms_acc) text += (uint32_t)ndigits(ms_acc) + 1u;
+        const uint64_t new_loc = (uint64_t)st + (uint64_t)(int64_t)start_location + 1;
+        c.rec[ci] = (uint32_t)new_loc;
+        c.info[ci] = (text & 0xffffu) | INFO_TRACE | ((new_loc >> 32) ? INFO_NEG : 0u);
+    }
+}
+
 /* ------------------------------------------------------------------ k_emit */
 
-__device__ __forceinline__ int ndigits(uint32_t v)
-{
-    int n = 0;
-    while (v) { n++; v /= 10; }
-    return n;
-}
-
-struct EmitState {
-    uint32_t cigar_num, cigar_len, last_loc;
-    uint32_t count;
-    uint32_t best_pos;
-    uint32_t best_dist;
-    uint32_t best_strand;
-};
-
-__device__ __forceinline__ void emit_record(EmitState &em, uint32_t loc, uint32_t dist, uint32_t strand,
-                                            uint32_t text_len)
-{
-    em.cigar_num++;
-    em.cigar_len += text_len;
-    em.last_loc = loc;
-    if (em.count == 0 || dist < em.best_dist) {
-        em.best_pos = loc;
-        em.best_dist = dist;
-        em.best_strand = strand;
-    }
-    em.count++;
-}
-
-/* align_reads for one passing lane (mapper_cpe.c:243-509). */
-__device__ void align_one(EmitState &em, const uint32_t *__restrict__ w, const uint32_t *__restrict__ b2,
-                          uint32_t win_start, uint32_t ref_len, uint32_t L, uint32_t e, int error, int location,
-                          uint32_t strand)
-{
-    int start_location = location - (int)L + 1;
-    int tmp_err = 0;
-    for (uint32_t ii = 0; ii * 32 < L; ii++) {                                   /* Hamming :252-276 */
-        uint32_t mask = 0xffffffffu;
-        if ((ii + 1) * 32 > L) mask >>= (32 - (L - ii * 32));
-        uint32_t l0 = genome_word(b2, (int64_t)win_start + 32 * ii, 0);
-        uint32_t l1 = genome_word(b2, (int64_t)win_start + 32 * (ii + 1), 0);
-        uint32_t h0 = genome_word(b2, (int64_t)win_start + 32 * ii, 1);
-        uint32_t h1 = genome_word(b2, (int64_t)win_start + 32 * (ii + 1), 1);
-        uint32_t lref = shr32(l0, start_location) | shl32(l1, 32 - start_location);
-        uint32_t href = shr32(h0, start_location) | shl32(h1, 32 - start_location);
-        uint32_t x = ((w[16 + 2 * ii + 1] ^ href) | (w[16 + 2 * ii] ^ lref)) & mask;
-        tmp_err += __popc(x);
-    }
-    if (tmp_err == error) {
-        emit_record(em, (uint32_t)((uint64_t)win_start + (uint64_t)(int64_t)start_location + 1), (uint32_t)error,
-                    strand, 2u + (uint32_t)ndigits(L) + 1u);
-        return;
-    }
-    /* traceback: the recurrence again, keeping D0 and HP (:288-362) */
-    uint32_t d0a[SWM_READ_MAX + 2], hpa[SWM_READ_MAX + 2];
-    {
-        const uint32_t band_length = 2 * e + 1, mask = 1u << (band_length - 1);
-        uint32_t pl = genome_word(b2, win_start, 0), ph = genome_word(b2, win_start, 1);
-        uint32_t peq[4] = {0, 0, 0, 0};
-        for (uint32_t j = 0; j < band_length; j++) peq[((pl >> j) & 1u) | (((ph >> j) & 1u) << 1)] |= 1u << j;
-        uint32_t vp = 0, vn = 0, i_bd = 2 * e;
-        for (uint32_t t = 0; t < L; t++) {
-            uint32_t be = (w[t >> 4] >> ((t & 15) * 2)) & 3u;
-            uint32_t x = peq[be] | vn;
-            uint32_t d0 = (((x & vp) + vp) ^ vp) | x;
-            uint32_t hn = vp & d0;
-            uint32_t hp = vn | ~(vp | d0);
-            x = d0 >> 1;
-            vn = x & hp;
-            vp = hn | ~(x | hp);
-            d0a[t] = d0;
-            hpa[t] = hp;
-            peq[0] >>= 1; peq[1] >>= 1; peq[2] >>= 1; peq[3] >>= 1;
-            ++i_bd;
-            uint32_t gl = genome_word(b2, (int64_t)win_start + (i_bd & ~31u), 0);
-            uint32_t gh = genome_word(b2, (int64_t)win_start + (i_bd & ~31u), 1);
-            peq[((gl >> (i_bd & 31u)) & 1u) | (((gh >> (i_bd & 31u)) & 1u) << 1)] |= mask;
-        }
-    }
-    const int last_high = (int)ref_len - (int)L;
-    int match_site = location;
-    const int site = (int)ref_len - last_high - 1;
-    int search_site = match_site - site;
-    int pre_size = 1;
-    char pre_char = 'N';
-    int i = (int)L - 1;
-    int sum_err = 0;
-    const int err = error;
-    /* runs in reverse order; only their lengths feed the CIGAR byte count */
-    int rsize[SWM_READ_MAX + 8];
-    char rchar[SWM_READ_MAX + 8];
-    auto refb = [&](int rfi) -> uint32_t {
-        int64_t g = (int64_t)win_start + rfi;
-        return (genome_word(b2, g, 0) & 1u) | ((genome_word(b2, g, 1) & 1u) << 1);
-    };
-    auto rdb = [&](int rdi) -> uint32_t { return (w[rdi / 16] >> (2 * (rdi % 16))) & 3u; };
-    auto bit = [&](uint32_t v, int s) -> uint32_t { return (s < 0 || s >= 32) ? 0u : ((v >> s) & 1u); };
-    int j = 1;
-    if (bit(d0a[i], search_site) && refb(match_site) == rdb(i)) {                /* :380-410 */
-        i--; pre_size = 1; pre_char = 'M'; match_site--;
-    } else if (!bit(d0a[i], search_site) && refb(match_site) != rdb(i)) {
-        i--; pre_size = 1; pre_char = 'S'; match_site--; sum_err++;
-    } else if (bit(hpa[i], search_site)) {
-        i--; search_site++; pre_size = 1; pre_char = 'S'; sum_err++; start_location++;
-    } else {
-        search_site--; pre_size = 1; pre_char = 'D'; match_site--; sum_err++; start_location--;
-    }
-    while (i >= 0) {                                                             /* :412-466 */
-        if (sum_err == err) break;
-        char nc;
-        if (bit(d0a[i], search_site) && refb(match_site) == rdb(i)) {
-            i--; match_site--; nc = 'M';
-        } else if (!bit(d0a[i], search_site) && refb(match_site) != rdb(i)) {
-            i--; match_site--; sum_err++; nc = 'S';
-        } else if (bit(hpa[i], search_site)) {
-            i--; search_site++; sum_err++; start_location++; nc = 'I';
-        } else {
-            search_site--; match_site--; sum_err++; start_location--; nc = 'D';
-        }
-        if (pre_char != nc) { rsize[j] = pre_size; rchar[j++] = pre_char; pre_size = 1; pre_char = nc; }
-        else pre_size++;
-    }
-    rsize[j] = pre_size; rchar[j++] = pre_char;                                  /* :468-473 */
-    if (i >= 0) { rsize[j] = i + 1; rchar[j++] = 'M'; }
-    const uint64_t new_loc = (uint64_t)win_start + (uint64_t)(int64_t)start_location + 1;
-    if (em.cigar_num > 0 && (uint64_t)em.last_loc == new_loc) return;          /* :475-477 */
-    uint32_t text_len = 2;                                                       /* :479-507 */
-    for (j = j - 1; j > 0; j--) {
-        if (rchar[j] == 'M' || rchar[j] == 'S') {
-            int size_sm = 0;
-            while (j > 0 && (rchar[j] == 'M' || rchar[j] == 'S')) { size_sm += rsize[j]; j--; }
-            j++;
-            text_len += (uint32_t)ndigits((uint32_t)size_sm) + 1u;
-        } else {
-            text_len += (uint32_t)ndigits((uint32_t)rsize[j]) + 1u;
-        }
-    }
-    emit_record(em, (uint32_t)new_loc, (uint32_t)error, strand, text_len);
-}
-
-__global__ void __launch_bounds__(64) k_emit(uint32_t nreads, const uint32_t *__restrict__ lens,
-                                             const uint32_t *__restrict__ words, const uint32_t *__restrict__ b2,
-                                             uint64_t ref_length, uint32_t e, const uint32_t *__restrict__ cand_off,
-                                             CandSet c, swm_read_result_t *__restrict__ out,
-                                             unsigned long long *n_records)
+/* One read: its candidates in emission order (forward strand, then reverse;
+ * batch by batch; groups of 8 valid lanes).  A Hamming record is always
+ * written; a traceback record is skipped when it repeats the previous record
+ * of the current block (:475-477).  After every group the block is flushed
+ * when 8 more records or 256 more CIGAR bytes might not fit (:1790, :1858),
+ * which resets the "previous record". */
+__global__ void __launch_bounds__(128) k_emit(uint32_t nreads, uint32_t e, const uint32_t *__restrict__ cand_off,
+                                              CandSet c, swm_read_result_t *__restrict__ out,
+                                              unsigned long long *n_records)
 {
     uint32_t rd = blockIdx.x * blockDim.x + threadIdx.x;
-    if (rd >= nreads) return;
-    const uint32_t L = lens[rd];
-    EmitState em;
-    em.count = 0;
-    em.best_pos = 0xffffffffu;
-    em.best_dist = 0xffu;
-    em.best_strand = 0xffu;
-    for (uint32_t strand = 0; strand < 2; strand++) {
+    const bool live = rd < nreads;
+    if (!live) rd = 0;
+    uint32_t count = 0, best_pos = 0xffffffffu, best_dist = 0xffu, best_strand = 0xffu;
+    for (uint32_t strand = 0; strand < 2 && live; strand++) {
         const uint32_t rs = 2 * rd + strand;
-        const uint32_t *w = words + (uint64_t)rs * 32;
-        em.cigar_num = 0;
-        em.cigar_len = 0;
-        em.last_loc = 0;
+        uint32_t cigar_num = 0, cigar_len = 0, last_loc = 0;
         uint32_t cur_batch = 0xffffffffu, vcount = 0;
         bool open = false;
-        auto flush_check = [&]() {                                               /* :1790, :1858 */
-            if (em.cigar_num + 8 > 256 || em.cigar_len + 16 * 16 > 256 * 16) {
-                em.cigar_num = 0;
-                em.cigar_len = 0;
-            }
+        auto flush_check = [&]() {
+            if (cigar_num + 8 > 256 || cigar_len + 16 * 16 > 256 * 16) { cigar_num = 0; cigar_len = 0; }
         };
-        for (uint32_t ci = cand_off[rs]; ci < cand_off[rs + 1]; ci++) {
+        const uint32_t cend = cand_off[rs + 1];
+        for (uint32_t ci = cand_off[rs]; ci < cend; ci++) {
             const uint32_t b = c.bi[ci] >> 12;
             if (b != cur_batch) {
                 if (open) { flush_check(); open = false; }
@@ -715,68 +906,103 @@ __global__ void __launch_bounds__(64) k_emit(uint32_t nreads, const uint32_t *__
                 vcount = 0;
             }
             const int32_t er = c.err[ci];
-            if (er == -2) continue;                                              /* not a lane */
+            if (er == -2) continue;
             if (open && (vcount & 7u) == 0) { flush_check(); open = false; }
             open = true;
             vcount++;
-            if (er <= (int32_t)e) {
-                uint32_t ref_len;
-                const uint32_t st = pos_start(ref_length, e, c.val[ci], L, &ref_len);
-                align_one(em, w, b2, st, ref_len, L, e, er, c.loc[ci], strand);
-            }
+            if (er > (int32_t)e) continue;
+            const uint32_t info = c.info[ci], loc = c.rec[ci];
+            if ((info & INFO_TRACE) && cigar_num > 0 && !(info & INFO_NEG) && last_loc == loc) continue;
+            cigar_num++;
+            cigar_len += info & 0xffffu;
+            last_loc = loc;
+            if (count == 0 || (uint32_t)er < best_dist) { best_pos = loc; best_dist = (uint32_t)er; best_strand = strand; }
+            count++;
         }
         if (open) flush_check();
     }
     swm_read_result_t res;
-    res.pos = em.count ? em.best_pos : 0xffffffffu;
-    res.strand = em.count ? (uint8_t)em.best_strand : (uint8_t)0xff;
-    res.dist = em.count ? (uint8_t)em.best_dist : (uint8_t)0xff;
+    res.pos = count ? best_pos : 0xffffffffu;
+    res.strand = count ? (uint8_t)best_strand : (uint8_t)0xff;
+    res.dist = count ? (uint8_t)best_dist : (uint8_t)0xff;
     res.pad = 0;
-    res.count = em.count;
-    out[rd] = res;
-    if (em.count) atomicAdd(n_records, (unsigned long long)em.count);
+    res.count = count;
+    if (live) out[rd] = res;
+    __shared__ uint32_t s_sum;
+    if (threadIdx.x == 0) s_sum = 0;
+    __syncthreads();
+    if (count) atomicAdd(&s_sum, count);
+    __syncthreads();
+    if (threadIdx.x == 0 && s_sum) atomicAdd(n_records, (unsigned long long)s_sum);
 }
 
 /* --------------------------------------------------------------- host side */
-
-template <typename T>
-int dalloc(T **p, size_t n)
-{
-    *p = nullptr;
-    if (n == 0) n = 1;
-    return hipMalloc((void **)p, n * sizeof(T)) == hipSuccess ? SWM_OK : SWM_ENOMEM;
-}
-
-struct Bufs {
-    void *ptrs[32];
-    int n = 0;
-    ~Bufs() { for (int i = 0; i < n; i++) (void)hipFree(ptrs[i]); }
-    template <typename T>
-    int get(T **p, size_t cnt)
-    {
-        int rc = dalloc(p, cnt);
-        if (rc == SWM_OK) ptrs[n++] = (void *)*p;
-        return rc;
-    }
-};
-
-template <typename T>
-int exclusive_scan(T *d_in, T *d_out, size_t n, hipStream_t st)
-{
-    size_t tmp = 0;
-    if (hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, d_in, d_out, n, st) != hipSuccess) return SWM_EDEVICE;
-    void *d_tmp = nullptr;
-    if (hipMalloc(&d_tmp, tmp ? tmp : 1) != hipSuccess) return SWM_ENOMEM;
-    hipError_t e = hipcub::DeviceScan::ExclusiveSum(d_tmp, tmp, d_in, d_out, n, st);
-    (void)hipFree(d_tmp);
-    return e == hipSuccess ? SWM_OK : SWM_EDEVICE;
-}
 
 inline uint32_t blocks_for(uint64_t n, uint32_t bs) { return (uint32_t)((n + bs - 1) / bs); }
 
 }  // namespace
 
-struct swm_ws {};
+/* Device workspace of a context, grown on demand and reused by every batch
+ * (no allocation on the steady path). */
+struct swm_ws {
+    struct Buf { void *p = nullptr; size_t cap = 0; };
+    enum { WORDS, SEL, NSEL, NPOS, NCHUNK, SURV_OFF, BD_OFF, SURV, BDESC, NBATCH, NUNIQ, CAND_OFF, C_VAL, C_RS, C_BI,
+           C_ERR, C_LOC, C_REC, C_INFO, COUNTERS, BIG, TRACE, PASS, SCAN_TMP, NBUF };
+    Buf b[NBUF];
+    ~swm_ws() { for (auto &x : b) if (x.p) (void)hipFree(x.p); }
+    template <typename T>
+    int get(int id, T **out, size_t n)
+    {
+        size_t bytes = (n ? n : 1) * sizeof(T);
+        if (b[id].cap < bytes) {
+            if (b[id].p) (void)hipFree(b[id].p);
+            b[id].p = nullptr;
+            b[id].cap = 0;
+            size_t want = bytes + bytes / 4;
+            if (hipMalloc(&b[id].p, want) != hipSuccess) return SWM_ENOMEM;
+            b[id].cap = want;
+        }
+        *out = (T *)b[id].p;
+        return SWM_OK;
+    }
+};
+
+static int ws_scan_u32(swm_ws *ws, uint32_t *in, uint32_t *out, size_t n, hipStream_t st)
+{
+    size_t tmp = 0;
+    if (hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, in, out, n, st) != hipSuccess) return SWM_EDEVICE;
+    void *t;
+    int rc = ws->get(swm_ws::SCAN_TMP, (char **)&t, tmp);
+    if (rc) return rc;
+    return hipcub::DeviceScan::ExclusiveSum(t, tmp, in, out, n, st) == hipSuccess ? SWM_OK : SWM_EDEVICE;
+}
+
+static int ws_scan_u64(swm_ws *ws, uint64_t *in, uint64_t *out, size_t n, hipStream_t st)
+{
+    size_t tmp = 0;
+    if (hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, in, out, n, st) != hipSuccess) return SWM_EDEVICE;
+    void *t;
+    int rc = ws->get(swm_ws::SCAN_TMP, (char **)&t, tmp);
+    if (rc) return rc;
+    return hipcub::DeviceScan::ExclusiveSum(t, tmp, in, out, n, st) == hipSuccess ? SWM_OK : SWM_EDEVICE;
+}
+
+/* The device keeps each bucket entry as one 8-byte (position, residual)
+ * pair, so a bucket chunk costs one random cache line, not two. */
+static int upload_pl(uint2 *d, const uint32_t *pos, const uint16_t *ls, uint64_t n)
+{
+    const uint64_t step = 1ull << 22;
+    uint2 *h = (uint2 *)malloc((n < step ? n : step) * sizeof(uint2));
+    if (!h) return SWM_ENOMEM;
+    int rc = SWM_OK;
+    for (uint64_t b = 0; b < n && rc == SWM_OK; b += step) {
+        uint64_t m = n - b < step ? n - b : step;
+        for (uint64_t i = 0; i < m; i++) h[i] = make_uint2(pos[b + i], ls[b + i]);
+        if (hipMemcpy(d + b, h, m * sizeof(uint2), hipMemcpyHostToDevice) != hipSuccess) rc = SWM_EDEVICE;
+    }
+    free(h);
+    return rc;
+}
 
 /* ---------------------------------------------------------------- C ABI */
 
@@ -795,33 +1021,43 @@ extern "C" int swm_ctx_create(uint32_t k, const uint32_t *hash_count, const uint
         free(c);
         return SWM_EDEVICE;
     }
+    c->ws = new (std::nothrow) swm_ws();
+    if (!c->ws) { swm_ctx_destroy(c); return SWM_ENOMEM; }
     c->k = k;
     c->n_buckets = 1ull << (2 * k);
     c->n_pos = n_pos;
     c->n_2bits_words = n_2bits_words;
     c->ref_length = 16ull * (n_2bits_words * 4ull);
+    /* read-encoding recipes for every length (reads of length 0 map to nothing) */
+    swm_recipe_t *rec = (swm_recipe_t *)calloc(SWM_RECIPE_MAX_L + 1, sizeof(swm_recipe_t));
+    if (!rec) { swm_ctx_destroy(c); return SWM_ENOMEM; }
+    for (uint32_t L = 0; L <= SWM_RECIPE_MAX_L; L++) {
+        if (L / (k + SWM_SEED_STEP - 1) > 16) continue;       /* such reads are rejected before use */
+        if (swm_recipe_build(L, k, &rec[L]) != 0) { free(rec); swm_ctx_destroy(c); return SWM_EINVAL; }
+    }
     /* hash_offset = exclusive scan of hash_count in u32 (mapper_mpe.c:250-255) */
     uint32_t *h_off = (uint32_t *)malloc((c->n_buckets + 1) * sizeof(uint32_t));
-    if (!h_off) { swm_ctx_destroy(c); return SWM_ENOMEM; }
+    if (!h_off) { free(rec); swm_ctx_destroy(c); return SWM_ENOMEM; }
     h_off[0] = 0;
     for (uint64_t i = 0; i < c->n_buckets; i++) h_off[i + 1] = h_off[i] + hash_count[i];
     int rc = SWM_OK;
+    const size_t rec_bytes = (SWM_RECIPE_MAX_L + 1) * sizeof(swm_recipe_t);
     if (hipMalloc((void **)&c->d_hash_offset, (c->n_buckets + 1) * 4) != hipSuccess ||
-        hipMalloc((void **)&c->d_positions, (n_pos + 64) * 4) != hipSuccess ||
-        hipMalloc((void **)&c->d_lseeds, (n_pos + 64) * 2) != hipSuccess ||
-        hipMalloc((void **)&c->d_base_2bits, (n_2bits_words + 2 * SWM_WIN_PAIRS_PAD) * 4) != hipSuccess)
+        hipMalloc((void **)&c->d_pl, (n_pos + 64) * 8) != hipSuccess ||
+        hipMalloc((void **)&c->d_base_2bits, (n_2bits_words + 2 * SWM_WIN_PAIRS_PAD) * 4) != hipSuccess ||
+        hipMalloc((void **)&c->d_recipe, rec_bytes) != hipSuccess)
         rc = SWM_ENOMEM;
     if (rc == SWM_OK) {
         if (hipMemcpy(c->d_hash_offset, h_off, (c->n_buckets + 1) * 4, hipMemcpyHostToDevice) != hipSuccess ||
-            hipMemset(c->d_positions, 0, (n_pos + 64) * 4) != hipSuccess ||
-            hipMemset(c->d_lseeds, 0, (n_pos + 64) * 2) != hipSuccess ||
-            (n_pos && hipMemcpy(c->d_positions, positions, n_pos * 4, hipMemcpyHostToDevice) != hipSuccess) ||
-            (n_pos && hipMemcpy(c->d_lseeds, pos_lseeds, n_pos * 2, hipMemcpyHostToDevice) != hipSuccess) ||
+            hipMemset(c->d_pl, 0, (n_pos + 64) * 8) != hipSuccess ||
+            (n_pos && upload_pl(c->d_pl, positions, pos_lseeds, n_pos) != SWM_OK) ||
             hipMemset(c->d_base_2bits, 0, (n_2bits_words + 2 * SWM_WIN_PAIRS_PAD) * 4) != hipSuccess ||
-            hipMemcpy(c->d_base_2bits, base_2bits, n_2bits_words * 4, hipMemcpyHostToDevice) != hipSuccess)
+            hipMemcpy(c->d_base_2bits, base_2bits, n_2bits_words * 4, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(c->d_recipe, rec, rec_bytes, hipMemcpyHostToDevice) != hipSuccess)
             rc = SWM_EDEVICE;
     }
     free(h_off);
+    free(rec);
     if (rc != SWM_OK) { swm_ctx_destroy(c); return rc; }
     *out = c;
     return SWM_OK;
@@ -830,10 +1066,12 @@ extern "C" int swm_ctx_create(uint32_t k, const uint32_t *hash_count, const uint
 extern "C" void swm_ctx_destroy(swm_ctx_t *c)
 {
     if (!c) return;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    delete c->ws;
     if (c->d_hash_offset) (void)hipFree(c->d_hash_offset);
-    if (c->d_positions) (void)hipFree(c->d_positions);
-    if (c->d_lseeds) (void)hipFree(c->d_lseeds);
+    if (c->d_pl) (void)hipFree(c->d_pl);
     if (c->d_base_2bits) (void)hipFree(c->d_base_2bits);
+    if (c->d_recipe) (void)hipFree(c->d_recipe);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     free(c);
 }
@@ -849,34 +1087,40 @@ static int map_device_impl(swm_ctx_t *ctx, const char *d_reads, uint64_t stride,
                            uint64_t n, uint32_t e, swm_read_result_t *d_out)
 {
     hipStream_t st = ctx->stream;
+    swm_ws *ws = ctx->ws;
     memset(&ctx->stats, 0, sizeof(ctx->stats));
     ctx->stats.n_reads = n;
     if (n == 0) return SWM_OK;
     const uint64_t nrs = 2 * n;
-    if (nrs >= 0xffffffffull) return SWM_EINVAL;
-    Bufs B;
-    SeedOut so;
+    if (nrs >= 0xfffffff0ull) return SWM_EINVAL;
+    const uint32_t seln = 4 * ((e + 1) < 16 ? (e + 1) : 16);
     int rc;
     hipEvent_t ev[7];
     for (int i = 0; i < 7; i++) SWM_HIP_CHECK(hipEventCreate(&ev[i]));
     struct EvGuard { hipEvent_t *e; ~EvGuard() { for (int i = 0; i < 7; i++) (void)hipEventDestroy(e[i]); } } eg{ev};
 
-    if ((rc = B.get(&so.words, nrs * 32)) || (rc = B.get(&so.sel, nrs * 64)) || (rc = B.get(&so.nsel, nrs)) ||
-        (rc = B.get(&so.npos, nrs + 1)) || (rc = B.get(&so.nchunk, nrs + 1)))
+    SeedOut so;
+    unsigned long long *counters;
+    if ((rc = ws->get(swm_ws::WORDS, &so.words, nrs * 32)) || (rc = ws->get(swm_ws::SEL, &so.sel, nrs * seln)) ||
+        (rc = ws->get(swm_ws::NSEL, &so.nsel, nrs)) || (rc = ws->get(swm_ws::NPOS, &so.npos, nrs + 1)) ||
+        (rc = ws->get(swm_ws::NCHUNK, &so.nchunk, nrs + 1)) || (rc = ws->get(swm_ws::COUNTERS, &counters, 8)))
         return rc;
+    uint32_t *u32c = reinterpret_cast<uint32_t *>(counters + 4);   /* [0] big batches, [1] tracebacks, [2] rejected windows */
+    SWM_HIP_CHECK(hipMemsetAsync(counters, 0, 8 * sizeof(unsigned long long), st));
     SWM_HIP_CHECK(hipMemsetAsync(so.npos + nrs, 0, sizeof(uint64_t), st));
     SWM_HIP_CHECK(hipMemsetAsync(so.nchunk + nrs, 0, sizeof(uint32_t), st));
     SWM_HIP_CHECK(hipEventRecord(ev[0], st));
-    k_seed<<<blocks_for(nrs, 128), 128, 0, st>>>(d_reads, stride, d_lens, (uint32_t)nrs, ctx->k, e, 64,
-                                                 ctx->d_hash_offset, so);
+    k_seed<<<blocks_for(n, SEED_READS), 2 * SEED_READS, 0, st>>>(d_reads, stride, d_lens, (uint32_t)n, ctx->k, e, seln,
+                                                                 ctx->d_hash_offset, ctx->d_recipe, so);
     SWM_HIP_CHECK(hipGetLastError());
     SWM_HIP_CHECK(hipEventRecord(ev[1], st));
 
     PlanIO pio;
     uint64_t *surv_off;
     uint32_t *bd_off;
-    if ((rc = B.get(&surv_off, nrs + 1)) || (rc = B.get(&bd_off, nrs + 1))) return rc;
-    if ((rc = exclusive_scan(so.npos, surv_off, nrs + 1, st)) || (rc = exclusive_scan(so.nchunk, bd_off, nrs + 1, st)))
+    if ((rc = ws->get(swm_ws::SURV_OFF, &surv_off, nrs + 1)) || (rc = ws->get(swm_ws::BD_OFF, &bd_off, nrs + 1)))
+        return rc;
+    if ((rc = ws_scan_u64(ws, so.npos, surv_off, nrs + 1, st)) || (rc = ws_scan_u32(ws, so.nchunk, bd_off, nrs + 1, st)))
         return rc;
     uint64_t tot_pos = 0;
     uint32_t tot_chunk = 0;
@@ -884,57 +1128,57 @@ static int map_device_impl(swm_ctx_t *ctx, const char *d_reads, uint64_t stride,
     SWM_HIP_CHECK(hipMemcpyAsync(&tot_chunk, bd_off + nrs, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
     SWM_HIP_CHECK(hipStreamSynchronize(st));
     pio.sel = so.sel;
+    pio.seln = seln;
     pio.nsel = so.nsel;
     pio.surv_off = surv_off;
     pio.bd_off = bd_off;
-    uint32_t *surv;
-    uint2 *bdesc;
-    uint32_t *nbatch, *nuniq;
-    if ((rc = B.get(&surv, tot_pos + 1)) || (rc = B.get(&bdesc, (uint64_t)tot_chunk + 1)) || (rc = B.get(&nbatch, nrs)) ||
-        (rc = B.get(&nuniq, nrs + 1)))
+    if ((rc = ws->get(swm_ws::SURV, &pio.surv, tot_pos + 1)) || (rc = ws->get(swm_ws::BDESC, &pio.bdesc, (uint64_t)tot_chunk + 1)) ||
+        (rc = ws->get(swm_ws::NBATCH, &pio.nbatch, nrs)) || (rc = ws->get(swm_ws::NUNIQ, &pio.nuniq, nrs + 1)) ||
+        (rc = ws->get(swm_ws::BIG, &pio.big, (uint64_t)tot_chunk + 1)))
         return rc;
-    pio.surv = surv;
-    pio.bdesc = bdesc;
-    pio.nbatch = nbatch;
-    pio.nuniq = nuniq;
-    SWM_HIP_CHECK(hipMemsetAsync(nuniq + nrs, 0, sizeof(uint32_t), st));
-    k_plan<<<blocks_for(nrs, 64), 64, 0, st>>>((uint32_t)nrs, ctx->d_positions, ctx->d_lseeds, pio);
+    pio.big_count = u32c;
+    SWM_HIP_CHECK(hipMemsetAsync(pio.nuniq + nrs, 0, sizeof(uint32_t), st));
+    k_plan<<<blocks_for(nrs, 64), 64, 0, st>>>((uint32_t)nrs, ctx->d_pl, pio);
+    SWM_HIP_CHECK(hipGetLastError());
+    k_bigsort<<<1024, 256, 0, st>>>(pio.big, pio.big_count, surv_off, bd_off, pio.surv, pio.bdesc, pio.nuniq);
     SWM_HIP_CHECK(hipGetLastError());
     SWM_HIP_CHECK(hipEventRecord(ev[2], st));
 
     uint32_t *cand_off;
-    if ((rc = B.get(&cand_off, nrs + 1))) return rc;
-    if ((rc = exclusive_scan(nuniq, cand_off, nrs + 1, st))) return rc;
+    if ((rc = ws->get(swm_ws::CAND_OFF, &cand_off, nrs + 1))) return rc;
+    if ((rc = ws_scan_u32(ws, pio.nuniq, cand_off, nrs + 1, st))) return rc;
     uint32_t ncand = 0;
     SWM_HIP_CHECK(hipMemcpyAsync(&ncand, cand_off + nrs, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
     SWM_HIP_CHECK(hipStreamSynchronize(st));
     CandSet cs;
-    if ((rc = B.get(&cs.val, (uint64_t)ncand + 1)) || (rc = B.get(&cs.rs, (uint64_t)ncand + 1)) ||
-        (rc = B.get(&cs.bi, (uint64_t)ncand + 1)) || (rc = B.get(&cs.err, (uint64_t)ncand + 1)) ||
-        (rc = B.get(&cs.loc, (uint64_t)ncand + 1)))
+    uint32_t *trace;
+    if ((rc = ws->get(swm_ws::C_VAL, &cs.val, (uint64_t)ncand + 1)) || (rc = ws->get(swm_ws::C_RS, &cs.rs, (uint64_t)ncand + 1)) ||
+        (rc = ws->get(swm_ws::C_BI, &cs.bi, (uint64_t)ncand + 1)) || (rc = ws->get(swm_ws::C_ERR, &cs.err, (uint64_t)ncand + 1)) ||
+        (rc = ws->get(swm_ws::C_LOC, &cs.loc, (uint64_t)ncand + 1)) || (rc = ws->get(swm_ws::C_REC, &cs.rec, (uint64_t)ncand + 1)) ||
+        (rc = ws->get(swm_ws::C_INFO, &cs.info, (uint64_t)ncand + 1)) || (rc = ws->get(swm_ws::TRACE, &trace, (uint64_t)ncand + 1)))
         return rc;
-    k_compact<<<blocks_for(nrs, 128), 128, 0, st>>>((uint32_t)nrs, surv_off, surv, bd_off, bdesc, nbatch, cand_off,
-                                                    cs);
+    k_compact<<<blocks_for(nrs, 128), 128, 0, st>>>((uint32_t)nrs, surv_off, pio.surv, bd_off, pio.bdesc, pio.nbatch,
+                                                    cand_off, cs);
     SWM_HIP_CHECK(hipGetLastError());
     SWM_HIP_CHECK(hipEventRecord(ev[3], st));
-
-    unsigned long long *counters;
-    if ((rc = B.get(&counters, 2))) return rc;
-    SWM_HIP_CHECK(hipMemsetAsync(counters, 0, 2 * sizeof(unsigned long long), st));
     if (ncand)
-        k_myers<<<blocks_for(ncand, 256), 256, 0, st>>>(ncand, d_lens, so.words, ctx->d_base_2bits,
-                                                         ctx->ref_length, e, cs, counters);
+        k_myers<<<blocks_for(ncand, 256), 256, 0, st>>>(ncand, d_lens, so.words, ctx->d_base_2bits, ctx->ref_length, e,
+                                                         cs);
     SWM_HIP_CHECK(hipGetLastError());
     SWM_HIP_CHECK(hipEventRecord(ev[4], st));
-    k_emit<<<blocks_for(n, 64), 64, 0, st>>>((uint32_t)n, d_lens, so.words, ctx->d_base_2bits, ctx->ref_length, e,
-                                             cand_off, cs, d_out, counters + 1);
+    if (ncand) {
+        k_hamming<<<blocks_for(ncand, 256), 256, 0, st>>>(ncand, d_lens, so.words, ctx->d_base_2bits, ctx->ref_length,
+                                                           e, cs, trace, u32c + 1, u32c + 2);
+        k_trace<<<1024, 64, 0, st>>>(trace, u32c + 1, d_lens, so.words, ctx->d_base_2bits, ctx->ref_length, e, cs);
+    }
+    k_emit<<<blocks_for(n, 128), 128, 0, st>>>((uint32_t)n, e, cand_off, cs, d_out, counters + 1);
     SWM_HIP_CHECK(hipGetLastError());
     SWM_HIP_CHECK(hipEventRecord(ev[5], st));
-    unsigned long long hc[2] = {0, 0};
+    unsigned long long hc[8];
     SWM_HIP_CHECK(hipMemcpyAsync(hc, counters, sizeof(hc), hipMemcpyDeviceToHost, st));
     SWM_HIP_CHECK(hipStreamSynchronize(st));
     float ms;
-    ctx->stats.n_verifications = hc[0];
+    ctx->stats.n_verifications = ncand - reinterpret_cast<const uint32_t *>(hc + 4)[2];
     ctx->stats.n_records = hc[1];
     (void)hipEventElapsedTime(&ms, ev[0], ev[1]); ctx->stats.ms_seed = ms;
     (void)hipEventElapsedTime(&ms, ev[1], ev[2]); ctx->stats.ms_plan = ms;
@@ -970,8 +1214,10 @@ extern "C" int swm_map_batch(swm_ctx_t *ctx, const char *reads, uint64_t stride,
     if (rc) return rc;
     if (n == 0) return SWM_OK;
     if (!reads || !lens || !out) return SWM_EINVAL;
-    for (uint64_t i = 0; i < n; i++)
+    for (uint64_t i = 0; i < n; i++) {
         if ((uint64_t)lens[i] + 2 > stride) return SWM_EINVAL;   /* kseq stride rule (Q20) */
+        if (lens[i] / (ctx->k + SWM_SEED_STEP - 1) > 16) return SWM_EINVAL;   /* >16 long seeds */
+    }
     if (hipSetDevice(ctx->device) != hipSuccess) return SWM_EDEVICE;
     char *d_reads = nullptr;
     uint32_t *d_lens = nullptr;
@@ -1047,6 +1293,12 @@ extern "C" int swm_copy_d2h(void *dst, const void *src, uint64_t bytes)
 {
     if (bytes && (!dst || !src)) return SWM_EINVAL;
     return (!bytes || hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost) == hipSuccess) ? SWM_OK : SWM_EDEVICE;
+}
+
+extern "C" int swm_copy_d2d(void *dst, const void *src, uint64_t bytes)
+{
+    if (bytes && (!dst || !src)) return SWM_EINVAL;
+    return (!bytes || hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice) == hipSuccess) ? SWM_OK : SWM_EDEVICE;
 }
 
 extern "C" int swm_dev_memset(void *dst, int value, uint64_t bytes)

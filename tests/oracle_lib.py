@@ -148,7 +148,7 @@ def encode_read(read, k=12):
     buf = np.zeros(r.size + 64, dtype=np.uint8)
     buf[:r.size] = r
     words = np.zeros(32, dtype=np.uint32)
-    seeds = np.zeros(64, dtype=np.uint32)
+    seeds = np.zeros(128, dtype=np.uint32)
     n = lib().orc_encode_read(_p(buf), r.size, k, _p(words), _p(seeds))
     return words, seeds[:n]
 
