@@ -66,6 +66,7 @@ typedef struct swm_index_host {
 typedef struct swm_stats {
     uint64_t n_reads;
     uint64_t n_verifications;  /* (read, strand, candidate) windows run through banded Myers */
+    uint64_t n_band_steps;     /* sum of read lengths over those verifications (Myers steps) */
     uint64_t n_records;
     /* stage times (HIP events on the context stream): seed lookup; candidate
      * batches; compaction; banded Myers; Hamming + traceback + emission */
@@ -134,6 +135,11 @@ int swm_myers_device(swm_ctx_t *ctx, const uint32_t *d_read_words, const uint32_
                      const uint32_t *d_cand_read, const uint32_t *d_win_start,
                      const uint32_t *d_ref_len, uint64_t n, uint32_t e,
                      int32_t *d_err, int32_t *d_loc);
+
+/* Measured int32 VALU throughput of the current device (lane ops/s), the
+ * peak the Myers roofline is quoted against.  No counterpart in the
+ * reference. */
+int swm_valu_probe(double *ops_per_s);
 
 /* ---- device memory plumbing for callers without their own HIP setup
  *      (no counterpart in the reference, whose host/CPE memory is shared) ---- */

@@ -44,7 +44,8 @@ class IndexHost(C.Structure):
 
 
 class Stats(C.Structure):
-    _fields_ = [("n_reads", C.c_uint64), ("n_verifications", C.c_uint64), ("n_records", C.c_uint64),
+    _fields_ = [("n_reads", C.c_uint64), ("n_verifications", C.c_uint64), ("n_band_steps", C.c_uint64),
+                ("n_records", C.c_uint64),
                 ("ms_seed", C.c_double), ("ms_plan", C.c_double), ("ms_compact", C.c_double),
                 ("ms_myers", C.c_double), ("ms_emit", C.c_double), ("ms_total", C.c_double)]
 
@@ -74,6 +75,7 @@ def lib():
             "swm_myers_device": ([vp, vp, vp, vp, vp, vp, u64, u32, vp, vp], C.c_int),
             "swm_synth_genome": ([vp, u64, u64, C.c_int, C.c_double], C.c_int),
             "swm_device_count": ([C.POINTER(C.c_int)], C.c_int),
+            "swm_valu_probe": ([C.POINTER(C.c_double)], C.c_int),
             "swm_dev_alloc": ([C.POINTER(vp), u64], C.c_int),
             "swm_dev_free": ([vp], C.c_int),
             "swm_copy_h2d": ([vp, vp, u64], C.c_int),
@@ -226,6 +228,13 @@ class Mapper:
         s = Stats()
         _check("swm_last_stats", lib().swm_last_stats(self._h, C.byref(s)))
         return {f: getattr(s, f) for f, _ in Stats._fields_}
+
+
+def valu_peak():
+    """Measured int32 VALU lane-ops/s of the device."""
+    v = C.c_double(0)
+    _check("swm_valu_probe", lib().swm_valu_probe(C.byref(v)))
+    return v.value
 
 
 def device_count():

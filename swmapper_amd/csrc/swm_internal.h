@@ -34,6 +34,7 @@ struct swm_ws;  /* per-context device workspace */
 
 struct swm_ctx {
     int device;
+    int n_cu;                   /* compute units: sizes the persistent work-queue grids */
     hipStream_t stream;
     uint32_t k;
     uint64_t n_buckets;

@@ -109,6 +109,8 @@ __device__ __forceinline__ void block_append(bool pred, uint32_t value, uint32_t
 
 #define SEED_READS 64        /* reads per workgroup (2 threads each) */
 
+#define NSEL_HEAVY 0x80000000u
+
 struct SeedOut {
     uint32_t *words;    /* [nrs][32]: 16 interleaved 2-bit words, 8 plane pairs */
     swm_sel_t *sel;     /* [nrs][seln] */
@@ -260,6 +262,7 @@ __global__ void __launch_bounds__(128) k_seed(const char *__restrict__ reads, ui
     const uint32_t n_sel_long = (e + 1 < n_long) ? e + 1 : n_long;
     uint64_t npos = 0;
     uint32_t nchunk = 0;
+    bool heavy = false;
     swm_sel_t *so = o.sel + (uint64_t)r * seln;
 #pragma unroll
     for (int jm = 0; jm < 16; jm++) {
@@ -281,12 +284,15 @@ __global__ void __launch_bounds__(128) k_seed(const char *__restrict__ reads, ui
                 s.cnt = s_cnt[(4 * m + q) * (2 * SEED_READS) + threadIdx.x];
                 s.meta = (base + (uint32_t)q) | (rl << 16) | (rh << 19) | ((uint32_t)q << 22);
                 so[4 * jm + q] = s;
+                heavy |= s.cnt > SWM_CHUNK;
                 npos += s.cnt;
                 nchunk += (s.cnt + SWM_CHUNK - 1) / SWM_CHUNK;
             }
         }
     }
-    o.nsel[r] = n_sel_long * SWM_SEED_STEP;
+    /* bit 31: some selected 12-mer has more than one chunk of hits, so the
+     * item needs the seed heap (k_plan_heavy) */
+    o.nsel[r] = n_sel_long * SWM_SEED_STEP | (heavy ? NSEL_HEAVY : 0u);
     o.npos[r] = npos;
     o.nchunk[r] = nchunk + 1;   /* batches <= chunks + 1 (an empty first batch) */
 }
@@ -307,134 +313,266 @@ struct PlanIO {
     uint32_t *nuniq;            /* [nrs + 1] */
     uint2 *big;                 /* (rs, batch) of batches left for k_bigsort */
     uint32_t *big_count;
+    uint32_t *heavy;            /* (read, strand) items whose seeds need the seed heap */
+    uint32_t *heavy_count;
+    uint32_t *heavy_head;       /* work-queue head of k_plan_heavy */
 };
 
+/* 15-mer filter of one bucket entry (long_seeds_filter :2106-2117) and the
+ * read-start adjustment (adjust_ref_pos_rmdup :1903-1905). */
+__device__ __forceinline__ bool filter_pass(uint32_t ls, uint32_t meta)
+{
+    const uint32_t rl = (meta >> 16) & 7u, rh = (meta >> 19) & 7u, q = (meta >> 22) & 3u;
+    return ((((ls >> (3 - q)) ^ rl) | ((ls >> (9 - q)) ^ rh)) & 7u) == 0;
+}
+__device__ __forceinline__ uint32_t adjust(uint32_t p, uint32_t meta)
+{
+    const uint32_t bo = meta & 0xffffu;
+    return p > bo ? p - bo : 0u;
+}
+
+/* Light (read, strand): every selected 12-mer has <= 32 hits, so
+ * alignment_rmdup (:2124-2426) fetches all of them in phase 1, the seed heap
+ * stays empty and there is exactly one batch.  One thread per item; items
+ * needing the heap are queued for k_plan_heavy. */
 __global__ void __launch_bounds__(64) k_plan(uint32_t nrs, const uint2 *__restrict__ pl, PlanIO io)
 {
+    __shared__ uint32_t s_tmp[2];
     uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= nrs) return;
-    const uint32_t nsel = io.nsel[r];
+    const bool live = r < nrs;
+    if (!live) r = 0;
+    const uint32_t nsw = live ? io.nsel[r] : 0u;
+    const uint32_t nsel = nsw & ~NSEL_HEAVY;
     const swm_sel_t *sel = io.sel + (uint64_t)r * io.seln;
+    const bool heavy = (nsw & NSEL_HEAVY) != 0;
+    block_append(live && heavy, r, io.heavy, io.heavy_count, s_tmp);
+    if (!live || heavy) return;
+
     uint32_t *surv = io.surv + io.surv_off[r];
     uint2 *bd = io.bdesc + io.bd_off[r];
-
-    uint32_t pcount[64];
-    uint32_t hloc[66], hind[66];      /* seed heap, 1-based (mapper_cpe.c:2149) */
-    uint32_t hsize = 0;
-    uint32_t nsurv = 0, bstart = 0, batch = 0, pos_size = 0, blocks = 0;
+    uint32_t nsurv = 0;
     bool first_pending = true;        /* Q7: the batch's first position is never kept */
-
-    /* positions [g, g+len) of one selected 12-mer enter the batch: 15-mer
-     * filter (long_seeds_filter :2106-2117) and read-start adjustment
-     * (adjust_ref_pos_rmdup :1903-1905).  The <= 32 loads go out together. */
-    auto take = [&](uint32_t g, uint32_t len, uint32_t meta) {
-        const uint32_t bo = meta & 0xffffu, rl = (meta >> 16) & 7u, rh = (meta >> 19) & 7u, q = (meta >> 22) & 3u;
+    swm_sel_t s_next = nsel ? sel[0] : swm_sel_t{0, 0, 0};
+    for (uint32_t i = 0; i < nsel; i++) {                     /* phase 1 (:2194-2236) */
+        const swm_sel_t s = s_next;
+        if (i + 1 < nsel) s_next = sel[i + 1];
+        const uint32_t len = s.cnt;
         uint32_t lv[SWM_CHUNK], pv[SWM_CHUNK];
 #pragma unroll
         for (int j = 0; j < SWM_CHUNK; j++) {
-            if ((uint32_t)j < len) { const uint2 v = pl[g + j]; pv[j] = v.x; lv[j] = v.y; }
+            if ((uint32_t)j < len) { const uint2 v = pl[s.off + j]; pv[j] = v.x; lv[j] = v.y; }
         }
 #pragma unroll
         for (int j = 0; j < SWM_CHUNK; j++) {
             if ((uint32_t)j < len) {
-                if (first_pending) {
-                    first_pending = false;
-                } else {
-                    const uint32_t res = (((lv[j] >> (3 - q)) ^ rl) | ((lv[j] >> (9 - q)) ^ rh)) & 7u;
-                    if (res == 0) surv[nsurv++] = pv[j] > bo ? pv[j] - bo : 0u;
-                }
+                if (first_pending) first_pending = false;
+                else if (filter_pass(lv[j], s.meta)) surv[nsurv++] = adjust(pv[j], s.meta);
             }
         }
-    };
-    auto heap_insert = [&](uint32_t loc, uint32_t ind) {      /* :1919-1935 */
-        hsize++;
-        hloc[hsize] = loc; hind[hsize] = ind;
-        uint32_t i = hsize;
-        while (i > 1) {
-            uint32_t ti = i / 2;
-            if (hloc[ti] > hloc[i]) {
-                uint32_t t = hloc[ti]; hloc[ti] = hloc[i]; hloc[i] = t;
-                t = hind[ti]; hind[ti] = hind[i]; hind[i] = t;
-            } else break;
-            i = ti;
+    }
+    /* the one flush: sorted unique list (block_heap_sort's result) */
+    uint32_t n = nsurv;
+    if (n <= SMALL_BATCH) {
+        for (uint32_t i = 1; i < n; i++) {
+            uint32_t v = surv[i], j = i;
+            while (j > 0 && surv[j - 1] > v) { surv[j] = surv[j - 1]; j--; }
+            surv[j] = v;
         }
+        uint32_t u = n ? 1u : 0u;
+        for (uint32_t i = 1; i < n; i++)
+            if (surv[i] != surv[u - 1]) surv[u++] = surv[i];
+        bd[0] = make_uint2(0, u);
+        io.nuniq[r] = u;
+    } else {
+        bd[0] = make_uint2(0, n);
+        uint32_t slot = atomicAdd(io.big_count, 1u);
+        io.big[slot] = make_uint2(r, 0);
+        io.nuniq[r] = n;
+    }
+    io.nbatch[r] = 1;
+}
+
+/* Heavy (read, strand): a device work queue drained by persistent
+ * single-wave workgroups.  alignment_rmdup's batch composition -- which
+ * 32-position chunks, in what heap order, where each batch ends -- depends
+ * only on the selected seeds' counts and on the raw last position of every
+ * fetched chunk (the heap key, Q6), never on the filter.  So per batch one
+ * lane replays the seed heap (:1919-1962, 2261-2356) from keys prefetched
+ * into LDS, and then all 64 lanes fetch, filter and compact the batch's
+ * positions together. */
+#define HK 16          /* buffered chunk keys per selected 12-mer */
+
+__global__ void __launch_bounds__(64) k_plan_heavy(const uint2 *__restrict__ pl, PlanIO io)
+{
+    __shared__ swm_sel_t ssel[64];
+    __shared__ uint32_t spc[64];                 /* positions fetched per 12-mer */
+    __shared__ uint32_t skey[64 * HK];           /* keys of chunks skbase[i] .. skbase[i]+HK-1 */
+    __shared__ uint32_t skbase[64];
+    __shared__ uint32_t sstage[64];
+    __shared__ uint32_t hloc[66], hind[66];
+    __shared__ uint32_t cseed[SWM_MAX_BLOCKS], cg[SWM_MAX_BLOCKS], cpre[SWM_MAX_BLOCKS + 1];
+    __shared__ uint32_t sv[8];                   /* broadcast slots */
+    const uint32_t lane = threadIdx.x;
+    const uint32_t nq = *io.heavy_count;
+
+    /* heap key after fetching chunk c of 12-mer i: the raw last position of
+     * that chunk (seeds_heap_pop_and_insert's pos argument, :2350) */
+    auto key_addr = [&](uint32_t i, uint32_t c) -> uint32_t {
+        const uint32_t end = min(SWM_CHUNK * (c + 1), ssel[i].cnt);
+        return ssel[i].off + end - 1;
     };
-    auto heap_pop_insert = [&](uint32_t loc, bool del) {      /* :1936-1962 (Q6) */
-        if (del) { hloc[1] = hloc[hsize]; hind[1] = hind[hsize]; hsize--; }
-        else hloc[1] = loc;
-        uint32_t i = 1;
-        while (i * 2 < hsize) {
-            uint32_t ti = i * 2;
-            if (i * 2 + 1 <= hsize && hloc[i * 2] > hloc[i * 2 + 1]) ti = i * 2 + 1;
-            if (hloc[ti] < hloc[i]) {
-                uint32_t t = hloc[ti]; hloc[ti] = hloc[i]; hloc[i] = t;
-                t = hind[ti]; hind[ti] = hind[i]; hind[i] = t;
-            } else break;
-            i = ti;
+    auto refill = [&](uint32_t i) {              /* keys of the next HK chunks */
+        const uint32_t kb = spc[i] / SWM_CHUNK;
+        skbase[i] = kb;
+#pragma unroll
+        for (uint32_t k = 0; k < HK; k++) {
+            const uint32_t c = kb + k;
+            skey[i * HK + k] = (SWM_CHUNK * c < ssel[i].cnt) ? pl[key_addr(i, c)].x : 0u;
         }
     };
 
-    /* phase 1 (:2194-2236); the next selection entry is in flight while
-     * the current chunk is filtered */
-    bool heavy = false;
-    swm_sel_t s_next = nsel ? sel[0] : swm_sel_t{0, 0, 0};
-    for (uint32_t i = 0; i < nsel; i++) {
-        const swm_sel_t s = s_next;
-        if (i + 1 < nsel) s_next = sel[i + 1];
-        uint32_t get = s.cnt > SWM_CHUNK ? SWM_CHUNK : s.cnt;
-        if (get > 0) { take(s.off, get, s.meta); blocks++; }
-        pos_size += get;
-        pcount[i] = get;
-        heavy |= get != s.cnt;
-    }
-    if (heavy) {
-        for (uint32_t i = 0; i < nsel; i++) {                 /* :2244-2252 */
-            swm_sel_t s = sel[i];
-            if (pcount[i] != s.cnt) heap_insert(pl[s.off + pcount[i] - 1].x, i);
+    for (;;) {
+        if (lane == 0) sv[0] = atomicAdd(io.heavy_head, 1u);
+        __syncthreads();
+        const uint32_t qi = sv[0];
+        __syncthreads();
+        if (qi >= nq) break;
+        const uint32_t r = io.heavy[qi];
+        const uint32_t nsel = io.nsel[r] & ~NSEL_HEAVY;
+        if (lane < nsel) ssel[lane] = io.sel[(uint64_t)r * io.seln + lane];
+        __syncthreads();
+        uint32_t *surv = io.surv + io.surv_off[r];
+        uint2 *bd = io.bdesc + io.bd_off[r];
+
+        /* phase 1 (:2194-2236): the first chunk of every 12-mer; the heap
+         * key of those with more hits (:2244-2252) */
+        if (lane < nsel) {
+            spc[lane] = min(ssel[lane].cnt, (uint32_t)SWM_CHUNK);
+            sstage[lane] = ssel[lane].cnt > SWM_CHUNK ? pl[ssel[lane].off + SWM_CHUNK - 1].x : 0u;
+            refill(lane);
         }
-    }
-    for (;;) {                                                /* :2261-2419 */
-        if (hsize > 0 && pos_size + SWM_CHUNK <= SWM_BATCH_POS && blocks < SWM_MAX_BLOCKS) {
-            uint32_t i = hind[1];
-            swm_sel_t s = sel[i];
-            uint32_t rem = s.cnt - pcount[i];
-            uint32_t get = rem > SWM_CHUNK ? SWM_CHUNK : rem;
-            take(s.off + pcount[i], get, s.meta);
-            pos_size += get;
-            pcount[i] += get;
-            blocks++;
-            heap_pop_insert(pl[s.off + pcount[i] - 1].x, pcount[i] == s.cnt);
-        }
-        if (pos_size + SWM_CHUNK > SWM_BATCH_POS || hsize == 0 || blocks >= SWM_MAX_BLOCKS) {
-            /* end of batch: the sorted unique list (block_heap_sort's result) */
-            uint32_t n = nsurv - bstart;
-            if (n <= SMALL_BATCH) {
-                uint32_t *a = surv + bstart;
-                for (uint32_t i = 1; i < n; i++) {
-                    uint32_t v = a[i], j = i;
-                    while (j > 0 && a[j - 1] > v) { a[j] = a[j - 1]; j--; }
-                    a[j] = v;
+        __syncthreads();
+        uint32_t nch = 0, pos_size = 0, blocks = 0;
+        if (lane == 0) {
+            for (uint32_t i = 0; i < nsel; i++) {
+                const uint32_t get = spc[i];
+                if (get > 0) { cseed[nch] = i; cg[nch] = ssel[i].off; cpre[nch] = pos_size; nch++; blocks++; }
+                pos_size += get;
+            }
+            uint32_t hsize = 0;
+            for (uint32_t i = 0; i < nsel; i++) {
+                if (spc[i] != ssel[i].cnt) {                    /* seeds_heap_insert (:1919-1935) */
+                    hsize++;
+                    hloc[hsize] = sstage[i]; hind[hsize] = i;
+                    uint32_t x = hsize;
+                    while (x > 1) {
+                        uint32_t t = x / 2;
+                        if (hloc[t] > hloc[x]) {
+                            uint32_t a0 = hloc[t]; hloc[t] = hloc[x]; hloc[x] = a0;
+                            a0 = hind[t]; hind[t] = hind[x]; hind[x] = a0;
+                        } else break;
+                        x = t;
+                    }
                 }
-                uint32_t u = n ? 1u : 0u;
-                for (uint32_t i = 1; i < n; i++)
-                    if (a[i] != a[u - 1]) a[u++] = a[i];
-                bd[batch] = make_uint2(bstart, u);
+            }
+            sv[1] = hsize;
+        }
+        __syncthreads();
+        uint32_t nsurv = 0, batch = 0;
+        for (;;) {
+            /* ---- phase A (lane 0): the rest of this batch's chunk list ---- */
+            if (lane == 0) {
+                uint32_t hsize = sv[1];
+                for (;;) {
+                    if (hsize > 0 && pos_size + SWM_CHUNK <= SWM_BATCH_POS && blocks < SWM_MAX_BLOCKS) {
+                        const uint32_t i = hind[1];
+                        const uint32_t cnt = ssel[i].cnt, pc = spc[i];
+                        const uint32_t c = pc / SWM_CHUNK;           /* chunk fetched now */
+                        const uint32_t get = min((uint32_t)SWM_CHUNK, cnt - pc);
+                        cseed[nch] = i; cg[nch] = ssel[i].off + pc; cpre[nch] = pos_size; nch++;
+                        pos_size += get;
+                        blocks++;
+                        spc[i] = pc + get;
+                        if (spc[i] == cnt) {                          /* seeds_heap_pop_and_insert (:1936-1962, Q6) */
+                            hloc[1] = hloc[hsize]; hind[1] = hind[hsize]; hsize--;
+                        } else {
+                            const uint32_t kb = skbase[i];
+                            hloc[1] = (c >= kb && c < kb + HK) ? skey[i * HK + (c - kb)] : pl[key_addr(i, c)].x;
+                        }
+                        uint32_t x = 1;
+                        while (x * 2 < hsize) {
+                            uint32_t t = x * 2;
+                            if (x * 2 + 1 <= hsize && hloc[x * 2] > hloc[x * 2 + 1]) t = x * 2 + 1;
+                            if (hloc[t] < hloc[x]) {
+                                uint32_t a0 = hloc[t]; hloc[t] = hloc[x]; hloc[x] = a0;
+                                a0 = hind[t]; hind[t] = hind[x]; hind[x] = a0;
+                            } else break;
+                            x = t;
+                        }
+                    }
+                    if (pos_size + SWM_CHUNK > SWM_BATCH_POS || hsize == 0 || blocks >= SWM_MAX_BLOCKS) break;
+                }
+                cpre[nch] = pos_size;
+                sv[1] = hsize;
+                sv[2] = nch;
+                sv[3] = pos_size;
+            }
+            __syncthreads();
+            const uint32_t bn = sv[2], bpos = sv[3], hsize = sv[1];
+            /* ---- phase B (all lanes): fetch, filter, compact ---- */
+            const uint32_t bstart = nsurv;
+            for (uint32_t e0 = 0; e0 < bpos; e0 += 64) {
+                const uint32_t ent = e0 + lane;
+                bool keep = false;
+                uint32_t val = 0;
+                if (ent < bpos && ent > 0) {                     /* entry 0: the Q7 drop */
+                    uint32_t lo = 0, hi = bn;                    /* chunk with cpre[lo] <= ent < cpre[lo+1] */
+                    while (hi - lo > 1) {
+                        const uint32_t mid = (lo + hi) >> 1;
+                        if (cpre[mid] <= ent) lo = mid; else hi = mid;
+                    }
+                    const uint2 v = pl[cg[lo] + (ent - cpre[lo])];
+                    const uint32_t meta = ssel[cseed[lo]].meta;
+                    keep = filter_pass(v.y, meta);
+                    val = adjust(v.x, meta);
+                }
+                const uint64_t m = __ballot(keep);
+                if (keep) surv[nsurv + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = val;
+                nsurv += (uint32_t)__popcll(m);
+            }
+            /* ---- end of batch: sorted unique list (block_heap_sort's result) ---- */
+            const uint32_t n = nsurv - bstart;
+            if (n <= 64) {
+                uint32_t v = lane < n ? surv[bstart + lane] : 0xffffffffu;
+                for (uint32_t kk = 2; kk <= 64; kk <<= 1) {
+                    for (uint32_t j = kk >> 1; j > 0; j >>= 1) {
+                        const uint32_t o = __shfl_xor(v, (int)j);
+                        const bool up = (lane & kk) == 0, lower = (lane & j) == 0;
+                        v = (lower == up) ? min(v, o) : max(v, o);
+                    }
+                }
+                const uint32_t prev = __shfl_up(v, 1);
+                const bool head = lane < n && (lane == 0 || v != prev);
+                const uint64_t m = __ballot(head);
+                if (head) surv[bstart + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = v;
+                const uint32_t u = (uint32_t)__popcll(m);
+                if (lane == 0) bd[batch] = make_uint2(bstart, u);
                 nsurv = bstart + u;
-            } else {
+            } else if (lane == 0) {
                 bd[batch] = make_uint2(bstart, n);
-                uint32_t slot = atomicAdd(io.big_count, 1u);
+                const uint32_t slot = atomicAdd(io.big_count, 1u);
                 io.big[slot] = make_uint2(r, batch);
             }
-            bstart = nsurv;
             batch++;
-            pos_size = 0;
-            blocks = 0;
-            first_pending = true;
             if (hsize == 0) break;
+            /* next batch: refill the key buffers, start an empty chunk list */
+            if (lane < nsel) refill(lane);
+            nch = 0; pos_size = 0; blocks = 0;
+            __syncthreads();
         }
+        if (lane == 0) { io.nbatch[r] = batch; io.nuniq[r] = nsurv; }
+        __syncthreads();
     }
-    io.nbatch[r] = batch;
-    io.nuniq[r] = nsurv;
 }
 
 /* One workgroup per large batch (<= 2048 values): bitonic sort in LDS, then
@@ -512,13 +650,21 @@ struct CandSet {
 #define INFO_TRACE (1u << 16)
 #define INFO_NEG (1u << 17)
 
+#define HEAVY_CANDS 256   /* items with more candidates go to the cooperative kernels */
+
 __global__ void __launch_bounds__(128) k_compact(uint32_t nrs, const uint64_t *__restrict__ surv_off,
                                                  const uint32_t *__restrict__ surv, const uint32_t *__restrict__ bd_off,
                                                  const uint2 *__restrict__ bdesc, const uint32_t *__restrict__ nbatch,
-                                                 const uint32_t *__restrict__ cand_off, CandSet c)
+                                                 const uint32_t *__restrict__ cand_off, CandSet c,
+                                                 uint32_t *__restrict__ heavy, uint32_t *__restrict__ heavy_count)
 {
+    __shared__ uint32_t s_tmp[2];
     uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= nrs) return;
+    const bool live = r < nrs;
+    if (!live) r = 0;
+    const bool big = live && cand_off[r + 1] - cand_off[r] > HEAVY_CANDS;
+    block_append(big, r, heavy, heavy_count, s_tmp);
+    if (!live || big) return;
     const uint32_t *s = surv + surv_off[r];
     const uint2 *bd = bdesc + bd_off[r];
     uint32_t o = cand_off[r];
@@ -530,6 +676,35 @@ __global__ void __launch_bounds__(128) k_compact(uint32_t nrs, const uint64_t *_
             c.rs[o] = r;
             c.bi[o] = (b << 12) | j;
             o++;
+        }
+    }
+}
+
+/* One workgroup per (read, strand) with many candidates. */
+__global__ void __launch_bounds__(256) k_compact_heavy(const uint32_t *__restrict__ heavy,
+                                                       const uint32_t *__restrict__ heavy_count,
+                                                       const uint64_t *__restrict__ surv_off,
+                                                       const uint32_t *__restrict__ surv,
+                                                       const uint32_t *__restrict__ bd_off,
+                                                       const uint2 *__restrict__ bdesc,
+                                                       const uint32_t *__restrict__ nbatch,
+                                                       const uint32_t *__restrict__ cand_off, CandSet c)
+{
+    const uint32_t nh = *heavy_count;
+    for (uint32_t h = blockIdx.x; h < nh; h += gridDim.x) {
+        const uint32_t r = heavy[h];
+        const uint32_t *s = surv + surv_off[r];
+        const uint2 *bd = bdesc + bd_off[r];
+        uint32_t o = cand_off[r];
+        const uint32_t nb = nbatch[r];
+        for (uint32_t b = 0; b < nb; b++) {
+            const uint2 d = bd[b];
+            for (uint32_t j = threadIdx.x; j < d.y; j += blockDim.x) {
+                c.val[o + j] = s[d.x + j];
+                c.rs[o + j] = r;
+                c.bi[o + j] = (b << 12) | j;
+            }
+            o += d.y;
         }
     }
 }
@@ -667,16 +842,22 @@ __global__ void __launch_bounds__(256) k_hamming(uint32_t ncand, const uint32_t 
                                                  const uint32_t *__restrict__ words, const uint32_t *__restrict__ b2,
                                                  uint64_t ref_length, uint32_t e, CandSet c,
                                                  uint32_t *__restrict__ trace, uint32_t *__restrict__ trace_count,
-                                                 uint32_t *__restrict__ n_rejected)
+                                                 uint32_t *__restrict__ n_rejected, unsigned long long *n_steps)
 {
     __shared__ uint32_t s_tmp[2];
+    __shared__ uint32_t s_steps;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     bool to_trace = false;
+    uint32_t steps = 0;
+    if (threadIdx.x == 0) s_steps = 0;
     if (i < ncand) {
         const int32_t er = c.err[i];
         if (er == -2) {
             atomicAdd(n_rejected, 1u);         /* pos_start rejected the window: not a lane */
-        } else if (er <= (int32_t)e) {
+        } else {
+            steps = lens[c.rs[i] >> 1];
+        }
+        if (er != -2 && er <= (int32_t)e) {
             const int32_t lc = c.loc[i];
             const uint32_t rs = c.rs[i];
             const uint32_t L = lens[rs >> 1];
@@ -708,6 +889,9 @@ __global__ void __launch_bounds__(256) k_hamming(uint32_t ncand, const uint32_t 
         }
     }
     block_append(to_trace, i, trace, trace_count, s_tmp);
+    if (steps) atomicAdd(&s_steps, steps);
+    __syncthreads();
+    if (threadIdx.x == 0 && s_steps) atomicAdd(n_steps, (unsigned long long)s_steps);
 }
 
 /* Stage entry: explicit candidates. */
@@ -875,65 +1059,147 @@ __global__ void __launch_bounds__(64) k_trace(const uint32_t *__restrict__ trace
 
 /* ------------------------------------------------------------------ k_emit */
 
-/* One read: its candidates in emission order (forward strand, then reverse;
- * batch by batch; groups of 8 valid lanes).  A Hamming record is always
- * written; a traceback record is skipped when it repeats the previous record
- * of the current block (:475-477).  After every group the block is flushed
- * when 8 more records or 256 more CIGAR bytes might not fit (:1790, :1858),
- * which resets the "previous record". */
+/* Emission of one read's records: its candidates in emission order (forward
+ * strand, then reverse; batch by batch; groups of 8 valid lanes).  A Hamming
+ * record is always written; a traceback record is skipped when it repeats
+ * the previous record of the current block (:475-477).  After every group
+ * the block is flushed when 8 more records or 256 more CIGAR bytes might not
+ * fit (:1790, :1858), which resets the "previous record". */
+struct Emit {
+    uint32_t count, best_pos, best_dist, best_strand;
+    uint32_t cigar_num, cigar_len, last_loc, cur_batch, vcount;
+    bool open;
+    __device__ void init()
+    {
+        count = 0; best_pos = 0xffffffffu; best_dist = 0xffu; best_strand = 0xffu;
+    }
+    __device__ void start_strand()
+    {
+        cigar_num = 0; cigar_len = 0; last_loc = 0; cur_batch = 0xffffffffu; vcount = 0; open = false;
+    }
+    __device__ void flush_check()
+    {
+        if (cigar_num + 8 > 256 || cigar_len + 16 * 16 > 256 * 16) { cigar_num = 0; cigar_len = 0; }
+    }
+    __device__ void step(uint32_t bi, int32_t er, uint32_t rec, uint32_t info, uint32_t e, uint32_t strand)
+    {
+        const uint32_t b = bi >> 12;
+        if (b != cur_batch) {
+            if (open) { flush_check(); open = false; }
+            cur_batch = b;
+            vcount = 0;
+        }
+        if (er == -2) return;
+        if (open && (vcount & 7u) == 0) { flush_check(); open = false; }
+        open = true;
+        vcount++;
+        if (er > (int32_t)e) return;
+        if ((info & INFO_TRACE) && cigar_num > 0 && !(info & INFO_NEG) && last_loc == rec) return;
+        cigar_num++;
+        cigar_len += info & 0xffffu;
+        last_loc = rec;
+        if (count == 0 || (uint32_t)er < best_dist) { best_pos = rec; best_dist = (uint32_t)er; best_strand = strand; }
+        count++;
+    }
+    __device__ void end_strand()
+    {
+        if (open) flush_check();
+    }
+    __device__ swm_read_result_t result() const
+    {
+        swm_read_result_t res;
+        res.pos = count ? best_pos : 0xffffffffu;
+        res.strand = count ? (uint8_t)best_strand : (uint8_t)0xff;
+        res.dist = count ? (uint8_t)best_dist : (uint8_t)0xff;
+        res.pad = 0;
+        res.count = count;
+        return res;
+    }
+};
+
+__device__ __forceinline__ void add_records(uint32_t count, unsigned long long *n_records, uint32_t *s_sum)
+{
+    if (threadIdx.x == 0) *s_sum = 0;
+    __syncthreads();
+    if (count) atomicAdd(s_sum, count);
+    __syncthreads();
+    if (threadIdx.x == 0 && *s_sum) atomicAdd(n_records, (unsigned long long)*s_sum);
+}
+
 __global__ void __launch_bounds__(128) k_emit(uint32_t nreads, uint32_t e, const uint32_t *__restrict__ cand_off,
                                               CandSet c, swm_read_result_t *__restrict__ out,
-                                              unsigned long long *n_records)
+                                              unsigned long long *n_records, uint32_t *__restrict__ heavy,
+                                              uint32_t *__restrict__ heavy_count)
 {
+    __shared__ uint32_t s_tmp[2], s_sum;
     uint32_t rd = blockIdx.x * blockDim.x + threadIdx.x;
     const bool live = rd < nreads;
     if (!live) rd = 0;
-    uint32_t count = 0, best_pos = 0xffffffffu, best_dist = 0xffu, best_strand = 0xffu;
-    for (uint32_t strand = 0; strand < 2 && live; strand++) {
-        const uint32_t rs = 2 * rd + strand;
-        uint32_t cigar_num = 0, cigar_len = 0, last_loc = 0;
-        uint32_t cur_batch = 0xffffffffu, vcount = 0;
-        bool open = false;
-        auto flush_check = [&]() {
-            if (cigar_num + 8 > 256 || cigar_len + 16 * 16 > 256 * 16) { cigar_num = 0; cigar_len = 0; }
-        };
-        const uint32_t cend = cand_off[rs + 1];
-        for (uint32_t ci = cand_off[rs]; ci < cend; ci++) {
-            const uint32_t b = c.bi[ci] >> 12;
-            if (b != cur_batch) {
-                if (open) { flush_check(); open = false; }
-                cur_batch = b;
-                vcount = 0;
+    const bool big = live && cand_off[2 * rd + 2] - cand_off[2 * rd] > HEAVY_CANDS;
+    block_append(big, rd, heavy, heavy_count, s_tmp);
+    Emit em;
+    em.init();
+    if (live && !big) {
+        for (uint32_t strand = 0; strand < 2; strand++) {
+            const uint32_t rs = 2 * rd + strand;
+            em.start_strand();
+            const uint32_t cend = cand_off[rs + 1];
+            for (uint32_t ci = cand_off[rs]; ci < cend; ci++) {
+                const int32_t er = c.err[ci];
+                const bool pass = er >= 0 && er <= (int32_t)e;
+                em.step(c.bi[ci], er, pass ? c.rec[ci] : 0u, pass ? c.info[ci] : 0u, e, strand);
             }
-            const int32_t er = c.err[ci];
-            if (er == -2) continue;
-            if (open && (vcount & 7u) == 0) { flush_check(); open = false; }
-            open = true;
-            vcount++;
-            if (er > (int32_t)e) continue;
-            const uint32_t info = c.info[ci], loc = c.rec[ci];
-            if ((info & INFO_TRACE) && cigar_num > 0 && !(info & INFO_NEG) && last_loc == loc) continue;
-            cigar_num++;
-            cigar_len += info & 0xffffu;
-            last_loc = loc;
-            if (count == 0 || (uint32_t)er < best_dist) { best_pos = loc; best_dist = (uint32_t)er; best_strand = strand; }
-            count++;
+            em.end_strand();
         }
-        if (open) flush_check();
+        out[rd] = em.result();
     }
-    swm_read_result_t res;
-    res.pos = count ? best_pos : 0xffffffffu;
-    res.strand = count ? (uint8_t)best_strand : (uint8_t)0xff;
-    res.dist = count ? (uint8_t)best_dist : (uint8_t)0xff;
-    res.pad = 0;
-    res.count = count;
-    if (live) out[rd] = res;
-    __shared__ uint32_t s_sum;
-    if (threadIdx.x == 0) s_sum = 0;
-    __syncthreads();
-    if (count) atomicAdd(&s_sum, count);
-    __syncthreads();
-    if (threadIdx.x == 0 && s_sum) atomicAdd(n_records, (unsigned long long)s_sum);
+    add_records(live && !big ? em.count : 0u, n_records, &s_sum);
+}
+
+/* One wave per read with many candidates: each lane holds one of 64
+ * candidates, and the whole wave steps the emission over them in order on
+ * wave-uniform values (v_readlane), so there is no LDS traffic or barrier in
+ * the sequential part. */
+__global__ void __launch_bounds__(64) k_emit_heavy(const uint32_t *__restrict__ heavy,
+                                                   const uint32_t *__restrict__ heavy_count, uint32_t e,
+                                                   const uint32_t *__restrict__ cand_off, CandSet c,
+                                                   swm_read_result_t *__restrict__ out,
+                                                   unsigned long long *n_records)
+{
+    const uint32_t lane = threadIdx.x;
+    const uint32_t nh = *heavy_count;
+    for (uint32_t h = blockIdx.x; h < nh; h += gridDim.x) {
+        const uint32_t rd = heavy[h];
+        Emit em;
+        em.init();
+        for (uint32_t strand = 0; strand < 2; strand++) {
+            const uint32_t rs = 2 * rd + strand;
+            em.start_strand();
+            const uint32_t c0 = cand_off[rs], cend = cand_off[rs + 1];
+            for (uint32_t base = c0; base < cend; base += 64) {
+                const uint32_t ci = base + lane;
+                uint32_t vbi = 0, vrec = 0, vinfo = 0;
+                int32_t verr = -2;
+                if (ci < cend) {
+                    verr = c.err[ci];
+                    vbi = c.bi[ci];
+                    if (verr >= 0 && verr <= (int32_t)e) { vrec = c.rec[ci]; vinfo = c.info[ci]; }
+                }
+                const uint32_t m = min(64u, cend - base);
+                for (uint32_t q = 0; q < m; q++) {
+                    em.step((uint32_t)__builtin_amdgcn_readlane((int)vbi, (int)q),
+                            __builtin_amdgcn_readlane(verr, (int)q),
+                            (uint32_t)__builtin_amdgcn_readlane((int)vrec, (int)q),
+                            (uint32_t)__builtin_amdgcn_readlane((int)vinfo, (int)q), e, strand);
+                }
+            }
+            em.end_strand();
+        }
+        if (lane == 0) {
+            out[rd] = em.result();
+            if (em.count) atomicAdd(n_records, (unsigned long long)em.count);
+        }
+    }
 }
 
 /* --------------------------------------------------------------- host side */
@@ -947,7 +1213,7 @@ inline uint32_t blocks_for(uint64_t n, uint32_t bs) { return (uint32_t)((n + bs 
 struct swm_ws {
     struct Buf { void *p = nullptr; size_t cap = 0; };
     enum { WORDS, SEL, NSEL, NPOS, NCHUNK, SURV_OFF, BD_OFF, SURV, BDESC, NBATCH, NUNIQ, CAND_OFF, C_VAL, C_RS, C_BI,
-           C_ERR, C_LOC, C_REC, C_INFO, COUNTERS, BIG, TRACE, PASS, SCAN_TMP, NBUF };
+           C_ERR, C_LOC, C_REC, C_INFO, COUNTERS, BIG, TRACE, HEAVY, HEAVY_C, HEAVY_E, SCAN_TMP, NBUF };
     Buf b[NBUF];
     ~swm_ws() { for (auto &x : b) if (x.p) (void)hipFree(x.p); }
     template <typename T>
@@ -1024,6 +1290,8 @@ extern "C" int swm_ctx_create(uint32_t k, const uint32_t *hash_count, const uint
     c->ws = new (std::nothrow) swm_ws();
     if (!c->ws) { swm_ctx_destroy(c); return SWM_ENOMEM; }
     c->k = k;
+    c->n_cu = 256;
+    (void)hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, c->device);
     c->n_buckets = 1ull << (2 * k);
     c->n_pos = n_pos;
     c->n_2bits_words = n_2bits_words;
@@ -1105,8 +1373,10 @@ static int map_device_impl(swm_ctx_t *ctx, const char *d_reads, uint64_t stride,
         (rc = ws->get(swm_ws::NSEL, &so.nsel, nrs)) || (rc = ws->get(swm_ws::NPOS, &so.npos, nrs + 1)) ||
         (rc = ws->get(swm_ws::NCHUNK, &so.nchunk, nrs + 1)) || (rc = ws->get(swm_ws::COUNTERS, &counters, 8)))
         return rc;
-    uint32_t *u32c = reinterpret_cast<uint32_t *>(counters + 4);   /* [0] big batches, [1] tracebacks, [2] rejected windows */
-    SWM_HIP_CHECK(hipMemsetAsync(counters, 0, 8 * sizeof(unsigned long long), st));
+    /* [0] big batches, [1] tracebacks, [2] rejected windows, [3] heavy plan items, [4] their queue head,
+     * [5] heavy compaction items, [6] heavy emission items */
+    uint32_t *u32c = reinterpret_cast<uint32_t *>(counters + 4);
+    SWM_HIP_CHECK(hipMemsetAsync(counters, 0, 8 * sizeof(unsigned long long), st));   /* 32 B of u64, 16 u32 */
     SWM_HIP_CHECK(hipMemsetAsync(so.npos + nrs, 0, sizeof(uint64_t), st));
     SWM_HIP_CHECK(hipMemsetAsync(so.nchunk + nrs, 0, sizeof(uint32_t), st));
     SWM_HIP_CHECK(hipEventRecord(ev[0], st));
@@ -1137,8 +1407,13 @@ static int map_device_impl(swm_ctx_t *ctx, const char *d_reads, uint64_t stride,
         (rc = ws->get(swm_ws::BIG, &pio.big, (uint64_t)tot_chunk + 1)))
         return rc;
     pio.big_count = u32c;
+    if ((rc = ws->get(swm_ws::HEAVY, &pio.heavy, nrs))) return rc;
+    pio.heavy_count = u32c + 3;
+    pio.heavy_head = u32c + 4;
     SWM_HIP_CHECK(hipMemsetAsync(pio.nuniq + nrs, 0, sizeof(uint32_t), st));
     k_plan<<<blocks_for(nrs, 64), 64, 0, st>>>((uint32_t)nrs, ctx->d_pl, pio);
+    SWM_HIP_CHECK(hipGetLastError());
+    k_plan_heavy<<<ctx->n_cu * 8, 64, 0, st>>>(ctx->d_pl, pio);
     SWM_HIP_CHECK(hipGetLastError());
     k_bigsort<<<1024, 256, 0, st>>>(pio.big, pio.big_count, surv_off, bd_off, pio.surv, pio.bdesc, pio.nuniq);
     SWM_HIP_CHECK(hipGetLastError());
@@ -1157,8 +1432,12 @@ static int map_device_impl(swm_ctx_t *ctx, const char *d_reads, uint64_t stride,
         (rc = ws->get(swm_ws::C_LOC, &cs.loc, (uint64_t)ncand + 1)) || (rc = ws->get(swm_ws::C_REC, &cs.rec, (uint64_t)ncand + 1)) ||
         (rc = ws->get(swm_ws::C_INFO, &cs.info, (uint64_t)ncand + 1)) || (rc = ws->get(swm_ws::TRACE, &trace, (uint64_t)ncand + 1)))
         return rc;
+    uint32_t *heavy_c, *heavy_e;
+    if ((rc = ws->get(swm_ws::HEAVY_C, &heavy_c, nrs)) || (rc = ws->get(swm_ws::HEAVY_E, &heavy_e, n))) return rc;
     k_compact<<<blocks_for(nrs, 128), 128, 0, st>>>((uint32_t)nrs, surv_off, pio.surv, bd_off, pio.bdesc, pio.nbatch,
-                                                    cand_off, cs);
+                                                    cand_off, cs, heavy_c, u32c + 5);
+    k_compact_heavy<<<ctx->n_cu * 4, 256, 0, st>>>(heavy_c, u32c + 5, surv_off, pio.surv, bd_off, pio.bdesc, pio.nbatch,
+                                                   cand_off, cs);
     SWM_HIP_CHECK(hipGetLastError());
     SWM_HIP_CHECK(hipEventRecord(ev[3], st));
     if (ncand)
@@ -1168,10 +1447,11 @@ static int map_device_impl(swm_ctx_t *ctx, const char *d_reads, uint64_t stride,
     SWM_HIP_CHECK(hipEventRecord(ev[4], st));
     if (ncand) {
         k_hamming<<<blocks_for(ncand, 256), 256, 0, st>>>(ncand, d_lens, so.words, ctx->d_base_2bits, ctx->ref_length,
-                                                           e, cs, trace, u32c + 1, u32c + 2);
+                                                           e, cs, trace, u32c + 1, u32c + 2, counters + 2);
         k_trace<<<1024, 64, 0, st>>>(trace, u32c + 1, d_lens, so.words, ctx->d_base_2bits, ctx->ref_length, e, cs);
     }
-    k_emit<<<blocks_for(n, 128), 128, 0, st>>>((uint32_t)n, e, cand_off, cs, d_out, counters + 1);
+    k_emit<<<blocks_for(n, 128), 128, 0, st>>>((uint32_t)n, e, cand_off, cs, d_out, counters + 1, heavy_e, u32c + 6);
+    k_emit_heavy<<<ctx->n_cu * 32, 64, 0, st>>>(heavy_e, u32c + 6, e, cand_off, cs, d_out, counters + 1);
     SWM_HIP_CHECK(hipGetLastError());
     SWM_HIP_CHECK(hipEventRecord(ev[5], st));
     unsigned long long hc[8];
@@ -1179,6 +1459,7 @@ static int map_device_impl(swm_ctx_t *ctx, const char *d_reads, uint64_t stride,
     SWM_HIP_CHECK(hipStreamSynchronize(st));
     float ms;
     ctx->stats.n_verifications = ncand - reinterpret_cast<const uint32_t *>(hc + 4)[2];
+    ctx->stats.n_band_steps = hc[2];
     ctx->stats.n_records = hc[1];
     (void)hipEventElapsedTime(&ms, ev[0], ev[1]); ctx->stats.ms_seed = ms;
     (void)hipEventElapsedTime(&ms, ev[1], ev[2]); ctx->stats.ms_plan = ms;
@@ -1305,4 +1586,59 @@ extern "C" int swm_dev_memset(void *dst, int value, uint64_t bytes)
 {
     if (bytes && !dst) return SWM_EINVAL;
     return (!bytes || hipMemset(dst, value, bytes) == hipSuccess) ? SWM_OK : SWM_EDEVICE;
+}
+
+/* ------------------------------------------------------ VALU peak probe */
+
+/* Integer VALU throughput: 8 independent chains per lane with per-lane
+ * (VGPR) operands, each step an add followed by a 3-input boolean, so every
+ * SIMD always has independent work.  Counts 3 VALU instructions per chain
+ * step (add, xor, and-or). */
+__global__ void __launch_bounds__(256) k_valu_probe(uint32_t iters, uint32_t seed, uint32_t *__restrict__ out)
+{
+    uint32_t x[8], y[8], z[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+        x[j] = threadIdx.x * (j + 3) + seed;
+        y[j] = (threadIdx.x ^ blockIdx.x) * 0x9e3779b9u + (uint32_t)j;
+        z[j] = threadIdx.x * 0x85ebca6bu + (uint32_t)j * 7u;
+    }
+    for (uint32_t it = 0; it < iters; it++) {
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+#pragma unroll
+            for (int j = 0; j < 8; j++) x[j] = ((x[j] + y[j]) ^ z[j]) | (x[j] & y[(j + u) & 7]);
+        }
+    }
+    uint32_t acc = 0;
+#pragma unroll
+    for (int j = 0; j < 8; j++) acc ^= x[j];
+    if (acc == 0x12345678u) out[0] = acc;   /* keeps the chains live */
+}
+
+/* No counterpart in the reference: measures the chip's int32 VALU rate for
+ * the Myers roofline (ops counted as lane operations per second). */
+extern "C" int swm_valu_probe(double *ops_per_s)
+{
+    if (!ops_per_s) return SWM_EINVAL;
+    uint32_t *d = nullptr;
+    if (hipMalloc((void **)&d, 4) != hipSuccess) return SWM_ENOMEM;
+    hipEvent_t a, b;
+    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) { (void)hipFree(d); return SWM_EDEVICE; }
+    int dev = 0, cus = 256;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    const uint32_t blocks = (uint32_t)cus * 16, iters = 4096;
+    k_valu_probe<<<blocks, 256>>>(16, 1u, d);                 /* warm-up */
+    (void)hipEventRecord(a, 0);
+    k_valu_probe<<<blocks, 256>>>(iters, 1u, d);
+    (void)hipEventRecord(b, 0);
+    int rc = hipEventSynchronize(b) == hipSuccess ? SWM_OK : SWM_EDEVICE;
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, a, b);
+    *ops_per_s = (double)blocks * 256.0 * iters * 8.0 * 8.0 * 3.0 / (ms * 1e-3);   /* add, xor, and_or */
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    (void)hipFree(d);
+    return rc;
 }
