@@ -125,6 +125,71 @@ int swm_map_batch_device(swm_ctx_t *ctx, const char *d_reads, uint64_t stride,
 /* Statistics of the last batch run on ctx. */
 int swm_last_stats(const swm_ctx_t *ctx, swm_stats_t *st);
 
+/* ---- forms and tunables (no counterpart in the reference: its one form is
+ *      the CPE kernel with compile-time sizes, mapper_cpe.c:15-25) ---- */
+
+#define SWM_OPT_WORK_QUEUE 1        /* 1 (default): heavy items go to device work queues; 0: plain form */
+#define SWM_OPT_GRAPH 2             /* 1: fixed-capacity pipeline replayed as a hipGraph; 0 (default): synced */
+#define SWM_OPT_HEAVY_CANDS 3       /* candidates above which an item is heavy (1..65535, default 256) */
+#define SWM_OPT_MYERS_BLOCK 4       /* verification workgroup size: 64, 128 or 256 (default) */
+#define SWM_OPT_PLAN_HEAVY_WAVES 5  /* persistent plan waves per CU (1..32, default 8) */
+#define SWM_OPT_EMIT_HEAVY_WAVES 6  /* cooperative emission waves per CU (1..64, default 32) */
+
+/* Every form and tunable gives the same results; they change only speed. */
+int swm_ctx_set_option(swm_ctx_t *ctx, int opt, int64_t value);
+int swm_ctx_get_option(const swm_ctx_t *ctx, int opt, int64_t *value);
+
+/* Autotuner: times the pipeline on the given sample (host buffers, as
+ * swm_map_batch) for each candidate setting of the tunables, one at a time,
+ * keeps the fastest on ctx and writes the chosen values to chosen[6]
+ * (indexed by SWM_OPT_* - 1) when chosen != NULL.  reps >= 1 timed runs per
+ * setting (median). */
+int swm_autotune(swm_ctx_t *ctx, const char *reads, uint64_t stride, const uint32_t *lens, uint64_t n,
+                 uint32_t e, uint32_t reps, int64_t *chosen);
+
+/* ---- serving loop: continuous batching over a resident index (grows the
+ *      reference's double-buffered host pipeline, mapper_mpe.c:916-1106) ---- */
+
+typedef struct swm_server swm_server_t;
+
+typedef struct swm_server_config {
+    uint32_t n_slots;      /* batches on the device at once, each on its own stream (1..16; 0 = 4) */
+    uint32_t e;            /* -e */
+    uint64_t stride;       /* -l: read stride of every submitted batch */
+    uint64_t hbm_budget;   /* device bytes the admitted batches may hold (0 = 90% of free HBM at creation) */
+    uint64_t max_batch;    /* reads per batch accepted (0 = 65536) */
+    uint32_t graph;        /* 1: replay captured graphs, batch sizes padded to buckets of sqrt(2) steps */
+    uint32_t pad;
+} swm_server_config_t;
+
+typedef struct swm_batch_info {
+    uint64_t ticket;
+    uint64_t n_reads;
+    double t_submit, t_admit, t_done;  /* seconds on CLOCK_MONOTONIC */
+    int32_t status;                    /* SWM_OK or the mapping error */
+    uint32_t slot;
+    uint64_t est_bytes;                /* device bytes the batch was admitted against */
+    swm_stats_t stats;
+} swm_batch_info_t;
+
+/* Admission is first come first served: the oldest waiting batch starts when
+ * a slot is free and the budget holds its estimated device bytes.  Batches
+ * on different slots overlap on the device; each result is the same as the
+ * batch mapped alone (swm_map_batch). */
+int swm_server_create(swm_ctx_t *ctx, const swm_server_config_t *cfg, swm_server_t **out);
+/* Non-blocking.  reads/lens/out stay owned by the caller and must stay valid
+ * until swm_server_wait returns for the ticket.  SWM_EINVAL for a bad batch,
+ * SWM_ENOMEM if the batch alone exceeds the budget. */
+int swm_server_submit(swm_server_t *srv, const char *reads, const uint32_t *lens, uint64_t n,
+                      swm_read_result_t *out, uint64_t *ticket);
+/* Blocks until the batch is done; fills info (may be NULL) and forgets the ticket. */
+int swm_server_wait(swm_server_t *srv, uint64_t ticket, swm_batch_info_t *info);
+/* Totals since creation: batches, reads, peak admitted bytes, budget. */
+int swm_server_totals(const swm_server_t *srv, uint64_t *batches, uint64_t *reads, uint64_t *peak_bytes,
+                      uint64_t *budget);
+/* Waits for every submitted batch, then stops the workers. */
+void swm_server_destroy(swm_server_t *srv);
+
 /* ---- stage entry points (per-kernel parity) ---- */
 
 /* Banded Myers over explicit candidates, device pointers:
@@ -150,6 +215,9 @@ int swm_copy_h2d(void *dst, const void *src, uint64_t bytes);
 int swm_copy_d2h(void *dst, const void *src, uint64_t bytes);
 int swm_copy_d2d(void *dst, const void *src, uint64_t bytes);
 int swm_dev_memset(void *dst, int value, uint64_t bytes);
+/* page-locked host memory (DMA at full link speed; e.g. serving buffers) */
+int swm_host_alloc(void **p, uint64_t bytes);
+int swm_host_free(void *p);
 
 #ifdef __cplusplus
 }

@@ -49,6 +49,27 @@ class Stats(C.Structure):
                 ("ms_seed", C.c_double), ("ms_plan", C.c_double), ("ms_compact", C.c_double),
                 ("ms_myers", C.c_double), ("ms_emit", C.c_double), ("ms_total", C.c_double)]
 
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("n_slots", C.c_uint32), ("e", C.c_uint32), ("stride", C.c_uint64), ("hbm_budget", C.c_uint64),
+                ("max_batch", C.c_uint64), ("graph", C.c_uint32), ("pad", C.c_uint32)]
+
+
+class BatchInfo(C.Structure):
+    _fields_ = [("ticket", C.c_uint64), ("n_reads", C.c_uint64), ("t_submit", C.c_double),
+                ("t_admit", C.c_double), ("t_done", C.c_double), ("status", C.c_int32), ("slot", C.c_uint32),
+                ("est_bytes", C.c_uint64), ("stats", Stats)]
+
+
+# forms and tunables (swm_ctx_set_option)
+OPT_WORK_QUEUE, OPT_GRAPH, OPT_HEAVY_CANDS, OPT_MYERS_BLOCK, OPT_PLAN_HEAVY_WAVES, OPT_EMIT_HEAVY_WAVES = range(1, 7)
+OPTION_NAMES = {OPT_WORK_QUEUE: "work_queue", OPT_GRAPH: "graph", OPT_HEAVY_CANDS: "heavy_cands",
+                OPT_MYERS_BLOCK: "myers_block", OPT_PLAN_HEAVY_WAVES: "plan_heavy_waves",
+                OPT_EMIT_HEAVY_WAVES: "emit_heavy_waves"}
+
 
 _lib_handle = None
 
@@ -84,6 +105,16 @@ def lib():
             "swm_dev_memset": ([vp, C.c_int, u64], C.c_int),
             "swm_synth_reads": ([vp, u64, u64, u32, u32, C.c_double, C.c_double, C.c_double, u64,
                                  vp, u64, vp, vp, vp], C.c_int),
+            "swm_ctx_set_option": ([vp, C.c_int, C.c_int64], C.c_int),
+            "swm_ctx_get_option": ([vp, C.c_int, C.POINTER(C.c_int64)], C.c_int),
+            "swm_autotune": ([vp, vp, u64, vp, u64, u32, u32, C.POINTER(C.c_int64)], C.c_int),
+            "swm_server_create": ([vp, C.POINTER(ServerConfig), C.POINTER(vp)], C.c_int),
+            "swm_server_submit": ([vp, vp, vp, u64, vp, C.POINTER(u64)], C.c_int),
+            "swm_server_wait": ([vp, u64, C.POINTER(BatchInfo)], C.c_int),
+            "swm_server_totals": ([vp, C.POINTER(u64), C.POINTER(u64), C.POINTER(u64), C.POINTER(u64)], C.c_int),
+            "swm_server_destroy": ([vp], None),
+            "swm_host_alloc": ([C.POINTER(vp), u64], C.c_int),
+            "swm_host_free": ([vp], C.c_int),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -227,7 +258,80 @@ class Mapper:
     def stats(self):
         s = Stats()
         _check("swm_last_stats", lib().swm_last_stats(self._h, C.byref(s)))
-        return {f: getattr(s, f) for f, _ in Stats._fields_}
+        return s.as_dict()
+
+    def set_option(self, opt, value):
+        _check("swm_ctx_set_option", lib().swm_ctx_set_option(self._h, opt, int(value)))
+
+    def get_option(self, opt):
+        v = C.c_int64(0)
+        _check("swm_ctx_get_option", lib().swm_ctx_get_option(self._h, opt, C.byref(v)))
+        return v.value
+
+    def options(self):
+        return {name: self.get_option(o) for o, name in OPTION_NAMES.items()}
+
+    def autotune(self, reads, lens, e=5, reps=3):
+        """Times the pipeline on a sample for each tunable's candidates and
+        keeps the fastest; returns the chosen options."""
+        reads = np.ascontiguousarray(reads, dtype=np.uint8)
+        lens = np.ascontiguousarray(lens, dtype=np.uint32)
+        n, stride = reads.shape
+        ch = (C.c_int64 * 6)()
+        _check("swm_autotune", lib().swm_autotune(self._h, _ptr(reads), stride, _ptr(lens), n, e, reps, ch))
+        return {OPTION_NAMES[i + 1]: ch[i] for i in range(6)}
+
+
+class Server:
+    """Continuous-batching serving loop over a Mapper's resident index: the
+    oldest waiting batch is admitted when one of n_slots streams is free and
+    the HBM budget holds its estimated workspace."""
+
+    def __init__(self, mapper, e=5, stride=256, n_slots=4, hbm_budget=0, max_batch=65536, graph=True):
+        self.mapper = mapper
+        cfg = ServerConfig(n_slots, e, stride, hbm_budget, max_batch, 1 if graph else 0, 0)
+        h = C.c_void_p()
+        _check("swm_server_create", lib().swm_server_create(mapper._h, C.byref(cfg), C.byref(h)))
+        self._h = h
+        self._keep = {}
+
+    def submit(self, reads, lens):
+        """Non-blocking; returns a ticket.  The arrays are kept alive until wait()."""
+        reads = np.ascontiguousarray(reads, dtype=np.uint8)
+        lens = np.ascontiguousarray(lens, dtype=np.uint32)
+        out = np.zeros(lens.size, dtype=RESULT_DTYPE)
+        t = C.c_uint64(0)
+        _check("swm_server_submit", lib().swm_server_submit(self._h, _ptr(reads), _ptr(lens), lens.size, _ptr(out),
+                                                            C.byref(t)))
+        self._keep[t.value] = (reads, lens, out)
+        return t.value
+
+    def wait(self, ticket):
+        """Blocks until the batch is done: (results, info dict)."""
+        info = BatchInfo()
+        rc = lib().swm_server_wait(self._h, ticket, C.byref(info))
+        reads, lens, out = self._keep.pop(ticket)
+        _check("swm_server_wait", rc)
+        d = {f: getattr(info, f) for f, _ in BatchInfo._fields_ if f != "stats"}
+        d["stats"] = info.stats.as_dict()
+        return out, d
+
+    def totals(self):
+        v = [C.c_uint64(0) for _ in range(4)]
+        _check("swm_server_totals", lib().swm_server_totals(self._h, *(C.byref(x) for x in v)))
+        return dict(zip(("batches", "reads", "peak_bytes", "budget"), (x.value for x in v)))
+
+    def close(self):
+        if self._h:
+            lib().swm_server_destroy(self._h)
+            self._h = None
+            self._keep.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def valu_peak():
@@ -274,6 +378,32 @@ class DeviceBuffer:
     def free(self):
         if self.ptr:
             lib().swm_dev_free(C.c_void_p(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class PinnedArray:
+    """Page-locked host memory as a numpy array (``.a``): batches handed to
+    the serving loop from pinned memory move over PCIe by DMA."""
+
+    def __init__(self, shape, dtype):
+        dt = np.dtype(dtype)
+        nbytes = int(np.prod(shape)) * dt.itemsize
+        p = C.c_void_p()
+        _check("swm_host_alloc", lib().swm_host_alloc(C.byref(p), max(nbytes, 1)))
+        self.ptr = p.value
+        buf = (C.c_uint8 * max(nbytes, 1)).from_address(self.ptr)
+        self.a = np.frombuffer(buf, dtype=np.uint8, count=nbytes).view(dt).reshape(shape)
+
+    def free(self):
+        if self.ptr:
+            self.a = None
+            lib().swm_host_free(C.c_void_p(self.ptr))
             self.ptr = None
 
     def __del__(self):

@@ -26,7 +26,7 @@ HIP_FLAGS = [
 ]
 C_FLAGS = ["-O3", "-fPIC", "-std=gnu11", "-Wall", "-Wextra", "-Wno-unused-parameter", "-I" + INC, "-I" + CSRC]
 
-HIP_SOURCES = ["swm_map.hip", "swm_index.hip"]
+HIP_SOURCES = ["swm_map.hip", "swm_index.hip", "swm_serve.hip"]
 C_SOURCES = ["swm_io.c", "swm_synth.c", "swm_recipe.c"]
 CLIS = {"swmapper": "swm_cli_map.c", "swm_index": "swm_cli_index.c"}
 

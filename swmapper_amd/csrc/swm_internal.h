@@ -30,24 +30,48 @@ struct swm_sel_t {
     uint32_t meta;
 };
 
-struct swm_ws;  /* per-context device workspace */
+struct swm_ws;    /* device workspace of one slot */
+struct swm_slot;  /* stream + workspace + captured graphs */
+
+/* Forms and tunables of the pipeline (swm_ctx_set_option). */
+struct swm_opts {
+    uint32_t work_queue = 1;        /* 0: the plain one-thread-per-item kernels for every item */
+    uint32_t graph = 0;             /* 1: fixed-capacity pipeline replayed as a hipGraph */
+    uint32_t heavy_cands = 256;     /* items with more candidates go to the cooperative kernels */
+    uint32_t myers_block = 256;     /* workgroup size of the verification kernel */
+    uint32_t plan_heavy_waves = 8;  /* persistent waves per CU draining the heavy plan queue */
+    uint32_t emit_heavy_waves = 32; /* waves per CU of the cooperative emission */
+};
 
 struct swm_ctx {
-    int device;
-    int n_cu;                   /* compute units: sizes the persistent work-queue grids */
-    hipStream_t stream;
-    uint32_t k;
-    uint64_t n_buckets;
-    uint32_t *d_hash_offset;    /* exclusive scan of hash_count, 4^k + 1 */
-    uint2 *d_pl;                /* bucket entries: (position, 12-bit residual) */
-    uint64_t n_pos;
-    uint32_t *d_base_2bits;     /* file words + SWM_WIN_PAIRS_PAD zero pairs */
-    uint64_t n_2bits_words;
-    uint64_t ref_length;        /* 16 * bytes(.base_2bits) (mapper_mpe.c:288) */
-    swm_recipe_t *d_recipe;     /* read-encoding recipe per read length 0..254 */
-    swm_stats_t stats;
-    swm_ws *ws;
+    int device = 0;
+    int n_cu = 256;                     /* compute units: sizes the persistent work-queue grids */
+    uint32_t k = 12;
+    uint64_t n_buckets = 0;
+    uint32_t *d_hash_offset = nullptr;  /* exclusive scan of hash_count, 4^k + 1 */
+    uint2 *d_pl = nullptr;              /* bucket entries: (position, 12-bit residual) */
+    uint64_t n_pos = 0;
+    uint32_t *d_base_2bits = nullptr;   /* file words + SWM_WIN_PAIRS_PAD zero pairs */
+    uint64_t n_2bits_words = 0;
+    uint64_t ref_length = 0;            /* 16 * bytes(.base_2bits) (mapper_mpe.c:288) */
+    swm_recipe_t *d_recipe = nullptr;   /* read-encoding recipe per read length 0..254 */
+    swm_opts opt;
+    swm_slot *slot0 = nullptr;          /* the slot of the synchronous entry points */
 };
+
+swm_slot *swm_slot_create(swm_ctx_t *ctx);
+void swm_slot_destroy(swm_slot *s);
+const swm_stats_t *swm_slot_stats(const swm_slot *s);
+uint64_t swm_slot_bytes(const swm_slot *s);
+/* map n reads already on the device, on the slot's stream, in the synced or
+ * the graph form (returns after the results are written) */
+int swm_slot_run_device(swm_ctx_t *ctx, swm_slot *sl, const char *d_reads, uint64_t stride, const uint32_t *d_lens,
+                        uint64_t n, uint32_t e, swm_read_result_t *d_out, bool graph);
+/* map n host reads through the slot's buffers, padded to n_pad reads of length 0 */
+int swm_slot_run_host(swm_ctx_t *ctx, swm_slot *sl, const char *reads, uint64_t stride, const uint32_t *lens,
+                      uint64_t n, uint64_t n_pad, uint32_t e, swm_read_result_t *out, bool graph);
+/* SWM_EINVAL if a read breaks the stride rule or has more than 16 long seeds */
+int swm_check_lens(const swm_ctx_t *ctx, uint64_t stride, const uint32_t *lens, uint64_t n);
 
 #define SWM_HIP_CHECK(call)                                   \
     do {                                                      \

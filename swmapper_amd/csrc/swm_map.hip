@@ -28,7 +28,6 @@
  */
 #include "swm_internal.h"
 
-#include <hipcub/hipcub.hpp>
 #include <new>
 #include <stdlib.h>
 #include <string.h>
@@ -103,6 +102,27 @@ __device__ __forceinline__ void block_append(bool pred, uint32_t value, uint32_t
     if (threadIdx.x == 0) s_tmp[1] = s_tmp[0] ? atomicAdd(count, s_tmp[0]) : 0u;
     __syncthreads();
     if (pred) list[s_tmp[1] + woff + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = value;
+}
+
+/* Wave-wide inclusive prefix sum (64 lanes). */
+__device__ __forceinline__ uint32_t shfl_up_t(uint32_t v, int d) { return (uint32_t)__shfl_up((int)v, d, 64); }
+__device__ __forceinline__ uint64_t shfl_up_t(uint64_t v, int d)
+{
+    const uint32_t lo = (uint32_t)__shfl_up((int)(uint32_t)v, d, 64);
+    const uint32_t hi = (uint32_t)__shfl_up((int)(uint32_t)(v >> 32), d, 64);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_incl_scan(T v)
+{
+    const uint32_t lane = __lane_id();
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const T u = shfl_up_t(v, d);
+        if (lane >= (uint32_t)d) v += u;
+    }
+    return v;
 }
 
 /* ------------------------------------------------------------------ k_seed */
@@ -316,6 +336,7 @@ struct PlanIO {
     uint32_t *heavy;            /* (read, strand) items whose seeds need the seed heap */
     uint32_t *heavy_count;
     uint32_t *heavy_head;       /* work-queue head of k_plan_heavy */
+    const uint32_t *ovf;        /* set when a fixed-capacity run would overflow: every stage stands down */
 };
 
 /* 15-mer filter of one bucket entry (long_seeds_filter :2106-2117) and the
@@ -338,6 +359,7 @@ __device__ __forceinline__ uint32_t adjust(uint32_t p, uint32_t meta)
 __global__ void __launch_bounds__(64) k_plan(uint32_t nrs, const uint2 *__restrict__ pl, PlanIO io)
 {
     __shared__ uint32_t s_tmp[2];
+    if (*io.ovf) return;
     uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
     const bool live = r < nrs;
     if (!live) r = 0;
@@ -390,6 +412,125 @@ __global__ void __launch_bounds__(64) k_plan(uint32_t nrs, const uint2 *__restri
         io.nuniq[r] = n;
     }
     io.nbatch[r] = 1;
+}
+
+/* The plain form of the same state machine: one thread per (read, strand)
+ * for every item, the seed heap in registers (no work queue).  Kept as the
+ * one-launch-per-stage baseline the work queue is measured against. */
+__global__ void __launch_bounds__(64) k_plan_plain(uint32_t nrs, const uint2 *__restrict__ pl, PlanIO io)
+{
+    if (*io.ovf) return;
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrs) return;
+    const uint32_t nsel = io.nsel[r] & ~NSEL_HEAVY;
+    const swm_sel_t *sel = io.sel + (uint64_t)r * io.seln;
+    uint32_t *surv = io.surv + io.surv_off[r];
+    uint2 *bd = io.bdesc + io.bd_off[r];
+
+    uint32_t pcount[64];
+    uint32_t hloc[66], hind[66];      /* seed heap, 1-based (mapper_cpe.c:2149) */
+    uint32_t hsize = 0;
+    uint32_t nsurv = 0, bstart = 0, batch = 0, pos_size = 0, blocks = 0;
+    bool first_pending = true;        /* Q7 */
+
+    auto take = [&](uint32_t g, uint32_t len, uint32_t meta) {
+        uint32_t lv[SWM_CHUNK], pv[SWM_CHUNK];
+#pragma unroll
+        for (int j = 0; j < SWM_CHUNK; j++) {
+            if ((uint32_t)j < len) { const uint2 v = pl[g + j]; pv[j] = v.x; lv[j] = v.y; }
+        }
+#pragma unroll
+        for (int j = 0; j < SWM_CHUNK; j++) {
+            if ((uint32_t)j < len) {
+                if (first_pending) first_pending = false;
+                else if (filter_pass(lv[j], meta)) surv[nsurv++] = adjust(pv[j], meta);
+            }
+        }
+    };
+    auto heap_insert = [&](uint32_t loc, uint32_t ind) {      /* :1919-1935 */
+        hsize++;
+        hloc[hsize] = loc; hind[hsize] = ind;
+        uint32_t i = hsize;
+        while (i > 1) {
+            const uint32_t ti = i / 2;
+            if (hloc[ti] > hloc[i]) {
+                uint32_t t = hloc[ti]; hloc[ti] = hloc[i]; hloc[i] = t;
+                t = hind[ti]; hind[ti] = hind[i]; hind[i] = t;
+            } else break;
+            i = ti;
+        }
+    };
+    auto heap_pop_insert = [&](uint32_t loc, bool del) {      /* :1936-1962 (Q6) */
+        if (del) { hloc[1] = hloc[hsize]; hind[1] = hind[hsize]; hsize--; }
+        else hloc[1] = loc;
+        uint32_t i = 1;
+        while (i * 2 < hsize) {
+            uint32_t ti = i * 2;
+            if (i * 2 + 1 <= hsize && hloc[i * 2] > hloc[i * 2 + 1]) ti = i * 2 + 1;
+            if (hloc[ti] < hloc[i]) {
+                uint32_t t = hloc[ti]; hloc[ti] = hloc[i]; hloc[i] = t;
+                t = hind[ti]; hind[ti] = hind[i]; hind[i] = t;
+            } else break;
+            i = ti;
+        }
+    };
+
+    bool heavy = false;
+    for (uint32_t i = 0; i < nsel; i++) {                     /* phase 1 (:2194-2236) */
+        const swm_sel_t s = sel[i];
+        const uint32_t get = s.cnt > SWM_CHUNK ? SWM_CHUNK : s.cnt;
+        if (get > 0) { take(s.off, get, s.meta); blocks++; }
+        pos_size += get;
+        pcount[i] = get;
+        heavy |= get != s.cnt;
+    }
+    if (heavy) {
+        for (uint32_t i = 0; i < nsel; i++) {                 /* :2244-2252 */
+            const swm_sel_t s = sel[i];
+            if (pcount[i] != s.cnt) heap_insert(pl[s.off + pcount[i] - 1].x, i);
+        }
+    }
+    for (;;) {                                                /* :2261-2419 */
+        if (hsize > 0 && pos_size + SWM_CHUNK <= SWM_BATCH_POS && blocks < SWM_MAX_BLOCKS) {
+            const uint32_t i = hind[1];
+            const swm_sel_t s = sel[i];
+            const uint32_t rem = s.cnt - pcount[i];
+            const uint32_t get = rem > SWM_CHUNK ? SWM_CHUNK : rem;
+            take(s.off + pcount[i], get, s.meta);
+            pos_size += get;
+            pcount[i] += get;
+            blocks++;
+            heap_pop_insert(pl[s.off + pcount[i] - 1].x, pcount[i] == s.cnt);
+        }
+        if (pos_size + SWM_CHUNK > SWM_BATCH_POS || hsize == 0 || blocks >= SWM_MAX_BLOCKS) {
+            const uint32_t n = nsurv - bstart;
+            if (n <= SMALL_BATCH) {
+                uint32_t *a = surv + bstart;
+                for (uint32_t i = 1; i < n; i++) {
+                    uint32_t v = a[i], j = i;
+                    while (j > 0 && a[j - 1] > v) { a[j] = a[j - 1]; j--; }
+                    a[j] = v;
+                }
+                uint32_t u = n ? 1u : 0u;
+                for (uint32_t i = 1; i < n; i++)
+                    if (a[i] != a[u - 1]) a[u++] = a[i];
+                bd[batch] = make_uint2(bstart, u);
+                nsurv = bstart + u;
+            } else {
+                bd[batch] = make_uint2(bstart, n);
+                const uint32_t slot = atomicAdd(io.big_count, 1u);
+                io.big[slot] = make_uint2(r, batch);
+            }
+            bstart = nsurv;
+            batch++;
+            pos_size = 0;
+            blocks = 0;
+            first_pending = true;
+            if (hsize == 0) break;
+        }
+    }
+    io.nbatch[r] = batch;
+    io.nuniq[r] = nsurv;
 }
 
 /* Heavy (read, strand): a device work queue drained by persistent
@@ -645,24 +786,25 @@ struct CandSet {
     int32_t *loc;
     uint32_t *rec;      /* record location (cigar->location) for passing lanes */
     uint32_t *info;     /* CIGAR bytes | kind bit 16 (1 = traceback) | bit 17: record location < 0 */
+    const uint32_t *ovf;    /* capacity overflow flag (see PlanIO) */
 };
 
 #define INFO_TRACE (1u << 16)
 #define INFO_NEG (1u << 17)
 
-#define HEAVY_CANDS 256   /* items with more candidates go to the cooperative kernels */
-
+/* Items with more than heavy_cands candidates go to the cooperative kernels. */
 __global__ void __launch_bounds__(128) k_compact(uint32_t nrs, const uint64_t *__restrict__ surv_off,
                                                  const uint32_t *__restrict__ surv, const uint32_t *__restrict__ bd_off,
                                                  const uint2 *__restrict__ bdesc, const uint32_t *__restrict__ nbatch,
-                                                 const uint32_t *__restrict__ cand_off, CandSet c,
+                                                 const uint32_t *__restrict__ cand_off, CandSet c, uint32_t heavy_cands,
                                                  uint32_t *__restrict__ heavy, uint32_t *__restrict__ heavy_count)
 {
     __shared__ uint32_t s_tmp[2];
+    if (*c.ovf) return;
     uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
     const bool live = r < nrs;
     if (!live) r = 0;
-    const bool big = live && cand_off[r + 1] - cand_off[r] > HEAVY_CANDS;
+    const bool big = live && cand_off[r + 1] - cand_off[r] > heavy_cands;
     block_append(big, r, heavy, heavy_count, s_tmp);
     if (!live || big) return;
     const uint32_t *s = surv + surv_off[r];
@@ -810,12 +952,15 @@ __device__ __forceinline__ void load_window(const uint32_t *__restrict__ b2, uin
     }
 }
 
-__global__ void __launch_bounds__(256) k_myers(uint32_t ncand, const uint32_t *__restrict__ lens,
+/* The candidate count is read on the device (the last entry of the candidate
+ * offsets), so a fixed grid serves any batch: grid-stride loop. */
+__global__ void __launch_bounds__(256) k_myers(const uint32_t *__restrict__ ncand_p, const uint32_t *__restrict__ lens,
                                                const uint32_t *__restrict__ words, const uint32_t *__restrict__ b2,
                                                uint64_t ref_length, uint32_t e, CandSet c)
 {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < ncand) {
+    if (*c.ovf) return;
+    const uint32_t ncand = *ncand_p;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < ncand; i += gridDim.x * blockDim.x) {
         const uint32_t rs = c.rs[i];
         const uint32_t L = lens[rs >> 1];
         uint32_t ref_len;
@@ -835,63 +980,78 @@ __global__ void __launch_bounds__(256) k_myers(uint32_t ncand, const uint32_t *_
     }
 }
 
-/* Hamming check of align_reads (:252-287) for the lanes with err <= e: the
+/* Hamming check of align_reads (:252-287) for one lane with err <= e: the
  * read against the window at the implied start loc - L + 1.  Equal counts
- * give a "<L>M" record; the rest go to the traceback. */
-__global__ void __launch_bounds__(256) k_hamming(uint32_t ncand, const uint32_t *__restrict__ lens,
+ * give a "<L>M" record; the rest go to the traceback (returns true). */
+__device__ __forceinline__ bool hamming_one(uint32_t i, int32_t er, const uint32_t *__restrict__ lens,
+                                            const uint32_t *__restrict__ words, const uint32_t *__restrict__ b2,
+                                            uint64_t ref_length, uint32_t e, const CandSet &c)
+{
+    const int32_t lc = c.loc[i];
+    const uint32_t rs = c.rs[i];
+    const uint32_t L = lens[rs >> 1];
+    uint32_t ref_len;
+    const uint32_t st = pos_start(ref_length, e, c.val[i], L, &ref_len);
+    const uint32_t *w = words + (uint64_t)rs * 32;
+    uint32_t wl[SWM_NW], wh[SWM_NW];
+    load_window(b2, st, L, wl, wh);
+    const int sl = lc - (int)L + 1;
+    int tmp = 0;
+#pragma unroll
+    for (int ii = 0; ii < SWM_NW - 1; ii++) {
+        if ((uint32_t)(32 * ii) < L) {
+            uint32_t mask = 0xffffffffu;
+            if ((uint32_t)(32 * (ii + 1)) > L) mask >>= (32 - (L - 32 * ii));
+            const uint32_t lref = __builtin_amdgcn_alignbit(wl[ii + 1], wl[ii], (uint32_t)sl);
+            const uint32_t href = __builtin_amdgcn_alignbit(wh[ii + 1], wh[ii], (uint32_t)sl);
+            tmp += __popc(((w[17 + 2 * ii] ^ href) | (w[16 + 2 * ii] ^ lref)) & mask);
+        }
+    }
+    if (tmp == er) {
+        c.rec[i] = st + (uint32_t)sl + 1u;
+        c.info[i] = 2u + (uint32_t)ndigits(L) + 1u;
+        return false;
+    }
+    c.rec[i] = 0;
+    c.info[i] = INFO_TRACE;
+    return true;
+}
+
+/* Per candidate: the Hamming check of passing lanes, the traceback list, the
+ * count of rejected windows and the band steps of the verified ones.  The
+ * trip count is block-uniform, so every thread reaches block_append. */
+__global__ void __launch_bounds__(256) k_hamming(const uint32_t *__restrict__ ncand_p, const uint32_t *__restrict__ lens,
                                                  const uint32_t *__restrict__ words, const uint32_t *__restrict__ b2,
                                                  uint64_t ref_length, uint32_t e, CandSet c,
                                                  uint32_t *__restrict__ trace, uint32_t *__restrict__ trace_count,
                                                  uint32_t *__restrict__ n_rejected, unsigned long long *n_steps)
 {
     __shared__ uint32_t s_tmp[2];
-    __shared__ uint32_t s_steps;
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    bool to_trace = false;
-    uint32_t steps = 0;
-    if (threadIdx.x == 0) s_steps = 0;
-    if (i < ncand) {
-        const int32_t er = c.err[i];
-        if (er == -2) {
-            atomicAdd(n_rejected, 1u);         /* pos_start rejected the window: not a lane */
-        } else {
-            steps = lens[c.rs[i] >> 1];
-        }
-        if (er != -2 && er <= (int32_t)e) {
-            const int32_t lc = c.loc[i];
-            const uint32_t rs = c.rs[i];
-            const uint32_t L = lens[rs >> 1];
-            uint32_t ref_len;
-            const uint32_t st = pos_start(ref_length, e, c.val[i], L, &ref_len);
-            const uint32_t *w = words + (uint64_t)rs * 32;
-            uint32_t wl[SWM_NW], wh[SWM_NW];
-            load_window(b2, st, L, wl, wh);
-            const int sl = lc - (int)L + 1;
-            int tmp = 0;
-#pragma unroll
-            for (int ii = 0; ii < SWM_NW - 1; ii++) {
-                if ((uint32_t)(32 * ii) < L) {
-                    uint32_t mask = 0xffffffffu;
-                    if ((uint32_t)(32 * (ii + 1)) > L) mask >>= (32 - (L - 32 * ii));
-                    const uint32_t lref = __builtin_amdgcn_alignbit(wl[ii + 1], wl[ii], (uint32_t)sl);
-                    const uint32_t href = __builtin_amdgcn_alignbit(wh[ii + 1], wh[ii], (uint32_t)sl);
-                    tmp += __popc(((w[17 + 2 * ii] ^ href) | (w[16 + 2 * ii] ^ lref)) & mask);
-                }
-            }
-            if (tmp == er) {
-                c.rec[i] = st + (uint32_t)sl + 1u;
-                c.info[i] = 2u + (uint32_t)ndigits(L) + 1u;
+    __shared__ uint32_t s_steps, s_rej;
+    if (*c.ovf) return;
+    const uint32_t ncand = *ncand_p;
+    for (uint32_t b0 = blockIdx.x * blockDim.x; b0 < ncand; b0 += gridDim.x * blockDim.x) {
+        const uint32_t i = b0 + threadIdx.x;
+        bool to_trace = false, rej = false;
+        uint32_t steps = 0;
+        if (threadIdx.x == 0) { s_steps = 0; s_rej = 0; }
+        if (i < ncand) {
+            const int32_t er = c.err[i];
+            if (er == -2) {
+                rej = true;                    /* pos_start rejected the window: not a lane */
             } else {
-                c.rec[i] = 0;
-                c.info[i] = INFO_TRACE;
-                to_trace = true;
+                steps = lens[c.rs[i] >> 1];
+                if (er <= (int32_t)e) to_trace = hamming_one(i, er, lens, words, b2, ref_length, e, c);
             }
         }
+        block_append(to_trace, i, trace, trace_count, s_tmp);   /* its barriers order the resets above */
+        if (steps) atomicAdd(&s_steps, steps);
+        if (rej) atomicAdd(&s_rej, 1u);
+        __syncthreads();
+        if (threadIdx.x == 0 && s_steps) atomicAdd(n_steps, (unsigned long long)s_steps);
+        if (threadIdx.x == 0 && s_rej) atomicAdd(n_rejected, s_rej);
+        __syncthreads();
     }
-    block_append(to_trace, i, trace, trace_count, s_tmp);
-    if (steps) atomicAdd(&s_steps, steps);
-    __syncthreads();
-    if (threadIdx.x == 0 && s_steps) atomicAdd(n_steps, (unsigned long long)s_steps);
 }
 
 /* Stage entry: explicit candidates. */
@@ -1128,14 +1288,15 @@ __device__ __forceinline__ void add_records(uint32_t count, unsigned long long *
 
 __global__ void __launch_bounds__(128) k_emit(uint32_t nreads, uint32_t e, const uint32_t *__restrict__ cand_off,
                                               CandSet c, swm_read_result_t *__restrict__ out,
-                                              unsigned long long *n_records, uint32_t *__restrict__ heavy,
-                                              uint32_t *__restrict__ heavy_count)
+                                              unsigned long long *n_records, uint32_t heavy_cands,
+                                              uint32_t *__restrict__ heavy, uint32_t *__restrict__ heavy_count)
 {
     __shared__ uint32_t s_tmp[2], s_sum;
+    if (*c.ovf) return;
     uint32_t rd = blockIdx.x * blockDim.x + threadIdx.x;
     const bool live = rd < nreads;
     if (!live) rd = 0;
-    const bool big = live && cand_off[2 * rd + 2] - cand_off[2 * rd] > HEAVY_CANDS;
+    const bool big = live && cand_off[2 * rd + 2] - cand_off[2 * rd] > heavy_cands;
     block_append(big, rd, heavy, heavy_count, s_tmp);
     Emit em;
     em.init();
@@ -1156,10 +1317,38 @@ __global__ void __launch_bounds__(128) k_emit(uint32_t nreads, uint32_t e, const
     add_records(live && !big ? em.count : 0u, n_records, &s_sum);
 }
 
-/* One wave per read with many candidates: each lane holds one of 64
- * candidates, and the whole wave steps the emission over them in order on
- * wave-uniform values (v_readlane), so there is no LDS traffic or barrier in
- * the sequential part. */
+/* One wave per read with many candidates, 64 candidates per step in
+ * parallel.  The emission state machine (Emit) reduces to two facts:
+ *  - the "previous record" of the duplicate rule is always the record
+ *    location of the previous passing candidate of the strand (a skipped
+ *    duplicate has that location already), so a traceback record is a
+ *    duplicate candidate ("dup") iff it is not negative and equals its
+ *    passing predecessor; a dup is skipped unless no record has been written
+ *    in the current output block;
+ *  - the block counters only change at flush checks, which close each group
+ *    of <= 8 valid lanes of a batch (:1790, :1858).
+ * So the lanes classify themselves, and only the per-group counter update is
+ * sequential: a dup that opens a group entered with an empty block is
+ * written, every other dup is skipped, every other passing lane is written.
+ * The best hit is the first minimum of (distance, strand, order) over the
+ * written records. */
+__device__ __forceinline__ uint64_t wave_min_u64(uint64_t v)
+{
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)v, d, 64);
+        const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(v >> 32), d, 64);
+        const uint64_t u = ((uint64_t)hi << 32) | lo;
+        v = u < v ? u : v;
+    }
+    return v;
+}
+
+__device__ __forceinline__ uint32_t rdlane(uint32_t v, uint32_t l)
+{
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)l);
+}
+
 __global__ void __launch_bounds__(64) k_emit_heavy(const uint32_t *__restrict__ heavy,
                                                    const uint32_t *__restrict__ heavy_count, uint32_t e,
                                                    const uint32_t *__restrict__ cand_off, CandSet c,
@@ -1167,39 +1356,233 @@ __global__ void __launch_bounds__(64) k_emit_heavy(const uint32_t *__restrict__ 
                                                    unsigned long long *n_records)
 {
     const uint32_t lane = threadIdx.x;
+    const uint64_t below = (1ull << lane) - 1ull;
     const uint32_t nh = *heavy_count;
     for (uint32_t h = blockIdx.x; h < nh; h += gridDim.x) {
         const uint32_t rd = heavy[h];
-        Emit em;
-        em.init();
+        uint32_t count = 0, best_rec = 0;
+        uint64_t best = ~0ull;
         for (uint32_t strand = 0; strand < 2; strand++) {
             const uint32_t rs = 2 * rd + strand;
-            em.start_strand();
             const uint32_t c0 = cand_off[rs], cend = cand_off[rs + 1];
+            /* carries between windows */
+            uint32_t cur_b = 0xffffffffu, vcnt = 0, prevrec = 0;
+            bool has_prev = false;
+            uint32_t num = 0, len = 0;          /* output block counters */
+            /* the open group */
+            bool g_open = false, g_seen = false, g_firstd = false;
+            uint32_t g_n = 0, g_len = 0, g_ftext = 0, g_frec = 0;
+            uint64_t g_fkey = 0;
+            auto close_group = [&]() {
+                if (!g_open) return;
+                if (num == 0 && g_firstd) {
+                    num = 1;
+                    len = g_ftext;
+                    count++;
+                    if (g_fkey < best) { best = g_fkey; best_rec = g_frec; }
+                }
+                num += g_n;
+                len += g_len;
+                if (num + 8 > 256 || len + 16 * 16 > 256 * 16) { num = 0; len = 0; }
+                g_open = false;
+            };
             for (uint32_t base = c0; base < cend; base += 64) {
                 const uint32_t ci = base + lane;
-                uint32_t vbi = 0, vrec = 0, vinfo = 0;
-                int32_t verr = -2;
-                if (ci < cend) {
-                    verr = c.err[ci];
-                    vbi = c.bi[ci];
-                    if (verr >= 0 && verr <= (int32_t)e) { vrec = c.rec[ci]; vinfo = c.info[ci]; }
-                }
                 const uint32_t m = min(64u, cend - base);
-                for (uint32_t q = 0; q < m; q++) {
-                    em.step((uint32_t)__builtin_amdgcn_readlane((int)vbi, (int)q),
-                            __builtin_amdgcn_readlane(verr, (int)q),
-                            (uint32_t)__builtin_amdgcn_readlane((int)vrec, (int)q),
-                            (uint32_t)__builtin_amdgcn_readlane((int)vinfo, (int)q), e, strand);
+                const bool in = ci < cend;
+                const int32_t er = in ? c.err[ci] : -2;
+                const uint32_t b = in ? (c.bi[ci] >> 12) : 0xffffffffu;
+                const bool valid = er != -2;
+                const bool pass = valid && er <= (int32_t)e;
+                const uint32_t rec = pass ? c.rec[ci] : 0u;
+                const uint32_t info = pass ? c.info[ci] : 0u;
+                /* duplicate candidates: predecessor = last passing lane below, else the carry */
+                const uint64_t pm = __ballot(pass);
+                const uint64_t pb = pm & below;
+                const uint32_t src = pb ? 63u - (uint32_t)__clzll(pb) : lane;
+                const uint32_t prv_l = (uint32_t)__shfl((int)rec, (int)src, 64);
+                const bool hp = pb ? true : has_prev;
+                const uint32_t prv = pb ? prv_l : prevrec;
+                const bool dup = pass && (info & INFO_TRACE) && !(info & INFO_NEG) && hp && rec == prv;
+                /* index of the lane among the valid lanes of its batch */
+                uint32_t bprev = (uint32_t)__shfl_up((int)b, 1, 64);
+                if (lane == 0) bprev = cur_b;
+                const uint64_t sm = __ballot(in && b != bprev);
+                const uint64_t vm = __ballot(valid);
+                const uint64_t smb = sm & (below | (1ull << lane));
+                uint32_t v;
+                if (smb) {
+                    const uint32_t s0 = 63u - (uint32_t)__clzll(smb);
+                    v = (uint32_t)__popcll(vm & below & ~((1ull << s0) - 1ull));
+                } else {
+                    v = vcnt + (uint32_t)__popcll(vm & below);
                 }
+                const uint64_t gm = __ballot(valid && (v & 7u) == 0);
+                /* plainly written lanes */
+                const bool wr = pass && !dup;
+                const uint64_t am = __ballot(wr);
+                const uint32_t tinc = wave_incl_scan<uint32_t>(wr ? (info & 0xffffu) : 0u);
+                const uint64_t key = wr ? (((uint64_t)(uint32_t)er << 40) | ((uint64_t)strand << 32) | (ci - c0)) : ~0ull;
+                const uint64_t kmin = wave_min_u64(key);
+                if (kmin < best) {
+                    const uint64_t at = __ballot(key == kmin);
+                    best = kmin;
+                    best_rec = rdlane(rec, (uint32_t)__ffsll((unsigned long long)at) - 1u);
+                }
+                count += (uint32_t)__popcll(am);
+                /* lanes [lo, hi) join the open group */
+                auto add_range = [&](uint32_t lo, uint32_t hi) {
+                    if (!g_open || hi <= lo) return;
+                    const uint64_t rm = (hi >= 64 ? ~0ull : ((1ull << hi) - 1ull)) & ~((1ull << lo) - 1ull);
+                    g_n += (uint32_t)__popcll(am & rm);
+                    g_len += rdlane(tinc, hi - 1) - (lo ? rdlane(tinc, lo - 1) : 0u);
+                    const uint64_t pr = pm & rm;
+                    if (!g_seen && pr) {
+                        const uint32_t f = (uint32_t)__ffsll((unsigned long long)pr) - 1u;
+                        g_seen = true;
+                        const uint64_t dm = __ballot(dup);
+                        if ((dm >> f) & 1ull) {
+                            g_firstd = true;
+                            g_ftext = rdlane(info, f) & 0xffffu;
+                            g_frec = rdlane(rec, f);
+                            g_fkey = ((uint64_t)rdlane((uint32_t)er, f) << 40) | ((uint64_t)strand << 32) |
+                                     (uint64_t)(base + f - c0);
+                        }
+                    }
+                };
+                uint32_t from = 0;
+                for (uint64_t rest = gm; rest; rest &= rest - 1ull) {
+                    const uint32_t p = (uint32_t)__ffsll((unsigned long long)rest) - 1u;
+                    add_range(from, p);
+                    close_group();
+                    g_open = true; g_seen = false; g_firstd = false; g_n = 0; g_len = 0;
+                    from = p;
+                }
+                add_range(from, m);
+                /* carries */
+                const uint32_t last = m - 1u;
+                cur_b = rdlane(b, last);
+                vcnt = rdlane(v, last) + (uint32_t)((vm >> last) & 1ull);
+                if (pm) { has_prev = true; prevrec = rdlane(rec, 63u - (uint32_t)__clzll(pm)); }
             }
-            em.end_strand();
+            close_group();
         }
         if (lane == 0) {
-            out[rd] = em.result();
-            if (em.count) atomicAdd(n_records, (unsigned long long)em.count);
+            swm_read_result_t res;
+            res.pos = count ? best_rec : 0xffffffffu;
+            res.strand = count ? (uint8_t)((best >> 32) & 1ull) : (uint8_t)0xff;
+            res.dist = count ? (uint8_t)(best >> 40) : (uint8_t)0xff;
+            res.pad = 0;
+            res.count = count;
+            out[rd] = res;
+            if (count) atomicAdd(n_records, (unsigned long long)count);
         }
     }
+}
+
+/* ------------------------------------------------------- scans and checks */
+
+/* Exclusive prefix sums in three launches (tile sums, scan of the tile sums
+ * in one workgroup, tile rescan), graph-safe: plain kernels, no memset and
+ * no temporary-size query. */
+#define SCAN_NT 256
+#define SCAN_PER 16
+#define SCAN_TILE (SCAN_NT * SCAN_PER)
+
+/* Inclusive scan over the workgroup (NT threads); s_w holds NT/64 values. */
+template <typename T, int NT>
+__device__ __forceinline__ T block_incl_scan(T v, T *s_w, T *total)
+{
+    const uint32_t lane = __lane_id(), w = threadIdx.x >> 6;
+    v = wave_incl_scan(v);
+    if (lane == 63) s_w[w] = v;
+    __syncthreads();
+    T off = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < NT / 64; i++) {
+        const T x = s_w[i];
+        if ((uint32_t)i < w) off += x;
+        tot += x;
+    }
+    __syncthreads();
+    *total = tot;
+    return v + off;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(SCAN_NT) k_scan_reduce(const T *__restrict__ in, uint64_t n, T *__restrict__ sums)
+{
+    __shared__ T s_w[SCAN_NT / 64];
+    const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE;
+    T v = 0;
+#pragma unroll
+    for (int j = 0; j < SCAN_PER; j++) {
+        const uint64_t g = base + threadIdx.x + (uint64_t)j * SCAN_NT;
+        if (g < n) v += in[g];
+    }
+    T tot;
+    (void)block_incl_scan<T, SCAN_NT>(v, s_w, &tot);
+    if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(1024) k_scan_top(T *__restrict__ sums, uint32_t nb)
+{
+    __shared__ T s_w[16];
+    T carry = 0;
+    for (uint32_t b = 0; b < nb; b += 1024) {
+        const uint32_t i = b + threadIdx.x;
+        const T v = i < nb ? sums[i] : (T)0;
+        T tot;
+        const T incl = block_incl_scan<T, 1024>(v, s_w, &tot);
+        if (i < nb) sums[i] = carry + incl - v;
+        carry += tot;
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(SCAN_NT) k_scan_down(const T *__restrict__ in, T *__restrict__ out, uint64_t n,
+                                                       const T *__restrict__ sums)
+{
+    __shared__ T s_w[SCAN_NT / 64];
+    const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_PER;
+    T v[SCAN_PER];
+    T t = 0;
+#pragma unroll
+    for (int j = 0; j < SCAN_PER; j++) {
+        v[j] = base + j < n ? in[base + j] : (T)0;
+        t += v[j];
+    }
+    T tot;
+    T run = block_incl_scan<T, SCAN_NT>(t, s_w, &tot) - t + sums[blockIdx.x];
+#pragma unroll
+    for (int j = 0; j < SCAN_PER; j++) {
+        if (base + j < n) out[base + j] = run;
+        run += v[j];
+    }
+}
+
+/* Zeroes the run's counters and the sentinel entries the scans read. */
+__global__ void k_reset(unsigned long long *counters, uint64_t *npos_end, uint32_t *nchunk_end, uint32_t *nuniq_end)
+{
+    if (threadIdx.x < 8) counters[threadIdx.x] = 0;
+    if (threadIdx.x == 0) { *npos_end = 0; *nchunk_end = 0; *nuniq_end = 0; }
+}
+
+/* Capacity checks of the fixed-capacity form.  They also keep the totals in
+ * the counters: [0] candidates, [3] surviving positions. */
+__global__ void k_check_plan(const uint64_t *__restrict__ tot_pos, const uint32_t *__restrict__ tot_chunk,
+                             uint64_t cap_pos, uint64_t cap_chunk, unsigned long long *counters, uint32_t *ovf)
+{
+    counters[3] = *tot_pos;
+    if (*tot_pos > cap_pos || (uint64_t)*tot_chunk > cap_chunk) *ovf = 1u;
+}
+
+__global__ void k_check_cand(const uint32_t *__restrict__ ncand, uint64_t cap_cand, unsigned long long *counters,
+                             uint32_t *ovf)
+{
+    counters[0] = *ncand;
+    if ((uint64_t)*ncand > cap_cand) *ovf = 1u;
 }
 
 /* --------------------------------------------------------------- host side */
@@ -1208,13 +1591,16 @@ inline uint32_t blocks_for(uint64_t n, uint32_t bs) { return (uint32_t)((n + bs 
 
 }  // namespace
 
-/* Device workspace of a context, grown on demand and reused by every batch
- * (no allocation on the steady path). */
+/* Device workspace of a slot, grown on demand and reused by every batch (no
+ * allocation on the steady path).  gen changes whenever a buffer moves, which
+ * retires the graphs captured over the old addresses. */
 struct swm_ws {
     struct Buf { void *p = nullptr; size_t cap = 0; };
     enum { WORDS, SEL, NSEL, NPOS, NCHUNK, SURV_OFF, BD_OFF, SURV, BDESC, NBATCH, NUNIQ, CAND_OFF, C_VAL, C_RS, C_BI,
-           C_ERR, C_LOC, C_REC, C_INFO, COUNTERS, BIG, TRACE, HEAVY, HEAVY_C, HEAVY_E, SCAN_TMP, NBUF };
+           C_ERR, C_LOC, C_REC, C_INFO, COUNTERS, BIG, TRACE, HEAVY, HEAVY_C, HEAVY_E, SCAN_TMP, IN_READS, IN_LENS,
+           OUT, NBUF };
     Buf b[NBUF];
+    uint64_t gen = 1;
     ~swm_ws() { for (auto &x : b) if (x.p) (void)hipFree(x.p); }
     template <typename T>
     int get(int id, T **out, size_t n)
@@ -1224,6 +1610,7 @@ struct swm_ws {
             if (b[id].p) (void)hipFree(b[id].p);
             b[id].p = nullptr;
             b[id].cap = 0;
+            gen++;
             size_t want = bytes + bytes / 4;
             if (hipMalloc(&b[id].p, want) != hipSuccess) return SWM_ENOMEM;
             b[id].cap = want;
@@ -1231,26 +1618,442 @@ struct swm_ws {
         *out = (T *)b[id].p;
         return SWM_OK;
     }
+    uint64_t bytes() const
+    {
+        uint64_t t = 0;
+        for (const auto &x : b) t += x.cap;
+        return t;
+    }
 };
 
-static int ws_scan_u32(swm_ws *ws, uint32_t *in, uint32_t *out, size_t n, hipStream_t st)
+#define SLOT_GRAPHS 8
+
+/* One stream with its own workspace: batches on different slots run
+ * concurrently over the shared, read-only index. */
+struct swm_slot {
+    hipStream_t stream = nullptr;
+    swm_ws ws;
+    swm_stats_t stats{};
+    /* largest observed per-read totals: they size the fixed capacities */
+    double r_pos = 0, r_chunk = 0, r_cand = 0;
+    struct GraphEnt {
+        bool used = false;
+        const void *reads = nullptr, *lens = nullptr, *out = nullptr;
+        uint64_t n = 0, stride = 0, gen = 0, sig = 0, tick = 0;
+        uint32_t e = 0;
+        hipGraphExec_t exec = nullptr;
+    };
+    GraphEnt g[SLOT_GRAPHS];
+    uint64_t tick = 0;
+    hipEvent_t ev[6] = {};
+    /* page-locked staging, so that no copy goes through the runtime's shared
+     * pageable-memory path (which serialises the slots): totals read back
+     * between stages, counters, and host batches in and out */
+    uint64_t *h_small = nullptr;     /* [0..7] counters, [8] positions, [9] chunks, [10] candidates */
+    struct Pin { void *p = nullptr; size_t cap = 0; };
+    Pin h_reads, h_lens, h_out;
+    ~swm_slot()
+    {
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (auto &x : g) if (x.exec) (void)hipGraphExecDestroy(x.exec);
+        for (auto &x : ev) if (x) (void)hipEventDestroy(x);
+        for (Pin *x : {&h_reads, &h_lens, &h_out}) if (x->p) (void)hipHostFree(x->p);
+        if (h_small) (void)hipHostFree(h_small);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+    int pin(Pin &x, size_t bytes)
+    {
+        if (x.cap >= bytes) return SWM_OK;
+        if (x.p) (void)hipHostFree(x.p);
+        x.p = nullptr;
+        x.cap = 0;
+        const size_t want = bytes + bytes / 4;
+        if (hipHostMalloc(&x.p, want, hipHostMallocDefault) != hipSuccess) return SWM_ENOMEM;
+        x.cap = want;
+        return SWM_OK;
+    }
+};
+
+swm_slot *swm_slot_create(swm_ctx_t *ctx)
 {
-    size_t tmp = 0;
-    if (hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, in, out, n, st) != hipSuccess) return SWM_EDEVICE;
-    void *t;
-    int rc = ws->get(swm_ws::SCAN_TMP, (char **)&t, tmp);
-    if (rc) return rc;
-    return hipcub::DeviceScan::ExclusiveSum(t, tmp, in, out, n, st) == hipSuccess ? SWM_OK : SWM_EDEVICE;
+    (void)ctx;
+    swm_slot *s = new (std::nothrow) swm_slot();
+    if (!s) return nullptr;
+    bool ok = hipStreamCreate(&s->stream) == hipSuccess;
+    for (auto &x : s->ev) ok = ok && hipEventCreate(&x) == hipSuccess;
+    ok = ok && hipHostMalloc((void **)&s->h_small, 16 * sizeof(uint64_t), hipHostMallocDefault) == hipSuccess;
+    if (!ok) { delete s; return nullptr; }
+    return s;
 }
 
-static int ws_scan_u64(swm_ws *ws, uint64_t *in, uint64_t *out, size_t n, hipStream_t st)
+void swm_slot_destroy(swm_slot *s) { delete s; }
+
+const swm_stats_t *swm_slot_stats(const swm_slot *s) { return &s->stats; }
+
+uint64_t swm_slot_bytes(const swm_slot *s) { return s->ws.bytes(); }
+
+/* Device pointers of one run. */
+struct Pipe {
+    uint64_t n = 0, nrs = 0;
+    uint32_t seln = 0;
+    SeedOut so;
+    PlanIO pio;
+    CandSet cs;
+    uint64_t *surv_off = nullptr;
+    uint32_t *bd_off = nullptr, *cand_off = nullptr, *trace = nullptr, *heavy_c = nullptr, *heavy_e = nullptr;
+    unsigned long long *counters = nullptr;
+    uint32_t *u32c = nullptr;
+    uint64_t *scan_tmp = nullptr;
+};
+
+/* counters: u64 [0] candidates, [1] records, [2] band steps, [3] surviving
+ * positions; u32 view of [4..7]: [0] large batches, [1] tracebacks,
+ * [2] rejected windows, [3] heavy plan items, [4] their queue head,
+ * [5] heavy compaction items, [6] heavy emission items, [7] overflow */
+static int pipe_alloc_a(swm_ws *ws, uint64_t n, uint32_t seln, Pipe &P)
 {
-    size_t tmp = 0;
-    if (hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, in, out, n, st) != hipSuccess) return SWM_EDEVICE;
-    void *t;
-    int rc = ws->get(swm_ws::SCAN_TMP, (char **)&t, tmp);
+    const uint64_t nrs = 2 * n;
+    P.n = n;
+    P.nrs = nrs;
+    P.seln = seln;
+    int rc;
+    if ((rc = ws->get(swm_ws::WORDS, &P.so.words, nrs * 32)) || (rc = ws->get(swm_ws::SEL, &P.so.sel, nrs * seln)) ||
+        (rc = ws->get(swm_ws::NSEL, &P.so.nsel, nrs)) || (rc = ws->get(swm_ws::NPOS, &P.so.npos, nrs + 1)) ||
+        (rc = ws->get(swm_ws::NCHUNK, &P.so.nchunk, nrs + 1)) || (rc = ws->get(swm_ws::COUNTERS, &P.counters, 8)) ||
+        (rc = ws->get(swm_ws::SURV_OFF, &P.surv_off, nrs + 1)) || (rc = ws->get(swm_ws::BD_OFF, &P.bd_off, nrs + 1)) ||
+        (rc = ws->get(swm_ws::NBATCH, &P.pio.nbatch, nrs)) || (rc = ws->get(swm_ws::NUNIQ, &P.pio.nuniq, nrs + 1)) ||
+        (rc = ws->get(swm_ws::HEAVY, &P.pio.heavy, nrs)) || (rc = ws->get(swm_ws::CAND_OFF, &P.cand_off, nrs + 1)) ||
+        (rc = ws->get(swm_ws::HEAVY_C, &P.heavy_c, nrs)) || (rc = ws->get(swm_ws::HEAVY_E, &P.heavy_e, n)) ||
+        (rc = ws->get(swm_ws::SCAN_TMP, &P.scan_tmp, blocks_for(nrs + 1, SCAN_TILE))))
+        return rc;
+    P.u32c = reinterpret_cast<uint32_t *>(P.counters + 4);
+    P.pio.sel = P.so.sel;
+    P.pio.seln = seln;
+    P.pio.nsel = P.so.nsel;
+    P.pio.surv_off = P.surv_off;
+    P.pio.bd_off = P.bd_off;
+    P.pio.big_count = P.u32c;
+    P.pio.heavy_count = P.u32c + 3;
+    P.pio.heavy_head = P.u32c + 4;
+    P.pio.ovf = P.u32c + 7;
+    P.cs.ovf = P.u32c + 7;
+    return SWM_OK;
+}
+
+static int pipe_alloc_b(swm_ws *ws, uint64_t cap_pos, uint64_t cap_chunk, Pipe &P)
+{
+    int rc;
+    if ((rc = ws->get(swm_ws::SURV, &P.pio.surv, cap_pos + 1)) ||
+        (rc = ws->get(swm_ws::BDESC, &P.pio.bdesc, cap_chunk + 1)) ||
+        (rc = ws->get(swm_ws::BIG, &P.pio.big, cap_chunk + 1)))
+        return rc;
+    return SWM_OK;
+}
+
+static int pipe_alloc_c(swm_ws *ws, uint64_t cap, Pipe &P)
+{
+    int rc;
+    if ((rc = ws->get(swm_ws::C_VAL, &P.cs.val, cap + 1)) || (rc = ws->get(swm_ws::C_RS, &P.cs.rs, cap + 1)) ||
+        (rc = ws->get(swm_ws::C_BI, &P.cs.bi, cap + 1)) || (rc = ws->get(swm_ws::C_ERR, &P.cs.err, cap + 1)) ||
+        (rc = ws->get(swm_ws::C_LOC, &P.cs.loc, cap + 1)) || (rc = ws->get(swm_ws::C_REC, &P.cs.rec, cap + 1)) ||
+        (rc = ws->get(swm_ws::C_INFO, &P.cs.info, cap + 1)) || (rc = ws->get(swm_ws::TRACE, &P.trace, cap + 1)))
+        return rc;
+    return SWM_OK;
+}
+
+template <typename T>
+static void scan_launch(const T *in, T *out, uint64_t n, T *tmp, hipStream_t st)
+{
+    const uint32_t nb = blocks_for(n, SCAN_TILE);
+    k_scan_reduce<T><<<nb, SCAN_NT, 0, st>>>(in, n, tmp);
+    k_scan_top<T><<<1, 1024, 0, st>>>(tmp, nb);
+    k_scan_down<T><<<nb, SCAN_NT, 0, st>>>(in, out, n, tmp);
+}
+
+/* Stage A: counters, seeds, offsets of the positions and chunks. */
+static int enq_a(const swm_ctx_t *ctx, Pipe &P, const char *d_reads, uint64_t stride, const uint32_t *d_lens,
+                 uint32_t e, hipStream_t st, hipEvent_t *ev)
+{
+    if (ev) SWM_HIP_CHECK(hipEventRecord(ev[0], st));
+    k_reset<<<1, 64, 0, st>>>(P.counters, P.so.npos + P.nrs, P.so.nchunk + P.nrs, P.pio.nuniq + P.nrs);
+    k_seed<<<blocks_for(P.n, SEED_READS), 2 * SEED_READS, 0, st>>>(d_reads, stride, d_lens, (uint32_t)P.n, ctx->k, e,
+                                                                   P.seln, ctx->d_hash_offset, ctx->d_recipe, P.so);
+    SWM_HIP_CHECK(hipGetLastError());
+    if (ev) SWM_HIP_CHECK(hipEventRecord(ev[1], st));
+    scan_launch<uint64_t>(P.so.npos, P.surv_off, P.nrs + 1, P.scan_tmp, st);
+    scan_launch<uint32_t>(P.so.nchunk, P.bd_off, P.nrs + 1, reinterpret_cast<uint32_t *>(P.scan_tmp), st);
+    SWM_HIP_CHECK(hipGetLastError());
+    return SWM_OK;
+}
+
+/* Stage B: capacity check, batch plans, large-batch sort, candidate offsets. */
+static int enq_b(const swm_ctx_t *ctx, Pipe &P, uint64_t cap_pos, uint64_t cap_chunk, hipStream_t st, hipEvent_t *ev)
+{
+    const swm_opts &o = ctx->opt;
+    k_check_plan<<<1, 1, 0, st>>>(P.surv_off + P.nrs, P.bd_off + P.nrs, cap_pos, cap_chunk, P.counters, P.u32c + 7);
+    if (o.work_queue) {
+        k_plan<<<blocks_for(P.nrs, 64), 64, 0, st>>>((uint32_t)P.nrs, ctx->d_pl, P.pio);
+        k_plan_heavy<<<ctx->n_cu * o.plan_heavy_waves, 64, 0, st>>>(ctx->d_pl, P.pio);
+    } else {
+        k_plan_plain<<<blocks_for(P.nrs, 64), 64, 0, st>>>((uint32_t)P.nrs, ctx->d_pl, P.pio);
+    }
+    k_bigsort<<<1024, 256, 0, st>>>(P.pio.big, P.pio.big_count, P.surv_off, P.bd_off, P.pio.surv, P.pio.bdesc,
+                                    P.pio.nuniq);
+    SWM_HIP_CHECK(hipGetLastError());
+    if (ev) SWM_HIP_CHECK(hipEventRecord(ev[2], st));
+    scan_launch<uint32_t>(P.pio.nuniq, P.cand_off, P.nrs + 1, reinterpret_cast<uint32_t *>(P.scan_tmp), st);
+    SWM_HIP_CHECK(hipGetLastError());
+    return SWM_OK;
+}
+
+/* Stage C: candidate list, verification, records, per-read results.
+ * grid_cand sizes the candidate kernels (their count is read on the device). */
+static int enq_c(const swm_ctx_t *ctx, Pipe &P, const uint32_t *d_lens, uint32_t e, swm_read_result_t *d_out,
+                 uint64_t cap_cand, uint64_t grid_cand, hipStream_t st, hipEvent_t *ev)
+{
+    const swm_opts &o = ctx->opt;
+    const uint32_t heavy_cands = o.work_queue ? o.heavy_cands : 0xffffffffu;
+    const uint32_t *ncand = P.cand_off + P.nrs;
+    k_check_cand<<<1, 1, 0, st>>>(ncand, cap_cand, P.counters, P.u32c + 7);
+    k_compact<<<blocks_for(P.nrs, 128), 128, 0, st>>>((uint32_t)P.nrs, P.surv_off, P.pio.surv, P.bd_off, P.pio.bdesc,
+                                                      P.pio.nbatch, P.cand_off, P.cs, heavy_cands, P.heavy_c,
+                                                      P.u32c + 5);
+    if (o.work_queue)
+        k_compact_heavy<<<ctx->n_cu * 4, 256, 0, st>>>(P.heavy_c, P.u32c + 5, P.surv_off, P.pio.surv, P.bd_off,
+                                                       P.pio.bdesc, P.pio.nbatch, P.cand_off, P.cs);
+    SWM_HIP_CHECK(hipGetLastError());
+    if (ev) SWM_HIP_CHECK(hipEventRecord(ev[3], st));
+    const uint32_t mb = o.myers_block;
+    k_myers<<<blocks_for(grid_cand ? grid_cand : 1, mb), mb, 0, st>>>(ncand, d_lens, P.so.words, ctx->d_base_2bits,
+                                                                      ctx->ref_length, e, P.cs);
+    SWM_HIP_CHECK(hipGetLastError());
+    if (ev) SWM_HIP_CHECK(hipEventRecord(ev[4], st));
+    k_hamming<<<blocks_for(grid_cand ? grid_cand : 1, 256), 256, 0, st>>>(ncand, d_lens, P.so.words,
+                                                                          ctx->d_base_2bits, ctx->ref_length, e, P.cs,
+                                                                          P.trace, P.u32c + 1, P.u32c + 2,
+                                                                          P.counters + 2);
+    k_trace<<<1024, 64, 0, st>>>(P.trace, P.u32c + 1, d_lens, P.so.words, ctx->d_base_2bits, ctx->ref_length, e, P.cs);
+    k_emit<<<blocks_for(P.n, 128), 128, 0, st>>>((uint32_t)P.n, e, P.cand_off, P.cs, d_out, P.counters + 1,
+                                                 heavy_cands, P.heavy_e, P.u32c + 6);
+    if (o.work_queue)
+        k_emit_heavy<<<ctx->n_cu * o.emit_heavy_waves, 64, 0, st>>>(P.heavy_e, P.u32c + 6, e, P.cand_off, P.cs, d_out,
+                                                                   P.counters + 1);
+    SWM_HIP_CHECK(hipGetLastError());
+    if (ev) SWM_HIP_CHECK(hipEventRecord(ev[5], st));
+    return SWM_OK;
+}
+
+static void stats_from_counters(swm_stats_t *s, uint64_t n, const unsigned long long *hc)
+{
+    const uint32_t *u = reinterpret_cast<const uint32_t *>(hc + 4);
+    s->n_reads = n;
+    s->n_verifications = hc[0] - u[2];
+    s->n_band_steps = hc[2];
+    s->n_records = hc[1];
+}
+
+static void observe(swm_slot *sl, uint64_t n, uint64_t tot_pos, uint64_t tot_chunk, uint64_t ncand)
+{
+    if (!n) return;
+    const double dn = (double)n;
+    if (tot_pos / dn > sl->r_pos) sl->r_pos = tot_pos / dn;
+    if (tot_chunk / dn > sl->r_chunk) sl->r_chunk = tot_chunk / dn;
+    if (ncand / dn > sl->r_cand) sl->r_cand = ncand / dn;
+}
+
+/* One launch per stage, every buffer sized from the totals read back after
+ * stages A and B (two host round trips). */
+/* h_out (page-locked, may be NULL): the first n_out results are copied there
+ * before the run's last synchronisation. */
+static int run_synced(swm_ctx_t *ctx, swm_slot *sl, const char *d_reads, uint64_t stride, const uint32_t *d_lens,
+                      uint64_t n, uint32_t e, swm_read_result_t *d_out, swm_read_result_t *h_out, uint64_t n_out)
+{
+    hipStream_t st = sl->stream;
+    swm_ws *ws = &sl->ws;
+    memset(&sl->stats, 0, sizeof(sl->stats));
+    sl->stats.n_reads = n;
+    if (n == 0) return SWM_OK;
+    const uint32_t seln = 4 * ((e + 1) < 16 ? (e + 1) : 16);
+    Pipe P;
+    int rc;
+    if ((rc = pipe_alloc_a(ws, n, seln, P))) return rc;
+    if ((rc = enq_a(ctx, P, d_reads, stride, d_lens, e, st, sl->ev))) return rc;
+    uint64_t *hs = sl->h_small;
+    hs[9] = 0;
+    SWM_HIP_CHECK(hipMemcpyAsync(hs + 8, P.surv_off + P.nrs, sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+    SWM_HIP_CHECK(hipMemcpyAsync(hs + 9, P.bd_off + P.nrs, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    SWM_HIP_CHECK(hipStreamSynchronize(st));
+    const uint64_t tot_pos = hs[8], tot_chunk = (uint32_t)hs[9];
+    if ((rc = pipe_alloc_b(ws, tot_pos, tot_chunk, P))) return rc;
+    if ((rc = enq_b(ctx, P, tot_pos, tot_chunk, st, sl->ev))) return rc;
+    hs[10] = 0;
+    SWM_HIP_CHECK(hipMemcpyAsync(hs + 10, P.cand_off + P.nrs, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    SWM_HIP_CHECK(hipStreamSynchronize(st));
+    const uint32_t ncand = (uint32_t)hs[10];
+    if ((rc = pipe_alloc_c(ws, ncand, P))) return rc;
+    if ((rc = enq_c(ctx, P, d_lens, e, d_out, ncand, ncand, st, sl->ev))) return rc;
+    SWM_HIP_CHECK(hipMemcpyAsync(hs, P.counters, 8 * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+    if (h_out && n_out)
+        SWM_HIP_CHECK(hipMemcpyAsync(h_out, d_out, n_out * sizeof(swm_read_result_t), hipMemcpyDeviceToHost, st));
+    SWM_HIP_CHECK(hipStreamSynchronize(st));
+    stats_from_counters(&sl->stats, n, reinterpret_cast<const unsigned long long *>(hs));
+    observe(sl, n, tot_pos, tot_chunk, ncand);
+    float ms;
+    (void)hipEventElapsedTime(&ms, sl->ev[0], sl->ev[1]); sl->stats.ms_seed = ms;
+    (void)hipEventElapsedTime(&ms, sl->ev[1], sl->ev[2]); sl->stats.ms_plan = ms;
+    (void)hipEventElapsedTime(&ms, sl->ev[2], sl->ev[3]); sl->stats.ms_compact = ms;
+    (void)hipEventElapsedTime(&ms, sl->ev[3], sl->ev[4]); sl->stats.ms_myers = ms;
+    (void)hipEventElapsedTime(&ms, sl->ev[4], sl->ev[5]); sl->stats.ms_emit = ms;
+    (void)hipEventElapsedTime(&ms, sl->ev[0], sl->ev[5]); sl->stats.ms_total = ms;
+    return SWM_OK;
+}
+
+static uint64_t opt_sig(const swm_opts &o)
+{
+    return (uint64_t)o.work_queue | ((uint64_t)o.heavy_cands << 1) | ((uint64_t)o.myers_block << 33) |
+           ((uint64_t)o.plan_heavy_waves << 43) | ((uint64_t)o.emit_heavy_waves << 53);
+}
+
+/* Fixed-capacity form: the whole lookup -> filter -> verify -> emit pipeline
+ * captured once per (buffers, n, e) into a hipGraph and replayed with no host
+ * round trip.  Capacities come from the largest per-read totals seen so far
+ * with 25% headroom; a batch that would overflow them stands down on the
+ * device and is run again in the synced form, which also raises them. */
+static int run_graph(swm_ctx_t *ctx, swm_slot *sl, const char *d_reads, uint64_t stride, const uint32_t *d_lens,
+                     uint64_t n, uint32_t e, swm_read_result_t *d_out, swm_read_result_t *h_out, uint64_t n_out)
+{
+    if (n == 0 || sl->r_cand == 0) return run_synced(ctx, sl, d_reads, stride, d_lens, n, e, d_out, h_out, n_out);
+    hipStream_t st = sl->stream;
+    const uint64_t sig = opt_sig(ctx->opt);
+    swm_slot::GraphEnt *ent = nullptr;
+    for (auto &x : sl->g)
+        if (x.used && x.reads == d_reads && x.lens == d_lens && x.out == d_out && x.n == n && x.stride == stride &&
+            x.e == e && x.sig == sig)
+            ent = &x;
+    const uint32_t seln = 4 * ((e + 1) < 16 ? (e + 1) : 16);
+    Pipe P;
+    int rc;
+    if (!ent || ent->gen != sl->ws.gen) {
+        if (!ent) {
+            ent = &sl->g[0];
+            for (auto &x : sl->g) {
+                if (!x.used) { ent = &x; break; }
+                if (x.tick < ent->tick) ent = &x;
+            }
+        }
+        if (ent->exec) { (void)hipGraphExecDestroy(ent->exec); ent->exec = nullptr; }
+        ent->used = false;
+        const double dn = (double)n;
+        const uint64_t cap_pos = (uint64_t)(dn * sl->r_pos * 1.25) + 4096;
+        const uint64_t cap_chunk = (uint64_t)(dn * sl->r_chunk * 1.25) + 1024;
+        const uint64_t cap_cand = (uint64_t)(dn * sl->r_cand * 1.25) + 4096;
+        if ((rc = pipe_alloc_a(&sl->ws, n, seln, P)) || (rc = pipe_alloc_b(&sl->ws, cap_pos, cap_chunk, P)) ||
+            (rc = pipe_alloc_c(&sl->ws, cap_cand, P)))
+            return rc;
+        hipGraph_t graph = nullptr;
+        SWM_HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        int crc = enq_a(ctx, P, d_reads, stride, d_lens, e, st, nullptr);
+        if (!crc) crc = enq_b(ctx, P, cap_pos, cap_chunk, st, nullptr);
+        if (!crc) crc = enq_c(ctx, P, d_lens, e, d_out, cap_cand, cap_cand, st, nullptr);
+        const hipError_t ce = hipStreamEndCapture(st, &graph);
+        if (crc || ce != hipSuccess) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return crc ? crc : SWM_EDEVICE;
+        }
+        const hipError_t ie = hipGraphInstantiate(&ent->exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (ie != hipSuccess) { ent->exec = nullptr; return SWM_EDEVICE; }
+        ent->used = true;
+        ent->reads = d_reads; ent->lens = d_lens; ent->out = d_out;
+        ent->n = n; ent->stride = stride; ent->e = e; ent->sig = sig; ent->gen = sl->ws.gen;
+    } else if ((rc = pipe_alloc_a(&sl->ws, n, seln, P))) {
+        return rc;   /* no growth here (the generation matched): only the counters' address is needed */
+    }
+    ent->tick = ++sl->tick;
+    memset(&sl->stats, 0, sizeof(sl->stats));
+    SWM_HIP_CHECK(hipEventRecord(sl->ev[0], st));
+    SWM_HIP_CHECK(hipGraphLaunch(ent->exec, st));
+    SWM_HIP_CHECK(hipEventRecord(sl->ev[5], st));
+    const unsigned long long *hc = reinterpret_cast<const unsigned long long *>(sl->h_small);
+    SWM_HIP_CHECK(hipMemcpyAsync(sl->h_small, P.counters, 8 * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+    if (h_out && n_out)
+        SWM_HIP_CHECK(hipMemcpyAsync(h_out, d_out, n_out * sizeof(swm_read_result_t), hipMemcpyDeviceToHost, st));
+    SWM_HIP_CHECK(hipStreamSynchronize(st));
+    if (reinterpret_cast<const uint32_t *>(hc + 4)[7]) {
+        /* over capacity: nothing was written; the synced form maps the batch
+         * and records the larger totals, and the graph is captured again */
+        (void)hipGraphExecDestroy(ent->exec);
+        ent->exec = nullptr;
+        ent->used = false;
+        return run_synced(ctx, sl, d_reads, stride, d_lens, n, e, d_out, h_out, n_out);
+    }
+    stats_from_counters(&sl->stats, n, hc);
+    float ms;
+    (void)hipEventElapsedTime(&ms, sl->ev[0], sl->ev[5]);
+    sl->stats.ms_total = ms;
+    return SWM_OK;
+}
+
+int swm_slot_run_device(swm_ctx_t *ctx, swm_slot *sl, const char *d_reads, uint64_t stride, const uint32_t *d_lens,
+                        uint64_t n, uint32_t e, swm_read_result_t *d_out, bool graph)
+{
+    if (2 * n >= 0xfffffff0ull) return SWM_EINVAL;
+    return graph ? run_graph(ctx, sl, d_reads, stride, d_lens, n, e, d_out, nullptr, 0)
+                 : run_synced(ctx, sl, d_reads, stride, d_lens, n, e, d_out, nullptr, 0);
+}
+
+static bool is_pinned(const void *p)
+{
+    hipPointerAttribute_t a;
+    if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+        (void)hipGetLastError();   /* plain pageable memory: clear the status */
+        return false;
+    }
+    return a.type == hipMemoryTypeHost;
+}
+
+/* Host buffers in and out through the slot's own device buffers; reads
+ * n .. n_pad - 1 are given length 0 (they map to nothing), so batches of
+ * nearby sizes share one captured graph.  Page-locked input is copied by DMA
+ * straight from the caller's memory, pageable input through the slot's
+ * page-locked staging; results come back through that staging. */
+int swm_slot_run_host(swm_ctx_t *ctx, swm_slot *sl, const char *reads, uint64_t stride, const uint32_t *lens,
+                      uint64_t n, uint64_t n_pad, uint32_t e, swm_read_result_t *out, bool graph)
+{
+    if (n_pad < n) n_pad = n;
+    if (n == 0) { memset(&sl->stats, 0, sizeof(sl->stats)); return SWM_OK; }
+    if (2 * n_pad >= 0xfffffff0ull) return SWM_EINVAL;
+    char *d_reads;
+    uint32_t *d_lens;
+    swm_read_result_t *d_out;
+    int rc;
+    if ((rc = sl->ws.get(swm_ws::IN_READS, &d_reads, n_pad * stride + 4)) ||
+        (rc = sl->ws.get(swm_ws::IN_LENS, &d_lens, n_pad)) || (rc = sl->ws.get(swm_ws::OUT, &d_out, n_pad)) ||
+        (rc = sl->pin(sl->h_out, n * sizeof(swm_read_result_t))))
+        return rc;
+    hipStream_t st = sl->stream;
+    const char *src_r = reads;
+    const uint32_t *src_l = lens;
+    if (!is_pinned(reads)) {
+        if ((rc = sl->pin(sl->h_reads, n * stride))) return rc;
+        memcpy(sl->h_reads.p, reads, n * stride);
+        src_r = (const char *)sl->h_reads.p;
+    }
+    if (!is_pinned(lens)) {
+        if ((rc = sl->pin(sl->h_lens, n * 4))) return rc;
+        memcpy(sl->h_lens.p, lens, n * 4);
+        src_l = (const uint32_t *)sl->h_lens.p;
+    }
+    SWM_HIP_CHECK(hipMemcpyAsync(d_reads, src_r, n * stride, hipMemcpyHostToDevice, st));
+    SWM_HIP_CHECK(hipMemcpyAsync(d_lens, src_l, n * 4, hipMemcpyHostToDevice, st));
+    if (n_pad > n) SWM_HIP_CHECK(hipMemsetAsync(d_lens + n, 0, (n_pad - n) * 4, st));
+    swm_read_result_t *h_out = (swm_read_result_t *)sl->h_out.p;
+    rc = graph ? run_graph(ctx, sl, d_reads, stride, d_lens, n_pad, e, d_out, h_out, n)
+               : run_synced(ctx, sl, d_reads, stride, d_lens, n_pad, e, d_out, h_out, n);
     if (rc) return rc;
-    return hipcub::DeviceScan::ExclusiveSum(t, tmp, in, out, n, st) == hipSuccess ? SWM_OK : SWM_EDEVICE;
+    sl->stats.n_reads = n;
+    memcpy(out, h_out, n * sizeof(swm_read_result_t));
+    return SWM_OK;
 }
 
 /* The device keeps each bucket entry as one 8-byte (position, residual)
@@ -1281,14 +2084,11 @@ extern "C" int swm_ctx_create(uint32_t k, const uint32_t *hash_count, const uint
     *out = nullptr;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return SWM_EDEVICE;
-    swm_ctx_t *c = (swm_ctx_t *)calloc(1, sizeof(swm_ctx_t));
+    swm_ctx_t *c = new (std::nothrow) swm_ctx_t();
     if (!c) return SWM_ENOMEM;
-    if (hipGetDevice(&c->device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess) {
-        free(c);
-        return SWM_EDEVICE;
-    }
-    c->ws = new (std::nothrow) swm_ws();
-    if (!c->ws) { swm_ctx_destroy(c); return SWM_ENOMEM; }
+    if (hipGetDevice(&c->device) != hipSuccess) { delete c; return SWM_EDEVICE; }
+    c->slot0 = swm_slot_create(c);
+    if (!c->slot0) { swm_ctx_destroy(c); return SWM_EDEVICE; }
     c->k = k;
     c->n_cu = 256;
     (void)hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, c->device);
@@ -1334,139 +2134,53 @@ extern "C" int swm_ctx_create(uint32_t k, const uint32_t *hash_count, const uint
 extern "C" void swm_ctx_destroy(swm_ctx_t *c)
 {
     if (!c) return;
-    if (c->stream) (void)hipStreamSynchronize(c->stream);
-    delete c->ws;
+    swm_slot_destroy(c->slot0);
     if (c->d_hash_offset) (void)hipFree(c->d_hash_offset);
     if (c->d_pl) (void)hipFree(c->d_pl);
     if (c->d_base_2bits) (void)hipFree(c->d_base_2bits);
     if (c->d_recipe) (void)hipFree(c->d_recipe);
-    if (c->stream) (void)hipStreamDestroy(c->stream);
-    free(c);
+    delete c;
 }
 
 extern "C" int swm_last_stats(const swm_ctx_t *ctx, swm_stats_t *st)
 {
     if (!ctx || !st) return SWM_EINVAL;
-    *st = ctx->stats;
+    *st = ctx->slot0->stats;
     return SWM_OK;
 }
 
-static int map_device_impl(swm_ctx_t *ctx, const char *d_reads, uint64_t stride, const uint32_t *d_lens,
-                           uint64_t n, uint32_t e, swm_read_result_t *d_out)
+extern "C" int swm_ctx_set_option(swm_ctx_t *ctx, int opt, int64_t value)
 {
-    hipStream_t st = ctx->stream;
-    swm_ws *ws = ctx->ws;
-    memset(&ctx->stats, 0, sizeof(ctx->stats));
-    ctx->stats.n_reads = n;
-    if (n == 0) return SWM_OK;
-    const uint64_t nrs = 2 * n;
-    if (nrs >= 0xfffffff0ull) return SWM_EINVAL;
-    const uint32_t seln = 4 * ((e + 1) < 16 ? (e + 1) : 16);
-    int rc;
-    hipEvent_t ev[7];
-    for (int i = 0; i < 7; i++) SWM_HIP_CHECK(hipEventCreate(&ev[i]));
-    struct EvGuard { hipEvent_t *e; ~EvGuard() { for (int i = 0; i < 7; i++) (void)hipEventDestroy(e[i]); } } eg{ev};
-
-    SeedOut so;
-    unsigned long long *counters;
-    if ((rc = ws->get(swm_ws::WORDS, &so.words, nrs * 32)) || (rc = ws->get(swm_ws::SEL, &so.sel, nrs * seln)) ||
-        (rc = ws->get(swm_ws::NSEL, &so.nsel, nrs)) || (rc = ws->get(swm_ws::NPOS, &so.npos, nrs + 1)) ||
-        (rc = ws->get(swm_ws::NCHUNK, &so.nchunk, nrs + 1)) || (rc = ws->get(swm_ws::COUNTERS, &counters, 8)))
-        return rc;
-    /* [0] big batches, [1] tracebacks, [2] rejected windows, [3] heavy plan items, [4] their queue head,
-     * [5] heavy compaction items, [6] heavy emission items */
-    uint32_t *u32c = reinterpret_cast<uint32_t *>(counters + 4);
-    SWM_HIP_CHECK(hipMemsetAsync(counters, 0, 8 * sizeof(unsigned long long), st));   /* 32 B of u64, 16 u32 */
-    SWM_HIP_CHECK(hipMemsetAsync(so.npos + nrs, 0, sizeof(uint64_t), st));
-    SWM_HIP_CHECK(hipMemsetAsync(so.nchunk + nrs, 0, sizeof(uint32_t), st));
-    SWM_HIP_CHECK(hipEventRecord(ev[0], st));
-    k_seed<<<blocks_for(n, SEED_READS), 2 * SEED_READS, 0, st>>>(d_reads, stride, d_lens, (uint32_t)n, ctx->k, e, seln,
-                                                                 ctx->d_hash_offset, ctx->d_recipe, so);
-    SWM_HIP_CHECK(hipGetLastError());
-    SWM_HIP_CHECK(hipEventRecord(ev[1], st));
-
-    PlanIO pio;
-    uint64_t *surv_off;
-    uint32_t *bd_off;
-    if ((rc = ws->get(swm_ws::SURV_OFF, &surv_off, nrs + 1)) || (rc = ws->get(swm_ws::BD_OFF, &bd_off, nrs + 1)))
-        return rc;
-    if ((rc = ws_scan_u64(ws, so.npos, surv_off, nrs + 1, st)) || (rc = ws_scan_u32(ws, so.nchunk, bd_off, nrs + 1, st)))
-        return rc;
-    uint64_t tot_pos = 0;
-    uint32_t tot_chunk = 0;
-    SWM_HIP_CHECK(hipMemcpyAsync(&tot_pos, surv_off + nrs, sizeof(uint64_t), hipMemcpyDeviceToHost, st));
-    SWM_HIP_CHECK(hipMemcpyAsync(&tot_chunk, bd_off + nrs, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-    SWM_HIP_CHECK(hipStreamSynchronize(st));
-    pio.sel = so.sel;
-    pio.seln = seln;
-    pio.nsel = so.nsel;
-    pio.surv_off = surv_off;
-    pio.bd_off = bd_off;
-    if ((rc = ws->get(swm_ws::SURV, &pio.surv, tot_pos + 1)) || (rc = ws->get(swm_ws::BDESC, &pio.bdesc, (uint64_t)tot_chunk + 1)) ||
-        (rc = ws->get(swm_ws::NBATCH, &pio.nbatch, nrs)) || (rc = ws->get(swm_ws::NUNIQ, &pio.nuniq, nrs + 1)) ||
-        (rc = ws->get(swm_ws::BIG, &pio.big, (uint64_t)tot_chunk + 1)))
-        return rc;
-    pio.big_count = u32c;
-    if ((rc = ws->get(swm_ws::HEAVY, &pio.heavy, nrs))) return rc;
-    pio.heavy_count = u32c + 3;
-    pio.heavy_head = u32c + 4;
-    SWM_HIP_CHECK(hipMemsetAsync(pio.nuniq + nrs, 0, sizeof(uint32_t), st));
-    k_plan<<<blocks_for(nrs, 64), 64, 0, st>>>((uint32_t)nrs, ctx->d_pl, pio);
-    SWM_HIP_CHECK(hipGetLastError());
-    k_plan_heavy<<<ctx->n_cu * 8, 64, 0, st>>>(ctx->d_pl, pio);
-    SWM_HIP_CHECK(hipGetLastError());
-    k_bigsort<<<1024, 256, 0, st>>>(pio.big, pio.big_count, surv_off, bd_off, pio.surv, pio.bdesc, pio.nuniq);
-    SWM_HIP_CHECK(hipGetLastError());
-    SWM_HIP_CHECK(hipEventRecord(ev[2], st));
-
-    uint32_t *cand_off;
-    if ((rc = ws->get(swm_ws::CAND_OFF, &cand_off, nrs + 1))) return rc;
-    if ((rc = ws_scan_u32(ws, pio.nuniq, cand_off, nrs + 1, st))) return rc;
-    uint32_t ncand = 0;
-    SWM_HIP_CHECK(hipMemcpyAsync(&ncand, cand_off + nrs, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-    SWM_HIP_CHECK(hipStreamSynchronize(st));
-    CandSet cs;
-    uint32_t *trace;
-    if ((rc = ws->get(swm_ws::C_VAL, &cs.val, (uint64_t)ncand + 1)) || (rc = ws->get(swm_ws::C_RS, &cs.rs, (uint64_t)ncand + 1)) ||
-        (rc = ws->get(swm_ws::C_BI, &cs.bi, (uint64_t)ncand + 1)) || (rc = ws->get(swm_ws::C_ERR, &cs.err, (uint64_t)ncand + 1)) ||
-        (rc = ws->get(swm_ws::C_LOC, &cs.loc, (uint64_t)ncand + 1)) || (rc = ws->get(swm_ws::C_REC, &cs.rec, (uint64_t)ncand + 1)) ||
-        (rc = ws->get(swm_ws::C_INFO, &cs.info, (uint64_t)ncand + 1)) || (rc = ws->get(swm_ws::TRACE, &trace, (uint64_t)ncand + 1)))
-        return rc;
-    uint32_t *heavy_c, *heavy_e;
-    if ((rc = ws->get(swm_ws::HEAVY_C, &heavy_c, nrs)) || (rc = ws->get(swm_ws::HEAVY_E, &heavy_e, n))) return rc;
-    k_compact<<<blocks_for(nrs, 128), 128, 0, st>>>((uint32_t)nrs, surv_off, pio.surv, bd_off, pio.bdesc, pio.nbatch,
-                                                    cand_off, cs, heavy_c, u32c + 5);
-    k_compact_heavy<<<ctx->n_cu * 4, 256, 0, st>>>(heavy_c, u32c + 5, surv_off, pio.surv, bd_off, pio.bdesc, pio.nbatch,
-                                                   cand_off, cs);
-    SWM_HIP_CHECK(hipGetLastError());
-    SWM_HIP_CHECK(hipEventRecord(ev[3], st));
-    if (ncand)
-        k_myers<<<blocks_for(ncand, 256), 256, 0, st>>>(ncand, d_lens, so.words, ctx->d_base_2bits, ctx->ref_length, e,
-                                                         cs);
-    SWM_HIP_CHECK(hipGetLastError());
-    SWM_HIP_CHECK(hipEventRecord(ev[4], st));
-    if (ncand) {
-        k_hamming<<<blocks_for(ncand, 256), 256, 0, st>>>(ncand, d_lens, so.words, ctx->d_base_2bits, ctx->ref_length,
-                                                           e, cs, trace, u32c + 1, u32c + 2, counters + 2);
-        k_trace<<<1024, 64, 0, st>>>(trace, u32c + 1, d_lens, so.words, ctx->d_base_2bits, ctx->ref_length, e, cs);
+    if (!ctx) return SWM_EINVAL;
+    swm_opts &o = ctx->opt;
+    switch (opt) {
+    case SWM_OPT_WORK_QUEUE: if (value != 0 && value != 1) return SWM_EINVAL; o.work_queue = (uint32_t)value; break;
+    case SWM_OPT_GRAPH: if (value != 0 && value != 1) return SWM_EINVAL; o.graph = (uint32_t)value; break;
+    case SWM_OPT_HEAVY_CANDS: if (value < 1 || value > 65535) return SWM_EINVAL; o.heavy_cands = (uint32_t)value; break;
+    case SWM_OPT_MYERS_BLOCK:
+        if (value != 64 && value != 128 && value != 256) return SWM_EINVAL;
+        o.myers_block = (uint32_t)value;
+        break;
+    case SWM_OPT_PLAN_HEAVY_WAVES: if (value < 1 || value > 32) return SWM_EINVAL; o.plan_heavy_waves = (uint32_t)value; break;
+    case SWM_OPT_EMIT_HEAVY_WAVES: if (value < 1 || value > 64) return SWM_EINVAL; o.emit_heavy_waves = (uint32_t)value; break;
+    default: return SWM_EINVAL;
     }
-    k_emit<<<blocks_for(n, 128), 128, 0, st>>>((uint32_t)n, e, cand_off, cs, d_out, counters + 1, heavy_e, u32c + 6);
-    k_emit_heavy<<<ctx->n_cu * 32, 64, 0, st>>>(heavy_e, u32c + 6, e, cand_off, cs, d_out, counters + 1);
-    SWM_HIP_CHECK(hipGetLastError());
-    SWM_HIP_CHECK(hipEventRecord(ev[5], st));
-    unsigned long long hc[8];
-    SWM_HIP_CHECK(hipMemcpyAsync(hc, counters, sizeof(hc), hipMemcpyDeviceToHost, st));
-    SWM_HIP_CHECK(hipStreamSynchronize(st));
-    float ms;
-    ctx->stats.n_verifications = ncand - reinterpret_cast<const uint32_t *>(hc + 4)[2];
-    ctx->stats.n_band_steps = hc[2];
-    ctx->stats.n_records = hc[1];
-    (void)hipEventElapsedTime(&ms, ev[0], ev[1]); ctx->stats.ms_seed = ms;
-    (void)hipEventElapsedTime(&ms, ev[1], ev[2]); ctx->stats.ms_plan = ms;
-    (void)hipEventElapsedTime(&ms, ev[2], ev[3]); ctx->stats.ms_compact = ms;
-    (void)hipEventElapsedTime(&ms, ev[3], ev[4]); ctx->stats.ms_myers = ms;
-    (void)hipEventElapsedTime(&ms, ev[4], ev[5]); ctx->stats.ms_emit = ms;
-    (void)hipEventElapsedTime(&ms, ev[0], ev[5]); ctx->stats.ms_total = ms;
+    return SWM_OK;
+}
+
+extern "C" int swm_ctx_get_option(const swm_ctx_t *ctx, int opt, int64_t *value)
+{
+    if (!ctx || !value) return SWM_EINVAL;
+    const swm_opts &o = ctx->opt;
+    switch (opt) {
+    case SWM_OPT_WORK_QUEUE: *value = o.work_queue; break;
+    case SWM_OPT_GRAPH: *value = o.graph; break;
+    case SWM_OPT_HEAVY_CANDS: *value = o.heavy_cands; break;
+    case SWM_OPT_MYERS_BLOCK: *value = o.myers_block; break;
+    case SWM_OPT_PLAN_HEAVY_WAVES: *value = o.plan_heavy_waves; break;
+    case SWM_OPT_EMIT_HEAVY_WAVES: *value = o.emit_heavy_waves; break;
+    default: return SWM_EINVAL;
+    }
     return SWM_OK;
 }
 
@@ -1478,6 +2192,15 @@ static int check_map_args(const swm_ctx_t *ctx, uint64_t stride, uint32_t e)
     return SWM_OK;
 }
 
+int swm_check_lens(const swm_ctx_t *ctx, uint64_t stride, const uint32_t *lens, uint64_t n)
+{
+    for (uint64_t i = 0; i < n; i++) {
+        if ((uint64_t)lens[i] + 2 > stride) return SWM_EINVAL;                 /* kseq stride rule (Q20) */
+        if (lens[i] / (ctx->k + SWM_SEED_STEP - 1) > 16) return SWM_EINVAL;   /* >16 long seeds */
+    }
+    return SWM_OK;
+}
+
 extern "C" int swm_map_batch_device(swm_ctx_t *ctx, const char *d_reads, uint64_t stride, const uint32_t *d_lens,
                                     uint64_t n, uint32_t e, swm_read_result_t *d_out)
 {
@@ -1485,7 +2208,7 @@ extern "C" int swm_map_batch_device(swm_ctx_t *ctx, const char *d_reads, uint64_
     if (rc) return rc;
     if (n && (!d_reads || !d_lens || !d_out)) return SWM_EINVAL;
     if (hipSetDevice(ctx->device) != hipSuccess) return SWM_EDEVICE;
-    return map_device_impl(ctx, d_reads, stride, d_lens, n, e, d_out);
+    return swm_slot_run_device(ctx, ctx->slot0, d_reads, stride, d_lens, n, e, d_out, ctx->opt.graph != 0);
 }
 
 extern "C" int swm_map_batch(swm_ctx_t *ctx, const char *reads, uint64_t stride, const uint32_t *lens, uint64_t n,
@@ -1493,33 +2216,11 @@ extern "C" int swm_map_batch(swm_ctx_t *ctx, const char *reads, uint64_t stride,
 {
     int rc = check_map_args(ctx, stride, e);
     if (rc) return rc;
-    if (n == 0) return SWM_OK;
+    if (n == 0) { memset(&ctx->slot0->stats, 0, sizeof(swm_stats_t)); return SWM_OK; }
     if (!reads || !lens || !out) return SWM_EINVAL;
-    for (uint64_t i = 0; i < n; i++) {
-        if ((uint64_t)lens[i] + 2 > stride) return SWM_EINVAL;   /* kseq stride rule (Q20) */
-        if (lens[i] / (ctx->k + SWM_SEED_STEP - 1) > 16) return SWM_EINVAL;   /* >16 long seeds */
-    }
+    if ((rc = swm_check_lens(ctx, stride, lens, n))) return rc;
     if (hipSetDevice(ctx->device) != hipSuccess) return SWM_EDEVICE;
-    char *d_reads = nullptr;
-    uint32_t *d_lens = nullptr;
-    swm_read_result_t *d_out = nullptr;
-    rc = SWM_OK;
-    if (hipMalloc((void **)&d_reads, n * stride) != hipSuccess || hipMalloc((void **)&d_lens, n * 4) != hipSuccess ||
-        hipMalloc((void **)&d_out, n * sizeof(swm_read_result_t)) != hipSuccess)
-        rc = SWM_ENOMEM;
-    if (rc == SWM_OK &&
-        (hipMemcpyAsync(d_reads, reads, n * stride, hipMemcpyHostToDevice, ctx->stream) != hipSuccess ||
-         hipMemcpyAsync(d_lens, lens, n * 4, hipMemcpyHostToDevice, ctx->stream) != hipSuccess))
-        rc = SWM_EDEVICE;
-    if (rc == SWM_OK) rc = map_device_impl(ctx, d_reads, stride, d_lens, n, e, d_out);
-    if (rc == SWM_OK && (hipMemcpyAsync(out, d_out, n * sizeof(swm_read_result_t), hipMemcpyDeviceToHost,
-                                        ctx->stream) != hipSuccess ||
-                         hipStreamSynchronize(ctx->stream) != hipSuccess))
-        rc = SWM_EDEVICE;
-    if (d_reads) (void)hipFree(d_reads);
-    if (d_lens) (void)hipFree(d_lens);
-    if (d_out) (void)hipFree(d_out);
-    return rc;
+    return swm_slot_run_host(ctx, ctx->slot0, reads, stride, lens, n, n, e, out, ctx->opt.graph != 0);
 }
 
 extern "C" int swm_myers_device(swm_ctx_t *ctx, const uint32_t *d_read_words, const uint32_t *d_lens,
@@ -1532,11 +2233,11 @@ extern "C" int swm_myers_device(swm_ctx_t *ctx, const uint32_t *d_read_words, co
         !d_loc)
         return SWM_EINVAL;
     if (hipSetDevice(ctx->device) != hipSuccess) return SWM_EDEVICE;
-    k_myers_explicit<<<blocks_for(n, 256), 256, 0, ctx->stream>>>((uint32_t)n, d_read_words, d_lens, d_cand_read,
-                                                                  d_win_start, d_ref_len, ctx->d_base_2bits, e,
-                                                                  d_err, d_loc);
+    hipStream_t st = ctx->slot0->stream;
+    k_myers_explicit<<<blocks_for(n, 256), 256, 0, st>>>((uint32_t)n, d_read_words, d_lens, d_cand_read, d_win_start,
+                                                         d_ref_len, ctx->d_base_2bits, e, d_err, d_loc);
     SWM_HIP_CHECK(hipGetLastError());
-    SWM_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    SWM_HIP_CHECK(hipStreamSynchronize(st));
     return SWM_OK;
 }
 
@@ -1562,6 +2263,19 @@ extern "C" int swm_dev_alloc(void **p, uint64_t bytes)
 extern "C" int swm_dev_free(void *p)
 {
     return (!p || hipFree(p) == hipSuccess) ? SWM_OK : SWM_EDEVICE;
+}
+
+extern "C" int swm_host_alloc(void **p, uint64_t bytes)
+{
+    if (!p || !bytes) return SWM_EINVAL;
+    *p = nullptr;
+    return hipHostMalloc(p, bytes, hipHostMallocDefault) == hipSuccess ? SWM_OK : SWM_ENOMEM;
+}
+
+extern "C" int swm_host_free(void *p)
+{
+    if (!p) return SWM_OK;
+    return hipHostFree(p) == hipSuccess ? SWM_OK : SWM_EDEVICE;
 }
 
 extern "C" int swm_copy_h2d(void *dst, const void *src, uint64_t bytes)

@@ -1,0 +1,122 @@
+"""Serving loop on the GPU: continuous batching over the resident index.
+
+Batches of mixed sizes and read lengths are submitted together to a server
+with several slots (streams), so their work overlaps on the device; every
+result must equal the same batch mapped alone, and the oracle.
+"""
+import numpy as np
+import pytest
+
+import swmapper_amd as swm
+import oracle_lib as orc
+import workloads as wl
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print("[case]", *a, flush=True)
+
+
+def _assert_same(got, exp, what=""):
+    for f in ("count", "pos", "strand", "dist"):
+        bad = np.flatnonzero(got[f] != exp[f])
+        assert bad.size == 0, f"{what} {f}: {bad.size} reads differ, first {bad[:5]}"
+
+
+@pytest.fixture(scope="module")
+def setup():
+    g = wl.genome(600_000, seed=91, kind=swm.GENOME_REPEAT_FAMILIES, frac=0.1)
+    pool_r, pool_l, _, _ = wl.reads(g, 30_000, len_min=50, len_max=250, err=(0.01, 0.04), seed=92)
+    gi = swm.build_index(g, 12)
+    rng = np.random.default_rng(93)
+    batches = []
+    for _ in range(14):
+        n = int(rng.integers(200, 4000))
+        s = int(rng.integers(0, pool_l.size - n))
+        batches.append((pool_r[s:s + n], pool_l[s:s + n]))
+    return g, gi, batches
+
+
+@pytest.mark.parametrize("graph,slots", [(True, 4), (False, 3), (True, 1)])
+def test_mix_equals_alone(setup, graph, slots):
+    g, gi, batches = setup
+    m = swm.Mapper.from_index(gi)
+    alone = [m.map(r, l, 5) for r, l in batches]
+    _say("serve mix graph", graph, "slots", slots)
+    srv = swm.Server(m, e=5, stride=256, n_slots=slots, graph=graph)
+    tickets = [srv.submit(r, l) for r, l in batches]
+    infos = []
+    for t, a in zip(tickets, alone):
+        got, info = srv.wait(t)
+        _assert_same(got, a, f"ticket {t}")
+        assert info["status"] == 0 and info["t_submit"] <= info["t_admit"] <= info["t_done"]
+        infos.append(info)
+    tot = srv.totals()
+    assert tot["batches"] == len(batches)
+    assert tot["reads"] == sum(l.size for _, l in batches)
+    assert tot["peak_bytes"] <= tot["budget"]
+    if slots > 1:
+        assert len({i["slot"] for i in infos}) > 1
+    srv.close()
+    m.close()
+
+
+def test_serve_matches_oracle(setup):
+    g, gi, batches = setup
+    ref = orc.OracleRef.from_index(orc.OracleIndex(g, 12))
+    m = swm.Mapper.from_index(gi)
+    srv = swm.Server(m, e=5, n_slots=4, graph=True)
+    _say("serve vs oracle")
+    sel = batches[:5]
+    tickets = [srv.submit(r, l) for r, l in sel]
+    for t, (r, l) in zip(tickets, sel):
+        got, info = srv.wait(t)
+        exp, nv = ref.map(r, l, 5)
+        _assert_same(got, exp, f"ticket {t}")
+        assert info["stats"]["n_verifications"] == nv
+    srv.close()
+    m.close()
+
+
+def test_admission_by_budget(setup):
+    """A budget that holds one batch at a time still completes every batch in
+    order of submission; a batch larger than the budget is refused."""
+    g, gi, batches = setup
+    m = swm.Mapper.from_index(gi)
+    big = max(l.size for _, l in batches)
+    srv = swm.Server(m, e=5, n_slots=4, graph=False, hbm_budget=int(big * 8000 + (1 << 21)))
+    _say("serve budget")
+    tickets = [srv.submit(r, l) for r, l in batches[:6]]
+    admits = []
+    for t, (r, l) in zip(tickets, batches[:6]):
+        got, info = srv.wait(t)
+        _assert_same(got, m.map(r, l, 5), f"ticket {t}")
+        admits.append(info["t_admit"])
+    assert admits == sorted(admits)
+    tot = srv.totals()
+    assert tot["peak_bytes"] <= tot["budget"]
+    r, l = batches[0]
+    reps = int(np.ceil(tot["budget"] / 4000 / l.size)) + 2
+    huge_r = np.tile(r, (reps, 1))
+    huge_l = np.tile(l, reps)
+    if huge_l.size <= 65536:
+        with pytest.raises(swm.SwmError):
+            srv.submit(huge_r, huge_l)
+    srv.close()
+    m.close()
+
+
+def test_empty_and_invalid_batches(setup):
+    g, gi, batches = setup
+    m = swm.Mapper.from_index(gi)
+    srv = swm.Server(m, e=5, n_slots=2)
+    _say("serve empty/invalid")
+    t = srv.submit(np.zeros((0, 256), np.uint8), np.zeros(0, np.uint32))
+    got, info = srv.wait(t)
+    assert got.size == 0 and info["status"] == 0
+    bad = np.full(3, 255, np.uint32)      # L + 2 > stride
+    with pytest.raises(swm.SwmError):
+        srv.submit(np.zeros((3, 256), np.uint8), bad)
+    srv.close()
+    m.close()

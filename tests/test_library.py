@@ -40,6 +40,16 @@ def test_argument_validation_without_gpu():
     ih = swm.IndexHost()
     assert L.swm_index_build_mem(None, 10, 12, C.byref(ih), None) == -1
     assert L.swm_index_build(b"/nonexistent.fa", b"/tmp/x", 12) == -2
+    v = C.c_int64(0)
+    assert L.swm_ctx_set_option(None, swm.OPT_GRAPH, 1) == -1
+    assert L.swm_ctx_get_option(None, swm.OPT_GRAPH, C.byref(v)) == -1
+    assert L.swm_autotune(None, reads.ctypes.data, 256, lens.ctypes.data, 4, 5, 1, None) == -1
+    cfg = swm.ServerConfig(4, 5, 256, 0, 0, 1, 0)
+    assert L.swm_server_create(None, C.byref(cfg), C.byref(h)) == -1
+    t = C.c_uint64(0)
+    assert L.swm_server_submit(None, reads.ctypes.data, lens.ctypes.data, 4, out.ctypes.data, C.byref(t)) == -1
+    assert L.swm_server_wait(None, 1, None) == -1
+    L.swm_server_destroy(None)
 
 
 def test_index_files_layout(tmp_path):
