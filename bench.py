@@ -64,7 +64,7 @@ def time_form(m, opts, step, steps, warmup):
     return time.perf_counter() - t0, lat, st_all
 
 
-def serve_bench(swm, m, g, e, seconds, log, slots=4, graph=True):
+def serve_bench(swm, m, g, e, seconds, log, slots=2, graph=True, merge=262144):
     """Continuous batching on the resident index: batches of U[1k, 64k] reads
     with lengths U[50, 250] from page-locked memory, Poisson arrivals at
     0.3 / 0.6 / 0.9 of the measured saturation rate."""
@@ -87,7 +87,7 @@ def serve_bench(swm, m, g, e, seconds, log, slots=4, graph=True):
             out.append((s, n))
         return out
 
-    srv = swm.Server(m, e=e, stride=256, n_slots=slots, graph=graph)
+    srv = swm.Server(m, e=e, stride=256, n_slots=slots, graph=graph, max_launch=merge)
     exact = True
 
     def check(s, n, got):
@@ -97,25 +97,32 @@ def serve_bench(swm, m, g, e, seconds, log, slots=4, graph=True):
             if not np.array_equal(got[f], ref[f]):
                 exact = False
 
-    # warm every slot's graphs, then saturate
-    for s, n in batches(16):
+    # capacities and graphs of every slot up front, then a few batches, then saturate
+    srv.warmup(pr[:8192], pl[:8192])
+    for s, n in batches(8):
         t = srv.submit(pin_r.a[s:s + n], pin_l.a[s:s + n])
         got, _ = srv.wait(t)
         check(s, n, got)
     sat = batches(48)
     t0 = time.monotonic()
     ts = [srv.submit(pin_r.a[s:s + n], pin_l.a[s:s + n]) for s, n in sat]
-    done, service, gpu = [], [], []
+    done, service, gpu, henq, hwait, lb = [], [], [], [], [], []
     for t, (s, n) in zip(ts, sat):
         got, info = srv.wait(t)
         check(s, n, got)
         done.append(info["t_done"])
         service.append((info["t_done"] - info["t_admit"]) * 1e3)
         gpu.append(info["stats"]["ms_total"])
+        henq.append(info["stats"]["ms_host_enqueue"])
+        hwait.append(info["stats"]["ms_host_wait"])
+        lb.append(info["launch_batches"])
     max_rps = sum(n for _, n in sat) / (max(done) - t0)
     sat_info = {"reads_per_s": round(max_rps, 1), "batches": len(sat),
                 "service_ms_mean": round(float(np.mean(service)), 3),
-                "device_ms_mean": round(float(np.mean(gpu)), 3)}
+                "device_ms_mean": round(float(np.mean(gpu)), 3),
+                "host_enqueue_ms_mean": round(float(np.mean(henq)), 3),
+                "host_wait_ms_mean": round(float(np.mean(hwait)), 3),
+                "batches_per_launch_mean": round(float(np.mean(lb)), 2)}
     log(f"serving saturation: {sat_info}")
     mean_n = (1024 + 65536) / 2
     loads = []
@@ -133,7 +140,7 @@ def serve_bench(swm, m, g, e, seconds, log, slots=4, graph=True):
             for (tk, arr, s, n) in tickets:
                 got, info = srv.wait(tk)
                 results.append((info["t_done"] - arr, n, info["t_done"], info["t_done"] - info["t_admit"],
-                                info["stats"]["ms_total"]))
+                                info["stats"]["ms_total"], info["launch_batches"]))
                 check(s, n, got)
 
         tickets = []
@@ -155,7 +162,8 @@ def serve_bench(swm, m, g, e, seconds, log, slots=4, graph=True):
                       "p50_ms": round(float(np.percentile(lat, 50)), 3),
                       "p99_ms": round(float(np.percentile(lat, 99)), 3),
                       "service_ms_mean": round(float(np.mean([r[3] for r in results])) * 1e3, 3),
-                      "device_ms_mean": round(float(np.mean([r[4] for r in results])), 3)})
+                      "device_ms_mean": round(float(np.mean([r[4] for r in results])), 3),
+                      "batches_per_launch_mean": round(float(np.mean([r[5] for r in results])), 2)})
         log(f"serving load {load}: {loads[-1]}")
     tot = srv.totals()
     srv.close()
@@ -163,7 +171,7 @@ def serve_bench(swm, m, g, e, seconds, log, slots=4, graph=True):
     pin_l.free()
     return {"workload": "batches of U[1k,64k] reads, lengths U[50,250], error U[1%,4%], from page-locked "
                         f"memory; Poisson arrivals; {slots} slots (streams), {'graph' if graph else 'synced'} form, "
-                        "FIFO admission by HBM budget",
+                        f"FIFO admission by HBM budget, waiting batches merged up to {merge} reads per launch",
             "max_reads_per_s": round(max_rps, 1), "saturation": sat_info, "loads": loads,
             "bit_exact_vs_offline": exact,
             "peak_admitted_bytes": tot["peak_bytes"], "hbm_budget_bytes": tot["budget"]}
@@ -179,8 +187,13 @@ def main():
     ap.add_argument("--forms", default="plain,queue,graph", help="pipeline forms to time (plain, queue, graph)")
     ap.add_argument("--autotune", action="store_true", help="autotune the tunables on a 100k-read sample first")
     ap.add_argument("--serve-seconds", type=float, default=3.0, help="length of each serving load (0: no serving)")
-    ap.add_argument("--serve-slots", type=int, default=4, help="serving slots (streams)")
+    ap.add_argument("--serve-slots", type=int, default=2, help="serving slots (streams)")
+    ap.add_argument("--serve-merge", type=int, default=262144,
+                    help="reads per serving launch when waiting batches are merged (0: one batch per launch)")
     ap.add_argument("--serve-form", default="graph", choices=("graph", "synced"))
+    ap.add_argument("--hw-queues", type=int, default=8,
+                    help="GPU_MAX_HW_QUEUES for this process when not set by the caller above 4: the serving "
+                         "slots' streams each get a hardware queue")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target length of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
@@ -191,6 +204,9 @@ def main():
               "(mapping shards by reads across GPUs; see the sharding notes)", file=sys.stderr)
         sys.exit(2)
 
+    # before the HIP runtime starts (it reads the variable once)
+    if int(os.environ.get("GPU_MAX_HW_QUEUES", "4")) <= 4 and 1 <= args.hw_queues <= 32:
+        os.environ["GPU_MAX_HW_QUEUES"] = str(args.hw_queues)
     import swmapper_amd as swm
     cfg = dict(CONFIGS[args.config])
     if args.e is not None:
@@ -282,7 +298,7 @@ def main():
     serving = None
     if args.serve_seconds > 0:
         serving = serve_bench(swm, m, g, e, args.serve_seconds, log, slots=args.serve_slots,
-                              graph=args.serve_form == "graph")
+                              graph=args.serve_form == "graph", merge=args.serve_merge)
 
     cpu = None
     if not args.no_cpu:

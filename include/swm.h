@@ -71,6 +71,9 @@ typedef struct swm_stats {
     /* stage times (HIP events on the context stream): seed lookup; candidate
      * batches; compaction; banded Myers; Hamming + traceback + emission */
     double ms_seed, ms_plan, ms_compact, ms_myers, ms_emit, ms_total;
+    /* host wall clock of the call: enqueueing (input staging, copies,
+     * launches, readbacks between stages) and the final wait */
+    double ms_host_enqueue, ms_host_wait;
 } swm_stats_t;
 
 typedef struct swm_ctx swm_ctx_t;
@@ -158,7 +161,9 @@ typedef struct swm_server_config {
     uint64_t stride;       /* -l: read stride of every submitted batch */
     uint64_t hbm_budget;   /* device bytes the admitted batches may hold (0 = 90% of free HBM at creation) */
     uint64_t max_batch;    /* reads per batch accepted (0 = 65536) */
-    uint32_t graph;        /* 1: replay captured graphs, batch sizes padded to buckets of sqrt(2) steps */
+    uint64_t max_launch;   /* continuous batching: a free slot takes every waiting batch that fits, up to
+                            * this many reads, in one pipeline launch (0: one batch per launch) */
+    uint32_t graph;        /* 1: replay captured graphs, launch sizes padded to powers of two (>= 1024) */
     uint32_t pad;
 } swm_server_config_t;
 
@@ -169,19 +174,27 @@ typedef struct swm_batch_info {
     int32_t status;                    /* SWM_OK or the mapping error */
     uint32_t slot;
     uint64_t est_bytes;                /* device bytes the batch was admitted against */
-    swm_stats_t stats;
+    uint64_t launch_reads;             /* reads of the launch the batch ran in (merged batches share one) */
+    uint32_t launch_batches;           /* batches in that launch */
+    uint32_t pad;
+    swm_stats_t stats;                 /* of that launch */
 } swm_batch_info_t;
 
-/* Admission is first come first served: the oldest waiting batch starts when
- * a slot is free and the budget holds its estimated device bytes.  Batches
- * on different slots overlap on the device; each result is the same as the
- * batch mapped alone (swm_map_batch). */
+/* Admission is first come first served: when a slot is free it takes the
+ * oldest waiting batches -- as many as fit max_launch reads and the budget
+ * (each batch held against its estimated device bytes) -- and maps them in
+ * one launch.  Launches on different slots overlap on the device; each
+ * result is the same as the batch mapped alone (swm_map_batch). */
 int swm_server_create(swm_ctx_t *ctx, const swm_server_config_t *cfg, swm_server_t **out);
 /* Non-blocking.  reads/lens/out stay owned by the caller and must stay valid
  * until swm_server_wait returns for the ticket.  SWM_EINVAL for a bad batch,
  * SWM_ENOMEM if the batch alone exceeds the budget. */
 int swm_server_submit(swm_server_t *srv, const char *reads, const uint32_t *lens, uint64_t n,
                       swm_read_result_t *out, uint64_t *ticket);
+/* Before traffic: maps the sample (repeated to size) on every slot at every
+ * padded batch size up to max_batch, so that capacities are learned and
+ * graphs captured up front.  Must not overlap submissions. */
+int swm_server_warmup(swm_server_t *srv, const char *reads, const uint32_t *lens, uint64_t n);
 /* Blocks until the batch is done; fills info (may be NULL) and forgets the ticket. */
 int swm_server_wait(swm_server_t *srv, uint64_t ticket, swm_batch_info_t *info);
 /* Totals since creation: batches, reads, peak admitted bytes, budget. */

@@ -47,7 +47,8 @@ class Stats(C.Structure):
     _fields_ = [("n_reads", C.c_uint64), ("n_verifications", C.c_uint64), ("n_band_steps", C.c_uint64),
                 ("n_records", C.c_uint64),
                 ("ms_seed", C.c_double), ("ms_plan", C.c_double), ("ms_compact", C.c_double),
-                ("ms_myers", C.c_double), ("ms_emit", C.c_double), ("ms_total", C.c_double)]
+                ("ms_myers", C.c_double), ("ms_emit", C.c_double), ("ms_total", C.c_double),
+                ("ms_host_enqueue", C.c_double), ("ms_host_wait", C.c_double)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -55,13 +56,14 @@ class Stats(C.Structure):
 
 class ServerConfig(C.Structure):
     _fields_ = [("n_slots", C.c_uint32), ("e", C.c_uint32), ("stride", C.c_uint64), ("hbm_budget", C.c_uint64),
-                ("max_batch", C.c_uint64), ("graph", C.c_uint32), ("pad", C.c_uint32)]
+                ("max_batch", C.c_uint64), ("max_launch", C.c_uint64), ("graph", C.c_uint32), ("pad", C.c_uint32)]
 
 
 class BatchInfo(C.Structure):
     _fields_ = [("ticket", C.c_uint64), ("n_reads", C.c_uint64), ("t_submit", C.c_double),
                 ("t_admit", C.c_double), ("t_done", C.c_double), ("status", C.c_int32), ("slot", C.c_uint32),
-                ("est_bytes", C.c_uint64), ("stats", Stats)]
+                ("est_bytes", C.c_uint64), ("launch_reads", C.c_uint64), ("launch_batches", C.c_uint32),
+                ("pad", C.c_uint32), ("stats", Stats)]
 
 
 # forms and tunables (swm_ctx_set_option)
@@ -111,6 +113,7 @@ def lib():
             "swm_server_create": ([vp, C.POINTER(ServerConfig), C.POINTER(vp)], C.c_int),
             "swm_server_submit": ([vp, vp, vp, u64, vp, C.POINTER(u64)], C.c_int),
             "swm_server_wait": ([vp, u64, C.POINTER(BatchInfo)], C.c_int),
+            "swm_server_warmup": ([vp, vp, vp, u64], C.c_int),
             "swm_server_totals": ([vp, C.POINTER(u64), C.POINTER(u64), C.POINTER(u64), C.POINTER(u64)], C.c_int),
             "swm_server_destroy": ([vp], None),
             "swm_host_alloc": ([C.POINTER(vp), u64], C.c_int),
@@ -283,13 +286,15 @@ class Mapper:
 
 
 class Server:
-    """Continuous-batching serving loop over a Mapper's resident index: the
-    oldest waiting batch is admitted when one of n_slots streams is free and
-    the HBM budget holds its estimated workspace."""
+    """Continuous-batching serving loop over a Mapper's resident index: when
+    one of n_slots streams is free it takes the oldest waiting batches that
+    fit max_launch reads and the HBM budget and maps them in one launch
+    (max_launch=0: one batch per launch)."""
 
-    def __init__(self, mapper, e=5, stride=256, n_slots=4, hbm_budget=0, max_batch=65536, graph=True):
+    def __init__(self, mapper, e=5, stride=256, n_slots=4, hbm_budget=0, max_batch=65536, graph=True,
+                 max_launch=262144):
         self.mapper = mapper
-        cfg = ServerConfig(n_slots, e, stride, hbm_budget, max_batch, 1 if graph else 0, 0)
+        cfg = ServerConfig(n_slots, e, stride, hbm_budget, max_batch, max_launch, 1 if graph else 0, 0)
         h = C.c_void_p()
         _check("swm_server_create", lib().swm_server_create(mapper._h, C.byref(cfg), C.byref(h)))
         self._h = h
@@ -305,6 +310,12 @@ class Server:
                                                             C.byref(t)))
         self._keep[t.value] = (reads, lens, out)
         return t.value
+
+    def warmup(self, reads, lens):
+        """Learn capacities and capture graphs on every slot before traffic."""
+        reads = np.ascontiguousarray(reads, dtype=np.uint8)
+        lens = np.ascontiguousarray(lens, dtype=np.uint32)
+        _check("swm_server_warmup", lib().swm_server_warmup(self._h, _ptr(reads), _ptr(lens), lens.size))
 
     def wait(self, ticket):
         """Blocks until the batch is done: (results, info dict)."""

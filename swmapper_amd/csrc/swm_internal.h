@@ -63,6 +63,8 @@ swm_slot *swm_slot_create(swm_ctx_t *ctx);
 void swm_slot_destroy(swm_slot *s);
 const swm_stats_t *swm_slot_stats(const swm_slot *s);
 uint64_t swm_slot_bytes(const swm_slot *s);
+/* size the slot's workspace for batches of up to n reads once graphs are used */
+void swm_slot_reserve(swm_slot *s, uint64_t n);
 /* map n reads already on the device, on the slot's stream, in the synced or
  * the graph form (returns after the results are written) */
 int swm_slot_run_device(swm_ctx_t *ctx, swm_slot *sl, const char *d_reads, uint64_t stride, const uint32_t *d_lens,
@@ -70,6 +72,15 @@ int swm_slot_run_device(swm_ctx_t *ctx, swm_slot *sl, const char *d_reads, uint6
 /* map n host reads through the slot's buffers, padded to n_pad reads of length 0 */
 int swm_slot_run_host(swm_ctx_t *ctx, swm_slot *sl, const char *reads, uint64_t stride, const uint32_t *lens,
                       uint64_t n, uint64_t n_pad, uint32_t e, swm_read_result_t *out, bool graph);
+/* several host batches mapped in one run (results per segment) */
+struct swm_segment_t {
+    const char *reads;
+    const uint32_t *lens;
+    uint64_t n;
+    swm_read_result_t *out;
+};
+int swm_slot_run_segments(swm_ctx_t *ctx, swm_slot *sl, const swm_segment_t *seg, uint32_t nseg, uint64_t stride,
+                          uint64_t n_pad, uint32_t e, bool graph);
 /* SWM_EINVAL if a read breaks the stride rule or has more than 16 long seeds */
 int swm_check_lens(const swm_ctx_t *ctx, uint64_t stride, const uint32_t *lens, uint64_t n);
 

@@ -31,6 +31,7 @@
 #include <new>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 
 #include "swm_recipe.h"
 
@@ -137,6 +138,8 @@ struct SeedOut {
     uint32_t *nsel;     /* [nrs] selected 12-mers (4 per selected long seed) */
     uint64_t *npos;     /* [nrs + 1] positions over the selected seeds */
     uint32_t *nchunk;   /* [nrs + 1] sum of ceil(cnt / 32), + 1 */
+    unsigned long long *counters;   /* the run's counters, zeroed here */
+    uint32_t *nuniq;    /* [nrs + 1]: its last entry is zeroed here */
 };
 
 /* A read base as the mapper sees it: A,C,G,T -> 0..3 (complemented on the
@@ -176,6 +179,15 @@ __global__ void __launch_bounds__(128) k_seed(const char *__restrict__ reads, ui
      * gathers so that the selected ones are not fetched twice */
     __shared__ uint32_t s_off[64 * 2 * SEED_READS], s_cnt[64 * 2 * SEED_READS];
     uint8_t *s_rd = reinterpret_cast<uint8_t *>(s_rd32);
+    if (blockIdx.x == 0) {
+        /* the run's counters and the sentinel entries the scans read */
+        if (threadIdx.x < 8) o.counters[threadIdx.x] = 0;
+        if (threadIdx.x == 8) {
+            o.npos[2 * nreads] = 0;
+            o.nchunk[2 * nreads] = 0;
+            o.nuniq[2 * nreads] = 0;
+        }
+    }
     const uint32_t r0 = blockIdx.x * SEED_READS;
     const uint32_t nr = min((uint32_t)SEED_READS, nreads - r0);
     {
@@ -1509,80 +1521,87 @@ __device__ __forceinline__ T block_incl_scan(T v, T *s_w, T *total)
     return v + off;
 }
 
-template <typename T>
-__global__ void __launch_bounds__(SCAN_NT) k_scan_reduce(const T *__restrict__ in, uint64_t n, T *__restrict__ sums)
+/* One scan job covers up to two arrays of the same length (blockIdx.y picks
+ * the array): array 0 is u64, array 1 is u32; tile sums are u64 for both.
+ * The top kernel also checks each total against its capacity (setting the
+ * overflow flag) and keeps it in the counters. */
+struct ScanJob {
+    const uint64_t *in0;
+    uint64_t *out0;
+    const uint32_t *in1;
+    uint32_t *out1;
+    uint64_t n;
+    uint64_t *sums;          /* [2][nb] */
+    uint32_t nb;
+    uint32_t first;          /* first array (0 or 1); blockIdx.y counts from it */
+    uint64_t cap[2];
+    unsigned long long *tot[2];
+    uint32_t *ovf;
+};
+
+__device__ __forceinline__ uint64_t scan_in(const ScanJob &j, uint32_t y, uint64_t g)
 {
-    __shared__ T s_w[SCAN_NT / 64];
-    const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE;
-    T v = 0;
-#pragma unroll
-    for (int j = 0; j < SCAN_PER; j++) {
-        const uint64_t g = base + threadIdx.x + (uint64_t)j * SCAN_NT;
-        if (g < n) v += in[g];
-    }
-    T tot;
-    (void)block_incl_scan<T, SCAN_NT>(v, s_w, &tot);
-    if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+    return y == 0 ? j.in0[g] : (uint64_t)j.in1[g];
 }
 
-template <typename T>
-__global__ void __launch_bounds__(1024) k_scan_top(T *__restrict__ sums, uint32_t nb)
+__global__ void __launch_bounds__(SCAN_NT) k_scan_reduce(ScanJob j)
 {
-    __shared__ T s_w[16];
-    T carry = 0;
-    for (uint32_t b = 0; b < nb; b += 1024) {
+    __shared__ uint64_t s_w[SCAN_NT / 64];
+    const uint32_t y = j.first + blockIdx.y;
+    const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE;
+    uint64_t v = 0;
+#pragma unroll
+    for (int q = 0; q < SCAN_PER; q++) {
+        const uint64_t g = base + threadIdx.x + (uint64_t)q * SCAN_NT;
+        if (g < j.n) v += scan_in(j, y, g);
+    }
+    uint64_t tot;
+    (void)block_incl_scan<uint64_t, SCAN_NT>(v, s_w, &tot);
+    if (threadIdx.x == 0) j.sums[(uint64_t)y * j.nb + blockIdx.x] = tot;
+}
+
+__global__ void __launch_bounds__(1024) k_scan_top(ScanJob j)
+{
+    __shared__ uint64_t s_w[16];
+    const uint32_t y = j.first + blockIdx.x;
+    uint64_t *sums = j.sums + (uint64_t)y * j.nb;
+    uint64_t carry = 0;
+    for (uint32_t b = 0; b < j.nb; b += 1024) {
         const uint32_t i = b + threadIdx.x;
-        const T v = i < nb ? sums[i] : (T)0;
-        T tot;
-        const T incl = block_incl_scan<T, 1024>(v, s_w, &tot);
-        if (i < nb) sums[i] = carry + incl - v;
+        const uint64_t v = i < j.nb ? sums[i] : 0ull;
+        uint64_t tot;
+        const uint64_t incl = block_incl_scan<uint64_t, 1024>(v, s_w, &tot);
+        if (i < j.nb) sums[i] = carry + incl - v;
         carry += tot;
     }
+    if (threadIdx.x == 0) {
+        if (j.tot[y]) *j.tot[y] = carry;
+        if (carry > j.cap[y]) *j.ovf = 1u;
+    }
 }
 
-template <typename T>
-__global__ void __launch_bounds__(SCAN_NT) k_scan_down(const T *__restrict__ in, T *__restrict__ out, uint64_t n,
-                                                       const T *__restrict__ sums)
+__global__ void __launch_bounds__(SCAN_NT) k_scan_down(ScanJob j)
 {
-    __shared__ T s_w[SCAN_NT / 64];
+    __shared__ uint64_t s_w[SCAN_NT / 64];
+    const uint32_t y = j.first + blockIdx.y;
     const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_PER;
-    T v[SCAN_PER];
-    T t = 0;
+    uint64_t v[SCAN_PER];
+    uint64_t t = 0;
 #pragma unroll
-    for (int j = 0; j < SCAN_PER; j++) {
-        v[j] = base + j < n ? in[base + j] : (T)0;
-        t += v[j];
+    for (int q = 0; q < SCAN_PER; q++) {
+        v[q] = base + q < j.n ? scan_in(j, y, base + q) : 0ull;
+        t += v[q];
     }
-    T tot;
-    T run = block_incl_scan<T, SCAN_NT>(t, s_w, &tot) - t + sums[blockIdx.x];
+    uint64_t tot;
+    uint64_t run = block_incl_scan<uint64_t, SCAN_NT>(t, s_w, &tot) - t + j.sums[(uint64_t)y * j.nb + blockIdx.x];
 #pragma unroll
-    for (int j = 0; j < SCAN_PER; j++) {
-        if (base + j < n) out[base + j] = run;
-        run += v[j];
+    for (int q = 0; q < SCAN_PER; q++) {
+        if (base + q < j.n) {
+            if (y == 0) j.out0[base + q] = run;
+            else j.out1[base + q] = (uint32_t)run;
+        }
+        run += v[q];
     }
-}
-
-/* Zeroes the run's counters and the sentinel entries the scans read. */
-__global__ void k_reset(unsigned long long *counters, uint64_t *npos_end, uint32_t *nchunk_end, uint32_t *nuniq_end)
-{
-    if (threadIdx.x < 8) counters[threadIdx.x] = 0;
-    if (threadIdx.x == 0) { *npos_end = 0; *nchunk_end = 0; *nuniq_end = 0; }
-}
-
-/* Capacity checks of the fixed-capacity form.  They also keep the totals in
- * the counters: [0] candidates, [3] surviving positions. */
-__global__ void k_check_plan(const uint64_t *__restrict__ tot_pos, const uint32_t *__restrict__ tot_chunk,
-                             uint64_t cap_pos, uint64_t cap_chunk, unsigned long long *counters, uint32_t *ovf)
-{
-    counters[3] = *tot_pos;
-    if (*tot_pos > cap_pos || (uint64_t)*tot_chunk > cap_chunk) *ovf = 1u;
-}
-
-__global__ void k_check_cand(const uint32_t *__restrict__ ncand, uint64_t cap_cand, unsigned long long *counters,
-                             uint32_t *ovf)
-{
-    counters[0] = *ncand;
-    if ((uint64_t)*ncand > cap_cand) *ovf = 1u;
 }
 
 /* --------------------------------------------------------------- host side */
@@ -1652,6 +1671,11 @@ struct swm_slot {
     uint64_t *h_small = nullptr;     /* [0..7] counters, [8] positions, [9] chunks, [10] candidates */
     struct Pin { void *p = nullptr; size_t cap = 0; };
     Pin h_reads, h_lens, h_out;
+    double t_entry = 0;   /* host clock at the start of the current call */
+    /* reads the workspace is sized for from the first graph capture on, so
+     * that smaller batches never move a buffer (which would retire every
+     * captured graph of the slot) */
+    uint64_t reserve_n = 0;
     ~swm_slot()
     {
         if (stream) (void)hipStreamSynchronize(stream);
@@ -1692,6 +1716,26 @@ const swm_stats_t *swm_slot_stats(const swm_slot *s) { return &s->stats; }
 
 uint64_t swm_slot_bytes(const swm_slot *s) { return s->ws.bytes(); }
 
+void swm_slot_reserve(swm_slot *s, uint64_t n) { s->reserve_n = n; }
+
+static double host_ms()
+{
+    timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return 1e3 * (double)ts.tv_sec + 1e-6 * (double)ts.tv_nsec;
+}
+
+/* The final wait of a run, with the host time split around it. */
+static int final_sync(swm_slot *sl)
+{
+    const double t0 = host_ms();
+    const hipError_t e = hipStreamSynchronize(sl->stream);
+    const double t1 = host_ms();
+    sl->stats.ms_host_enqueue = t0 - sl->t_entry;
+    sl->stats.ms_host_wait = t1 - t0;
+    return e == hipSuccess ? SWM_OK : SWM_EDEVICE;
+}
+
 /* Device pointers of one run. */
 struct Pipe {
     uint64_t n = 0, nrs = 0;
@@ -1724,8 +1768,10 @@ static int pipe_alloc_a(swm_ws *ws, uint64_t n, uint32_t seln, Pipe &P)
         (rc = ws->get(swm_ws::NBATCH, &P.pio.nbatch, nrs)) || (rc = ws->get(swm_ws::NUNIQ, &P.pio.nuniq, nrs + 1)) ||
         (rc = ws->get(swm_ws::HEAVY, &P.pio.heavy, nrs)) || (rc = ws->get(swm_ws::CAND_OFF, &P.cand_off, nrs + 1)) ||
         (rc = ws->get(swm_ws::HEAVY_C, &P.heavy_c, nrs)) || (rc = ws->get(swm_ws::HEAVY_E, &P.heavy_e, n)) ||
-        (rc = ws->get(swm_ws::SCAN_TMP, &P.scan_tmp, blocks_for(nrs + 1, SCAN_TILE))))
+        (rc = ws->get(swm_ws::SCAN_TMP, &P.scan_tmp, 2 * (uint64_t)blocks_for(nrs + 1, SCAN_TILE))))
         return rc;
+    P.so.counters = P.counters;
+    P.so.nuniq = P.pio.nuniq;
     P.u32c = reinterpret_cast<uint32_t *>(P.counters + 4);
     P.pio.sel = P.so.sel;
     P.pio.seln = seln;
@@ -1761,36 +1807,43 @@ static int pipe_alloc_c(swm_ws *ws, uint64_t cap, Pipe &P)
     return SWM_OK;
 }
 
-template <typename T>
-static void scan_launch(const T *in, T *out, uint64_t n, T *tmp, hipStream_t st)
+static void scan_launch(ScanJob j, uint32_t n_arrays, hipStream_t st)
 {
-    const uint32_t nb = blocks_for(n, SCAN_TILE);
-    k_scan_reduce<T><<<nb, SCAN_NT, 0, st>>>(in, n, tmp);
-    k_scan_top<T><<<1, 1024, 0, st>>>(tmp, nb);
-    k_scan_down<T><<<nb, SCAN_NT, 0, st>>>(in, out, n, tmp);
+    j.nb = blocks_for(j.n, SCAN_TILE);
+    k_scan_reduce<<<dim3(j.nb, n_arrays), SCAN_NT, 0, st>>>(j);
+    k_scan_top<<<n_arrays, 1024, 0, st>>>(j);
+    k_scan_down<<<dim3(j.nb, n_arrays), SCAN_NT, 0, st>>>(j);
 }
 
-/* Stage A: counters, seeds, offsets of the positions and chunks. */
+/* Stage A: seeds (which also zero the counters), offsets of the positions
+ * and chunks with their capacity check. */
 static int enq_a(const swm_ctx_t *ctx, Pipe &P, const char *d_reads, uint64_t stride, const uint32_t *d_lens,
-                 uint32_t e, hipStream_t st, hipEvent_t *ev)
+                 uint32_t e, uint64_t cap_pos, uint64_t cap_chunk, hipStream_t st, hipEvent_t *ev)
 {
     if (ev) SWM_HIP_CHECK(hipEventRecord(ev[0], st));
-    k_reset<<<1, 64, 0, st>>>(P.counters, P.so.npos + P.nrs, P.so.nchunk + P.nrs, P.pio.nuniq + P.nrs);
     k_seed<<<blocks_for(P.n, SEED_READS), 2 * SEED_READS, 0, st>>>(d_reads, stride, d_lens, (uint32_t)P.n, ctx->k, e,
                                                                    P.seln, ctx->d_hash_offset, ctx->d_recipe, P.so);
     SWM_HIP_CHECK(hipGetLastError());
     if (ev) SWM_HIP_CHECK(hipEventRecord(ev[1], st));
-    scan_launch<uint64_t>(P.so.npos, P.surv_off, P.nrs + 1, P.scan_tmp, st);
-    scan_launch<uint32_t>(P.so.nchunk, P.bd_off, P.nrs + 1, reinterpret_cast<uint32_t *>(P.scan_tmp), st);
+    ScanJob j{};
+    j.in0 = P.so.npos; j.out0 = P.surv_off;
+    j.in1 = P.so.nchunk; j.out1 = P.bd_off;
+    j.n = P.nrs + 1;
+    j.sums = P.scan_tmp;
+    j.first = 0;
+    j.cap[0] = cap_pos; j.cap[1] = cap_chunk;
+    j.tot[0] = P.counters + 3; j.tot[1] = nullptr;
+    j.ovf = P.u32c + 7;
+    scan_launch(j, 2, st);
     SWM_HIP_CHECK(hipGetLastError());
     return SWM_OK;
 }
 
-/* Stage B: capacity check, batch plans, large-batch sort, candidate offsets. */
-static int enq_b(const swm_ctx_t *ctx, Pipe &P, uint64_t cap_pos, uint64_t cap_chunk, hipStream_t st, hipEvent_t *ev)
+/* Stage B: batch plans, large-batch sort, candidate offsets with their
+ * capacity check. */
+static int enq_b(const swm_ctx_t *ctx, Pipe &P, uint64_t cap_cand, hipStream_t st, hipEvent_t *ev)
 {
     const swm_opts &o = ctx->opt;
-    k_check_plan<<<1, 1, 0, st>>>(P.surv_off + P.nrs, P.bd_off + P.nrs, cap_pos, cap_chunk, P.counters, P.u32c + 7);
     if (o.work_queue) {
         k_plan<<<blocks_for(P.nrs, 64), 64, 0, st>>>((uint32_t)P.nrs, ctx->d_pl, P.pio);
         k_plan_heavy<<<ctx->n_cu * o.plan_heavy_waves, 64, 0, st>>>(ctx->d_pl, P.pio);
@@ -1801,7 +1854,15 @@ static int enq_b(const swm_ctx_t *ctx, Pipe &P, uint64_t cap_pos, uint64_t cap_c
                                     P.pio.nuniq);
     SWM_HIP_CHECK(hipGetLastError());
     if (ev) SWM_HIP_CHECK(hipEventRecord(ev[2], st));
-    scan_launch<uint32_t>(P.pio.nuniq, P.cand_off, P.nrs + 1, reinterpret_cast<uint32_t *>(P.scan_tmp), st);
+    ScanJob j{};
+    j.in1 = P.pio.nuniq; j.out1 = P.cand_off;
+    j.n = P.nrs + 1;
+    j.sums = P.scan_tmp;
+    j.first = 1;
+    j.cap[0] = ~0ull; j.cap[1] = cap_cand;
+    j.tot[0] = nullptr; j.tot[1] = P.counters;
+    j.ovf = P.u32c + 7;
+    scan_launch(j, 1, st);
     SWM_HIP_CHECK(hipGetLastError());
     return SWM_OK;
 }
@@ -1809,12 +1870,11 @@ static int enq_b(const swm_ctx_t *ctx, Pipe &P, uint64_t cap_pos, uint64_t cap_c
 /* Stage C: candidate list, verification, records, per-read results.
  * grid_cand sizes the candidate kernels (their count is read on the device). */
 static int enq_c(const swm_ctx_t *ctx, Pipe &P, const uint32_t *d_lens, uint32_t e, swm_read_result_t *d_out,
-                 uint64_t cap_cand, uint64_t grid_cand, hipStream_t st, hipEvent_t *ev)
+                 uint64_t grid_cand, hipStream_t st, hipEvent_t *ev)
 {
     const swm_opts &o = ctx->opt;
     const uint32_t heavy_cands = o.work_queue ? o.heavy_cands : 0xffffffffu;
     const uint32_t *ncand = P.cand_off + P.nrs;
-    k_check_cand<<<1, 1, 0, st>>>(ncand, cap_cand, P.counters, P.u32c + 7);
     k_compact<<<blocks_for(P.nrs, 128), 128, 0, st>>>((uint32_t)P.nrs, P.surv_off, P.pio.surv, P.bd_off, P.pio.bdesc,
                                                       P.pio.nbatch, P.cand_off, P.cs, heavy_cands, P.heavy_c,
                                                       P.u32c + 5);
@@ -1877,7 +1937,7 @@ static int run_synced(swm_ctx_t *ctx, swm_slot *sl, const char *d_reads, uint64_
     Pipe P;
     int rc;
     if ((rc = pipe_alloc_a(ws, n, seln, P))) return rc;
-    if ((rc = enq_a(ctx, P, d_reads, stride, d_lens, e, st, sl->ev))) return rc;
+    if ((rc = enq_a(ctx, P, d_reads, stride, d_lens, e, ~0ull, ~0ull, st, sl->ev))) return rc;
     uint64_t *hs = sl->h_small;
     hs[9] = 0;
     SWM_HIP_CHECK(hipMemcpyAsync(hs + 8, P.surv_off + P.nrs, sizeof(uint64_t), hipMemcpyDeviceToHost, st));
@@ -1885,17 +1945,17 @@ static int run_synced(swm_ctx_t *ctx, swm_slot *sl, const char *d_reads, uint64_
     SWM_HIP_CHECK(hipStreamSynchronize(st));
     const uint64_t tot_pos = hs[8], tot_chunk = (uint32_t)hs[9];
     if ((rc = pipe_alloc_b(ws, tot_pos, tot_chunk, P))) return rc;
-    if ((rc = enq_b(ctx, P, tot_pos, tot_chunk, st, sl->ev))) return rc;
+    if ((rc = enq_b(ctx, P, ~0ull, st, sl->ev))) return rc;
     hs[10] = 0;
     SWM_HIP_CHECK(hipMemcpyAsync(hs + 10, P.cand_off + P.nrs, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
     SWM_HIP_CHECK(hipStreamSynchronize(st));
     const uint32_t ncand = (uint32_t)hs[10];
     if ((rc = pipe_alloc_c(ws, ncand, P))) return rc;
-    if ((rc = enq_c(ctx, P, d_lens, e, d_out, ncand, ncand, st, sl->ev))) return rc;
+    if ((rc = enq_c(ctx, P, d_lens, e, d_out, ncand, st, sl->ev))) return rc;
     SWM_HIP_CHECK(hipMemcpyAsync(hs, P.counters, 8 * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
     if (h_out && n_out)
         SWM_HIP_CHECK(hipMemcpyAsync(h_out, d_out, n_out * sizeof(swm_read_result_t), hipMemcpyDeviceToHost, st));
-    SWM_HIP_CHECK(hipStreamSynchronize(st));
+    if ((rc = final_sync(sl))) return rc;
     stats_from_counters(&sl->stats, n, reinterpret_cast<const unsigned long long *>(hs));
     observe(sl, n, tot_pos, tot_chunk, ncand);
     float ms;
@@ -1943,18 +2003,28 @@ static int run_graph(swm_ctx_t *ctx, swm_slot *sl, const char *d_reads, uint64_t
         }
         if (ent->exec) { (void)hipGraphExecDestroy(ent->exec); ent->exec = nullptr; }
         ent->used = false;
-        const double dn = (double)n;
-        const uint64_t cap_pos = (uint64_t)(dn * sl->r_pos * 1.25) + 4096;
-        const uint64_t cap_chunk = (uint64_t)(dn * sl->r_chunk * 1.25) + 1024;
-        const uint64_t cap_cand = (uint64_t)(dn * sl->r_cand * 1.25) + 4096;
+        auto caps = [&](uint64_t m, uint64_t *cp, uint64_t *cc, uint64_t *cd) {
+            const double dm = (double)m;
+            *cp = (uint64_t)(dm * sl->r_pos * 1.25) + 4096;
+            *cc = (uint64_t)(dm * sl->r_chunk * 1.25) + 1024;
+            *cd = (uint64_t)(dm * sl->r_cand * 1.25) + 4096;
+        };
+        /* capacities: the buffers' (sized for the reserve when there is one),
+         * which bound the overflow checks; the candidate grid follows n (its
+         * loops are grid-stride) */
+        const uint64_t n_cap = sl->reserve_n > n ? sl->reserve_n : n;
+        uint64_t cap_pos, cap_chunk, cap_cand, g_pos, g_chunk, g_cand;
+        caps(n_cap, &cap_pos, &cap_chunk, &cap_cand);
+        caps(n, &g_pos, &g_chunk, &g_cand);
+        if (n_cap > n && (rc = pipe_alloc_a(&sl->ws, n_cap, seln, P))) return rc;
         if ((rc = pipe_alloc_a(&sl->ws, n, seln, P)) || (rc = pipe_alloc_b(&sl->ws, cap_pos, cap_chunk, P)) ||
             (rc = pipe_alloc_c(&sl->ws, cap_cand, P)))
             return rc;
         hipGraph_t graph = nullptr;
         SWM_HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-        int crc = enq_a(ctx, P, d_reads, stride, d_lens, e, st, nullptr);
-        if (!crc) crc = enq_b(ctx, P, cap_pos, cap_chunk, st, nullptr);
-        if (!crc) crc = enq_c(ctx, P, d_lens, e, d_out, cap_cand, cap_cand, st, nullptr);
+        int crc = enq_a(ctx, P, d_reads, stride, d_lens, e, cap_pos, cap_chunk, st, nullptr);
+        if (!crc) crc = enq_b(ctx, P, cap_cand, st, nullptr);
+        if (!crc) crc = enq_c(ctx, P, d_lens, e, d_out, g_cand, st, nullptr);
         const hipError_t ce = hipStreamEndCapture(st, &graph);
         if (crc || ce != hipSuccess) {
             if (graph) (void)hipGraphDestroy(graph);
@@ -1978,7 +2048,7 @@ static int run_graph(swm_ctx_t *ctx, swm_slot *sl, const char *d_reads, uint64_t
     SWM_HIP_CHECK(hipMemcpyAsync(sl->h_small, P.counters, 8 * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
     if (h_out && n_out)
         SWM_HIP_CHECK(hipMemcpyAsync(h_out, d_out, n_out * sizeof(swm_read_result_t), hipMemcpyDeviceToHost, st));
-    SWM_HIP_CHECK(hipStreamSynchronize(st));
+    if ((rc = final_sync(sl))) return rc;
     if (reinterpret_cast<const uint32_t *>(hc + 4)[7]) {
         /* over capacity: nothing was written; the synced form maps the batch
          * and records the larger totals, and the graph is captured again */
@@ -1998,6 +2068,7 @@ int swm_slot_run_device(swm_ctx_t *ctx, swm_slot *sl, const char *d_reads, uint6
                         uint64_t n, uint32_t e, swm_read_result_t *d_out, bool graph)
 {
     if (2 * n >= 0xfffffff0ull) return SWM_EINVAL;
+    sl->t_entry = host_ms();
     return graph ? run_graph(ctx, sl, d_reads, stride, d_lens, n, e, d_out, nullptr, 0)
                  : run_synced(ctx, sl, d_reads, stride, d_lens, n, e, d_out, nullptr, 0);
 }
@@ -2012,48 +2083,74 @@ static bool is_pinned(const void *p)
     return a.type == hipMemoryTypeHost;
 }
 
-/* Host buffers in and out through the slot's own device buffers; reads
- * n .. n_pad - 1 are given length 0 (they map to nothing), so batches of
- * nearby sizes share one captured graph.  Page-locked input is copied by DMA
- * straight from the caller's memory, pageable input through the slot's
- * page-locked staging; results come back through that staging. */
-int swm_slot_run_host(swm_ctx_t *ctx, swm_slot *sl, const char *reads, uint64_t stride, const uint32_t *lens,
-                      uint64_t n, uint64_t n_pad, uint32_t e, swm_read_result_t *out, bool graph)
+/* Host batches in and out through the slot's own device buffers: the
+ * segments are mapped together in one pipeline run, back to back in the
+ * slot's read buffer (every read maps on its own, so a read's result does
+ * not depend on its neighbours).  Reads from the total up to n_pad are given
+ * length 0 (they map to nothing), so runs of nearby sizes share one captured
+ * graph.  Page-locked input is copied by DMA straight from the caller's
+ * memory, pageable input through the slot's page-locked staging; results
+ * come back through that staging. */
+int swm_slot_run_segments(swm_ctx_t *ctx, swm_slot *sl, const swm_segment_t *seg, uint32_t nseg, uint64_t stride,
+                          uint64_t n_pad, uint32_t e, bool graph)
 {
+    uint64_t n = 0;
+    for (uint32_t i = 0; i < nseg; i++) n += seg[i].n;
     if (n_pad < n) n_pad = n;
     if (n == 0) { memset(&sl->stats, 0, sizeof(sl->stats)); return SWM_OK; }
     if (2 * n_pad >= 0xfffffff0ull) return SWM_EINVAL;
+    sl->t_entry = host_ms();
     char *d_reads;
     uint32_t *d_lens;
     swm_read_result_t *d_out;
     int rc;
-    if ((rc = sl->ws.get(swm_ws::IN_READS, &d_reads, n_pad * stride + 4)) ||
-        (rc = sl->ws.get(swm_ws::IN_LENS, &d_lens, n_pad)) || (rc = sl->ws.get(swm_ws::OUT, &d_out, n_pad)) ||
+    const uint64_t n_buf = n_pad > sl->reserve_n ? n_pad : sl->reserve_n;
+    if ((rc = sl->ws.get(swm_ws::IN_READS, &d_reads, n_buf * stride + 4)) ||
+        (rc = sl->ws.get(swm_ws::IN_LENS, &d_lens, n_buf)) || (rc = sl->ws.get(swm_ws::OUT, &d_out, n_buf)) ||
         (rc = sl->pin(sl->h_out, n * sizeof(swm_read_result_t))))
         return rc;
     hipStream_t st = sl->stream;
-    const char *src_r = reads;
-    const uint32_t *src_l = lens;
-    if (!is_pinned(reads)) {
-        if ((rc = sl->pin(sl->h_reads, n * stride))) return rc;
-        memcpy(sl->h_reads.p, reads, n * stride);
-        src_r = (const char *)sl->h_reads.p;
+    uint64_t off = 0;
+    for (uint32_t i = 0; i < nseg; i++) {
+        const uint64_t m = seg[i].n;
+        if (!m) continue;
+        const char *src_r = seg[i].reads;
+        const uint32_t *src_l = seg[i].lens;
+        if (!is_pinned(src_r)) {
+            if ((rc = sl->pin(sl->h_reads, n * stride))) return rc;
+            char *h = (char *)sl->h_reads.p + off * stride;
+            memcpy(h, src_r, m * stride);
+            src_r = h;
+        }
+        if (!is_pinned(src_l)) {
+            if ((rc = sl->pin(sl->h_lens, n * 4))) return rc;
+            uint32_t *h = (uint32_t *)sl->h_lens.p + off;
+            memcpy(h, src_l, m * 4);
+            src_l = h;
+        }
+        SWM_HIP_CHECK(hipMemcpyAsync(d_reads + off * stride, src_r, m * stride, hipMemcpyHostToDevice, st));
+        SWM_HIP_CHECK(hipMemcpyAsync(d_lens + off, src_l, m * 4, hipMemcpyHostToDevice, st));
+        off += m;
     }
-    if (!is_pinned(lens)) {
-        if ((rc = sl->pin(sl->h_lens, n * 4))) return rc;
-        memcpy(sl->h_lens.p, lens, n * 4);
-        src_l = (const uint32_t *)sl->h_lens.p;
-    }
-    SWM_HIP_CHECK(hipMemcpyAsync(d_reads, src_r, n * stride, hipMemcpyHostToDevice, st));
-    SWM_HIP_CHECK(hipMemcpyAsync(d_lens, src_l, n * 4, hipMemcpyHostToDevice, st));
     if (n_pad > n) SWM_HIP_CHECK(hipMemsetAsync(d_lens + n, 0, (n_pad - n) * 4, st));
     swm_read_result_t *h_out = (swm_read_result_t *)sl->h_out.p;
     rc = graph ? run_graph(ctx, sl, d_reads, stride, d_lens, n_pad, e, d_out, h_out, n)
                : run_synced(ctx, sl, d_reads, stride, d_lens, n_pad, e, d_out, h_out, n);
     if (rc) return rc;
     sl->stats.n_reads = n;
-    memcpy(out, h_out, n * sizeof(swm_read_result_t));
+    off = 0;
+    for (uint32_t i = 0; i < nseg; i++) {
+        memcpy(seg[i].out, h_out + off, seg[i].n * sizeof(swm_read_result_t));
+        off += seg[i].n;
+    }
     return SWM_OK;
+}
+
+int swm_slot_run_host(swm_ctx_t *ctx, swm_slot *sl, const char *reads, uint64_t stride, const uint32_t *lens,
+                      uint64_t n, uint64_t n_pad, uint32_t e, swm_read_result_t *out, bool graph)
+{
+    const swm_segment_t seg = {reads, lens, n, out};
+    return swm_slot_run_segments(ctx, sl, &seg, 1, stride, n_pad, e, graph);
 }
 
 /* The device keeps each bucket entry as one 8-byte (position, residual)

@@ -32,15 +32,14 @@ double now_s()
     return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
 }
 
-/* Batch sizes are padded to buckets in sqrt(2) steps from 1024 reads, so a
- * stream of mixed sizes replays a handful of captured graphs. */
+/* Batch sizes are padded to powers of two from 1024 reads, so a stream of
+ * mixed sizes replays a handful of captured graphs per slot (the padding
+ * reads have length 0 and cost almost nothing). */
 uint64_t bucket(uint64_t n)
 {
-    if (n <= 1024) return 1024;
-    double b = 1024.0;
-    while ((uint64_t)b < n) b *= 1.4142135623730951;
-    uint64_t v = ((uint64_t)b + 63) & ~63ull;
-    return v < n ? n : v;
+    uint64_t b = 1024;
+    while (b < n) b <<= 1;
+    return b;
 }
 
 struct Req {
@@ -53,6 +52,8 @@ struct Req {
     int status = SWM_OK;
     uint32_t slot = 0;
     bool done = false;
+    uint64_t launch_reads = 0;
+    uint32_t launch_batches = 0;
     swm_stats_t stats{};
 };
 
@@ -68,7 +69,7 @@ struct swm_server {
     std::vector<std::thread> workers;
     std::vector<swm_slot *> slots;
     uint64_t next_ticket = 1, reserved = 0, peak = 0, budget = 0;
-    uint64_t n_batches = 0, n_reads = 0;
+    uint64_t n_batches = 0, n_reads = 0, n_launches = 0;
     /* device bytes per padded read, learned from the slots' workspaces */
     double bytes_per_read = 0;
     uint64_t slot_max_pad[16] = {};
@@ -83,6 +84,8 @@ struct swm_server {
     {
         (void)hipSetDevice(ctx->device);
         swm_slot *sl = slots[si];
+        std::vector<Req *> take;
+        std::vector<swm_segment_t> segs;
         std::unique_lock<std::mutex> lk(mu);
         for (;;) {
             cv_work.wait(lk, [&] {
@@ -90,29 +93,51 @@ struct swm_server {
                 return reserved + pending.front()->est <= budget;
             });
             if (pending.empty()) return;   /* stop and drained */
-            Req *r = pending.front();
-            pending.pop_front();
-            reserved += r->est;
+            /* the oldest batch (it fits, by the wait above), then every next
+             * one that still fits the launch and the budget */
+            take.clear();
+            uint64_t tot = 0, est = 0;
+            while (!pending.empty()) {
+                Req *r = pending.front();
+                if (!take.empty() && (cfg.max_launch == 0 || tot + r->n > cfg.max_launch)) break;
+                if (!take.empty() && reserved + est + r->est > budget) break;
+                pending.pop_front();
+                take.push_back(r);
+                tot += r->n;
+                est += r->est;
+            }
+            reserved += est;
             peak = std::max(peak, reserved);
-            r->t_admit = now_s();
-            r->slot = si;
+            const double ta = now_s();
+            segs.clear();
+            for (Req *r : take) {
+                r->t_admit = ta;
+                r->slot = si;
+                segs.push_back(swm_segment_t{r->reads, r->lens, r->n, r->out});
+            }
             lk.unlock();
-            /* another waiting batch may fit beside this one */
+            /* another waiting batch may fit on another slot */
             cv_work.notify_one();
-            const int rc = swm_slot_run_host(ctx, sl, r->reads, cfg.stride, r->lens, r->n, r->n_pad, cfg.e, r->out,
-                                           cfg.graph != 0);
+            const uint64_t n_pad = cfg.graph ? bucket(tot) : tot;
+            const int rc = swm_slot_run_segments(ctx, sl, segs.data(), (uint32_t)segs.size(), cfg.stride, n_pad,
+                                                 cfg.e, cfg.graph != 0);
             const double t = now_s();
             const uint64_t ws = swm_slot_bytes(sl);
             lk.lock();
-            r->status = rc;
-            r->stats = *swm_slot_stats(sl);
-            r->t_done = t;
-            r->done = true;
-            reserved -= r->est;
-            slot_max_pad[si] = std::max(slot_max_pad[si], r->n_pad);
+            for (Req *r : take) {
+                r->status = rc;
+                r->stats = *swm_slot_stats(sl);
+                r->t_done = t;
+                r->launch_reads = tot;
+                r->launch_batches = (uint32_t)take.size();
+                r->done = true;
+                n_reads += r->n;
+            }
+            reserved -= est;
+            slot_max_pad[si] = std::max(slot_max_pad[si], n_pad);
             bytes_per_read = std::max(bytes_per_read, (double)ws / (double)slot_max_pad[si]);
-            n_batches++;
-            n_reads += r->n;
+            n_batches += take.size();
+            n_launches++;
             cv_done.notify_all();
             cv_work.notify_all();
         }
@@ -127,7 +152,7 @@ extern "C" int swm_server_create(swm_ctx_t *ctx, const swm_server_config_t *cfg,
     if (c.n_slots == 0) c.n_slots = 4;
     if (c.max_batch == 0) c.max_batch = 65536;
     if (c.n_slots > 16 || c.e > SWM_MAX_E || c.stride < 3 || c.stride > 256 || c.max_batch > (1ull << 26) ||
-        c.graph > 1)
+        c.graph > 1 || (c.max_launch && (c.max_launch < c.max_batch || c.max_launch > (1ull << 26))))
         return SWM_EINVAL;
     if (hipSetDevice(ctx->device) != hipSuccess) return SWM_EDEVICE;
     swm_server *s = new (std::nothrow) swm_server();
@@ -147,6 +172,7 @@ extern "C" int swm_server_create(swm_ctx_t *ctx, const swm_server_config_t *cfg,
     for (uint32_t i = 0; i < c.n_slots; i++) {
         swm_slot *sl = swm_slot_create(ctx);
         if (!sl) { for (auto *x : s->slots) swm_slot_destroy(x); delete s; return SWM_EDEVICE; }
+        if (c.graph) swm_slot_reserve(sl, bucket(std::max(c.max_batch, c.max_launch)));
         s->slots.push_back(sl);
     }
     try {
@@ -214,10 +240,63 @@ extern "C" int swm_server_wait(swm_server_t *s, uint64_t ticket, swm_batch_info_
         info->status = r->status;
         info->slot = r->slot;
         info->est_bytes = r->est;
+        info->launch_reads = r->launch_reads;
+        info->launch_batches = r->launch_batches;
         info->stats = r->stats;
     }
     const int rc = r->status;
     delete r;
+    return rc;
+}
+
+/* Maps the sample (repeated to size) once per slot at every batch bucket up
+ * to max_batch, before any traffic: each slot learns its capacities and
+ * captures its graphs, so served batches never pay for a capture.  Must not
+ * overlap submissions. */
+extern "C" int swm_server_warmup(swm_server_t *s, const char *reads, const uint32_t *lens, uint64_t n)
+{
+    if (!s || !reads || !lens || n == 0) return SWM_EINVAL;
+    int rc = swm_check_lens(s->ctx, s->cfg.stride, lens, n);
+    if (rc) return rc;
+    if (hipSetDevice(s->ctx->device) != hipSuccess) return SWM_EDEVICE;
+    const uint64_t most = std::max(s->cfg.max_batch, s->cfg.max_launch);
+    const uint64_t top = s->cfg.graph ? bucket(most) : most;
+    std::vector<char> rb;
+    std::vector<uint32_t> lb;
+    std::vector<swm_read_result_t> ob;
+    try {
+        rb.resize(top * s->cfg.stride);
+        lb.resize(top);
+        ob.resize(top);
+    } catch (...) {
+        return SWM_ENOMEM;
+    }
+    for (uint64_t i = 0; i < top; i++) {
+        memcpy(&rb[i * s->cfg.stride], reads + (i % n) * s->cfg.stride, s->cfg.stride);
+        lb[i] = lens[i % n];
+    }
+    std::vector<uint64_t> sizes;
+    if (s->cfg.graph) {
+        for (uint64_t b = 1024; b <= top; b <<= 1) sizes.push_back(b);
+        sizes.insert(sizes.begin(), top);   /* the largest first: one growth, then none */
+    } else {
+        sizes.push_back(top);
+    }
+    for (swm_slot *sl : s->slots) {
+        for (uint64_t b : sizes) {
+            /* twice: the first run of a slot sizes its capacities (synced), the next captures */
+            for (int k = 0; k < 2 && rc == SWM_OK; k++)
+                rc = swm_slot_run_host(s->ctx, sl, rb.data(), s->cfg.stride, lb.data(), b, b, s->cfg.e, ob.data(),
+                                       s->cfg.graph != 0);
+        }
+    }
+    if (rc == SWM_OK) {
+        std::lock_guard<std::mutex> g(s->mu);
+        for (size_t i = 0; i < s->slots.size(); i++) {
+            s->slot_max_pad[i] = std::max(s->slot_max_pad[i], top);
+            s->bytes_per_read = std::max(s->bytes_per_read, (double)swm_slot_bytes(s->slots[i]) / (double)top);
+        }
+    }
     return rc;
 }
 

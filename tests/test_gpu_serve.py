@@ -38,13 +38,14 @@ def setup():
     return g, gi, batches
 
 
-@pytest.mark.parametrize("graph,slots", [(True, 4), (False, 3), (True, 1)])
-def test_mix_equals_alone(setup, graph, slots):
+@pytest.mark.parametrize("graph,slots,merge", [(True, 4, 0), (False, 3, 0), (True, 1, 0), (True, 3, 20000),
+                                               (False, 2, 16384), (True, 1, 262144)])
+def test_mix_equals_alone(setup, graph, slots, merge):
     g, gi, batches = setup
     m = swm.Mapper.from_index(gi)
     alone = [m.map(r, l, 5) for r, l in batches]
-    _say("serve mix graph", graph, "slots", slots)
-    srv = swm.Server(m, e=5, stride=256, n_slots=slots, graph=graph)
+    _say("serve mix graph", graph, "slots", slots, "merge", merge)
+    srv = swm.Server(m, e=5, stride=256, n_slots=slots, graph=graph, max_batch=4000, max_launch=merge)
     tickets = [srv.submit(r, l) for r, l in batches]
     infos = []
     for t, a in zip(tickets, alone):
@@ -56,8 +57,11 @@ def test_mix_equals_alone(setup, graph, slots):
     assert tot["batches"] == len(batches)
     assert tot["reads"] == sum(l.size for _, l in batches)
     assert tot["peak_bytes"] <= tot["budget"]
-    if slots > 1:
-        assert len({i["slot"] for i in infos}) > 1
+    for i in infos:
+        assert i["launch_reads"] >= i["n_reads"] and (merge or i["launch_batches"] == 1)
+        assert i["launch_reads"] <= max(merge, 4000)
+    if merge and slots == 1:
+        assert max(i["launch_batches"] for i in infos) > 1      # submitted together: merged
     srv.close()
     m.close()
 
@@ -66,7 +70,7 @@ def test_serve_matches_oracle(setup):
     g, gi, batches = setup
     ref = orc.OracleRef.from_index(orc.OracleIndex(g, 12))
     m = swm.Mapper.from_index(gi)
-    srv = swm.Server(m, e=5, n_slots=4, graph=True)
+    srv = swm.Server(m, e=5, n_slots=4, graph=True, max_launch=0)
     _say("serve vs oracle")
     sel = batches[:5]
     tickets = [srv.submit(r, l) for r, l in sel]

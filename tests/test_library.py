@@ -44,7 +44,7 @@ def test_argument_validation_without_gpu():
     assert L.swm_ctx_set_option(None, swm.OPT_GRAPH, 1) == -1
     assert L.swm_ctx_get_option(None, swm.OPT_GRAPH, C.byref(v)) == -1
     assert L.swm_autotune(None, reads.ctypes.data, 256, lens.ctypes.data, 4, 5, 1, None) == -1
-    cfg = swm.ServerConfig(4, 5, 256, 0, 0, 1, 0)
+    cfg = swm.ServerConfig(4, 5, 256, 0, 0, 0, 1, 0)
     assert L.swm_server_create(None, C.byref(cfg), C.byref(h)) == -1
     t = C.c_uint64(0)
     assert L.swm_server_submit(None, reads.ctypes.data, lens.ctypes.data, 4, out.ctypes.data, C.byref(t)) == -1
