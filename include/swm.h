@@ -68,6 +68,7 @@ typedef struct swm_stats {
     uint64_t n_verifications;  /* (read, strand, candidate) windows run through banded Myers */
     uint64_t n_band_steps;     /* sum of read lengths over those verifications (Myers steps) */
     uint64_t n_records;
+    uint64_t n_positions;      /* bucket entries fetched for the selected seeds (both strands) */
     /* stage times (HIP events on the context stream): seed lookup; candidate
      * batches; compaction; banded Myers; Hamming + traceback + emission */
     double ms_seed, ms_plan, ms_compact, ms_myers, ms_emit, ms_total;

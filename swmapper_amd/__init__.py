@@ -45,7 +45,7 @@ class IndexHost(C.Structure):
 
 class Stats(C.Structure):
     _fields_ = [("n_reads", C.c_uint64), ("n_verifications", C.c_uint64), ("n_band_steps", C.c_uint64),
-                ("n_records", C.c_uint64),
+                ("n_records", C.c_uint64), ("n_positions", C.c_uint64),
                 ("ms_seed", C.c_double), ("ms_plan", C.c_double), ("ms_compact", C.c_double),
                 ("ms_myers", C.c_double), ("ms_emit", C.c_double), ("ms_total", C.c_double),
                 ("ms_host_enqueue", C.c_double), ("ms_host_wait", C.c_double)]

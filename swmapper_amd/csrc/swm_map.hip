@@ -390,17 +390,20 @@ __global__ void __launch_bounds__(64) k_plan(uint32_t nrs, const uint2 *__restri
     for (uint32_t i = 0; i < nsel; i++) {                     /* phase 1 (:2194-2236) */
         const swm_sel_t s = s_next;
         if (i + 1 < nsel) s_next = sel[i + 1];
-        const uint32_t len = s.cnt;
-        uint32_t lv[SWM_CHUNK], pv[SWM_CHUNK];
+        const uint32_t len = s.cnt;                           /* <= 32 for a light item */
+        const uint32_t last = len ? len - 1 : 0;
+        /* 8 entries per group, loaded together with the index clamped into
+         * the chunk (the bucket array has 64 spare entries past its end) */
+        for (uint32_t j0 = 0; j0 < len; j0 += 8) {
+            uint2 v[8];
 #pragma unroll
-        for (int j = 0; j < SWM_CHUNK; j++) {
-            if ((uint32_t)j < len) { const uint2 v = pl[s.off + j]; pv[j] = v.x; lv[j] = v.y; }
-        }
+            for (int q = 0; q < 8; q++) v[q] = pl[s.off + min(j0 + (uint32_t)q, last)];
 #pragma unroll
-        for (int j = 0; j < SWM_CHUNK; j++) {
-            if ((uint32_t)j < len) {
-                if (first_pending) first_pending = false;
-                else if (filter_pass(lv[j], s.meta)) surv[nsurv++] = adjust(pv[j], s.meta);
+            for (int q = 0; q < 8; q++) {
+                if (j0 + (uint32_t)q < len) {
+                    if (first_pending) first_pending = false;
+                    else if (filter_pass(v[q].y, s.meta)) surv[nsurv++] = adjust(v[q].x, s.meta);
+                }
             }
         }
     }
@@ -1008,16 +1011,17 @@ __device__ __forceinline__ bool hamming_one(uint32_t i, int32_t er, const uint32
     uint32_t wl[SWM_NW], wh[SWM_NW];
     load_window(b2, st, L, wl, wh);
     const int sl = lc - (int)L + 1;
+    /* branch-free over all 8 words (a mask of 0 past the read), so every
+     * window and read load is issued up front instead of one per branch */
     int tmp = 0;
 #pragma unroll
     for (int ii = 0; ii < SWM_NW - 1; ii++) {
-        if ((uint32_t)(32 * ii) < L) {
-            uint32_t mask = 0xffffffffu;
-            if ((uint32_t)(32 * (ii + 1)) > L) mask >>= (32 - (L - 32 * ii));
-            const uint32_t lref = __builtin_amdgcn_alignbit(wl[ii + 1], wl[ii], (uint32_t)sl);
-            const uint32_t href = __builtin_amdgcn_alignbit(wh[ii + 1], wh[ii], (uint32_t)sl);
-            tmp += __popc(((w[17 + 2 * ii] ^ href) | (w[16 + 2 * ii] ^ lref)) & mask);
-        }
+        const uint32_t lo = 32u * (uint32_t)ii;
+        const uint32_t left = L > lo ? L - lo : 0u;
+        const uint32_t mask = left >= 32u ? 0xffffffffu : ((1u << left) - 1u);
+        const uint32_t lref = __builtin_amdgcn_alignbit(wl[ii + 1], wl[ii], (uint32_t)sl);
+        const uint32_t href = __builtin_amdgcn_alignbit(wh[ii + 1], wh[ii], (uint32_t)sl);
+        tmp += __popc(((w[17 + 2 * ii] ^ href) | (w[16 + 2 * ii] ^ lref)) & mask);
     }
     if (tmp == er) {
         c.rec[i] = st + (uint32_t)sl + 1u;
@@ -1910,6 +1914,7 @@ static void stats_from_counters(swm_stats_t *s, uint64_t n, const unsigned long 
     s->n_verifications = hc[0] - u[2];
     s->n_band_steps = hc[2];
     s->n_records = hc[1];
+    s->n_positions = hc[3];
 }
 
 static void observe(swm_slot *sl, uint64_t n, uint64_t tot_pos, uint64_t tot_chunk, uint64_t ncand)
