@@ -241,7 +241,18 @@ def main():
     g = swm.synth_genome(cfg["n"], seed=20240601, kind=cfg["kind"], frac=cfg["frac"])
     log(f"genome {cfg['n']} bp in {time.time() - t0:.1f}s")
     ix = swm.build_index(g, 12)
-    log(f"index: {ix.positions.size} positions, GPU build {ix.build_ms:.1f} ms")
+    ix2 = swm.build_index(g, 12)      # timed again with the device warm
+    build_ms = min(ix.build_ms, ix2.build_ms)
+    del ix2
+    log(f"index: {ix.positions.size} positions, GPU build {build_ms:.1f} ms")
+    # algorithmic bytes of the build: FASTA bytes + planes (N/4) + N mask (N/8)
+    # + counts and offsets (2 x 4 B x 4^k) + positions and residuals (6 B each)
+    build_bytes = cfg["n"] + cfg["n"] // 4 + cfg["n"] // 8 + 2 * 4 * 4 ** 12 + 6 * int(ix.positions.size)
+    index_build = {"kernels": "k_encode, k_keys, radix sort, k_runs, scan, k_split, k_tail", "ms": round(build_ms, 3),
+                   "bound": "hbm", "algorithmic_bytes": build_bytes,
+                   "achieved": round(build_bytes / (build_ms / 1e3) / 1e9, 1), "peak": HBM_PEAK_GBS,
+                   "unit": "GB/s", "frac": round(build_bytes / (build_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+                   "bases": cfg["n"], "positions": int(ix.positions.size)}
     reads, lens, _, _ = swm.synth_reads(g, cfg["reads"], len_min=cfg["lmin"], len_max=cfg["lmax"],
                                         err_min=cfg["err"][0], err_max=cfg["err"][1], indel_share=0.2,
                                         seed=777, stride=256)
@@ -387,7 +398,7 @@ def main():
                   for f, r in results.items()},
         "config": {"workload": cfg["desc"], "name": args.config, "form": best, "reads": n, "e": e,
                    "tuned": tuned},
-        "roofline": roof, "roofline_plan": roof_plan, "roofline_myers": roof_myers,
+        "roofline": roof, "roofline_plan": roof_plan, "roofline_myers": roof_myers, "index_build": index_build,
         "serving": serving, "cpu_baseline": cpu,
     }
     print(json.dumps(res), flush=True)

@@ -7,14 +7,15 @@
  *              (get_seed/get_base, serial_index.c:146-235), including the
  *              final flush that zeroes the last word when the genome length
  *              is a multiple of the word size (:423-426)
- *   k_keys     one thread per sampled start p = 0, 4, 8, ... <= N-k: the
- *              k-mer code and N-freeness of the histogram (:253-269), with a
- *              global atomic histogram (.hash_count)
+ *   k_keys     one thread per sampled start p = 0, 4, 8, ... <= N-k, in
+ *              genome order: the k-mer code and N-freeness of the histogram
+ *              (:253-269) as the key, the 12-bit residual code
+ *              (get_seed_from_enc_bits :280-331) with p as the value
+ *   sort       stable LSD radix sort of (code, residual|p): the counting
+ *              sort's bucket order with positions ascending (:343-352)
+ *   k_runs     bucket counts (.hash_count) from the sorted runs, no atomics
  *   scan       exclusive scan of the counts (store_positions :337-340)
- *   sort       stable LSD radix sort of (code, p): the counting sort's
- *              bucket order with positions ascending (:343-352)
- *   k_gather   positions and 12-bit residual codes (get_seed_from_enc_bits
- *              :280-331) in bucket order
+ *   k_split    .positions and .pos_lseeds in bucket order
  *   k_tail     one thread replays the scatter for the last starts, whose
  *              scatter reads the zeroed words, so that slots, overflow and
  *              the END offsets of .hash_offset (Q19) match the serial build
@@ -86,25 +87,6 @@ __global__ void __launch_bounds__(256) k_encode(const char *__restrict__ seq, ui
     s2[2 * w + 1] = s1;
 }
 
-__global__ void __launch_bounds__(256) k_keys(const char *__restrict__ seq, uint64_t nsamp, uint32_t k,
-                                              uint32_t *__restrict__ keys, uint32_t *__restrict__ vals,
-                                              uint32_t *__restrict__ count)
-{
-    uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= nsamp) return;
-    const uint64_t p = j * SWM_SEED_STEP;
-    uint32_t code = 0;
-    bool valid = true;
-    for (uint32_t i = 0; i < k; i++) {
-        int c = base_code(seq[p + i]);
-        if (c < 0) valid = false;
-        code |= (uint32_t)(c < 0 ? 0 : c) << (2 * i);
-    }
-    keys[j] = valid ? code : (1u << (2 * k));
-    vals[j] = (uint32_t)p;
-    if (valid) atomicAdd(&count[code], 1u);
-}
-
 /* get_seed_from_enc_bits residual (serial_index.c:290-315) from the planes
  * as stored. */
 __device__ __forceinline__ uint32_t lseed_at(const uint32_t *__restrict__ b2, uint64_t pos, uint32_t k)
@@ -124,15 +106,61 @@ __device__ __forceinline__ uint32_t lseed_at(const uint32_t *__restrict__ b2, ui
     return fl | (fh << 6);
 }
 
-__global__ void __launch_bounds__(256) k_gather(uint64_t M, const uint32_t *__restrict__ sorted_p,
-                                                const uint32_t *__restrict__ b2, uint32_t k,
-                                                uint32_t *__restrict__ positions, uint16_t *__restrict__ lseeds)
+/* One thread per sampled start p = 0, 4, 8, ... <= N-k (in genome order, so
+ * the plane reads are coalesced): the k-mer code and N-freeness of the
+ * histogram (:253-269) as the sort key (4^k for a start with an N), and the
+ * value (residual << 32 | p) that the stable sort carries into bucket
+ * order. */
+__global__ void __launch_bounds__(256) k_keys(const char *__restrict__ seq, uint64_t nsamp, uint32_t k,
+                                              const uint32_t *__restrict__ b2, uint32_t *__restrict__ keys,
+                                              unsigned long long *__restrict__ vals)
 {
-    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nsamp) return;
+    const uint64_t p = j * SWM_SEED_STEP;
+    uint32_t code = 0;
+    bool valid = true;
+    for (uint32_t i = 0; i < k; i++) {
+        int c = base_code(seq[p + i]);
+        if (c < 0) valid = false;
+        code |= (uint32_t)(c < 0 ? 0 : c) << (2 * i);
+    }
+    keys[j] = valid ? code : (1u << (2 * k));
+    vals[j] = ((unsigned long long)lseed_at(b2, p, k) << 32) | (uint32_t)p;
+}
+
+/* Bucket counts from the sorted keys: the thread at the end of each run of
+ * one valid code gallops back to the run's start (runs average a few
+ * entries; long ones cost log steps). */
+__global__ void __launch_bounds__(256) k_runs(uint64_t nsamp, uint32_t nb, const uint32_t *__restrict__ key,
+                                              uint32_t *__restrict__ count)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nsamp) return;
+    const uint32_t c = key[i];
+    if (c >= nb || (i + 1 < nsamp && key[i + 1] == c)) return;
+    uint64_t lo = i, step = 1;             /* key[lo] == c */
+    while (step <= lo && key[lo - step] == c) { lo -= step; step <<= 1; }
+    uint64_t a = step <= lo ? lo - step : 0;   /* key[a] != c or a == 0 */
+    if (key[a] == c) lo = a;
+    else {
+        while (lo - a > 1) {               /* key[a] != c, key[lo] == c */
+            const uint64_t mid = a + (lo - a) / 2;
+            if (key[mid] == c) lo = mid; else a = mid;
+        }
+    }
+    count[c] = (uint32_t)(i + 1 - lo);
+}
+
+/* positions and residuals in bucket order */
+__global__ void __launch_bounds__(256) k_split(uint64_t M, const unsigned long long *__restrict__ v,
+                                               uint32_t *__restrict__ positions, uint16_t *__restrict__ lseeds)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= M) return;
-    uint32_t p = sorted_p[i];
-    positions[i] = p;
-    lseeds[i] = (uint16_t)lseed_at(b2, p, k);
+    const unsigned long long x = v[i];
+    positions[i] = (uint32_t)x;
+    lseeds[i] = (uint16_t)(x >> 32);
 }
 
 __global__ void k_end_offsets(uint64_t nb, const uint32_t *__restrict__ start, uint32_t *__restrict__ end)
@@ -253,7 +281,8 @@ extern "C" int swm_index_build_mem(const char *seq, uint64_t n, uint32_t k, swm_
     struct SG { hipStream_t s; ~SG() { (void)hipStreamDestroy(s); } } sg{st};
     DevBufs B;
     char *d_seq;
-    uint32_t *b2, *b3, *s2, *cnt, *start, *end, *kin, *kout, *vin, *vout, *pos;
+    uint32_t *b2, *b3, *s2, *cnt, *start, *end, *kin, *kout, *pos;
+    unsigned long long *vin, *vout;
     uint16_t *ls;
     if (!B.get(&d_seq, n + 64) || !B.get(&b2, 2 * nwords + 2 * SWM_WIN_PAIRS_PAD) ||
         !B.get(&b3, nwords + SWM_WIN_PAIRS_PAD) || !B.get(&s2, 2 * nwords + 2 * SWM_WIN_PAIRS_PAD) ||
@@ -272,21 +301,25 @@ extern "C" int swm_index_build_mem(const char *seq, uint64_t n, uint32_t k, swm_
     struct EG { hipEvent_t a, b; ~EG() { (void)hipEventDestroy(a); (void)hipEventDestroy(b); } } eg{e0, e1};
     SWM_HIP_CHECK(hipEventRecord(e0, st));
     k_encode<<<nblk(nwords, 256), 256, 0, st>>>(d_seq, n, nwords, b2, b3, s2);
-    k_keys<<<nblk(nsamp, 256), 256, 0, st>>>(d_seq, nsamp, k, kin, vin, cnt);
+    k_keys<<<nblk(nsamp, 256), 256, 0, st>>>(d_seq, nsamp, k, b2, kin, vin);
     SWM_HIP_CHECK(hipGetLastError());
     size_t t1 = 0, t2 = 0;
     SWM_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, t1, cnt, start, nb + 1, st));
     SWM_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, t2, kin, kout, vin, vout, nsamp, 0, (int)(2 * k + 1), st));
     void *tmp;
     if (!B.get((char **)&tmp, t1 > t2 ? t1 : t2)) return SWM_ENOMEM;
-    SWM_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(tmp, t1, cnt, start, nb + 1, st));
+    /* stable sort by code (the counting sort's order, positions ascending),
+     * counts from the sorted runs, offsets by scan */
     SWM_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(tmp, t2, kin, kout, vin, vout, nsamp, 0, (int)(2 * k + 1), st));
+    k_runs<<<nblk(nsamp, 256), 256, 0, st>>>(nsamp, (uint32_t)nb, kout, cnt);
+    SWM_HIP_CHECK(hipGetLastError());
+    SWM_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(tmp, t1, cnt, start, nb + 1, st));
     uint32_t M32 = 0;
     SWM_HIP_CHECK(hipMemcpyAsync(&M32, start + nb, 4, hipMemcpyDeviceToHost, st));
     SWM_HIP_CHECK(hipStreamSynchronize(st));
     const uint64_t M = M32;
     if (!B.get(&pos, M + 64) || !B.get(&ls, M + 64)) return SWM_ENOMEM;
-    k_gather<<<nblk(M ? M : 1, 256), 256, 0, st>>>(M, vout, b2, k, pos, ls);
+    k_split<<<nblk(M ? M : 1, 256), 256, 0, st>>>(M, vout, pos, ls);
     k_end_offsets<<<nblk(nb + 1, 256), 256, 0, st>>>(nb, start, end);
     const uint64_t t0 = (n > 64 + (uint64_t)k) ? ((n - 64 - k) & ~(uint64_t)3) : 0;
     k_tail<<<1, 1, 0, st>>>(d_seq, n, k, t0, b2, b3, s2, cnt, start, M, pos, ls, end);

@@ -28,6 +28,7 @@
  */
 #include "swm_internal.h"
 
+#include <algorithm>
 #include <new>
 #include <stdlib.h>
 #include <string.h>
@@ -124,6 +125,26 @@ __device__ __forceinline__ T wave_incl_scan(T v)
         if (lane >= (uint32_t)d) v += u;
     }
     return v;
+}
+
+/* Inclusive scan over the workgroup (NT threads); s_w holds NT/64 values. */
+template <typename T, int NT>
+__device__ __forceinline__ T block_incl_scan(T v, T *s_w, T *total)
+{
+    const uint32_t lane = __lane_id(), w = threadIdx.x >> 6;
+    v = wave_incl_scan(v);
+    if (lane == 63) s_w[w] = v;
+    __syncthreads();
+    T off = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < NT / 64; i++) {
+        const T x = s_w[i];
+        if ((uint32_t)i < w) off += x;
+        tot += x;
+    }
+    __syncthreads();
+    *total = tot;
+    return v + off;
 }
 
 /* ------------------------------------------------------------------ k_seed */
@@ -558,34 +579,27 @@ __global__ void __launch_bounds__(64) k_plan_plain(uint32_t nrs, const uint2 *__
  * positions together. */
 #define HK 16          /* buffered chunk keys per selected 12-mer */
 
+/* Lane reads (v_readlane) and writes (a compare + select) at a wave-uniform
+ * lane index: the seed heap and the per-seed state live one entry per lane,
+ * so the replay is scalar code with no LDS round trip per heap level. */
+__device__ __forceinline__ uint32_t rl(uint32_t v, uint32_t l)
+{
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)l);
+}
+__device__ __forceinline__ uint32_t wl(uint32_t v, uint32_t l, uint32_t x)
+{
+    return __lane_id() == l ? x : v;
+}
+
 __global__ void __launch_bounds__(64) k_plan_heavy(const uint2 *__restrict__ pl, PlanIO io)
 {
-    __shared__ swm_sel_t ssel[64];
-    __shared__ uint32_t spc[64];                 /* positions fetched per 12-mer */
-    __shared__ uint32_t skey[64 * HK];           /* keys of chunks skbase[i] .. skbase[i]+HK-1 */
-    __shared__ uint32_t skbase[64];
-    __shared__ uint32_t sstage[64];
-    __shared__ uint32_t hloc[66], hind[66];
+    __shared__ uint32_t skey[64 * HK];           /* keys of chunks kb .. kb+HK-1 of each 12-mer */
+    __shared__ uint32_t smeta[64];
     __shared__ uint32_t cseed[SWM_MAX_BLOCKS], cg[SWM_MAX_BLOCKS], cpre[SWM_MAX_BLOCKS + 1];
-    __shared__ uint32_t sv[8];                   /* broadcast slots */
+    __shared__ uint32_t sv[1];
     const uint32_t lane = threadIdx.x;
+    const uint64_t below = (1ull << lane) - 1ull;
     const uint32_t nq = *io.heavy_count;
-
-    /* heap key after fetching chunk c of 12-mer i: the raw last position of
-     * that chunk (seeds_heap_pop_and_insert's pos argument, :2350) */
-    auto key_addr = [&](uint32_t i, uint32_t c) -> uint32_t {
-        const uint32_t end = min(SWM_CHUNK * (c + 1), ssel[i].cnt);
-        return ssel[i].off + end - 1;
-    };
-    auto refill = [&](uint32_t i) {              /* keys of the next HK chunks */
-        const uint32_t kb = spc[i] / SWM_CHUNK;
-        skbase[i] = kb;
-#pragma unroll
-        for (uint32_t k = 0; k < HK; k++) {
-            const uint32_t c = kb + k;
-            skey[i * HK + k] = (SWM_CHUNK * c < ssel[i].cnt) ? pl[key_addr(i, c)].x : 0u;
-        }
-    };
 
     for (;;) {
         if (lane == 0) sv[0] = atomicAdd(io.heavy_head, 1u);
@@ -595,86 +609,110 @@ __global__ void __launch_bounds__(64) k_plan_heavy(const uint2 *__restrict__ pl,
         if (qi >= nq) break;
         const uint32_t r = io.heavy[qi];
         const uint32_t nsel = io.nsel[r] & ~NSEL_HEAVY;
-        if (lane < nsel) ssel[lane] = io.sel[(uint64_t)r * io.seln + lane];
-        __syncthreads();
+        const bool act = lane < nsel;
+        /* per 12-mer state, lane i = selected 12-mer i */
+        swm_sel_t my = {0, 0, 0};
+        if (act) my = io.sel[(uint64_t)r * io.seln + lane];
+        const uint32_t cnt_v = my.cnt, off_v = my.off;
+        smeta[lane] = my.meta;
+        uint32_t spc_v = min(cnt_v, (uint32_t)SWM_CHUNK);     /* positions fetched */
+        uint32_t kb_v = 0;                                     /* first chunk in the key buffer */
+        /* heap key after fetching chunk c: the raw last position of that
+         * chunk (seeds_heap_pop_and_insert's pos argument, :2350) */
+        auto refill = [&]() {
+            if (act) {
+                kb_v = spc_v / SWM_CHUNK;
+#pragma unroll
+                for (uint32_t k = 0; k < HK; k++) {
+                    const uint32_t c = kb_v + k;
+                    skey[lane * HK + k] =
+                        (SWM_CHUNK * c < cnt_v) ? pl[off_v + min(SWM_CHUNK * (c + 1), cnt_v) - 1].x : 0u;
+                }
+            }
+        };
+        refill();
         uint32_t *surv = io.surv + io.surv_off[r];
         uint2 *bd = io.bdesc + io.bd_off[r];
 
-        /* phase 1 (:2194-2236): the first chunk of every 12-mer; the heap
-         * key of those with more hits (:2244-2252) */
-        if (lane < nsel) {
-            spc[lane] = min(ssel[lane].cnt, (uint32_t)SWM_CHUNK);
-            sstage[lane] = ssel[lane].cnt > SWM_CHUNK ? pl[ssel[lane].off + SWM_CHUNK - 1].x : 0u;
-            refill(lane);
+        /* phase 1 (:2194-2236): the first chunk of every 12-mer, in order */
+        const uint32_t get1 = act ? spc_v : 0u;
+        const uint64_t has = __ballot(get1 > 0);
+        const uint32_t incl = wave_incl_scan<uint32_t>(get1);
+        if (get1 > 0) {
+            const uint32_t slot = (uint32_t)__popcll(has & below);
+            cseed[slot] = lane;
+            cg[slot] = off_v;
+            cpre[slot] = incl - get1;
         }
-        __syncthreads();
-        uint32_t nch = 0, pos_size = 0, blocks = 0;
-        if (lane == 0) {
-            for (uint32_t i = 0; i < nsel; i++) {
-                const uint32_t get = spc[i];
-                if (get > 0) { cseed[nch] = i; cg[nch] = ssel[i].off; cpre[nch] = pos_size; nch++; blocks++; }
-                pos_size += get;
+        uint32_t nch = (uint32_t)__popcll(has), blocks = nch;
+        uint32_t pos_size = rl(incl, 63);
+        /* the heap (:2244-2252): 12-mers with more hits, inserted in order
+         * (seeds_heap_insert :1919-1935); heap slot x lives in lane x - 1 */
+        const uint32_t stage_v = (act && cnt_v > SWM_CHUNK) ? pl[off_v + SWM_CHUNK - 1].x : 0u;
+        uint32_t hloc = 0, hind = 0, hsize = 0;
+        for (uint64_t m = __ballot(act && spc_v != cnt_v); m; m &= m - 1ull) {
+            const uint32_t i = (uint32_t)__ffsll((unsigned long long)m) - 1u;
+            hsize++;
+            hloc = wl(hloc, hsize - 1, rl(stage_v, i));
+            hind = wl(hind, hsize - 1, i);
+            uint32_t x = hsize;
+            while (x > 1) {
+                const uint32_t t = x / 2;
+                const uint32_t lt = rl(hloc, t - 1), lx = rl(hloc, x - 1);
+                if (lt > lx) {
+                    const uint32_t it = rl(hind, t - 1), ix = rl(hind, x - 1);
+                    hloc = wl(wl(hloc, t - 1, lx), x - 1, lt);
+                    hind = wl(wl(hind, t - 1, ix), x - 1, it);
+                } else break;
+                x = t;
             }
-            uint32_t hsize = 0;
-            for (uint32_t i = 0; i < nsel; i++) {
-                if (spc[i] != ssel[i].cnt) {                    /* seeds_heap_insert (:1919-1935) */
-                    hsize++;
-                    hloc[hsize] = sstage[i]; hind[hsize] = i;
-                    uint32_t x = hsize;
-                    while (x > 1) {
-                        uint32_t t = x / 2;
-                        if (hloc[t] > hloc[x]) {
-                            uint32_t a0 = hloc[t]; hloc[t] = hloc[x]; hloc[x] = a0;
-                            a0 = hind[t]; hind[t] = hind[x]; hind[x] = a0;
+        }
+        __syncthreads();   /* key buffers and the phase-1 chunk list are read below */
+        uint32_t nsurv = 0, batch = 0;
+        for (;;) {
+            /* ---- phase A: the rest of this batch's chunk list (:2261-2356), wave-uniform ---- */
+            for (;;) {
+                if (hsize > 0 && pos_size + SWM_CHUNK <= SWM_BATCH_POS && blocks < SWM_MAX_BLOCKS) {
+                    const uint32_t i = rl(hind, 0);
+                    const uint32_t cnt = rl(cnt_v, i), pc = rl(spc_v, i), off = rl(off_v, i);
+                    const uint32_t c = pc / SWM_CHUNK;            /* chunk fetched now */
+                    const uint32_t get = min((uint32_t)SWM_CHUNK, cnt - pc);
+                    if (lane == 0) { cseed[nch] = i; cg[nch] = off + pc; cpre[nch] = pos_size; }
+                    nch++;
+                    pos_size += get;
+                    blocks++;
+                    spc_v = wl(spc_v, i, pc + get);
+                    if (pc + get == cnt) {                        /* seeds_heap_pop_and_insert (:1936-1962, Q6) */
+                        hloc = wl(hloc, 0, rl(hloc, hsize - 1));
+                        hind = wl(hind, 0, rl(hind, hsize - 1));
+                        hsize--;
+                    } else {
+                        const uint32_t kb = rl(kb_v, i);
+                        uint32_t key;
+                        if (c >= kb && c < kb + HK) key = skey[i * HK + (c - kb)];
+                        else key = pl[off + min(SWM_CHUNK * (c + 1), cnt) - 1].x;
+                        hloc = wl(hloc, 0, key);
+                    }
+                    uint32_t x = 1;
+                    while (x * 2 < hsize) {
+                        uint32_t t = x * 2;
+                        const uint32_t a2 = rl(hloc, 2 * x - 1), a3 = rl(hloc, 2 * x);
+                        if (x * 2 + 1 <= hsize && a2 > a3) t = x * 2 + 1;
+                        const uint32_t lt = rl(hloc, t - 1), lx = rl(hloc, x - 1);
+                        if (lt < lx) {
+                            const uint32_t it = rl(hind, t - 1), ix = rl(hind, x - 1);
+                            hloc = wl(wl(hloc, t - 1, lx), x - 1, lt);
+                            hind = wl(wl(hind, t - 1, ix), x - 1, it);
                         } else break;
                         x = t;
                     }
                 }
+                if (pos_size + SWM_CHUNK > SWM_BATCH_POS || hsize == 0 || blocks >= SWM_MAX_BLOCKS) break;
             }
-            sv[1] = hsize;
-        }
-        __syncthreads();
-        uint32_t nsurv = 0, batch = 0;
-        for (;;) {
-            /* ---- phase A (lane 0): the rest of this batch's chunk list ---- */
-            if (lane == 0) {
-                uint32_t hsize = sv[1];
-                for (;;) {
-                    if (hsize > 0 && pos_size + SWM_CHUNK <= SWM_BATCH_POS && blocks < SWM_MAX_BLOCKS) {
-                        const uint32_t i = hind[1];
-                        const uint32_t cnt = ssel[i].cnt, pc = spc[i];
-                        const uint32_t c = pc / SWM_CHUNK;           /* chunk fetched now */
-                        const uint32_t get = min((uint32_t)SWM_CHUNK, cnt - pc);
-                        cseed[nch] = i; cg[nch] = ssel[i].off + pc; cpre[nch] = pos_size; nch++;
-                        pos_size += get;
-                        blocks++;
-                        spc[i] = pc + get;
-                        if (spc[i] == cnt) {                          /* seeds_heap_pop_and_insert (:1936-1962, Q6) */
-                            hloc[1] = hloc[hsize]; hind[1] = hind[hsize]; hsize--;
-                        } else {
-                            const uint32_t kb = skbase[i];
-                            hloc[1] = (c >= kb && c < kb + HK) ? skey[i * HK + (c - kb)] : pl[key_addr(i, c)].x;
-                        }
-                        uint32_t x = 1;
-                        while (x * 2 < hsize) {
-                            uint32_t t = x * 2;
-                            if (x * 2 + 1 <= hsize && hloc[x * 2] > hloc[x * 2 + 1]) t = x * 2 + 1;
-                            if (hloc[t] < hloc[x]) {
-                                uint32_t a0 = hloc[t]; hloc[t] = hloc[x]; hloc[x] = a0;
-                                a0 = hind[t]; hind[t] = hind[x]; hind[x] = a0;
-                            } else break;
-                            x = t;
-                        }
-                    }
-                    if (pos_size + SWM_CHUNK > SWM_BATCH_POS || hsize == 0 || blocks >= SWM_MAX_BLOCKS) break;
-                }
-                cpre[nch] = pos_size;
-                sv[1] = hsize;
-                sv[2] = nch;
-                sv[3] = pos_size;
-            }
+            if (lane == 0) cpre[nch] = pos_size;
+            const uint32_t bn = nch, bpos = pos_size;
             __syncthreads();
-            const uint32_t bn = sv[2], bpos = sv[3], hsize = sv[1];
+
             /* ---- phase B (all lanes): fetch, filter, compact ---- */
             const uint32_t bstart = nsurv;
             for (uint32_t e0 = 0; e0 < bpos; e0 += 64) {
@@ -688,7 +726,7 @@ __global__ void __launch_bounds__(64) k_plan_heavy(const uint2 *__restrict__ pl,
                         if (cpre[mid] <= ent) lo = mid; else hi = mid;
                     }
                     const uint2 v = pl[cg[lo] + (ent - cpre[lo])];
-                    const uint32_t meta = ssel[cseed[lo]].meta;
+                    const uint32_t meta = smeta[cseed[lo]];
                     keep = filter_pass(v.y, meta);
                     val = adjust(v.x, meta);
                 }
@@ -722,7 +760,7 @@ __global__ void __launch_bounds__(64) k_plan_heavy(const uint2 *__restrict__ pl,
             batch++;
             if (hsize == 0) break;
             /* next batch: refill the key buffers, start an empty chunk list */
-            if (lane < nsel) refill(lane);
+            refill();
             nch = 0; pos_size = 0; blocks = 0;
             __syncthreads();
         }
@@ -731,58 +769,99 @@ __global__ void __launch_bounds__(64) k_plan_heavy(const uint2 *__restrict__ pl,
     }
 }
 
-/* One workgroup per large batch (<= 2048 values): bitonic sort in LDS, then
- * unique by a block-wide prefix over "differs from the left neighbour". */
+/* One workgroup per large batch (<= 2048 values): bitonic sort with 8
+ * values per thread in registers (element i = thread * 8 + register).
+ * Compare-exchange partners i ^ j are in the same thread for j < 8, in the
+ * same wave for j < 512 (shuffles), and only the stages with j >= 512 go
+ * through LDS (laid out register-major, so conflict-free).  Then unique by a
+ * block-wide scan over "differs from the left neighbour". */
+__device__ __forceinline__ void cas_reg(uint32_t &x, uint32_t &y, bool asc)
+{
+    const uint32_t lo = min(x, y), hi = max(x, y);
+    x = asc ? lo : hi;
+    y = asc ? hi : lo;
+}
+
+template <int J>
+__device__ __forceinline__ void bitonic_in_thread(uint32_t (&v)[8], uint32_t i0, uint32_t kk)
+{
+#pragma unroll
+    for (int r = 0; r < 8; r++)
+        if ((r & J) == 0) cas_reg(v[r], v[r | J], ((i0 + (uint32_t)r) & kk) == 0);
+}
+
 __global__ void __launch_bounds__(256) k_bigsort(const uint2 *__restrict__ big, const uint32_t *__restrict__ big_count,
                                                  const uint64_t *__restrict__ surv_off,
                                                  const uint32_t *__restrict__ bd_off, uint32_t *__restrict__ surv,
                                                  uint2 *__restrict__ bdesc, uint32_t *__restrict__ nuniq)
 {
-    __shared__ uint32_t a[SWM_BATCH_POS];
-    __shared__ uint32_t part[256];
+    __shared__ uint32_t xs[8 * 256];
+    __shared__ uint32_t s_w[4], s_edge[4];
     const uint32_t nb = *big_count;
-    const uint32_t tid = threadIdx.x;
-    for (uint32_t b = blockIdx.x; b < nb; b += gridDim.x) {
-        const uint2 rb = big[b];
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
+    const uint32_t i0 = tid * 8u;
+    for (uint32_t bt = blockIdx.x; bt < nb; bt += gridDim.x) {
+        const uint2 rb = big[bt];
         uint2 *dp = bdesc + bd_off[rb.x] + rb.y;
         const uint2 d = *dp;
         const uint32_t n = d.y;
         uint32_t *base = surv + surv_off[rb.x] + d.x;
         uint32_t P = 64;
         while (P < n) P <<= 1;
-        for (uint32_t i = tid; i < P; i += 256) a[i] = i < n ? base[i] : 0xffffffffu;
-        __syncthreads();
+        uint32_t v[8];
+#pragma unroll
+        for (int r = 0; r < 8; r++) v[r] = i0 + (uint32_t)r < n ? base[i0 + r] : 0xffffffffu;
         for (uint32_t kk = 2; kk <= P; kk <<= 1) {
             for (uint32_t j = kk >> 1; j > 0; j >>= 1) {
-                for (uint32_t i = tid; i < P; i += 256) {
-                    const uint32_t ixj = i ^ j;
-                    if (ixj > i) {
-                        const bool up = (i & kk) == 0;
-                        const uint32_t x = a[i], y = a[ixj];
-                        if ((x > y) == up) { a[i] = y; a[ixj] = x; }
+                if (j >= 512) {
+#pragma unroll
+                    for (int r = 0; r < 8; r++) xs[r * 256 + tid] = v[r];
+                    __syncthreads();
+                    const uint32_t pt = tid ^ (j >> 3);
+                    const bool keep_min = ((i0 & j) == 0) == ((i0 & kk) == 0);
+#pragma unroll
+                    for (int r = 0; r < 8; r++) {
+                        const uint32_t o = xs[r * 256 + pt];
+                        v[r] = keep_min ? min(v[r], o) : max(v[r], o);
                     }
+                    __syncthreads();
+                } else if (j >= 8) {
+                    const int m = (int)(j >> 3);
+                    const bool keep_min = ((i0 & j) == 0) == ((i0 & kk) == 0);
+#pragma unroll
+                    for (int r = 0; r < 8; r++) {
+                        const uint32_t o = (uint32_t)__shfl_xor((int)v[r], m, 64);
+                        v[r] = keep_min ? min(v[r], o) : max(v[r], o);
+                    }
+                } else if (j == 4) {
+                    bitonic_in_thread<4>(v, i0, kk);
+                } else if (j == 2) {
+                    bitonic_in_thread<2>(v, i0, kk);
+                } else {
+                    bitonic_in_thread<1>(v, i0, kk);
                 }
-                __syncthreads();
             }
         }
-        /* thread tid owns elements [tid*per, tid*per + per) */
-        const uint32_t per = (P + 255) / 256;
-        const uint32_t lo = tid * per;
+        /* unique: element i is kept when i < n and it differs from i - 1 */
+        uint32_t prev = (uint32_t)__shfl_up((int)v[7], 1, 64);
+        if (lane == 63) s_edge[w] = v[7];
+        __syncthreads();
+        if (lane == 0) prev = w ? s_edge[w - 1] : 0u;
         uint32_t cnt = 0;
-        for (uint32_t i = lo; i < lo + per && i < n; i++) cnt += (i == 0 || a[i] != a[i - 1]) ? 1u : 0u;
-        part[tid] = cnt;
-        __syncthreads();
-        for (uint32_t off = 1; off < 256; off <<= 1) {
-            uint32_t v = tid >= off ? part[tid - off] : 0u;
-            __syncthreads();
-            part[tid] += v;
-            __syncthreads();
+#pragma unroll
+        for (int r = 0; r < 8; r++) {
+            const uint32_t i = i0 + (uint32_t)r;
+            const uint32_t left = r ? v[r - 1] : prev;
+            cnt += (i < n && (i == 0 || v[r] != left)) ? 1u : 0u;
         }
-        uint32_t w = part[tid] - cnt;
-        const uint32_t total = part[255];
-        for (uint32_t i = lo; i < lo + per && i < n; i++)
-            if (i == 0 || a[i] != a[i - 1]) base[w++] = a[i];
-        __syncthreads();
+        uint32_t total;
+        uint32_t wpos = block_incl_scan<uint32_t, 256>(cnt, s_w, &total) - cnt;
+#pragma unroll
+        for (int r = 0; r < 8; r++) {
+            const uint32_t i = i0 + (uint32_t)r;
+            const uint32_t left = r ? v[r - 1] : prev;
+            if (i < n && (i == 0 || v[r] != left)) base[wpos++] = v[r];
+        }
         if (tid == 0) {
             dp->y = total;
             atomicSub(&nuniq[rb.x], n - total);
@@ -1034,39 +1113,79 @@ __device__ __forceinline__ bool hamming_one(uint32_t i, int32_t er, const uint32
 }
 
 /* Per candidate: the Hamming check of passing lanes, the traceback list, the
- * count of rejected windows and the band steps of the verified ones.  The
- * trip count is block-uniform, so every thread reaches block_append. */
+ * count of rejected windows and the band steps of the verified ones.  A
+ * fixed grid loops over the candidates (block-uniform trip count); counts
+ * stay in registers and the traceback list in an LDS buffer flushed to the
+ * global list when nearly full, so a workgroup issues a handful of global
+ * atomics in its lifetime instead of a few per 256 candidates (same-address
+ * atomics serialise at one L2 channel). */
+#define TRACE_BUF 2048
+
 __global__ void __launch_bounds__(256) k_hamming(const uint32_t *__restrict__ ncand_p, const uint32_t *__restrict__ lens,
                                                  const uint32_t *__restrict__ words, const uint32_t *__restrict__ b2,
                                                  uint64_t ref_length, uint32_t e, CandSet c,
                                                  uint32_t *__restrict__ trace, uint32_t *__restrict__ trace_count,
                                                  uint32_t *__restrict__ n_rejected, unsigned long long *n_steps)
 {
-    __shared__ uint32_t s_tmp[2];
-    __shared__ uint32_t s_steps, s_rej;
+    __shared__ uint32_t s_list[TRACE_BUF];
+    __shared__ uint32_t s_n, s_base, s_rej;
+    __shared__ unsigned long long s_steps;
     if (*c.ovf) return;
     const uint32_t ncand = *ncand_p;
+    const uint32_t tid = threadIdx.x, lane = __lane_id();
+    const uint64_t below = (1ull << lane) - 1ull;
+    if (tid == 0) { s_n = 0; s_rej = 0; s_steps = 0; }
+    __syncthreads();
+    uint64_t my_steps = 0;
+    uint32_t my_rej = 0;
+    auto flush = [&]() {                       /* block-uniform */
+        const uint32_t n = s_n;
+        if (n) {
+            if (tid == 0) s_base = atomicAdd(trace_count, n);
+            __syncthreads();
+            for (uint32_t k = tid; k < n; k += blockDim.x) trace[s_base + k] = s_list[k];
+            __syncthreads();
+            if (tid == 0) s_n = 0;
+        }
+        __syncthreads();
+    };
     for (uint32_t b0 = blockIdx.x * blockDim.x; b0 < ncand; b0 += gridDim.x * blockDim.x) {
-        const uint32_t i = b0 + threadIdx.x;
-        bool to_trace = false, rej = false;
-        uint32_t steps = 0;
-        if (threadIdx.x == 0) { s_steps = 0; s_rej = 0; }
+        const uint32_t i = b0 + tid;
+        bool to_trace = false;
         if (i < ncand) {
             const int32_t er = c.err[i];
             if (er == -2) {
-                rej = true;                    /* pos_start rejected the window: not a lane */
+                my_rej++;                          /* pos_start rejected the window: not a lane */
             } else {
-                steps = lens[c.rs[i] >> 1];
+                my_steps += lens[c.rs[i] >> 1];
                 if (er <= (int32_t)e) to_trace = hamming_one(i, er, lens, words, b2, ref_length, e, c);
             }
         }
-        block_append(to_trace, i, trace, trace_count, s_tmp);   /* its barriers order the resets above */
-        if (steps) atomicAdd(&s_steps, steps);
-        if (rej) atomicAdd(&s_rej, 1u);
+        const uint64_t m = __ballot(to_trace);
+        uint32_t woff = 0;
+        if (lane == 0 && m) woff = atomicAdd(&s_n, (uint32_t)__popcll(m));
+        woff = (uint32_t)__shfl((int)woff, 0, 64);
+        if (to_trace) s_list[woff + (uint32_t)__popcll(m & below)] = i;
         __syncthreads();
-        if (threadIdx.x == 0 && s_steps) atomicAdd(n_steps, (unsigned long long)s_steps);
-        if (threadIdx.x == 0 && s_rej) atomicAdd(n_rejected, s_rej);
-        __syncthreads();
+        if (s_n > TRACE_BUF - 256) flush();
+    }
+    flush();
+    /* counts: wave sums, then one global atomic each per workgroup */
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) {
+        const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)my_steps, d, 64);
+        const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(my_steps >> 32), d, 64);
+        my_steps += ((uint64_t)hi << 32) | lo;
+        my_rej += (uint32_t)__shfl_xor((int)my_rej, d, 64);
+    }
+    if (lane == 0) {
+        if (my_steps) atomicAdd(&s_steps, (unsigned long long)my_steps);
+        if (my_rej) atomicAdd(&s_rej, my_rej);
+    }
+    __syncthreads();
+    if (tid == 0) {
+        if (s_steps) atomicAdd(n_steps, s_steps);
+        if (s_rej) atomicAdd(n_rejected, s_rej);
     }
 }
 
@@ -1505,26 +1624,6 @@ __global__ void __launch_bounds__(64) k_emit_heavy(const uint32_t *__restrict__ 
 #define SCAN_PER 16
 #define SCAN_TILE (SCAN_NT * SCAN_PER)
 
-/* Inclusive scan over the workgroup (NT threads); s_w holds NT/64 values. */
-template <typename T, int NT>
-__device__ __forceinline__ T block_incl_scan(T v, T *s_w, T *total)
-{
-    const uint32_t lane = __lane_id(), w = threadIdx.x >> 6;
-    v = wave_incl_scan(v);
-    if (lane == 63) s_w[w] = v;
-    __syncthreads();
-    T off = 0, tot = 0;
-#pragma unroll
-    for (int i = 0; i < NT / 64; i++) {
-        const T x = s_w[i];
-        if ((uint32_t)i < w) off += x;
-        tot += x;
-    }
-    __syncthreads();
-    *total = tot;
-    return v + off;
-}
-
 /* One scan job covers up to two arrays of the same length (blockIdx.y picks
  * the array): array 0 is u64, array 1 is u32; tile sums are u64 for both.
  * The top kernel also checks each total against its capacity (setting the
@@ -1892,7 +1991,8 @@ static int enq_c(const swm_ctx_t *ctx, Pipe &P, const uint32_t *d_lens, uint32_t
                                                                       ctx->ref_length, e, P.cs);
     SWM_HIP_CHECK(hipGetLastError());
     if (ev) SWM_HIP_CHECK(hipEventRecord(ev[4], st));
-    k_hamming<<<blocks_for(grid_cand ? grid_cand : 1, 256), 256, 0, st>>>(ncand, d_lens, P.so.words,
+    const uint32_t ham_grid = std::min<uint32_t>(blocks_for(grid_cand ? grid_cand : 1, 256), (uint32_t)ctx->n_cu * 8);
+    k_hamming<<<ham_grid, 256, 0, st>>>(ncand, d_lens, P.so.words,
                                                                           ctx->d_base_2bits, ctx->ref_length, e, P.cs,
                                                                           P.trace, P.u32c + 1, P.u32c + 2,
                                                                           P.counters + 2);
