@@ -296,6 +296,13 @@ def main():
         for s in st_all:
             log("  " + " ".join(f"{k}={s[k]:.2f}" for k in ("ms_seed", "ms_plan", "ms_compact", "ms_myers",
                                                              "ms_emit", "ms_total")))
+    # per-read candidate histogram of the workload (one extra run, synced form)
+    m.set_option(swm.OPT_GRAPH, 0)
+    m.set_option(swm.OPT_HISTOGRAM, 1)
+    hs = step()
+    m.set_option(swm.OPT_HISTOGRAM, 0)
+    hist = {("0" if b == 0 else f"{1 << (b - 1)}-{(1 << b) - 1}"): int(v) for b, v in enumerate(hs["cand_hist"]) if v}
+    log(f"candidates per read: {hist}, max {hs['max_cands']}")
     best = min(results, key=lambda f: results[f]["elapsed"])
     R = results[best]
     nv = R["nv"]
@@ -391,6 +398,7 @@ def main():
         "scaling": "weak", "vs_baseline": None, "dtype": "u32", "data": "synthetic",
         "reads_per_s": round(reads_per_s, 1), "verifications_per_step": nv,
         "verifications_per_read": round(nv / n, 3),
+        "candidates_per_read_hist": {"bins": hist, "max": int(hs["max_cands"])},
         "batch_latency_ms": lat_block,
         "forms": {f: {"ms_per_step": round(r["elapsed"] * 1e3 / args.steps, 3),
                       "verifications_per_s": round(r["nv"] * args.steps / r["elapsed"], 1),

@@ -75,6 +75,10 @@ typedef struct swm_stats {
     /* host wall clock of the call: enqueueing (input staging, copies,
      * launches, readbacks between stages) and the final wait */
     double ms_host_enqueue, ms_host_wait;
+    /* with SWM_OPT_HISTOGRAM: reads by candidate count over both strands,
+     * bin 0 = none, bin b = [2^(b-1), 2^b); and the largest count */
+    uint64_t cand_hist[24];
+    uint64_t max_cands;
 } swm_stats_t;
 
 typedef struct swm_ctx swm_ctx_t;
@@ -138,6 +142,7 @@ int swm_last_stats(const swm_ctx_t *ctx, swm_stats_t *st);
 #define SWM_OPT_MYERS_BLOCK 4       /* verification workgroup size: 64, 128 or 256 (default) */
 #define SWM_OPT_PLAN_HEAVY_WAVES 5  /* persistent plan waves per CU (1..32, default 8) */
 #define SWM_OPT_EMIT_HEAVY_WAVES 6  /* cooperative emission waves per CU (1..64, default 32) */
+#define SWM_OPT_HISTOGRAM 7         /* 1: also collect the per-read candidate histogram (stats) */
 
 /* Every form and tunable gives the same results; they change only speed. */
 int swm_ctx_set_option(swm_ctx_t *ctx, int opt, int64_t value);

@@ -48,10 +48,13 @@ class Stats(C.Structure):
                 ("n_records", C.c_uint64), ("n_positions", C.c_uint64),
                 ("ms_seed", C.c_double), ("ms_plan", C.c_double), ("ms_compact", C.c_double),
                 ("ms_myers", C.c_double), ("ms_emit", C.c_double), ("ms_total", C.c_double),
-                ("ms_host_enqueue", C.c_double), ("ms_host_wait", C.c_double)]
+                ("ms_host_enqueue", C.c_double), ("ms_host_wait", C.c_double),
+                ("cand_hist", C.c_uint64 * 24), ("max_cands", C.c_uint64)]
 
     def as_dict(self):
-        return {f: getattr(self, f) for f, _ in self._fields_}
+        d = {f: getattr(self, f) for f, _ in self._fields_}
+        d["cand_hist"] = list(self.cand_hist)
+        return d
 
 
 class ServerConfig(C.Structure):
@@ -68,6 +71,7 @@ class BatchInfo(C.Structure):
 
 # forms and tunables (swm_ctx_set_option)
 OPT_WORK_QUEUE, OPT_GRAPH, OPT_HEAVY_CANDS, OPT_MYERS_BLOCK, OPT_PLAN_HEAVY_WAVES, OPT_EMIT_HEAVY_WAVES = range(1, 7)
+OPT_HISTOGRAM = 7
 OPTION_NAMES = {OPT_WORK_QUEUE: "work_queue", OPT_GRAPH: "graph", OPT_HEAVY_CANDS: "heavy_cands",
                 OPT_MYERS_BLOCK: "myers_block", OPT_PLAN_HEAVY_WAVES: "plan_heavy_waves",
                 OPT_EMIT_HEAVY_WAVES: "emit_heavy_waves"}

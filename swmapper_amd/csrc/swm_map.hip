@@ -715,24 +715,32 @@ __global__ void __launch_bounds__(64) k_plan_heavy(const uint2 *__restrict__ pl,
 
             /* ---- phase B (all lanes): fetch, filter, compact ---- */
             const uint32_t bstart = nsurv;
-            for (uint32_t e0 = 0; e0 < bpos; e0 += 64) {
-                const uint32_t ent = e0 + lane;
-                bool keep = false;
-                uint32_t val = 0;
-                if (ent < bpos && ent > 0) {                     /* entry 0: the Q7 drop */
+            /* 256 entries per pass, 4 per lane, so 4 loads are in flight per
+             * lane; kept entries are written in entry order */
+            for (uint32_t e0 = 0; e0 < bpos; e0 += 256) {
+                uint2 v[4];
+                uint32_t meta[4];
+                bool live[4];
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const uint32_t ent = e0 + 64u * (uint32_t)q + lane;
+                    live[q] = ent < bpos && ent > 0;             /* entry 0: the Q7 drop */
                     uint32_t lo = 0, hi = bn;                    /* chunk with cpre[lo] <= ent < cpre[lo+1] */
                     while (hi - lo > 1) {
                         const uint32_t mid = (lo + hi) >> 1;
                         if (cpre[mid] <= ent) lo = mid; else hi = mid;
                     }
-                    const uint2 v = pl[cg[lo] + (ent - cpre[lo])];
-                    const uint32_t meta = smeta[cseed[lo]];
-                    keep = filter_pass(v.y, meta);
-                    val = adjust(v.x, meta);
+                    const uint32_t at = live[q] ? cg[lo] + (ent - cpre[lo]) : cg[0];
+                    v[q] = pl[at];
+                    meta[q] = smeta[cseed[lo]];
                 }
-                const uint64_t m = __ballot(keep);
-                if (keep) surv[nsurv + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = val;
-                nsurv += (uint32_t)__popcll(m);
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const bool keep = live[q] && filter_pass(v[q].y, meta[q]);
+                    const uint64_t m = __ballot(keep);
+                    if (keep) surv[nsurv + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = adjust(v[q].x, meta[q]);
+                    nsurv += (uint32_t)__popcll(m);
+                }
             }
             /* ---- end of batch: sorted unique list (block_heap_sort's result) ---- */
             const uint32_t n = nsurv - bstart;
@@ -1521,16 +1529,27 @@ __global__ void __launch_bounds__(64) k_emit_heavy(const uint32_t *__restrict__ 
                 if (num + 8 > 256 || len + 16 * 16 > 256 * 16) { num = 0; len = 0; }
                 g_open = false;
             };
+            /* the four fields of a window are loaded together, and the next
+             * window's are in flight while this one is processed */
+            int32_t f_er = -2;
+            uint32_t f_bi = 0, f_rec = 0, f_info = 0;
+            auto fetch = [&](uint32_t at) {
+                const uint32_t ci = at + lane;
+                if (ci < cend) { f_er = c.err[ci]; f_bi = c.bi[ci]; f_rec = c.rec[ci]; f_info = c.info[ci]; }
+                else { f_er = -2; f_bi = 0xffffffffu << 12; f_rec = 0; f_info = 0; }
+            };
+            if (c0 < cend) fetch(c0);
             for (uint32_t base = c0; base < cend; base += 64) {
                 const uint32_t ci = base + lane;
                 const uint32_t m = min(64u, cend - base);
                 const bool in = ci < cend;
-                const int32_t er = in ? c.err[ci] : -2;
-                const uint32_t b = in ? (c.bi[ci] >> 12) : 0xffffffffu;
+                const int32_t er = f_er;
+                const uint32_t b = in ? (f_bi >> 12) : 0xffffffffu;
                 const bool valid = er != -2;
                 const bool pass = valid && er <= (int32_t)e;
-                const uint32_t rec = pass ? c.rec[ci] : 0u;
-                const uint32_t info = pass ? c.info[ci] : 0u;
+                const uint32_t rec = pass ? f_rec : 0u;
+                const uint32_t info = pass ? f_info : 0u;
+                if (base + 64 < cend) fetch(base + 64);
                 /* duplicate candidates: predecessor = last passing lane below, else the carry */
                 const uint64_t pm = __ballot(pass);
                 const uint64_t pb = pm & below;
@@ -1613,6 +1632,24 @@ __global__ void __launch_bounds__(64) k_emit_heavy(const uint32_t *__restrict__ 
             if (count) atomicAdd(n_records, (unsigned long long)count);
         }
     }
+}
+
+/* Reads by candidate count (both strands) in log2 bins, and the maximum. */
+__global__ void __launch_bounds__(256) k_hist(uint32_t nreads, const uint32_t *__restrict__ cand_off,
+                                              unsigned long long *__restrict__ hist)
+{
+    __shared__ uint32_t bins[25];
+    if (threadIdx.x < 25) bins[threadIdx.x] = 0;
+    __syncthreads();
+    for (uint32_t rd = blockIdx.x * blockDim.x + threadIdx.x; rd < nreads; rd += gridDim.x * blockDim.x) {
+        const uint32_t c = cand_off[2 * rd + 2] - cand_off[2 * rd];
+        const uint32_t b = c ? min(32u - (uint32_t)__clz(c), 23u) : 0u;
+        atomicAdd(&bins[b], 1u);
+        atomicMax(&bins[24], c);
+    }
+    __syncthreads();
+    if (threadIdx.x < 24 && bins[threadIdx.x]) atomicAdd(&hist[threadIdx.x], (unsigned long long)bins[threadIdx.x]);
+    if (threadIdx.x == 24) atomicMax(&hist[24], (unsigned long long)bins[24]);
 }
 
 /* ------------------------------------------------------- scans and checks */
@@ -1720,7 +1757,7 @@ struct swm_ws {
     struct Buf { void *p = nullptr; size_t cap = 0; };
     enum { WORDS, SEL, NSEL, NPOS, NCHUNK, SURV_OFF, BD_OFF, SURV, BDESC, NBATCH, NUNIQ, CAND_OFF, C_VAL, C_RS, C_BI,
            C_ERR, C_LOC, C_REC, C_INFO, COUNTERS, BIG, TRACE, HEAVY, HEAVY_C, HEAVY_E, SCAN_TMP, IN_READS, IN_LENS,
-           OUT, NBUF };
+           OUT, HIST, NBUF };
     Buf b[NBUF];
     uint64_t gen = 1;
     ~swm_ws() { for (auto &x : b) if (x.p) (void)hipFree(x.p); }
@@ -2017,6 +2054,25 @@ static void stats_from_counters(swm_stats_t *s, uint64_t n, const unsigned long 
     s->n_positions = hc[3];
 }
 
+/* The per-read candidate histogram of the run just finished (stats). */
+static int collect_hist(const swm_ctx_t *ctx, swm_slot *sl, const Pipe &P)
+{
+    unsigned long long *d_hist;
+    int rc;
+    if ((rc = sl->ws.get(swm_ws::HIST, &d_hist, 32))) return rc;
+    hipStream_t st = sl->stream;
+    SWM_HIP_CHECK(hipMemsetAsync(d_hist, 0, 32 * sizeof(unsigned long long), st));
+    k_hist<<<std::min<uint32_t>(blocks_for(P.n, 256), (uint32_t)ctx->n_cu * 4), 256, 0, st>>>((uint32_t)P.n, P.cand_off,
+                                                                                             d_hist);
+    SWM_HIP_CHECK(hipGetLastError());
+    unsigned long long h[25];
+    SWM_HIP_CHECK(hipMemcpyAsync(h, d_hist, sizeof(h), hipMemcpyDeviceToHost, st));
+    SWM_HIP_CHECK(hipStreamSynchronize(st));
+    for (int b = 0; b < 24; b++) sl->stats.cand_hist[b] = h[b];
+    sl->stats.max_cands = h[24];
+    return SWM_OK;
+}
+
 static void observe(swm_slot *sl, uint64_t n, uint64_t tot_pos, uint64_t tot_chunk, uint64_t ncand)
 {
     if (!n) return;
@@ -2063,6 +2119,7 @@ static int run_synced(swm_ctx_t *ctx, swm_slot *sl, const char *d_reads, uint64_
     if ((rc = final_sync(sl))) return rc;
     stats_from_counters(&sl->stats, n, reinterpret_cast<const unsigned long long *>(hs));
     observe(sl, n, tot_pos, tot_chunk, ncand);
+    if (ctx->opt.histogram && (rc = collect_hist(ctx, sl, P)) != SWM_OK) return rc;
     float ms;
     (void)hipEventElapsedTime(&ms, sl->ev[0], sl->ev[1]); sl->stats.ms_seed = ms;
     (void)hipEventElapsedTime(&ms, sl->ev[1], sl->ev[2]); sl->stats.ms_plan = ms;
@@ -2365,6 +2422,7 @@ extern "C" int swm_ctx_set_option(swm_ctx_t *ctx, int opt, int64_t value)
         break;
     case SWM_OPT_PLAN_HEAVY_WAVES: if (value < 1 || value > 32) return SWM_EINVAL; o.plan_heavy_waves = (uint32_t)value; break;
     case SWM_OPT_EMIT_HEAVY_WAVES: if (value < 1 || value > 64) return SWM_EINVAL; o.emit_heavy_waves = (uint32_t)value; break;
+    case SWM_OPT_HISTOGRAM: if (value != 0 && value != 1) return SWM_EINVAL; o.histogram = (uint32_t)value; break;
     default: return SWM_EINVAL;
     }
     return SWM_OK;
@@ -2381,6 +2439,7 @@ extern "C" int swm_ctx_get_option(const swm_ctx_t *ctx, int opt, int64_t *value)
     case SWM_OPT_MYERS_BLOCK: *value = o.myers_block; break;
     case SWM_OPT_PLAN_HEAVY_WAVES: *value = o.plan_heavy_waves; break;
     case SWM_OPT_EMIT_HEAVY_WAVES: *value = o.emit_heavy_waves; break;
+    case SWM_OPT_HISTOGRAM: *value = o.histogram; break;
     default: return SWM_EINVAL;
     }
     return SWM_OK;
