@@ -118,12 +118,19 @@ __global__ void __launch_bounds__(256) k_keys(const char *__restrict__ seq, uint
     uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= nsamp) return;
     const uint64_t p = j * SWM_SEED_STEP;
+    /* starts are multiples of 4, so the k <= 14 bases come in four aligned
+     * 32-bit loads (the device copy of the sequence has 64 spare bytes) */
+    const uint32_t *w = reinterpret_cast<const uint32_t *>(seq + p);
+    const uint32_t wv[4] = {w[0], w[1], w[2], w[3]};
     uint32_t code = 0;
     bool valid = true;
-    for (uint32_t i = 0; i < k; i++) {
-        int c = base_code(seq[p + i]);
-        if (c < 0) valid = false;
-        code |= (uint32_t)(c < 0 ? 0 : c) << (2 * i);
+#pragma unroll
+    for (uint32_t i = 0; i < 16; i++) {
+        if (i < k) {
+            const int c = base_code((char)((wv[i >> 2] >> (8 * (i & 3))) & 0xffu));
+            if (c < 0) valid = false;
+            code |= (uint32_t)(c < 0 ? 0 : c) << (2 * i);
+        }
     }
     keys[j] = valid ? code : (1u << (2 * k));
     vals[j] = ((unsigned long long)lseed_at(b2, p, k) << 32) | (uint32_t)p;

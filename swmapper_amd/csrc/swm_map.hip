@@ -1990,7 +1990,7 @@ static int enq_b(const swm_ctx_t *ctx, Pipe &P, uint64_t cap_cand, hipStream_t s
     } else {
         k_plan_plain<<<blocks_for(P.nrs, 64), 64, 0, st>>>((uint32_t)P.nrs, ctx->d_pl, P.pio);
     }
-    k_bigsort<<<1024, 256, 0, st>>>(P.pio.big, P.pio.big_count, P.surv_off, P.bd_off, P.pio.surv, P.pio.bdesc,
+    k_bigsort<<<ctx->n_cu * 8, 256, 0, st>>>(P.pio.big, P.pio.big_count, P.surv_off, P.bd_off, P.pio.surv, P.pio.bdesc,
                                     P.pio.nuniq);
     SWM_HIP_CHECK(hipGetLastError());
     if (ev) SWM_HIP_CHECK(hipEventRecord(ev[2], st));
@@ -2033,7 +2033,9 @@ static int enq_c(const swm_ctx_t *ctx, Pipe &P, const uint32_t *d_lens, uint32_t
                                                                           ctx->d_base_2bits, ctx->ref_length, e, P.cs,
                                                                           P.trace, P.u32c + 1, P.u32c + 2,
                                                                           P.counters + 2);
-    k_trace<<<1024, 64, 0, st>>>(P.trace, P.u32c + 1, d_lens, P.so.words, ctx->d_base_2bits, ctx->ref_length, e, P.cs);
+    /* persistent: 8 waves per SIMD, the traceback chains are latency-bound */
+    k_trace<<<ctx->n_cu * 32, 64, 0, st>>>(P.trace, P.u32c + 1, d_lens, P.so.words, ctx->d_base_2bits, ctx->ref_length, e,
+                                          P.cs);
     k_emit<<<blocks_for(P.n, 128), 128, 0, st>>>((uint32_t)P.n, e, P.cand_off, P.cs, d_out, P.counters + 1,
                                                  heavy_cands, P.heavy_e, P.u32c + 6);
     if (o.work_queue)
