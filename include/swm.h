@@ -143,6 +143,7 @@ int swm_last_stats(const swm_ctx_t *ctx, swm_stats_t *st);
 #define SWM_OPT_PLAN_HEAVY_WAVES 5  /* persistent plan waves per CU (1..32, default 16) */
 #define SWM_OPT_EMIT_HEAVY_WAVES 6  /* cooperative emission waves per CU (1..64, default 32) */
 #define SWM_OPT_HISTOGRAM 7         /* 1: also collect the per-read candidate histogram (stats) */
+#define SWM_OPT_MYERS_PACKED 8      /* 1 (default): for e <= 7, two candidates per lane in 16-bit halves */
 
 /* Every form and tunable gives the same results; they change only speed. */
 int swm_ctx_set_option(swm_ctx_t *ctx, int opt, int64_t value);
@@ -150,7 +151,7 @@ int swm_ctx_get_option(const swm_ctx_t *ctx, int opt, int64_t *value);
 
 /* Autotuner: times the pipeline on the given sample (host buffers, as
  * swm_map_batch) for each candidate setting of the tunables, one at a time,
- * keeps the fastest on ctx and writes the chosen values to chosen[6]
+ * keeps the fastest on ctx and writes every option's value to chosen[8]
  * (indexed by SWM_OPT_* - 1) when chosen != NULL.  reps >= 1 timed runs per
  * setting (median). */
 int swm_autotune(swm_ctx_t *ctx, const char *reads, uint64_t stride, const uint32_t *lens, uint64_t n,

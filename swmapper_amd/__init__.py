@@ -72,9 +72,12 @@ class BatchInfo(C.Structure):
 # forms and tunables (swm_ctx_set_option)
 OPT_WORK_QUEUE, OPT_GRAPH, OPT_HEAVY_CANDS, OPT_MYERS_BLOCK, OPT_PLAN_HEAVY_WAVES, OPT_EMIT_HEAVY_WAVES = range(1, 7)
 OPT_HISTOGRAM = 7
+OPT_MYERS_PACKED = 8
 OPTION_NAMES = {OPT_WORK_QUEUE: "work_queue", OPT_GRAPH: "graph", OPT_HEAVY_CANDS: "heavy_cands",
                 OPT_MYERS_BLOCK: "myers_block", OPT_PLAN_HEAVY_WAVES: "plan_heavy_waves",
-                OPT_EMIT_HEAVY_WAVES: "emit_heavy_waves"}
+                OPT_EMIT_HEAVY_WAVES: "emit_heavy_waves", OPT_HISTOGRAM: "histogram",
+                OPT_MYERS_PACKED: "myers_packed"}
+N_OPTIONS = 8
 
 
 _lib_handle = None
@@ -284,9 +287,9 @@ class Mapper:
         reads = np.ascontiguousarray(reads, dtype=np.uint8)
         lens = np.ascontiguousarray(lens, dtype=np.uint32)
         n, stride = reads.shape
-        ch = (C.c_int64 * 6)()
+        ch = (C.c_int64 * N_OPTIONS)()
         _check("swm_autotune", lib().swm_autotune(self._h, _ptr(reads), stride, _ptr(lens), n, e, reps, ch))
-        return {OPTION_NAMES[i + 1]: ch[i] for i in range(6)}
+        return {OPTION_NAMES[i + 1]: ch[i] for i in range(N_OPTIONS)}
 
 
 class Server:

@@ -1120,6 +1120,172 @@ __device__ __forceinline__ bool hamming_one(uint32_t i, int32_t er, const uint32
     return true;
 }
 
+/* Two candidates per lane for e <= 7: the band (2e+1 <= 15 bits) fits a
+ * 16-bit half, and a 16-bit band word gives the same distances and end
+ * locations as the 32-bit one (the bits above the band converge the same
+ * way; cf. the 32/64-bit equivalence), so candidate A runs in bits 0-15 and
+ * B in bits 16-31 with packed 16-bit adds and shifts and shared bitwise ops.
+ * Each half has its own read and window; both need the same read length. */
+#define SWM_MAX_E_PACKED 7
+
+typedef unsigned short swm_u16x2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ uint32_t pk_add16(uint32_t a, uint32_t b)
+{
+    return __builtin_bit_cast(uint32_t, __builtin_bit_cast(swm_u16x2, a) + __builtin_bit_cast(swm_u16x2, b));
+}
+__device__ __forceinline__ uint32_t pk_shr16(uint32_t a)
+{
+    const swm_u16x2 one = {1, 1};
+    return __builtin_bit_cast(uint32_t, __builtin_bit_cast(swm_u16x2, a) >> one);
+}
+
+__device__ __forceinline__ void myers_end_scan(uint32_t vp, uint32_t vn, int err, uint32_t L, uint32_t e,
+                                               uint32_t ref_len, int32_t *err_out, int32_t *loc_out)
+{
+    const int last_high = (int)ref_len - (int)L;
+    const int site = (int)L - 1;
+    int location = -1, error = SWM_MAXERROR;
+    if (err <= (int)e && err < error) { error = err; location = site; }
+    for (int i = 0; i < last_high; i++) {
+        err += (int)((vp >> i) & 1u);
+        err -= (int)((vn >> i) & 1u);
+        if (err <= (int)e && err < error) { error = err; location = site + i + 1; }
+    }
+    *err_out = error;
+    *loc_out = location;
+}
+
+template <bool SHARED>
+__device__ __forceinline__ void myers_pair(const uint32_t *__restrict__ rwa, const uint32_t *__restrict__ rwb,
+                                           uint32_t L, uint32_t e, uint32_t rla, uint32_t rlb,
+                                           uint32_t (&wla)[SWM_NW], uint32_t (&wha)[SWM_NW],
+                                           uint32_t (&wlb)[SWM_NW], uint32_t (&whb)[SWM_NW], int32_t *ea,
+                                           int32_t *la, int32_t *eb, int32_t *lb)
+{
+    uint32_t ra[16], rb[16];
+#pragma unroll
+    for (int j = 0; j < 16; j++) { ra[j] = rwa[j]; rb[j] = SHARED ? 0u : rwb[j]; }
+    const uint32_t m16 = (1u << (2 * e + 1)) - 1u;
+    const uint32_t mask = m16 | (m16 << 16);
+    uint32_t vp = 0, vn = 0, ones = 0;
+    const uint32_t nfull = L >> 5, rem = L & 31u;
+
+#define SWM_PSTEP(S, RA, RB)                                                                \
+    {                                                                                       \
+        const int sb = 2 * ((S) & 15);                                                      \
+        uint32_t RL, RH;                                                                    \
+        if (SHARED) {                     /* one read: its base masks serve both halves */  \
+            RL = (uint32_t)__builtin_amdgcn_sbfe((int)(RA), sb, 1);                         \
+            RH = (uint32_t)__builtin_amdgcn_sbfe((int)(RA), sb + 1, 1);                     \
+        } else {                                                                            \
+            const uint32_t rla_ = (uint32_t)__builtin_amdgcn_sbfe((int)(RA), sb, 1);        \
+            const uint32_t rha_ = (uint32_t)__builtin_amdgcn_sbfe((int)(RA), sb + 1, 1);    \
+            const uint32_t rlb_ = (uint32_t)__builtin_amdgcn_sbfe((int)(RB), sb, 1);        \
+            const uint32_t rhb_ = (uint32_t)__builtin_amdgcn_sbfe((int)(RB), sb + 1, 1);    \
+            RL = (rla_ & 0xffffu) | (rlb_ & 0xffff0000u);                                   \
+            RH = (rha_ & 0xffffu) | (rhb_ & 0xffff0000u);                                   \
+        }                                                                                   \
+        const uint32_t bl = __builtin_amdgcn_perm(__builtin_amdgcn_alignbit(lb1, lb0, (S)), \
+                                                  __builtin_amdgcn_alignbit(la1, la0, (S)), 0x05040100u); \
+        const uint32_t bh = __builtin_amdgcn_perm(__builtin_amdgcn_alignbit(hb1, hb0, (S)), \
+                                                  __builtin_amdgcn_alignbit(ha1, ha0, (S)), 0x05040100u); \
+        const uint32_t eq = ~((bl ^ RL) | (bh ^ RH)) & mask;                                \
+        uint32_t x = eq | vn;                                                               \
+        const uint32_t d0 = (pk_add16(x & vp, vp) ^ vp) | x;                                \
+        const uint32_t hn = vp & d0;                                                        \
+        const uint32_t hp = vn | ~(vp | d0);                                                \
+        x = pk_shr16(d0);                                                                   \
+        vn = x & hp;                                                                        \
+        vp = hn | ~(x | hp);                                                                \
+        ones = pk_add16(ones, d0 & 0x00010001u);                                            \
+    }
+
+    for (uint32_t cc = 0; cc < nfull; cc++) {
+        const uint32_t a0 = ra[0], a1 = ra[1], b0 = rb[0], b1 = rb[1];
+        const uint32_t la0 = wla[0], la1 = wla[1], ha0 = wha[0], ha1 = wha[1];
+        const uint32_t lb0 = wlb[0], lb1 = wlb[1], hb0 = whb[0], hb1 = whb[1];
+#pragma unroll
+        for (int s = 0; s < 16; s++) SWM_PSTEP(s, a0, b0)
+#pragma unroll
+        for (int s = 16; s < 32; s++) SWM_PSTEP(s, a1, b1)
+#pragma unroll
+        for (int j = 0; j < SWM_NW - 1; j++) { wla[j] = wla[j + 1]; wha[j] = wha[j + 1]; wlb[j] = wlb[j + 1]; whb[j] = whb[j + 1]; }
+#pragma unroll
+        for (int j = 0; j < 14; j++) { ra[j] = ra[j + 2]; rb[j] = rb[j + 2]; }
+    }
+    if (rem) {
+        const uint32_t a0 = ra[0], a1 = ra[1], b0 = rb[0], b1 = rb[1];
+        const uint32_t la0 = wla[0], la1 = wla[1], ha0 = wha[0], ha1 = wha[1];
+        const uint32_t lb0 = wlb[0], lb1 = wlb[1], hb0 = whb[0], hb1 = whb[1];
+#pragma unroll
+        for (int s = 0; s < 16; s++) {
+            if ((uint32_t)s < rem) SWM_PSTEP(s, a0, b0)
+        }
+#pragma unroll
+        for (int s = 16; s < 32; s++) {
+            if ((uint32_t)s < rem) SWM_PSTEP(s, a1, b1)
+        }
+    }
+#undef SWM_PSTEP
+    myers_end_scan(vp & 0xffffu, vn & 0xffffu, (int)L - (int)(ones & 0xffffu), L, e, rla, ea, la);
+    myers_end_scan(vp >> 16, vn >> 16, (int)L - (int)(ones >> 16), L, e, rlb, eb, lb);
+}
+
+__device__ __forceinline__ void myers_one(uint32_t i, uint32_t st, uint32_t L, uint32_t ref_len,
+                                          const uint32_t *__restrict__ words, const uint32_t *__restrict__ b2,
+                                          uint32_t e, const CandSet &c)
+{
+    uint32_t wl[SWM_NW], wh[SWM_NW];
+    load_window(b2, st, L, wl, wh);
+    int32_t er, lc;
+    myers_lane<false>(words + (uint64_t)c.rs[i] * 32, L, e, ref_len, wl, wh, &er, &lc, nullptr, nullptr);
+    c.err[i] = er;
+    c.loc[i] = lc;
+}
+
+__global__ void __launch_bounds__(256) k_myers2(const uint32_t *__restrict__ ncand_p, const uint32_t *__restrict__ lens,
+                                                const uint32_t *__restrict__ words, const uint32_t *__restrict__ b2,
+                                                uint64_t ref_length, uint32_t e, CandSet c)
+{
+    if (*c.ovf) return;
+    const uint32_t ncand = *ncand_p;
+    const uint32_t npair = (ncand + 1) / 2;
+    for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < npair; g += gridDim.x * blockDim.x) {
+        const uint32_t i0 = 2 * g, i1 = 2 * g + 1;
+        const bool has1 = i1 < ncand;
+        const uint32_t rs0 = c.rs[i0], rs1 = has1 ? c.rs[i1] : rs0;
+        const uint32_t L0 = lens[rs0 >> 1], L1 = lens[rs1 >> 1];
+        uint32_t rl0, rl1 = 0;
+        const uint32_t st0 = pos_start(ref_length, e, c.val[i0], L0, &rl0);
+        const uint32_t st1 = has1 ? pos_start(ref_length, e, c.val[i1], L1, &rl1) : 0xffffffffu;
+        const bool v0 = st0 != 0xffffffffu, v1 = st1 != 0xffffffffu;
+        if (!v0) { c.err[i0] = -2; c.loc[i0] = -1; }
+        if (has1 && !v1) { c.err[i1] = -2; c.loc[i1] = -1; }
+        const bool packed = v0 && v1 && L0 == L1;
+        /* the shared-read variant when every packed pair of the wave is one
+         * (read, strand): wave-uniform, so no lane runs both */
+        const bool shared = __all(!packed || rs0 == rs1);
+        if (packed) {
+            uint32_t wla[SWM_NW], wha[SWM_NW], wlb[SWM_NW], whb[SWM_NW];
+            load_window(b2, st0, L0, wla, wha);
+            load_window(b2, st1, L1, wlb, whb);
+            int32_t e0, l0, e1, l1;
+            if (shared)
+                myers_pair<true>(words + (uint64_t)rs0 * 32, words + (uint64_t)rs1 * 32, L0, e, rl0, rl1, wla, wha,
+                                 wlb, whb, &e0, &l0, &e1, &l1);
+            else
+                myers_pair<false>(words + (uint64_t)rs0 * 32, words + (uint64_t)rs1 * 32, L0, e, rl0, rl1, wla, wha,
+                                  wlb, whb, &e0, &l0, &e1, &l1);
+            c.err[i0] = e0; c.loc[i0] = l0;
+            c.err[i1] = e1; c.loc[i1] = l1;
+        } else {
+            if (v0) myers_one(i0, st0, L0, rl0, words, b2, e, c);
+            if (v1) myers_one(i1, st1, L1, rl1, words, b2, e, c);
+        }
+    }
+}
+
 /* Per candidate: the Hamming check of passing lanes, the traceback list, the
  * count of rejected windows and the band steps of the verified ones.  A
  * fixed grid loops over the candidates (block-uniform trip count); counts
@@ -2024,8 +2190,14 @@ static int enq_c(const swm_ctx_t *ctx, Pipe &P, const uint32_t *d_lens, uint32_t
     SWM_HIP_CHECK(hipGetLastError());
     if (ev) SWM_HIP_CHECK(hipEventRecord(ev[3], st));
     const uint32_t mb = o.myers_block;
-    k_myers<<<blocks_for(grid_cand ? grid_cand : 1, mb), mb, 0, st>>>(ncand, d_lens, P.so.words, ctx->d_base_2bits,
-                                                                      ctx->ref_length, e, P.cs);
+    if (o.myers_packed && e <= SWM_MAX_E_PACKED)
+        k_myers2<<<blocks_for((grid_cand ? grid_cand : 1) / 2 + 1, mb), mb, 0, st>>>(ncand, d_lens, P.so.words,
+                                                                                    ctx->d_base_2bits,
+                                                                                    ctx->ref_length, e, P.cs);
+    else
+        k_myers<<<blocks_for(grid_cand ? grid_cand : 1, mb), mb, 0, st>>>(ncand, d_lens, P.so.words,
+                                                                          ctx->d_base_2bits, ctx->ref_length, e,
+                                                                          P.cs);
     SWM_HIP_CHECK(hipGetLastError());
     if (ev) SWM_HIP_CHECK(hipEventRecord(ev[4], st));
     const uint32_t ham_grid = std::min<uint32_t>(blocks_for(grid_cand ? grid_cand : 1, 256), (uint32_t)ctx->n_cu * 8);
@@ -2134,8 +2306,9 @@ static int run_synced(swm_ctx_t *ctx, swm_slot *sl, const char *d_reads, uint64_
 
 static uint64_t opt_sig(const swm_opts &o)
 {
-    return (uint64_t)o.work_queue | ((uint64_t)o.heavy_cands << 1) | ((uint64_t)o.myers_block << 33) |
-           ((uint64_t)o.plan_heavy_waves << 43) | ((uint64_t)o.emit_heavy_waves << 53);
+    return (uint64_t)o.work_queue | ((uint64_t)o.heavy_cands << 1) | ((uint64_t)o.myers_block << 17) |
+           ((uint64_t)o.plan_heavy_waves << 27) | ((uint64_t)o.emit_heavy_waves << 37) |
+           ((uint64_t)o.myers_packed << 47) | ((uint64_t)o.histogram << 48);
 }
 
 /* Fixed-capacity form: the whole lookup -> filter -> verify -> emit pipeline
@@ -2425,6 +2598,7 @@ extern "C" int swm_ctx_set_option(swm_ctx_t *ctx, int opt, int64_t value)
     case SWM_OPT_PLAN_HEAVY_WAVES: if (value < 1 || value > 32) return SWM_EINVAL; o.plan_heavy_waves = (uint32_t)value; break;
     case SWM_OPT_EMIT_HEAVY_WAVES: if (value < 1 || value > 64) return SWM_EINVAL; o.emit_heavy_waves = (uint32_t)value; break;
     case SWM_OPT_HISTOGRAM: if (value != 0 && value != 1) return SWM_EINVAL; o.histogram = (uint32_t)value; break;
+    case SWM_OPT_MYERS_PACKED: if (value != 0 && value != 1) return SWM_EINVAL; o.myers_packed = (uint32_t)value; break;
     default: return SWM_EINVAL;
     }
     return SWM_OK;
@@ -2442,6 +2616,7 @@ extern "C" int swm_ctx_get_option(const swm_ctx_t *ctx, int opt, int64_t *value)
     case SWM_OPT_PLAN_HEAVY_WAVES: *value = o.plan_heavy_waves; break;
     case SWM_OPT_EMIT_HEAVY_WAVES: *value = o.emit_heavy_waves; break;
     case SWM_OPT_HISTOGRAM: *value = o.histogram; break;
+    case SWM_OPT_MYERS_PACKED: *value = o.myers_packed; break;
     default: return SWM_EINVAL;
     }
     return SWM_OK;

@@ -378,6 +378,7 @@ extern "C" int swm_autotune(swm_ctx_t *ctx, const char *reads, uint64_t stride, 
     struct Knob { int opt; std::vector<int64_t> vals; };
     const Knob knobs[] = {
         {SWM_OPT_WORK_QUEUE, {1, 0}},
+        {SWM_OPT_MYERS_PACKED, {1, 0}},
         {SWM_OPT_MYERS_BLOCK, {256, 128, 64}},
         {SWM_OPT_HEAVY_CANDS, {256, 64, 128, 512, 1024}},
         {SWM_OPT_PLAN_HEAVY_WAVES, {16, 4, 8, 32}},
@@ -398,7 +399,7 @@ extern "C" int swm_autotune(swm_ctx_t *ctx, const char *reads, uint64_t stride, 
         if (!rc) rc = set_opt(ctx, kb.opt, best_v);
     }
     if (!rc && chosen)
-        for (int o = 1; o <= 6; o++) (void)swm_ctx_get_option(ctx, o, &chosen[o - 1]);
+        for (int o = 1; o <= 8; o++) (void)swm_ctx_get_option(ctx, o, &chosen[o - 1]);
     (void)hipFree(d_reads);
     (void)hipFree(d_lens);
     (void)hipFree(d_out);

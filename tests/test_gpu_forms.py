@@ -47,6 +47,8 @@ SETTINGS = [
     {swm.OPT_MYERS_BLOCK: 128, swm.OPT_PLAN_HEAVY_WAVES: 1, swm.OPT_EMIT_HEAVY_WAVES: 1},
     {swm.OPT_GRAPH: 1},
     {swm.OPT_GRAPH: 1, swm.OPT_WORK_QUEUE: 0},
+    {swm.OPT_MYERS_PACKED: 0},
+    {swm.OPT_MYERS_PACKED: 1, swm.OPT_MYERS_BLOCK: 64},
 ]
 
 

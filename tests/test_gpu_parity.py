@@ -75,11 +75,22 @@ def test_map_uniform_150bp_e5():
     assert (exp["count"] > 0).mean() > 0.5
 
 
-@pytest.mark.parametrize("e", [0, 3, 8, 15])
+@pytest.mark.parametrize("e", [0, 3, 7, 8, 15])
 def test_map_errors_range(e):
     g = wl.genome(300_000, seed=21)
     r, l, _, _ = wl.reads(g, 4000, len_min=50, len_max=250, err=(0.01, 0.04), seed=22 + e)
     _say("map mixed lengths e", e)
+    got, exp, st, nv = _map_both(g, r, l, e)
+    _assert_same(got, exp)
+    assert st["n_verifications"] == nv
+
+
+@pytest.mark.parametrize("e", [1, 4, 6, 7])
+def test_map_fixed_length_packed_pairs(e):
+    """Equal read lengths: every candidate pair runs in one lane's 16-bit halves."""
+    g = wl.genome(300_000, seed=23, kind=swm.GENOME_REPEAT_FAMILIES, frac=0.2)
+    r, l, _, _ = wl.reads(g, 3000, len_min=150, len_max=150, err=(0.0, 0.05), seed=24 + e)
+    _say("map packed pairs e", e)
     got, exp, st, nv = _map_both(g, r, l, e)
     _assert_same(got, exp)
     assert st["n_verifications"] == nv
