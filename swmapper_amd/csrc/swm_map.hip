@@ -682,30 +682,34 @@ __global__ void __launch_bounds__(64) k_plan_heavy(const uint2 *__restrict__ pl,
                     pos_size += get;
                     blocks++;
                     spc_v = wl(spc_v, i, pc + get);
-                    if (pc + get == cnt) {                        /* seeds_heap_pop_and_insert (:1936-1962, Q6) */
-                        hloc = wl(hloc, 0, rl(hloc, hsize - 1));
-                        hind = wl(hind, 0, rl(hind, hsize - 1));
+                    /* seeds_heap_pop_and_insert (:1936-1962, Q6): the new root
+                     * (the last entry, or this 12-mer with its next key) sifts
+                     * down; its value stays in scalars and only the hole moves,
+                     * which is the same sequence of swaps */
+                    uint32_t mloc, mind;
+                    if (pc + get == cnt) {
+                        mloc = rl(hloc, hsize - 1);
+                        mind = rl(hind, hsize - 1);
                         hsize--;
                     } else {
                         const uint32_t kb = rl(kb_v, i);
-                        uint32_t key;
-                        if (c >= kb && c < kb + HK) key = skey[i * HK + (c - kb)];
-                        else key = pl[off + min(SWM_CHUNK * (c + 1), cnt) - 1].x;
-                        hloc = wl(hloc, 0, key);
+                        if (c >= kb && c < kb + HK) mloc = skey[i * HK + (c - kb)];
+                        else mloc = pl[off + min(SWM_CHUNK * (c + 1), cnt) - 1].x;
+                        mind = i;
                     }
                     uint32_t x = 1;
                     while (x * 2 < hsize) {
-                        uint32_t t = x * 2;
                         const uint32_t a2 = rl(hloc, 2 * x - 1), a3 = rl(hloc, 2 * x);
-                        if (x * 2 + 1 <= hsize && a2 > a3) t = x * 2 + 1;
-                        const uint32_t lt = rl(hloc, t - 1), lx = rl(hloc, x - 1);
-                        if (lt < lx) {
-                            const uint32_t it = rl(hind, t - 1), ix = rl(hind, x - 1);
-                            hloc = wl(wl(hloc, t - 1, lx), x - 1, lt);
-                            hind = wl(wl(hind, t - 1, ix), x - 1, it);
+                        uint32_t t = x * 2, lt = a2;
+                        if (x * 2 + 1 <= hsize && a2 > a3) { t = x * 2 + 1; lt = a3; }
+                        if (lt < mloc) {
+                            hloc = wl(hloc, x - 1, lt);
+                            hind = wl(hind, x - 1, rl(hind, t - 1));
                         } else break;
                         x = t;
                     }
+                    hloc = wl(hloc, x - 1, mloc);
+                    hind = wl(hind, x - 1, mind);
                 }
                 if (pos_size + SWM_CHUNK > SWM_BATCH_POS || hsize == 0 || blocks >= SWM_MAX_BLOCKS) break;
             }
