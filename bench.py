@@ -209,6 +209,8 @@ def main():
     ap.add_argument("--serve-merge", type=int, default=262144,
                     help="reads per serving launch when waiting batches are merged (0: one batch per launch)")
     ap.add_argument("--serve-form", default="graph", choices=("graph", "synced"))
+    ap.add_argument("--serve-wq", type=int, default=None, choices=(0, 1),
+                    help="work queues in the serving launches (default: those of the fastest offline form)")
     ap.add_argument("--hw-queues", type=int, default=8,
                     help="GPU_MAX_HW_QUEUES for this process when not set by the caller above 4: the serving "
                          "slots' streams each get a hardware queue")
@@ -363,6 +365,9 @@ def main():
 
     serving = None
     if args.serve_seconds > 0:
+        wq = args.serve_wq if args.serve_wq is not None else FORMS[best][swm.OPT_WORK_QUEUE]
+        m.set_option(swm.OPT_WORK_QUEUE, wq)
+        log(f"serving with work queues {'on' if wq else 'off'}")
         serving = serve_bench(swm, m, g, e, args.serve_seconds, log, slots=args.serve_slots,
                               graph=args.serve_form == "graph", merge=args.serve_merge)
 
