@@ -133,6 +133,56 @@ int swm_map_batch_device(swm_ctx_t *ctx, const char *d_reads, uint64_t stride,
 /* Statistics of the last batch run on ctx. */
 int swm_last_stats(const swm_ctx_t *ctx, swm_stats_t *st);
 
+/* ---- alignment records and CIGAR text (the per-block output of put_cigar,
+ *      mapper_cpe.c:1437-1507, and the CIGAR strings of align_reads,
+ *      mapper_cpe.c:277-287, 475-507) ---- */
+
+/* One written record.  The reference hands them to the host in blocks of at
+ * most 256 per (read, strand) whose order depends on CPE timing; here they
+ * come in one deterministic order: reads in batch order, the forward strand
+ * before the reverse one, each strand's records in emission order. */
+typedef struct swm_record {
+    uint32_t read;       /* index of the read in the batch */
+    uint32_t pos;        /* cigar->location: 1-based position in the concatenated reference */
+    uint32_t cigar_off;  /* offset of the CIGAR text in swm_records_t.cigar */
+    uint8_t strand;      /* 0 forward, 1 reverse */
+    uint8_t dist;        /* edit distance (the CIGAR buffer's distance byte) */
+    uint8_t primary;     /* 1 for the first record of its (read, strand): main_flag */
+    uint8_t cigar_len;   /* bytes of CIGAR text (the CIGAR buffer's length byte) */
+} swm_record_t;
+
+typedef struct swm_records {
+    uint64_t n;          /* records */
+    swm_record_t *rec;
+    uint64_t text_bytes;
+    char *cigar;         /* the CIGAR texts back to back, no terminators */
+} swm_records_t;
+
+/* swm_map_batch that also returns every record with its CIGAR text (always
+ * the synced form).  recs is filled with library-owned memory: release it
+ * with swm_records_free. */
+int swm_map_batch_records(swm_ctx_t *ctx, const char *reads, uint64_t stride, const uint32_t *lens,
+                          uint64_t n, uint32_t e, swm_read_result_t *out, swm_records_t *recs);
+void swm_records_free(swm_records_t *recs);
+
+/* ---- SAM output (replaces sam_header and sam_format_2, mapper_mpe.c:315-687) ---- */
+
+typedef struct swm_sam swm_sam_t;
+
+/* Opens path for writing and writes the header (@HD, one @SQ per contig of
+ * <index_prefix>.ref_info, @PG).  gzip_level 1..9 writes gzip (the
+ * reference writes <out>.sam.gz at level 1); 0 writes plain text. */
+int swm_sam_open(const char *path, const char *index_prefix, int gzip_level, swm_sam_t **out);
+/* One batch: the unmapped reads (no record) in batch order, then every
+ * record in recs' order.  names[i] (name_lens[i] bytes, or NUL-terminated
+ * when name_lens is NULL), seqs/quals at i * stride with lens[i] bytes;
+ * quals may be NULL (FASTA input: QUAL written as '*'). */
+int swm_sam_write_batch(swm_sam_t *w, uint64_t n, const char *const *names, const uint32_t *name_lens,
+                        const char *seqs, const char *quals, uint64_t stride, const uint32_t *lens,
+                        const swm_records_t *recs);
+/* Flushes and closes; returns the first write error, if any. */
+int swm_sam_close(swm_sam_t *w);
+
 /* ---- forms and tunables (no counterpart in the reference: its one form is
  *      the CPE kernel with compile-time sizes, mapper_cpe.c:15-25) ---- */
 

@@ -497,8 +497,17 @@ typedef struct {
 
 static int ndigits(uint32_t v) { int n = 0; while (v) { n++; v /= 10; } return n; }  /* sprint_length :214-228 */
 
-static void emit_record(emit_t *em, uint32_t loc, int dist, uint32_t text_len)
+/* sprint_length (mapper_cpe.c:214-228): the decimal digits of v (none for 0). */
+static int sprint_length(char *out, uint32_t v)
 {
+    int n = ndigits(v), k = n;
+    while (v) { out[--k] = (char)('0' + v % 10); v /= 10; }
+    return n;
+}
+
+static void emit_record(emit_t *em, uint32_t loc, int dist, uint32_t text_len, const char *cigar)
+{
+    const int primary = em->main_flag && em->cigar_num == 0;   /* first record of the strand's first block */
     em->location[em->cigar_num++] = loc;
     em->cigar_len += text_len;
     if (em->hit) {
@@ -516,6 +525,13 @@ static void emit_record(emit_t *em, uint32_t loc, int dist, uint32_t text_len)
         orc_record_t *r = &em->recs[em->n_recs++];
         r->read = em->read; r->strand = (uint8_t)em->strand; r->dist = (uint8_t)dist;
         r->block = (uint16_t)em->block; r->loc = loc; r->text_len = text_len;
+        r->primary = (uint8_t)primary;
+        memset(r->pad, 0, sizeof(r->pad));
+        memset(r->cigar, 0, sizeof(r->cigar));
+        size_t cl = strlen(cigar);
+        if (cl >= ORC_CIGAR_TEXT) cl = ORC_CIGAR_TEXT - 1;
+        memcpy(r->cigar, cigar, cl);
+        r->cigar[cl] = 0;
     }
 }
 
@@ -563,7 +579,11 @@ static void align_one(const orc_ref_t *ref, emit_t *em, const uint32_t *w, const
     }
     if (tmp_err == error) {                                                 /* :277-287 */
         uint32_t text_len = 2 + (uint32_t)ndigits(L) + 1;
-        emit_record(em, (uint32_t)((uint64_t)win_start + (uint64_t)(int64_t)start_location + 1), error, text_len);
+        char cig[16];
+        int cl = sprint_length(cig, L);                                     /* :283-287 "<L>M" */
+        cig[cl++] = 'M';
+        cig[cl] = 0;
+        emit_record(em, (uint32_t)((uint64_t)win_start + (uint64_t)(int64_t)start_location + 1), error, text_len, cig);
         return;
     }
     /* indel path: re-run the recurrence keeping D0/HP per read base (:288-362) */
@@ -649,17 +669,22 @@ static void align_one(const orc_ref_t *ref, emit_t *em, const uint32_t *w, const
     uint64_t new_loc = (uint64_t)win_start + (uint64_t)(int64_t)start_location + 1;
     if (em->cigar_num > 0 && (uint64_t)em->location[em->cigar_num - 1] == new_loc) return; /* :475-477 */
     uint32_t text_len = 2;                                                  /* :479-507 */
+    char cig[ORC_CIGAR_TEXT + 16];
+    int cl = 0;
     for (j = j - 1; j > 0; j--) {
         if (rchar[j] == 'M' || rchar[j] == 'S') {
             int size_sm = 0;
             while (j > 0 && (rchar[j] == 'M' || rchar[j] == 'S')) { size_sm += rsize[j]; j--; }
             j++;
             text_len += (uint32_t)ndigits((uint32_t)size_sm) + 1;
+            if (cl < ORC_CIGAR_TEXT - 12) { cl += sprint_length(cig + cl, (uint32_t)size_sm); cig[cl++] = 'M'; }
         } else {
             text_len += (uint32_t)ndigits((uint32_t)rsize[j]) + 1;
+            if (cl < ORC_CIGAR_TEXT - 12) { cl += sprint_length(cig + cl, (uint32_t)rsize[j]); cig[cl++] = rchar[j]; }
         }
     }
-    emit_record(em, (uint32_t)new_loc, error, text_len);
+    cig[cl] = 0;
+    emit_record(em, (uint32_t)new_loc, error, text_len, cig);
 }
 
 /* banded_myers (mapper_cpe.c:1590-1864): groups of up to 8 valid lanes, each

@@ -66,6 +66,7 @@ typedef struct orc_hit {
 } orc_hit_t;
 
 /* One emitted record, in emission order. */
+#define ORC_CIGAR_TEXT 160
 typedef struct orc_record {
     uint32_t read;
     uint8_t  strand;
@@ -73,6 +74,9 @@ typedef struct orc_record {
     uint16_t block;   /* flush block number within (read, strand) */
     uint32_t loc;     /* cigar->location value: window_start + start_location + 1 */
     uint32_t text_len;/* bytes this record adds to the CIGAR buffer */
+    uint8_t  primary; /* the first record of its (read, strand): main_flag (mapper_cpe.c:1462) */
+    uint8_t  pad[3];
+    char     cigar[ORC_CIGAR_TEXT];  /* the CIGAR text (align_reads :283-287, :479-507), NUL-terminated */
 } orc_record_t;
 
 /* One verification (one valid Myers lane), in evaluation order. */

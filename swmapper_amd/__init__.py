@@ -24,6 +24,9 @@ _ERRORS = {-1: "invalid argument", -2: "index/sequence file error", -3: "out of 
            -4: "HIP device error", -5: "busy"}
 
 RESULT_DTYPE = np.dtype([("pos", "<u4"), ("strand", "u1"), ("dist", "u1"), ("pad", "<u2"), ("count", "<u4")])
+# swm_record_t: one written record (include/swm.h)
+RECORD_DTYPE = np.dtype([("read", "<u4"), ("pos", "<u4"), ("cigar_off", "<u4"), ("strand", "u1"), ("dist", "u1"),
+                         ("primary", "u1"), ("cigar_len", "u1")])
 UNMAPPED_POS = 0xFFFFFFFF
 
 GENOME_UNIFORM, GENOME_REPEAT_FAMILIES, GENOME_TANDEM = 0, 1, 2
@@ -55,6 +58,10 @@ class Stats(C.Structure):
         d = {f: getattr(self, f) for f, _ in self._fields_}
         d["cand_hist"] = list(self.cand_hist)
         return d
+
+
+class Records(C.Structure):
+    _fields_ = [("n", C.c_uint64), ("rec", C.c_void_p), ("text_bytes", C.c_uint64), ("cigar", C.c_void_p)]
 
 
 class ServerConfig(C.Structure):
@@ -125,6 +132,12 @@ def lib():
             "swm_server_destroy": ([vp], None),
             "swm_host_alloc": ([C.POINTER(vp), u64], C.c_int),
             "swm_host_free": ([vp], C.c_int),
+            "swm_map_batch_records": ([vp, vp, u64, vp, u64, u32, vp, C.POINTER(Records)], C.c_int),
+            "swm_records_free": ([C.POINTER(Records)], None),
+            "swm_sam_open": ([C.c_char_p, C.c_char_p, C.c_int, C.POINTER(vp)], C.c_int),
+            "swm_sam_write_batch": ([vp, u64, C.POINTER(C.c_char_p), vp, vp, vp, u64, vp, C.POINTER(Records)],
+                                    C.c_int),
+            "swm_sam_close": ([vp], C.c_int),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -256,6 +269,23 @@ class Mapper:
         _check("swm_map_batch", lib().swm_map_batch(self._h, _ptr(reads), stride, _ptr(lens), n, e, _ptr(out)))
         return out
 
+    def map_records(self, reads, lens, e=5):
+        """swm_map_batch_records: (RESULT_DTYPE[n], RECORD_DTYPE[m], CIGAR text bytes)."""
+        reads = np.ascontiguousarray(reads, dtype=np.uint8)
+        lens = np.ascontiguousarray(lens, dtype=np.uint32)
+        n, stride = reads.shape
+        out = np.zeros(n, dtype=RESULT_DTYPE)
+        r = Records()
+        _check("swm_map_batch_records", lib().swm_map_batch_records(self._h, _ptr(reads), stride, _ptr(lens), n, e,
+                                                                    _ptr(out), C.byref(r)))
+        try:
+            recs = np.frombuffer(C.string_at(r.rec, r.n * RECORD_DTYPE.itemsize), dtype=RECORD_DTYPE).copy() \
+                if r.n else np.zeros(0, RECORD_DTYPE)
+            text = C.string_at(r.cigar, r.text_bytes) if r.text_bytes else b""
+        finally:
+            lib().swm_records_free(C.byref(r))
+        return out, recs, text
+
     def map_device(self, d_reads, stride, d_lens, n, e, d_out):
         """Device pointers (ints), e.g. torch tensors' data_ptr()."""
         _check("swm_map_batch_device", lib().swm_map_batch_device(self._h, C.c_void_p(d_reads), stride,
@@ -290,6 +320,49 @@ class Mapper:
         ch = (C.c_int64 * N_OPTIONS)()
         _check("swm_autotune", lib().swm_autotune(self._h, _ptr(reads), stride, _ptr(lens), n, e, reps, ch))
         return {OPTION_NAMES[i + 1]: ch[i] for i in range(N_OPTIONS)}
+
+
+def cigar_strings(recs, text):
+    """The CIGAR text of each record."""
+    return [text[int(o):int(o) + int(n)] for o, n in zip(recs["cigar_off"], recs["cigar_len"])]
+
+
+class SamWriter:
+    """swm_sam_*: the reference's SAM output (header, unmapped lines, record
+    lines) for batches mapped with Mapper.map_records."""
+
+    def __init__(self, path, index_prefix, gzip_level=1):
+        h = C.c_void_p()
+        _check("swm_sam_open", lib().swm_sam_open(path.encode(), index_prefix.encode(), gzip_level, C.byref(h)))
+        self._h = h
+
+    def write(self, names, reads, quals, lens, recs, text):
+        """names: list of bytes; reads / quals: (n, stride) uint8 (quals may be
+        None); recs / text as returned by Mapper.map_records."""
+        reads = np.ascontiguousarray(reads, dtype=np.uint8)
+        lens = np.ascontiguousarray(lens, dtype=np.uint32)
+        n, stride = reads.shape
+        if quals is not None:
+            quals = np.ascontiguousarray(quals, dtype=np.uint8)
+            assert quals.shape == reads.shape
+        recs = np.ascontiguousarray(recs, dtype=RECORD_DTYPE)
+        tbuf = np.frombuffer(text + b"\0", dtype=np.uint8)
+        arr = (C.c_char_p * max(n, 1))(*names)
+        nlen = np.array([len(x) for x in names], dtype=np.uint32)
+        r = Records(recs.size, recs.ctypes.data if recs.size else None, len(text), tbuf.ctypes.data)
+        _check("swm_sam_write_batch", lib().swm_sam_write_batch(self._h, n, arr, _ptr(nlen), _ptr(reads), _ptr(quals),
+                                                                stride, _ptr(lens), C.byref(r)))
+
+    def close(self):
+        if self._h:
+            h, self._h = self._h, None
+            _check("swm_sam_close", lib().swm_sam_close(h))
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Server:

@@ -27,7 +27,7 @@ HIP_FLAGS = [
 C_FLAGS = ["-O3", "-fPIC", "-std=gnu11", "-Wall", "-Wextra", "-Wno-unused-parameter", "-I" + INC, "-I" + CSRC]
 
 HIP_SOURCES = ["swm_map.hip", "swm_index.hip", "swm_serve.hip"]
-C_SOURCES = ["swm_io.c", "swm_synth.c", "swm_recipe.c"]
+C_SOURCES = ["swm_io.c", "swm_synth.c", "swm_recipe.c", "swm_sam.c"]
 CLIS = {"swmapper": "swm_cli_map.c", "swm_index": "swm_cli_index.c"}
 
 
@@ -60,7 +60,7 @@ def build(verbose=False):
         if verbose and out:
             sys.stderr.write(out)
     tmp = LIB + ".tmp"
-    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp, *objs])
+    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp, *objs, "-lz"])
     os.replace(tmp, LIB)
     for exe, src in CLIS.items():
         _run(["gcc", *C_FLAGS, "-o", os.path.join(HERE, exe), os.path.join(CSRC, src),

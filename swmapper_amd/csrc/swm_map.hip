@@ -885,15 +885,29 @@ __global__ void __launch_bounds__(256) k_bigsort(const uint2 *__restrict__ big, 
 /* --------------------------------------------------------------- k_compact */
 
 struct CandSet {
-    uint32_t *val;      /* read-start candidate */
-    uint32_t *rs;       /* (read << 1) | strand */
-    uint32_t *bi;       /* batch << 12 | index within batch */
-    int32_t *err;       /* -2: not verified (pos_start rejected) */
-    int32_t *loc;
-    uint32_t *rec;      /* record location (cigar->location) for passing lanes */
-    uint32_t *info;     /* CIGAR bytes | kind bit 16 (1 = traceback) | bit 17: record location < 0 */
-    const uint32_t *ovf;    /* capacity overflow flag (see PlanIO) */
+    uint32_t *val = nullptr;      /* read-start candidate */
+    uint32_t *rs = nullptr;       /* (read << 1) | strand */
+    uint32_t *bi = nullptr;       /* batch << 12 | index within batch */
+    int32_t *err = nullptr;       /* -2: not verified (pos_start rejected) */
+    int32_t *loc = nullptr;
+    uint32_t *rec = nullptr;      /* record location (cigar->location) for passing lanes */
+    uint32_t *info = nullptr;     /* CIGAR bytes | kind bit 16 (1 = traceback) | bit 17: record location < 0 */
+    const uint32_t *ovf = nullptr;    /* capacity overflow flag (see PlanIO) */
+    /* record output (null unless records are asked for): per candidate, 1
+     * if the emission writes its record, the bytes of its CIGAR text, and the
+     * traceback's merged runs */
+    uint64_t *wr = nullptr;
+    uint32_t *tb = nullptr;
+    uint16_t *runs = nullptr;     /* [ci * RUN_SLOT]: count, then runs in traceback order */
 };
+
+/* A traceback's CIGAR runs, M and S runs merged as the emission merges them
+ * (mapper_cpe.c:479-507): at most 2e+1 <= 31 entries (an I/D run or an S
+ * costs an error, the M groups sit between them), each size | kind << 8. */
+#define RUN_SLOT 32
+#define RUN_M 0u
+#define RUN_I 1u
+#define RUN_D 2u
 
 #define INFO_TRACE (1u << 16)
 #define INFO_NEG (1u << 17)
@@ -1400,13 +1414,21 @@ __global__ void __launch_bounds__(256) k_myers_explicit(uint32_t n, const uint32
  * registers (static indices, no scratch) and walks it from its last base.
  * The CIGAR byte count is accumulated as the runs close: adjacent M/S runs
  * form one "<n>M" group, every I or D run its own "<n>I"/"<n>D". */
-__device__ __forceinline__ void close_run(char c, int n, uint32_t &ms_acc, uint32_t &text)
+__device__ __forceinline__ void close_run(char c, int n, uint32_t &ms_acc, uint32_t &text, uint16_t *runs,
+                                          uint32_t &nr)
 {
     if (c == 'M' || c == 'S') {
         ms_acc += (uint32_t)n;
     } else {
-        if (ms_acc) { text += (uint32_t)ndigits(ms_acc) + 1u; ms_acc = 0; }
+        if (ms_acc) {
+            text += (uint32_t)ndigits(ms_acc) + 1u;
+            if (runs && nr < RUN_SLOT - 1) runs[1 + nr] = (uint16_t)(ms_acc | (RUN_M << 8));
+            nr++;
+            ms_acc = 0;
+        }
         text += (uint32_t)ndigits((uint32_t)n) + 1u;
+        if (runs && nr < RUN_SLOT - 1) runs[1 + nr] = (uint16_t)((uint32_t)n | ((c == 'I' ? RUN_I : RUN_D) << 8));
+        nr++;
     }
 }
 
@@ -1473,7 +1495,8 @@ __global__ void __launch_bounds__(64) k_trace(const uint32_t *__restrict__ trace
         bool first = true, done = false;
         char pre_char = 'N';
         int pre_size = 0;
-        uint32_t ms_acc = 0, text = 2;
+        uint32_t ms_acc = 0, text = 2, nr = 0;
+        uint16_t *runs = c.runs ? c.runs + (uint64_t)ci * RUN_SLOT : nullptr;
         for (int cc = (int)nch - 1; cc >= 0 && !done && i >= 0; cc--) {
             const uint32_t steps = ((uint32_t)cc == nfull) ? rem : 32u;
             vp = ck[2 * cc];
@@ -1515,15 +1538,20 @@ __global__ void __launch_bounds__(64) k_trace(const uint32_t *__restrict__ trace
                     else if (bh) { i--; search_site++; sum_err++; start_location++; nc = first ? 'S' : 'I'; }
                     else { search_site--; match_site--; sum_err++; start_location--; nc = 'D'; }
                     if (first) { pre_char = nc; pre_size = 1; first = false; }
-                    else if (nc != pre_char) { close_run(pre_char, pre_size, ms_acc, text); pre_char = nc; pre_size = 1; }
+                    else if (nc != pre_char) { close_run(pre_char, pre_size, ms_acc, text, runs, nr); pre_char = nc; pre_size = 1; }
                     else pre_size++;
                 }
             }
         }
 #undef TR_STEP
-        close_run(pre_char, pre_size, ms_acc, text);                                 /* :468-473 */
-        if (i >= 0) close_run('M', i + 1, ms_acc, text);
-        if (ms_acc) text += (uint32_t)ndigits(ms_acc) + 1u;
+        close_run(pre_char, pre_size, ms_acc, text, runs, nr);                       /* :468-473 */
+        if (i >= 0) close_run('M', i + 1, ms_acc, text, runs, nr);
+        if (ms_acc) {
+            text += (uint32_t)ndigits(ms_acc) + 1u;
+            if (runs && nr < RUN_SLOT - 1) runs[1 + nr] = (uint16_t)(ms_acc | (RUN_M << 8));
+            nr++;
+        }
+        if (runs) runs[0] = (uint16_t)nr;
         const uint64_t new_loc = (uint64_t)st + (uint64_t)(int64_t)start_location + 1;
         c.rec[ci] = (uint32_t)new_loc;
         c.info[ci] = (text & 0xffffu) | INFO_TRACE | ((new_loc >> 32) ? INFO_NEG : 0u);
@@ -1554,7 +1582,8 @@ struct Emit {
     {
         if (cigar_num + 8 > 256 || cigar_len + 16 * 16 > 256 * 16) { cigar_num = 0; cigar_len = 0; }
     }
-    __device__ void step(uint32_t bi, int32_t er, uint32_t rec, uint32_t info, uint32_t e, uint32_t strand)
+    /* true when the candidate's record is written */
+    __device__ bool step(uint32_t bi, int32_t er, uint32_t rec, uint32_t info, uint32_t e, uint32_t strand)
     {
         const uint32_t b = bi >> 12;
         if (b != cur_batch) {
@@ -1562,17 +1591,18 @@ struct Emit {
             cur_batch = b;
             vcount = 0;
         }
-        if (er == -2) return;
+        if (er == -2) return false;
         if (open && (vcount & 7u) == 0) { flush_check(); open = false; }
         open = true;
         vcount++;
-        if (er > (int32_t)e) return;
-        if ((info & INFO_TRACE) && cigar_num > 0 && !(info & INFO_NEG) && last_loc == rec) return;
+        if (er > (int32_t)e) return false;
+        if ((info & INFO_TRACE) && cigar_num > 0 && !(info & INFO_NEG) && last_loc == rec) return false;
         cigar_num++;
         cigar_len += info & 0xffffu;
         last_loc = rec;
         if (count == 0 || (uint32_t)er < best_dist) { best_pos = rec; best_dist = (uint32_t)er; best_strand = strand; }
         count++;
+        return true;
     }
     __device__ void end_strand()
     {
@@ -1621,7 +1651,12 @@ __global__ void __launch_bounds__(128) k_emit(uint32_t nreads, uint32_t e, const
             for (uint32_t ci = cand_off[rs]; ci < cend; ci++) {
                 const int32_t er = c.err[ci];
                 const bool pass = er >= 0 && er <= (int32_t)e;
-                em.step(c.bi[ci], er, pass ? c.rec[ci] : 0u, pass ? c.info[ci] : 0u, e, strand);
+                const uint32_t info = pass ? c.info[ci] : 0u;
+                const bool w = em.step(c.bi[ci], er, pass ? c.rec[ci] : 0u, info, e, strand);
+                if (c.wr) {
+                    c.wr[ci] = w ? 1ull : 0ull;
+                    c.tb[ci] = w ? (info & 0xffffu) - 2u : 0u;
+                }
             }
             em.end_strand();
         }
@@ -1684,7 +1719,7 @@ __global__ void __launch_bounds__(64) k_emit_heavy(const uint32_t *__restrict__ 
             uint32_t num = 0, len = 0;          /* output block counters */
             /* the open group */
             bool g_open = false, g_seen = false, g_firstd = false;
-            uint32_t g_n = 0, g_len = 0, g_ftext = 0, g_frec = 0;
+            uint32_t g_n = 0, g_len = 0, g_ftext = 0, g_frec = 0, g_fci = 0;
             uint64_t g_fkey = 0;
             auto close_group = [&]() {
                 if (!g_open) return;
@@ -1693,6 +1728,11 @@ __global__ void __launch_bounds__(64) k_emit_heavy(const uint32_t *__restrict__ 
                     len = g_ftext;
                     count++;
                     if (g_fkey < best) { best = g_fkey; best_rec = g_frec; }
+                    /* the lane that stored this candidate's "not written" flag overwrites it */
+                    if (c.wr && lane == ((g_fci - c0) & 63u)) {
+                        c.wr[g_fci] = 1ull;
+                        c.tb[g_fci] = g_ftext - 2u;
+                    }
                 }
                 num += g_n;
                 len += g_len;
@@ -1744,6 +1784,10 @@ __global__ void __launch_bounds__(64) k_emit_heavy(const uint32_t *__restrict__ 
                 const uint64_t gm = __ballot(valid && (v & 7u) == 0);
                 /* plainly written lanes */
                 const bool wr = pass && !dup;
+                if (c.wr && in) {
+                    c.wr[ci] = wr ? 1ull : 0ull;
+                    c.tb[ci] = wr ? (info & 0xffffu) - 2u : 0u;
+                }
                 const uint64_t am = __ballot(wr);
                 const uint32_t tinc = wave_incl_scan<uint32_t>(wr ? (info & 0xffffu) : 0u);
                 const uint64_t key = wr ? (((uint64_t)(uint32_t)er << 40) | ((uint64_t)strand << 32) | (ci - c0)) : ~0ull;
@@ -1769,6 +1813,7 @@ __global__ void __launch_bounds__(64) k_emit_heavy(const uint32_t *__restrict__ 
                             g_firstd = true;
                             g_ftext = rdlane(info, f) & 0xffffu;
                             g_frec = rdlane(rec, f);
+                            g_fci = base + f;
                             g_fkey = ((uint64_t)rdlane((uint32_t)er, f) << 40) | ((uint64_t)strand << 32) |
                                      (uint64_t)(base + f - c0);
                         }
@@ -1801,6 +1846,63 @@ __global__ void __launch_bounds__(64) k_emit_heavy(const uint32_t *__restrict__ 
             out[rd] = res;
             if (count) atomicAdd(n_records, (unsigned long long)count);
         }
+    }
+}
+
+/* ---------------------------------------------------------------- k_records */
+
+/* Decimal digits of v (none for 0, as sprint_length, mapper_cpe.c:214-228). */
+__device__ __forceinline__ uint32_t put_dec(char *o, uint32_t v)
+{
+    const uint32_t n = (uint32_t)ndigits(v);
+    for (uint32_t k = n; k > 0; k--) {
+        o[k - 1] = (char)('0' + v % 10u);
+        v /= 10u;
+    }
+    return n;
+}
+
+/* Every written record at its rank among the written candidates (ridx, the
+ * exclusive scan of wr) with its CIGAR text at toff (the scan of tb): "<L>M"
+ * for a Hamming record (mapper_cpe.c:283-287), else the traceback's merged
+ * runs from the last recorded to the first (:479-507).  A record is primary
+ * when no candidate of its (read, strand) before it was written (main_flag,
+ * :1462-1465). */
+__global__ void __launch_bounds__(256) k_records(const uint32_t *__restrict__ ncand_p, const uint32_t *__restrict__ lens,
+                                                 const uint32_t *__restrict__ cand_off, CandSet c,
+                                                 const uint64_t *__restrict__ ridx, const uint32_t *__restrict__ toff,
+                                                 swm_record_t *__restrict__ out, char *__restrict__ text)
+{
+    const uint32_t ncand = *ncand_p;
+    for (uint32_t ci = blockIdx.x * blockDim.x + threadIdx.x; ci < ncand; ci += gridDim.x * blockDim.x) {
+        if (!c.wr[ci]) continue;
+        const uint32_t rs = c.rs[ci];
+        const uint64_t r = ridx[ci];
+        const uint32_t off = toff[ci];
+        char *o = text + off;
+        uint32_t len = 0;
+        if (c.info[ci] & INFO_TRACE) {
+            const uint16_t *ru = c.runs + (uint64_t)ci * RUN_SLOT;
+            const uint32_t nr = min((uint32_t)ru[0], (uint32_t)RUN_SLOT - 1u);
+            for (uint32_t k = nr; k > 0; k--) {
+                const uint32_t v = ru[k];
+                len += put_dec(o + len, v & 0xffu);
+                const uint32_t kind = v >> 8;
+                o[len++] = kind == RUN_M ? 'M' : (kind == RUN_I ? 'I' : 'D');
+            }
+        } else {
+            len = put_dec(o, lens[rs >> 1]);
+            o[len++] = 'M';
+        }
+        swm_record_t x;
+        x.read = rs >> 1;
+        x.pos = c.rec[ci];
+        x.cigar_off = off;
+        x.strand = (uint8_t)(rs & 1u);
+        x.dist = (uint8_t)c.err[ci];
+        x.primary = ridx[cand_off[rs]] == r ? 1 : 0;
+        x.cigar_len = (uint8_t)len;
+        out[r] = x;
     }
 }
 
@@ -1927,7 +2029,7 @@ struct swm_ws {
     struct Buf { void *p = nullptr; size_t cap = 0; };
     enum { WORDS, SEL, NSEL, NPOS, NCHUNK, SURV_OFF, BD_OFF, SURV, BDESC, NBATCH, NUNIQ, CAND_OFF, C_VAL, C_RS, C_BI,
            C_ERR, C_LOC, C_REC, C_INFO, COUNTERS, BIG, TRACE, HEAVY, HEAVY_C, HEAVY_E, SCAN_TMP, IN_READS, IN_LENS,
-           OUT, HIST, NBUF };
+           OUT, HIST, R_WR, R_TB, R_RUNS, R_IDX, R_TOFF, R_SCAN, R_TOT, R_OUT, R_TEXT, NBUF };
     Buf b[NBUF];
     uint64_t gen = 1;
     ~swm_ws() { for (auto &x : b) if (x.p) (void)hipFree(x.p); }
@@ -2264,8 +2366,13 @@ static void observe(swm_slot *sl, uint64_t n, uint64_t tot_pos, uint64_t tot_chu
  * stages A and B (two host round trips). */
 /* h_out (page-locked, may be NULL): the first n_out results are copied there
  * before the run's last synchronisation. */
+static int records_out(const swm_ctx_t *ctx, swm_slot *sl, Pipe &P, const uint32_t *d_lens, uint32_t ncand,
+                       swm_records_t *recs);
+
+/* recs (may be NULL): also collect every record and its CIGAR text. */
 static int run_synced(swm_ctx_t *ctx, swm_slot *sl, const char *d_reads, uint64_t stride, const uint32_t *d_lens,
-                      uint64_t n, uint32_t e, swm_read_result_t *d_out, swm_read_result_t *h_out, uint64_t n_out)
+                      uint64_t n, uint32_t e, swm_read_result_t *d_out, swm_read_result_t *h_out, uint64_t n_out,
+                      swm_records_t *recs = nullptr)
 {
     hipStream_t st = sl->stream;
     swm_ws *ws = &sl->ws;
@@ -2290,7 +2397,12 @@ static int run_synced(swm_ctx_t *ctx, swm_slot *sl, const char *d_reads, uint64_
     SWM_HIP_CHECK(hipStreamSynchronize(st));
     const uint32_t ncand = (uint32_t)hs[10];
     if ((rc = pipe_alloc_c(ws, ncand, P))) return rc;
+    if (recs && ((rc = ws->get(swm_ws::R_WR, &P.cs.wr, (uint64_t)ncand + 1)) ||
+                 (rc = ws->get(swm_ws::R_TB, &P.cs.tb, (uint64_t)ncand + 1)) ||
+                 (rc = ws->get(swm_ws::R_RUNS, &P.cs.runs, ((uint64_t)ncand + 1) * RUN_SLOT))))
+        return rc;
     if ((rc = enq_c(ctx, P, d_lens, e, d_out, ncand, st, sl->ev))) return rc;
+    if (recs && (rc = records_out(ctx, sl, P, d_lens, ncand, recs))) return rc;
     SWM_HIP_CHECK(hipMemcpyAsync(hs, P.counters, 8 * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
     if (h_out && n_out)
         SWM_HIP_CHECK(hipMemcpyAsync(h_out, d_out, n_out * sizeof(swm_read_result_t), hipMemcpyDeviceToHost, st));
@@ -2305,6 +2417,71 @@ static int run_synced(swm_ctx_t *ctx, swm_slot *sl, const char *d_reads, uint64_
     (void)hipEventElapsedTime(&ms, sl->ev[3], sl->ev[4]); sl->stats.ms_myers = ms;
     (void)hipEventElapsedTime(&ms, sl->ev[4], sl->ev[5]); sl->stats.ms_emit = ms;
     (void)hipEventElapsedTime(&ms, sl->ev[0], sl->ev[5]); sl->stats.ms_total = ms;
+    return SWM_OK;
+}
+
+/* The records of the run just enqueued: ranks and text offsets by one scan
+ * over the candidates, the totals read back, then k_records and the copies
+ * to library-owned host memory. */
+static int records_out(const swm_ctx_t *ctx, swm_slot *sl, Pipe &P, const uint32_t *d_lens, uint32_t ncand,
+                       swm_records_t *recs)
+{
+    memset(recs, 0, sizeof(*recs));
+    if (ncand == 0) return SWM_OK;
+    swm_ws *ws = &sl->ws;
+    hipStream_t st = sl->stream;
+    uint64_t *ridx, *sums;
+    uint32_t *toff;
+    unsigned long long *tot;
+    int rc;
+    const uint32_t nb = blocks_for(ncand, SCAN_TILE);
+    if ((rc = ws->get(swm_ws::R_IDX, &ridx, (uint64_t)ncand + 1)) || (rc = ws->get(swm_ws::R_TOFF, &toff, (uint64_t)ncand + 1)) ||
+        (rc = ws->get(swm_ws::R_SCAN, &sums, 2ull * nb)) || (rc = ws->get(swm_ws::R_TOT, &tot, 4)))
+        return rc;
+    ScanJob j{};
+    j.in0 = P.cs.wr; j.out0 = ridx;
+    j.in1 = P.cs.tb; j.out1 = toff;
+    j.n = ncand;
+    j.sums = sums;
+    j.first = 0;
+    j.cap[0] = ~0ull; j.cap[1] = 0xffffffffull;
+    j.tot[0] = tot; j.tot[1] = tot + 1;
+    j.ovf = reinterpret_cast<uint32_t *>(tot + 2);
+    SWM_HIP_CHECK(hipMemsetAsync(tot, 0, 4 * sizeof(unsigned long long), st));
+    scan_launch(j, 2, st);
+    SWM_HIP_CHECK(hipGetLastError());
+    uint64_t *hs = sl->h_small;
+    SWM_HIP_CHECK(hipMemcpyAsync(hs + 11, tot, 3 * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+    SWM_HIP_CHECK(hipStreamSynchronize(st));
+    const uint64_t nrec = hs[11], nbytes = hs[12];
+    if ((uint32_t)hs[13]) return SWM_ENOMEM;           /* CIGAR text beyond 4 GiB in one batch */
+    if (nrec == 0) return SWM_OK;
+    swm_record_t *d_rec;
+    char *d_text;
+    if ((rc = ws->get(swm_ws::R_OUT, &d_rec, nrec)) || (rc = ws->get(swm_ws::R_TEXT, &d_text, nbytes + 1))) return rc;
+    const uint32_t grid = std::min<uint32_t>(blocks_for(ncand, 256), (uint32_t)ctx->n_cu * 16);
+    k_records<<<grid, 256, 0, st>>>(P.cand_off + P.nrs, d_lens, P.cand_off, P.cs, ridx, toff, d_rec, d_text);
+    SWM_HIP_CHECK(hipGetLastError());
+    recs->rec = (swm_record_t *)malloc(nrec * sizeof(swm_record_t));
+    recs->cigar = (char *)malloc(nbytes + 1);
+    if (!recs->rec || !recs->cigar) {
+        free(recs->rec);
+        free(recs->cigar);
+        memset(recs, 0, sizeof(*recs));
+        (void)hipStreamSynchronize(st);
+        return SWM_ENOMEM;
+    }
+    if (hipMemcpyAsync(recs->rec, d_rec, nrec * sizeof(swm_record_t), hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipMemcpyAsync(recs->cigar, d_text, nbytes, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess) {
+        free(recs->rec);
+        free(recs->cigar);
+        memset(recs, 0, sizeof(*recs));
+        return SWM_EDEVICE;
+    }
+    recs->cigar[nbytes] = 0;
+    recs->n = nrec;
+    recs->text_bytes = nbytes;
     return SWM_OK;
 }
 
@@ -2663,6 +2840,48 @@ extern "C" int swm_map_batch(swm_ctx_t *ctx, const char *reads, uint64_t stride,
     if ((rc = swm_check_lens(ctx, stride, lens, n))) return rc;
     if (hipSetDevice(ctx->device) != hipSuccess) return SWM_EDEVICE;
     return swm_slot_run_host(ctx, ctx->slot0, reads, stride, lens, n, n, e, out, ctx->opt.graph != 0);
+}
+
+extern "C" int swm_map_batch_records(swm_ctx_t *ctx, const char *reads, uint64_t stride, const uint32_t *lens,
+                                     uint64_t n, uint32_t e, swm_read_result_t *out, swm_records_t *recs)
+{
+    if (!recs) return SWM_EINVAL;
+    memset(recs, 0, sizeof(*recs));
+    int rc = check_map_args(ctx, stride, e);
+    if (rc) return rc;
+    if (n == 0) { memset(&ctx->slot0->stats, 0, sizeof(swm_stats_t)); return SWM_OK; }
+    if (!reads || !lens || !out) return SWM_EINVAL;
+    if (2 * n >= 0xfffffff0ull) return SWM_EINVAL;
+    if ((rc = swm_check_lens(ctx, stride, lens, n))) return rc;
+    if (hipSetDevice(ctx->device) != hipSuccess) return SWM_EDEVICE;
+    swm_slot *sl = ctx->slot0;
+    sl->t_entry = host_ms();
+    char *d_reads;
+    uint32_t *d_lens;
+    swm_read_result_t *d_out;
+    if ((rc = sl->ws.get(swm_ws::IN_READS, &d_reads, n * stride + 4)) || (rc = sl->ws.get(swm_ws::IN_LENS, &d_lens, n)) ||
+        (rc = sl->ws.get(swm_ws::OUT, &d_out, n)))
+        return rc;
+    hipStream_t st = sl->stream;
+    SWM_HIP_CHECK(hipMemcpyAsync(d_reads, reads, n * stride, hipMemcpyHostToDevice, st));
+    SWM_HIP_CHECK(hipMemcpyAsync(d_lens, lens, n * 4, hipMemcpyHostToDevice, st));
+    if ((rc = run_synced(ctx, sl, d_reads, stride, d_lens, n, e, d_out, nullptr, 0, recs))) {
+        swm_records_free(recs);
+        return rc;
+    }
+    if (hipMemcpy(out, d_out, n * sizeof(swm_read_result_t), hipMemcpyDeviceToHost) != hipSuccess) {
+        swm_records_free(recs);
+        return SWM_EDEVICE;
+    }
+    return SWM_OK;
+}
+
+extern "C" void swm_records_free(swm_records_t *recs)
+{
+    if (!recs) return;
+    free(recs->rec);
+    free(recs->cigar);
+    memset(recs, 0, sizeof(*recs));
 }
 
 extern "C" int swm_myers_device(swm_ctx_t *ctx, const uint32_t *d_read_words, const uint32_t *d_lens,

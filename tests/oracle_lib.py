@@ -12,7 +12,7 @@ ORACLE_SO = os.path.join(ROOT, "oracle", "_build", "libswm_oracle.so")
 
 HIT_DTYPE = np.dtype([("pos", "<u4"), ("strand", "u1"), ("dist", "u1"), ("pad", "<u2"), ("count", "<u4")])
 REC_DTYPE = np.dtype([("read", "<u4"), ("strand", "u1"), ("dist", "u1"), ("block", "<u2"), ("loc", "<u4"),
-                      ("text_len", "<u4")])
+                      ("text_len", "<u4"), ("primary", "u1"), ("pad", "u1", (3,)), ("cigar", "S160")])
 CAND_DTYPE = np.dtype([("read", "<u4"), ("strand", "u1"), ("pad0", "u1"), ("batch", "<u2"), ("idx", "<u4"),
                        ("cand", "<u4"), ("win_start", "<u4"), ("ref_len", "<u4"), ("err", "<i4"), ("loc", "<i4")])
 
