@@ -195,6 +195,49 @@ def serve_bench(swm, m, g, e, seconds, log, slots=2, graph=True, merge=262144):
             "peak_admitted_bytes": tot["peak_bytes"], "hbm_budget_bytes": tot["budget"]}
 
 
+def records_bench(swm, m, reads, lens, e, ref_out, n_bases, log):
+    """The records path (every record with its CIGAR text, host buffers in
+    and out) on the whole batch, then the SAM writer (gzip level 1, as the
+    reference's <out>.sam.gz) on the first 100k reads' records."""
+    import tempfile
+    n = lens.size
+    m.map_records(reads[:1000], lens[:1000], e)                 # grows the record buffers
+    t0 = time.perf_counter()
+    res, recs, text = m.map_records(reads, lens, e)
+    dt = time.perf_counter() - t0
+    same = all(np.array_equal(res[f], ref_out[f]) for f in ("count", "pos", "strand", "dist"))
+    k = min(n, 100_000)
+    sub = recs[recs["read"] < k]
+    with tempfile.TemporaryDirectory(dir="/tmp") as td:
+        with open(os.path.join(td, "ix.ref_info"), "wb") as f:
+            f.write(b"%d\n9\nsynthetic\n" % n_bases)
+        names = [b"r%d" % i for i in range(k)]
+        quals = np.full((k, reads.shape[1]), ord("I"), np.uint8)
+        path = os.path.join(td, "out.sam.gz")
+        t1 = time.perf_counter()
+        w = swm.SamWriter(path, os.path.join(td, "ix"), gzip_level=1)
+        w.write(names, reads[:k], quals, lens[:k], sub, text)
+        w.close()
+        ds = time.perf_counter() - t1
+        gz_bytes = os.path.getsize(path)
+        path = os.path.join(td, "out.sam")
+        t2 = time.perf_counter()
+        w = swm.SamWriter(path, os.path.join(td, "ix"), gzip_level=0)
+        w.write(names, reads[:k], quals, lens[:k], sub, text)
+        w.close()
+        dp = time.perf_counter() - t2
+        sam_bytes = os.path.getsize(path)
+    out = {"entry": "swm_map_batch_records", "ms": round(dt * 1e3, 3), "reads_per_s": round(n / dt, 1),
+           "records": int(recs.size), "cigar_bytes": len(text), "identical_results": same,
+           "note": "wall time: host reads in (pageable), synced pipeline, record compaction, records + CIGAR "
+                   "text back to host memory",
+           "sam": {"reads": k, "records": int(sub.size), "sam_bytes": sam_bytes, "gzip_bytes": gz_bytes,
+                   "gzip1_ms": round(ds * 1e3, 1), "plain_ms": round(dp * 1e3, 1),
+                   "plain_mb_per_s": round(sam_bytes / dp / 1e6, 1), "gzip1_mb_per_s": round(sam_bytes / ds / 1e6, 1)}}
+    log(f"records: {out}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -216,6 +259,9 @@ def main():
                          "slots' streams each get a hardware queue")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target length of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-records", action="store_true", help="skip the records + SAM measurement")
+    ap.add_argument("--set", action="append", default=[], metavar="NAME=VALUE",
+                    help="set a tunable (swm_ctx_set_option by name, e.g. seed_cache=0) before timing")
     ap.add_argument("--copy-probe", action="store_true",
                     help="one 256 MiB device-to-device copy first (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -265,6 +311,11 @@ def main():
     d_lens = swm.DeviceBuffer.from_array(lens)
     d_out = swm.DeviceBuffer(n * swm.RESULT_DTYPE.itemsize).zero()
     e = cfg["e"]
+    for kv in args.set:
+        name, val = kv.split("=", 1)
+        opt = {v: k for k, v in swm.OPTION_NAMES.items()}[name]
+        m.set_option(opt, int(val))
+        log(f"option {name} = {val}")
     tuned = None
     if args.autotune:
         k = min(n, 100_000)
@@ -305,6 +356,7 @@ def main():
     m.set_option(swm.OPT_HISTOGRAM, 0)
     hist = {("0" if b == 0 else f"{1 << (b - 1)}-{(1 << b) - 1}"): int(v) for b, v in enumerate(hs["cand_hist"]) if v}
     log(f"candidates per read: {hist}, max {hs['max_cands']}")
+    records = None if args.no_records else records_bench(swm, m, reads, lens, e, ref_out, cfg["n"], log)
     best = min(results, key=lambda f: results[f]["elapsed"])
     R = results[best]
     nv = R["nv"]
@@ -412,7 +464,7 @@ def main():
         "config": {"workload": cfg["desc"], "name": args.config, "form": best, "reads": n, "e": e,
                    "tuned": tuned},
         "roofline": roof, "roofline_plan": roof_plan, "roofline_myers": roof_myers, "index_build": index_build,
-        "serving": serving, "cpu_baseline": cpu,
+        "serving": serving, "records": records, "cpu_baseline": cpu,
     }
     print(json.dumps(res), flush=True)
 

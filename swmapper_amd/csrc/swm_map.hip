@@ -190,14 +190,32 @@ __device__ __forceinline__ uint32_t seed_val(const Base &b, uint32_t u, uint32_t
     return b.valid ? b.code : (u < k ? (u & 3u) : 3u);
 }
 
+/* The 12-mer codes of long seed m at shifts q = 0..3 (choose_seed :1372-1390). */
+__device__ __forceinline__ void seed_codes(const uint8_t *rd, uint32_t L, int strand, uint32_t k, uint32_t base,
+                                           uint32_t cq[4])
+{
+    uint32_t code = 0;
+    for (uint32_t t = 0; t < k; t++) code |= seed_val(read_base(rd, L, strand, base + t), t, k) << (2 * t);
+    cq[0] = code;
+#pragma unroll
+    for (int q = 1; q < 4; q++) {
+        const uint32_t u = k - 1 + (uint32_t)q;
+        code = (code >> 2) | (seed_val(read_base(rd, L, strand, base + u), u, k) << (2 * (k - 1)));
+        cq[q] = code;
+    }
+}
+
+/* The offset and count of every 12-mer of the thread's strand are kept in
+ * LDS from the ranking gathers, so the selected ones are not fetched twice.
+ * That takes 80 KB of LDS per workgroup (one wave per SIMD), and still wins:
+ * the kernel is bound by the rate of random line fetches, not by latency --
+ * gathering the selected 12-mers again with twice the waves was 25% slower. */
 __global__ void __launch_bounds__(128) k_seed(const char *__restrict__ reads, uint64_t stride,
                                               const uint32_t *__restrict__ lens, uint32_t nreads, uint32_t k,
                                               uint32_t e, uint32_t seln, const uint32_t *__restrict__ hash_offset,
                                               const swm_recipe_t *__restrict__ recipes, SeedOut o)
 {
     __shared__ uint32_t s_rd32[SEED_READS * 256 / 4];
-    /* offset and count of every 12-mer of the thread's strand, kept from the
-     * gathers so that the selected ones are not fetched twice */
     __shared__ uint32_t s_off[64 * 2 * SEED_READS], s_cnt[64 * 2 * SEED_READS];
     uint8_t *s_rd = reinterpret_cast<uint8_t *>(s_rd32);
     if (blockIdx.x == 0) {
@@ -277,16 +295,8 @@ __global__ void __launch_bounds__(128) k_seed(const char *__restrict__ reads, ui
         idx[m] = (uint32_t)m;
         if ((uint32_t)m < n_long) {
             const uint32_t base = (uint32_t)m * seed_len;
-            uint32_t code = 0;
-            for (uint32_t t = 0; t < k; t++) code |= seed_val(read_base(rd, L, strand, base + t), t, k) << (2 * t);
             uint32_t cq[4];
-            cq[0] = code;
-#pragma unroll
-            for (int q = 1; q < 4; q++) {
-                const uint32_t u = k - 1 + (uint32_t)q;
-                code = (code >> 2) | (seed_val(read_base(rd, L, strand, base + u), u, k) << (2 * (k - 1)));
-                cq[q] = code;
-            }
+            seed_codes(rd, L, strand, k, base, cq);
             uint32_t a[4], b[4];
 #pragma unroll
             for (int q = 0; q < 4; q++) { a[q] = hash_offset[cq[q]]; b[q] = hash_offset[cq[q] + 1]; }
