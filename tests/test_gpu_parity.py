@@ -31,11 +31,25 @@ def _compare_index(gi, oi):
     (65_552, 0, 12),         # N % 16 == 0, N % 32 != 0
     (200_000, 2, 10),        # tandem repeats, k = 10
     (4_000, 1, 8),
+    (1_000_032, 1, 12),      # repeat families, N % 32 == 0
+    (4_000_000, 0, 12),      # the small configuration's genome size
+    (300_001, 0, 14),        # k = 14: 2k + 12 + start bits = 63, still the 64-bit key sort
 ])
 def test_index_build_matches_oracle(n, kind, k):
     _say("index", n, kind, k)
     g = wl.genome(n, seed=n, kind=kind, frac=0.3)
     _compare_index(swm.build_index(g, k), orc.OracleIndex(g, k))
+
+
+def test_index_build_n_runs():
+    """Long runs of N (many starts dropped from the histogram) and N in the
+    last bases."""
+    g = wl.genome(400_000, seed=9).copy()
+    g[10_000:60_000] = ord("N")
+    g[200_000:200_013] = ord("n")
+    g[-3:] = ord("N")
+    _say("index N runs", g.size)
+    _compare_index(swm.build_index(g, 12), orc.OracleIndex(g, 12))
 
 
 def test_index_build_with_n_and_lowercase():
