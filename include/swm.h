@@ -180,6 +180,9 @@ int swm_sam_open(const char *path, const char *index_prefix, int gzip_level, swm
 int swm_sam_write_batch(swm_sam_t *w, uint64_t n, const char *const *names, const uint32_t *name_lens,
                         const char *seqs, const char *quals, uint64_t stride, const uint32_t *lens,
                         const swm_records_t *recs);
+/* Worker threads that format and compress a batch (default: the online
+ * CPUs, at most 16).  The text is the same for any count. */
+int swm_sam_set_threads(swm_sam_t *w, int n);
 /* Flushes and closes; returns the first write error, if any. */
 int swm_sam_close(swm_sam_t *w);
 

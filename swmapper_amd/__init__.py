@@ -138,6 +138,7 @@ def lib():
             "swm_sam_write_batch": ([vp, u64, C.POINTER(C.c_char_p), vp, vp, vp, u64, vp, C.POINTER(Records)],
                                     C.c_int),
             "swm_sam_close": ([vp], C.c_int),
+            "swm_sam_set_threads": ([vp, C.c_int], C.c_int),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -331,10 +332,12 @@ class SamWriter:
     """swm_sam_*: the reference's SAM output (header, unmapped lines, record
     lines) for batches mapped with Mapper.map_records."""
 
-    def __init__(self, path, index_prefix, gzip_level=1):
+    def __init__(self, path, index_prefix, gzip_level=1, threads=None):
         h = C.c_void_p()
         _check("swm_sam_open", lib().swm_sam_open(path.encode(), index_prefix.encode(), gzip_level, C.byref(h)))
         self._h = h
+        if threads is not None:
+            _check("swm_sam_set_threads", lib().swm_sam_set_threads(h, threads))
 
     def write(self, names, reads, quals, lens, recs, text):
         """names: list of bytes; reads / quals: (n, stride) uint8 (quals may be

@@ -16,6 +16,13 @@
  * of the next one, and the last base of the reference as base 0 of an
  * unnamed contig, as the reference does.  The distance digit is '0' + d
  * (a single character, also for d >= 10).
+ *
+ * A batch is cut into items of up to 8192 lines (its unmapped reads first,
+ * then its records), which worker threads format -- and with gzip compress
+ * into one gzip member each -- in parallel; the items are written in order.
+ * Concatenated members are one gzip stream (RFC 1952), so the file reads
+ * back as the same text whatever the thread count.  The reference writes
+ * through one gzFile at level 1 (mapper_mpe.c:866).
  */
 #include <errno.h>
 #include <stdint.h>
@@ -23,46 +30,80 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <pthread.h>
+#include <unistd.h>
 #include <zlib.h>
 
 #include "swm.h"
 
 struct swm_sam {
     FILE *f;
-    gzFile gz;
+    int level;              /* 0 plain text, 1..9 gzip members */
+    int threads;
     int err;
     uint32_t n_contigs;
     uint32_t *start;        /* n_contigs + 1 cumulative starts (u32, as the reference's ref_len) */
     char **name;            /* n_contigs + 1 (the last one empty) */
     uint32_t *name_len;     /* bytes written for RNAME (the .ref_info name-length line) */
-    char *line;
-    size_t cap;
-    char *rseq, *rqual;     /* reverse-complement scratch */
-    size_t rcap;
 };
 
-static void out(swm_sam_t *w, const char *p, size_t n)
+/* A growable byte buffer. */
+typedef struct {
+    char *p;
+    size_t n, cap;
+} buf_t;
+
+static char *buf_room(buf_t *b, size_t more)
 {
-    if (w->err || n == 0) return;
-    if (w->gz) {
-        if (gzwrite(w->gz, p, (unsigned)n) != (int)n) w->err = SWM_EIO;
-    } else if (fwrite(p, 1, n, w->f) != n) {
-        w->err = SWM_EIO;
+    if (b->n + more > b->cap) {
+        size_t c = b->cap ? b->cap : 65536;
+        while (c < b->n + more) c *= 2;
+        char *q = (char *)realloc(b->p, c);
+        if (!q) return NULL;
+        b->p = q;
+        b->cap = c;
     }
+    return b->p + b->n;
 }
 
-/* One line buffer, grown to fit the longest line. */
-static char *need(swm_sam_t *w, size_t n)
+/* One gzip member (RFC 1952) of the text: concatenated members are one
+ * valid gzip stream, so chunks compress in parallel. */
+static int gzip_member(const char *src, size_t n, int level, buf_t *dst)
 {
-    if (n > w->cap) {
-        size_t c = w->cap ? w->cap : 1024;
-        while (c < n) c *= 2;
-        char *p = (char *)realloc(w->line, c);
-        if (!p) return NULL;
-        w->line = p;
-        w->cap = c;
-    }
-    return w->line;
+    z_stream zs;
+    memset(&zs, 0, sizeof(zs));
+    if (deflateInit2(&zs, level, Z_DEFLATED, 15 + 16, 8, Z_DEFAULT_STRATEGY) != Z_OK) return SWM_ENOMEM;
+    const uLong bound = deflateBound(&zs, (uLong)n) + 64;
+    char *o = buf_room(dst, bound);
+    if (!o) { deflateEnd(&zs); return SWM_ENOMEM; }
+    zs.next_in = (Bytef *)src;
+    zs.avail_in = (uInt)n;
+    zs.next_out = (Bytef *)o;
+    zs.avail_out = (uInt)bound;
+    const int r = deflate(&zs, Z_FINISH);
+    const size_t made = bound - zs.avail_out;
+    deflateEnd(&zs);
+    if (r != Z_STREAM_END) return SWM_EIO;
+    dst->n += made;
+    return SWM_OK;
+}
+
+static void emit(swm_sam_t *w, const char *p, size_t n)
+{
+    if (w->err || n == 0) return;
+    if (fwrite(p, 1, n, w->f) != n) w->err = SWM_EIO;
+}
+
+/* Text out: compressed to one member when gzip is on. */
+static void emit_text(swm_sam_t *w, const char *p, size_t n)
+{
+    if (w->err || n == 0) return;
+    if (w->level == 0) { emit(w, p, n); return; }
+    buf_t z = {0, 0, 0};
+    const int rc = gzip_member(p, n, w->level, &z);
+    if (rc) w->err = rc;
+    else emit(w, z.p, z.n);
+    free(z.p);
 }
 
 /* append_int (mapper_mpe.c:410-425) on an int: digits of n, most
@@ -148,9 +189,6 @@ static void free_writer(swm_sam_t *w)
     free(w->name);
     free(w->name_len);
     free(w->start);
-    free(w->line);
-    free(w->rseq);
-    free(w->rqual);
     free(w);
 }
 
@@ -162,32 +200,39 @@ int swm_sam_open(const char *path, const char *index_prefix, int gzip_level, swm
     if (!w) return SWM_ENOMEM;
     int rc = load_contigs(w, index_prefix);
     if (rc != SWM_OK) { free_writer(w); return rc; }
-    if (gzip_level > 0) {
-        char mode[4] = {'w', (char)('0' + gzip_level), 0, 0};
-        w->gz = gzopen(path, mode);
-        if (!w->gz) { free_writer(w); return SWM_EIO; }
-    } else {
-        w->f = fopen(path, "wb");
-        if (!w->f) { free_writer(w); return SWM_EIO; }
-    }
+    w->level = gzip_level;
+    long nc = sysconf(_SC_NPROCESSORS_ONLN);
+    w->threads = nc < 1 ? 1 : (nc > 16 ? 16 : (int)nc);
+    w->f = fopen(path, "wb");
+    if (!w->f) { free_writer(w); return SWM_EIO; }
     /* header (mapper_mpe.c:315-341) */
-    char *o = need(w, 256);
-    if (!o) { swm_sam_close(w); return SWM_ENOMEM; }
+    buf_t h = {0, 0, 0};
     static const char hd[] = "@HD\tVN:1.4\tSO:unsorted\n";
-    out(w, hd, sizeof(hd) - 1);
-    for (uint32_t i = 0; i < w->n_contigs; i++) {
+    static const char pg[] = "@PG\tID:XXX\tVN:1.0\tPN:XXX\tCL:\"CMDLINE\"\n";
+    char *o = buf_room(&h, sizeof(hd));
+    if (o) { memcpy(o, hd, sizeof(hd) - 1); h.n += sizeof(hd) - 1; }
+    for (uint32_t i = 0; i < w->n_contigs && o; i++) {
         const size_t nl = strlen(w->name[i]);
-        o = need(w, nl + 64);
-        if (!o) { swm_sam_close(w); return SWM_ENOMEM; }
+        o = buf_room(&h, nl + 64);
+        if (!o) break;
         char *p = PUTS(o, "@SQ\tSN:");
         p = put(p, w->name[i], nl);
         p += sprintf(p, "\tLN:%u\n", (unsigned)(w->start[i + 1] - w->start[i]));
-        out(w, o, (size_t)(p - o));
+        h.n += (size_t)(p - o);
     }
-    static const char pg[] = "@PG\tID:XXX\tVN:1.0\tPN:XXX\tCL:\"CMDLINE\"\n";
-    out(w, pg, sizeof(pg) - 1);
+    if (o && (o = buf_room(&h, sizeof(pg)))) { memcpy(o, pg, sizeof(pg) - 1); h.n += sizeof(pg) - 1; }
+    if (!o) { free(h.p); swm_sam_close(w); return SWM_ENOMEM; }
+    emit_text(w, h.p, h.n);
+    free(h.p);
     if (w->err) { rc = w->err; swm_sam_close(w); return rc; }
     *outp = w;
+    return SWM_OK;
+}
+
+int swm_sam_set_threads(swm_sam_t *w, int n)
+{
+    if (!w || n < 1 || n > 256) return SWM_EINVAL;
+    w->threads = n;
     return SWM_OK;
 }
 
@@ -203,51 +248,70 @@ static uint32_t contig_of(const swm_sam_t *w, uint32_t loc)
     return lo;
 }
 
-static uint32_t name_bytes(const char *const *names, const uint32_t *name_lens, uint64_t i)
+/* ---------------------------------------------------------------- batches */
+
+typedef struct {
+    const swm_sam_t *w;
+    uint64_t n;
+    const char *const *names;
+    const uint32_t *name_lens;
+    const char *seqs, *quals;
+    uint64_t stride;
+    const uint32_t *lens;
+    const swm_records_t *recs;
+    const uint8_t *mapped;      /* per read: has a record */
+} batch_t;
+
+#define ITEM_LINES 8192
+
+/* A work item: unmapped lines of reads [a, b) (kind 0) or the lines of
+ * records [a, b) (kind 1), formatted and (with gzip) compressed alone. */
+typedef struct {
+    int kind;
+    uint64_t a, b;
+    buf_t text, gz;
+    int rc;
+} item_t;
+
+static uint32_t name_bytes(const batch_t *B, uint64_t i)
 {
-    return name_lens ? name_lens[i] : (uint32_t)strlen(names[i]);
+    return B->name_lens ? B->name_lens[i] : (uint32_t)strlen(B->names[i]);
 }
 
-int swm_sam_write_batch(swm_sam_t *w, uint64_t n, const char *const *names, const uint32_t *name_lens,
-                        const char *seqs, const char *quals, uint64_t stride, const uint32_t *lens,
-                        const swm_records_t *recs)
+static int fmt_unmapped(const batch_t *B, uint64_t a, uint64_t b, buf_t *t)
 {
-    if (!w || (n && (!names || !seqs || !lens)) || !recs || (recs->n && (!recs->rec || !recs->cigar)))
-        return SWM_EINVAL;
-    if (w->err) return w->err;
-    for (uint64_t r = 0; r < recs->n; r++) {
-        const swm_record_t *x = &recs->rec[r];
-        if (x->read >= n || (uint64_t)x->cigar_off + x->cigar_len > recs->text_bytes) return SWM_EINVAL;
-        if (r && (x->read < recs->rec[r - 1].read)) return SWM_EINVAL;   /* records come in read order */
-    }
-    /* unmapped reads: no record (put_cigar's unmap case, cigar_size == 0) */
-    uint64_t r = 0;
-    for (uint64_t i = 0; i < n; i++) {
-        if (r < recs->n && recs->rec[r].read == i) {
-            while (r < recs->n && recs->rec[r].read == i) r++;
-            continue;
-        }
-        const uint32_t nl = name_bytes(names, name_lens, i), L = lens[i];
-        char *o = need(w, (size_t)nl + 2 * (size_t)L + 64);
+    for (uint64_t i = a; i < b; i++) {
+        if (B->mapped[i]) continue;
+        const uint32_t nl = name_bytes(B, i), L = B->lens[i];
+        char *o = buf_room(t, (size_t)nl + 2 * (size_t)L + 64);
         if (!o) return SWM_ENOMEM;
-        char *p = put(o, names[i], nl);
+        char *p = put(o, B->names[i], nl);
         p = PUTS(p, "\t4\t*\t0\t0\t*\t*\t0\t0\t");
-        p = put(p, seqs + i * stride, L);
+        p = put(p, B->seqs + i * B->stride, L);
         *p++ = '\t';
-        if (quals) p = put(p, quals + i * stride, L);
+        if (B->quals) p = put(p, B->quals + i * B->stride, L);
         else *p++ = '*';
         *p++ = '\n';
-        out(w, o, (size_t)(p - o));
+        t->n += (size_t)(p - o);
     }
-    /* records */
-    for (r = 0; r < recs->n; r++) {
+    return SWM_OK;
+}
+
+static int fmt_records(const batch_t *B, uint64_t a, uint64_t b, buf_t *t)
+{
+    const swm_sam_t *w = B->w;
+    const swm_records_t *recs = B->recs;
+    char *rseq = NULL, *rqual = NULL;
+    size_t rcap = 0;
+    int rc = SWM_OK;
+    for (uint64_t r = a; r < b && rc == SWM_OK; r++) {
         const swm_record_t *x = &recs->rec[r];
         const uint64_t i = x->read;
-        const uint32_t nl = name_bytes(names, name_lens, i), L = lens[i];
+        const uint32_t nl = name_bytes(B, i), L = B->lens[i];
         const uint32_t c = contig_of(w, x->pos);
-        char *o = need(w, (size_t)nl + w->name_len[c] + 2 * (size_t)L + x->cigar_len + 128);
-        if (!o) return SWM_ENOMEM;
-        char *p = put(o, names[i], nl);
+        char *o = buf_room(t, (size_t)nl + w->name_len[c] + 2 * (size_t)L + x->cigar_len + 128);
+        if (!o) { rc = SWM_ENOMEM; break; }
+        char *p = put(o, B->names[i], nl);
         *p++ = '\t';
         if (x->strand == 0) p = x->primary ? PUTS(p, "0") : PUTS(p, "256");
         else p = x->primary ? PUTS(p, "16") : PUTS(p, "272");
@@ -259,27 +323,27 @@ int swm_sam_write_batch(swm_sam_t *w, uint64_t n, const char *const *names, cons
         p = put(p, recs->cigar + x->cigar_off, x->cigar_len);
         p = PUTS(p, "\t*\t0\t0\t");
         if (x->primary) {
-            const char *sq = seqs + i * stride, *ql = quals ? quals + i * stride : NULL;
+            const char *sq = B->seqs + i * B->stride, *ql = B->quals ? B->quals + i * B->stride : NULL;
             if (x->strand) {
-                if (L > w->rcap) {
-                    char *a = (char *)realloc(w->rseq, L), *b = a ? (char *)realloc(w->rqual, L) : NULL;
-                    if (a) w->rseq = a;
-                    if (b) w->rqual = b;
-                    if (!a || !b) return SWM_ENOMEM;
-                    w->rcap = L;
+                if (L > rcap) {
+                    char *na = (char *)realloc(rseq, L), *nb = na ? (char *)realloc(rqual, L) : NULL;
+                    if (na) rseq = na;
+                    if (nb) rqual = nb;
+                    if (!na || !nb) { rc = SWM_ENOMEM; break; }
+                    rcap = L;
                 }
                 for (uint32_t k = 0; k < L; k++) {           /* reverse_read (mapper_mpe.c:342-366) */
-                    const char b = sq[k];
-                    char v = b;
-                    if (b == 'A') v = 'T';
-                    else if (b == 'C') v = 'G';
-                    else if (b == 'G') v = 'C';
-                    else if (b == 'T') v = 'A';
-                    w->rseq[L - 1 - k] = v;
-                    if (ql) w->rqual[L - 1 - k] = ql[k];
+                    const char bc = sq[k];
+                    char v = bc;
+                    if (bc == 'A') v = 'T';
+                    else if (bc == 'C') v = 'G';
+                    else if (bc == 'G') v = 'C';
+                    else if (bc == 'T') v = 'A';
+                    rseq[L - 1 - k] = v;
+                    if (ql) rqual[L - 1 - k] = ql[k];
                 }
-                sq = w->rseq;
-                if (ql) ql = w->rqual;
+                sq = rseq;
+                if (ql) ql = rqual;
             }
             p = put(p, sq, L);
             *p++ = '\t';
@@ -295,8 +359,111 @@ int swm_sam_write_batch(swm_sam_t *w, uint64_t n, const char *const *names, cons
         p = PUTS(p, "\tXM:i:");
         *p++ = d;
         *p++ = '\n';
-        out(w, o, (size_t)(p - o));
+        t->n += (size_t)(p - o);
     }
+    free(rseq);
+    free(rqual);
+    return rc;
+}
+
+static void run_item(const batch_t *B, item_t *it)
+{
+    it->rc = it->kind == 0 ? fmt_unmapped(B, it->a, it->b, &it->text) : fmt_records(B, it->a, it->b, &it->text);
+    if (it->rc == SWM_OK && B->w->level > 0 && it->text.n) {
+        it->rc = gzip_member(it->text.p, it->text.n, B->w->level, &it->gz);
+        free(it->text.p);
+        it->text.p = NULL;
+        it->text.n = it->text.cap = 0;
+    }
+}
+
+typedef struct {
+    const batch_t *B;
+    item_t *items;
+    uint64_t n_items;
+    uint64_t next;
+    pthread_mutex_t mu;
+} pool_t;
+
+static void *pool_worker(void *arg)
+{
+    pool_t *P = (pool_t *)arg;
+    for (;;) {
+        pthread_mutex_lock(&P->mu);
+        const uint64_t k = P->next < P->n_items ? P->next++ : P->n_items;
+        pthread_mutex_unlock(&P->mu);
+        if (k >= P->n_items) break;
+        run_item(P->B, &P->items[k]);
+    }
+    return NULL;
+}
+
+int swm_sam_write_batch(swm_sam_t *w, uint64_t n, const char *const *names, const uint32_t *name_lens,
+                        const char *seqs, const char *quals, uint64_t stride, const uint32_t *lens,
+                        const swm_records_t *recs)
+{
+    if (!w || (n && (!names || !seqs || !lens)) || !recs || (recs->n && (!recs->rec || !recs->cigar)))
+        return SWM_EINVAL;
+    if (w->err) return w->err;
+    for (uint64_t r = 0; r < recs->n; r++) {
+        const swm_record_t *x = &recs->rec[r];
+        if (x->read >= n || (uint64_t)x->cigar_off + x->cigar_len > recs->text_bytes) return SWM_EINVAL;
+        if (r && (x->read < recs->rec[r - 1].read)) return SWM_EINVAL;   /* records come in read order */
+    }
+    uint8_t *mapped = (uint8_t *)calloc(n ? n : 1, 1);
+    if (!mapped) return SWM_ENOMEM;
+    for (uint64_t r = 0; r < recs->n; r++) mapped[recs->rec[r].read] = 1;   /* put_cigar's unmap: cigar_size == 0 */
+    batch_t B = {w, n, names, name_lens, seqs, quals, stride, lens, recs, mapped};
+    /* unmapped reads first (batch order), then the records */
+    const uint64_t nu = (n + ITEM_LINES - 1) / ITEM_LINES, nr = (recs->n + ITEM_LINES - 1) / ITEM_LINES;
+    const uint64_t n_items = nu + nr;
+    item_t *items = (item_t *)calloc(n_items ? n_items : 1, sizeof(item_t));
+    if (!items) { free(mapped); return SWM_ENOMEM; }
+    for (uint64_t k = 0; k < nu; k++) {
+        items[k].kind = 0;
+        items[k].a = k * ITEM_LINES;
+        items[k].b = (k + 1) * ITEM_LINES < n ? (k + 1) * ITEM_LINES : n;
+    }
+    for (uint64_t k = 0; k < nr; k++) {
+        items[nu + k].kind = 1;
+        items[nu + k].a = k * ITEM_LINES;
+        items[nu + k].b = (k + 1) * ITEM_LINES < recs->n ? (k + 1) * ITEM_LINES : recs->n;
+    }
+    /* items in groups, so that at most a group's output is held at once;
+     * each group is formatted (and compressed) in parallel, then written in order */
+    const uint64_t group = (uint64_t)w->threads * 4;
+    int rc = SWM_OK;
+    for (uint64_t g0 = 0; g0 < n_items && rc == SWM_OK; g0 += group) {
+        const uint64_t g1 = g0 + group < n_items ? g0 + group : n_items;
+        pool_t P;
+        P.B = &B;
+        P.items = items + g0;
+        P.n_items = g1 - g0;
+        P.next = 0;
+        pthread_mutex_init(&P.mu, NULL);
+        int nt = w->threads;
+        if ((uint64_t)nt > P.n_items) nt = (int)P.n_items;
+        pthread_t th[256];
+        int started = 0;
+        for (int t = 1; t < nt; t++)
+            if (pthread_create(&th[started], NULL, pool_worker, &P) == 0) started++;
+        pool_worker(&P);
+        for (int t = 0; t < started; t++) pthread_join(th[t], NULL);
+        pthread_mutex_destroy(&P.mu);
+        for (uint64_t k = g0; k < g1; k++) {
+            item_t *it = &items[k];
+            if (it->rc && rc == SWM_OK) rc = it->rc;
+            if (rc == SWM_OK) {
+                if (w->level > 0) emit(w, it->gz.p, it->gz.n);
+                else emit(w, it->text.p, it->text.n);
+            }
+            free(it->text.p);
+            free(it->gz.p);
+        }
+    }
+    free(items);
+    free(mapped);
+    if (rc != SWM_OK && w->err == SWM_OK) w->err = rc;
     return w->err;
 }
 
@@ -304,11 +471,7 @@ int swm_sam_close(swm_sam_t *w)
 {
     if (!w) return SWM_EINVAL;
     int rc = w->err;
-    if (w->gz) {
-        if (gzclose(w->gz) != Z_OK && rc == SWM_OK) rc = SWM_EIO;
-    } else if (w->f) {
-        if (fclose(w->f) != 0 && rc == SWM_OK) rc = SWM_EIO;
-    }
+    if (w->f && fclose(w->f) != 0 && rc == SWM_OK) rc = SWM_EIO;
     free_writer(w);
     return rc;
 }

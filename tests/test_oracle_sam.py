@@ -278,3 +278,38 @@ def test_sam_writer_rejects_bad_input(tmp_path):
     with pytest.raises(swm.SwmError):
         w.write([b"a"], np.zeros((1, 8), np.uint8), None, np.ones(1, np.uint32), recs, b"")
     w.close()
+
+
+@pytest.mark.parametrize("gz", [0, 1, 6])
+def test_product_sam_writer_threads_and_chunks(tmp_path, gz):
+    """20k reads (several 8192-line work items): any thread count gives the
+    oracle's text; with gzip, the file is one valid multi-member stream."""
+    prefix = str(tmp_path / "ix")
+    _write_ref_info(prefix + ".ref_info", [(b"c1", 5_000_000), (b"c2", 3_000_000)])
+    rng = np.random.default_rng(41)
+    n = 20_000
+    L = rng.integers(30, 200, n).astype(np.uint32)
+    reads = np.frombuffer(b"ACGTN", np.uint8)[rng.integers(0, 5, (n, 256))]
+    quals = rng.integers(33, 74, (n, 256)).astype(np.uint8)
+    rows = []
+    for i in range(n):
+        for strand in (0, 1):
+            k = int(rng.integers(0, 3)) if rng.random() < 0.7 else 0
+            for j in range(k):
+                rows.append((i, strand, j == 0, int(rng.integers(0, 8_000_010)), int(rng.integers(0, 6)),
+                             b"%dM" % L[i] if rng.random() < 0.5 else b"%dM1I%dM" % (L[i] // 2, L[i] - L[i] // 2 - 1)))
+    orecs = np.zeros(len(rows), orc.REC_DTYPE)
+    for t, (i, strand, prim, loc, d, cig) in enumerate(rows):
+        orecs[t] = (i, strand, d, 0, loc, len(cig) + 2, prim, (0, 0, 0), cig)
+    names = [b"read:%d" % i for i in range(n)]
+    recs, text = _product_records(orecs)
+    exp = _expected_sam(prefix, names, [bytes(reads[i, :L[i]]) for i in range(n)],
+                        [bytes(quals[i, :L[i]]) for i in range(n)], [(0, n, orecs)])
+    for threads in (1, 3, 16):
+        path = str(tmp_path / f"t{threads}.sam")
+        w = swm.SamWriter(path, prefix, gzip_level=gz, threads=threads)
+        w.write(names, reads, quals, L, recs, text)
+        w.close()
+        raw = open(path, "rb").read()
+        got = gzip.decompress(raw) if gz else raw
+        assert got == exp, threads
