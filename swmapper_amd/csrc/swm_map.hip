@@ -115,6 +115,21 @@ __device__ __forceinline__ uint64_t shfl_up_t(uint64_t v, int d)
     return ((uint64_t)hi << 32) | lo;
 }
 
+/* The same for u32 in six DPP adds, no LDS: Hillis-Steele within each row
+ * of 16 lanes (row_shr 1, 2, 4, 8), then the row totals carried by
+ * row_bcast:15 (into rows 1, 3) and row_bcast:31 (into rows 2, 3).  Lanes
+ * whose source is out of range (or masked off) add 0. */
+__device__ __forceinline__ uint32_t wave_incl_scan_dpp(uint32_t v)
+{
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, false);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, false);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, false);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, false);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false);
+    return v;
+}
+
 template <typename T>
 __device__ __forceinline__ T wave_incl_scan(T v)
 {
@@ -220,7 +235,7 @@ __global__ void __launch_bounds__(128) k_seed(const char *__restrict__ reads, ui
     uint8_t *s_rd = reinterpret_cast<uint8_t *>(s_rd32);
     if (blockIdx.x == 0) {
         /* the run's counters and the sentinel entries the scans read */
-        if (threadIdx.x < 8) o.counters[threadIdx.x] = 0;
+        if (threadIdx.x < 10) o.counters[threadIdx.x] = 0;
         if (threadIdx.x == 8) {
             o.npos[2 * nreads] = 0;
             o.nchunk[2 * nreads] = 0;
@@ -379,7 +394,17 @@ struct PlanIO {
     uint32_t *heavy;            /* (read, strand) items whose seeds need the seed heap */
     uint32_t *heavy_count;
     uint32_t *heavy_head;       /* work-queue head of k_plan_heavy */
+    uint2 *cdesc;               /* heavy items' chunk lists, at bd_off: (bucket offset, batch prefix | seed << 16) */
+    struct HBatch *hbatch;      /* heavy items' batches, for k_plan_fetch */
+    uint32_t *hb_count;
     const uint32_t *ovf;        /* set when a fixed-capacity run would overflow: every stage stands down */
+};
+
+/* One batch of a heavy item: its chunks are cdesc[bd_off[r] + cstart ..
+ * + nch), its positions go to surv[surv_off[r] + posoff ..). */
+struct HBatch {
+    uint32_t r, batch, cstart, npack;   /* npack = nch | pos_size << 8 */
+    uint32_t posoff, pad0, pad1, pad2;
 };
 
 /* 15-mer filter of one bucket entry (long_seeds_filter :2106-2117) and the
@@ -583,10 +608,12 @@ __global__ void __launch_bounds__(64) k_plan_plain(uint32_t nrs, const uint2 *__
  * single-wave workgroups.  alignment_rmdup's batch composition -- which
  * 32-position chunks, in what heap order, where each batch ends -- depends
  * only on the selected seeds' counts and on the raw last position of every
- * fetched chunk (the heap key, Q6), never on the filter.  So per batch one
- * lane replays the seed heap (:1919-1962, 2261-2356) from keys prefetched
- * into LDS, and then all 64 lanes fetch, filter and compact the batch's
- * positions together. */
+ * fetched chunk (the heap key, Q6), never on the filter.  So k_plan_heavy
+ * only replays the seed heap (:1919-1962, 2261-2356) from keys prefetched
+ * into LDS and writes each batch's chunk list; the batches' positions are
+ * fetched, filtered and compacted by k_plan_fetch, all batches in parallel,
+ * each at its own offset (its first position's index in the item), so the
+ * serial part of a giant item is the heap replay alone. */
 #define HK 16          /* buffered chunk keys per selected 12-mer */
 
 /* Lane reads (v_readlane) and writes (a compare + select) at a wave-uniform
@@ -604,8 +631,6 @@ __device__ __forceinline__ uint32_t wl(uint32_t v, uint32_t l, uint32_t x)
 __global__ void __launch_bounds__(64) k_plan_heavy(const uint2 *__restrict__ pl, PlanIO io)
 {
     __shared__ uint32_t skey[64 * HK];           /* keys of chunks kb .. kb+HK-1 of each 12-mer */
-    __shared__ uint32_t smeta[64];
-    __shared__ uint32_t cseed[SWM_MAX_BLOCKS], cg[SWM_MAX_BLOCKS], cpre[SWM_MAX_BLOCKS + 1];
     __shared__ uint32_t sv[1];
     const uint32_t lane = threadIdx.x;
     const uint64_t below = (1ull << lane) - 1ull;
@@ -624,7 +649,6 @@ __global__ void __launch_bounds__(64) k_plan_heavy(const uint2 *__restrict__ pl,
         swm_sel_t my = {0, 0, 0};
         if (act) my = io.sel[(uint64_t)r * io.seln + lane];
         const uint32_t cnt_v = my.cnt, off_v = my.off;
-        smeta[lane] = my.meta;
         uint32_t spc_v = min(cnt_v, (uint32_t)SWM_CHUNK);     /* positions fetched */
         uint32_t kb_v = 0;                                     /* first chunk in the key buffer */
         /* heap key after fetching chunk c: the raw last position of that
@@ -641,19 +665,13 @@ __global__ void __launch_bounds__(64) k_plan_heavy(const uint2 *__restrict__ pl,
             }
         };
         refill();
-        uint32_t *surv = io.surv + io.surv_off[r];
-        uint2 *bd = io.bdesc + io.bd_off[r];
+        uint2 *cd = io.cdesc + io.bd_off[r];
 
         /* phase 1 (:2194-2236): the first chunk of every 12-mer, in order */
         const uint32_t get1 = act ? spc_v : 0u;
         const uint64_t has = __ballot(get1 > 0);
         const uint32_t incl = wave_incl_scan<uint32_t>(get1);
-        if (get1 > 0) {
-            const uint32_t slot = (uint32_t)__popcll(has & below);
-            cseed[slot] = lane;
-            cg[slot] = off_v;
-            cpre[slot] = incl - get1;
-        }
+        if (get1 > 0) cd[(uint32_t)__popcll(has & below)] = make_uint2(off_v, (incl - get1) | (lane << 16));
         uint32_t nch = (uint32_t)__popcll(has), blocks = nch;
         uint32_t pos_size = rl(incl, 63);
         /* the heap (:2244-2252): 12-mers with more hits, inserted in order
@@ -677,17 +695,17 @@ __global__ void __launch_bounds__(64) k_plan_heavy(const uint2 *__restrict__ pl,
                 x = t;
             }
         }
-        __syncthreads();   /* key buffers and the phase-1 chunk list are read below */
-        uint32_t nsurv = 0, batch = 0;
+        __syncthreads();   /* key buffers are read below */
+        uint32_t batch = 0, cstart = 0, posoff = 0;
         for (;;) {
-            /* ---- phase A: the rest of this batch's chunk list (:2261-2356), wave-uniform ---- */
+            /* the rest of this batch's chunk list (:2261-2356), wave-uniform */
             for (;;) {
                 if (hsize > 0 && pos_size + SWM_CHUNK <= SWM_BATCH_POS && blocks < SWM_MAX_BLOCKS) {
                     const uint32_t i = rl(hind, 0);
                     const uint32_t cnt = rl(cnt_v, i), pc = rl(spc_v, i), off = rl(off_v, i);
                     const uint32_t c = pc / SWM_CHUNK;            /* chunk fetched now */
                     const uint32_t get = min((uint32_t)SWM_CHUNK, cnt - pc);
-                    if (lane == 0) { cseed[nch] = i; cg[nch] = off + pc; cpre[nch] = pos_size; }
+                    if (lane == 0) cd[cstart + nch] = make_uint2(off + pc, pos_size | (i << 16));
                     nch++;
                     pos_size += get;
                     blocks++;
@@ -723,70 +741,106 @@ __global__ void __launch_bounds__(64) k_plan_heavy(const uint2 *__restrict__ pl,
                 }
                 if (pos_size + SWM_CHUNK > SWM_BATCH_POS || hsize == 0 || blocks >= SWM_MAX_BLOCKS) break;
             }
-            if (lane == 0) cpre[nch] = pos_size;
-            const uint32_t bn = nch, bpos = pos_size;
-            __syncthreads();
-
-            /* ---- phase B (all lanes): fetch, filter, compact ---- */
-            const uint32_t bstart = nsurv;
-            /* 256 entries per pass, 4 per lane, so 4 loads are in flight per
-             * lane; kept entries are written in entry order */
-            for (uint32_t e0 = 0; e0 < bpos; e0 += 256) {
-                uint2 v[4];
-                uint32_t meta[4];
-                bool live[4];
-#pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    const uint32_t ent = e0 + 64u * (uint32_t)q + lane;
-                    live[q] = ent < bpos && ent > 0;             /* entry 0: the Q7 drop */
-                    uint32_t lo = 0, hi = bn;                    /* chunk with cpre[lo] <= ent < cpre[lo+1] */
-                    while (hi - lo > 1) {
-                        const uint32_t mid = (lo + hi) >> 1;
-                        if (cpre[mid] <= ent) lo = mid; else hi = mid;
-                    }
-                    const uint32_t at = live[q] ? cg[lo] + (ent - cpre[lo]) : cg[0];
-                    v[q] = pl[at];
-                    meta[q] = smeta[cseed[lo]];
-                }
-#pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    const bool keep = live[q] && filter_pass(v[q].y, meta[q]);
-                    const uint64_t m = __ballot(keep);
-                    if (keep) surv[nsurv + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = adjust(v[q].x, meta[q]);
-                    nsurv += (uint32_t)__popcll(m);
-                }
-            }
-            /* ---- end of batch: sorted unique list (block_heap_sort's result) ---- */
-            const uint32_t n = nsurv - bstart;
-            if (n <= 64) {
-                uint32_t v = lane < n ? surv[bstart + lane] : 0xffffffffu;
-                for (uint32_t kk = 2; kk <= 64; kk <<= 1) {
-                    for (uint32_t j = kk >> 1; j > 0; j >>= 1) {
-                        const uint32_t o = __shfl_xor(v, (int)j);
-                        const bool up = (lane & kk) == 0, lower = (lane & j) == 0;
-                        v = (lower == up) ? min(v, o) : max(v, o);
-                    }
-                }
-                const uint32_t prev = __shfl_up(v, 1);
-                const bool head = lane < n && (lane == 0 || v != prev);
-                const uint64_t m = __ballot(head);
-                if (head) surv[bstart + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = v;
-                const uint32_t u = (uint32_t)__popcll(m);
-                if (lane == 0) bd[batch] = make_uint2(bstart, u);
-                nsurv = bstart + u;
-            } else if (lane == 0) {
-                bd[batch] = make_uint2(bstart, n);
-                const uint32_t slot = atomicAdd(io.big_count, 1u);
-                io.big[slot] = make_uint2(r, batch);
+            if (lane == 0) {
+                HBatch hb;
+                hb.r = r; hb.batch = batch; hb.cstart = cstart; hb.npack = nch | (pos_size << 8);
+                hb.posoff = posoff; hb.pad0 = hb.pad1 = hb.pad2 = 0;
+                io.hbatch[atomicAdd(io.hb_count, 1u)] = hb;
             }
             batch++;
+            cstart += nch;
+            posoff += pos_size;
             if (hsize == 0) break;
             /* next batch: refill the key buffers, start an empty chunk list */
+            __syncthreads();
             refill();
             nch = 0; pos_size = 0; blocks = 0;
             __syncthreads();
         }
-        if (lane == 0) { io.nbatch[r] = batch; io.nuniq[r] = nsurv; }
+        if (lane == 0) { io.nbatch[r] = batch; io.nuniq[r] = 0; }
+        __syncthreads();
+    }
+}
+
+/* The batches of heavy items, one wave each, in parallel: fetch the batch's
+ * positions chunk by chunk, 15-mer filter and read-start adjustment, compact
+ * in entry order at the batch's own offset; then the sorted unique list of a
+ * batch of <= 64 survivors (block_heap_sort's result), larger batches to
+ * k_bigsort.  The item's survivor count is the sum over its batches. */
+__global__ void __launch_bounds__(64) k_plan_fetch(const uint2 *__restrict__ pl, PlanIO io)
+{
+    __shared__ uint32_t sg[SWM_MAX_BLOCKS], spre[SWM_MAX_BLOCKS + 1], sseed[SWM_MAX_BLOCKS];
+    __shared__ uint32_t smeta[64];
+    const uint32_t lane = threadIdx.x;
+    const uint32_t nb_all = *io.hb_count;
+    for (uint32_t h = blockIdx.x; h < nb_all; h += gridDim.x) {
+        const HBatch hb = io.hbatch[h];
+        const uint32_t r = hb.r, bn = hb.npack & 0xffu, bpos = hb.npack >> 8;
+        const uint2 *cd = io.cdesc + io.bd_off[r] + hb.cstart;
+        for (uint32_t k = lane; k < bn; k += 64) {
+            const uint2 d = cd[k];
+            sg[k] = d.x;
+            spre[k] = d.y & 0xffffu;
+            sseed[k] = d.y >> 16;
+        }
+        if (lane == 0) spre[bn] = bpos;
+        const uint32_t nsel = io.nsel[r] & ~NSEL_HEAVY;
+        smeta[lane] = lane < nsel ? io.sel[(uint64_t)r * io.seln + lane].meta : 0u;
+        __syncthreads();
+        uint32_t *surv = io.surv + io.surv_off[r];
+        const uint32_t bstart = hb.posoff;
+        uint32_t nsurv = bstart;
+        /* 256 entries per pass, 4 per lane, so 4 loads are in flight per
+         * lane; kept entries are written in entry order */
+        for (uint32_t e0 = 0; e0 < bpos; e0 += 256) {
+            uint2 v[4];
+            uint32_t meta[4];
+            bool live[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const uint32_t ent = e0 + 64u * (uint32_t)q + lane;
+                live[q] = ent < bpos && ent > 0;             /* entry 0: the Q7 drop */
+                uint32_t lo = 0, hi = bn;                    /* chunk with spre[lo] <= ent < spre[lo+1] */
+                while (hi - lo > 1) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if (spre[mid] <= ent) lo = mid; else hi = mid;
+                }
+                const uint32_t at = live[q] ? sg[lo] + (ent - spre[lo]) : sg[0];
+                v[q] = pl[at];
+                meta[q] = smeta[sseed[lo]];
+            }
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const bool keep = live[q] && filter_pass(v[q].y, meta[q]);
+                const uint64_t m = __ballot(keep);
+                if (keep) surv[nsurv + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = adjust(v[q].x, meta[q]);
+                nsurv += (uint32_t)__popcll(m);
+            }
+        }
+        /* end of batch: sorted unique list (block_heap_sort's result) */
+        const uint32_t n = nsurv - bstart;
+        uint2 *bd = io.bdesc + io.bd_off[r];
+        uint32_t u = n;
+        if (n <= 64) {
+            uint32_t v = lane < n ? surv[bstart + lane] : 0xffffffffu;
+            for (uint32_t kk = 2; kk <= 64; kk <<= 1) {
+                for (uint32_t j = kk >> 1; j > 0; j >>= 1) {
+                    const uint32_t o = __shfl_xor(v, (int)j);
+                    const bool up = (lane & kk) == 0, lower = (lane & j) == 0;
+                    v = (lower == up) ? min(v, o) : max(v, o);
+                }
+            }
+            const uint32_t prev = __shfl_up(v, 1);
+            const bool head = lane < n && (lane == 0 || v != prev);
+            const uint64_t m = __ballot(head);
+            if (head) surv[bstart + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = v;
+            u = (uint32_t)__popcll(m);
+            if (lane == 0) bd[hb.batch] = make_uint2(bstart, u);
+        } else if (lane == 0) {
+            bd[hb.batch] = make_uint2(bstart, n);
+            io.big[atomicAdd(io.big_count, 1u)] = make_uint2(r, hb.batch);
+        }
+        if (lane == 0 && u) atomicAdd(&io.nuniq[r], u);
         __syncthreads();
     }
 }
@@ -1442,6 +1496,9 @@ __device__ __forceinline__ void close_run(char c, int n, uint32_t &ms_acc, uint3
     }
 }
 
+/* RUNS: also store the merged runs (records path); a separate instance, so
+ * the plain pipeline keeps the smaller register footprint. */
+template <bool RUNS>
 __global__ void __launch_bounds__(64) k_trace(const uint32_t *__restrict__ trace, const uint32_t *__restrict__ trace_count,
                                               const uint32_t *__restrict__ lens, const uint32_t *__restrict__ words,
                                               const uint32_t *__restrict__ b2, uint64_t ref_length, uint32_t e,
@@ -1506,7 +1563,7 @@ __global__ void __launch_bounds__(64) k_trace(const uint32_t *__restrict__ trace
         char pre_char = 'N';
         int pre_size = 0;
         uint32_t ms_acc = 0, text = 2, nr = 0;
-        uint16_t *runs = c.runs ? c.runs + (uint64_t)ci * RUN_SLOT : nullptr;
+        uint16_t *runs = RUNS ? c.runs + (uint64_t)ci * RUN_SLOT : nullptr;
         for (int cc = (int)nch - 1; cc >= 0 && !done && i >= 0; cc--) {
             const uint32_t steps = ((uint32_t)cc == nfull) ? rem : 32u;
             vp = ck[2 * cc];
@@ -1561,7 +1618,7 @@ __global__ void __launch_bounds__(64) k_trace(const uint32_t *__restrict__ trace
             if (runs && nr < RUN_SLOT - 1) runs[1 + nr] = (uint16_t)(ms_acc | (RUN_M << 8));
             nr++;
         }
-        if (runs) runs[0] = (uint16_t)nr;
+        if (RUNS) runs[0] = (uint16_t)nr;
         const uint64_t new_loc = (uint64_t)st + (uint64_t)(int64_t)start_location + 1;
         c.rec[ci] = (uint32_t)new_loc;
         c.info[ci] = (text & 0xffffu) | INFO_TRACE | ((new_loc >> 32) ? INFO_NEG : 0u);
@@ -1690,18 +1747,6 @@ __global__ void __launch_bounds__(128) k_emit(uint32_t nreads, uint32_t e, const
  * written, every other dup is skipped, every other passing lane is written.
  * The best hit is the first minimum of (distance, strand, order) over the
  * written records. */
-__device__ __forceinline__ uint64_t wave_min_u64(uint64_t v)
-{
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)v, d, 64);
-        const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(v >> 32), d, 64);
-        const uint64_t u = ((uint64_t)hi << 32) | lo;
-        v = u < v ? u : v;
-    }
-    return v;
-}
-
 __device__ __forceinline__ uint32_t rdlane(uint32_t v, uint32_t l)
 {
     return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)l);
@@ -1749,17 +1794,23 @@ __global__ void __launch_bounds__(64) k_emit_heavy(const uint32_t *__restrict__ 
                 if (num + 8 > 256 || len + 16 * 16 > 256 * 16) { num = 0; len = 0; }
                 g_open = false;
             };
-            /* the four fields of a window are loaded together, and the next
-             * window's are in flight while this one is processed */
-            int32_t f_er = -2;
-            uint32_t f_bi = 0, f_rec = 0, f_info = 0;
-            auto fetch = [&](uint32_t at) {
+            /* the four fields of a window are loaded together, four windows
+             * ahead: a giant read is one wave's serial walk, so the loads
+             * must not wait on the walk.  Four named slots, rotated by
+             * position (no loaded register is ever moved). */
+            auto fetch = [&](uint32_t at, int32_t &fe, uint32_t &fb, uint32_t &fr, uint32_t &fi) {
                 const uint32_t ci = at + lane;
-                if (ci < cend) { f_er = c.err[ci]; f_bi = c.bi[ci]; f_rec = c.rec[ci]; f_info = c.info[ci]; }
-                else { f_er = -2; f_bi = 0xffffffffu << 12; f_rec = 0; f_info = 0; }
+                const bool ok = ci < cend;
+                const uint32_t cc = ok ? ci : c0;             /* in-range address; value discarded */
+                const int32_t le = c.err[cc];
+                const uint32_t lb = c.bi[cc], lr = c.rec[cc], li = c.info[cc];
+                fe = ok ? le : -2;
+                fb = ok ? lb : 0xffffffffu << 12;
+                fr = ok ? lr : 0u;
+                fi = ok ? li : 0u;
             };
-            if (c0 < cend) fetch(c0);
-            for (uint32_t base = c0; base < cend; base += 64) {
+/* one window of 64 candidates, its fields already loaded */
+            auto window = [&](uint32_t base, int32_t f_er, uint32_t f_bi, uint32_t f_rec, uint32_t f_info) {
                 const uint32_t ci = base + lane;
                 const uint32_t m = min(64u, cend - base);
                 const bool in = ci < cend;
@@ -1769,7 +1820,6 @@ __global__ void __launch_bounds__(64) k_emit_heavy(const uint32_t *__restrict__ 
                 const bool pass = valid && er <= (int32_t)e;
                 const uint32_t rec = pass ? f_rec : 0u;
                 const uint32_t info = pass ? f_info : 0u;
-                if (base + 64 < cend) fetch(base + 64);
                 /* duplicate candidates: predecessor = last passing lane below, else the carry */
                 const uint64_t pm = __ballot(pass);
                 const uint64_t pb = pm & below;
@@ -1778,9 +1828,9 @@ __global__ void __launch_bounds__(64) k_emit_heavy(const uint32_t *__restrict__ 
                 const bool hp = pb ? true : has_prev;
                 const uint32_t prv = pb ? prv_l : prevrec;
                 const bool dup = pass && (info & INFO_TRACE) && !(info & INFO_NEG) && hp && rec == prv;
-                /* index of the lane among the valid lanes of its batch */
-                uint32_t bprev = (uint32_t)__shfl_up((int)b, 1, 64);
-                if (lane == 0) bprev = cur_b;
+                /* index of the lane among the valid lanes of its batch; the
+                 * previous lane's batch by wave_shr:1 (lane 0 takes the carry) */
+                const uint32_t bprev = (uint32_t)__builtin_amdgcn_update_dpp((int)cur_b, (int)b, 0x138, 0xf, 0xf, false);
                 const uint64_t sm = __ballot(in && b != bprev);
                 const uint64_t vm = __ballot(valid);
                 const uint64_t smb = sm & (below | (1ull << lane));
@@ -1799,13 +1849,17 @@ __global__ void __launch_bounds__(64) k_emit_heavy(const uint32_t *__restrict__ 
                     c.tb[ci] = wr ? (info & 0xffffu) - 2u : 0u;
                 }
                 const uint64_t am = __ballot(wr);
-                const uint32_t tinc = wave_incl_scan<uint32_t>(wr ? (info & 0xffffu) : 0u);
-                const uint64_t key = wr ? (((uint64_t)(uint32_t)er << 40) | ((uint64_t)strand << 32) | (ci - c0)) : ~0ull;
-                const uint64_t kmin = wave_min_u64(key);
-                if (kmin < best) {
-                    const uint64_t at = __ballot(key == kmin);
-                    best = kmin;
-                    best_rec = rdlane(rec, (uint32_t)__ffsll((unsigned long long)at) - 1u);
+                const uint32_t tinc = wave_incl_scan_dpp(wr ? (info & 0xffffu) : 0u);
+                /* best hit: every earlier record (written or pending) precedes
+                 * this window, so a lane here beats it only with a smaller
+                 * distance; take the first lane of the smallest one */
+                uint32_t best_er = best == ~0ull ? 0xffffffffu : (uint32_t)(best >> 40);
+                for (uint64_t cm = __ballot(wr && (uint32_t)er < best_er); cm;
+                     cm = __ballot(wr && (uint32_t)er < best_er)) {
+                    const uint32_t f = (uint32_t)__ffsll((unsigned long long)cm) - 1u;
+                    best_er = rdlane((uint32_t)er, f);
+                    best = ((uint64_t)best_er << 40) | ((uint64_t)strand << 32) | (uint64_t)(base + f - c0);
+                    best_rec = rdlane(rec, f);
                 }
                 count += (uint32_t)__popcll(am);
                 /* lanes [lo, hi) join the open group */
@@ -1843,7 +1897,27 @@ __global__ void __launch_bounds__(64) k_emit_heavy(const uint32_t *__restrict__ 
                 cur_b = rdlane(b, last);
                 vcnt = rdlane(v, last) + (uint32_t)((vm >> last) & 1ull);
                 if (pm) { has_prev = true; prevrec = rdlane(rec, 63u - (uint32_t)__clzll(pm)); }
+            };
+            int32_t e0, e1, e2, e3;
+            uint32_t b0, b1, b2, b3, r0, r1, r2, r3, i0, i1, i2, i3;
+            fetch(c0, e0, b0, r0, i0);
+            fetch(c0 + 64u, e1, b1, r1, i1);
+            fetch(c0 + 128u, e2, b2, r2, i2);
+            fetch(c0 + 192u, e3, b3, r3, i3);
+#define EMIT_SLOT(E, B, R, I, OFF)                                                  \
+            if (base + (OFF) < cend) {                                              \
+                const int32_t fe = E;                                               \
+                const uint32_t fb = B, fr = R, fi = I;                              \
+                fetch(base + (OFF) + 256u, E, B, R, I);                             \
+                window(base + (OFF), fe, fb, fr, fi);                               \
             }
+            for (uint32_t base = c0; base < cend; base += 256u) {
+                EMIT_SLOT(e0, b0, r0, i0, 0u)
+                EMIT_SLOT(e1, b1, r1, i1, 64u)
+                EMIT_SLOT(e2, b2, r2, i2, 128u)
+                EMIT_SLOT(e3, b3, r3, i3, 192u)
+            }
+#undef EMIT_SLOT
             close_group();
         }
         if (lane == 0) {
@@ -2039,7 +2113,7 @@ struct swm_ws {
     struct Buf { void *p = nullptr; size_t cap = 0; };
     enum { WORDS, SEL, NSEL, NPOS, NCHUNK, SURV_OFF, BD_OFF, SURV, BDESC, NBATCH, NUNIQ, CAND_OFF, C_VAL, C_RS, C_BI,
            C_ERR, C_LOC, C_REC, C_INFO, COUNTERS, BIG, TRACE, HEAVY, HEAVY_C, HEAVY_E, SCAN_TMP, IN_READS, IN_LENS,
-           OUT, HIST, R_WR, R_TB, R_RUNS, R_IDX, R_TOFF, R_SCAN, R_TOT, R_OUT, R_TEXT, NBUF };
+           OUT, HIST, R_WR, R_TB, R_RUNS, R_IDX, R_TOFF, R_SCAN, R_TOT, R_OUT, R_TEXT, CDESC, HBATCH, NBUF };
     Buf b[NBUF];
     uint64_t gen = 1;
     ~swm_ws() { for (auto &x : b) if (x.p) (void)hipFree(x.p); }
@@ -2173,9 +2247,10 @@ struct Pipe {
 };
 
 /* counters: u64 [0] candidates, [1] records, [2] band steps, [3] surviving
- * positions; u32 view of [4..7]: [0] large batches, [1] tracebacks,
+ * positions; u32 view of [4..9]: [0] large batches, [1] tracebacks,
  * [2] rejected windows, [3] heavy plan items, [4] their queue head,
- * [5] heavy compaction items, [6] heavy emission items, [7] overflow */
+ * [5] heavy compaction items, [6] heavy emission items, [7] overflow,
+ * [8] heavy plan batches */
 static int pipe_alloc_a(swm_ws *ws, uint64_t n, uint32_t seln, Pipe &P)
 {
     const uint64_t nrs = 2 * n;
@@ -2185,7 +2260,7 @@ static int pipe_alloc_a(swm_ws *ws, uint64_t n, uint32_t seln, Pipe &P)
     int rc;
     if ((rc = ws->get(swm_ws::WORDS, &P.so.words, nrs * 32)) || (rc = ws->get(swm_ws::SEL, &P.so.sel, nrs * seln)) ||
         (rc = ws->get(swm_ws::NSEL, &P.so.nsel, nrs)) || (rc = ws->get(swm_ws::NPOS, &P.so.npos, nrs + 1)) ||
-        (rc = ws->get(swm_ws::NCHUNK, &P.so.nchunk, nrs + 1)) || (rc = ws->get(swm_ws::COUNTERS, &P.counters, 8)) ||
+        (rc = ws->get(swm_ws::NCHUNK, &P.so.nchunk, nrs + 1)) || (rc = ws->get(swm_ws::COUNTERS, &P.counters, 10)) ||
         (rc = ws->get(swm_ws::SURV_OFF, &P.surv_off, nrs + 1)) || (rc = ws->get(swm_ws::BD_OFF, &P.bd_off, nrs + 1)) ||
         (rc = ws->get(swm_ws::NBATCH, &P.pio.nbatch, nrs)) || (rc = ws->get(swm_ws::NUNIQ, &P.pio.nuniq, nrs + 1)) ||
         (rc = ws->get(swm_ws::HEAVY, &P.pio.heavy, nrs)) || (rc = ws->get(swm_ws::CAND_OFF, &P.cand_off, nrs + 1)) ||
@@ -2203,6 +2278,7 @@ static int pipe_alloc_a(swm_ws *ws, uint64_t n, uint32_t seln, Pipe &P)
     P.pio.big_count = P.u32c;
     P.pio.heavy_count = P.u32c + 3;
     P.pio.heavy_head = P.u32c + 4;
+    P.pio.hb_count = P.u32c + 8;
     P.pio.ovf = P.u32c + 7;
     P.cs.ovf = P.u32c + 7;
     return SWM_OK;
@@ -2213,7 +2289,9 @@ static int pipe_alloc_b(swm_ws *ws, uint64_t cap_pos, uint64_t cap_chunk, Pipe &
     int rc;
     if ((rc = ws->get(swm_ws::SURV, &P.pio.surv, cap_pos + 1)) ||
         (rc = ws->get(swm_ws::BDESC, &P.pio.bdesc, cap_chunk + 1)) ||
-        (rc = ws->get(swm_ws::BIG, &P.pio.big, cap_chunk + 1)))
+        (rc = ws->get(swm_ws::BIG, &P.pio.big, cap_chunk + 1)) ||
+        (rc = ws->get(swm_ws::CDESC, &P.pio.cdesc, cap_chunk + 1)) ||
+        (rc = ws->get(swm_ws::HBATCH, &P.pio.hbatch, cap_chunk + 1)))
         return rc;
     return SWM_OK;
 }
@@ -2269,6 +2347,7 @@ static int enq_b(const swm_ctx_t *ctx, Pipe &P, uint64_t cap_cand, hipStream_t s
     if (o.work_queue) {
         k_plan<<<blocks_for(P.nrs, 64), 64, 0, st>>>((uint32_t)P.nrs, ctx->d_pl, P.pio);
         k_plan_heavy<<<ctx->n_cu * o.plan_heavy_waves, 64, 0, st>>>(ctx->d_pl, P.pio);
+        k_plan_fetch<<<ctx->n_cu * 16, 64, 0, st>>>(ctx->d_pl, P.pio);
     } else {
         k_plan_plain<<<blocks_for(P.nrs, 64), 64, 0, st>>>((uint32_t)P.nrs, ctx->d_pl, P.pio);
     }
@@ -2322,7 +2401,7 @@ static int enq_c(const swm_ctx_t *ctx, Pipe &P, const uint32_t *d_lens, uint32_t
                                                                           P.trace, P.u32c + 1, P.u32c + 2,
                                                                           P.counters + 2);
     /* persistent: 8 waves per SIMD, the traceback chains are latency-bound */
-    k_trace<<<ctx->n_cu * 32, 64, 0, st>>>(P.trace, P.u32c + 1, d_lens, P.so.words, ctx->d_base_2bits, ctx->ref_length, e,
+    (P.cs.runs ? k_trace<true> : k_trace<false>)<<<ctx->n_cu * 32, 64, 0, st>>>(P.trace, P.u32c + 1, d_lens, P.so.words, ctx->d_base_2bits, ctx->ref_length, e,
                                           P.cs);
     k_emit<<<blocks_for(P.n, 128), 128, 0, st>>>((uint32_t)P.n, e, P.cand_off, P.cs, d_out, P.counters + 1,
                                                  heavy_cands, P.heavy_e, P.u32c + 6);
