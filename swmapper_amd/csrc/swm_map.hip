@@ -378,6 +378,8 @@ __global__ void __launch_bounds__(128) k_seed(const char *__restrict__ reads, ui
 /* ------------------------------------------------------------------ k_plan */
 
 #define SMALL_BATCH 32
+#define GIANT_POS 16384      /* heavy plan items this large are taken first */
+#define GIANT_CANDS 16384    /* heavy emission reads this large are taken first */
 
 struct PlanIO {
     const swm_sel_t *sel;
@@ -394,6 +396,8 @@ struct PlanIO {
     uint32_t *heavy;            /* (read, strand) items whose seeds need the seed heap */
     uint32_t *heavy_count;
     uint32_t *heavy_head;       /* work-queue head of k_plan_heavy */
+    uint32_t *giant;            /* heavy items with more than GIANT_POS positions: drained first */
+    uint32_t *giant_count;
     uint2 *cdesc;               /* heavy items' chunk lists, at bd_off: (bucket offset, batch prefix | seed << 16) */
     struct HBatch *hbatch;      /* heavy items' batches, for k_plan_fetch */
     uint32_t *hb_count;
@@ -435,7 +439,11 @@ __global__ void __launch_bounds__(64) k_plan(uint32_t nrs, const uint2 *__restri
     const uint32_t nsel = nsw & ~NSEL_HEAVY;
     const swm_sel_t *sel = io.sel + (uint64_t)r * io.seln;
     const bool heavy = (nsw & NSEL_HEAVY) != 0;
-    block_append(live && heavy, r, io.heavy, io.heavy_count, s_tmp);
+    /* the largest items go to their own list, which the persistent waves
+     * drain first: the slowest serial replays start at once */
+    const bool giant = live && heavy && io.surv_off[r + 1] - io.surv_off[r] > GIANT_POS;
+    block_append(live && heavy && !giant, r, io.heavy, io.heavy_count, s_tmp);
+    block_append(giant, r, io.giant, io.giant_count, s_tmp);
     if (!live || heavy) return;
 
     uint32_t *surv = io.surv + io.surv_off[r];
@@ -634,7 +642,7 @@ __global__ void __launch_bounds__(64) k_plan_heavy(const uint2 *__restrict__ pl,
     __shared__ uint32_t sv[1];
     const uint32_t lane = threadIdx.x;
     const uint64_t below = (1ull << lane) - 1ull;
-    const uint32_t nq = *io.heavy_count;
+    const uint32_t ng = *io.giant_count, nq = ng + *io.heavy_count;
 
     for (;;) {
         if (lane == 0) sv[0] = atomicAdd(io.heavy_head, 1u);
@@ -642,7 +650,7 @@ __global__ void __launch_bounds__(64) k_plan_heavy(const uint2 *__restrict__ pl,
         const uint32_t qi = sv[0];
         __syncthreads();
         if (qi >= nq) break;
-        const uint32_t r = io.heavy[qi];
+        const uint32_t r = qi < ng ? io.giant[qi] : io.heavy[qi - ng];
         const uint32_t nsel = io.nsel[r] & ~NSEL_HEAVY;
         const bool act = lane < nsel;
         /* per 12-mer state, lane i = selected 12-mer i */
@@ -1706,8 +1714,14 @@ __global__ void __launch_bounds__(128) k_emit(uint32_t nreads, uint32_t e, const
     uint32_t rd = blockIdx.x * blockDim.x + threadIdx.x;
     const bool live = rd < nreads;
     if (!live) rd = 0;
-    const bool big = live && cand_off[2 * rd + 2] - cand_off[2 * rd] > heavy_cands;
-    block_append(big, rd, heavy, heavy_count, s_tmp);
+    const uint32_t nc = cand_off[2 * rd + 2] - cand_off[2 * rd];
+    const bool big = live && nc > heavy_cands;
+    /* the largest reads first (heavy[0] .. [giant count)), the rest after
+     * them from the list's other end: k_emit_heavy's waves start on the
+     * slowest serial walks at once */
+    const bool giant = big && nc > GIANT_CANDS;
+    block_append(big && !giant, rd, heavy + nreads, heavy_count + 1, s_tmp);
+    block_append(giant, rd, heavy, heavy_count, s_tmp);
     Emit em;
     em.init();
     if (live && !big) {
@@ -1752,17 +1766,19 @@ __device__ __forceinline__ uint32_t rdlane(uint32_t v, uint32_t l)
     return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)l);
 }
 
+/* heavy[0 .. heavy_count[0]): giant reads; heavy[nreads ..
+ * + heavy_count[1]): the other heavy reads */
 __global__ void __launch_bounds__(64) k_emit_heavy(const uint32_t *__restrict__ heavy,
-                                                   const uint32_t *__restrict__ heavy_count, uint32_t e,
+                                                   const uint32_t *__restrict__ heavy_count, uint32_t nreads, uint32_t e,
                                                    const uint32_t *__restrict__ cand_off, CandSet c,
                                                    swm_read_result_t *__restrict__ out,
                                                    unsigned long long *n_records)
 {
     const uint32_t lane = threadIdx.x;
     const uint64_t below = (1ull << lane) - 1ull;
-    const uint32_t nh = *heavy_count;
+    const uint32_t ng = heavy_count[0], nh = ng + heavy_count[1];
     for (uint32_t h = blockIdx.x; h < nh; h += gridDim.x) {
-        const uint32_t rd = heavy[h];
+        const uint32_t rd = h < ng ? heavy[h] : heavy[nreads + (h - ng)];
         uint32_t count = 0, best_rec = 0;
         uint64_t best = ~0ull;
         for (uint32_t strand = 0; strand < 2; strand++) {
@@ -2113,7 +2129,7 @@ struct swm_ws {
     struct Buf { void *p = nullptr; size_t cap = 0; };
     enum { WORDS, SEL, NSEL, NPOS, NCHUNK, SURV_OFF, BD_OFF, SURV, BDESC, NBATCH, NUNIQ, CAND_OFF, C_VAL, C_RS, C_BI,
            C_ERR, C_LOC, C_REC, C_INFO, COUNTERS, BIG, TRACE, HEAVY, HEAVY_C, HEAVY_E, SCAN_TMP, IN_READS, IN_LENS,
-           OUT, HIST, R_WR, R_TB, R_RUNS, R_IDX, R_TOFF, R_SCAN, R_TOT, R_OUT, R_TEXT, CDESC, HBATCH, NBUF };
+           OUT, HIST, R_WR, R_TB, R_RUNS, R_IDX, R_TOFF, R_SCAN, R_TOT, R_OUT, R_TEXT, CDESC, HBATCH, GIANT, NBUF };
     Buf b[NBUF];
     uint64_t gen = 1;
     ~swm_ws() { for (auto &x : b) if (x.p) (void)hipFree(x.p); }
@@ -2249,8 +2265,9 @@ struct Pipe {
 /* counters: u64 [0] candidates, [1] records, [2] band steps, [3] surviving
  * positions; u32 view of [4..9]: [0] large batches, [1] tracebacks,
  * [2] rejected windows, [3] heavy plan items, [4] their queue head,
- * [5] heavy compaction items, [6] heavy emission items, [7] overflow,
- * [8] heavy plan batches */
+ * [5] heavy compaction items, [6] (unused), [7] overflow, [8] heavy plan
+ * batches, [9] giant plan items, [10] giant / [11] other heavy emission
+ * reads */
 static int pipe_alloc_a(swm_ws *ws, uint64_t n, uint32_t seln, Pipe &P)
 {
     const uint64_t nrs = 2 * n;
@@ -2263,8 +2280,8 @@ static int pipe_alloc_a(swm_ws *ws, uint64_t n, uint32_t seln, Pipe &P)
         (rc = ws->get(swm_ws::NCHUNK, &P.so.nchunk, nrs + 1)) || (rc = ws->get(swm_ws::COUNTERS, &P.counters, 10)) ||
         (rc = ws->get(swm_ws::SURV_OFF, &P.surv_off, nrs + 1)) || (rc = ws->get(swm_ws::BD_OFF, &P.bd_off, nrs + 1)) ||
         (rc = ws->get(swm_ws::NBATCH, &P.pio.nbatch, nrs)) || (rc = ws->get(swm_ws::NUNIQ, &P.pio.nuniq, nrs + 1)) ||
-        (rc = ws->get(swm_ws::HEAVY, &P.pio.heavy, nrs)) || (rc = ws->get(swm_ws::CAND_OFF, &P.cand_off, nrs + 1)) ||
-        (rc = ws->get(swm_ws::HEAVY_C, &P.heavy_c, nrs)) || (rc = ws->get(swm_ws::HEAVY_E, &P.heavy_e, n)) ||
+        (rc = ws->get(swm_ws::HEAVY, &P.pio.heavy, nrs)) || (rc = ws->get(swm_ws::GIANT, &P.pio.giant, nrs)) || (rc = ws->get(swm_ws::CAND_OFF, &P.cand_off, nrs + 1)) ||
+        (rc = ws->get(swm_ws::HEAVY_C, &P.heavy_c, nrs)) || (rc = ws->get(swm_ws::HEAVY_E, &P.heavy_e, 2 * n)) ||
         (rc = ws->get(swm_ws::SCAN_TMP, &P.scan_tmp, 2 * (uint64_t)blocks_for(nrs + 1, SCAN_TILE))))
         return rc;
     P.so.counters = P.counters;
@@ -2279,6 +2296,7 @@ static int pipe_alloc_a(swm_ws *ws, uint64_t n, uint32_t seln, Pipe &P)
     P.pio.heavy_count = P.u32c + 3;
     P.pio.heavy_head = P.u32c + 4;
     P.pio.hb_count = P.u32c + 8;
+    P.pio.giant_count = P.u32c + 9;
     P.pio.ovf = P.u32c + 7;
     P.cs.ovf = P.u32c + 7;
     return SWM_OK;
@@ -2404,10 +2422,10 @@ static int enq_c(const swm_ctx_t *ctx, Pipe &P, const uint32_t *d_lens, uint32_t
     (P.cs.runs ? k_trace<true> : k_trace<false>)<<<ctx->n_cu * 32, 64, 0, st>>>(P.trace, P.u32c + 1, d_lens, P.so.words, ctx->d_base_2bits, ctx->ref_length, e,
                                           P.cs);
     k_emit<<<blocks_for(P.n, 128), 128, 0, st>>>((uint32_t)P.n, e, P.cand_off, P.cs, d_out, P.counters + 1,
-                                                 heavy_cands, P.heavy_e, P.u32c + 6);
+                                                 heavy_cands, P.heavy_e, P.u32c + 10);
     if (o.work_queue)
-        k_emit_heavy<<<ctx->n_cu * o.emit_heavy_waves, 64, 0, st>>>(P.heavy_e, P.u32c + 6, e, P.cand_off, P.cs, d_out,
-                                                                   P.counters + 1);
+        k_emit_heavy<<<ctx->n_cu * o.emit_heavy_waves, 64, 0, st>>>(P.heavy_e, P.u32c + 10, (uint32_t)P.n, e,
+                                                                   P.cand_off, P.cs, d_out, P.counters + 1);
     SWM_HIP_CHECK(hipGetLastError());
     if (ev) SWM_HIP_CHECK(hipEventRecord(ev[5], st));
     return SWM_OK;
