@@ -673,6 +673,10 @@ __global__ void __launch_bounds__(64) k_plan_heavy(const uint2 *__restrict__ pl,
             }
         };
         refill();
+        /* key of each 12-mer's next chunk, one per lane: a pop reads it with
+         * one readlane, and the key after it is loaded during the sift-down */
+        __syncthreads();
+        uint32_t nk_v = act ? skey[lane * HK] : 0u;
         uint2 *cd = io.cdesc + io.bd_off[r];
 
         /* phase 1 (:2194-2236): the first chunk of every 12-mer, in order */
@@ -722,16 +726,17 @@ __global__ void __launch_bounds__(64) k_plan_heavy(const uint2 *__restrict__ pl,
                      * (the last entry, or this 12-mer with its next key) sifts
                      * down; its value stays in scalars and only the hole moves,
                      * which is the same sequence of swaps */
-                    uint32_t mloc, mind;
+                    uint32_t mloc, mind, nk = 0;
                     if (pc + get == cnt) {
                         mloc = rl(hloc, hsize - 1);
                         mind = rl(hind, hsize - 1);
                         hsize--;
                     } else {
-                        const uint32_t kb = rl(kb_v, i);
-                        if (c >= kb && c < kb + HK) mloc = skey[i * HK + (c - kb)];
-                        else mloc = pl[off + min(SWM_CHUNK * (c + 1), cnt) - 1].x;
+                        mloc = rl(nk_v, i);
                         mind = i;
+                        const uint32_t kb = rl(kb_v, i), c1 = c + 1;
+                        if (c1 < kb + HK) nk = skey[i * HK + (c1 - kb)];
+                        else if (SWM_CHUNK * c1 < cnt) nk = pl[off + min(SWM_CHUNK * (c1 + 1), cnt) - 1].x;
                     }
                     uint32_t x = 1;
                     while (x * 2 < hsize) {
@@ -746,6 +751,7 @@ __global__ void __launch_bounds__(64) k_plan_heavy(const uint2 *__restrict__ pl,
                     }
                     hloc = wl(hloc, x - 1, mloc);
                     hind = wl(hind, x - 1, mind);
+                    nk_v = wl(nk_v, i, nk);
                 }
                 if (pos_size + SWM_CHUNK > SWM_BATCH_POS || hsize == 0 || blocks >= SWM_MAX_BLOCKS) break;
             }
@@ -764,6 +770,7 @@ __global__ void __launch_bounds__(64) k_plan_heavy(const uint2 *__restrict__ pl,
             refill();
             nch = 0; pos_size = 0; blocks = 0;
             __syncthreads();
+            nk_v = act ? skey[lane * HK] : 0u;
         }
         if (lane == 0) { io.nbatch[r] = batch; io.nuniq[r] = 0; }
         __syncthreads();
