@@ -41,6 +41,15 @@ def pmc_traffic(config, kernel):
             return {"bytes": int((float(f[3]) + float(f[4])) * 1e6), "source": "profiles/" + name}
     return None
 
+
+def traffic_rate(traffic, ms):
+    """The memory system's side of a gather-bound kernel: the HBM bytes it
+    moves per launch (whole lines) over its launch time, against the peak."""
+    if not traffic or not ms:
+        return None
+    gbs = traffic["bytes"] / (ms / 1e3) / 1e9
+    return {"gbs": round(gbs, 1), "frac_of_peak": round(gbs / HBM_PEAK_GBS, 4)}
+
 CONFIGS = {
     "repeat5": dict(n=256_000_000, kind=1, frac=0.05, reads=1_000_000, lmin=150, lmax=150, err=(0.025, 0.025), e=5,
                     desc="256 Mbp uniform + 5% repeat families (300 bp x 500 copies, 10% div), 12-mer index; "
@@ -381,7 +390,8 @@ def main():
             ach = seed_bytes / (ms_seed / 1e3) / 1e9
             roof = {"bound": "hbm", "kernel": "k_seed", "form": timed, "achieved": round(ach, 1),
                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
-                    "traffic": traffic, "algorithmic_bytes_per_launch": seed_bytes,
+                    "traffic": traffic, "traffic_rate": traffic_rate(traffic, ms_seed),
+                    "algorithmic_bytes_per_launch": seed_bytes,
                     "avg_launch_ms": round(ms_seed, 3),
                     "note": "random 8-byte gathers into the 64 MiB offset table: each costs a whole cache line, "
                             "so HBM traffic (rocprofv3 FETCH_SIZE/WRITE_SIZE, calibrated on a copy) is many "
@@ -396,6 +406,7 @@ def main():
                          "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": pmc_traffic(args.config, "k_plan"),
                          "algorithmic_bytes_per_launch": 6 * npos, "avg_launch_ms": round(ms_plan, 3)}
+            roof_plan["traffic_rate"] = traffic_rate(roof_plan["traffic"], ms_plan)
         sum_l_ver = st_all[-1]["n_band_steps"]
         ms_myers = statistics.mean(s["ms_myers"] for s in st_all)
         if sum_l_ver and ms_myers > 0:

@@ -159,3 +159,68 @@ def test_cli_writes_reference_sam(tmp_path):
                             [bytes(r[i, :l[i]]) for i in range(lo, hi)],
                             [bytes(quals[i, :l[i]]) for i in range(lo, hi)], orecs)
     assert got == exp
+
+
+@pytest.mark.parametrize("e", [0, 15])
+def test_records_extreme_errors(e):
+    """e = 0 (Hamming only) and e = 15 (the widest band, up to 31 runs per
+    CIGAR) on mixed lengths."""
+    g = wl.genome(200_000, seed=95, kind=swm.GENOME_REPEAT_FAMILIES, frac=0.2)
+    r, l, _, _ = wl.reads(g, 1200, len_min=40, len_max=254, err=(0.0, 0.08), seed=96)
+    _say("records e", e)
+    ref = orc.OracleRef.from_index(orc.OracleIndex(g, 12))
+    hits, orecs, _ = ref.trace(r, l, e)
+    m = swm.Mapper.from_index(swm.build_index(g, 12))
+    res, recs, text = m.map_records(r, l, e)
+    _assert_results(res, hits, f"e={e}")
+    _assert_records(recs, text, orecs, f"e={e}")
+    m.close()
+
+
+def test_records_genome_ends_and_n():
+    """Reads at both ends of the reference (windows past the end read 0, the
+    traceback's start may fall before base 0) and reads with N."""
+    g = wl.genome(64_000, seed=51)
+    starts = [0, 1, 2, 3, 4, 5, 7, 31, 32, g.size - 150, g.size - 151, g.size - 160, g.size - 200]
+    r, l = wl.exact_reads(g, starts, 150, [0, 1] * 7)
+    r2, l2, _, _ = wl.reads(g, 500, seed=52)
+    r2[::7, 30] = ord("N")
+    r2[::11, 100:104] = ord("N")
+    reads = np.vstack([r, r2])
+    lens = np.concatenate([l, l2])
+    _say("records ends/N")
+    ref = orc.OracleRef.from_index(orc.OracleIndex(g, 12))
+    hits, orecs, _ = ref.trace(reads, lens, 5)
+    m = swm.Mapper.from_index(swm.build_index(g, 12))
+    res, recs, text = m.map_records(reads, lens, 5)
+    _assert_results(res, hits, "ends")
+    _assert_records(recs, text, orecs, "ends")
+    m.close()
+
+
+def test_cli_fasta_reads_write_star_quality(tmp_path):
+    """FASTA reads have no qualities: QUAL is '*' on unmapped and primary lines."""
+    g = wl.genome(120_000, seed=75)
+    r, l, _, _ = wl.reads(g, 700, len_min=60, len_max=150, err=(0.0, 0.05), seed=76)
+    r[::9, :] = np.frombuffer(b"ACGT", np.uint8)[np.random.default_rng(77).integers(0, 4, (r[::9].shape))]
+    fa = tmp_path / "ref.fa"
+    fa.write_bytes(b">c1\n" + bytes(g[:60_000]) + b"\n>c2 second\n" + bytes(g[60_000:]) + b"\n")
+    rd = tmp_path / "reads.fa"
+    with open(rd, "wb") as f:
+        for i in range(l.size):
+            f.write(b">f%d\n%s\n" % (i, bytes(r[i, :l[i]])))
+    here = os.path.dirname(swm.LIB_PATH)
+    prefix = str(tmp_path / "ix")
+    _say("cli fasta")
+    subprocess.run([os.path.join(here, "swm_index"), "-i", str(fa), "-p", prefix], check=True, timeout=120,
+                   stdout=subprocess.DEVNULL)
+    subprocess.run([os.path.join(here, "swmapper"), "-p", prefix, f"-i{rd}", f"-o{tmp_path / 'o'}", "-e", "4",
+                    "-l200"], check=True, timeout=120, stdout=subprocess.DEVNULL)
+    got = gzip.open(str(tmp_path / "o.sam.gz")).read()
+    ref = orc.OracleRef.from_index(orc.OracleIndex(g, 12))
+    ref_len, cn, cnl = so.load_ref_info(prefix + ".ref_info")
+    assert cn == [b"c1", b"c2"] and ref_len == [0, 60_000, g.size]
+    _, orecs, _ = ref.trace(r, l, 4)
+    exp = so.sam_header(ref_len, cn) + so.sam_batch(ref_len, cn, cnl, [b"f%d" % i for i in range(l.size)],
+                                                    [bytes(r[i, :l[i]]) for i in range(l.size)], None, orecs)
+    assert got == exp
