@@ -9,22 +9,25 @@
  *              (read, strand): read encoding (applied from a per-length
  *              recipe), 12-mer hash-offset gathers, long-seed ranking,
  *              residuals (mapper_cpe.c:1246-1401, 2451-2499)
- *   k_plan     one thread per (read, strand): the batch/heap state machine of
- *              alignment_rmdup (mapper_cpe.c:2124-2426); each fetched chunk
- *              of <= 32 positions is loaded at once, filtered by the 15-mer
- *              test (:2090-2123) and shifted to read starts (:1893-1917);
- *              small batches are sorted/uniqued in place (block_heap_sort's
- *              result, :1999-2089), large ones are queued for
- *   k_bigsort  one workgroup per large batch: LDS bitonic sort + unique
+ *   k_plan_wave  one wave per light (read, strand): the one batch of
+ *              alignment_rmdup (mapper_cpe.c:2124-2426) -- the selected
+ *              12-mers' positions fetched 64 lanes at a time, filtered by the
+ *              15-mer test (:2090-2123), shifted to read starts
+ *              (:1893-1917), sorted and uniqued (block_heap_sort's result,
+ *              :1999-2089); heavy items (a 12-mer with more than one chunk)
+ *              go to k_plan_heavy (seed-heap replay) + k_plan_fetch;
+ *              k_plan_plain is the whole state machine one thread per item
+ *   k_bigsort  one workgroup per large batch: bitonic sort + unique
  *   k_compact  one thread per (read, strand): dense candidate list
- *   k_myers    one candidate per lane: window fetch, banded Myers (semantics
- *              mapper_cpe.c:519-660 == :668-976)
+ *   k_myers2   two candidates per lane (e <= 7) / k_myers one: window
+ *              fetch, banded Myers (semantics mapper_cpe.c:519-660 == :668-976)
  *   k_hamming  one lane per passing candidate: the Hamming check of
  *              align_reads (:252-287)
  *   k_trace    one lane per passing candidate that failed the Hamming check:
  *              the traceback of align_reads (:288-509)
- *   k_emit     one thread per read: duplicate rule and flush state machine
- *              (:475-477, 1790, 1858, 1437-1507), per-read reduction
+ *   k_emit     one thread per read (k_emit_heavy one wave per heavy read):
+ *              duplicate rule and flush state machine (:475-477, 1790, 1858,
+ *              1437-1507), per-read reduction
  */
 #include "swm_internal.h"
 
@@ -375,7 +378,7 @@ __global__ void __launch_bounds__(128) k_seed(const char *__restrict__ reads, ui
     o.nchunk[r] = nchunk + 1;   /* batches <= chunks + 1 (an empty first batch) */
 }
 
-/* ------------------------------------------------------------------ k_plan */
+/* ------------------------------------------------------------ plan kernels */
 
 #define SMALL_BATCH 32
 #define GIANT_POS 16384      /* heavy plan items this large are taken first */
@@ -422,75 +425,6 @@ __device__ __forceinline__ uint32_t adjust(uint32_t p, uint32_t meta)
 {
     const uint32_t bo = meta & 0xffffu;
     return p > bo ? p - bo : 0u;
-}
-
-/* Light (read, strand): every selected 12-mer has <= 32 hits, so
- * alignment_rmdup (:2124-2426) fetches all of them in phase 1, the seed heap
- * stays empty and there is exactly one batch.  One thread per item; items
- * needing the heap are queued for k_plan_heavy. */
-__global__ void __launch_bounds__(64) k_plan(uint32_t nrs, const uint2 *__restrict__ pl, PlanIO io)
-{
-    __shared__ uint32_t s_tmp[2];
-    if (*io.ovf) return;
-    uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool live = r < nrs;
-    if (!live) r = 0;
-    const uint32_t nsw = live ? io.nsel[r] : 0u;
-    const uint32_t nsel = nsw & ~NSEL_HEAVY;
-    const swm_sel_t *sel = io.sel + (uint64_t)r * io.seln;
-    const bool heavy = (nsw & NSEL_HEAVY) != 0;
-    /* the largest items go to their own list, which the persistent waves
-     * drain first: the slowest serial replays start at once */
-    const bool giant = live && heavy && io.surv_off[r + 1] - io.surv_off[r] > GIANT_POS;
-    block_append(live && heavy && !giant, r, io.heavy, io.heavy_count, s_tmp);
-    block_append(giant, r, io.giant, io.giant_count, s_tmp);
-    if (!live || heavy) return;
-
-    uint32_t *surv = io.surv + io.surv_off[r];
-    uint2 *bd = io.bdesc + io.bd_off[r];
-    uint32_t nsurv = 0;
-    bool first_pending = true;        /* Q7: the batch's first position is never kept */
-    swm_sel_t s_next = nsel ? sel[0] : swm_sel_t{0, 0, 0};
-    for (uint32_t i = 0; i < nsel; i++) {                     /* phase 1 (:2194-2236) */
-        const swm_sel_t s = s_next;
-        if (i + 1 < nsel) s_next = sel[i + 1];
-        const uint32_t len = s.cnt;                           /* <= 32 for a light item */
-        const uint32_t last = len ? len - 1 : 0;
-        /* 8 entries per group, loaded together with the index clamped into
-         * the chunk (the bucket array has 64 spare entries past its end) */
-        for (uint32_t j0 = 0; j0 < len; j0 += 8) {
-            uint2 v[8];
-#pragma unroll
-            for (int q = 0; q < 8; q++) v[q] = pl[s.off + min(j0 + (uint32_t)q, last)];
-#pragma unroll
-            for (int q = 0; q < 8; q++) {
-                if (j0 + (uint32_t)q < len) {
-                    if (first_pending) first_pending = false;
-                    else if (filter_pass(v[q].y, s.meta)) surv[nsurv++] = adjust(v[q].x, s.meta);
-                }
-            }
-        }
-    }
-    /* the one flush: sorted unique list (block_heap_sort's result) */
-    uint32_t n = nsurv;
-    if (n <= SMALL_BATCH) {
-        for (uint32_t i = 1; i < n; i++) {
-            uint32_t v = surv[i], j = i;
-            while (j > 0 && surv[j - 1] > v) { surv[j] = surv[j - 1]; j--; }
-            surv[j] = v;
-        }
-        uint32_t u = n ? 1u : 0u;
-        for (uint32_t i = 1; i < n; i++)
-            if (surv[i] != surv[u - 1]) surv[u++] = surv[i];
-        bd[0] = make_uint2(0, u);
-        io.nuniq[r] = u;
-    } else {
-        bd[0] = make_uint2(0, n);
-        uint32_t slot = atomicAdd(io.big_count, 1u);
-        io.big[slot] = make_uint2(r, 0);
-        io.nuniq[r] = n;
-    }
-    io.nbatch[r] = 1;
 }
 
 /* The plain form of the same state machine: one thread per (read, strand)
@@ -612,6 +546,122 @@ __global__ void __launch_bounds__(64) k_plan_plain(uint32_t nrs, const uint2 *__
     io.nuniq[r] = nsurv;
 }
 
+/* Lane reads (v_readlane) and writes (a compare + select) at a wave-uniform
+ * lane index.  The seed heap and the per-seed state of k_plan_heavy live one
+ * entry per lane, so its replay is scalar code with no LDS round trip per
+ * heap level. */
+__device__ __forceinline__ uint32_t rl(uint32_t v, uint32_t l)
+{
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)l);
+}
+__device__ __forceinline__ uint32_t wl(uint32_t v, uint32_t l, uint32_t x)
+{
+    return __lane_id() == l ? x : v;
+}
+
+/* Light items, one wave per item: the selected 12-mers' entries are laid
+ * end to end (a prefix sum of the counts) and fetched 64 at a time, lane j
+ * on entry e0 + j, so the lanes of one chunk read consecutive addresses and
+ * one line costs one request -- one thread per item issued a request per
+ * entry (and duplicates of the last one), 2.7 per line.  A wave takes 64
+ * consecutive items: their heavy ones are appended with one atomic per
+ * list, the light ones processed one after the other.  Results as k_plan:
+ * the 15-mer filter and read-start adjustment in entry order with the first
+ * entry dropped (Q7), then the sorted unique list of the one batch (<= 64
+ * survivors sorted in the wave by rank, larger ones left to k_bigsort). */
+__device__ __forceinline__ void wave_append(bool pred, uint32_t value, uint32_t *list, uint32_t *count)
+{
+    const uint64_t m = __ballot(pred);
+    if (!m) return;
+    const uint32_t lane = __lane_id();
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(count, (uint32_t)__popcll(m));
+    base = (uint32_t)__builtin_amdgcn_readfirstlane((int)base);
+    if (pred) list[base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = value;
+}
+
+__global__ void __launch_bounds__(64) k_plan_wave(uint32_t nrs, const uint2 *__restrict__ pl, PlanIO io)
+{
+    __shared__ uint32_t s_off[64], s_pre[64], s_meta[64], s_surv[64];
+    if (*io.ovf) return;
+    const uint32_t lane = threadIdx.x;
+    const uint64_t below = (1ull << lane) - 1ull;
+    for (uint32_t r0 = blockIdx.x * 64u; r0 < nrs; r0 += gridDim.x * 64u) {
+        const uint32_t rr = r0 + lane;
+        const bool live = rr < nrs;
+        const uint32_t nsw = live ? io.nsel[rr] : 0u;
+        const bool heavy = live && (nsw & NSEL_HEAVY) != 0;
+        const bool giant = heavy && io.surv_off[rr + 1] - io.surv_off[rr] > GIANT_POS;
+        wave_append(heavy && !giant, rr, io.heavy, io.heavy_count);
+        wave_append(giant, rr, io.giant, io.giant_count);
+        for (uint64_t todo = __ballot(live && !heavy); todo; todo &= todo - 1ull) {
+            const uint32_t l = (uint32_t)__ffsll((unsigned long long)todo) - 1u;
+            const uint32_t r = r0 + l;
+            const uint32_t nsel = rl(nsw, l);
+            swm_sel_t my = {0, 0, 0};
+            if (lane < nsel) my = io.sel[(uint64_t)r * io.seln + lane];
+            const uint32_t incl = wave_incl_scan_dpp(my.cnt);
+            const uint32_t total = rl(incl, 63);
+            s_off[lane] = my.off;
+            s_pre[lane] = incl - my.cnt;        /* lanes >= nsel: total */
+            s_meta[lane] = my.meta;
+            __syncthreads();
+            uint32_t *surv = io.surv + io.surv_off[r];
+            uint32_t nsurv = 0;
+            for (uint32_t e0 = 0; e0 < total; e0 += 64u) {
+                const uint32_t ent = e0 + lane;
+                const bool in = ent < total;
+                /* the 12-mer of the entry: the last k with s_pre[k] <= ent
+                 * (a 12-mer with no entries shares its start with the next) */
+                uint32_t lo = 0, hi = 64;
+                while (hi - lo > 1) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if (s_pre[mid] <= ent) lo = mid; else hi = mid;
+                }
+                const uint32_t meta = s_meta[lo];
+                const uint2 v = pl[in ? s_off[lo] + (ent - s_pre[lo]) : s_off[0]];
+                const bool keep = in && ent > 0 && filter_pass(v.y, meta);
+                const uint64_t m = __ballot(keep);
+                if (keep) {
+                    const uint32_t at = nsurv + (uint32_t)__popcll(m & below);
+                    const uint32_t val = adjust(v.x, meta);
+                    surv[at] = val;
+                    if (at < 64u) s_surv[at] = val;
+                }
+                nsurv += (uint32_t)__popcll(m);
+            }
+            __syncthreads();
+            uint2 *bd = io.bdesc + io.bd_off[r];
+            if (nsurv <= 64u) {
+                /* sorted unique by rank: a value's place is the number of
+                 * distinct smaller values */
+                const bool have = lane < nsurv;
+                const uint32_t v = have ? s_surv[lane] : 0xffffffffu;
+                bool first = have;
+                for (uint32_t q = 0; q < nsurv; q++) {
+                    const uint32_t w = (uint32_t)__builtin_amdgcn_readlane((int)v, (int)q);
+                    if (w == v && q < lane) first = false;
+                }
+                const uint64_t fm = __ballot(first);
+                uint32_t rank = 0;
+                for (uint64_t f = fm; f; f &= f - 1ull) {
+                    const uint32_t q = (uint32_t)__ffsll((unsigned long long)f) - 1u;
+                    rank += (uint32_t)__builtin_amdgcn_readlane((int)v, (int)q) < v ? 1u : 0u;
+                }
+                if (first) surv[rank] = v;
+                const uint32_t u = (uint32_t)__popcll(fm);
+                if (lane == 0) { bd[0] = make_uint2(0, u); io.nuniq[r] = u; }
+            } else if (lane == 0) {
+                bd[0] = make_uint2(0, nsurv);
+                io.big[atomicAdd(io.big_count, 1u)] = make_uint2(r, 0);
+                io.nuniq[r] = nsurv;
+            }
+            if (lane == 0) io.nbatch[r] = 1;
+            __syncthreads();
+        }
+    }
+}
+
 /* Heavy (read, strand): a device work queue drained by persistent
  * single-wave workgroups.  alignment_rmdup's batch composition -- which
  * 32-position chunks, in what heap order, where each batch ends -- depends
@@ -624,17 +674,6 @@ __global__ void __launch_bounds__(64) k_plan_plain(uint32_t nrs, const uint2 *__
  * serial part of a giant item is the heap replay alone. */
 #define HK 16          /* buffered chunk keys per selected 12-mer */
 
-/* Lane reads (v_readlane) and writes (a compare + select) at a wave-uniform
- * lane index: the seed heap and the per-seed state live one entry per lane,
- * so the replay is scalar code with no LDS round trip per heap level. */
-__device__ __forceinline__ uint32_t rl(uint32_t v, uint32_t l)
-{
-    return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)l);
-}
-__device__ __forceinline__ uint32_t wl(uint32_t v, uint32_t l, uint32_t x)
-{
-    return __lane_id() == l ? x : v;
-}
 
 __global__ void __launch_bounds__(64) k_plan_heavy(const uint2 *__restrict__ pl, PlanIO io)
 {
@@ -2370,7 +2409,8 @@ static int enq_b(const swm_ctx_t *ctx, Pipe &P, uint64_t cap_cand, hipStream_t s
 {
     const swm_opts &o = ctx->opt;
     if (o.work_queue) {
-        k_plan<<<blocks_for(P.nrs, 64), 64, 0, st>>>((uint32_t)P.nrs, ctx->d_pl, P.pio);
+        k_plan_wave<<<std::min<uint32_t>(blocks_for(P.nrs, 64), (uint32_t)ctx->n_cu * 32), 64, 0, st>>>(
+            (uint32_t)P.nrs, ctx->d_pl, P.pio);
         k_plan_heavy<<<ctx->n_cu * o.plan_heavy_waves, 64, 0, st>>>(ctx->d_pl, P.pio);
         k_plan_fetch<<<ctx->n_cu * 16, 64, 0, st>>>(ctx->d_pl, P.pio);
     } else {
