@@ -189,14 +189,19 @@ struct Base {
     bool valid;
 };
 
-__device__ __forceinline__ Base read_base(const uint8_t *rd, uint32_t L, int strand, uint32_t t)
+/* The workgroup's reads are staged in LDS packed: per read 64 bytes of
+ * 2-bit base codes (4 bases a byte) and 64 bytes of "not ACGT" flags (4 a
+ * byte).  A base as the mapper sees it: A,C,G,T -> 0..3 (complemented on
+ * the reverse strand); anything else is invalid and reads as code 1 (C in
+ * the planes, Q15). */
+__device__ __forceinline__ Base read_base(const uint8_t *sq, const uint8_t *sv, uint32_t L, int strand, uint32_t t)
 {
-    uint32_t ch = strand ? rd[L - 1 - t] : rd[t];
+    const uint32_t tt = strand ? L - 1 - t : t;
+    const uint32_t c = (sq[tt >> 2] >> (2 * (tt & 3u))) & 3u;
+    const bool inv = (sv[tt >> 2] >> (tt & 3u)) & 1u;
     Base b;
-    b.valid = (ch == 'A') | (ch == 'C') | (ch == 'G') | (ch == 'T');
-    uint32_t c = ((ch >> 1) ^ (ch >> 2)) & 3u;   /* A 0, C 1, G 2, T 3 */
-    if (strand) c = 3u - c;
-    b.code = b.valid ? c : 1u;
+    b.valid = !inv;
+    b.code = inv ? 1u : (strand ? 3u - c : c);
     return b;
 }
 
@@ -209,33 +214,52 @@ __device__ __forceinline__ uint32_t seed_val(const Base &b, uint32_t u, uint32_t
 }
 
 /* The 12-mer codes of long seed m at shifts q = 0..3 (choose_seed :1372-1390). */
-__device__ __forceinline__ void seed_codes(const uint8_t *rd, uint32_t L, int strand, uint32_t k, uint32_t base,
-                                           uint32_t cq[4])
+__device__ __forceinline__ void seed_codes(const uint8_t *sq, const uint8_t *sv, uint32_t L, int strand, uint32_t k,
+                                           uint32_t base, uint32_t cq[4])
 {
     uint32_t code = 0;
-    for (uint32_t t = 0; t < k; t++) code |= seed_val(read_base(rd, L, strand, base + t), t, k) << (2 * t);
+    for (uint32_t t = 0; t < k; t++) code |= seed_val(read_base(sq, sv, L, strand, base + t), t, k) << (2 * t);
     cq[0] = code;
 #pragma unroll
     for (int q = 1; q < 4; q++) {
         const uint32_t u = k - 1 + (uint32_t)q;
-        code = (code >> 2) | (seed_val(read_base(rd, L, strand, base + u), u, k) << (2 * (k - 1)));
+        code = (code >> 2) | (seed_val(read_base(sq, sv, L, strand, base + u), u, k) << (2 * (k - 1)));
         cq[q] = code;
     }
 }
 
-/* The offset and count of every 12-mer of the thread's strand are kept in
- * LDS from the ranking gathers, so the selected ones are not fetched twice.
- * That takes 80 KB of LDS per workgroup (one wave per SIMD), and still wins:
- * the kernel is bound by the rate of random line fetches, not by latency --
- * gathering the selected 12-mers again with twice the waves was 25% slower. */
+/* Packed code byte and invalid nibble of 4 raw bases. */
+__device__ __forceinline__ void pack_quad(uint32_t raw, uint8_t *q, uint8_t *v)
+{
+    uint32_t cb = 0, ib = 0;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const uint32_t ch = (raw >> (8 * j)) & 0xffu;
+        const bool valid = (ch == 'A') | (ch == 'C') | (ch == 'G') | (ch == 'T');
+        cb |= (((ch >> 1) ^ (ch >> 2)) & 3u) << (2 * j);     /* A 0, C 1, G 2, T 3 */
+        ib |= (valid ? 0u : 1u) << j;
+    }
+    *q = (uint8_t)cb;
+    *v = (uint8_t)ib;
+}
+
+/* The offset and count of the 12-mers of the first SEED_CACHE_M long seeds
+ * of the thread's strand are kept in LDS from the ranking gathers, so the
+ * selected ones are not fetched twice; the few selected beyond them (reads
+ * of 165 bases and more) and counts of 65535 or more are gathered again.
+ * With the reads packed, the workgroup holds 38 KB of LDS, so two waves per
+ * SIMD issue gathers: the kernel is bound by the rate of random line fetches
+ * (one per lookup), and that rate grows with the waves in flight. */
+#define SEED_CACHE_M 10
+
 __global__ void __launch_bounds__(128) k_seed(const char *__restrict__ reads, uint64_t stride,
                                               const uint32_t *__restrict__ lens, uint32_t nreads, uint32_t k,
                                               uint32_t e, uint32_t seln, const uint32_t *__restrict__ hash_offset,
                                               const swm_recipe_t *__restrict__ recipes, SeedOut o)
 {
-    __shared__ uint32_t s_rd32[SEED_READS * 256 / 4];
-    __shared__ uint32_t s_off[64 * 2 * SEED_READS], s_cnt[64 * 2 * SEED_READS];
-    uint8_t *s_rd = reinterpret_cast<uint8_t *>(s_rd32);
+    __shared__ uint8_t s_q[SEED_READS * 64], s_v[SEED_READS * 64];
+    __shared__ uint32_t s_off[4 * SEED_CACHE_M * 2 * SEED_READS];
+    __shared__ uint16_t s_cnt[4 * SEED_CACHE_M * 2 * SEED_READS];
     if (blockIdx.x == 0) {
         /* the run's counters and the sentinel entries the scans read */
         if (threadIdx.x < 10) o.counters[threadIdx.x] = 0;
@@ -248,13 +272,25 @@ __global__ void __launch_bounds__(128) k_seed(const char *__restrict__ reads, ui
     const uint32_t r0 = blockIdx.x * SEED_READS;
     const uint32_t nr = min((uint32_t)SEED_READS, nreads - r0);
     {
+        /* quad kq of read lr = its bases 4kq .. 4kq+3 (bytes past the stride,
+         * or past the batch, read as 0: they lie beyond every read's length) */
         const char *src = reads + (uint64_t)r0 * stride;
-        const uint32_t nbytes = nr * (uint32_t)stride;
-        if ((((uintptr_t)src) & 3u) == 0 && (nbytes & 3u) == 0) {
-            const uint32_t *s32 = reinterpret_cast<const uint32_t *>(src);
-            for (uint32_t i = threadIdx.x; i < nbytes / 4; i += blockDim.x) s_rd32[i] = s32[i];
-        } else {
-            for (uint32_t i = threadIdx.x; i < nbytes; i += blockDim.x) s_rd[i] = (uint8_t)src[i];
+        const uint64_t nbytes = (uint64_t)nr * stride;
+        const bool words = (((uintptr_t)src) & 3u) == 0 && (stride & 3u) == 0;
+        for (uint32_t qd = threadIdx.x; qd < nr * 64u; qd += blockDim.x) {
+            const uint32_t lr = qd >> 6, kq = qd & 63u;
+            uint32_t raw = 0;
+            if (4u * kq < stride) {
+                const uint64_t at = (uint64_t)lr * stride + 4u * kq;
+                if (words) {
+                    raw = *reinterpret_cast<const uint32_t *>(src + at);
+                } else {
+#pragma unroll
+                    for (uint32_t j = 0; j < 4; j++)
+                        if (4u * kq + j < stride && at + j < nbytes) raw |= (uint32_t)(uint8_t)src[at + j] << (8 * j);
+                }
+            }
+            pack_quad(raw, &s_q[qd], &s_v[qd]);
         }
     }
     __syncthreads();
@@ -263,7 +299,7 @@ __global__ void __launch_bounds__(128) k_seed(const char *__restrict__ reads, ui
     const int strand = (int)(threadIdx.x & 1u);
     const uint32_t read = r0 + lr;
     const uint32_t r = 2 * read + (uint32_t)strand;
-    const uint8_t *rd = s_rd + lr * (uint32_t)stride;
+    const uint8_t *sq = s_q + lr * 64u, *sv = s_v + lr * 64u;
     uint32_t L = lens[read];
     const uint32_t seed_len = k + SWM_SEED_STEP - 1;
     /* reads that do not fit the reference's fixed tables map to nothing:
@@ -286,7 +322,7 @@ __global__ void __launch_bounds__(128) k_seed(const char *__restrict__ reads, ui
             for (int s = 0; s < 16; s++) {
                 const uint32_t t = 16 * j + s;
                 if (t < L) {
-                    Base b = read_base(rd, L, strand, t);
+                    Base b = read_base(sq, sv, L, strand, t);
                     v |= b.code << (2 * s);
                     if (t < n_planes) {
                         pl[j >> 1] |= (b.code & 1u) << (t & 31u);
@@ -314,15 +350,17 @@ __global__ void __launch_bounds__(128) k_seed(const char *__restrict__ reads, ui
         if ((uint32_t)m < n_long) {
             const uint32_t base = (uint32_t)m * seed_len;
             uint32_t cq[4];
-            seed_codes(rd, L, strand, k, base, cq);
+            seed_codes(sq, sv, L, strand, k, base, cq);
             uint32_t a[4], b[4];
 #pragma unroll
             for (int q = 0; q < 4; q++) { a[q] = hash_offset[cq[q]]; b[q] = hash_offset[cq[q] + 1]; }
 #pragma unroll
             for (int q = 0; q < 4; q++) {
                 tot[m] += b[q] - a[q];
-                s_off[(4 * m + q) * (2 * SEED_READS) + threadIdx.x] = a[q];
-                s_cnt[(4 * m + q) * (2 * SEED_READS) + threadIdx.x] = b[q] - a[q];
+                if (m < SEED_CACHE_M) {
+                    s_off[(4 * m + q) * (2 * SEED_READS) + threadIdx.x] = a[q];
+                    s_cnt[(4 * m + q) * (2 * SEED_READS) + threadIdx.x] = (uint16_t)min(b[q] - a[q], 0xffffu);
+                }
             }
         }
     }
@@ -350,19 +388,38 @@ __global__ void __launch_bounds__(128) k_seed(const char *__restrict__ reads, ui
         if ((uint32_t)jm < n_sel_long) {
             const uint32_t m = idx[jm];
             const uint32_t base = m * seed_len;
+            uint32_t off4[4], cnt4[4];
+            bool again = m >= SEED_CACHE_M;
+            if (!again) {
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    off4[q] = s_off[(4 * m + q) * (2 * SEED_READS) + threadIdx.x];
+                    cnt4[q] = s_cnt[(4 * m + q) * (2 * SEED_READS) + threadIdx.x];
+                    again |= cnt4[q] == 0xffffu;
+                }
+            }
+            if (again) {
+                uint32_t cq[4];
+                seed_codes(sq, sv, L, strand, k, base, cq);
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    off4[q] = hash_offset[cq[q]];
+                    cnt4[q] = hash_offset[cq[q] + 1] - off4[q];
+                }
+            }
 #pragma unroll
             for (int q = 0; q < 4; q++) {
                 uint32_t rl = 0, rh = 0;
 #pragma unroll
                 for (int bb = 0; bb < 3; bb++) {
                     const uint32_t u = (bb < q) ? (uint32_t)bb : k + (uint32_t)bb;
-                    const uint32_t c = read_base(rd, L, strand, base + u).code;
+                    const uint32_t c = read_base(sq, sv, L, strand, base + u).code;
                     rl |= (c & 1u) << bb;
                     rh |= (c >> 1) << bb;
                 }
                 swm_sel_t s;
-                s.off = s_off[(4 * m + q) * (2 * SEED_READS) + threadIdx.x];
-                s.cnt = s_cnt[(4 * m + q) * (2 * SEED_READS) + threadIdx.x];
+                s.off = off4[q];
+                s.cnt = cnt4[q];
                 s.meta = (base + (uint32_t)q) | (rl << 16) | (rh << 19) | ((uint32_t)q << 22);
                 so[4 * jm + q] = s;
                 heavy |= s.cnt > SWM_CHUNK;
