@@ -402,9 +402,9 @@ def main():
         npos = st_all[-1]["n_positions"]
         if ms_plan > 0 and npos:
             ach = 6 * npos / (ms_plan / 1e3) / 1e9
-            roof_plan = {"bound": "hbm", "kernel": "k_plan (+heavy, large-batch sort, scans)",
+            roof_plan = {"bound": "hbm", "kernel": "k_plan_wave (+heavy, large-batch sort, scans)",
                          "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": pmc_traffic(args.config, "k_plan"),
+                         "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": pmc_traffic(args.config, "k_plan_wave"),
                          "algorithmic_bytes_per_launch": 6 * npos, "avg_launch_ms": round(ms_plan, 3)}
             roof_plan["traffic_rate"] = traffic_rate(roof_plan["traffic"], ms_plan)
         sum_l_ver = st_all[-1]["n_band_steps"]
@@ -415,7 +415,7 @@ def main():
             measured = swm.valu_peak()
             peak = max(VALU_PEAK_OPS, measured)
             ach = MYERS_OPS_PER_STEP * sum_l_ver / (ms_myers / 1e3)
-            roof_myers = {"bound": "valu", "kernel": "k_myers", "form": timed, "achieved": round(ach / 1e12, 3),
+            roof_myers = {"bound": "valu", "kernel": "k_myers2 (e <= 7; k_myers above)", "form": timed, "achieved": round(ach / 1e12, 3),
                           "peak": round(peak / 1e12, 2), "unit": "Tops/s (int32)",
                           "frac": round(ach / peak, 4), "traffic": None,
                           "peak_probe_tops": round(measured / 1e12, 2),
