@@ -44,7 +44,6 @@ namespace {
 /* ---------------------------------------------------------------- helpers */
 
 __device__ __forceinline__ uint32_t shl32(uint32_t x, int s) { return (s < 0 || s >= 32) ? 0u : (x << s); }
-__device__ __forceinline__ uint32_t shr32(uint32_t x, int s) { return (s < 0 || s >= 32) ? 0u : (x >> s); }
 
 /* Plane word (0 = low bit plane, 1 = high) of the genome starting at absolute
  * base g.  The device copy of .base_2bits carries SWM_WIN_PAIRS_PAD zero
@@ -620,21 +619,26 @@ __device__ __forceinline__ uint32_t wl(uint32_t v, uint32_t l, uint32_t x)
  * end to end (a prefix sum of the counts) and fetched 64 at a time, lane j
  * on entry e0 + j, so the lanes of one chunk read consecutive addresses and
  * one line costs one request -- one thread per item issued a request per
- * entry (and duplicates of the last one), 2.7 per line.  A wave takes 64
- * consecutive items: their heavy ones are appended with one atomic per
- * list, the light ones processed one after the other.  Results as k_plan:
+ * entry (and duplicates of the last one), 2.7 per line.  Waves take items
+ * one at a time (grid-stride), so a small batch is spread over all waves
+ * instead of 64 items in a row per wave; k_plan_classify appends the heavy
+ * items beforehand, one atomic per list per workgroup.  Results as k_plan:
  * the 15-mer filter and read-start adjustment in entry order with the first
  * entry dropped (Q7), then the sorted unique list of the one batch (<= 64
  * survivors sorted in the wave by rank, larger ones left to k_bigsort). */
-__device__ __forceinline__ void wave_append(bool pred, uint32_t value, uint32_t *list, uint32_t *count)
+/* Heavy items (a selected 12-mer with more than one chunk) to the seed-heap
+ * kernels' lists, the largest ones to their own list. */
+__global__ void __launch_bounds__(256) k_plan_classify(uint32_t nrs, PlanIO io)
 {
-    const uint64_t m = __ballot(pred);
-    if (!m) return;
-    const uint32_t lane = __lane_id();
-    uint32_t base = 0;
-    if (lane == 0) base = atomicAdd(count, (uint32_t)__popcll(m));
-    base = (uint32_t)__builtin_amdgcn_readfirstlane((int)base);
-    if (pred) list[base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = value;
+    __shared__ uint32_t s_tmp[2];
+    if (*io.ovf) return;
+    uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = r < nrs;
+    if (!live) r = 0;
+    const bool heavy = live && (io.nsel[r] & NSEL_HEAVY) != 0;
+    const bool giant = heavy && io.surv_off[r + 1] - io.surv_off[r] > GIANT_POS;
+    block_append(heavy && !giant, r, io.heavy, io.heavy_count, s_tmp);
+    block_append(giant, r, io.giant, io.giant_count, s_tmp);
 }
 
 __global__ void __launch_bounds__(64) k_plan_wave(uint32_t nrs, const uint2 *__restrict__ pl, PlanIO io)
@@ -643,18 +647,11 @@ __global__ void __launch_bounds__(64) k_plan_wave(uint32_t nrs, const uint2 *__r
     if (*io.ovf) return;
     const uint32_t lane = threadIdx.x;
     const uint64_t below = (1ull << lane) - 1ull;
-    for (uint32_t r0 = blockIdx.x * 64u; r0 < nrs; r0 += gridDim.x * 64u) {
-        const uint32_t rr = r0 + lane;
-        const bool live = rr < nrs;
-        const uint32_t nsw = live ? io.nsel[rr] : 0u;
-        const bool heavy = live && (nsw & NSEL_HEAVY) != 0;
-        const bool giant = heavy && io.surv_off[rr + 1] - io.surv_off[rr] > GIANT_POS;
-        wave_append(heavy && !giant, rr, io.heavy, io.heavy_count);
-        wave_append(giant, rr, io.giant, io.giant_count);
-        for (uint64_t todo = __ballot(live && !heavy); todo; todo &= todo - 1ull) {
-            const uint32_t l = (uint32_t)__ffsll((unsigned long long)todo) - 1u;
-            const uint32_t r = r0 + l;
-            const uint32_t nsel = rl(nsw, l);
+    {
+        for (uint32_t r = blockIdx.x; r < nrs; r += gridDim.x) {
+            const uint32_t nsw = io.nsel[r];
+            if (nsw & NSEL_HEAVY) continue;
+            const uint32_t nsel = nsw;
             swm_sel_t my = {0, 0, 0};
             if (lane < nsel) my = io.sel[(uint64_t)r * io.seln + lane];
             const uint32_t incl = wave_incl_scan_dpp(my.cnt);
@@ -2466,7 +2463,8 @@ static int enq_b(const swm_ctx_t *ctx, Pipe &P, uint64_t cap_cand, hipStream_t s
 {
     const swm_opts &o = ctx->opt;
     if (o.work_queue) {
-        k_plan_wave<<<std::min<uint32_t>(blocks_for(P.nrs, 64), (uint32_t)ctx->n_cu * 32), 64, 0, st>>>(
+        k_plan_classify<<<blocks_for(P.nrs, 256), 256, 0, st>>>((uint32_t)P.nrs, P.pio);
+        k_plan_wave<<<(uint32_t)std::min<uint64_t>(P.nrs ? P.nrs : 1, (uint64_t)ctx->n_cu * 32), 64, 0, st>>>(
             (uint32_t)P.nrs, ctx->d_pl, P.pio);
         k_plan_heavy<<<ctx->n_cu * o.plan_heavy_waves, 64, 0, st>>>(ctx->d_pl, P.pio);
         k_plan_fetch<<<ctx->n_cu * 16, 64, 0, st>>>(ctx->d_pl, P.pio);
