@@ -647,13 +647,27 @@ __global__ void __launch_bounds__(64) k_plan_wave(uint32_t nrs, const uint2 *__r
     if (*io.ovf) return;
     const uint32_t lane = threadIdx.x;
     const uint64_t below = (1ull << lane) - 1ull;
-    {
-        for (uint32_t r = blockIdx.x; r < nrs; r += gridDim.x) {
-            const uint32_t nsw = io.nsel[r];
+    /* the next item's selection count and selections are loaded while this
+     * one is processed (its loads would otherwise head a dependent chain) */
+    uint32_t r = blockIdx.x;
+    uint32_t nsw_n = 0;
+    swm_sel_t my_n = {0, 0, 0};
+    if (r < nrs) {
+        nsw_n = io.nsel[r];
+        if (lane < io.seln) my_n = io.sel[(uint64_t)r * io.seln + lane];
+    }
+    for (; r < nrs; r += gridDim.x) {
+        const uint32_t nsw = nsw_n;
+        swm_sel_t my = my_n;
+        const uint32_t rn = r + gridDim.x;
+        if (rn < nrs) {
+            nsw_n = io.nsel[rn];
+            if (lane < io.seln) my_n = io.sel[(uint64_t)rn * io.seln + lane];
+        }
+        {
             if (nsw & NSEL_HEAVY) continue;
             const uint32_t nsel = nsw;
-            swm_sel_t my = {0, 0, 0};
-            if (lane < nsel) my = io.sel[(uint64_t)r * io.seln + lane];
+            if (lane >= nsel) my = swm_sel_t{0, 0, 0};       /* slots past nsel are not written */
             const uint32_t incl = wave_incl_scan_dpp(my.cnt);
             const uint32_t total = rl(incl, 63);
             s_off[lane] = my.off;
