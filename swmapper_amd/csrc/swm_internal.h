@@ -7,6 +7,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <mutex>
+
 #include "swm.h"
 #include "swm_recipe.h"
 
@@ -32,6 +34,14 @@ struct swm_sel_t {
 
 struct swm_ws;    /* device workspace of one slot */
 struct swm_slot;  /* stream + workspace + captured graphs */
+
+/* Order of the slots' pipeline runs on the device: each run waits for the
+ * previous one (whatever its slot) before its kernels start, so runs do not
+ * share the GPU -- only one slot's copies overlap another's kernels. */
+struct swm_gate {
+    std::mutex mu;
+    hipEvent_t last = nullptr;   /* the previous run's end (an event of its slot) */
+};
 
 /* Forms and tunables of the pipeline (swm_ctx_set_option). */
 struct swm_opts {
@@ -67,6 +77,8 @@ const swm_stats_t *swm_slot_stats(const swm_slot *s);
 uint64_t swm_slot_bytes(const swm_slot *s);
 /* size the slot's workspace for batches of up to n reads once graphs are used */
 void swm_slot_reserve(swm_slot *s, uint64_t n);
+/* runs of this slot take their turn through the gate (NULL: no gate) */
+void swm_slot_set_gate(swm_slot *s, swm_gate *g);
 /* map n reads already on the device, on the slot's stream, in the synced or
  * the graph form (returns after the results are written) */
 int swm_slot_run_device(swm_ctx_t *ctx, swm_slot *sl, const char *d_reads, uint64_t stride, const uint32_t *d_lens,

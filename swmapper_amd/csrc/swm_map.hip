@@ -2302,8 +2302,11 @@ struct swm_slot {
      * that smaller batches never move a buffer (which would retire every
      * captured graph of the slot) */
     uint64_t reserve_n = 0;
+    swm_gate *gate = nullptr;         /* shared with the other slots of a server */
+    hipEvent_t ev_gate = nullptr;     /* the end of this slot's last run */
     ~swm_slot()
     {
+        if (ev_gate) (void)hipEventDestroy(ev_gate);
         if (stream) (void)hipStreamSynchronize(stream);
         for (auto &x : g) if (x.exec) (void)hipGraphExecDestroy(x.exec);
         for (auto &x : ev) if (x) (void)hipEventDestroy(x);
@@ -2331,6 +2334,7 @@ swm_slot *swm_slot_create(swm_ctx_t *ctx)
     if (!s) return nullptr;
     bool ok = hipStreamCreate(&s->stream) == hipSuccess;
     for (auto &x : s->ev) ok = ok && hipEventCreate(&x) == hipSuccess;
+    ok = ok && hipEventCreateWithFlags(&s->ev_gate, hipEventDisableTiming) == hipSuccess;
     ok = ok && hipHostMalloc((void **)&s->h_small, 16 * sizeof(uint64_t), hipHostMallocDefault) == hipSuccess;
     if (!ok) { delete s; return nullptr; }
     return s;
@@ -2343,6 +2347,28 @@ const swm_stats_t *swm_slot_stats(const swm_slot *s) { return &s->stats; }
 uint64_t swm_slot_bytes(const swm_slot *s) { return s->ws.bytes(); }
 
 void swm_slot_reserve(swm_slot *s, uint64_t n) { s->reserve_n = n; }
+
+void swm_slot_set_gate(swm_slot *s, swm_gate *g) { s->gate = g; }
+
+/* The gate around one run's kernels: wait for the previous run (any slot)
+ * on this slot's stream, and after the kernels publish this slot's end. */
+struct GateTurn {
+    swm_slot *sl;
+    std::unique_lock<std::mutex> lk;
+    explicit GateTurn(swm_slot *s) : sl(s)
+    {
+        if (!sl->gate) return;
+        lk = std::unique_lock<std::mutex>(sl->gate->mu);
+        if (sl->gate->last) (void)hipStreamWaitEvent(sl->stream, sl->gate->last, 0);
+    }
+    void done()
+    {
+        if (!sl->gate || !lk.owns_lock()) return;
+        if (hipEventRecord(sl->ev_gate, sl->stream) == hipSuccess) sl->gate->last = sl->ev_gate;
+        lk.unlock();
+    }
+    ~GateTurn() { done(); }
+};
 
 static double host_ms()
 {
@@ -2605,6 +2631,7 @@ static int run_synced(swm_ctx_t *ctx, swm_slot *sl, const char *d_reads, uint64_
     const uint32_t seln = 4 * ((e + 1) < 16 ? (e + 1) : 16);
     Pipe P;
     int rc;
+    GateTurn turn(sl);
     if ((rc = pipe_alloc_a(ws, n, seln, P))) return rc;
     if ((rc = enq_a(ctx, P, d_reads, stride, d_lens, e, ~0ull, ~0ull, st, sl->ev))) return rc;
     uint64_t *hs = sl->h_small;
@@ -2626,6 +2653,7 @@ static int run_synced(swm_ctx_t *ctx, swm_slot *sl, const char *d_reads, uint64_
         return rc;
     if ((rc = enq_c(ctx, P, d_lens, e, d_out, ncand, st, sl->ev))) return rc;
     if (recs && (rc = records_out(ctx, sl, P, d_lens, ncand, recs))) return rc;
+    turn.done();
     SWM_HIP_CHECK(hipMemcpyAsync(hs, P.counters, 8 * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
     if (h_out && n_out)
         SWM_HIP_CHECK(hipMemcpyAsync(h_out, d_out, n_out * sizeof(swm_read_result_t), hipMemcpyDeviceToHost, st));
@@ -2782,9 +2810,12 @@ static int run_graph(swm_ctx_t *ctx, swm_slot *sl, const char *d_reads, uint64_t
     }
     ent->tick = ++sl->tick;
     memset(&sl->stats, 0, sizeof(sl->stats));
-    SWM_HIP_CHECK(hipEventRecord(sl->ev[0], st));
-    SWM_HIP_CHECK(hipGraphLaunch(ent->exec, st));
-    SWM_HIP_CHECK(hipEventRecord(sl->ev[5], st));
+    {
+        GateTurn turn(sl);
+        SWM_HIP_CHECK(hipEventRecord(sl->ev[0], st));
+        SWM_HIP_CHECK(hipGraphLaunch(ent->exec, st));
+        SWM_HIP_CHECK(hipEventRecord(sl->ev[5], st));
+    }
     const unsigned long long *hc = reinterpret_cast<const unsigned long long *>(sl->h_small);
     SWM_HIP_CHECK(hipMemcpyAsync(sl->h_small, P.counters, 8 * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
     if (h_out && n_out)

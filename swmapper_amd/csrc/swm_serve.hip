@@ -63,6 +63,7 @@ struct swm_server {
     swm_ctx_t *ctx = nullptr;
     swm_server_config_t cfg{};
     std::mutex mu;
+    swm_gate gate;              /* the slots' runs take turns on the device; copies overlap */
     std::condition_variable cv_work, cv_done;
     std::deque<Req *> pending;
     std::unordered_map<uint64_t, Req *> reqs;
@@ -173,6 +174,7 @@ extern "C" int swm_server_create(swm_ctx_t *ctx, const swm_server_config_t *cfg,
         swm_slot *sl = swm_slot_create(ctx);
         if (!sl) { for (auto *x : s->slots) swm_slot_destroy(x); delete s; return SWM_EDEVICE; }
         if (c.graph) swm_slot_reserve(sl, bucket(std::max(c.max_batch, c.max_launch)));
+        swm_slot_set_gate(sl, &s->gate);
         s->slots.push_back(sl);
     }
     try {
