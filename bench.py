@@ -357,7 +357,8 @@ def main():
             f"same results as {forms[0]}: {same}")
         for s in st_all:
             log("  " + " ".join(f"{k}={s[k]:.2f}" for k in ("ms_seed", "ms_plan", "ms_compact", "ms_myers",
-                                                             "ms_emit", "ms_total")))
+                                                             "ms_emit", "ms_total"))
+                + " " + " ".join(f"{k}={s[k]}" for k in ("n_heavy_items", "n_large_batches", "n_heavy_reads")))
     # per-read candidate histogram of the workload (one extra run, synced form)
     m.set_option(swm.OPT_GRAPH, 0)
     m.set_option(swm.OPT_HISTOGRAM, 1)

@@ -79,6 +79,9 @@ typedef struct swm_stats {
      * bin 0 = none, bin b = [2^(b-1), 2^b); and the largest count */
     uint64_t cand_hist[24];
     uint64_t max_cands;
+    /* work-queue load: heavy (read, strand) items of the seed heap, batches
+     * sorted by the large-batch kernel, reads emitted one wave each */
+    uint64_t n_heavy_items, n_large_batches, n_heavy_reads;
 } swm_stats_t;
 
 typedef struct swm_ctx swm_ctx_t;

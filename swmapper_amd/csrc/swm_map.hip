@@ -2294,7 +2294,8 @@ struct swm_slot {
     /* page-locked staging, so that no copy goes through the runtime's shared
      * pageable-memory path (which serialises the slots): totals read back
      * between stages, counters, and host batches in and out */
-    uint64_t *h_small = nullptr;     /* [0..7] counters, [8] positions, [9] chunks, [10] candidates */
+    uint64_t *h_small = nullptr;     /* [0..9] counters at the end; before that [8] positions, [9] chunks,
+                                        [10] candidates, [11..13] record totals */
     struct Pin { void *p = nullptr; size_t cap = 0; };
     Pin h_reads, h_lens, h_out;
     double t_entry = 0;   /* host clock at the start of the current call */
@@ -2581,6 +2582,10 @@ static void stats_from_counters(swm_stats_t *s, uint64_t n, const unsigned long 
     s->n_band_steps = hc[2];
     s->n_records = hc[1];
     s->n_positions = hc[3];
+    const uint32_t *u2 = reinterpret_cast<const uint32_t *>(hc + 8);   /* u32 view of [8..9] */
+    s->n_large_batches = u[0];
+    s->n_heavy_items = u[3] + u2[1];
+    s->n_heavy_reads = u2[2] + u2[3];
 }
 
 /* The per-read candidate histogram of the run just finished (stats). */
@@ -2654,7 +2659,7 @@ static int run_synced(swm_ctx_t *ctx, swm_slot *sl, const char *d_reads, uint64_
     if ((rc = enq_c(ctx, P, d_lens, e, d_out, ncand, st, sl->ev))) return rc;
     if (recs && (rc = records_out(ctx, sl, P, d_lens, ncand, recs))) return rc;
     turn.done();
-    SWM_HIP_CHECK(hipMemcpyAsync(hs, P.counters, 8 * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+    SWM_HIP_CHECK(hipMemcpyAsync(hs, P.counters, 10 * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
     if (h_out && n_out)
         SWM_HIP_CHECK(hipMemcpyAsync(h_out, d_out, n_out * sizeof(swm_read_result_t), hipMemcpyDeviceToHost, st));
     if ((rc = final_sync(sl))) return rc;
@@ -2817,7 +2822,7 @@ static int run_graph(swm_ctx_t *ctx, swm_slot *sl, const char *d_reads, uint64_t
         SWM_HIP_CHECK(hipEventRecord(sl->ev[5], st));
     }
     const unsigned long long *hc = reinterpret_cast<const unsigned long long *>(sl->h_small);
-    SWM_HIP_CHECK(hipMemcpyAsync(sl->h_small, P.counters, 8 * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+    SWM_HIP_CHECK(hipMemcpyAsync(sl->h_small, P.counters, 10 * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
     if (h_out && n_out)
         SWM_HIP_CHECK(hipMemcpyAsync(h_out, d_out, n_out * sizeof(swm_read_result_t), hipMemcpyDeviceToHost, st));
     if ((rc = final_sync(sl))) return rc;
