@@ -614,6 +614,14 @@ __device__ __forceinline__ uint32_t wl(uint32_t v, uint32_t l, uint32_t x)
 {
     return __lane_id() == l ? x : v;
 }
+/* a > b ? x : y on scalars, as s_cmp + s_cselect (the compiler's own form
+ * of this select goes through a vector register and back) */
+__device__ __forceinline__ uint32_t sel_gt(uint32_t a, uint32_t b, uint32_t x, uint32_t y)
+{
+    uint32_t r;
+    asm("s_cmp_gt_u32 %1, %2\n\ts_cselect_b32 %0, %3, %4" : "=s"(r) : "s"(a), "s"(b), "s"(x), "s"(y) : "scc");
+    return r;
+}
 
 /* Light items, one wave per item: the selected 12-mers' entries are laid
  * end to end (a prefix sum of the counts) and fetched 64 at a time, lane j
@@ -739,13 +747,21 @@ __global__ void __launch_bounds__(64) k_plan_wave(uint32_t nrs, const uint2 *__r
  * into LDS and writes each batch's chunk list; the batches' positions are
  * fetched, filtered and compacted by k_plan_fetch, all batches in parallel,
  * each at its own offset (its first position's index in the item), so the
- * serial part of a giant item is the heap replay alone. */
-#define HK 16          /* buffered chunk keys per selected 12-mer */
-
+ * serial part of a giant item is the heap replay alone.
+ *
+ * The replay is one dependent chain per item (a pop decides the next root),
+ * so a pop is kept short: the heap keys of each 12-mer's next two chunks sit
+ * in registers (one lane per 12-mer) and the key after them is read from an
+ * LDS ring during the sift-down and written one pop later; the sift-down is
+ * unrolled (scalar selects on lane reads); the ring is
+ * refilled only when some 12-mer runs low, and then only its consumed
+ * slots are loaded. */
+#define HK 16          /* ring of buffered chunk keys per selected 12-mer */
+#define HK_LOW 6       /* refill when a 12-mer has fewer buffered keys left */
 
 __global__ void __launch_bounds__(64) k_plan_heavy(const uint2 *__restrict__ pl, PlanIO io)
 {
-    __shared__ uint32_t skey[64 * HK];           /* keys of chunks kb .. kb+HK-1 of each 12-mer */
+    __shared__ uint32_t skey[64 * HK];           /* key of chunk c of 12-mer i at [i * HK + c % HK] */
     __shared__ uint32_t sv[1];
     const uint32_t lane = threadIdx.x;
     const uint64_t below = (1ull << lane) - 1ull;
@@ -765,25 +781,32 @@ __global__ void __launch_bounds__(64) k_plan_heavy(const uint2 *__restrict__ pl,
         if (act) my = io.sel[(uint64_t)r * io.seln + lane];
         const uint32_t cnt_v = my.cnt, off_v = my.off;
         uint32_t spc_v = min(cnt_v, (uint32_t)SWM_CHUNK);     /* positions fetched */
-        uint32_t kb_v = 0;                                     /* first chunk in the key buffer */
         /* heap key after fetching chunk c: the raw last position of that
-         * chunk (seeds_heap_pop_and_insert's pos argument, :2350) */
-        auto refill = [&]() {
-            if (act) {
-                kb_v = spc_v / SWM_CHUNK;
+         * chunk (seeds_heap_pop_and_insert's pos argument, :2350).  The
+         * ring holds chunks kb .. kb + HK - 1; (re)fill chunks
+         * max(lo, kbn) .. kbn + HK - 1 and make kbn the new base */
+        uint32_t kb_v = 1;
+        auto fill = [&](bool live, uint32_t kbn, uint32_t lo) {
+            /* all loads first (clamped to an entry of the item), then the stores */
+            uint32_t kv[HK];
 #pragma unroll
-                for (uint32_t k = 0; k < HK; k++) {
-                    const uint32_t c = kb_v + k;
-                    skey[lane * HK + k] =
-                        (SWM_CHUNK * c < cnt_v) ? pl[off_v + min(SWM_CHUNK * (c + 1), cnt_v) - 1].x : 0u;
-                }
+            for (uint32_t k = 0; k < HK; k++) {
+                const uint32_t c = kbn + k;
+                const bool ok = live && c >= lo && SWM_CHUNK * c < cnt_v;
+                kv[k] = pl[ok ? off_v + min(SWM_CHUNK * (c + 1), cnt_v) - 1 : off_v].x;
             }
+#pragma unroll
+            for (uint32_t k = 0; k < HK; k++) {
+                const uint32_t c = kbn + k;
+                if (live && c >= lo && SWM_CHUNK * c < cnt_v) skey[lane * HK + (c & (HK - 1))] = kv[k];
+            }
+            if (live) kb_v = kbn;
         };
-        refill();
-        /* key of each 12-mer's next chunk, one per lane: a pop reads it with
-         * one readlane, and the key after it is loaded during the sift-down */
+        fill(act, 1, 1);
         __syncthreads();
-        uint32_t nk_v = act ? skey[lane * HK] : 0u;
+        /* keys of each 12-mer's next chunk (n = fetched / 32) and the one after */
+        uint32_t nk_v = (act && SWM_CHUNK < cnt_v) ? skey[lane * HK + 1] : 0u;
+        uint32_t nk2_v = (act && 2 * SWM_CHUNK < cnt_v) ? skey[lane * HK + 2] : 0u;
         uint2 *cd = io.cdesc + io.bd_off[r];
 
         /* phase 1 (:2194-2236): the first chunk of every 12-mer, in order */
@@ -816,52 +839,75 @@ __global__ void __launch_bounds__(64) k_plan_heavy(const uint2 *__restrict__ pl,
         }
         __syncthreads();   /* key buffers are read below */
         uint32_t batch = 0, cstart = 0, posoff = 0;
+        /* the key read by the last pop for nk2_v of lane pend_l: from the
+         * ring (pend_s, read by every pop so that it is never copied while
+         * in flight) or, past the ring, from memory (pend_g) */
+        uint32_t pend_s = 0, pend_g = 0, pend_l = 64u;
+        bool pend_fb = false;
+        /* chunk descriptors nfl .. nch - 1 of the batch wait in registers,
+         * entry j in lane j % 64, and are stored 64 at a time */
+        uint32_t cdx_v = 0, cdy_v = 0, nfl = nch;
+        auto flush = [&]() {
+            const uint32_t j = nfl + ((lane - nfl) & 63u);
+            if (j < nch) cd[cstart + j] = make_uint2(cdx_v, cdy_v);
+            nfl = nch;
+        };
         for (;;) {
             /* the rest of this batch's chunk list (:2261-2356), wave-uniform */
-            for (;;) {
-                if (hsize > 0 && pos_size + SWM_CHUNK <= SWM_BATCH_POS && blocks < SWM_MAX_BLOCKS) {
-                    const uint32_t i = rl(hind, 0);
-                    const uint32_t cnt = rl(cnt_v, i), pc = rl(spc_v, i), off = rl(off_v, i);
-                    const uint32_t c = pc / SWM_CHUNK;            /* chunk fetched now */
-                    const uint32_t get = min((uint32_t)SWM_CHUNK, cnt - pc);
-                    if (lane == 0) cd[cstart + nch] = make_uint2(off + pc, pos_size | (i << 16));
-                    nch++;
-                    pos_size += get;
-                    blocks++;
-                    spc_v = wl(spc_v, i, pc + get);
-                    /* seeds_heap_pop_and_insert (:1936-1962, Q6): the new root
-                     * (the last entry, or this 12-mer with its next key) sifts
-                     * down; its value stays in scalars and only the hole moves,
-                     * which is the same sequence of swaps */
-                    uint32_t mloc, mind, nk = 0;
-                    if (pc + get == cnt) {
-                        mloc = rl(hloc, hsize - 1);
-                        mind = rl(hind, hsize - 1);
-                        hsize--;
-                    } else {
-                        mloc = rl(nk_v, i);
-                        mind = i;
-                        const uint32_t kb = rl(kb_v, i), c1 = c + 1;
-                        if (c1 < kb + HK) nk = skey[i * HK + (c1 - kb)];
-                        else if (SWM_CHUNK * c1 < cnt) nk = pl[off + min(SWM_CHUNK * (c1 + 1), cnt) - 1].x;
-                    }
-                    uint32_t x = 1;
-                    while (x * 2 < hsize) {
-                        const uint32_t a2 = rl(hloc, 2 * x - 1), a3 = rl(hloc, 2 * x);
-                        uint32_t t = x * 2, lt = a2;
-                        if (x * 2 + 1 <= hsize && a2 > a3) { t = x * 2 + 1; lt = a3; }
-                        if (lt < mloc) {
-                            hloc = wl(hloc, x - 1, lt);
-                            hind = wl(hind, x - 1, rl(hind, t - 1));
-                        } else break;
-                        x = t;
-                    }
-                    hloc = wl(hloc, x - 1, mloc);
-                    hind = wl(hind, x - 1, mind);
-                    nk_v = wl(nk_v, i, nk);
+            while (hsize > 0 && pos_size + SWM_CHUNK <= SWM_BATCH_POS && blocks < SWM_MAX_BLOCKS) {
+                nk2_v = wl(nk2_v, pend_l, pend_fb ? pend_g : pend_s);
+                const uint32_t i = rl(hind, 0);
+                const uint32_t cnt = rl(cnt_v, i), pc = rl(spc_v, i), off = rl(off_v, i);
+                const uint32_t c = pc / SWM_CHUNK;            /* chunk fetched now */
+                const uint32_t get = min((uint32_t)SWM_CHUNK, cnt - pc);
+                const uint32_t c2 = c + 2;
+                pend_s = skey[i * HK + (c2 & (HK - 1))];
+                cdx_v = wl(cdx_v, nch & 63u, off + pc);
+                cdy_v = wl(cdy_v, nch & 63u, pos_size | (i << 16));
+                nch++;
+                if (nch - nfl == 64u) flush();
+                pos_size += get;
+                blocks++;
+                spc_v = wl(spc_v, i, pc + get);
+                /* seeds_heap_pop_and_insert (:1936-1962, Q6): the new root
+                 * (the last entry, or this 12-mer with its next key) sifts
+                 * down; its value stays in scalars and only the hole moves,
+                 * which is the same sequence of swaps */
+                uint32_t mloc, mind;
+                if (pc + get == cnt) {
+                    mloc = rl(hloc, hsize - 1);
+                    mind = rl(hind, hsize - 1);
+                    hsize--;
+                    pend_l = 64u;
+                } else {
+                    mloc = rl(nk_v, i);
+                    mind = i;
+                    nk_v = wl(nk_v, i, rl(nk2_v, i));
+                    pend_l = i;
+                    pend_fb = c2 >= rl(kb_v, i) + HK;
+                    if (pend_fb) pend_g = (SWM_CHUNK * c2 < cnt) ? pl[off + min(SWM_CHUNK * (c2 + 1), cnt) - 1].x : 0u;
                 }
-                if (pos_size + SWM_CHUNK > SWM_BATCH_POS || hsize == 0 || blocks >= SWM_MAX_BLOCKS) break;
+                /* while (i * 2 < size): the smaller child (the left one on a
+                 * tie) moves up while it is below the new root's key; at
+                 * most 5 moves in a heap of <= 64 */
+                uint32_t x = 1;
+#pragma unroll
+                for (int lev = 0; lev < 5; lev++) {
+                    const uint32_t c2x = 2 * x;
+                    if (c2x >= hsize) break;
+                    const uint32_t a2 = rl(hloc, c2x - 1), a3 = rl(hloc, c2x);
+                    const uint32_t t = sel_gt(a2, a3, c2x + 1, c2x), lt = min(a2, a3);
+                    if (lt >= mloc) break;
+                    hind = wl(hind, x - 1, rl(hind, t - 1));
+                    hloc = wl(hloc, x - 1, lt);
+                    x = t;
+                }
+                hloc = wl(hloc, x - 1, mloc);
+                hind = wl(hind, x - 1, mind);
             }
+            nk2_v = wl(nk2_v, pend_l, pend_fb ? pend_g : pend_s);
+            pend_l = 64u;
+            flush();
             if (lane == 0) {
                 HBatch hb;
                 hb.r = r; hb.batch = batch; hb.cstart = cstart; hb.npack = nch | (pos_size << 8);
@@ -872,12 +918,18 @@ __global__ void __launch_bounds__(64) k_plan_heavy(const uint2 *__restrict__ pl,
             cstart += nch;
             posoff += pos_size;
             if (hsize == 0) break;
-            /* next batch: refill the key buffers, start an empty chunk list */
-            __syncthreads();
-            refill();
-            nch = 0; pos_size = 0; blocks = 0;
-            __syncthreads();
-            nk_v = act ? skey[lane * HK] : 0u;
+            /* next batch: top up the rings if a 12-mer runs low (its keys
+             * from chunk n + 2 on are read from the ring), start an empty
+             * chunk list */
+            {
+                const uint32_t n_v = spc_v / SWM_CHUNK;
+                const bool live = act && spc_v < cnt_v;
+                if (__ballot(live && n_v + 2 + HK_LOW > kb_v + HK)) {
+                    fill(live, n_v + 2, max(kb_v + HK, n_v + 2));
+                    __syncthreads();
+                }
+            }
+            nch = 0; nfl = 0; pos_size = 0; blocks = 0;
         }
         if (lane == 0) { io.nbatch[r] = batch; io.nuniq[r] = 0; }
         __syncthreads();
