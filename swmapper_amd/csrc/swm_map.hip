@@ -17,7 +17,7 @@
  *              :1999-2089); heavy items (a 12-mer with more than one chunk)
  *              go to k_plan_heavy (seed-heap replay) + k_plan_fetch;
  *              k_plan_plain is the whole state machine one thread per item
- *   k_bigsort  one workgroup per large batch: bitonic sort + unique
+ *   k_bigsort  one wave per large batch: register bitonic sort + unique
  *   k_compact  one thread per (read, strand): dense candidate list
  *   k_myers2   two candidates per lane (e <= 7) / k_myers one: window
  *              fetch, banded Myers (semantics mapper_cpe.c:519-660 == :668-976)
@@ -1019,104 +1019,157 @@ __global__ void __launch_bounds__(64) k_plan_fetch(const uint2 *__restrict__ pl,
     }
 }
 
-/* One workgroup per large batch (<= 2048 values): bitonic sort with 8
- * values per thread in registers (element i = thread * 8 + register).
- * Compare-exchange partners i ^ j are in the same thread for j < 8, in the
- * same wave for j < 512 (shuffles), and only the stages with j >= 512 go
- * through LDS (laid out register-major, so conflict-free).  Then unique by a
- * block-wide scan over "differs from the left neighbour". */
-__device__ __forceinline__ void cas_reg(uint32_t &x, uint32_t &y, bool asc)
+/* One wave per large batch (65 .. 2048 values), sorted in registers: R =
+ * 2 .. 32 values per lane (the batch padded to 64 R with ~0), logical
+ * element i = lane * R + r.  A bitonic network in its "flip" form -- the
+ * first comparator of each merge phase pairs i with its mirror in the
+ * block, the rest pair i with i ^ j -- keeps every comparator ascending,
+ * so no direction logic: a comparator inside a lane is a min and a max, one
+ * across lanes is a lane permute and one v_med3 with 0 (keep the smaller)
+ * or ~0 (keep the larger) by the lane's side.  Lane permutes are DPP for
+ * the masks within a row of 16 (quad_perm, row_mirror, row_half_mirror,
+ * row_ror:8), ds_swizzle for 16 and 31, ds_bpermute only for 32 and 63.
+ * Then unique: element i is kept when i < n and it differs from i - 1. */
+template <int M>
+__device__ __forceinline__ uint32_t xlane(uint32_t x)
 {
-    const uint32_t lo = min(x, y), hi = max(x, y);
-    x = asc ? lo : hi;
-    y = asc ? hi : lo;
+    if constexpr (M == 1) return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0xB1, 0xf, 0xf, false);  /* quad_perm 1,0,3,2 */
+    else if constexpr (M == 2) return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x4E, 0xf, 0xf, false);
+    else if constexpr (M == 3) return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x1B, 0xf, 0xf, false);
+    else if constexpr (M == 4)                                                              /* ^3 then ^7 */
+        return (uint32_t)__builtin_amdgcn_mov_dpp(__builtin_amdgcn_mov_dpp((int)x, 0x1B, 0xf, 0xf, false),
+                                                  0x141, 0xf, 0xf, false);
+    else if constexpr (M == 7) return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x141, 0xf, 0xf, false); /* row_half_mirror */
+    else if constexpr (M == 8) return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x128, 0xf, 0xf, false); /* row_ror:8 */
+    else if constexpr (M == 15) return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x140, 0xf, 0xf, false); /* row_mirror */
+    else if constexpr (M == 16) return (uint32_t)__builtin_amdgcn_ds_swizzle((int)x, 0x401F);  /* xor 16 within 32 */
+    else if constexpr (M == 31) return (uint32_t)__builtin_amdgcn_ds_swizzle((int)x, 0x7C1F);  /* xor 31 within 32 */
+    else return (uint32_t)__shfl_xor((int)x, M, 64);
 }
 
-template <int J>
-__device__ __forceinline__ void bitonic_in_thread(uint32_t (&v)[8], uint32_t i0, uint32_t kk)
+__device__ __forceinline__ uint32_t med3u(uint32_t a, uint32_t b, uint32_t c)
 {
+    return min(max(a, b), max(min(a, b), c));      /* v_med3_u32 */
+}
+
+/* half-cleaner steps: i with i ^ 2^B, B .. 0 */
+template <int R, int RB, int B>
+__device__ __forceinline__ void bitonic_steps(uint32_t (&v)[R], uint32_t lane)
+{
+    if constexpr (B >= 0) {
+        if constexpr (B < RB) {
+            constexpr int j = 1 << B;
 #pragma unroll
-    for (int r = 0; r < 8; r++)
-        if ((r & J) == 0) cas_reg(v[r], v[r | J], ((i0 + (uint32_t)r) & kk) == 0);
+            for (int r = 0; r < R; r++) {
+                if ((r & j) == 0) {
+                    const uint32_t lo = min(v[r], v[r | j]), hi = max(v[r], v[r | j]);
+                    v[r] = lo;
+                    v[r | j] = hi;
+                }
+            }
+        } else {
+            const uint32_t c = ((lane >> (B - RB)) & 1u) ? 0xffffffffu : 0u;
+#pragma unroll
+            for (int r = 0; r < R; r++) v[r] = med3u(v[r], xlane<(1 << (B - RB))>(v[r]), c);
+        }
+        bitonic_steps<R, RB, B - 1>(v, lane);
+    }
 }
 
-__global__ void __launch_bounds__(256) k_bigsort(const uint2 *__restrict__ big, const uint32_t *__restrict__ big_count,
-                                                 const uint64_t *__restrict__ surv_off,
-                                                 const uint32_t *__restrict__ bd_off, uint32_t *__restrict__ surv,
-                                                 uint2 *__restrict__ bdesc, uint32_t *__restrict__ nuniq)
+/* merge phases K .. RB + 6: the flip step (i with its mirror in a block of
+ * 2^K), then the half-cleaners */
+template <int R, int RB, int K>
+__device__ __forceinline__ void bitonic_phases(uint32_t (&v)[R], uint32_t lane)
 {
-    __shared__ uint32_t xs[8 * 256];
-    __shared__ uint32_t s_w[4], s_edge[4];
+    if constexpr (K <= RB + 6) {
+        if constexpr (K <= RB) {
+            constexpr int mk = (1 << K) - 1;
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                const int q = r ^ mk;
+                if (r < q) {
+                    const uint32_t lo = min(v[r], v[q]), hi = max(v[r], v[q]);
+                    v[r] = lo;
+                    v[q] = hi;
+                }
+            }
+        } else {
+            const uint32_t c = ((lane >> (K - RB - 1)) & 1u) ? 0xffffffffu : 0u;
+#pragma unroll
+            for (int r = 0; r < R / 2; r++) {
+                const uint32_t a = xlane<(1 << (K - RB)) - 1>(v[R - 1 - r]);
+                const uint32_t b = xlane<(1 << (K - RB)) - 1>(v[r]);
+                v[r] = med3u(v[r], a, c);
+                v[R - 1 - r] = med3u(v[R - 1 - r], b, c);
+            }
+        }
+        bitonic_steps<R, RB, K - 2>(v, lane);
+        bitonic_phases<R, RB, K + 1>(v, lane);
+    }
+}
+
+template <int R>
+__device__ __forceinline__ void wave_bitonic(uint32_t (&v)[R], uint32_t lane)
+{
+    constexpr int RB = R == 2 ? 1 : R == 4 ? 2 : R == 8 ? 3 : R == 16 ? 4 : 5;
+    bitonic_phases<R, RB, 1>(v, lane);
+}
+
+/* sort base[0 .. n) (n <= 64 R) and write its distinct values in place;
+ * returns their count */
+template <int R>
+__device__ __forceinline__ uint32_t wave_sort_unique(uint32_t *base, uint32_t n, uint32_t lane)
+{
+    uint32_t v[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const uint32_t i = (uint32_t)r * 64u + lane;
+        v[r] = i < n ? base[i] : 0xffffffffu;
+    }
+    wave_bitonic<R>(v, lane);
+    /* element i - 1 of element lane * R: the last of the lane below (wave_shr:1) */
+    const uint32_t prev = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v[R - 1], 0x138, 0xf, 0xf, false);
+    uint32_t cnt = 0;
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const uint32_t i = lane * R + (uint32_t)r;
+        const uint32_t left = r ? v[r - 1] : prev;
+        cnt += (i < n && (i == 0 || v[r] != left)) ? 1u : 0u;
+    }
+    const uint32_t incl = wave_incl_scan_dpp(cnt);
+    uint32_t wp = incl - cnt;
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const uint32_t i = lane * R + (uint32_t)r;
+        const uint32_t left = r ? v[r - 1] : prev;
+        if (i < n && (i == 0 || v[r] != left)) base[wp++] = v[r];
+    }
+    return (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
+}
+
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8))) k_bigsort(const uint2 *__restrict__ big, const uint32_t *__restrict__ big_count,
+                                                const uint64_t *__restrict__ surv_off,
+                                                const uint32_t *__restrict__ bd_off, uint32_t *__restrict__ surv,
+                                                uint2 *__restrict__ bdesc, uint32_t *__restrict__ nuniq)
+{
+    const uint32_t lane = threadIdx.x;
     const uint32_t nb = *big_count;
-    const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
-    const uint32_t i0 = tid * 8u;
     for (uint32_t bt = blockIdx.x; bt < nb; bt += gridDim.x) {
         const uint2 rb = big[bt];
         uint2 *dp = bdesc + bd_off[rb.x] + rb.y;
         const uint2 d = *dp;
         const uint32_t n = d.y;
         uint32_t *base = surv + surv_off[rb.x] + d.x;
-        uint32_t P = 64;
-        while (P < n) P <<= 1;
-        uint32_t v[8];
-#pragma unroll
-        for (int r = 0; r < 8; r++) v[r] = i0 + (uint32_t)r < n ? base[i0 + r] : 0xffffffffu;
-        for (uint32_t kk = 2; kk <= P; kk <<= 1) {
-            for (uint32_t j = kk >> 1; j > 0; j >>= 1) {
-                if (j >= 512) {
-#pragma unroll
-                    for (int r = 0; r < 8; r++) xs[r * 256 + tid] = v[r];
-                    __syncthreads();
-                    const uint32_t pt = tid ^ (j >> 3);
-                    const bool keep_min = ((i0 & j) == 0) == ((i0 & kk) == 0);
-#pragma unroll
-                    for (int r = 0; r < 8; r++) {
-                        const uint32_t o = xs[r * 256 + pt];
-                        v[r] = keep_min ? min(v[r], o) : max(v[r], o);
-                    }
-                    __syncthreads();
-                } else if (j >= 8) {
-                    const int m = (int)(j >> 3);
-                    const bool keep_min = ((i0 & j) == 0) == ((i0 & kk) == 0);
-#pragma unroll
-                    for (int r = 0; r < 8; r++) {
-                        const uint32_t o = (uint32_t)__shfl_xor((int)v[r], m, 64);
-                        v[r] = keep_min ? min(v[r], o) : max(v[r], o);
-                    }
-                } else if (j == 4) {
-                    bitonic_in_thread<4>(v, i0, kk);
-                } else if (j == 2) {
-                    bitonic_in_thread<2>(v, i0, kk);
-                } else {
-                    bitonic_in_thread<1>(v, i0, kk);
-                }
-            }
-        }
-        /* unique: element i is kept when i < n and it differs from i - 1 */
-        uint32_t prev = (uint32_t)__shfl_up((int)v[7], 1, 64);
-        if (lane == 63) s_edge[w] = v[7];
-        __syncthreads();
-        if (lane == 0) prev = w ? s_edge[w - 1] : 0u;
-        uint32_t cnt = 0;
-#pragma unroll
-        for (int r = 0; r < 8; r++) {
-            const uint32_t i = i0 + (uint32_t)r;
-            const uint32_t left = r ? v[r - 1] : prev;
-            cnt += (i < n && (i == 0 || v[r] != left)) ? 1u : 0u;
-        }
         uint32_t total;
-        uint32_t wpos = block_incl_scan<uint32_t, 256>(cnt, s_w, &total) - cnt;
-#pragma unroll
-        for (int r = 0; r < 8; r++) {
-            const uint32_t i = i0 + (uint32_t)r;
-            const uint32_t left = r ? v[r - 1] : prev;
-            if (i < n && (i == 0 || v[r] != left)) base[wpos++] = v[r];
-        }
-        if (tid == 0) {
+        if (n <= 128) total = wave_sort_unique<2>(base, n, lane);
+        else if (n <= 256) total = wave_sort_unique<4>(base, n, lane);
+        else if (n <= 512) total = wave_sort_unique<8>(base, n, lane);
+        else if (n <= 1024) total = wave_sort_unique<16>(base, n, lane);
+        else total = wave_sort_unique<32>(base, n, lane);
+        if (lane == 0) {
             dp->y = total;
             atomicSub(&nuniq[rb.x], n - total);
         }
-        __syncthreads();
     }
 }
 
@@ -2564,7 +2617,7 @@ static int enq_b(const swm_ctx_t *ctx, Pipe &P, uint64_t cap_cand, hipStream_t s
     } else {
         k_plan_plain<<<blocks_for(P.nrs, 64), 64, 0, st>>>((uint32_t)P.nrs, ctx->d_pl, P.pio);
     }
-    k_bigsort<<<ctx->n_cu * 8, 256, 0, st>>>(P.pio.big, P.pio.big_count, P.surv_off, P.bd_off, P.pio.surv, P.pio.bdesc,
+    k_bigsort<<<ctx->n_cu * 32, 64, 0, st>>>(P.pio.big, P.pio.big_count, P.surv_off, P.bd_off, P.pio.surv, P.pio.bdesc,
                                     P.pio.nuniq);
     SWM_HIP_CHECK(hipGetLastError());
     if (ev) SWM_HIP_CHECK(hipEventRecord(ev[2], st));

@@ -112,6 +112,22 @@ def test_graph_replays_between_other_work():
     m.close()
 
 
+def test_tandem_runs_the_large_batch_sort(tandem):
+    """The oracle comparisons above cover the large-batch sort (batches of
+    65 .. 2048 survivors, sorted and de-duplicated in registers, one wave
+    per batch) only if the fixture produces such batches, with both heavy
+    items (the seed heap) and light ones."""
+    gi, r, l, exp = tandem
+    _say("tandem large batches")
+    m = swm.Mapper.from_index(gi)
+    got = m.map(r, l, 5)
+    _assert_same(got, exp[5][0], "large batches")
+    st = m.stats()
+    _say("stats", {k: st[k] for k in ("n_large_batches", "n_heavy_items", "n_heavy_reads")})
+    assert st["n_large_batches"] > 200 and st["n_heavy_items"] > 100, st
+    m.close()
+
+
 def test_autotune_keeps_results(tandem):
     gi, r, l, exp = tandem
     _say("autotune")
