@@ -766,6 +766,21 @@ __global__ void __launch_bounds__(64) k_plan_heavy(const uint2 *__restrict__ pl,
     const uint32_t lane = threadIdx.x;
     const uint64_t below = (1ull << lane) - 1ull;
     const uint32_t ng = *io.giant_count, nq = ng + *io.heavy_count;
+    /* batch records wait in registers (record j in lane j) and are appended
+     * 64 at a time: one atomic on the shared counter per 64 batches */
+    uint32_t hq_r = 0, hq_b = 0, hq_c = 0, hq_n = 0, hq_p = 0, nhq = 0;
+    auto hb_flush = [&]() {
+        uint32_t at = 0;
+        if (lane == 0 && nhq) at = atomicAdd(io.hb_count, nhq);
+        at = (uint32_t)__builtin_amdgcn_readfirstlane((int)at);
+        if (lane < nhq) {
+            HBatch hb;
+            hb.r = hq_r; hb.batch = hq_b; hb.cstart = hq_c; hb.npack = hq_n;
+            hb.posoff = hq_p; hb.pad0 = hb.pad1 = hb.pad2 = 0;
+            io.hbatch[at + lane] = hb;
+        }
+        nhq = 0;
+    };
 
     for (;;) {
         if (lane == 0) sv[0] = atomicAdd(io.heavy_head, 1u);
@@ -908,12 +923,12 @@ __global__ void __launch_bounds__(64) k_plan_heavy(const uint2 *__restrict__ pl,
             nk2_v = wl(nk2_v, pend_l, pend_fb ? pend_g : pend_s);
             pend_l = 64u;
             flush();
-            if (lane == 0) {
-                HBatch hb;
-                hb.r = r; hb.batch = batch; hb.cstart = cstart; hb.npack = nch | (pos_size << 8);
-                hb.posoff = posoff; hb.pad0 = hb.pad1 = hb.pad2 = 0;
-                io.hbatch[atomicAdd(io.hb_count, 1u)] = hb;
-            }
+            hq_r = wl(hq_r, nhq, r);
+            hq_b = wl(hq_b, nhq, batch);
+            hq_c = wl(hq_c, nhq, cstart);
+            hq_n = wl(hq_n, nhq, nch | (pos_size << 8));
+            hq_p = wl(hq_p, nhq, posoff);
+            if (++nhq == 64u) hb_flush();
             batch++;
             cstart += nch;
             posoff += pos_size;
@@ -934,6 +949,7 @@ __global__ void __launch_bounds__(64) k_plan_heavy(const uint2 *__restrict__ pl,
         if (lane == 0) { io.nbatch[r] = batch; io.nuniq[r] = 0; }
         __syncthreads();
     }
+    hb_flush();
 }
 
 /* The batches of heavy items, one wave each, in parallel: fetch the batch's
@@ -947,6 +963,17 @@ __global__ void __launch_bounds__(64) k_plan_fetch(const uint2 *__restrict__ pl,
     __shared__ uint32_t smeta[64];
     const uint32_t lane = threadIdx.x;
     const uint32_t nb_all = *io.hb_count;
+    /* large batches for k_bigsort wait in registers (entry j in lane j) and
+     * are appended 64 at a time: one atomic on the shared counter per 64
+     * batches instead of one per batch */
+    uint32_t bq_r = 0, bq_b = 0, nbq = 0;
+    auto big_flush = [&]() {
+        uint32_t at = 0;
+        if (lane == 0 && nbq) at = atomicAdd(io.big_count, nbq);
+        at = (uint32_t)__builtin_amdgcn_readfirstlane((int)at);
+        if (lane < nbq) io.big[at + lane] = make_uint2(bq_r, bq_b);
+        nbq = 0;
+    };
     for (uint32_t h = blockIdx.x; h < nb_all; h += gridDim.x) {
         const HBatch hb = io.hbatch[h];
         const uint32_t r = hb.r, bn = hb.npack & 0xffu, bpos = hb.npack >> 8;
@@ -964,27 +991,26 @@ __global__ void __launch_bounds__(64) k_plan_fetch(const uint2 *__restrict__ pl,
         uint32_t *surv = io.surv + io.surv_off[r];
         const uint32_t bstart = hb.posoff;
         uint32_t nsurv = bstart;
-        /* 256 entries per pass, 4 per lane, so 4 loads are in flight per
-         * lane; kept entries are written in entry order */
-        for (uint32_t e0 = 0; e0 < bpos; e0 += 256) {
-            uint2 v[4];
-            uint32_t meta[4];
-            bool live[4];
+        /* a chunk (<= 32 entries) per half wave, lane & 31 on its entry, so
+         * a load covers two chunks with consecutive addresses and lane order
+         * is entry order; 16 chunks per pass, 8 loads in flight per lane;
+         * kept entries are written in entry order */
+        for (uint32_t k0 = 0; k0 < bn; k0 += 16) {
+            uint2 v[8];
+            uint32_t meta[8];
+            bool live[8];
 #pragma unroll
-            for (int q = 0; q < 4; q++) {
-                const uint32_t ent = e0 + 64u * (uint32_t)q + lane;
-                live[q] = ent < bpos && ent > 0;             /* entry 0: the Q7 drop */
-                uint32_t lo = 0, hi = bn;                    /* chunk with spre[lo] <= ent < spre[lo+1] */
-                while (hi - lo > 1) {
-                    const uint32_t mid = (lo + hi) >> 1;
-                    if (spre[mid] <= ent) lo = mid; else hi = mid;
-                }
-                const uint32_t at = live[q] ? sg[lo] + (ent - spre[lo]) : sg[0];
-                v[q] = pl[at];
-                meta[q] = smeta[sseed[lo]];
+            for (int q = 0; q < 8; q++) {
+                const uint32_t k = k0 + 2u * (uint32_t)q + (lane >> 5), ei = lane & 31u;
+                const bool kin = k < bn;
+                const uint32_t kk = kin ? k : 0u;
+                const uint32_t ent = spre[kk] + ei;
+                live[q] = kin && ent < spre[kk + 1] && ent > 0;      /* entry 0: the Q7 drop */
+                v[q] = pl[live[q] ? sg[kk] + ei : sg[0]];
+                meta[q] = smeta[sseed[kk]];
             }
 #pragma unroll
-            for (int q = 0; q < 4; q++) {
+            for (int q = 0; q < 8; q++) {
                 const bool keep = live[q] && filter_pass(v[q].y, meta[q]);
                 const uint64_t m = __ballot(keep);
                 if (keep) surv[nsurv + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = adjust(v[q].x, meta[q]);
@@ -1010,13 +1036,16 @@ __global__ void __launch_bounds__(64) k_plan_fetch(const uint2 *__restrict__ pl,
             if (head) surv[bstart + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = v;
             u = (uint32_t)__popcll(m);
             if (lane == 0) bd[hb.batch] = make_uint2(bstart, u);
-        } else if (lane == 0) {
-            bd[hb.batch] = make_uint2(bstart, n);
-            io.big[atomicAdd(io.big_count, 1u)] = make_uint2(r, hb.batch);
+        } else {
+            if (lane == 0) bd[hb.batch] = make_uint2(bstart, n);
+            bq_r = wl(bq_r, nbq, r);
+            bq_b = wl(bq_b, nbq, hb.batch);
+            if (++nbq == 64u) big_flush();
         }
         if (lane == 0 && u) atomicAdd(&io.nuniq[r], u);
         __syncthreads();
     }
+    big_flush();
 }
 
 /* One wave per large batch (65 .. 2048 values), sorted in registers: R =
@@ -2613,7 +2642,7 @@ static int enq_b(const swm_ctx_t *ctx, Pipe &P, uint64_t cap_cand, hipStream_t s
         k_plan_wave<<<(uint32_t)std::min<uint64_t>(P.nrs ? P.nrs : 1, (uint64_t)ctx->n_cu * 32), 64, 0, st>>>(
             (uint32_t)P.nrs, ctx->d_pl, P.pio);
         k_plan_heavy<<<ctx->n_cu * o.plan_heavy_waves, 64, 0, st>>>(ctx->d_pl, P.pio);
-        k_plan_fetch<<<ctx->n_cu * 16, 64, 0, st>>>(ctx->d_pl, P.pio);
+        k_plan_fetch<<<ctx->n_cu * 32, 64, 0, st>>>(ctx->d_pl, P.pio);
     } else {
         k_plan_plain<<<blocks_for(P.nrs, 64), 64, 0, st>>>((uint32_t)P.nrs, ctx->d_pl, P.pio);
     }
