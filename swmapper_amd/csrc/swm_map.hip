@@ -2014,22 +2014,33 @@ __device__ __forceinline__ uint32_t rdlane(uint32_t v, uint32_t l)
     return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)l);
 }
 
+/* One strand's share of a read's result: its record count and its best
+ * record (key and location); the read's result combines the two. */
+struct EmitPart {
+    uint32_t count, best_rec;
+    uint64_t best;
+};
+
 /* heavy[0 .. heavy_count[0]): giant reads; heavy[nreads ..
- * + heavy_count[1]): the other heavy reads */
+ * + heavy_count[1]): the other heavy reads.  A strand's emission state
+ * starts afresh (the output block counters, the duplicate rule), so the
+ * two strands of a read are separate work items -- slot 2h + strand, the
+ * giant reads' first -- and a giant read's serial walk is split in two;
+ * k_emit_combine then forms the read's result from the two parts. */
 __global__ void __launch_bounds__(64) k_emit_heavy(const uint32_t *__restrict__ heavy,
                                                    const uint32_t *__restrict__ heavy_count, uint32_t nreads, uint32_t e,
                                                    const uint32_t *__restrict__ cand_off, CandSet c,
-                                                   swm_read_result_t *__restrict__ out,
-                                                   unsigned long long *n_records)
+                                                   EmitPart *__restrict__ part, unsigned long long *n_records)
 {
     const uint32_t lane = threadIdx.x;
     const uint64_t below = (1ull << lane) - 1ull;
     const uint32_t ng = heavy_count[0], nh = ng + heavy_count[1];
-    for (uint32_t h = blockIdx.x; h < nh; h += gridDim.x) {
+    for (uint32_t hs = blockIdx.x; hs < 2 * nh; hs += gridDim.x) {
+        const uint32_t h = hs >> 1, strand = hs & 1u;
         const uint32_t rd = h < ng ? heavy[h] : heavy[nreads + (h - ng)];
         uint32_t count = 0, best_rec = 0;
         uint64_t best = ~0ull;
-        for (uint32_t strand = 0; strand < 2; strand++) {
+        {
             const uint32_t rs = 2 * rd + strand;
             const uint32_t c0 = cand_off[rs], cend = cand_off[rs + 1];
             /* carries between windows */
@@ -2185,15 +2196,37 @@ __global__ void __launch_bounds__(64) k_emit_heavy(const uint32_t *__restrict__ 
             close_group();
         }
         if (lane == 0) {
-            swm_read_result_t res;
-            res.pos = count ? best_rec : 0xffffffffu;
-            res.strand = count ? (uint8_t)((best >> 32) & 1ull) : (uint8_t)0xff;
-            res.dist = count ? (uint8_t)(best >> 40) : (uint8_t)0xff;
-            res.pad = 0;
-            res.count = count;
-            out[rd] = res;
+            EmitPart pt;
+            pt.count = count;
+            pt.best_rec = best_rec;
+            pt.best = best;
+            part[hs] = pt;
             if (count) atomicAdd(n_records, (unsigned long long)count);
         }
+    }
+}
+
+/* A heavy read's result from its strands' parts: the counts add up and the
+ * best record is the smaller key (distance, strand, order). */
+__global__ void __launch_bounds__(256) k_emit_combine(const uint32_t *__restrict__ heavy,
+                                                     const uint32_t *__restrict__ heavy_count, uint32_t nreads,
+                                                     const EmitPart *__restrict__ part,
+                                                     swm_read_result_t *__restrict__ out)
+{
+    const uint32_t ng = heavy_count[0], nh = ng + heavy_count[1];
+    for (uint32_t h = blockIdx.x * blockDim.x + threadIdx.x; h < nh; h += gridDim.x * blockDim.x) {
+        const uint32_t rd = h < ng ? heavy[h] : heavy[nreads + (h - ng)];
+        const EmitPart a = part[2 * h], b = part[2 * h + 1];
+        const uint32_t count = a.count + b.count;
+        const bool second = b.best < a.best;
+        const uint64_t best = second ? b.best : a.best;
+        swm_read_result_t res;
+        res.pos = count ? (second ? b.best_rec : a.best_rec) : 0xffffffffu;
+        res.strand = count ? (uint8_t)((best >> 32) & 1ull) : (uint8_t)0xff;
+        res.dist = count ? (uint8_t)(best >> 40) : (uint8_t)0xff;
+        res.pad = 0;
+        res.count = count;
+        out[rd] = res;
     }
 }
 
@@ -2377,7 +2410,7 @@ struct swm_ws {
     struct Buf { void *p = nullptr; size_t cap = 0; };
     enum { WORDS, SEL, NSEL, NPOS, NCHUNK, SURV_OFF, BD_OFF, SURV, BDESC, NBATCH, NUNIQ, CAND_OFF, C_VAL, C_RS, C_BI,
            C_ERR, C_LOC, C_REC, C_INFO, COUNTERS, BIG, TRACE, HEAVY, HEAVY_C, HEAVY_E, SCAN_TMP, IN_READS, IN_LENS,
-           OUT, HIST, R_WR, R_TB, R_RUNS, R_IDX, R_TOFF, R_SCAN, R_TOT, R_OUT, R_TEXT, CDESC, HBATCH, GIANT, NBUF };
+           OUT, HIST, R_WR, R_TB, R_RUNS, R_IDX, R_TOFF, R_SCAN, R_TOT, R_OUT, R_TEXT, CDESC, HBATCH, GIANT, EPART, NBUF };
     Buf b[NBUF];
     uint64_t gen = 1;
     ~swm_ws() { for (auto &x : b) if (x.p) (void)hipFree(x.p); }
@@ -2532,6 +2565,7 @@ struct Pipe {
     CandSet cs;
     uint64_t *surv_off = nullptr;
     uint32_t *bd_off = nullptr, *cand_off = nullptr, *trace = nullptr, *heavy_c = nullptr, *heavy_e = nullptr;
+    EmitPart *epart = nullptr;
     unsigned long long *counters = nullptr;
     uint32_t *u32c = nullptr;
     uint64_t *scan_tmp = nullptr;
@@ -2557,6 +2591,7 @@ static int pipe_alloc_a(swm_ws *ws, uint64_t n, uint32_t seln, Pipe &P)
         (rc = ws->get(swm_ws::NBATCH, &P.pio.nbatch, nrs)) || (rc = ws->get(swm_ws::NUNIQ, &P.pio.nuniq, nrs + 1)) ||
         (rc = ws->get(swm_ws::HEAVY, &P.pio.heavy, nrs)) || (rc = ws->get(swm_ws::GIANT, &P.pio.giant, nrs)) || (rc = ws->get(swm_ws::CAND_OFF, &P.cand_off, nrs + 1)) ||
         (rc = ws->get(swm_ws::HEAVY_C, &P.heavy_c, nrs)) || (rc = ws->get(swm_ws::HEAVY_E, &P.heavy_e, 2 * n)) ||
+        (rc = ws->get(swm_ws::EPART, &P.epart, 2 * n)) ||
         (rc = ws->get(swm_ws::SCAN_TMP, &P.scan_tmp, 2 * (uint64_t)blocks_for(nrs + 1, SCAN_TILE))))
         return rc;
     P.so.counters = P.counters;
@@ -2700,9 +2735,11 @@ static int enq_c(const swm_ctx_t *ctx, Pipe &P, const uint32_t *d_lens, uint32_t
                                           P.cs);
     k_emit<<<blocks_for(P.n, 128), 128, 0, st>>>((uint32_t)P.n, e, P.cand_off, P.cs, d_out, P.counters + 1,
                                                  heavy_cands, P.heavy_e, P.u32c + 10);
-    if (o.work_queue)
+    if (o.work_queue) {
         k_emit_heavy<<<ctx->n_cu * o.emit_heavy_waves, 64, 0, st>>>(P.heavy_e, P.u32c + 10, (uint32_t)P.n, e,
-                                                                   P.cand_off, P.cs, d_out, P.counters + 1);
+                                                                   P.cand_off, P.cs, P.epart, P.counters + 1);
+        k_emit_combine<<<ctx->n_cu * 2, 256, 0, st>>>(P.heavy_e, P.u32c + 10, (uint32_t)P.n, P.epart, d_out);
+    }
     SWM_HIP_CHECK(hipGetLastError());
     if (ev) SWM_HIP_CHECK(hipEventRecord(ev[5], st));
     return SWM_OK;
