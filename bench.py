@@ -305,7 +305,7 @@ def main():
     # algorithmic bytes of the build: FASTA bytes + planes (N/4) + N mask (N/8)
     # + counts and offsets (2 x 4 B x 4^k) + positions and residuals (6 B each)
     build_bytes = cfg["n"] + cfg["n"] // 4 + cfg["n"] // 8 + 2 * 4 * 4 ** 12 + 6 * int(ix.positions.size)
-    index_build = {"kernels": "k_encode, k_keys, radix sort, k_runs, scan, k_split, k_tail", "ms": round(build_ms, 3),
+    index_build = {"kernels": "k_encode, k_pack_count, scan, k_pack_write, radix sort (64-bit keys), k_finish64, k_counts, k_tail", "ms": round(build_ms, 3),
                    "bound": "hbm", "algorithmic_bytes": build_bytes,
                    "achieved": round(build_bytes / (build_ms / 1e3) / 1e9, 1), "peak": HBM_PEAK_GBS,
                    "unit": "GB/s", "frac": round(build_bytes / (build_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
