@@ -2043,6 +2043,12 @@ __global__ void __launch_bounds__(64) k_emit_heavy(const uint32_t *__restrict__ 
         {
             const uint32_t rs = 2 * rd + strand;
             const uint32_t c0 = cand_off[rs], cend = cand_off[rs + 1];
+            /* the longest walks set the kernel's length and share their SIMD
+             * with up to 7 other waves: they issue first */
+            if (cend - c0 > 8u * GIANT_CANDS) __builtin_amdgcn_s_setprio(3);
+            else if (cend - c0 > 2u * GIANT_CANDS) __builtin_amdgcn_s_setprio(2);
+            else if (cend - c0 > GIANT_CANDS / 2u) __builtin_amdgcn_s_setprio(1);
+            else __builtin_amdgcn_s_setprio(0);
             /* carries between windows */
             uint32_t cur_b = 0xffffffffu, vcnt = 0, prevrec = 0;
             bool has_prev = false;
@@ -2137,60 +2143,81 @@ __global__ void __launch_bounds__(64) k_emit_heavy(const uint32_t *__restrict__ 
                     best_rec = rdlane(rec, f);
                 }
                 count += (uint32_t)__popcll(am);
-                /* lanes [lo, hi) join the open group */
-                auto add_range = [&](uint32_t lo, uint32_t hi) {
-                    if (!g_open || hi <= lo) return;
-                    const uint64_t rm = (hi >= 64 ? ~0ull : ((1ull << hi) - 1ull)) & ~((1ull << lo) - 1ull);
-                    g_n += (uint32_t)__popcll(am & rm);
-                    g_len += rdlane(tinc, hi - 1) - (lo ? rdlane(tinc, lo - 1) : 0u);
-                    const uint64_t pr = pm & rm;
-                    if (!g_seen && pr) {
-                        const uint32_t f = (uint32_t)__ffsll((unsigned long long)pr) - 1u;
+                /* Segments: lane 0 (the open group's continuation) and every
+                 * group start.  The lane of a segment start computes the
+                 * segment's written count, CIGAR bytes and first passing
+                 * lane, so the serial part reads a few scalars per group. */
+                const uint64_t segs = gm | 1ull;
+                const uint64_t above = segs & ~((2ull << lane) - 1ull);
+                const uint32_t send = above ? min((uint32_t)__ffsll((unsigned long long)above) - 1u, m) : m;
+                const uint64_t srm = (send >= 64u ? ~0ull : ((1ull << send) - 1ull)) & ~((1ull << lane) - 1ull);
+                const uint32_t t_end = (uint32_t)__shfl((int)tinc, (int)(send ? send - 1u : 0u), 64);
+                const uint32_t t_beg = tinc - (wr ? (info & 0xffffu) : 0u);
+                const uint32_t s_len = send > lane ? t_end - t_beg : 0u;
+                const uint64_t spr = pm & srm;
+                const uint32_t sf = spr ? (uint32_t)__ffsll((unsigned long long)spr) - 1u : 0u;
+                const uint64_t dm = __ballot(dup);
+                const uint32_t s_pack = (uint32_t)__popcll(am & srm) | (spr ? 0x100u : 0u) |
+                                        ((spr && ((dm >> sf) & 1ull)) ? 0x200u : 0u);
+                const uint32_t q_info = (uint32_t)__shfl((int)info, (int)sf, 64);
+                const uint32_t q_rec = (uint32_t)__shfl((int)rec, (int)sf, 64);
+                const uint32_t q_er = (uint32_t)__shfl((int)er, (int)sf, 64);
+                for (uint64_t rest = segs; rest; rest &= rest - 1ull) {
+                    const uint32_t p = (uint32_t)__ffsll((unsigned long long)rest) - 1u;
+                    if ((gm >> p) & 1ull) {
+                        close_group();
+                        g_open = true; g_seen = false; g_firstd = false; g_n = 0; g_len = 0;
+                    }
+                    if (!g_open) continue;
+                    const uint32_t pk = rdlane(s_pack, p);
+                    g_n += pk & 0xffu;
+                    g_len += rdlane(s_len, p);
+                    if (!g_seen && (pk & 0x100u)) {
                         g_seen = true;
-                        const uint64_t dm = __ballot(dup);
-                        if ((dm >> f) & 1ull) {
+                        if (pk & 0x200u) {
+                            const uint32_t f = rdlane(sf, p);
                             g_firstd = true;
-                            g_ftext = rdlane(info, f) & 0xffffu;
-                            g_frec = rdlane(rec, f);
+                            g_ftext = rdlane(q_info, p) & 0xffffu;
+                            g_frec = rdlane(q_rec, p);
                             g_fci = base + f;
-                            g_fkey = ((uint64_t)rdlane((uint32_t)er, f) << 40) | ((uint64_t)strand << 32) |
+                            g_fkey = ((uint64_t)rdlane(q_er, p) << 40) | ((uint64_t)strand << 32) |
                                      (uint64_t)(base + f - c0);
                         }
                     }
-                };
-                uint32_t from = 0;
-                for (uint64_t rest = gm; rest; rest &= rest - 1ull) {
-                    const uint32_t p = (uint32_t)__ffsll((unsigned long long)rest) - 1u;
-                    add_range(from, p);
-                    close_group();
-                    g_open = true; g_seen = false; g_firstd = false; g_n = 0; g_len = 0;
-                    from = p;
                 }
-                add_range(from, m);
                 /* carries */
                 const uint32_t last = m - 1u;
                 cur_b = rdlane(b, last);
                 vcnt = rdlane(v, last) + (uint32_t)((vm >> last) & 1ull);
                 if (pm) { has_prev = true; prevrec = rdlane(rec, 63u - (uint32_t)__clzll(pm)); }
             };
-            int32_t e0, e1, e2, e3;
-            uint32_t b0, b1, b2, b3, r0, r1, r2, r3, i0, i1, i2, i3;
+            int32_t e0, e1, e2, e3, e4, e5, e6, e7;
+            uint32_t b0, b1, b2, b3, b4, b5, b6, b7, r0, r1, r2, r3, r4, r5, r6, r7;
+            uint32_t i0, i1, i2, i3, i4, i5, i6, i7;
             fetch(c0, e0, b0, r0, i0);
             fetch(c0 + 64u, e1, b1, r1, i1);
             fetch(c0 + 128u, e2, b2, r2, i2);
             fetch(c0 + 192u, e3, b3, r3, i3);
+            fetch(c0 + 256u, e4, b4, r4, i4);
+            fetch(c0 + 320u, e5, b5, r5, i5);
+            fetch(c0 + 384u, e6, b6, r6, i6);
+            fetch(c0 + 448u, e7, b7, r7, i7);
 #define EMIT_SLOT(E, B, R, I, OFF)                                                  \
             if (base + (OFF) < cend) {                                              \
                 const int32_t fe = E;                                               \
                 const uint32_t fb = B, fr = R, fi = I;                              \
-                fetch(base + (OFF) + 256u, E, B, R, I);                             \
+                fetch(base + (OFF) + 512u, E, B, R, I);                             \
                 window(base + (OFF), fe, fb, fr, fi);                               \
             }
-            for (uint32_t base = c0; base < cend; base += 256u) {
+            for (uint32_t base = c0; base < cend; base += 512u) {
                 EMIT_SLOT(e0, b0, r0, i0, 0u)
                 EMIT_SLOT(e1, b1, r1, i1, 64u)
                 EMIT_SLOT(e2, b2, r2, i2, 128u)
                 EMIT_SLOT(e3, b3, r3, i3, 192u)
+                EMIT_SLOT(e4, b4, r4, i4, 256u)
+                EMIT_SLOT(e5, b5, r5, i5, 320u)
+                EMIT_SLOT(e6, b6, r6, i6, 384u)
+                EMIT_SLOT(e7, b7, r7, i7, 448u)
             }
 #undef EMIT_SLOT
             close_group();
