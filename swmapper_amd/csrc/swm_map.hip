@@ -789,6 +789,14 @@ __global__ void __launch_bounds__(64) k_plan_heavy(const uint2 *__restrict__ pl,
         __syncthreads();
         if (qi >= nq) break;
         const uint32_t r = qi < ng ? io.giant[qi] : io.heavy[qi - ng];
+        {
+            /* the longest replays set the kernel's length: they issue first */
+            const uint64_t np = io.surv_off[r + 1] - io.surv_off[r];
+            if (np > 16u * GIANT_POS) __builtin_amdgcn_s_setprio(3);
+            else if (np > 4u * GIANT_POS) __builtin_amdgcn_s_setprio(2);
+            else if (np > GIANT_POS) __builtin_amdgcn_s_setprio(1);
+            else __builtin_amdgcn_s_setprio(0);
+        }
         const uint32_t nsel = io.nsel[r] & ~NSEL_HEAVY;
         const bool act = lane < nsel;
         /* per 12-mer state, lane i = selected 12-mer i */

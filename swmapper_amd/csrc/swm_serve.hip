@@ -383,7 +383,7 @@ extern "C" int swm_autotune(swm_ctx_t *ctx, const char *reads, uint64_t stride, 
         {SWM_OPT_MYERS_PACKED, {1, 0}},
         {SWM_OPT_MYERS_BLOCK, {256, 128, 64}},
         {SWM_OPT_HEAVY_CANDS, {256, 64, 128, 512, 1024}},
-        {SWM_OPT_PLAN_HEAVY_WAVES, {16, 4, 8, 32}},
+        {SWM_OPT_PLAN_HEAVY_WAVES, {32, 8, 16}},
         {SWM_OPT_EMIT_HEAVY_WAVES, {32, 8, 16, 64}},
     };
     for (const Knob &kb : knobs) {
