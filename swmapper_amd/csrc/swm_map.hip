@@ -1280,6 +1280,13 @@ __global__ void __launch_bounds__(256) k_compact_heavy(const uint32_t *__restric
                                                        const uint32_t *__restrict__ nbatch,
                                                        const uint32_t *__restrict__ cand_off, CandSet c)
 {
+    /* up to 256 batches at a time: their descriptors loaded at once and
+     * their counts scanned in LDS, then the span copied flat (thread t on
+     * elements t, t + 256, ..., four loads in flight, its batch found by a
+     * pointer that only moves forward), so no batch waits on its own
+     * descriptor load */
+    __shared__ uint32_t s_x[256], s_pre[257], s_w[4];
+    const uint32_t tid = threadIdx.x;
     const uint32_t nh = *heavy_count;
     for (uint32_t h = blockIdx.x; h < nh; h += gridDim.x) {
         const uint32_t r = heavy[h];
@@ -1287,14 +1294,41 @@ __global__ void __launch_bounds__(256) k_compact_heavy(const uint32_t *__restric
         const uint2 *bd = bdesc + bd_off[r];
         uint32_t o = cand_off[r];
         const uint32_t nb = nbatch[r];
-        for (uint32_t b = 0; b < nb; b++) {
-            const uint2 d = bd[b];
-            for (uint32_t j = threadIdx.x; j < d.y; j += blockDim.x) {
-                c.val[o + j] = s[d.x + j];
-                c.rs[o + j] = r;
-                c.bi[o + j] = (b << 12) | j;
+        for (uint32_t b0 = 0; b0 < nb; b0 += 256) {
+            const uint32_t nbc = min(256u, nb - b0);
+            const uint2 d = tid < nbc ? bd[b0 + tid] : make_uint2(0u, 0u);
+            uint32_t total;
+            const uint32_t incl = block_incl_scan<uint32_t, 256>(d.y, s_w, &total);
+            s_x[tid] = d.x;
+            s_pre[tid] = incl - d.y;
+            if (tid == 0) s_pre[nbc] = total;
+            __syncthreads();
+            uint32_t k = 0;
+            for (uint32_t e0 = tid; e0 < total; e0 += 1024) {
+                /* four elements per thread, their loads issued together */
+                uint32_t kq[4], jq[4], vq[4];
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const uint32_t e = e0 + 256u * (uint32_t)q;
+                    if (e < total) {
+                        while (s_pre[k + 1] <= e) k++;
+                        kq[q] = k;
+                        jq[q] = e - s_pre[k];
+                        vq[q] = s[s_x[k] + jq[q]];
+                    }
+                }
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const uint32_t e = e0 + 256u * (uint32_t)q;
+                    if (e < total) {
+                        c.val[o + e] = vq[q];
+                        c.rs[o + e] = r;
+                        c.bi[o + e] = ((b0 + kq[q]) << 12) | jq[q];
+                    }
+                }
             }
-            o += d.y;
+            o += total;
+            __syncthreads();
         }
     }
 }
@@ -2745,7 +2779,7 @@ static int enq_c(const swm_ctx_t *ctx, Pipe &P, const uint32_t *d_lens, uint32_t
                                                       P.pio.nbatch, P.cand_off, P.cs, heavy_cands, P.heavy_c,
                                                       P.u32c + 5);
     if (o.work_queue)
-        k_compact_heavy<<<ctx->n_cu * 4, 256, 0, st>>>(P.heavy_c, P.u32c + 5, P.surv_off, P.pio.surv, P.bd_off,
+        k_compact_heavy<<<ctx->n_cu * 8, 256, 0, st>>>(P.heavy_c, P.u32c + 5, P.surv_off, P.pio.surv, P.bd_off,
                                                        P.pio.bdesc, P.pio.nbatch, P.cand_off, P.cs);
     SWM_HIP_CHECK(hipGetLastError());
     if (ev) SWM_HIP_CHECK(hipEventRecord(ev[3], st));
