@@ -35,6 +35,7 @@ def tandem():
     ref = orc.OracleRef.from_index(orc.OracleIndex(g, 12))
     exp = {e: ref.map(r, l, e) for e in (5, 8)}
     assert exp[8][0]["count"].max() > 600          # several output blocks per strand
+    assert (l == 150).all()
     return gi, r, l, exp
 
 
@@ -67,7 +68,10 @@ def test_forms_match_oracle(tandem, e, si):
     for rep in range(3 if opts.get(swm.OPT_GRAPH) else 1):
         got = m.map(r, l, e)
         _assert_same(got, ref, f"rep {rep}")
-        assert m.stats()["n_verifications"] == nv
+        st = m.stats()
+        assert st["n_verifications"] == nv
+        # every read has 150 bases: one band step per base per verification
+        assert st["n_band_steps"] == 150 * nv, (st["n_band_steps"], nv)
         assert m.stats()["n_records"] == int(ref["count"].sum())
     m.close()
 
