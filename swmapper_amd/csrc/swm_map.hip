@@ -782,8 +782,10 @@ __global__ void __launch_bounds__(64) k_plan_heavy(const uint2 *__restrict__ pl,
         nhq = 0;
     };
 
-    for (;;) {
-        if (lane == 0) sv[0] = atomicAdd(io.heavy_head, 1u);
+    /* a wave's first item is its workgroup index, later ones come from the
+     * queue head (offset by the grid): no atomic for waves that find no work */
+    for (bool first = true;; first = false) {
+        if (lane == 0) sv[0] = first ? blockIdx.x : gridDim.x + atomicAdd(io.heavy_head, 1u);
         __syncthreads();
         const uint32_t qi = sv[0];
         __syncthreads();
@@ -1667,8 +1669,9 @@ __global__ void __launch_bounds__(256) k_myers2(const uint32_t *__restrict__ nca
 }
 
 /* Per candidate: the Hamming check of passing lanes, the traceback list, the
- * count of rejected windows and their read lengths (the band steps are read
- * length x candidates per read, added by k_emit, less these).  A
+ * count of rejected windows; the band steps as read length x candidates per
+ * read less the rejected windows' lengths (only err is read for a window
+ * that is not rejected and did not pass).  A
  * fixed grid loops over the candidates (block-uniform trip count); counts
  * stay in registers and the traceback list in an LDS buffer flushed to the
  * global list when nearly full, so a workgroup issues a handful of global
@@ -1680,7 +1683,8 @@ __global__ void __launch_bounds__(256) k_hamming(const uint32_t *__restrict__ nc
                                                  const uint32_t *__restrict__ words, const uint32_t *__restrict__ b2,
                                                  uint64_t ref_length, uint32_t e, CandSet c,
                                                  uint32_t *__restrict__ trace, uint32_t *__restrict__ trace_count,
-                                                 uint32_t *__restrict__ n_rejected, unsigned long long *n_steps)
+                                                 uint32_t *__restrict__ n_rejected, unsigned long long *n_steps,
+                                                 uint32_t nreads, const uint32_t *__restrict__ cand_off)
 {
     __shared__ uint32_t s_list[TRACE_BUF];
     __shared__ uint32_t s_n, s_base, s_rej;
@@ -1711,7 +1715,7 @@ __global__ void __launch_bounds__(256) k_hamming(const uint32_t *__restrict__ nc
             const int32_t er = c.err[i];
             if (er == -2) {
                 my_rej++;                          /* pos_start rejected the window: not a lane */
-                my_steps += lens[c.rs[i] >> 1];    /* taken off k_emit's per-read total */
+                my_steps -= lens[c.rs[i] >> 1];    /* taken off the per-read total below */
             } else if (er <= (int32_t)e) {
                 to_trace = hamming_one(i, er, lens, words, b2, ref_length, e, c);
             }
@@ -1725,6 +1729,10 @@ __global__ void __launch_bounds__(256) k_hamming(const uint32_t *__restrict__ nc
         if (s_n > TRACE_BUF - 256) flush();
     }
     flush();
+    /* band steps: read length x the read's candidates, less the rejected
+     * windows above (u64 arithmetic mod 2^64, exact in the sum) */
+    for (uint32_t rd = blockIdx.x * blockDim.x + tid; rd < nreads; rd += gridDim.x * blockDim.x)
+        my_steps += (uint64_t)lens[rd] * (cand_off[2 * rd + 2] - cand_off[2 * rd]);
     /* counts: wave sums, then one global atomic each per workgroup */
 #pragma unroll
     for (int d = 32; d > 0; d >>= 1) {
@@ -1739,7 +1747,7 @@ __global__ void __launch_bounds__(256) k_hamming(const uint32_t *__restrict__ nc
     }
     __syncthreads();
     if (tid == 0) {
-        if (s_steps) atomicAdd(n_steps, 0ull - s_steps);
+        if (s_steps) atomicAdd(n_steps, s_steps);
         if (s_rej) atomicAdd(n_rejected, s_rej);
     }
 }
@@ -1998,11 +2006,9 @@ __device__ __forceinline__ void add_records(uint32_t count, unsigned long long *
 __global__ void __launch_bounds__(128) k_emit(uint32_t nreads, uint32_t e, const uint32_t *__restrict__ cand_off,
                                               CandSet c, swm_read_result_t *__restrict__ out,
                                               unsigned long long *n_records, uint32_t heavy_cands,
-                                              uint32_t *__restrict__ heavy, uint32_t *__restrict__ heavy_count,
-                                              const uint32_t *__restrict__ lens, unsigned long long *n_steps)
+                                              uint32_t *__restrict__ heavy, uint32_t *__restrict__ heavy_count)
 {
     __shared__ uint32_t s_tmp[2], s_sum;
-    __shared__ unsigned long long s_steps;
     if (*c.ovf) return;
     uint32_t rd = blockIdx.x * blockDim.x + threadIdx.x;
     const bool live = rd < nreads;
@@ -2037,20 +2043,6 @@ __global__ void __launch_bounds__(128) k_emit(uint32_t nreads, uint32_t e, const
         out[rd] = em.result();
     }
     add_records(live && !big ? em.count : 0u, n_records, &s_sum);
-    /* band steps: read length x the read's candidates (k_hamming takes off
-     * the windows pos_start rejected) */
-    uint64_t st = live ? (uint64_t)lens[rd] * nc : 0ull;
-#pragma unroll
-    for (int d = 32; d > 0; d >>= 1) {
-        const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)st, d, 64);
-        const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(st >> 32), d, 64);
-        st += ((uint64_t)hi << 32) | lo;
-    }
-    if (threadIdx.x == 0) s_steps = 0;
-    __syncthreads();
-    if (__lane_id() == 0 && st) atomicAdd(&s_steps, (unsigned long long)st);
-    __syncthreads();
-    if (threadIdx.x == 0 && s_steps) atomicAdd(n_steps, s_steps);
 }
 
 /* One wave per read with many candidates, 64 candidates per step in
@@ -2815,12 +2807,12 @@ static int enq_c(const swm_ctx_t *ctx, Pipe &P, const uint32_t *d_lens, uint32_t
     k_hamming<<<ham_grid, 256, 0, st>>>(ncand, d_lens, P.so.words,
                                                                           ctx->d_base_2bits, ctx->ref_length, e, P.cs,
                                                                           P.trace, P.u32c + 1, P.u32c + 2,
-                                                                          P.counters + 2);
+                                                                          P.counters + 2, (uint32_t)P.n, P.cand_off);
     /* persistent: 8 waves per SIMD, the traceback chains are latency-bound */
     (P.cs.runs ? k_trace<true> : k_trace<false>)<<<ctx->n_cu * 32, 64, 0, st>>>(P.trace, P.u32c + 1, d_lens, P.so.words, ctx->d_base_2bits, ctx->ref_length, e,
                                           P.cs);
     k_emit<<<blocks_for(P.n, 128), 128, 0, st>>>((uint32_t)P.n, e, P.cand_off, P.cs, d_out, P.counters + 1,
-                                                 heavy_cands, P.heavy_e, P.u32c + 10, d_lens, P.counters + 2);
+                                                 heavy_cands, P.heavy_e, P.u32c + 10);
     if (o.work_queue) {
         k_emit_heavy<<<ctx->n_cu * o.emit_heavy_waves, 64, 0, st>>>(P.heavy_e, P.u32c + 10, (uint32_t)P.n, e,
                                                                    P.cand_off, P.cs, P.epart, P.counters + 1);
