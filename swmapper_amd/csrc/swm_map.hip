@@ -1994,18 +1994,21 @@ struct Emit {
     }
 };
 
-__device__ __forceinline__ void add_records(uint32_t count, unsigned long long *n_records, uint32_t *s_sum)
+__device__ __forceinline__ void record_partial(uint32_t count, uint32_t *part, uint32_t *s_sum)
 {
+    /* the workgroup's record count to its own slot: k_emit_combine adds the
+     * slots up (one atomic per workgroup on one address cost 58 us of a
+     * 1M-read step) */
     if (threadIdx.x == 0) *s_sum = 0;
     __syncthreads();
     if (count) atomicAdd(s_sum, count);
     __syncthreads();
-    if (threadIdx.x == 0 && *s_sum) atomicAdd(n_records, (unsigned long long)*s_sum);
+    if (threadIdx.x == 0) part[blockIdx.x] = *s_sum;
 }
 
 __global__ void __launch_bounds__(128) k_emit(uint32_t nreads, uint32_t e, const uint32_t *__restrict__ cand_off,
                                               CandSet c, swm_read_result_t *__restrict__ out,
-                                              unsigned long long *n_records, uint32_t heavy_cands,
+                                              uint32_t *__restrict__ rpart, uint32_t heavy_cands,
                                               uint32_t *__restrict__ heavy, uint32_t *__restrict__ heavy_count)
 {
     __shared__ uint32_t s_tmp[2], s_sum;
@@ -2042,7 +2045,7 @@ __global__ void __launch_bounds__(128) k_emit(uint32_t nreads, uint32_t e, const
         }
         out[rd] = em.result();
     }
-    add_records(live && !big ? em.count : 0u, n_records, &s_sum);
+    record_partial(live && !big ? em.count : 0u, rpart, &s_sum);
 }
 
 /* One wave per read with many candidates, 64 candidates per step in
@@ -2285,12 +2288,25 @@ __global__ void __launch_bounds__(64) k_emit_heavy(const uint32_t *__restrict__ 
 }
 
 /* A heavy read's result from its strands' parts: the counts add up and the
- * best record is the smaller key (distance, strand, order). */
+ * best record is the smaller key (distance, strand, order).  Workgroups
+ * 0..7 also add up k_emit's per-workgroup record counts. */
 __global__ void __launch_bounds__(256) k_emit_combine(const uint32_t *__restrict__ heavy,
                                                      const uint32_t *__restrict__ heavy_count, uint32_t nreads,
                                                      const EmitPart *__restrict__ part,
-                                                     swm_read_result_t *__restrict__ out)
+                                                     swm_read_result_t *__restrict__ out,
+                                                     const uint32_t *__restrict__ rpart, uint32_t nrpart,
+                                                     unsigned long long *n_records)
 {
+    if (blockIdx.x < 8) {
+        __shared__ unsigned long long s_sum;
+        if (threadIdx.x == 0) s_sum = 0;
+        __syncthreads();
+        unsigned long long t = 0;
+        for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < nrpart; k += 8 * blockDim.x) t += rpart[k];
+        if (t) atomicAdd(&s_sum, t);
+        __syncthreads();
+        if (threadIdx.x == 0 && s_sum) atomicAdd(n_records, s_sum);
+    }
     const uint32_t ng = heavy_count[0], nh = ng + heavy_count[1];
     for (uint32_t h = blockIdx.x * blockDim.x + threadIdx.x; h < nh; h += gridDim.x * blockDim.x) {
         const uint32_t rd = h < ng ? heavy[h] : heavy[nreads + (h - ng)];
@@ -2488,7 +2504,7 @@ struct swm_ws {
     struct Buf { void *p = nullptr; size_t cap = 0; };
     enum { WORDS, SEL, NSEL, NPOS, NCHUNK, SURV_OFF, BD_OFF, SURV, BDESC, NBATCH, NUNIQ, CAND_OFF, C_VAL, C_RS, C_BI,
            C_ERR, C_LOC, C_REC, C_INFO, COUNTERS, BIG, TRACE, HEAVY, HEAVY_C, HEAVY_E, SCAN_TMP, IN_READS, IN_LENS,
-           OUT, HIST, R_WR, R_TB, R_RUNS, R_IDX, R_TOFF, R_SCAN, R_TOT, R_OUT, R_TEXT, CDESC, HBATCH, GIANT, EPART, NBUF };
+           OUT, HIST, R_WR, R_TB, R_RUNS, R_IDX, R_TOFF, R_SCAN, R_TOT, R_OUT, R_TEXT, CDESC, HBATCH, GIANT, EPART, RPART, NBUF };
     Buf b[NBUF];
     uint64_t gen = 1;
     ~swm_ws() { for (auto &x : b) if (x.p) (void)hipFree(x.p); }
@@ -2644,6 +2660,7 @@ struct Pipe {
     uint64_t *surv_off = nullptr;
     uint32_t *bd_off = nullptr, *cand_off = nullptr, *trace = nullptr, *heavy_c = nullptr, *heavy_e = nullptr;
     EmitPart *epart = nullptr;
+    uint32_t *rpart = nullptr;
     unsigned long long *counters = nullptr;
     uint32_t *u32c = nullptr;
     uint64_t *scan_tmp = nullptr;
@@ -2670,6 +2687,7 @@ static int pipe_alloc_a(swm_ws *ws, uint64_t n, uint32_t seln, Pipe &P)
         (rc = ws->get(swm_ws::HEAVY, &P.pio.heavy, nrs)) || (rc = ws->get(swm_ws::GIANT, &P.pio.giant, nrs)) || (rc = ws->get(swm_ws::CAND_OFF, &P.cand_off, nrs + 1)) ||
         (rc = ws->get(swm_ws::HEAVY_C, &P.heavy_c, nrs)) || (rc = ws->get(swm_ws::HEAVY_E, &P.heavy_e, 2 * n)) ||
         (rc = ws->get(swm_ws::EPART, &P.epart, 2 * n)) ||
+        (rc = ws->get(swm_ws::RPART, &P.rpart, (uint64_t)blocks_for(n, 128))) ||
         (rc = ws->get(swm_ws::SCAN_TMP, &P.scan_tmp, 2 * (uint64_t)blocks_for(nrs + 1, SCAN_TILE))))
         return rc;
     P.so.counters = P.counters;
@@ -2811,13 +2829,13 @@ static int enq_c(const swm_ctx_t *ctx, Pipe &P, const uint32_t *d_lens, uint32_t
     /* persistent: 8 waves per SIMD, the traceback chains are latency-bound */
     (P.cs.runs ? k_trace<true> : k_trace<false>)<<<ctx->n_cu * 32, 64, 0, st>>>(P.trace, P.u32c + 1, d_lens, P.so.words, ctx->d_base_2bits, ctx->ref_length, e,
                                           P.cs);
-    k_emit<<<blocks_for(P.n, 128), 128, 0, st>>>((uint32_t)P.n, e, P.cand_off, P.cs, d_out, P.counters + 1,
+    k_emit<<<blocks_for(P.n, 128), 128, 0, st>>>((uint32_t)P.n, e, P.cand_off, P.cs, d_out, P.rpart,
                                                  heavy_cands, P.heavy_e, P.u32c + 10);
-    if (o.work_queue) {
+    if (o.work_queue)
         k_emit_heavy<<<ctx->n_cu * o.emit_heavy_waves, 64, 0, st>>>(P.heavy_e, P.u32c + 10, (uint32_t)P.n, e,
                                                                    P.cand_off, P.cs, P.epart, P.counters + 1);
-        k_emit_combine<<<ctx->n_cu * 2, 256, 0, st>>>(P.heavy_e, P.u32c + 10, (uint32_t)P.n, P.epart, d_out);
-    }
+    k_emit_combine<<<ctx->n_cu * 2, 256, 0, st>>>(P.heavy_e, P.u32c + 10, (uint32_t)P.n, P.epart, d_out, P.rpart,
+                                                  blocks_for(P.n, 128), P.counters + 1);
     SWM_HIP_CHECK(hipGetLastError());
     if (ev) SWM_HIP_CHECK(hipEventRecord(ev[5], st));
     return SWM_OK;
