@@ -1671,19 +1671,20 @@ __global__ void __launch_bounds__(256) k_myers2(const uint32_t *__restrict__ nca
 /* Per candidate: the Hamming check of passing lanes, the traceback list, the
  * count of rejected windows; the band steps as read length x candidates per
  * read less the rejected windows' lengths (only err is read for a window
- * that is not rejected and did not pass).  A
+ * that is not rejected and did not pass), per workgroup.  A
  * fixed grid loops over the candidates (block-uniform trip count); counts
  * stay in registers and the traceback list in an LDS buffer flushed to the
  * global list when nearly full, so a workgroup issues a handful of global
  * atomics in its lifetime instead of a few per 256 candidates (same-address
  * atomics serialise at one L2 channel). */
 #define TRACE_BUF 2048
+#define HAM_GRID_MAX 4096u   /* k_hamming workgroups at most (their count slots) */
 
 __global__ void __launch_bounds__(256) k_hamming(const uint32_t *__restrict__ ncand_p, const uint32_t *__restrict__ lens,
                                                  const uint32_t *__restrict__ words, const uint32_t *__restrict__ b2,
                                                  uint64_t ref_length, uint32_t e, CandSet c,
                                                  uint32_t *__restrict__ trace, uint32_t *__restrict__ trace_count,
-                                                 uint32_t *__restrict__ n_rejected, unsigned long long *n_steps,
+                                                 unsigned long long *__restrict__ hpart,
                                                  uint32_t nreads, const uint32_t *__restrict__ cand_off)
 {
     __shared__ uint32_t s_list[TRACE_BUF];
@@ -1746,9 +1747,10 @@ __global__ void __launch_bounds__(256) k_hamming(const uint32_t *__restrict__ nc
         if (my_rej) atomicAdd(&s_rej, my_rej);
     }
     __syncthreads();
+    /* the workgroup's counts to slots of its own, added up by k_emit_combine */
     if (tid == 0) {
-        if (s_steps) atomicAdd(n_steps, s_steps);
-        if (s_rej) atomicAdd(n_rejected, s_rej);
+        hpart[2 * blockIdx.x] = s_steps;
+        hpart[2 * blockIdx.x + 1] = s_rej;
     }
 }
 
@@ -2289,23 +2291,36 @@ __global__ void __launch_bounds__(64) k_emit_heavy(const uint32_t *__restrict__ 
 
 /* A heavy read's result from its strands' parts: the counts add up and the
  * best record is the smaller key (distance, strand, order).  Workgroups
- * 0..7 also add up k_emit's per-workgroup record counts. */
+ * 0..7 also add up k_emit's per-workgroup record counts and k_hamming's
+ * per-workgroup band steps and rejected windows. */
 __global__ void __launch_bounds__(256) k_emit_combine(const uint32_t *__restrict__ heavy,
                                                      const uint32_t *__restrict__ heavy_count, uint32_t nreads,
                                                      const EmitPart *__restrict__ part,
                                                      swm_read_result_t *__restrict__ out,
                                                      const uint32_t *__restrict__ rpart, uint32_t nrpart,
-                                                     unsigned long long *n_records)
+                                                     unsigned long long *n_records,
+                                                     const unsigned long long *__restrict__ hpart, uint32_t nhpart,
+                                                     unsigned long long *n_steps, uint32_t *n_rejected)
 {
     if (blockIdx.x < 8) {
-        __shared__ unsigned long long s_sum;
-        if (threadIdx.x == 0) s_sum = 0;
+        __shared__ unsigned long long s_sum[3];
+        if (threadIdx.x < 3) s_sum[threadIdx.x] = 0;
         __syncthreads();
-        unsigned long long t = 0;
+        unsigned long long t = 0, ts = 0, tr = 0;
         for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < nrpart; k += 8 * blockDim.x) t += rpart[k];
-        if (t) atomicAdd(&s_sum, t);
+        for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < nhpart; k += 8 * blockDim.x) {
+            ts += hpart[2 * k];
+            tr += hpart[2 * k + 1];
+        }
+        if (t) atomicAdd(&s_sum[0], t);
+        if (ts) atomicAdd(&s_sum[1], ts);
+        if (tr) atomicAdd(&s_sum[2], tr);
         __syncthreads();
-        if (threadIdx.x == 0 && s_sum) atomicAdd(n_records, s_sum);
+        if (threadIdx.x == 0) {
+            if (s_sum[0]) atomicAdd(n_records, s_sum[0]);
+            if (s_sum[1]) atomicAdd(n_steps, s_sum[1]);
+            if (s_sum[2]) atomicAdd(n_rejected, (uint32_t)s_sum[2]);
+        }
     }
     const uint32_t ng = heavy_count[0], nh = ng + heavy_count[1];
     for (uint32_t h = blockIdx.x * blockDim.x + threadIdx.x; h < nh; h += gridDim.x * blockDim.x) {
@@ -2504,7 +2519,7 @@ struct swm_ws {
     struct Buf { void *p = nullptr; size_t cap = 0; };
     enum { WORDS, SEL, NSEL, NPOS, NCHUNK, SURV_OFF, BD_OFF, SURV, BDESC, NBATCH, NUNIQ, CAND_OFF, C_VAL, C_RS, C_BI,
            C_ERR, C_LOC, C_REC, C_INFO, COUNTERS, BIG, TRACE, HEAVY, HEAVY_C, HEAVY_E, SCAN_TMP, IN_READS, IN_LENS,
-           OUT, HIST, R_WR, R_TB, R_RUNS, R_IDX, R_TOFF, R_SCAN, R_TOT, R_OUT, R_TEXT, CDESC, HBATCH, GIANT, EPART, RPART, NBUF };
+           OUT, HIST, R_WR, R_TB, R_RUNS, R_IDX, R_TOFF, R_SCAN, R_TOT, R_OUT, R_TEXT, CDESC, HBATCH, GIANT, EPART, RPART, HPART, NBUF };
     Buf b[NBUF];
     uint64_t gen = 1;
     ~swm_ws() { for (auto &x : b) if (x.p) (void)hipFree(x.p); }
@@ -2661,6 +2676,7 @@ struct Pipe {
     uint32_t *bd_off = nullptr, *cand_off = nullptr, *trace = nullptr, *heavy_c = nullptr, *heavy_e = nullptr;
     EmitPart *epart = nullptr;
     uint32_t *rpart = nullptr;
+    unsigned long long *hpart = nullptr;
     unsigned long long *counters = nullptr;
     uint32_t *u32c = nullptr;
     uint64_t *scan_tmp = nullptr;
@@ -2688,6 +2704,7 @@ static int pipe_alloc_a(swm_ws *ws, uint64_t n, uint32_t seln, Pipe &P)
         (rc = ws->get(swm_ws::HEAVY_C, &P.heavy_c, nrs)) || (rc = ws->get(swm_ws::HEAVY_E, &P.heavy_e, 2 * n)) ||
         (rc = ws->get(swm_ws::EPART, &P.epart, 2 * n)) ||
         (rc = ws->get(swm_ws::RPART, &P.rpart, (uint64_t)blocks_for(n, 128))) ||
+        (rc = ws->get(swm_ws::HPART, &P.hpart, 2 * (uint64_t)HAM_GRID_MAX)) ||
         (rc = ws->get(swm_ws::SCAN_TMP, &P.scan_tmp, 2 * (uint64_t)blocks_for(nrs + 1, SCAN_TILE))))
         return rc;
     P.so.counters = P.counters;
@@ -2821,11 +2838,12 @@ static int enq_c(const swm_ctx_t *ctx, Pipe &P, const uint32_t *d_lens, uint32_t
                                                                           P.cs);
     SWM_HIP_CHECK(hipGetLastError());
     if (ev) SWM_HIP_CHECK(hipEventRecord(ev[4], st));
-    const uint32_t ham_grid = std::min<uint32_t>(blocks_for(grid_cand ? grid_cand : 1, 256), (uint32_t)ctx->n_cu * 8);
+    const uint32_t ham_grid = std::min<uint32_t>(std::min<uint32_t>(blocks_for(grid_cand ? grid_cand : 1, 256),
+                                                                     (uint32_t)ctx->n_cu * 8), HAM_GRID_MAX);
     k_hamming<<<ham_grid, 256, 0, st>>>(ncand, d_lens, P.so.words,
                                                                           ctx->d_base_2bits, ctx->ref_length, e, P.cs,
-                                                                          P.trace, P.u32c + 1, P.u32c + 2,
-                                                                          P.counters + 2, (uint32_t)P.n, P.cand_off);
+                                                                          P.trace, P.u32c + 1, P.hpart,
+                                                                          (uint32_t)P.n, P.cand_off);
     /* persistent: 8 waves per SIMD, the traceback chains are latency-bound */
     (P.cs.runs ? k_trace<true> : k_trace<false>)<<<ctx->n_cu * 32, 64, 0, st>>>(P.trace, P.u32c + 1, d_lens, P.so.words, ctx->d_base_2bits, ctx->ref_length, e,
                                           P.cs);
@@ -2835,7 +2853,8 @@ static int enq_c(const swm_ctx_t *ctx, Pipe &P, const uint32_t *d_lens, uint32_t
         k_emit_heavy<<<ctx->n_cu * o.emit_heavy_waves, 64, 0, st>>>(P.heavy_e, P.u32c + 10, (uint32_t)P.n, e,
                                                                    P.cand_off, P.cs, P.epart, P.counters + 1);
     k_emit_combine<<<ctx->n_cu * 2, 256, 0, st>>>(P.heavy_e, P.u32c + 10, (uint32_t)P.n, P.epart, d_out, P.rpart,
-                                                  blocks_for(P.n, 128), P.counters + 1);
+                                                  blocks_for(P.n, 128), P.counters + 1, P.hpart, ham_grid,
+                                                  P.counters + 2, P.u32c + 2);
     SWM_HIP_CHECK(hipGetLastError());
     if (ev) SWM_HIP_CHECK(hipEventRecord(ev[5], st));
     return SWM_OK;
