@@ -1260,14 +1260,25 @@ __global__ void __launch_bounds__(128) k_compact(uint32_t nrs, const uint64_t *_
     const uint32_t *s = surv + surv_off[r];
     const uint2 *bd = bdesc + bd_off[r];
     uint32_t o = cand_off[r];
-    uint32_t nb = nbatch[r];
+    const uint32_t nb = nbatch[r];
     for (uint32_t b = 0; b < nb; b++) {
-        uint2 d = bd[b];
-        for (uint32_t j = 0; j < d.y; j++) {
-            c.val[o] = s[d.x + j];
-            c.rs[o] = r;
-            c.bi[o] = (b << 12) | j;
-            o++;
+        const uint2 d = bd[b];
+        /* four survivors loaded before any store (the stores could alias
+         * the loads as far as the compiler knows, which would otherwise
+         * put a memory round trip between candidates) */
+        for (uint32_t j0 = 0; j0 < d.y; j0 += 4) {
+            uint32_t v[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) v[q] = j0 + q < d.y ? s[d.x + j0 + q] : 0u;
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                if (j0 + q < d.y) {
+                    c.val[o + q] = v[q];
+                    c.rs[o + q] = r;
+                    c.bi[o + q] = (b << 12) | (j0 + q);
+                }
+            }
+            o += min(4u, d.y - j0);
         }
     }
 }
