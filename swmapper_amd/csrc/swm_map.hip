@@ -967,6 +967,9 @@ __global__ void __launch_bounds__(64) k_plan_heavy(const uint2 *__restrict__ pl,
  * in entry order at the batch's own offset; then the sorted unique list of a
  * batch of <= 64 survivors (block_heap_sort's result), larger batches to
  * k_bigsort.  The item's survivor count is the sum over its batches. */
+static_assert(SWM_CHUNK == 32, "k_plan_fetch maps a chunk onto half a wave");
+static_assert(SWM_BATCH_POS <= 2048, "k_bigsort sorts at most 64 x 32 values in a wave");
+
 __global__ void __launch_bounds__(64) k_plan_fetch(const uint2 *__restrict__ pl, PlanIO io)
 {
     __shared__ uint32_t sg[SWM_MAX_BLOCKS], spre[SWM_MAX_BLOCKS + 1], sseed[SWM_MAX_BLOCKS];
