@@ -194,7 +194,7 @@ int swm_sam_close(swm_sam_t *w);
 
 #define SWM_OPT_WORK_QUEUE 1        /* 1 (default): heavy items go to device work queues; 0: plain form */
 #define SWM_OPT_GRAPH 2             /* 1: fixed-capacity pipeline replayed as a hipGraph; 0 (default): synced */
-#define SWM_OPT_HEAVY_CANDS 3       /* candidates above which an item is heavy (1..65535, default 256) */
+#define SWM_OPT_HEAVY_CANDS 3       /* candidates above which an item is heavy (1..65535, default 64) */
 #define SWM_OPT_MYERS_BLOCK 4       /* verification workgroup size: 64, 128 or 256 (default) */
 #define SWM_OPT_PLAN_HEAVY_WAVES 5  /* persistent plan waves per CU (1..32, default 32) */
 #define SWM_OPT_EMIT_HEAVY_WAVES 6  /* cooperative emission waves per CU (1..64, default 32) */

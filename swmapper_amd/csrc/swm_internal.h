@@ -47,7 +47,7 @@ struct swm_gate {
 struct swm_opts {
     uint32_t work_queue = 1;        /* 0: the plain one-thread-per-item kernels for every item */
     uint32_t graph = 0;             /* 1: fixed-capacity pipeline replayed as a hipGraph */
-    uint32_t heavy_cands = 256;     /* items with more candidates go to the cooperative kernels */
+    uint32_t heavy_cands = 64;      /* items with more candidates go to the cooperative kernels */
     uint32_t myers_block = 256;     /* workgroup size of the verification kernel */
     uint32_t plan_heavy_waves = 32; /* persistent waves per CU draining the heavy plan queue */
     uint32_t emit_heavy_waves = 32; /* waves per CU of the cooperative emission */

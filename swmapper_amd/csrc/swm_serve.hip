@@ -382,7 +382,7 @@ extern "C" int swm_autotune(swm_ctx_t *ctx, const char *reads, uint64_t stride, 
         {SWM_OPT_WORK_QUEUE, {1, 0}},
         {SWM_OPT_MYERS_PACKED, {1, 0}},
         {SWM_OPT_MYERS_BLOCK, {256, 128, 64}},
-        {SWM_OPT_HEAVY_CANDS, {256, 64, 128, 512, 1024}},
+        {SWM_OPT_HEAVY_CANDS, {64, 32, 128, 256, 512}},
         {SWM_OPT_PLAN_HEAVY_WAVES, {32, 8, 16}},
         {SWM_OPT_EMIT_HEAVY_WAVES, {32, 8, 16, 64}},
     };
