@@ -629,26 +629,12 @@ __device__ __forceinline__ uint32_t sel_gt(uint32_t a, uint32_t b, uint32_t x, u
  * one line costs one request -- one thread per item issued a request per
  * entry (and duplicates of the last one), 2.7 per line.  Waves take items
  * one at a time (grid-stride), so a small batch is spread over all waves
- * instead of 64 items in a row per wave; k_plan_classify appends the heavy
- * items beforehand, one atomic per list per workgroup.  Results as k_plan:
+ * instead of 64 items in a row per wave; the scan of the position counts
+ * appends the heavy items beforehand, one atomic per list per workgroup.
+ * Results as k_plan:
  * the 15-mer filter and read-start adjustment in entry order with the first
  * entry dropped (Q7), then the sorted unique list of the one batch (<= 64
  * survivors sorted in the wave by rank, larger ones left to k_bigsort). */
-/* Heavy items (a selected 12-mer with more than one chunk) to the seed-heap
- * kernels' lists, the largest ones to their own list. */
-__global__ void __launch_bounds__(256) k_plan_classify(uint32_t nrs, PlanIO io)
-{
-    __shared__ uint32_t s_tmp[2];
-    if (*io.ovf) return;
-    uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool live = r < nrs;
-    if (!live) r = 0;
-    const bool heavy = live && (io.nsel[r] & NSEL_HEAVY) != 0;
-    const bool giant = heavy && io.surv_off[r + 1] - io.surv_off[r] > GIANT_POS;
-    block_append(heavy && !giant, r, io.heavy, io.heavy_count, s_tmp);
-    block_append(giant, r, io.giant, io.giant_count, s_tmp);
-}
-
 __global__ void __launch_bounds__(64) k_plan_wave(uint32_t nrs, const uint2 *__restrict__ pl, PlanIO io)
 {
     __shared__ uint32_t s_off[64], s_pre[64], s_meta[64], s_surv[64];
@@ -763,6 +749,9 @@ __global__ void __launch_bounds__(64) k_plan_heavy(const uint2 *__restrict__ pl,
 {
     __shared__ uint32_t skey[64 * HK];           /* key of chunk c of 12-mer i at [i * HK + c % HK] */
     __shared__ uint32_t sv[1];
+    /* the heavy lists are filled whatever the capacities: on overflow the
+     * positions and chunks they would write lie beyond their buffers */
+    if (*io.ovf) return;
     const uint32_t lane = threadIdx.x;
     const uint64_t below = (1ull << lane) - 1ull;
     const uint32_t ng = *io.giant_count, nq = ng + *io.heavy_count;
@@ -974,6 +963,7 @@ __global__ void __launch_bounds__(64) k_plan_fetch(const uint2 *__restrict__ pl,
 {
     __shared__ uint32_t sg[SWM_MAX_BLOCKS], spre[SWM_MAX_BLOCKS + 1], sseed[SWM_MAX_BLOCKS];
     __shared__ uint32_t smeta[64];
+    if (*io.ovf) return;
     const uint32_t lane = threadIdx.x;
     const uint32_t nb_all = *io.hb_count;
     /* large batches for k_bigsort wait in registers (entry j in lane j) and
@@ -2430,17 +2420,18 @@ __global__ void __launch_bounds__(256) k_hist(uint32_t nreads, const uint32_t *_
 
 /* ------------------------------------------------------- scans and checks */
 
-/* Exclusive prefix sums in three launches (tile sums, scan of the tile sums
- * in one workgroup, tile rescan), graph-safe: plain kernels, no memset and
- * no temporary-size query. */
+/* Exclusive prefix sums in two launches (tile sums; tile rescan, each tile
+ * adding up the sums of the tiles before it, tile 0 also the total and its
+ * capacity check), graph-safe: plain kernels, no memset and no
+ * temporary-size query. */
 #define SCAN_NT 256
 #define SCAN_PER 16
 #define SCAN_TILE (SCAN_NT * SCAN_PER)
 
 /* One scan job covers up to two arrays of the same length (blockIdx.y picks
  * the array): array 0 is u64, array 1 is u32; tile sums are u64 for both.
- * The top kernel also checks each total against its capacity (setting the
- * overflow flag) and keeps it in the counters. */
+ * The rescan's first tile also checks each total against its capacity
+ * (setting the overflow flag) and keeps it in the counters. */
 struct ScanJob {
     const uint64_t *in0;
     uint64_t *out0;
@@ -2453,6 +2444,11 @@ struct ScanJob {
     uint64_t cap[2];
     unsigned long long *tot[2];
     uint32_t *ovf;
+    /* with cls_nsel, the scan of the position counts (array 0) also sorts
+     * the heavy items (cls_n of them) into the seed-heap kernels' lists */
+    const uint32_t *cls_nsel;
+    uint32_t cls_n;
+    uint32_t *cls_heavy, *cls_heavy_count, *cls_giant, *cls_giant_count;
 };
 
 __device__ __forceinline__ uint64_t scan_in(const ScanJob &j, uint32_t y, uint64_t g)
@@ -2476,31 +2472,28 @@ __global__ void __launch_bounds__(SCAN_NT) k_scan_reduce(ScanJob j)
     if (threadIdx.x == 0) j.sums[(uint64_t)y * j.nb + blockIdx.x] = tot;
 }
 
-__global__ void __launch_bounds__(1024) k_scan_top(ScanJob j)
-{
-    __shared__ uint64_t s_w[16];
-    const uint32_t y = j.first + blockIdx.x;
-    uint64_t *sums = j.sums + (uint64_t)y * j.nb;
-    uint64_t carry = 0;
-    for (uint32_t b = 0; b < j.nb; b += 1024) {
-        const uint32_t i = b + threadIdx.x;
-        const uint64_t v = i < j.nb ? sums[i] : 0ull;
-        uint64_t tot;
-        const uint64_t incl = block_incl_scan<uint64_t, 1024>(v, s_w, &tot);
-        if (i < j.nb) sums[i] = carry + incl - v;
-        carry += tot;
-    }
-    if (threadIdx.x == 0) {
-        if (j.tot[y]) *j.tot[y] = carry;
-        if (carry > j.cap[y]) *j.ovf = 1u;
-    }
-}
-
 __global__ void __launch_bounds__(SCAN_NT) k_scan_down(ScanJob j)
 {
     __shared__ uint64_t s_w[SCAN_NT / 64];
+    __shared__ uint32_t s_base[2];
     const uint32_t y = j.first + blockIdx.y;
     const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_PER;
+    const uint64_t *sums = j.sums + (uint64_t)y * j.nb;
+    /* this tile's offset: the tile sums before it (tile 0: all of them,
+     * for the total) */
+    uint64_t pre = 0, all = 0;
+    for (uint32_t b = threadIdx.x; b < j.nb; b += SCAN_NT) {
+        const uint64_t x = sums[b];
+        if (b < blockIdx.x) pre += x;
+        all += x;
+    }
+    uint64_t tile_off, grand;
+    (void)block_incl_scan<uint64_t, SCAN_NT>(pre, s_w, &tile_off);
+    (void)block_incl_scan<uint64_t, SCAN_NT>(blockIdx.x == 0 ? all : 0ull, s_w, &grand);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (j.tot[y]) *j.tot[y] = grand;
+        if (grand > j.cap[y]) *j.ovf = 1u;
+    }
     uint64_t v[SCAN_PER];
     uint64_t t = 0;
 #pragma unroll
@@ -2509,7 +2502,7 @@ __global__ void __launch_bounds__(SCAN_NT) k_scan_down(ScanJob j)
         t += v[q];
     }
     uint64_t tot;
-    uint64_t run = block_incl_scan<uint64_t, SCAN_NT>(t, s_w, &tot) - t + j.sums[(uint64_t)y * j.nb + blockIdx.x];
+    uint64_t run = block_incl_scan<uint64_t, SCAN_NT>(t, s_w, &tot) - t + tile_off;
 #pragma unroll
     for (int q = 0; q < SCAN_PER; q++) {
         if (base + q < j.n) {
@@ -2517,6 +2510,34 @@ __global__ void __launch_bounds__(SCAN_NT) k_scan_down(ScanJob j)
             else j.out1[base + q] = (uint32_t)run;
         }
         run += v[q];
+    }
+    if (y == 0 && j.cls_nsel) {
+        /* heavy items (a selected 12-mer with more than one chunk) to the
+         * seed-heap lists, those over GIANT_POS positions to their own:
+         * one atomic per list per workgroup */
+        uint32_t hm = 0, gm = 0;
+#pragma unroll
+        for (int q = 0; q < SCAN_PER; q++) {
+            const uint64_t g = base + q;
+            if (g < j.cls_n && (j.cls_nsel[g] & NSEL_HEAVY)) {
+                if (v[q] > GIANT_POS) gm |= 1u << q;
+                else hm |= 1u << q;
+            }
+        }
+        const uint64_t c = (uint64_t)__popc(hm) | ((uint64_t)__popc(gm) << 32);
+        uint64_t ctot;
+        const uint64_t cex = block_incl_scan<uint64_t, SCAN_NT>(c, s_w, &ctot) - c;
+        if (threadIdx.x == 0) {
+            s_base[0] = (uint32_t)ctot ? atomicAdd(j.cls_heavy_count, (uint32_t)ctot) : 0u;
+            s_base[1] = (uint32_t)(ctot >> 32) ? atomicAdd(j.cls_giant_count, (uint32_t)(ctot >> 32)) : 0u;
+        }
+        __syncthreads();
+        uint32_t ph = s_base[0] + (uint32_t)cex, pg = s_base[1] + (uint32_t)(cex >> 32);
+#pragma unroll
+        for (int q = 0; q < SCAN_PER; q++) {
+            if ((hm >> q) & 1u) j.cls_heavy[ph++] = (uint32_t)(base + q);
+            if ((gm >> q) & 1u) j.cls_giant[pg++] = (uint32_t)(base + q);
+        }
     }
 }
 
@@ -2766,7 +2787,6 @@ static void scan_launch(ScanJob j, uint32_t n_arrays, hipStream_t st)
 {
     j.nb = blocks_for(j.n, SCAN_TILE);
     k_scan_reduce<<<dim3(j.nb, n_arrays), SCAN_NT, 0, st>>>(j);
-    k_scan_top<<<n_arrays, 1024, 0, st>>>(j);
     k_scan_down<<<dim3(j.nb, n_arrays), SCAN_NT, 0, st>>>(j);
 }
 
@@ -2789,6 +2809,12 @@ static int enq_a(const swm_ctx_t *ctx, Pipe &P, const char *d_reads, uint64_t st
     j.cap[0] = cap_pos; j.cap[1] = cap_chunk;
     j.tot[0] = P.counters + 3; j.tot[1] = nullptr;
     j.ovf = P.u32c + 7;
+    if (ctx->opt.work_queue) {
+        j.cls_nsel = P.so.nsel;
+        j.cls_n = (uint32_t)P.nrs;
+        j.cls_heavy = P.pio.heavy; j.cls_heavy_count = P.pio.heavy_count;
+        j.cls_giant = P.pio.giant; j.cls_giant_count = P.pio.giant_count;
+    }
     scan_launch(j, 2, st);
     SWM_HIP_CHECK(hipGetLastError());
     return SWM_OK;
@@ -2800,7 +2826,6 @@ static int enq_b(const swm_ctx_t *ctx, Pipe &P, uint64_t cap_cand, hipStream_t s
 {
     const swm_opts &o = ctx->opt;
     if (o.work_queue) {
-        k_plan_classify<<<blocks_for(P.nrs, 256), 256, 0, st>>>((uint32_t)P.nrs, P.pio);
         k_plan_wave<<<(uint32_t)std::min<uint64_t>(P.nrs ? P.nrs : 1, (uint64_t)ctx->n_cu * 32), 64, 0, st>>>(
             (uint32_t)P.nrs, ctx->d_pl, P.pio);
         k_plan_heavy<<<ctx->n_cu * o.plan_heavy_waves, 64, 0, st>>>(ctx->d_pl, P.pio);
