@@ -18,7 +18,8 @@
  *              go to k_plan_heavy (seed-heap replay) + k_plan_fetch;
  *              k_plan_plain is the whole state machine one thread per item
  *   k_bigsort  one wave per large batch: register bitonic sort + unique
- *   k_compact  one thread per (read, strand): dense candidate list
+ *   k_compact  256 (read, strand) items per workgroup, copied flat: dense
+ *              candidate list (k_compact_heavy: one heavy item per workgroup)
  *   k_myers2   two candidates per lane (e <= 7) / k_myers one: window
  *              fetch, banded Myers (semantics mapper_cpe.c:519-660 == :668-976)
  *   k_hamming  one lane per passing candidate: the Hamming check of
@@ -1236,43 +1237,140 @@ struct CandSet {
 #define INFO_NEG (1u << 17)
 
 /* Items with more than heavy_cands candidates go to the cooperative kernels. */
-__global__ void __launch_bounds__(128) k_compact(uint32_t nrs, const uint64_t *__restrict__ surv_off,
+/* The other items, 256 per workgroup, copied flat: their batch descriptors
+ * laid end to end (a scan of the batch counts) and loaded together, the
+ * survivor counts scanned in LDS, then every survivor copied by the thread
+ * of its place in the workgroup's span.  (A thread per item walking its
+ * batches four survivors per memory round trip left each wave waiting on
+ * its longest item.)  Items with more than heavy_cands candidates go to the
+ * heavy list for k_compact_heavy. */
+#define CPT_SLOTS 512   /* batch descriptors staged at a time */
+__global__ void __launch_bounds__(256) k_compact(uint32_t nrs, const uint64_t *__restrict__ surv_off,
                                                  const uint32_t *__restrict__ surv, const uint32_t *__restrict__ bd_off,
                                                  const uint2 *__restrict__ bdesc, const uint32_t *__restrict__ nbatch,
                                                  const uint32_t *__restrict__ cand_off, CandSet c, uint32_t heavy_cands,
                                                  uint32_t *__restrict__ heavy, uint32_t *__restrict__ heavy_count)
 {
-    __shared__ uint32_t s_tmp[2];
-    if (*c.ovf) return;
-    uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ uint32_t s_tmp[2], s_w[4];
+    __shared__ uint32_t s_bpre[257], s_bdo[256], s_o[256];   /* per item: first slot, descriptors, next output */
+    __shared__ uint64_t s_so[256];                            /* per item: first survivor */
+    __shared__ uint32_t s_it[CPT_SLOTS], s_pre[CPT_SLOTS + 1], s_ob[CPT_SLOTS], s_bj[CPT_SLOTS];
+    __shared__ uint64_t s_src[CPT_SLOTS];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t R0 = blockIdx.x * 256u;
+    const uint32_t r = R0 + tid;
     const bool live = r < nrs;
-    if (!live) r = 0;
-    const bool big = live && cand_off[r + 1] - cand_off[r] > heavy_cands;
+    /* the item's fields are loaded with the overflow flag (they lie in
+     * buffers sized by the item count, whatever the capacities): one
+     * memory round trip, then one for the descriptors and one for the
+     * survivors */
+    const uint32_t ovf = *c.ovf;
+    uint32_t co = 0, cn = 0, nbr = 0, bdo = 0;
+    uint64_t so = 0;
+    if (live) {
+        co = cand_off[r];
+        cn = cand_off[r + 1] - co;
+        nbr = nbatch[r];
+        bdo = bd_off[r];
+        so = surv_off[r];
+    }
+    if (ovf) return;
+    const bool big = cn > heavy_cands;
     block_append(big, r, heavy, heavy_count, s_tmp);
-    if (!live || big) return;
-    const uint32_t *s = surv + surv_off[r];
-    const uint2 *bd = bdesc + bd_off[r];
-    uint32_t o = cand_off[r];
-    const uint32_t nb = nbatch[r];
-    for (uint32_t b = 0; b < nb; b++) {
-        const uint2 d = bd[b];
-        /* four survivors loaded before any store (the stores could alias
-         * the loads as far as the compiler knows, which would otherwise
-         * put a memory round trip between candidates) */
-        for (uint32_t j0 = 0; j0 < d.y; j0 += 4) {
-            uint32_t v[4];
+    /* an item without candidates has only empty batches: nothing to copy */
+    const uint32_t nb = (live && !big && cn) ? nbr : 0u;
+    uint32_t tb;
+    const uint32_t bincl = block_incl_scan<uint32_t, 256>(nb, s_w, &tb);
+    s_bpre[tid] = bincl - nb;
+    if (tid == 0) s_bpre[256] = tb;
+    s_o[tid] = co;
+    s_bdo[tid] = bdo;
+    s_so[tid] = so;
+    __syncthreads();
+    for (uint32_t c0 = 0; c0 < tb; c0 += CPT_SLOTS) {
+        const uint32_t n = min((uint32_t)CPT_SLOTS, tb - c0);
+        /* slots 2 tid and 2 tid + 1: the owner is the last item whose first
+         * slot is not past it (items without batches share their first slot
+         * with the next item and come before it) */
+        uint32_t cq[2] = {0u, 0u};
 #pragma unroll
-            for (int q = 0; q < 4; q++) v[q] = j0 + q < d.y ? s[d.x + j0 + q] : 0u;
+        for (int q = 0; q < 2; q++) {
+            const uint32_t k = 2 * tid + (uint32_t)q;
+            if (k < n) {
+                const uint32_t slot = c0 + k;
+                uint32_t lo = 0, hi = 255;
+                while (lo < hi) {
+                    const uint32_t mid = (lo + hi + 1) >> 1;
+                    if (s_bpre[mid] <= slot) lo = mid;
+                    else hi = mid - 1;
+                }
+                const uint32_t b = slot - s_bpre[lo];
+                const uint2 d = bdesc[s_bdo[lo] + b];
+                s_it[k] = lo;
+                s_src[k] = s_so[lo] + d.x;
+                s_bj[k] = b << 12;
+                cq[q] = d.y;
+            }
+        }
+        const uint32_t pair = cq[0] + cq[1];
+        uint32_t te;
+        const uint32_t pex = block_incl_scan<uint32_t, 256>(pair, s_w, &te) - pair;
+        if (2 * tid < n) s_pre[2 * tid] = pex;
+        if (2 * tid + 1 < n) s_pre[2 * tid + 1] = pex + cq[0];
+        if (tid == 0) s_pre[n] = te;
+        __syncthreads();
+        /* output index of each slot's first survivor: its item's next output
+         * plus the survivors of the item's earlier slots in this chunk */
+#pragma unroll
+        for (int q = 0; q < 2; q++) {
+            const uint32_t k = 2 * tid + (uint32_t)q;
+            if (k < n) {
+                const uint32_t it = s_it[k];
+                const uint32_t first = s_bpre[it] > c0 ? s_bpre[it] - c0 : 0u;
+                s_ob[k] = s_o[it] + s_pre[k] - s_pre[first];
+            }
+        }
+        __syncthreads();
+        if (nb) {
+            const uint32_t f = s_bpre[tid], l = f + nb;
+            const uint32_t a = min(max(f, c0), c0 + n) - c0, z = min(max(l, c0), c0 + n) - c0;
+            s_o[tid] += s_pre[z] - s_pre[a];
+        }
+        /* survivor e of the chunk: slot = the last one whose prefix is not
+         * past e (empty slots share their prefix with the next), four
+         * loads in flight per thread */
+        for (uint32_t e0 = tid; e0 < te; e0 += 1024) {
+            uint32_t kq[4], jq[4], vq[4];
 #pragma unroll
             for (int q = 0; q < 4; q++) {
-                if (j0 + q < d.y) {
-                    c.val[o + q] = v[q];
-                    c.rs[o + q] = r;
-                    c.bi[o + q] = (b << 12) | (j0 + q);
+                const uint32_t e = e0 + 256u * (uint32_t)q;
+                kq[q] = 0;
+                jq[q] = 0;
+                vq[q] = 0;
+                if (e < te) {
+                    uint32_t lo = 0, hi = n - 1;
+                    while (lo < hi) {
+                        const uint32_t mid = (lo + hi + 1) >> 1;
+                        if (s_pre[mid] <= e) lo = mid;
+                        else hi = mid - 1;
+                    }
+                    kq[q] = lo;
+                    jq[q] = e - s_pre[lo];
+                    vq[q] = surv[s_src[lo] + jq[q]];
                 }
             }
-            o += min(4u, d.y - j0);
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const uint32_t e = e0 + 256u * (uint32_t)q;
+                if (e < te) {
+                    const uint32_t out = s_ob[kq[q]] + jq[q];
+                    c.val[out] = vq[q];
+                    c.rs[out] = R0 + s_it[kq[q]];
+                    c.bi[out] = s_bj[kq[q]] | jq[q];
+                }
+            }
         }
+        __syncthreads();
     }
 }
 
@@ -2858,7 +2956,7 @@ static int enq_c(const swm_ctx_t *ctx, Pipe &P, const uint32_t *d_lens, uint32_t
     const swm_opts &o = ctx->opt;
     const uint32_t heavy_cands = o.work_queue ? o.heavy_cands : 0xffffffffu;
     const uint32_t *ncand = P.cand_off + P.nrs;
-    k_compact<<<blocks_for(P.nrs, 128), 128, 0, st>>>((uint32_t)P.nrs, P.surv_off, P.pio.surv, P.bd_off, P.pio.bdesc,
+    k_compact<<<blocks_for(P.nrs, 256), 256, 0, st>>>((uint32_t)P.nrs, P.surv_off, P.pio.surv, P.bd_off, P.pio.bdesc,
                                                       P.pio.nbatch, P.cand_off, P.cs, heavy_cands, P.heavy_c,
                                                       P.u32c + 5);
     if (o.work_queue)
