@@ -2010,10 +2010,13 @@ __global__ void __launch_bounds__(64) k_trace(const uint32_t *__restrict__ trace
                     const uint32_t bd = (search_site < 0 || search_site >= 32) ? 0u : ((d0r[s] >> search_site) & 1u);
                     const uint32_t bh = (search_site < 0 || search_site >= 32) ? 0u : ((hpr[s] >> search_site) & 1u);
                     const uint32_t rb = refb(match_site);
+                    /* a substitution closes like a match (both add to the
+                     * pending "<n>M" group), so it takes the match's run
+                     * label: only indels start a new run */
                     char nc;
                     if (bd && rb == qb) { i--; match_site--; nc = 'M'; }
-                    else if (!bd && rb != qb) { i--; match_site--; sum_err++; nc = 'S'; }
-                    else if (bh) { i--; search_site++; sum_err++; start_location++; nc = first ? 'S' : 'I'; }
+                    else if (!bd && rb != qb) { i--; match_site--; sum_err++; nc = 'M'; }
+                    else if (bh) { i--; search_site++; sum_err++; start_location++; nc = first ? 'M' : 'I'; }
                     else { search_site--; match_site--; sum_err++; start_location--; nc = 'D'; }
                     if (first) { pre_char = nc; pre_size = 1; first = false; }
                     else if (nc != pre_char) { close_run(pre_char, pre_size, ms_acc, text, runs, nr); pre_char = nc; pre_size = 1; }
