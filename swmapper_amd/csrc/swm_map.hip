@@ -1781,9 +1781,8 @@ __global__ void __launch_bounds__(256) k_myers2(const uint32_t *__restrict__ nca
  * atomics serialise at one L2 channel). */
 #define TRACE_BUF 2048
 #define HAM_GRID_MAX 4096u   /* k_hamming workgroups at most (their count slots) */
-#define HAM_PF 8             /* candidate rounds whose errors are loaded together */
 
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) k_hamming(const uint32_t *__restrict__ ncand_p, const uint32_t *__restrict__ lens,
+__global__ void __launch_bounds__(256) k_hamming(const uint32_t *__restrict__ ncand_p, const uint32_t *__restrict__ lens,
                                                  const uint32_t *__restrict__ words, const uint32_t *__restrict__ b2,
                                                  uint64_t ref_length, uint32_t e, CandSet c,
                                                  uint32_t *__restrict__ trace, uint32_t *__restrict__ trace_count,
@@ -1812,39 +1811,25 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) k
         }
         __syncthreads();
     };
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t b0 = (uint64_t)blockIdx.x * blockDim.x; b0 < ncand; b0 += HAM_PF * stride) {
-        /* the errors of HAM_PF rounds loaded together: a round in which no
-         * lane passes waits on no load of its own */
-        int32_t q[HAM_PF];
-#pragma unroll
-        for (int k = 0; k < HAM_PF; k++) {
-            const uint64_t ii = b0 + (uint64_t)k * stride + tid;
-            q[k] = ii < ncand ? c.err[ii] : 0;
-        }
-        for (int k = 0; k < HAM_PF; k++) {
-            const int32_t er = q[0];
-#pragma unroll
-            for (int m = 0; m + 1 < HAM_PF; m++) q[m] = q[m + 1];
-            const uint64_t ii = b0 + (uint64_t)k * stride + tid;
-            const uint32_t i = (uint32_t)ii;
-            bool to_trace = false;
-            if (ii < ncand) {
-                if (er == -2) {
-                    my_rej++;                          /* pos_start rejected the window: not a lane */
-                    my_steps -= lens[c.rs[i] >> 1];    /* taken off the per-read total below */
-                } else if (er <= (int32_t)e) {
-                    to_trace = hamming_one(i, er, lens, words, b2, ref_length, e, c);
-                }
+    for (uint32_t b0 = blockIdx.x * blockDim.x; b0 < ncand; b0 += gridDim.x * blockDim.x) {
+        const uint32_t i = b0 + tid;
+        bool to_trace = false;
+        if (i < ncand) {
+            const int32_t er = c.err[i];
+            if (er == -2) {
+                my_rej++;                          /* pos_start rejected the window: not a lane */
+                my_steps -= lens[c.rs[i] >> 1];    /* taken off the per-read total below */
+            } else if (er <= (int32_t)e) {
+                to_trace = hamming_one(i, er, lens, words, b2, ref_length, e, c);
             }
-            const uint64_t m = __ballot(to_trace);
-            uint32_t woff = 0;
-            if (lane == 0 && m) woff = atomicAdd(&s_n, (uint32_t)__popcll(m));
-            woff = (uint32_t)__shfl((int)woff, 0, 64);
-            if (to_trace) s_list[woff + (uint32_t)__popcll(m & below)] = i;
-            __syncthreads();
-            if (s_n > TRACE_BUF - 256) flush();
         }
+        const uint64_t m = __ballot(to_trace);
+        uint32_t woff = 0;
+        if (lane == 0 && m) woff = atomicAdd(&s_n, (uint32_t)__popcll(m));
+        woff = (uint32_t)__shfl((int)woff, 0, 64);
+        if (to_trace) s_list[woff + (uint32_t)__popcll(m & below)] = i;
+        __syncthreads();
+        if (s_n > TRACE_BUF - 256) flush();
     }
     flush();
     /* band steps: read length x the read's candidates, less the rejected
