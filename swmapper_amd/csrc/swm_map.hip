@@ -179,6 +179,7 @@ struct SeedOut {
     uint32_t *nchunk;   /* [nrs + 1] sum of ceil(cnt / 32), + 1 */
     unsigned long long *counters;   /* the run's counters, zeroed here */
     uint32_t *nuniq;    /* [nrs + 1]: its last entry is zeroed here */
+    uint32_t *lbins;    /* [256] candidates per read length, zeroed here */
 };
 
 /* A read base as the mapper sees it: A,C,G,T -> 0..3 (complemented on the
@@ -268,6 +269,7 @@ __global__ void __launch_bounds__(128) k_seed(const char *__restrict__ reads, ui
             o.nchunk[2 * nreads] = 0;
             o.nuniq[2 * nreads] = 0;
         }
+        for (uint32_t j = threadIdx.x; j < 256u; j += blockDim.x) o.lbins[j] = 0;
     }
     const uint32_t r0 = blockIdx.x * SEED_READS;
     const uint32_t nr = min((uint32_t)SEED_READS, nreads - r0);
@@ -1223,6 +1225,13 @@ struct CandSet {
     uint64_t *wr = nullptr;
     uint32_t *tb = nullptr;
     uint16_t *runs = nullptr;     /* [ci * RUN_SLOT]: count, then runs in traceback order */
+    /* verification order (null: candidate order): perm[p] is the candidate
+     * the Myers kernels take at place p, unless *perm_id is set (one read
+     * length in the batch: the identity).  perm_id is 8-byte aligned with
+     * the overflow flag right after it (perm_id + 1 == ovf), so the Myers
+     * kernels load both in one 64-bit load */
+    const uint32_t *perm = nullptr;
+    const uint32_t *perm_id = nullptr;
 };
 
 /* A traceback's CIGAR runs, M and S runs merged as the emission merges them
@@ -1235,6 +1244,110 @@ struct CandSet {
 
 #define INFO_TRACE (1u << 16)
 #define INFO_NEG (1u << 17)
+
+/* (identity order, overflow) in one load: a workgroup of the Myers grid,
+ * which covers every candidate, starts on one round trip, not two.  The
+ * caller stands down on myers_stop(): the identity flag is 0 or 1, so its
+ * bit 31 never counts, but testing it keeps the compiler from deferring
+ * that word's load past the branch. */
+__device__ __forceinline__ uint2 myers_flags(const CandSet &c)
+{
+    return *reinterpret_cast<const uint2 *>(c.perm_id);
+}
+__device__ __forceinline__ bool myers_stop(uint2 fl) { return ((fl.x >> 31) | fl.y) != 0u; }
+
+/* Verification order by read length.  In candidate order a lane's pair
+ * straddles two reads of different lengths wherever a read's candidate
+ * count is odd (the packed kernel then runs both halves alone), and a wave
+ * holds reads of many lengths whose steps all wait for the longest.  Three
+ * small kernels lay the candidate indices out by read length instead: per
+ * length a bin, per (read, strand) one run of its candidates.  A batch of
+ * one length keeps the identity (the scatter stands down).  The order
+ * changes no result: each candidate writes only its own slots. */
+#define LSORT_WG_PER_CU 4
+
+__device__ __forceinline__ uint32_t lbin_of(uint32_t L) { return L < 255u ? L : 255u; }
+
+/* candidates per read length, one atomic per length per workgroup */
+__global__ void __launch_bounds__(256) k_lcount(uint32_t nrs, const uint32_t *__restrict__ cand_off,
+                                                const uint32_t *__restrict__ lens, const uint32_t *__restrict__ ovf,
+                                                uint32_t *__restrict__ bins)
+{
+    __shared__ uint32_t h[256];
+    if (*ovf) return;
+    const uint32_t tid = threadIdx.x;
+    h[tid] = 0;
+    __syncthreads();
+    for (uint64_t r = (uint64_t)blockIdx.x * 256u + tid; r < nrs; r += (uint64_t)gridDim.x * 256u) {
+        const uint32_t n = cand_off[r + 1] - cand_off[r];
+        if (n) atomicAdd(&h[lbin_of(lens[r >> 1])], n);
+    }
+    __syncthreads();
+    if (h[tid]) atomicAdd(&bins[tid], h[tid]);
+}
+
+/* one workgroup: each bin's first place (bins[256 + L]), and the identity
+ * flag when at most one bin is used */
+__global__ void __launch_bounds__(256) k_lscan(const uint32_t *__restrict__ ovf, uint32_t *__restrict__ bins,
+                                               uint32_t *__restrict__ ident)
+{
+    __shared__ uint32_t s_w[4];
+    if (*ovf) return;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t v = bins[tid];
+    uint32_t tot, used;
+    const uint32_t ex = block_incl_scan<uint32_t, 256>(v, s_w, &tot) - v;
+    (void)block_incl_scan<uint32_t, 256>(v ? 1u : 0u, s_w, &used);
+    bins[256 + tid] = ex;
+    if (tid == 0) *ident = used <= 1u ? 1u : 0u;
+}
+
+/* the same items per workgroup as k_lcount: its share of each bin (one
+ * atomic per length), then every item's run at its place in the share;
+ * runs of up to 16 written by their own lane, longer ones by the wave */
+__global__ void __launch_bounds__(256) k_lscatter(uint32_t nrs, const uint32_t *__restrict__ cand_off,
+                                                  const uint32_t *__restrict__ lens, const uint32_t *__restrict__ ovf,
+                                                  const uint32_t *__restrict__ ident, uint32_t *__restrict__ bins,
+                                                  uint32_t *__restrict__ perm)
+{
+    __shared__ uint32_t h[256], base[256];
+    if (*ovf || *ident) return;
+    const uint32_t tid = threadIdx.x, lane = __lane_id();
+    const uint64_t stride = (uint64_t)gridDim.x * 256u;
+    h[tid] = 0;
+    __syncthreads();
+    for (uint64_t r = (uint64_t)blockIdx.x * 256u + tid; r < nrs; r += stride) {
+        const uint32_t n = cand_off[r + 1] - cand_off[r];
+        if (n) atomicAdd(&h[lbin_of(lens[r >> 1])], n);
+    }
+    __syncthreads();
+    base[tid] = h[tid] ? atomicAdd(&bins[256 + tid], h[tid]) : 0u;
+    h[tid] = 0;
+    __syncthreads();
+    for (uint64_t r0 = (uint64_t)blockIdx.x * 256u; r0 < nrs; r0 += stride) {
+        const uint64_t r = r0 + tid;
+        uint32_t start = 0, n = 0, pos = 0;
+        if (r < nrs) {
+            start = cand_off[r];
+            n = cand_off[r + 1] - start;
+            if (n) {
+                const uint32_t b = lbin_of(lens[r >> 1]);
+                pos = base[b] + atomicAdd(&h[b], n);
+            }
+        }
+        const bool own = n <= 16u;
+        if (own)
+            for (uint32_t j = 0; j < n; j++) perm[pos + j] = start + j;
+        uint64_t m = __ballot(!own);
+        while (m) {
+            const int k = __builtin_ctzll(m);
+            m &= m - 1ull;
+            const uint32_t pk = (uint32_t)__shfl((int)pos, k, 64), sk = (uint32_t)__shfl((int)start, k, 64);
+            const uint32_t nk = (uint32_t)__shfl((int)n, k, 64);
+            for (uint32_t j = lane; j < nk; j += 64u) perm[pk + j] = sk + j;
+        }
+    }
+}
 
 /* Items with more than heavy_cands candidates go to the cooperative kernels. */
 /* The other items, 256 per workgroup, copied flat: their batch descriptors
@@ -1540,13 +1653,14 @@ __device__ __forceinline__ void load_window(const uint32_t *__restrict__ b2, uin
 
 /* The candidate count is read on the device (the last entry of the candidate
  * offsets), so a fixed grid serves any batch: grid-stride loop. */
-__global__ void __launch_bounds__(256) k_myers(const uint32_t *__restrict__ ncand_p, const uint32_t *__restrict__ lens,
-                                               const uint32_t *__restrict__ words, const uint32_t *__restrict__ b2,
-                                               uint64_t ref_length, uint32_t e, CandSet c)
+template <bool PERM>
+__device__ __forceinline__ void myers_loop(uint32_t ncand, const uint32_t *__restrict__ perm,
+                                           const uint32_t *__restrict__ lens, const uint32_t *__restrict__ words,
+                                           const uint32_t *__restrict__ b2, uint64_t ref_length, uint32_t e,
+                                           const CandSet &c)
 {
-    if (*c.ovf) return;
-    const uint32_t ncand = *ncand_p;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < ncand; i += gridDim.x * blockDim.x) {
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < ncand; p += gridDim.x * blockDim.x) {
+        const uint32_t i = PERM ? perm[p] : p;    /* place p of the verification order */
         const uint32_t rs = c.rs[i];
         const uint32_t L = lens[rs >> 1];
         uint32_t ref_len;
@@ -1564,6 +1678,17 @@ __global__ void __launch_bounds__(256) k_myers(const uint32_t *__restrict__ ncan
             c.loc[i] = lc;
         }
     }
+}
+
+__global__ void __launch_bounds__(256) k_myers(const uint32_t *__restrict__ ncand_p, const uint32_t *__restrict__ lens,
+                                               const uint32_t *__restrict__ words, const uint32_t *__restrict__ b2,
+                                               uint64_t ref_length, uint32_t e, CandSet c)
+{
+    const uint2 fl = myers_flags(c);
+    if (myers_stop(fl)) return;
+    const uint32_t ncand = *ncand_p;
+    if (c.perm && !fl.x) myers_loop<true>(ncand, c.perm, lens, words, b2, ref_length, e, c);
+    else myers_loop<false>(ncand, nullptr, lens, words, b2, ref_length, e, c);
 }
 
 /* Hamming check of align_reads (:252-287) for one lane with err <= e: the
@@ -1728,16 +1853,21 @@ __device__ __forceinline__ void myers_one(uint32_t i, uint32_t st, uint32_t L, u
     c.loc[i] = lc;
 }
 
-__global__ void __launch_bounds__(256) k_myers2(const uint32_t *__restrict__ ncand_p, const uint32_t *__restrict__ lens,
-                                                const uint32_t *__restrict__ words, const uint32_t *__restrict__ b2,
-                                                uint64_t ref_length, uint32_t e, CandSet c)
+/* The pair loop, once per order: with the identity a pair's loads do not
+ * wait on an order lookup (one body for both orders had serialised them,
+ * 2% of the skew step). */
+template <bool PERM>
+__device__ __forceinline__ void myers2_loop(uint32_t ncand, const uint32_t *__restrict__ perm,
+                                            const uint32_t *__restrict__ lens, const uint32_t *__restrict__ words,
+                                            const uint32_t *__restrict__ b2, uint64_t ref_length, uint32_t e,
+                                            const CandSet &c)
 {
-    if (*c.ovf) return;
-    const uint32_t ncand = *ncand_p;
     const uint32_t npair = (ncand + 1) / 2;
     for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < npair; g += gridDim.x * blockDim.x) {
-        const uint32_t i0 = 2 * g, i1 = 2 * g + 1;
-        const bool has1 = i1 < ncand;
+        /* places 2g and 2g + 1 of the verification order */
+        const bool has1 = 2 * g + 1 < ncand;
+        const uint32_t i0 = PERM ? perm[2 * g] : 2 * g;
+        const uint32_t i1 = PERM ? (has1 ? perm[2 * g + 1] : 0u) : 2 * g + 1;
         const uint32_t rs0 = c.rs[i0], rs1 = has1 ? c.rs[i1] : rs0;
         const uint32_t L0 = lens[rs0 >> 1], L1 = lens[rs1 >> 1];
         uint32_t rl0, rl1 = 0;
@@ -1768,6 +1898,17 @@ __global__ void __launch_bounds__(256) k_myers2(const uint32_t *__restrict__ nca
             if (v1) myers_one(i1, st1, L1, rl1, words, b2, e, c);
         }
     }
+}
+
+__global__ void __launch_bounds__(256) k_myers2(const uint32_t *__restrict__ ncand_p, const uint32_t *__restrict__ lens,
+                                                const uint32_t *__restrict__ words, const uint32_t *__restrict__ b2,
+                                                uint64_t ref_length, uint32_t e, CandSet c)
+{
+    const uint2 fl = myers_flags(c);
+    if (myers_stop(fl)) return;
+    const uint32_t ncand = *ncand_p;
+    if (c.perm && !fl.x) myers2_loop<true>(ncand, c.perm, lens, words, b2, ref_length, e, c);
+    else myers2_loop<false>(ncand, nullptr, lens, words, b2, ref_length, e, c);
 }
 
 /* Per candidate: the Hamming check of passing lanes, the traceback list, the
@@ -2655,7 +2796,7 @@ struct swm_ws {
     struct Buf { void *p = nullptr; size_t cap = 0; };
     enum { WORDS, SEL, NSEL, NPOS, NCHUNK, SURV_OFF, BD_OFF, SURV, BDESC, NBATCH, NUNIQ, CAND_OFF, C_VAL, C_RS, C_BI,
            C_ERR, C_LOC, C_REC, C_INFO, COUNTERS, BIG, TRACE, HEAVY, HEAVY_C, HEAVY_E, SCAN_TMP, IN_READS, IN_LENS,
-           OUT, HIST, R_WR, R_TB, R_RUNS, R_IDX, R_TOFF, R_SCAN, R_TOT, R_OUT, R_TEXT, CDESC, HBATCH, GIANT, EPART, RPART, HPART, NBUF };
+           OUT, HIST, R_WR, R_TB, R_RUNS, R_IDX, R_TOFF, R_SCAN, R_TOT, R_OUT, R_TEXT, CDESC, HBATCH, GIANT, EPART, RPART, HPART, LBINS, PERM, NBUF };
     Buf b[NBUF];
     uint64_t gen = 1;
     ~swm_ws() { for (auto &x : b) if (x.p) (void)hipFree(x.p); }
@@ -2816,12 +2957,14 @@ struct Pipe {
     unsigned long long *counters = nullptr;
     uint32_t *u32c = nullptr;
     uint64_t *scan_tmp = nullptr;
+    uint32_t *lbins = nullptr;       /* [512]: candidates per read length, then each length's next place */
+    uint32_t *perm = nullptr;        /* [cap]: verification order (CandSet::perm) */
 };
 
 /* counters: u64 [0] candidates, [1] records, [2] band steps, [3] surviving
  * positions; u32 view of [4..9]: [0] large batches, [1] tracebacks,
  * [2] rejected windows, [3] heavy plan items, [4] their queue head,
- * [5] heavy compaction items, [6] (unused), [7] overflow, [8] heavy plan
+ * [5] heavy compaction items, [6] verification order is the identity, [7] overflow, [8] heavy plan
  * batches, [9] giant plan items, [10] giant / [11] other heavy emission
  * reads */
 static int pipe_alloc_a(swm_ws *ws, uint64_t n, uint32_t seln, Pipe &P)
@@ -2841,9 +2984,11 @@ static int pipe_alloc_a(swm_ws *ws, uint64_t n, uint32_t seln, Pipe &P)
         (rc = ws->get(swm_ws::EPART, &P.epart, 2 * n)) ||
         (rc = ws->get(swm_ws::RPART, &P.rpart, (uint64_t)blocks_for(n, 128))) ||
         (rc = ws->get(swm_ws::HPART, &P.hpart, 2 * (uint64_t)HAM_GRID_MAX)) ||
+        (rc = ws->get(swm_ws::LBINS, &P.lbins, 512)) ||
         (rc = ws->get(swm_ws::SCAN_TMP, &P.scan_tmp, 2 * (uint64_t)blocks_for(nrs + 1, SCAN_TILE))))
         return rc;
     P.so.counters = P.counters;
+    P.so.lbins = P.lbins;
     P.so.nuniq = P.pio.nuniq;
     P.u32c = reinterpret_cast<uint32_t *>(P.counters + 4);
     P.pio.sel = P.so.sel;
@@ -2858,6 +3003,8 @@ static int pipe_alloc_a(swm_ws *ws, uint64_t n, uint32_t seln, Pipe &P)
     P.pio.giant_count = P.u32c + 9;
     P.pio.ovf = P.u32c + 7;
     P.cs.ovf = P.u32c + 7;
+    P.cs.perm_id = P.u32c + 6;                    /* 8-byte aligned, the overflow flag next */
+    if ((reinterpret_cast<uintptr_t>(P.cs.perm_id) & 7u) || P.cs.perm_id + 1 != P.cs.ovf) return SWM_EINVAL;
     return SWM_OK;
 }
 
@@ -2879,8 +3026,10 @@ static int pipe_alloc_c(swm_ws *ws, uint64_t cap, Pipe &P)
     if ((rc = ws->get(swm_ws::C_VAL, &P.cs.val, cap + 1)) || (rc = ws->get(swm_ws::C_RS, &P.cs.rs, cap + 1)) ||
         (rc = ws->get(swm_ws::C_BI, &P.cs.bi, cap + 1)) || (rc = ws->get(swm_ws::C_ERR, &P.cs.err, cap + 1)) ||
         (rc = ws->get(swm_ws::C_LOC, &P.cs.loc, cap + 1)) || (rc = ws->get(swm_ws::C_REC, &P.cs.rec, cap + 1)) ||
-        (rc = ws->get(swm_ws::C_INFO, &P.cs.info, cap + 1)) || (rc = ws->get(swm_ws::TRACE, &P.trace, cap + 1)))
+        (rc = ws->get(swm_ws::C_INFO, &P.cs.info, cap + 1)) || (rc = ws->get(swm_ws::TRACE, &P.trace, cap + 1)) ||
+        (rc = ws->get(swm_ws::PERM, &P.perm, cap + 1)))
         return rc;
+    P.cs.perm = P.perm;
     return SWM_OK;
 }
 
@@ -2965,6 +3114,13 @@ static int enq_c(const swm_ctx_t *ctx, Pipe &P, const uint32_t *d_lens, uint32_t
     if (o.work_queue)
         k_compact_heavy<<<ctx->n_cu * 8, 256, 0, st>>>(P.heavy_c, P.u32c + 5, P.surv_off, P.pio.surv, P.bd_off,
                                                        P.pio.bdesc, P.pio.nbatch, P.cand_off, P.cs);
+    if (P.cs.perm) {
+        /* verification order by read length (the identity for one length) */
+        const uint32_t lg = (uint32_t)ctx->n_cu * LSORT_WG_PER_CU;
+        k_lcount<<<lg, 256, 0, st>>>((uint32_t)P.nrs, P.cand_off, d_lens, P.cs.ovf, P.lbins);
+        k_lscan<<<1, 256, 0, st>>>(P.cs.ovf, P.lbins, P.u32c + 6);
+        k_lscatter<<<lg, 256, 0, st>>>((uint32_t)P.nrs, P.cand_off, d_lens, P.cs.ovf, P.u32c + 6, P.lbins, P.perm);
+    }
     SWM_HIP_CHECK(hipGetLastError());
     if (ev) SWM_HIP_CHECK(hipEventRecord(ev[3], st));
     const uint32_t mb = o.myers_block;
