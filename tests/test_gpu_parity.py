@@ -130,6 +130,19 @@ def test_map_repeats(kind, frac):
     assert st["n_verifications"] == nv
 
 
+def test_map_tandem_mixed_lengths():
+    """Many candidates per read and 201 read lengths in one batch: the
+    verification order by read length with runs longer than a lane writes
+    alone (the wave writes them), next to the heavy compaction."""
+    g = wl.genome(400_000, seed=61, kind=swm.GENOME_TANDEM, frac=0.3)
+    r, l, _, _ = wl.reads(g, 3000, len_min=50, len_max=250, err=(0.01, 0.04), seed=62)
+    _say("map tandem mixed lengths e5")
+    got, exp, st, nv = _map_both(g, r, l, 5)
+    _assert_same(got, exp)
+    assert st["n_verifications"] == nv
+    assert nv > 16 * l.size and np.unique(l).size > 100
+
+
 def test_map_genome_ends_and_n():
     g = wl.genome(64_000, seed=51)
     starts = [0, 1, 2, 3, 4, 5, 7, 31, 32, g.size - 150, g.size - 151, g.size - 160, g.size - 200]
