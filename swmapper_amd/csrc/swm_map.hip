@@ -171,6 +171,11 @@ __device__ __forceinline__ T block_incl_scan(T v, T *s_w, T *total)
 
 #define NSEL_HEAVY 0x80000000u
 
+/* rows of the per-length candidate counts: workgroup w adds to row w % 8, so
+ * that one read length per batch does not put every workgroup's atomic on
+ * one address */
+#define LBIN_ROWS 8u
+
 struct SeedOut {
     uint32_t *words;    /* [nrs][32]: 16 interleaved 2-bit words, 8 plane pairs */
     swm_sel_t *sel;     /* [nrs][seln] */
@@ -179,7 +184,7 @@ struct SeedOut {
     uint32_t *nchunk;   /* [nrs + 1] sum of ceil(cnt / 32), + 1 */
     unsigned long long *counters;   /* the run's counters, zeroed here */
     uint32_t *nuniq;    /* [nrs + 1]: its last entry is zeroed here */
-    uint32_t *lbins;    /* [256] candidates per read length, zeroed here */
+    uint32_t *lbins;    /* [LBIN_ROWS][256] candidates per read length, zeroed here */
 };
 
 /* A read base as the mapper sees it: A,C,G,T -> 0..3 (complemented on the
@@ -269,7 +274,7 @@ __global__ void __launch_bounds__(128) k_seed(const char *__restrict__ reads, ui
             o.nchunk[2 * nreads] = 0;
             o.nuniq[2 * nreads] = 0;
         }
-        for (uint32_t j = threadIdx.x; j < 256u; j += blockDim.x) o.lbins[j] = 0;
+        for (uint32_t j = threadIdx.x; j < LBIN_ROWS * 256u; j += blockDim.x) o.lbins[j] = 0;
     }
     const uint32_t r0 = blockIdx.x * SEED_READS;
     const uint32_t nr = min((uint32_t)SEED_READS, nreads - r0);
@@ -1268,6 +1273,53 @@ __device__ __forceinline__ bool myers_stop(uint2 fl) { return ((fl.x >> 31) | fl
 
 __device__ __forceinline__ uint32_t lbin_of(uint32_t L) { return L < 255u ? L : 255u; }
 
+/* n candidates to LDS bin b, per wave: one atomic when the wave's items
+ * (those with candidates) share one length, else one per lane (same-address
+ * LDS atomics serialise: one length per batch made a lane-per-atomic count
+ * take 19 us of a 1M-read step).  Every lane of the wave calls it. */
+__device__ __forceinline__ void lhist_add(uint32_t *h, uint32_t n, uint32_t b)
+{
+    const uint64_t has = __ballot(n != 0u);
+    if (!has) return;
+    const uint32_t b0 = (uint32_t)__shfl((int)b, __builtin_ctzll(has), 64);
+    if (__all(n == 0u || b == b0)) {
+        uint32_t t = n;
+#pragma unroll
+        for (int d = 32; d > 0; d >>= 1) t += (uint32_t)__shfl_xor((int)t, d, 64);
+        if (__lane_id() == 0) atomicAdd(&h[b0], t);
+    } else if (n) {
+        atomicAdd(&h[b], n);
+    }
+}
+
+/* The workgroup's items (grid-stride, stride = the grid's threads) into its
+ * LDS length bins, the fields of LC_PER items per thread loaded together
+ * (one item at a time left the count latency-bound: 21 us of a 1M-read
+ * step). */
+#define LC_PER 8
+__device__ __forceinline__ void lhist_count(uint32_t *h, uint32_t nrs, const uint32_t *__restrict__ cand_off,
+                                            const uint32_t *__restrict__ lens, uint64_t stride)
+{
+    for (uint64_t r0 = (uint64_t)blockIdx.x * blockDim.x; r0 < nrs; r0 += LC_PER * stride) {
+        /* unconditional loads (a place past the end reads item 0 and counts
+         * nothing), so that no load waits on a branch */
+        uint32_t aq[LC_PER], cq[LC_PER], lq[LC_PER];
+#pragma unroll
+        for (int k = 0; k < LC_PER; k++) {
+            const uint64_t r = r0 + (uint64_t)k * stride + threadIdx.x;
+            const uint64_t rr = r < nrs ? r : 0u;
+            aq[k] = cand_off[rr];
+            cq[k] = cand_off[rr + 1];
+            lq[k] = lens[rr >> 1];
+        }
+#pragma unroll
+        for (int k = 0; k < LC_PER; k++) {
+            const bool in = r0 + (uint64_t)k * stride + threadIdx.x < nrs;
+            lhist_add(h, in ? cq[k] - aq[k] : 0u, lbin_of(lq[k]));
+        }
+    }
+}
+
 /* candidates per read length, one atomic per length per workgroup */
 __global__ void __launch_bounds__(256) k_lcount(uint32_t nrs, const uint32_t *__restrict__ cand_off,
                                                 const uint32_t *__restrict__ lens, const uint32_t *__restrict__ ovf,
@@ -1278,27 +1330,26 @@ __global__ void __launch_bounds__(256) k_lcount(uint32_t nrs, const uint32_t *__
     const uint32_t tid = threadIdx.x;
     h[tid] = 0;
     __syncthreads();
-    for (uint64_t r = (uint64_t)blockIdx.x * 256u + tid; r < nrs; r += (uint64_t)gridDim.x * 256u) {
-        const uint32_t n = cand_off[r + 1] - cand_off[r];
-        if (n) atomicAdd(&h[lbin_of(lens[r >> 1])], n);
-    }
+    lhist_count(h, nrs, cand_off, lens, (uint64_t)gridDim.x * 256u);
     __syncthreads();
-    if (h[tid]) atomicAdd(&bins[tid], h[tid]);
+    if (h[tid]) atomicAdd(&bins[(blockIdx.x % LBIN_ROWS) * 256u + tid], h[tid]);
 }
 
-/* one workgroup: each bin's first place (bins[256 + L]), and the identity
- * flag when at most one bin is used */
+/* one workgroup: each length's first place (bins[LBIN_ROWS * 256 + L], the
+ * rows summed), and the identity flag when at most one length is used */
 __global__ void __launch_bounds__(256) k_lscan(const uint32_t *__restrict__ ovf, uint32_t *__restrict__ bins,
                                                uint32_t *__restrict__ ident)
 {
     __shared__ uint32_t s_w[4];
     if (*ovf) return;
     const uint32_t tid = threadIdx.x;
-    const uint32_t v = bins[tid];
+    uint32_t v = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < LBIN_ROWS; w++) v += bins[w * 256u + tid];
     uint32_t tot, used;
     const uint32_t ex = block_incl_scan<uint32_t, 256>(v, s_w, &tot) - v;
     (void)block_incl_scan<uint32_t, 256>(v ? 1u : 0u, s_w, &used);
-    bins[256 + tid] = ex;
+    bins[LBIN_ROWS * 256u + tid] = ex;
     if (tid == 0) *ident = used <= 1u ? 1u : 0u;
 }
 
@@ -1316,12 +1367,9 @@ __global__ void __launch_bounds__(256) k_lscatter(uint32_t nrs, const uint32_t *
     const uint64_t stride = (uint64_t)gridDim.x * 256u;
     h[tid] = 0;
     __syncthreads();
-    for (uint64_t r = (uint64_t)blockIdx.x * 256u + tid; r < nrs; r += stride) {
-        const uint32_t n = cand_off[r + 1] - cand_off[r];
-        if (n) atomicAdd(&h[lbin_of(lens[r >> 1])], n);
-    }
+    lhist_count(h, nrs, cand_off, lens, stride);
     __syncthreads();
-    base[tid] = h[tid] ? atomicAdd(&bins[256 + tid], h[tid]) : 0u;
+    base[tid] = h[tid] ? atomicAdd(&bins[LBIN_ROWS * 256u + tid], h[tid]) : 0u;
     h[tid] = 0;
     __syncthreads();
     for (uint64_t r0 = (uint64_t)blockIdx.x * 256u; r0 < nrs; r0 += stride) {
@@ -2957,7 +3005,7 @@ struct Pipe {
     unsigned long long *counters = nullptr;
     uint32_t *u32c = nullptr;
     uint64_t *scan_tmp = nullptr;
-    uint32_t *lbins = nullptr;       /* [512]: candidates per read length, then each length's next place */
+    uint32_t *lbins = nullptr;       /* [LBIN_ROWS + 1][256]: candidates per read length, then each length's next place */
     uint32_t *perm = nullptr;        /* [cap]: verification order (CandSet::perm) */
 };
 
@@ -2984,7 +3032,7 @@ static int pipe_alloc_a(swm_ws *ws, uint64_t n, uint32_t seln, Pipe &P)
         (rc = ws->get(swm_ws::EPART, &P.epart, 2 * n)) ||
         (rc = ws->get(swm_ws::RPART, &P.rpart, (uint64_t)blocks_for(n, 128))) ||
         (rc = ws->get(swm_ws::HPART, &P.hpart, 2 * (uint64_t)HAM_GRID_MAX)) ||
-        (rc = ws->get(swm_ws::LBINS, &P.lbins, 512)) ||
+        (rc = ws->get(swm_ws::LBINS, &P.lbins, (LBIN_ROWS + 1) * 256u)) ||
         (rc = ws->get(swm_ws::SCAN_TMP, &P.scan_tmp, 2 * (uint64_t)blocks_for(nrs + 1, SCAN_TILE))))
         return rc;
     P.so.counters = P.counters;
