@@ -1372,27 +1372,37 @@ __global__ void __launch_bounds__(256) k_lscatter(uint32_t nrs, const uint32_t *
     base[tid] = h[tid] ? atomicAdd(&bins[LBIN_ROWS * 256u + tid], h[tid]) : 0u;
     h[tid] = 0;
     __syncthreads();
-    for (uint64_t r0 = (uint64_t)blockIdx.x * 256u; r0 < nrs; r0 += stride) {
-        const uint64_t r = r0 + tid;
-        uint32_t start = 0, n = 0, pos = 0;
-        if (r < nrs) {
-            start = cand_off[r];
-            n = cand_off[r + 1] - start;
+    for (uint64_t r00 = (uint64_t)blockIdx.x * 256u; r00 < nrs; r00 += LC_PER * stride) {
+        /* the fields of LC_PER items loaded together, as in the count */
+        uint32_t aq[LC_PER], cq[LC_PER], lq[LC_PER];
+#pragma unroll
+        for (int q = 0; q < LC_PER; q++) {
+            const uint64_t r = r00 + (uint64_t)q * stride + tid;
+            const uint64_t rr = r < nrs ? r : 0u;
+            aq[q] = cand_off[rr];
+            cq[q] = cand_off[rr + 1];
+            lq[q] = lens[rr >> 1];
+        }
+#pragma unroll
+        for (int q = 0; q < LC_PER; q++) {
+            const bool in = r00 + (uint64_t)q * stride + tid < nrs;
+            const uint32_t start = aq[q], n = in ? cq[q] - aq[q] : 0u;
+            uint32_t pos = 0;
             if (n) {
-                const uint32_t b = lbin_of(lens[r >> 1]);
+                const uint32_t b = lbin_of(lq[q]);
                 pos = base[b] + atomicAdd(&h[b], n);
             }
-        }
-        const bool own = n <= 16u;
-        if (own)
-            for (uint32_t j = 0; j < n; j++) perm[pos + j] = start + j;
-        uint64_t m = __ballot(!own);
-        while (m) {
-            const int k = __builtin_ctzll(m);
-            m &= m - 1ull;
-            const uint32_t pk = (uint32_t)__shfl((int)pos, k, 64), sk = (uint32_t)__shfl((int)start, k, 64);
-            const uint32_t nk = (uint32_t)__shfl((int)n, k, 64);
-            for (uint32_t j = lane; j < nk; j += 64u) perm[pk + j] = sk + j;
+            const bool own = n <= 16u;
+            if (own)
+                for (uint32_t j = 0; j < n; j++) perm[pos + j] = start + j;
+            uint64_t m = __ballot(!own);
+            while (m) {
+                const int k = __builtin_ctzll(m);
+                m &= m - 1ull;
+                const uint32_t pk = (uint32_t)__shfl((int)pos, k, 64), sk = (uint32_t)__shfl((int)start, k, 64);
+                const uint32_t nk = (uint32_t)__shfl((int)n, k, 64);
+                for (uint32_t j = lane; j < nk; j += 64u) perm[pk + j] = sk + j;
+            }
         }
     }
 }
