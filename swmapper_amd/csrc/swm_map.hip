@@ -2772,17 +2772,23 @@ __global__ void __launch_bounds__(SCAN_NT) k_scan_reduce(ScanJob j)
     if (threadIdx.x == 0) j.sums[(uint64_t)y * j.nb + blockIdx.x] = tot;
 }
 
+/* LDS index of tile element e: one pad word per 16, so that a thread's 16
+ * consecutive elements (17 t + q) do not all sit in one bank */
+#define SCAN_PAD(e) ((e) + ((e) >> 4))
+
 __global__ void __launch_bounds__(SCAN_NT) k_scan_down(ScanJob j)
 {
     __shared__ uint64_t s_w[SCAN_NT / 64];
     __shared__ uint32_t s_base[2];
+    __shared__ uint64_t s_v[SCAN_TILE + SCAN_TILE / 16];
     const uint32_t y = j.first + blockIdx.y;
-    const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_PER;
+    const uint32_t tid = threadIdx.x;
+    const uint64_t tile = (uint64_t)blockIdx.x * SCAN_TILE;
     const uint64_t *sums = j.sums + (uint64_t)y * j.nb;
     /* this tile's offset: the tile sums before it (tile 0: all of them,
      * for the total) */
     uint64_t pre = 0, all = 0;
-    for (uint32_t b = threadIdx.x; b < j.nb; b += SCAN_NT) {
+    for (uint32_t b = tid; b < j.nb; b += SCAN_NT) {
         const uint64_t x = sums[b];
         if (b < blockIdx.x) pre += x;
         all += x;
@@ -2790,44 +2796,59 @@ __global__ void __launch_bounds__(SCAN_NT) k_scan_down(ScanJob j)
     uint64_t tile_off, grand;
     (void)block_incl_scan<uint64_t, SCAN_NT>(pre, s_w, &tile_off);
     (void)block_incl_scan<uint64_t, SCAN_NT>(blockIdx.x == 0 ? all : 0ull, s_w, &grand);
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (blockIdx.x == 0 && tid == 0) {
         if (j.tot[y]) *j.tot[y] = grand;
         if (grand > j.cap[y]) *j.ovf = 1u;
     }
+    /* the tile in and out through LDS, thread t on elements q * SCAN_NT + t
+     * (coalesced; with each thread on 16 consecutive elements every wave
+     * access touched 64 lines); the heavy items are classified on the way
+     * in (heavy: a selected 12-mer with more than one chunk; over GIANT_POS
+     * positions to the giant list) */
+    const bool cls = y == 0 && j.cls_nsel;
+    uint32_t hm = 0, gm = 0;
+#pragma unroll
+    for (int q = 0; q < SCAN_PER; q++) {
+        const uint32_t e = (uint32_t)q * SCAN_NT + tid;
+        const uint64_t g = tile + e;
+        const uint64_t x = g < j.n ? scan_in(j, y, g) : 0ull;
+        s_v[SCAN_PAD(e)] = x;
+        if (cls && g < j.cls_n && (j.cls_nsel[g] & NSEL_HEAVY)) {
+            if (x > GIANT_POS) gm |= 1u << q;
+            else hm |= 1u << q;
+        }
+    }
+    __syncthreads();
     uint64_t v[SCAN_PER];
     uint64_t t = 0;
 #pragma unroll
     for (int q = 0; q < SCAN_PER; q++) {
-        v[q] = base + q < j.n ? scan_in(j, y, base + q) : 0ull;
+        v[q] = s_v[SCAN_PAD(tid * SCAN_PER + (uint32_t)q)];
         t += v[q];
     }
     uint64_t tot;
-    uint64_t run = block_incl_scan<uint64_t, SCAN_NT>(t, s_w, &tot) - t + tile_off;
+    uint64_t run = block_incl_scan<uint64_t, SCAN_NT>(t, s_w, &tot) - t + tile_off;   /* its barriers end the reads */
 #pragma unroll
     for (int q = 0; q < SCAN_PER; q++) {
-        if (base + q < j.n) {
-            if (y == 0) j.out0[base + q] = run;
-            else j.out1[base + q] = (uint32_t)run;
-        }
+        s_v[SCAN_PAD(tid * SCAN_PER + (uint32_t)q)] = run;
         run += v[q];
     }
-    if (y == 0 && j.cls_nsel) {
-        /* heavy items (a selected 12-mer with more than one chunk) to the
-         * seed-heap lists, those over GIANT_POS positions to their own:
-         * one atomic per list per workgroup */
-        uint32_t hm = 0, gm = 0;
+    __syncthreads();
 #pragma unroll
-        for (int q = 0; q < SCAN_PER; q++) {
-            const uint64_t g = base + q;
-            if (g < j.cls_n && (j.cls_nsel[g] & NSEL_HEAVY)) {
-                if (v[q] > GIANT_POS) gm |= 1u << q;
-                else hm |= 1u << q;
-            }
+    for (int q = 0; q < SCAN_PER; q++) {
+        const uint32_t e = (uint32_t)q * SCAN_NT + tid;
+        const uint64_t g = tile + e;
+        if (g < j.n) {
+            if (y == 0) j.out0[g] = s_v[SCAN_PAD(e)];
+            else j.out1[g] = (uint32_t)s_v[SCAN_PAD(e)];
         }
+    }
+    if (cls) {
+        /* one atomic per list per workgroup */
         const uint64_t c = (uint64_t)__popc(hm) | ((uint64_t)__popc(gm) << 32);
         uint64_t ctot;
         const uint64_t cex = block_incl_scan<uint64_t, SCAN_NT>(c, s_w, &ctot) - c;
-        if (threadIdx.x == 0) {
+        if (tid == 0) {
             s_base[0] = (uint32_t)ctot ? atomicAdd(j.cls_heavy_count, (uint32_t)ctot) : 0u;
             s_base[1] = (uint32_t)(ctot >> 32) ? atomicAdd(j.cls_giant_count, (uint32_t)(ctot >> 32)) : 0u;
         }
@@ -2835,8 +2856,9 @@ __global__ void __launch_bounds__(SCAN_NT) k_scan_down(ScanJob j)
         uint32_t ph = s_base[0] + (uint32_t)cex, pg = s_base[1] + (uint32_t)(cex >> 32);
 #pragma unroll
         for (int q = 0; q < SCAN_PER; q++) {
-            if ((hm >> q) & 1u) j.cls_heavy[ph++] = (uint32_t)(base + q);
-            if ((gm >> q) & 1u) j.cls_giant[pg++] = (uint32_t)(base + q);
+            const uint32_t g = (uint32_t)(tile + (uint64_t)q * SCAN_NT + tid);
+            if ((hm >> q) & 1u) j.cls_heavy[ph++] = g;
+            if ((gm >> q) & 1u) j.cls_giant[pg++] = g;
         }
     }
 }
