@@ -2751,9 +2751,25 @@ struct ScanJob {
     uint32_t *cls_heavy, *cls_heavy_count, *cls_giant, *cls_giant_count;
 };
 
-__device__ __forceinline__ uint64_t scan_in(const ScanJob &j, uint32_t y, uint64_t g)
+/* The SCAN_PER values of thread t in a tile (elements q * SCAN_NT + t),
+ * loaded unconditionally (a place past the end reads element 0 and counts
+ * 0), so that all of them are in flight at once: a bounds test per load had
+ * made each load wait for the one before. */
+template <typename T>
+__device__ __forceinline__ void scan_tile_load(const T *__restrict__ in, uint64_t n, uint64_t tile,
+                                               uint64_t (&x)[SCAN_PER])
 {
-    return y == 0 ? j.in0[g] : (uint64_t)j.in1[g];
+    T raw[SCAN_PER];
+#pragma unroll
+    for (int q = 0; q < SCAN_PER; q++) {
+        const uint64_t g = tile + (uint64_t)q * SCAN_NT + threadIdx.x;
+        raw[q] = in[g < n ? g : 0u];
+    }
+#pragma unroll
+    for (int q = 0; q < SCAN_PER; q++) {
+        const uint64_t g = tile + (uint64_t)q * SCAN_NT + threadIdx.x;
+        x[q] = g < n ? (uint64_t)raw[q] : 0ull;
+    }
 }
 
 __global__ void __launch_bounds__(SCAN_NT) k_scan_reduce(ScanJob j)
@@ -2761,12 +2777,12 @@ __global__ void __launch_bounds__(SCAN_NT) k_scan_reduce(ScanJob j)
     __shared__ uint64_t s_w[SCAN_NT / 64];
     const uint32_t y = j.first + blockIdx.y;
     const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE;
+    uint64_t x[SCAN_PER];
+    if (y == 0) scan_tile_load(j.in0, j.n, base, x);
+    else scan_tile_load(j.in1, j.n, base, x);
     uint64_t v = 0;
 #pragma unroll
-    for (int q = 0; q < SCAN_PER; q++) {
-        const uint64_t g = base + threadIdx.x + (uint64_t)q * SCAN_NT;
-        if (g < j.n) v += scan_in(j, y, g);
-    }
+    for (int q = 0; q < SCAN_PER; q++) v += x[q];
     uint64_t tot;
     (void)block_incl_scan<uint64_t, SCAN_NT>(v, s_w, &tot);
     if (threadIdx.x == 0) j.sums[(uint64_t)y * j.nb + blockIdx.x] = tot;
@@ -2806,18 +2822,28 @@ __global__ void __launch_bounds__(SCAN_NT) k_scan_down(ScanJob j)
      * in (heavy: a selected 12-mer with more than one chunk; over GIANT_POS
      * positions to the giant list) */
     const bool cls = y == 0 && j.cls_nsel;
+    uint64_t xv[SCAN_PER];
+    if (y == 0) scan_tile_load(j.in0, j.n, tile, xv);
+    else scan_tile_load(j.in1, j.n, tile, xv);
     uint32_t hm = 0, gm = 0;
+    if (cls) {
+        uint32_t ns[SCAN_PER];
 #pragma unroll
-    for (int q = 0; q < SCAN_PER; q++) {
-        const uint32_t e = (uint32_t)q * SCAN_NT + tid;
-        const uint64_t g = tile + e;
-        const uint64_t x = g < j.n ? scan_in(j, y, g) : 0ull;
-        s_v[SCAN_PAD(e)] = x;
-        if (cls && g < j.cls_n && (j.cls_nsel[g] & NSEL_HEAVY)) {
-            if (x > GIANT_POS) gm |= 1u << q;
-            else hm |= 1u << q;
+        for (int q = 0; q < SCAN_PER; q++) {
+            const uint64_t g = tile + (uint64_t)q * SCAN_NT + tid;
+            ns[q] = j.cls_nsel[g < j.cls_n ? g : 0u];
+        }
+#pragma unroll
+        for (int q = 0; q < SCAN_PER; q++) {
+            const uint64_t g = tile + (uint64_t)q * SCAN_NT + tid;
+            if (g < j.cls_n && (ns[q] & NSEL_HEAVY)) {
+                if (xv[q] > GIANT_POS) gm |= 1u << q;
+                else hm |= 1u << q;
+            }
         }
     }
+#pragma unroll
+    for (int q = 0; q < SCAN_PER; q++) s_v[SCAN_PAD((uint32_t)q * SCAN_NT + tid)] = xv[q];
     __syncthreads();
     uint64_t v[SCAN_PER];
     uint64_t t = 0;
