@@ -284,18 +284,39 @@ __global__ void __launch_bounds__(128) k_seed(const char *__restrict__ reads, ui
         const char *src = reads + (uint64_t)r0 * stride;
         const uint64_t nbytes = (uint64_t)nr * stride;
         const bool words = (((uintptr_t)src) & 3u) == 0 && (stride & 3u) == 0;
-        for (uint32_t qd = threadIdx.x; qd < nr * 64u; qd += blockDim.x) {
+        const uint32_t nq = nr * 64u;
+        if (words) {
+            /* eight words per thread loaded before any is packed (one load
+             * per iteration had put a full memory round trip between the 32
+             * words of a thread); a quad past the stride reads the batch's
+             * first word and packs as 0 */
+            for (uint32_t q0 = threadIdx.x; q0 < nq; q0 += 8u * blockDim.x) {
+                uint32_t raw[8];
+#pragma unroll
+                for (int u = 0; u < 8; u++) {
+                    const uint32_t qd = q0 + (uint32_t)u * blockDim.x;
+                    const uint32_t kq = qd & 63u;
+                    const bool ok = qd < nq && 4u * kq < stride;
+                    const uint64_t at = ok ? (uint64_t)(qd >> 6) * stride + 4u * kq : 0u;
+                    raw[u] = *reinterpret_cast<const uint32_t *>(src + at);
+                }
+#pragma unroll
+                for (int u = 0; u < 8; u++) {
+                    const uint32_t qd = q0 + (uint32_t)u * blockDim.x;
+                    const uint32_t kq = qd & 63u;
+                    if (qd < nq) pack_quad(4u * kq < stride ? raw[u] : 0u, &s_q[qd], &s_v[qd]);
+                }
+            }
+        }
+        /* unaligned reads or stride: byte loads */
+        for (uint32_t qd = threadIdx.x; !words && qd < nq; qd += blockDim.x) {
             const uint32_t lr = qd >> 6, kq = qd & 63u;
             uint32_t raw = 0;
             if (4u * kq < stride) {
                 const uint64_t at = (uint64_t)lr * stride + 4u * kq;
-                if (words) {
-                    raw = *reinterpret_cast<const uint32_t *>(src + at);
-                } else {
 #pragma unroll
-                    for (uint32_t j = 0; j < 4; j++)
-                        if (4u * kq + j < stride && at + j < nbytes) raw |= (uint32_t)(uint8_t)src[at + j] << (8 * j);
-                }
+                for (uint32_t j = 0; j < 4; j++)
+                    if (4u * kq + j < stride && at + j < nbytes) raw |= (uint32_t)(uint8_t)src[at + j] << (8 * j);
             }
             pack_quad(raw, &s_q[qd], &s_v[qd]);
         }
