@@ -51,12 +51,13 @@ __device__ __forceinline__ uint32_t shl32(uint32_t x, int s) { return (s < 0 || 
  * pairs after the file, so windows past the end read 0; bases before 0 read 0. */
 __device__ __forceinline__ uint32_t genome_word(const uint32_t *__restrict__ b2, int64_t g, int plane)
 {
-    if (g <= -32) return 0u;
-    if (g < 0) return shl32(b2[plane], (int)(-g));
-    uint64_t wi = (uint64_t)g >> 5;
-    uint32_t sh = (uint32_t)g & 31u;
-    uint32_t a = b2[2 * wi + plane], b = b2[2 * wi + 2 + plane];
-    return __builtin_amdgcn_alignbit(b, a, sh);
+    /* without branches, so that a caller's loads issue together: a start
+     * before the genome reads word 0 (the word a negative start shifts) */
+    const uint64_t gc = g < 0 ? 0u : (uint64_t)g;
+    const uint64_t wi = gc >> 5;
+    const uint32_t a = b2[2 * wi + plane], b = b2[2 * wi + 2 + plane];
+    const uint32_t in = __builtin_amdgcn_alignbit(b, a, (uint32_t)gc & 31u);
+    return g >= 0 ? in : (g <= -32 ? 0u : shl32(a, (int)(-g)));
 }
 
 /* pos_start / adjust_ref_len (mapper_cpe.c:1403-1435). */
@@ -2130,7 +2131,7 @@ __device__ __forceinline__ void close_run(char c, int n, uint32_t &ms_acc, uint3
 /* RUNS: also store the merged runs (records path); a separate instance, so
  * the plain pipeline keeps the smaller register footprint. */
 template <bool RUNS>
-__global__ void __launch_bounds__(64) k_trace(const uint32_t *__restrict__ trace, const uint32_t *__restrict__ trace_count,
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_trace(const uint32_t *__restrict__ trace, const uint32_t *__restrict__ trace_count,
                                               const uint32_t *__restrict__ lens, const uint32_t *__restrict__ words,
                                               const uint32_t *__restrict__ b2, uint64_t ref_length, uint32_t e,
                                               CandSet c)
@@ -2167,15 +2168,25 @@ __global__ void __launch_bounds__(64) k_trace(const uint32_t *__restrict__ trace
             hpr[(S)] = hp;                                                                 \
         }
 
-        /* forward: checkpoints at chunk starts */
+        /* forward: checkpoints at chunk starts; each chunk's words are
+         * loaded during the previous chunk's steps (its low window words are
+         * the previous high ones; the last prefetch reads one word past the
+         * read's window, inside the genome's zero padding) */
         uint32_t vp = 0, vn = 0;
         uint32_t d0r[32], hpr[32];
+        uint32_t fr0 = w[0], fr1 = w[1];
+        uint32_t fl0 = genome_word(b2, (int64_t)st, 0), fh0 = genome_word(b2, (int64_t)st, 1);
+        uint32_t fl1 = genome_word(b2, (int64_t)st + 32, 0), fh1 = genome_word(b2, (int64_t)st + 32, 1);
         for (uint32_t cc = 0; cc < nfull; cc++) {
             ck[2 * cc] = vp;
             ck[2 * cc + 1] = vn;
-            const uint32_t r0 = w[2 * cc], r1 = w[2 * cc + 1];
-            const uint32_t l0 = genome_word(b2, (int64_t)st + 32 * cc, 0), l1 = genome_word(b2, (int64_t)st + 32 * cc + 32, 0);
-            const uint32_t h0 = genome_word(b2, (int64_t)st + 32 * cc, 1), h1 = genome_word(b2, (int64_t)st + 32 * cc + 32, 1);
+            const uint32_t r0 = fr0, r1 = fr1, l0 = fl0, l1 = fl1, h0 = fh0, h1 = fh1;
+            fr0 = w[2 * cc + 2];
+            fr1 = w[2 * cc + 3];
+            fl0 = l1;
+            fh0 = h1;
+            fl1 = genome_word(b2, (int64_t)st + 32 * cc + 64, 0);
+            fh1 = genome_word(b2, (int64_t)st + 32 * cc + 64, 1);
 #pragma unroll
             for (int s = 0; s < 16; s++) TR_STEP(s, r0)
 #pragma unroll
