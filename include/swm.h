@@ -82,6 +82,9 @@ typedef struct swm_stats {
     /* work-queue load: heavy (read, strand) items of the seed heap, batches
      * sorted by the large-batch kernel, reads emitted one wave each */
     uint64_t n_heavy_items, n_large_batches, n_heavy_reads;
+    /* of those, the items and reads over the giant thresholds
+     * (SWM_OPT_GIANT_POS / SWM_OPT_GIANT_CANDS), which are taken first */
+    uint64_t n_giant_items, n_giant_reads;
 } swm_stats_t;
 
 typedef struct swm_ctx swm_ctx_t;
@@ -200,6 +203,9 @@ int swm_sam_close(swm_sam_t *w);
 #define SWM_OPT_EMIT_HEAVY_WAVES 6  /* cooperative emission waves per CU (1..64, default 32) */
 #define SWM_OPT_HISTOGRAM 7         /* 1: also collect the per-read candidate histogram (stats) */
 #define SWM_OPT_MYERS_PACKED 8      /* 1 (default): for e <= 7, two candidates per lane in 16-bit halves */
+#define SWM_OPT_GIANT_POS 9         /* heavy plan items with more positions go first (1..2^24, default 16384) */
+#define SWM_OPT_GIANT_CANDS 10      /* heavy emission reads with more candidates go first (1..2^24, default 16384) */
+#define SWM_N_OPTIONS 10
 
 /* Every form and tunable gives the same results; they change only speed. */
 int swm_ctx_set_option(swm_ctx_t *ctx, int opt, int64_t value);
@@ -207,7 +213,7 @@ int swm_ctx_get_option(const swm_ctx_t *ctx, int opt, int64_t *value);
 
 /* Autotuner: times the pipeline on the given sample (host buffers, as
  * swm_map_batch) for each candidate setting of the tunables, one at a time,
- * keeps the fastest on ctx and writes every option's value to chosen[8]
+ * keeps the fastest on ctx and writes every option's value to chosen[10]
  * (indexed by SWM_OPT_* - 1) when chosen != NULL.  reps >= 1 timed runs per
  * setting (median). */
 int swm_autotune(swm_ctx_t *ctx, const char *reads, uint64_t stride, const uint32_t *lens, uint64_t n,

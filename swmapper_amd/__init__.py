@@ -53,7 +53,8 @@ class Stats(C.Structure):
                 ("ms_myers", C.c_double), ("ms_emit", C.c_double), ("ms_total", C.c_double),
                 ("ms_host_enqueue", C.c_double), ("ms_host_wait", C.c_double),
                 ("cand_hist", C.c_uint64 * 24), ("max_cands", C.c_uint64),
-                ("n_heavy_items", C.c_uint64), ("n_large_batches", C.c_uint64), ("n_heavy_reads", C.c_uint64)]
+                ("n_heavy_items", C.c_uint64), ("n_large_batches", C.c_uint64), ("n_heavy_reads", C.c_uint64),
+                ("n_giant_items", C.c_uint64), ("n_giant_reads", C.c_uint64)]
 
     def as_dict(self):
         d = {f: getattr(self, f) for f, _ in self._fields_}
@@ -81,11 +82,12 @@ class BatchInfo(C.Structure):
 OPT_WORK_QUEUE, OPT_GRAPH, OPT_HEAVY_CANDS, OPT_MYERS_BLOCK, OPT_PLAN_HEAVY_WAVES, OPT_EMIT_HEAVY_WAVES = range(1, 7)
 OPT_HISTOGRAM = 7
 OPT_MYERS_PACKED = 8
+OPT_GIANT_POS, OPT_GIANT_CANDS = 9, 10
 OPTION_NAMES = {OPT_WORK_QUEUE: "work_queue", OPT_GRAPH: "graph", OPT_HEAVY_CANDS: "heavy_cands",
                 OPT_MYERS_BLOCK: "myers_block", OPT_PLAN_HEAVY_WAVES: "plan_heavy_waves",
                 OPT_EMIT_HEAVY_WAVES: "emit_heavy_waves", OPT_HISTOGRAM: "histogram",
-                OPT_MYERS_PACKED: "myers_packed"}
-N_OPTIONS = 8
+                OPT_MYERS_PACKED: "myers_packed", OPT_GIANT_POS: "giant_pos", OPT_GIANT_CANDS: "giant_cands"}
+N_OPTIONS = 10
 
 
 _lib_handle = None

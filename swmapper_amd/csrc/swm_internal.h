@@ -53,6 +53,8 @@ struct swm_opts {
     uint32_t emit_heavy_waves = 32; /* waves per CU of the cooperative emission */
     uint32_t histogram = 0;         /* collect the per-read candidate histogram */
     uint32_t myers_packed = 1;      /* e <= 7: two candidates per lane in 16-bit halves */
+    uint32_t giant_pos = 16384;     /* heavy plan items with more positions are taken first */
+    uint32_t giant_cands = 16384;   /* heavy emission reads with more candidates are taken first */
 };
 
 struct swm_ctx {

@@ -467,8 +467,12 @@ __global__ void __launch_bounds__(128) k_seed(const char *__restrict__ reads, ui
 /* ------------------------------------------------------------ plan kernels */
 
 #define SMALL_BATCH 32
-#define GIANT_POS 16384      /* heavy plan items this large are taken first */
-#define GIANT_CANDS 16384    /* heavy emission reads this large are taken first */
+/* Default thresholds of the giant lists (SWM_OPT_GIANT_POS / _CANDS): heavy
+ * plan items with more positions, and heavy emission reads with more
+ * candidates, are taken first.  The wave priorities below stay keyed to
+ * these defaults. */
+#define GIANT_POS 16384
+#define GIANT_CANDS 16384
 
 struct PlanIO {
     const swm_sel_t *sel;
@@ -485,7 +489,7 @@ struct PlanIO {
     uint32_t *heavy;            /* (read, strand) items whose seeds need the seed heap */
     uint32_t *heavy_count;
     uint32_t *heavy_head;       /* work-queue head of k_plan_heavy */
-    uint32_t *giant;            /* heavy items with more than GIANT_POS positions: drained first */
+    uint32_t *giant;            /* heavy items over the giant threshold (positions): drained first */
     uint32_t *giant_count;
     uint2 *cdesc;               /* heavy items' chunk lists, at bd_off: (bucket offset, batch prefix | seed << 16) */
     struct HBatch *hbatch;      /* heavy items' batches, for k_plan_fetch */
@@ -2347,7 +2351,8 @@ __device__ __forceinline__ void record_partial(uint32_t count, uint32_t *part, u
 __global__ void __launch_bounds__(128) k_emit(uint32_t nreads, uint32_t e, const uint32_t *__restrict__ cand_off,
                                               CandSet c, swm_read_result_t *__restrict__ out,
                                               uint32_t *__restrict__ rpart, uint32_t heavy_cands,
-                                              uint32_t *__restrict__ heavy, uint32_t *__restrict__ heavy_count)
+                                              uint32_t giant_cands, uint32_t *__restrict__ heavy,
+                                              uint32_t *__restrict__ heavy_count)
 {
     __shared__ uint32_t s_tmp[2], s_sum;
     if (*c.ovf) return;
@@ -2359,7 +2364,7 @@ __global__ void __launch_bounds__(128) k_emit(uint32_t nreads, uint32_t e, const
     /* the largest reads first (heavy[0] .. [giant count)), the rest after
      * them from the list's other end: k_emit_heavy's waves start on the
      * slowest serial walks at once */
-    const bool giant = big && nc > GIANT_CANDS;
+    const bool giant = big && nc > giant_cands;
     block_append(big && !giant, rd, heavy + nreads, heavy_count + 1, s_tmp);
     block_append(giant, rd, heavy, heavy_count, s_tmp);
     Emit em;
@@ -2781,6 +2786,7 @@ struct ScanJob {
     const uint32_t *cls_nsel;
     uint32_t cls_n;
     uint32_t *cls_heavy, *cls_heavy_count, *cls_giant, *cls_giant_count;
+    uint32_t giant_pos;      /* items with more positions go to the giant list */
 };
 
 /* The SCAN_PER values of thread t in a tile (elements q * SCAN_NT + t),
@@ -2851,7 +2857,7 @@ __global__ void __launch_bounds__(SCAN_NT) k_scan_down(ScanJob j)
     /* the tile in and out through LDS, thread t on elements q * SCAN_NT + t
      * (coalesced; with each thread on 16 consecutive elements every wave
      * access touched 64 lines); the heavy items are classified on the way
-     * in (heavy: a selected 12-mer with more than one chunk; over GIANT_POS
+     * in (heavy: a selected 12-mer with more than one chunk; over giant_pos
      * positions to the giant list) */
     const bool cls = y == 0 && j.cls_nsel;
     uint64_t xv[SCAN_PER];
@@ -2869,7 +2875,7 @@ __global__ void __launch_bounds__(SCAN_NT) k_scan_down(ScanJob j)
         for (int q = 0; q < SCAN_PER; q++) {
             const uint64_t g = tile + (uint64_t)q * SCAN_NT + tid;
             if (g < j.cls_n && (ns[q] & NSEL_HEAVY)) {
-                if (xv[q] > GIANT_POS) gm |= 1u << q;
+                if (xv[q] > j.giant_pos) gm |= 1u << q;
                 else hm |= 1u << q;
             }
         }
@@ -2975,7 +2981,8 @@ struct swm_slot {
     struct GraphEnt {
         bool used = false;
         const void *reads = nullptr, *lens = nullptr, *out = nullptr;
-        uint64_t n = 0, stride = 0, gen = 0, sig = 0, tick = 0;
+        uint64_t n = 0, stride = 0, gen = 0, tick = 0;
+        swm_opts opt{};
         uint32_t e = 0;
         hipGraphExec_t exec = nullptr;
     };
@@ -3202,6 +3209,7 @@ static int enq_a(const swm_ctx_t *ctx, Pipe &P, const char *d_reads, uint64_t st
         j.cls_n = (uint32_t)P.nrs;
         j.cls_heavy = P.pio.heavy; j.cls_heavy_count = P.pio.heavy_count;
         j.cls_giant = P.pio.giant; j.cls_giant_count = P.pio.giant_count;
+        j.giant_pos = ctx->opt.giant_pos;
     }
     scan_launch(j, 2, st);
     SWM_HIP_CHECK(hipGetLastError());
@@ -3282,7 +3290,7 @@ static int enq_c(const swm_ctx_t *ctx, Pipe &P, const uint32_t *d_lens, uint32_t
     (P.cs.runs ? k_trace<true> : k_trace<false>)<<<ctx->n_cu * 32, 64, 0, st>>>(P.trace, P.u32c + 1, d_lens, P.so.words, ctx->d_base_2bits, ctx->ref_length, e,
                                           P.cs);
     k_emit<<<blocks_for(P.n, 128), 128, 0, st>>>((uint32_t)P.n, e, P.cand_off, P.cs, d_out, P.rpart,
-                                                 heavy_cands, P.heavy_e, P.u32c + 10);
+                                                 heavy_cands, o.giant_cands, P.heavy_e, P.u32c + 10);
     if (o.work_queue)
         k_emit_heavy<<<ctx->n_cu * o.emit_heavy_waves, 64, 0, st>>>(P.heavy_e, P.u32c + 10, (uint32_t)P.n, e,
                                                                    P.cand_off, P.cs, P.epart, P.counters + 1);
@@ -3306,6 +3314,8 @@ static void stats_from_counters(swm_stats_t *s, uint64_t n, const unsigned long 
     s->n_large_batches = u[0];
     s->n_heavy_items = u[3] + u2[1];
     s->n_heavy_reads = u2[2] + u2[3];
+    s->n_giant_items = u2[1];
+    s->n_giant_reads = u2[2];
 }
 
 /* The per-read candidate histogram of the run just finished (stats). */
@@ -3461,12 +3471,7 @@ static int records_out(const swm_ctx_t *ctx, swm_slot *sl, Pipe &P, const uint32
     return SWM_OK;
 }
 
-static uint64_t opt_sig(const swm_opts &o)
-{
-    return (uint64_t)o.work_queue | ((uint64_t)o.heavy_cands << 1) | ((uint64_t)o.myers_block << 17) |
-           ((uint64_t)o.plan_heavy_waves << 27) | ((uint64_t)o.emit_heavy_waves << 37) |
-           ((uint64_t)o.myers_packed << 47) | ((uint64_t)o.histogram << 48);
-}
+static bool same_opts(const swm_opts &a, const swm_opts &b) { return memcmp(&a, &b, sizeof(swm_opts)) == 0; }
 
 /* Fixed-capacity form: the whole lookup -> filter -> verify -> emit pipeline
  * captured once per (buffers, n, e) into a hipGraph and replayed with no host
@@ -3478,11 +3483,10 @@ static int run_graph(swm_ctx_t *ctx, swm_slot *sl, const char *d_reads, uint64_t
 {
     if (n == 0 || sl->r_cand == 0) return run_synced(ctx, sl, d_reads, stride, d_lens, n, e, d_out, h_out, n_out);
     hipStream_t st = sl->stream;
-    const uint64_t sig = opt_sig(ctx->opt);
     swm_slot::GraphEnt *ent = nullptr;
     for (auto &x : sl->g)
         if (x.used && x.reads == d_reads && x.lens == d_lens && x.out == d_out && x.n == n && x.stride == stride &&
-            x.e == e && x.sig == sig)
+            x.e == e && same_opts(x.opt, ctx->opt))
             ent = &x;
     const uint32_t seln = 4 * ((e + 1) < 16 ? (e + 1) : 16);
     Pipe P;
@@ -3529,7 +3533,7 @@ static int run_graph(swm_ctx_t *ctx, swm_slot *sl, const char *d_reads, uint64_t
         if (ie != hipSuccess) { ent->exec = nullptr; return SWM_EDEVICE; }
         ent->used = true;
         ent->reads = d_reads; ent->lens = d_lens; ent->out = d_out;
-        ent->n = n; ent->stride = stride; ent->e = e; ent->sig = sig; ent->gen = sl->ws.gen;
+        ent->n = n; ent->stride = stride; ent->e = e; ent->opt = ctx->opt; ent->gen = sl->ws.gen;
     } else if ((rc = pipe_alloc_a(&sl->ws, n, seln, P))) {
         return rc;   /* no growth here (the generation matched): only the counters' address is needed */
     }
@@ -3759,6 +3763,8 @@ extern "C" int swm_ctx_set_option(swm_ctx_t *ctx, int opt, int64_t value)
     case SWM_OPT_EMIT_HEAVY_WAVES: if (value < 1 || value > 64) return SWM_EINVAL; o.emit_heavy_waves = (uint32_t)value; break;
     case SWM_OPT_HISTOGRAM: if (value != 0 && value != 1) return SWM_EINVAL; o.histogram = (uint32_t)value; break;
     case SWM_OPT_MYERS_PACKED: if (value != 0 && value != 1) return SWM_EINVAL; o.myers_packed = (uint32_t)value; break;
+    case SWM_OPT_GIANT_POS: if (value < 1 || value > (1 << 24)) return SWM_EINVAL; o.giant_pos = (uint32_t)value; break;
+    case SWM_OPT_GIANT_CANDS: if (value < 1 || value > (1 << 24)) return SWM_EINVAL; o.giant_cands = (uint32_t)value; break;
     default: return SWM_EINVAL;
     }
     return SWM_OK;
@@ -3777,6 +3783,8 @@ extern "C" int swm_ctx_get_option(const swm_ctx_t *ctx, int opt, int64_t *value)
     case SWM_OPT_EMIT_HEAVY_WAVES: *value = o.emit_heavy_waves; break;
     case SWM_OPT_HISTOGRAM: *value = o.histogram; break;
     case SWM_OPT_MYERS_PACKED: *value = o.myers_packed; break;
+    case SWM_OPT_GIANT_POS: *value = o.giant_pos; break;
+    case SWM_OPT_GIANT_CANDS: *value = o.giant_cands; break;
     default: return SWM_EINVAL;
     }
     return SWM_OK;

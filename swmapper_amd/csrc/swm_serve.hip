@@ -401,7 +401,7 @@ extern "C" int swm_autotune(swm_ctx_t *ctx, const char *reads, uint64_t stride, 
         if (!rc) rc = set_opt(ctx, kb.opt, best_v);
     }
     if (!rc && chosen)
-        for (int o = 1; o <= 8; o++) (void)swm_ctx_get_option(ctx, o, &chosen[o - 1]);
+        for (int o = 1; o <= SWM_N_OPTIONS; o++) (void)swm_ctx_get_option(ctx, o, &chosen[o - 1]);
     (void)hipFree(d_reads);
     (void)hipFree(d_lens);
     (void)hipFree(d_out);
