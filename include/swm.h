@@ -31,6 +31,7 @@ extern "C" {
 #define SWM_ENOMEM -3      /* host or device allocation failed */
 #define SWM_EDEVICE -4     /* HIP runtime error or no device */
 #define SWM_EBUSY -5       /* serving loop: no admission possible right now */
+#define SWM_ENOSPC -6      /* no-allocation entry: the batch does not fit the fixed scratch */
 
 /* Per-read result: the four per-read outputs of the mapping path.
  * count = records emitted over both strands (the reference's cigar records,
@@ -135,6 +136,24 @@ int swm_map_batch(swm_ctx_t *ctx, const char *reads, uint64_t stride, const uint
 int swm_map_batch_device(swm_ctx_t *ctx, const char *d_reads, uint64_t stride,
                          const uint32_t *d_lens, uint64_t n, uint32_t e,
                          swm_read_result_t *d_out);
+
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (tight serving loops, harnesses that forbid hipMalloc).  The same
+ * mapping as swm_map_batch_device, with no allocation of any kind: its
+ * intermediate buffers are carved from one static device array of
+ * SWM_NOALLOC_SCRATCH_BYTES per GPU (held from the first context on that
+ * GPU until the process ends).  n <= SWM_NOALLOC_MAX_READS; a batch whose
+ * seed positions, bucket chunks or candidates exceed the scratch's share
+ * returns SWM_ENOSPC with d_out unspecified (split it and call again).
+ * Every call on one GPU uses that GPU's one array: calls that overlap on a
+ * GPU (any contexts, any threads) race and must be serialised by the
+ * caller.  Works in the form set by SWM_OPT_WORK_QUEUE; SWM_OPT_GRAPH and
+ * SWM_OPT_HISTOGRAM are ignored. */
+#define SWM_NOALLOC_SCRATCH_BYTES (64ull << 20)
+#define SWM_NOALLOC_MAX_READS 8192u
+int swm_map_batch_device_noalloc(swm_ctx_t *ctx, const char *d_reads, uint64_t stride,
+                                 const uint32_t *d_lens, uint64_t n, uint32_t e,
+                                 swm_read_result_t *d_out);
 
 /* Statistics of the last batch run on ctx. */
 int swm_last_stats(const swm_ctx_t *ctx, swm_stats_t *st);

@@ -21,7 +21,9 @@ LIB_PATH = os.path.join(_HERE, "libswm.so")
 
 SWM_OK = 0
 _ERRORS = {-1: "invalid argument", -2: "index/sequence file error", -3: "out of memory",
-           -4: "HIP device error", -5: "busy"}
+           -4: "HIP device error", -5: "busy", -6: "batch exceeds the no-allocation scratch"}
+SWM_ENOSPC = -6
+NOALLOC_MAX_READS = 8192
 
 RESULT_DTYPE = np.dtype([("pos", "<u4"), ("strand", "u1"), ("dist", "u1"), ("pad", "<u2"), ("count", "<u4")])
 # swm_record_t: one written record (include/swm.h)
@@ -111,6 +113,7 @@ def lib():
             "swm_ctx_destroy": ([vp], None),
             "swm_map_batch": ([vp, vp, u64, vp, u64, u32, vp], C.c_int),
             "swm_map_batch_device": ([vp, vp, u64, vp, u64, u32, vp], C.c_int),
+            "swm_map_batch_device_noalloc": ([vp, vp, u64, vp, u64, u32, vp], C.c_int),
             "swm_last_stats": ([vp, C.POINTER(Stats)], C.c_int),
             "swm_myers_device": ([vp, vp, vp, vp, vp, vp, u64, u32, vp, vp], C.c_int),
             "swm_synth_genome": ([vp, u64, u64, C.c_int, C.c_double], C.c_int),
@@ -294,6 +297,12 @@ class Mapper:
         """Device pointers (ints), e.g. torch tensors' data_ptr()."""
         _check("swm_map_batch_device", lib().swm_map_batch_device(self._h, C.c_void_p(d_reads), stride,
                                                                   C.c_void_p(d_lens), n, e, C.c_void_p(d_out)))
+
+    def map_device_noalloc(self, d_reads, stride, d_lens, n, e, d_out):
+        """swm_map_batch_device_noalloc: the same mapping with no allocation
+        (buffers carved from the library's fixed device scratch)."""
+        _check("swm_map_batch_device_noalloc", lib().swm_map_batch_device_noalloc(
+            self._h, C.c_void_p(d_reads), stride, C.c_void_p(d_lens), n, e, C.c_void_p(d_out)))
 
     def myers_device(self, d_read_words, d_lens, d_cread, d_wstart, d_rlen, n, e, d_err, d_loc):
         _check("swm_myers_device", lib().swm_myers_device(self._h, *(C.c_void_p(x) for x in (

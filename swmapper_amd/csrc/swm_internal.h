@@ -71,6 +71,7 @@ struct swm_ctx {
     swm_recipe_t *d_recipe = nullptr;   /* read-encoding recipe per read length 0..254 */
     swm_opts opt;
     swm_slot *slot0 = nullptr;          /* the slot of the synchronous entry points */
+    uint8_t *d_scratch = nullptr;       /* this device's no-allocation scratch (SWM_NOALLOC_SCRATCH_BYTES) */
 };
 
 swm_slot *swm_slot_create(swm_ctx_t *ctx);

@@ -3112,7 +3112,8 @@ struct Pipe {
  * [5] heavy compaction items, [6] verification order is the identity, [7] overflow, [8] heavy plan
  * batches, [9] giant plan items, [10] giant / [11] other heavy emission
  * reads */
-static int pipe_alloc_a(swm_ws *ws, uint64_t n, uint32_t seln, Pipe &P)
+template <class Alloc>
+static int pipe_alloc_a(Alloc *ws, uint64_t n, uint32_t seln, Pipe &P)
 {
     const uint64_t nrs = 2 * n;
     P.n = n;
@@ -3153,7 +3154,8 @@ static int pipe_alloc_a(swm_ws *ws, uint64_t n, uint32_t seln, Pipe &P)
     return SWM_OK;
 }
 
-static int pipe_alloc_b(swm_ws *ws, uint64_t cap_pos, uint64_t cap_chunk, Pipe &P)
+template <class Alloc>
+static int pipe_alloc_b(Alloc *ws, uint64_t cap_pos, uint64_t cap_chunk, Pipe &P)
 {
     int rc;
     if ((rc = ws->get(swm_ws::SURV, &P.pio.surv, cap_pos + 1)) ||
@@ -3165,7 +3167,8 @@ static int pipe_alloc_b(swm_ws *ws, uint64_t cap_pos, uint64_t cap_chunk, Pipe &
     return SWM_OK;
 }
 
-static int pipe_alloc_c(swm_ws *ws, uint64_t cap, Pipe &P)
+template <class Alloc>
+static int pipe_alloc_c(Alloc *ws, uint64_t cap, Pipe &P)
 {
     int rc;
     if ((rc = ws->get(swm_ws::C_VAL, &P.cs.val, cap + 1)) || (rc = ws->get(swm_ws::C_RS, &P.cs.rs, cap + 1)) ||
@@ -3565,6 +3568,107 @@ static int run_graph(swm_ctx_t *ctx, swm_slot *sl, const char *d_reads, uint64_t
     return SWM_OK;
 }
 
+/* ------------------------------------------ no-allocation form (scratch) */
+
+/* The device scratch of swm_map_batch_device_noalloc: one array per device
+ * (the code object's own), held until the process ends.  Host code never
+ * passes the array's name to a kernel (on the host it is only a stand-in);
+ * each device's address comes from hipGetSymbolAddress, once per device,
+ * at context creation. */
+static __device__ __align__(256) uint8_t scratch[SWM_NOALLOC_SCRATCH_BYTES];
+
+#define SCRATCH_MAX_DEVICES 64
+static std::mutex g_scratch_mu;
+static uint8_t *g_scratch_ptr[SCRATCH_MAX_DEVICES];
+
+/* The scratch address on device dev, which must be current. */
+static int scratch_address(int dev, uint8_t **out)
+{
+    if (dev < 0 || dev >= SCRATCH_MAX_DEVICES) return SWM_EDEVICE;
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return SWM_EDEVICE;
+        g_scratch_ptr[dev] = (uint8_t *)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return SWM_OK;
+}
+
+/* Bump allocator over the scratch: every buffer 256-byte aligned, nothing
+ * freed (a run carves its buffers afresh). */
+struct ScratchArena {
+    uint8_t *base;
+    size_t size, used;
+    template <typename T>
+    int get(int, T **out, size_t n)
+    {
+        const size_t bytes = (n ? n : 1) * sizeof(T);
+        const size_t off = (used + 255) & ~(size_t)255;
+        if (off > size || bytes > size - off) return SWM_ENOSPC;
+        *out = reinterpret_cast<T *>(base + off);
+        used = off + bytes;
+        return SWM_OK;
+    }
+};
+
+/* What is left after the per-read arrays goes to the capacities: 60% to
+ * bucket chunks (56 bytes each: descriptors, large-batch and heavy-chunk
+ * lists, heavy batch records), 25% to positions (4 bytes), 15% to
+ * candidates (36 bytes: nine per-candidate arrays).  A batch over any of
+ * them stands down on the device (the fixed-capacity checks of the graph
+ * form) and the call reports SWM_ENOSPC. */
+static void scratch_caps(size_t left, uint64_t *cap_pos, uint64_t *cap_chunk, uint64_t *cap_cand)
+{
+    const size_t slack = 16 * 256 + 256;   /* alignment of 14 buffers and their one extra element */
+    const uint64_t r = left > slack ? left - slack : 0;
+    *cap_chunk = r * 6 / 10 / 56;
+    *cap_pos = r / 4 / 4;
+    *cap_cand = r * 15 / 100 / 36;
+}
+
+/* The fixed-capacity pipeline (the graph form's kernels, enqueued directly
+ * on the context's stream) over buffers carved from the scratch.  Nothing
+ * is allocated; the counters come back through the slot's page-locked
+ * staging, allocated at context creation.  Every buffer that must start at
+ * zero is cleared by the pipeline's own kernels (k_seed zeroes the
+ * counters), as in every graph replay. */
+static int run_noalloc(swm_ctx_t *ctx, const char *d_reads, uint64_t stride, const uint32_t *d_lens, uint64_t n,
+                       uint32_t e, swm_read_result_t *d_out)
+{
+    swm_slot *sl = ctx->slot0;
+    hipStream_t st = sl->stream;
+    memset(&sl->stats, 0, sizeof(sl->stats));
+    sl->stats.n_reads = n;
+    if (n == 0) return SWM_OK;
+    sl->t_entry = host_ms();
+    const uint32_t seln = 4 * ((e + 1) < 16 ? (e + 1) : 16);
+    ScratchArena ar{ctx->d_scratch, SWM_NOALLOC_SCRATCH_BYTES, 0};
+    Pipe P;
+    int rc;
+    if ((rc = pipe_alloc_a(&ar, n, seln, P))) return rc;
+    uint64_t cap_pos, cap_chunk, cap_cand;
+    scratch_caps(ar.size - ar.used, &cap_pos, &cap_chunk, &cap_cand);
+    if ((rc = pipe_alloc_b(&ar, cap_pos, cap_chunk, P)) || (rc = pipe_alloc_c(&ar, cap_cand, P))) return rc;
+    {
+        GateTurn turn(sl);
+        SWM_HIP_CHECK(hipEventRecord(sl->ev[0], st));
+        if ((rc = enq_a(ctx, P, d_reads, stride, d_lens, e, cap_pos, cap_chunk, st, nullptr)) ||
+            (rc = enq_b(ctx, P, cap_cand, st, nullptr)) || (rc = enq_c(ctx, P, d_lens, e, d_out, cap_cand, st, nullptr)))
+            return rc;
+        SWM_HIP_CHECK(hipEventRecord(sl->ev[5], st));
+    }
+    const unsigned long long *hc = reinterpret_cast<const unsigned long long *>(sl->h_small);
+    SWM_HIP_CHECK(hipMemcpyAsync(sl->h_small, P.counters, 10 * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+    if ((rc = final_sync(sl))) return rc;
+    if (reinterpret_cast<const uint32_t *>(hc + 4)[7]) return SWM_ENOSPC;   /* over a capacity: nothing written */
+    stats_from_counters(&sl->stats, n, hc);
+    float ms;
+    (void)hipEventElapsedTime(&ms, sl->ev[0], sl->ev[5]);
+    sl->stats.ms_total = ms;
+    return SWM_OK;
+}
+
 int swm_slot_run_device(swm_ctx_t *ctx, swm_slot *sl, const char *d_reads, uint64_t stride, const uint32_t *d_lens,
                         uint64_t n, uint32_t e, swm_read_result_t *d_out, bool graph)
 {
@@ -3685,6 +3789,7 @@ extern "C" int swm_ctx_create(uint32_t k, const uint32_t *hash_count, const uint
     swm_ctx_t *c = new (std::nothrow) swm_ctx_t();
     if (!c) return SWM_ENOMEM;
     if (hipGetDevice(&c->device) != hipSuccess) { delete c; return SWM_EDEVICE; }
+    if (scratch_address(c->device, &c->d_scratch) != SWM_OK) { delete c; return SWM_EDEVICE; }
     c->slot0 = swm_slot_create(c);
     if (!c->slot0) { swm_ctx_destroy(c); return SWM_EDEVICE; }
     c->k = k;
@@ -3815,6 +3920,19 @@ extern "C" int swm_map_batch_device(swm_ctx_t *ctx, const char *d_reads, uint64_
     if (n && (!d_reads || !d_lens || !d_out)) return SWM_EINVAL;
     if (hipSetDevice(ctx->device) != hipSuccess) return SWM_EDEVICE;
     return swm_slot_run_device(ctx, ctx->slot0, d_reads, stride, d_lens, n, e, d_out, ctx->opt.graph != 0);
+}
+
+extern "C" int swm_map_batch_device_noalloc(swm_ctx_t *ctx, const char *d_reads, uint64_t stride,
+                                            const uint32_t *d_lens, uint64_t n, uint32_t e,
+                                            swm_read_result_t *d_out)
+{
+    int rc = check_map_args(ctx, stride, e);
+    if (rc) return rc;
+    if (n && (!d_reads || !d_lens || !d_out)) return SWM_EINVAL;
+    if (n > SWM_NOALLOC_MAX_READS) return SWM_ENOSPC;
+    if (!ctx->d_scratch) return SWM_EDEVICE;
+    if (hipSetDevice(ctx->device) != hipSuccess) return SWM_EDEVICE;
+    return run_noalloc(ctx, d_reads, stride, d_lens, n, e, d_out);
 }
 
 extern "C" int swm_map_batch(swm_ctx_t *ctx, const char *reads, uint64_t stride, const uint32_t *lens, uint64_t n,

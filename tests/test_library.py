@@ -33,6 +33,7 @@ def test_argument_validation_without_gpu():
     reads = np.zeros((4, 256), np.uint8)
     lens = np.full(4, 150, np.uint32)
     assert L.swm_map_batch(None, reads.ctypes.data, 256, lens.ctypes.data, 4, 5, out.ctypes.data) == -1
+    assert L.swm_map_batch_device_noalloc(None, None, 256, None, 4, 5, None) == -1
     h = C.c_void_p()
     assert L.swm_index_load(b"/nonexistent/prefix", 12, C.byref(h)) == -2
     assert L.swm_index_load(b"/nonexistent/prefix", 1, C.byref(h)) == -1
