@@ -50,21 +50,6 @@ def traffic_rate(traffic, ms):
     gbs = traffic["bytes"] / (ms / 1e3) / 1e9
     return {"gbs": round(gbs, 1), "frac_of_peak": round(gbs / HBM_PEAK_GBS, 4)}
 
-CONFIGS = {
-    "repeat5": dict(n=256_000_000, kind=1, frac=0.05, reads=1_000_000, lmin=150, lmax=150, err=(0.025, 0.025), e=5,
-                    desc="256 Mbp uniform + 5% repeat families (300 bp x 500 copies, 10% div), 12-mer index; "
-                         "1M x 150 bp reads, 2% sub + 0.5% 1-3 bp indel; e=5"),
-    "small": dict(n=4_000_000, kind=0, frac=0.0, reads=100_000, lmin=150, lmax=150, err=(0.025, 0.025), e=5,
-                  desc="4 Mbp uniform, 12-mer index; 100k x 150 bp reads, 2% sub + 0.5% indel; e=5"),
-    "uneven": dict(n=256_000_000, kind=1, frac=0.05, reads=1_000_000, lmin=50, lmax=250, err=(0.01, 0.04), e=5,
-                   desc="256 Mbp + 5% repeat families, 12-mer index; 1M reads, length U[50,250], error U[1%,4%] "
-                        "(80% sub / 20% 1-3 bp indel)"),
-    "skew": dict(n=256_000_000, kind=2, frac=0.30, reads=500_000, lmin=150, lmax=150, err=(0.025, 0.025), e=5,
-                 desc="256 Mbp + 30% tandem repeats (period U[2,50]), 12-mer index; 500k x 150 bp reads, "
-                      "2% sub + 0.5% indel; e=5"),
-}
-
-
 FORMS = {
     # one launch per stage, one thread per (read, strand) for every item
     "plain": {1: 0, 2: 0},
@@ -256,7 +241,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="repeat5", choices=sorted(CONFIGS))
+    ap.add_argument("--config", default="repeat5", choices=("repeat5", "small", "uneven", "skew"))
     ap.add_argument("--e", type=int, default=None, help="override the configuration's error bound")
     ap.add_argument("--forms", default="plain,queue,graph", help="pipeline forms to time (plain, queue, graph)")
     ap.add_argument("--autotune", action="store_true", help="autotune the tunables on a 100k-read sample first")
@@ -287,6 +272,7 @@ def main():
     if int(os.environ.get("GPU_MAX_HW_QUEUES", "4")) <= 4 and 1 <= args.hw_queues <= 32:
         os.environ["GPU_MAX_HW_QUEUES"] = str(args.hw_queues)
     import swmapper_amd as swm
+    from swmapper_amd.configs import CONFIGS, GENOME_SEED, READS_SEED
     cfg = dict(CONFIGS[args.config])
     if args.e is not None:
         cfg["e"] = args.e
@@ -299,7 +285,7 @@ def main():
     assert forms and all(f in FORMS for f in forms), forms
 
     t0 = time.time()
-    g = swm.synth_genome(cfg["n"], seed=20240601, kind=cfg["kind"], frac=cfg["frac"])
+    g = swm.synth_genome(cfg["n"], seed=GENOME_SEED, kind=cfg["kind"], frac=cfg["frac"])
     log(f"genome {cfg['n']} bp in {time.time() - t0:.1f}s")
     ix = swm.build_index(g, 12)
     ix2 = swm.build_index(g, 12)      # timed again with the device warm
@@ -316,7 +302,7 @@ def main():
                    "bases": cfg["n"], "positions": int(ix.positions.size)}
     reads, lens, _, _ = swm.synth_reads(g, cfg["reads"], len_min=cfg["lmin"], len_max=cfg["lmax"],
                                         err_min=cfg["err"][0], err_max=cfg["err"][1], indel_share=0.2,
-                                        seed=777, stride=256)
+                                        seed=READS_SEED, stride=256)
     log(f"reads {cfg['reads']} in {time.time() - t0:.1f}s")
     m = swm.Mapper.from_index(ix)
     n = cfg["reads"]

@@ -86,6 +86,9 @@ typedef struct swm_stats {
     /* of those, the items and reads over the giant thresholds
      * (SWM_OPT_GIANT_POS / SWM_OPT_GIANT_CANDS), which are taken first */
     uint64_t n_giant_items, n_giant_reads;
+    /* (read, strand) items whose selected 12-mers include a bucket that the
+     * reference's index holds out of ascending order */
+    uint64_t n_irregular_items;
 } swm_stats_t;
 
 typedef struct swm_ctx swm_ctx_t;

@@ -16,8 +16,20 @@
  *       reference's fixed 16-word lane slots and stale tails are not modelled).
  *   Q23 (index) slots of .positions / .pos_lseeds that the scatter never
  *       writes hold 0 (the reference leaves malloc memory there).
+ *
+ * ORC_MUTANT (0 unless set) builds a copy with one planted mistake, for the
+ * tests that show a known answer fails a plausible wrong restatement:
+ *   1 the textbook seed-heap sift-down (a lone left child compared)
+ *   2 the seed heap keyed by read-start-adjusted positions
+ *   3 a batch cut one chunk early (pos_size + 32 < 2048)
+ *   4 an insertion at the traceback's first step labelled 'I'
+ *   5 the traceback trying an insertion before a match
  */
 #include "swm_oracle.h"
+
+#ifndef ORC_MUTANT
+#define ORC_MUTANT 0
+#endif
 
 #include <ctype.h>
 #include <stdlib.h>
@@ -632,17 +644,34 @@ static void align_one(const orc_ref_t *ref, emit_t *em, const uint32_t *w, const
     char rchar[ORC_READ_LEN + 8];
 #define RB(rdi) ((w[(rdi) / 16] >> (2 * ((rdi) % 16))) & 3u)
     int j = 1;                                                              /* :379-410 first step */
+#if ORC_MUTANT == 5
+    if (bit_at(hpa[i], search_site)) {    /* planted mistake: an insertion preferred to a match */
+        i--; search_site++; pre_size = 1; pre_char = 'S'; sum_err++; start_location++;
+    } else
+#endif
     if (bit_at(d0a[i], search_site) && win_base(ref, win_start, match_site) == RB(i)) {
         i--; pre_size = 1; pre_char = 'M'; match_site--;
     } else if (!bit_at(d0a[i], search_site) && win_base(ref, win_start, match_site) != RB(i)) {
         i--; pre_size = 1; pre_char = 'S'; match_site--; sum_err++;
     } else if (bit_at(hpa[i], search_site)) {
+#if ORC_MUTANT == 4
+        i--; search_site++; pre_size = 1; pre_char = 'I'; sum_err++; start_location++;   /* planted mistake */
+#else
         i--; search_site++; pre_size = 1; pre_char = 'S'; sum_err++; start_location++;
+#endif
     } else {
         search_site--; pre_size = 1; pre_char = 'D'; match_site--; sum_err++; start_location--;
     }
     while (i >= 0) {                                                        /* :412-466 */
         if (sum_err == err) break;
+#if ORC_MUTANT == 5
+        if (bit_at(hpa[i], search_site)) {
+            i--; search_site++; sum_err++;
+            if (pre_char != 'I') { rsize[j] = pre_size; rchar[j++] = pre_char; pre_size = 1; pre_char = 'I'; }
+            else pre_size++;
+            start_location++;
+        } else
+#endif
         if (bit_at(d0a[i], search_site) && win_base(ref, win_start, match_site) == RB(i)) {
             i--; match_site--;
             if (pre_char != 'M') { rsize[j] = pre_size; rchar[j++] = pre_char; pre_size = 1; pre_char = 'M'; }
@@ -779,7 +808,11 @@ static void seeds_heap_pop_and_insert(sh_t *h, uint32_t *size, uint32_t pos, uin
     if (del_flag == 1) { h[1] = h[*size]; *size -= 1; }
     else h[1].loc = pos;
     uint32_t i = 1, ti;
+#if ORC_MUTANT == 1
+    while (i * 2 <= *size) {            /* planted mistake: the textbook sift-down */
+#else
     while (i * 2 < *size) {
+#endif
         ti = i * 2;
         if (i * 2 + 1 <= *size)
             if (h[i * 2].loc > h[i * 2 + 1].loc) ti = i * 2 + 1;
@@ -863,6 +896,20 @@ static void long_seeds_filter(const strand_ws_t *ws, uint32_t *positions, const 
     }
 }
 
+/* The seed heap's key of a fetch by 12-mer i: its raw last position (Q6:
+ * the reference keys the heap before adjust_ref_pos_rmdup shifts it). */
+static inline uint32_t heap_key(uint32_t k, const hc_t *info, uint64_t i, uint32_t raw)
+{
+#if ORC_MUTANT == 2
+    /* planted mistake: the key shifted to the read start */
+    uint64_t off = (uint64_t)info[i / ORC_SEED_STEP].index * (k + ORC_SEED_STEP - 1) + i % ORC_SEED_STEP;
+    return raw > off ? (uint32_t)(raw - off) : 0u;
+#else
+    (void)k; (void)info; (void)i;
+    return raw;
+#endif
+}
+
 /* alignment_rmdup (mapper_cpe.c:2124-2426) for one strand. */
 static void alignment_rmdup(const orc_ref_t *ref, const strand_ws_t *ws, uint32_t L, uint32_t e,
                             emit_t *em, uint64_t *n_verif,
@@ -923,12 +970,16 @@ static void alignment_rmdup(const orc_ref_t *ref, const strand_ws_t *ws, uint32_
     for (uint64_t i = 0; i < seeds_num; i++) {                              /* :2244-2252 */
         uint64_t si = (uint64_t)ws->info[i / ORC_SEED_STEP].index * ORC_SEED_STEP + i % ORC_SEED_STEP;
         if (pos_count[i] != ws->cnt[si])
-            seeds_heap_insert(sheap, &sheap_size, positions[pos_offset[i] - 1], (uint32_t)i);
+            seeds_heap_insert(sheap, &sheap_size, heap_key(k, ws->info, i, positions[pos_offset[i] - 1]), (uint32_t)i);
     }
     uint32_t pre_pos_size = 0, ajr_flag = 0;
     uint64_t pre_i = 0;
     for (;;) {                                                              /* :2261-2419 */
+#if ORC_MUTANT == 3
+        if (sheap_size > 0 && pos_size + per_block < positions_size && block_size < 256) {   /* planted mistake */
+#else
         if (sheap_size > 0 && pos_size + per_block <= positions_size && block_size < 256) {
+#endif
             uint64_t i = sheap[1].seed_ind;
             uint64_t si = (uint64_t)ws->info[i / ORC_SEED_STEP].index * ORC_SEED_STEP + i % ORC_SEED_STEP;
             uint64_t cnt = ws->cnt[si];
@@ -960,9 +1011,13 @@ static void alignment_rmdup(const orc_ref_t *ref, const strand_ws_t *ws, uint32_
                 pos_ind[block_size++] = pos_size;
             }
             uint32_t del_flag = pos_count[i] == cnt ? 1u : 0u;
-            seeds_heap_pop_and_insert(sheap, &sheap_size, positions[pos_size - 1], del_flag);
+            seeds_heap_pop_and_insert(sheap, &sheap_size, heap_key(k, ws->info, i, positions[pos_size - 1]), del_flag);
         }
+#if ORC_MUTANT == 3
+        if (pos_size + per_block >= positions_size || sheap_size == 0 || block_size >= 256) {
+#else
         if (pos_size + per_block > positions_size || sheap_size == 0 || block_size >= 256) {
+#endif
             adjust_ref_pos(k, positions + pre_pos_size, ws->info, pos_offset, pre_i, ajr_flag, seeds_num);
             if (pos_size > 0) {
                 long_seeds_filter(ws, positions, lseeds, pos_ind, lseed_ind, lseed_off, block_size - 1);

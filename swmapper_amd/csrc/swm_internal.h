@@ -69,6 +69,8 @@ struct swm_ctx {
     uint64_t n_2bits_words = 0;
     uint64_t ref_length = 0;            /* 16 * bytes(.base_2bits) (mapper_mpe.c:288) */
     swm_recipe_t *d_recipe = nullptr;   /* read-encoding recipe per read length 0..254 */
+    uint32_t *d_irreg = nullptr;        /* start offsets of the buckets not in ascending order */
+    uint32_t n_irreg = 0;
     swm_opts opt;
     swm_slot *slot0 = nullptr;          /* the slot of the synchronous entry points */
     uint8_t *d_scratch = nullptr;       /* this device's no-allocation scratch (SWM_NOALLOC_SCRATCH_BYTES) */

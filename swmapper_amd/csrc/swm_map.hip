@@ -37,6 +37,7 @@
 #include <stdlib.h>
 #include <string.h>
 #include <time.h>
+#include <vector>
 
 #include "swm_recipe.h"
 
@@ -171,6 +172,10 @@ __device__ __forceinline__ T block_incl_scan(T v, T *s_w, T *total)
 #define SEED_READS 64        /* reads per workgroup (2 threads each) */
 
 #define NSEL_HEAVY 0x80000000u
+/* some selected 12-mer's bucket is not in ascending order: the item is
+ * planned by k_plan_irreg (block_heap_sort restated), see there */
+#define NSEL_IRREG 0x40000000u
+#define NSEL_FLAGS (NSEL_HEAVY | NSEL_IRREG)
 
 /* rows of the per-length candidate counts: workgroup w adds to row w % 8, so
  * that one read length per batch does not put every workgroup's atomic on
@@ -186,7 +191,22 @@ struct SeedOut {
     unsigned long long *counters;   /* the run's counters, zeroed here */
     uint32_t *nuniq;    /* [nrs + 1]: its last entry is zeroed here */
     uint32_t *lbins;    /* [LBIN_ROWS][256] candidates per read length, zeroed here */
+    const uint32_t *irreg;   /* start offsets of the buckets not in ascending order, ascending */
+    uint32_t n_irreg;
 };
+
+/* Is the bucket starting at off one of the n irregular ones (sorted list)? */
+__device__ __forceinline__ bool bucket_irregular(const uint32_t *__restrict__ irreg, uint32_t n, uint32_t off)
+{
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        const uint32_t v = irreg[mid];
+        if (v == off) return true;
+        if (v < off) lo = mid + 1; else hi = mid;
+    }
+    return false;
+}
 
 /* A read base as the mapper sees it: A,C,G,T -> 0..3 (complemented on the
  * reverse strand, reverse_read :2429-2450); anything else keeps its byte and
@@ -269,7 +289,7 @@ __global__ void __launch_bounds__(128) k_seed(const char *__restrict__ reads, ui
     __shared__ uint16_t s_cnt[4 * SEED_CACHE_M * 2 * SEED_READS];
     if (blockIdx.x == 0) {
         /* the run's counters and the sentinel entries the scans read */
-        if (threadIdx.x < 10) o.counters[threadIdx.x] = 0;
+        if (threadIdx.x < 12) o.counters[threadIdx.x] = 0;
         if (threadIdx.x == 8) {
             o.npos[2 * nreads] = 0;
             o.nchunk[2 * nreads] = 0;
@@ -410,7 +430,7 @@ __global__ void __launch_bounds__(128) k_seed(const char *__restrict__ reads, ui
     const uint32_t n_sel_long = (e + 1 < n_long) ? e + 1 : n_long;
     uint64_t npos = 0;
     uint32_t nchunk = 0;
-    bool heavy = false;
+    bool heavy = false, irr = false;
     swm_sel_t *so = o.sel + (uint64_t)r * seln;
 #pragma unroll
     for (int jm = 0; jm < 16; jm++) {
@@ -452,6 +472,7 @@ __global__ void __launch_bounds__(128) k_seed(const char *__restrict__ reads, ui
                 s.meta = (base + (uint32_t)q) | (rl << 16) | (rh << 19) | ((uint32_t)q << 22);
                 so[4 * jm + q] = s;
                 heavy |= s.cnt > SWM_CHUNK;
+                if (o.n_irreg && s.cnt > 1) irr |= bucket_irregular(o.irreg, o.n_irreg, s.off);
                 npos += s.cnt;
                 nchunk += (s.cnt + SWM_CHUNK - 1) / SWM_CHUNK;
             }
@@ -459,7 +480,7 @@ __global__ void __launch_bounds__(128) k_seed(const char *__restrict__ reads, ui
     }
     /* bit 31: some selected 12-mer has more than one chunk of hits, so the
      * item needs the seed heap (k_plan_heavy) */
-    o.nsel[r] = n_sel_long * SWM_SEED_STEP | (heavy ? NSEL_HEAVY : 0u);
+    o.nsel[r] = n_sel_long * SWM_SEED_STEP | (heavy ? NSEL_HEAVY : 0u) | (irr ? NSEL_IRREG : 0u);
     o.npos[r] = npos;
     o.nchunk[r] = nchunk + 1;   /* batches <= chunks + 1 (an empty first batch) */
 }
@@ -495,7 +516,16 @@ struct PlanIO {
     struct HBatch *hbatch;      /* heavy items' batches, for k_plan_fetch */
     uint32_t *hb_count;
     const uint32_t *ovf;        /* set when a fixed-capacity run would overflow: every stage stands down */
+    uint32_t *irreg;            /* items with an irregular bucket among their selected 12-mers */
+    uint32_t *irreg_count;
+    uint32_t *merge;            /* heavy batches with a descending chunk (hbatch indices), for k_merge */
+    uint32_t *merge_count;
 };
+
+/* Bit 31 of a device bucket entry's second word: its position is smaller
+ * than the previous entry's in the same bucket.  A fetched chunk whose
+ * entries after the first carry it is not ascending. */
+#define PL_DESCENT 0x80000000u
 
 /* One batch of a heavy item: its chunks are cdesc[bd_off[r] + cstart ..
  * + nch), its positions go to surv[surv_off[r] + posoff ..). */
@@ -525,7 +555,8 @@ __global__ void __launch_bounds__(64) k_plan_plain(uint32_t nrs, const uint2 *__
     if (*io.ovf) return;
     const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= nrs) return;
-    const uint32_t nsel = io.nsel[r] & ~NSEL_HEAVY;
+    if (io.nsel[r] & NSEL_IRREG) return;       /* k_plan_irreg's */
+    const uint32_t nsel = io.nsel[r] & ~NSEL_FLAGS;
     const swm_sel_t *sel = io.sel + (uint64_t)r * io.seln;
     uint32_t *surv = io.surv + io.surv_off[r];
     uint2 *bd = io.bdesc + io.bd_off[r];
@@ -636,6 +667,182 @@ __global__ void __launch_bounds__(64) k_plan_plain(uint32_t nrs, const uint2 *__
     io.nuniq[r] = nsurv;
 }
 
+/* block_heap_sort (mapper_cpe.c:1999-2089) restated for one batch: a k-way
+ * merge of the batch's blocks (one per fetch, each the chunk's surviving
+ * read starts in fetch order) through a binary heap keyed by each block's
+ * current head, an output skipped when it equals the one before.  On
+ * ascending blocks this is the sorted unique list the other plan kernels
+ * build; on a block that descends (a bucket of the reference's index that
+ * its serial scatter left out of order: the zeroed tail starts of Q23 land
+ * at the head of a later bucket, and their own bucket keeps an unwritten 0
+ * slot) the merge's order and its surviving duplicates depend on the heap's
+ * steps, so they are taken literally.  pos: the blocks back to back; ind:
+ * [2b] start, [2b + 1] end of block b (advanced in place); returns the
+ * count written to out.  An empty batch gives nothing (Q8). */
+__device__ uint32_t block_merge(const uint32_t *pos, uint32_t *ind, uint32_t nblk, uint32_t *hp, uint32_t *hb,
+                                uint32_t *__restrict__ out)
+{
+    uint32_t hs = 0;
+    for (uint32_t i = 0; i < nblk; i++) {                   /* :2003-2026 */
+        if (ind[2 * i] < ind[2 * i + 1]) {
+            hs++;
+            hp[hs] = pos[ind[2 * i]];
+            hb[hs] = 2 * i;
+            ind[2 * i]++;
+            uint32_t cur = hs;
+            while (cur > 1) {
+                const uint32_t par = cur / 2;
+                if (hp[par] > hp[cur]) {
+                    uint32_t t = hp[cur]; hp[cur] = hp[par]; hp[par] = t;
+                    t = hb[cur]; hb[cur] = hb[par]; hb[par] = t;
+                } else {
+                    break;
+                }
+                cur = par;
+            }
+        }
+    }
+    if (hs == 0) return 0;
+    uint32_t pre = 0xffffffffu, cnt = 0;
+    while (hs > 1) {                                        /* :2031-2074 */
+        const uint32_t id = hb[1];
+        if (pre != hp[1]) { out[cnt++] = hp[1]; pre = hp[1]; }
+        if (ind[id] < ind[id + 1]) {
+            hp[1] = pos[ind[id]];
+            ind[id]++;
+        } else {
+            hp[1] = hp[hs];
+            hb[1] = hb[hs];
+            hs--;
+            if (hs == 1) break;
+        }
+        uint32_t h = 1;
+        while (h * 2 <= hs) {
+            const uint32_t l = h * 2, rr = l + 1;
+            uint32_t sm = l;
+            if (rr <= hs && hp[l] > hp[rr]) sm = rr;
+            if (hp[h] > hp[sm]) {
+                uint32_t t = hp[h]; hp[h] = hp[sm]; hp[sm] = t;
+                t = hb[h]; hb[h] = hb[sm]; hb[sm] = t;
+            } else {
+                break;
+            }
+            h = sm;
+        }
+    }
+    const uint32_t id = hb[1];                              /* :2077-2084 */
+    for (uint32_t i = ind[id] - 1; i < ind[id + 1]; i++) {
+        const uint32_t v = pos[i];
+        if (v != pre) { out[cnt++] = v; pre = v; }
+    }
+    return cnt;
+}
+
+/* Items whose selected 12-mers include an irregular bucket (NSEL_IRREG):
+ * the whole alignment_rmdup state machine (:2124-2426) for the item on one
+ * lane, as k_plan_plain, with each batch's blocks kept in LDS and merged by
+ * block_merge.  Such items are rare (the reference's index has at most a
+ * few irregular buckets), so one lane per item, a workgroup each, taken
+ * from the list grid-stride; the item's plan output (batch descriptors,
+ * candidates, counts) has the layout every other plan kernel writes. */
+__global__ void __launch_bounds__(64) k_plan_irreg(const uint2 *__restrict__ pl, PlanIO io)
+{
+    __shared__ uint32_t s_pos[SWM_BATCH_POS];
+    __shared__ uint32_t s_ind[2 * SWM_MAX_BLOCKS + 4];
+    __shared__ uint32_t s_hp[SWM_MAX_BLOCKS + 2], s_hb[SWM_MAX_BLOCKS + 2];
+    __shared__ uint32_t s_loc[68], s_sid[68], s_pc[64];
+    if (*io.ovf || threadIdx.x != 0) return;
+    const uint32_t nq = *io.irreg_count;
+    for (uint32_t qi = blockIdx.x; qi < nq; qi += gridDim.x) {
+        const uint32_t r = io.irreg[qi];
+        const uint32_t nsel = io.nsel[r] & ~NSEL_FLAGS;
+        const swm_sel_t *sel = io.sel + (uint64_t)r * io.seln;
+        uint32_t *surv = io.surv + io.surv_off[r];
+        uint2 *bd = io.bdesc + io.bd_off[r];
+        uint32_t hsize = 0, nsurv = 0, batch = 0, pos_size = 0, blocks = 0, nb = 0;
+        bool first_pending = true;                          /* Q7 */
+        auto take = [&](uint32_t g, uint32_t len, uint32_t meta) {
+            const uint32_t b0 = nb;
+            uint2 v[SWM_CHUNK];
+#pragma unroll
+            for (int j = 0; j < SWM_CHUNK; j++) v[j] = pl[(uint32_t)j < len ? g + (uint32_t)j : g];   /* all in flight */
+#pragma unroll
+            for (int j = 0; j < SWM_CHUNK; j++) {
+                if ((uint32_t)j < len) {
+                    if (first_pending) first_pending = false;
+                    else if (filter_pass(v[j].y, meta)) s_pos[nb++] = adjust(v[j].x, meta);
+                }
+            }
+            s_ind[2 * blocks] = b0;
+            s_ind[2 * blocks + 1] = nb;
+            blocks++;
+        };
+        auto heap_insert = [&](uint32_t loc, uint32_t ind) {    /* :1919-1935 */
+            hsize++;
+            s_loc[hsize] = loc; s_sid[hsize] = ind;
+            uint32_t i = hsize;
+            while (i > 1) {
+                const uint32_t ti = i / 2;
+                if (s_loc[ti] > s_loc[i]) {
+                    uint32_t t = s_loc[ti]; s_loc[ti] = s_loc[i]; s_loc[i] = t;
+                    t = s_sid[ti]; s_sid[ti] = s_sid[i]; s_sid[i] = t;
+                } else break;
+                i = ti;
+            }
+        };
+        auto heap_pop_insert = [&](uint32_t loc, bool del) {    /* :1936-1962 (Q6) */
+            if (del) { s_loc[1] = s_loc[hsize]; s_sid[1] = s_sid[hsize]; hsize--; }
+            else s_loc[1] = loc;
+            uint32_t i = 1;
+            while (i * 2 < hsize) {
+                uint32_t ti = i * 2;
+                if (i * 2 + 1 <= hsize && s_loc[i * 2] > s_loc[i * 2 + 1]) ti = i * 2 + 1;
+                if (s_loc[ti] < s_loc[i]) {
+                    uint32_t t = s_loc[ti]; s_loc[ti] = s_loc[i]; s_loc[i] = t;
+                    t = s_sid[ti]; s_sid[ti] = s_sid[i]; s_sid[i] = t;
+                } else break;
+                i = ti;
+            }
+        };
+        for (uint32_t i = 0; i < nsel; i++) {                   /* phase 1 (:2194-2236) */
+            const swm_sel_t sv = sel[i];
+            const uint32_t get = sv.cnt > SWM_CHUNK ? SWM_CHUNK : sv.cnt;
+            if (get > 0) take(sv.off, get, sv.meta);
+            pos_size += get;
+            s_pc[i] = get;
+        }
+        for (uint32_t i = 0; i < nsel; i++) {                   /* :2244-2252 */
+            const swm_sel_t sv = sel[i];
+            if (s_pc[i] != sv.cnt) heap_insert(pl[sv.off + s_pc[i] - 1].x, i);
+        }
+        for (;;) {                                              /* :2261-2419 */
+            if (hsize > 0 && pos_size + SWM_CHUNK <= SWM_BATCH_POS && blocks < SWM_MAX_BLOCKS) {
+                const uint32_t i = s_sid[1];
+                const swm_sel_t sv = sel[i];
+                const uint32_t rem = sv.cnt - s_pc[i];
+                const uint32_t get = rem > SWM_CHUNK ? SWM_CHUNK : rem;
+                take(sv.off + s_pc[i], get, sv.meta);
+                pos_size += get;
+                s_pc[i] += get;
+                heap_pop_insert(pl[sv.off + s_pc[i] - 1].x, s_pc[i] == sv.cnt);
+            }
+            if (pos_size + SWM_CHUNK > SWM_BATCH_POS || hsize == 0 || blocks >= SWM_MAX_BLOCKS) {
+                const uint32_t u = block_merge(s_pos, s_ind, blocks, s_hp, s_hb, surv + nsurv);
+                bd[batch] = make_uint2(nsurv, u);
+                nsurv += u;
+                batch++;
+                pos_size = 0;
+                blocks = 0;
+                nb = 0;
+                first_pending = true;
+                if (hsize == 0) break;
+            }
+        }
+        io.nbatch[r] = batch;
+        io.nuniq[r] = nsurv;
+    }
+}
+
 /* Lane reads (v_readlane) and writes (a compare + select) at a wave-uniform
  * lane index.  The seed heap and the per-seed state of k_plan_heavy live one
  * entry per lane, so its replay is scalar code with no LDS round trip per
@@ -693,7 +900,7 @@ __global__ void __launch_bounds__(64) k_plan_wave(uint32_t nrs, const uint2 *__r
             if (lane < io.seln) my_n = io.sel[(uint64_t)rn * io.seln + lane];
         }
         {
-            if (nsw & NSEL_HEAVY) continue;
+            if (nsw & NSEL_FLAGS) continue;      /* heavy: k_plan_heavy's; irregular: k_plan_irreg's */
             const uint32_t nsel = nsw;
             if (lane >= nsel) my = swm_sel_t{0, 0, 0};       /* slots past nsel are not written */
             const uint32_t incl = wave_incl_scan_dpp(my.cnt);
@@ -822,7 +1029,7 @@ __global__ void __launch_bounds__(64) k_plan_heavy(const uint2 *__restrict__ pl,
             else if (np > GIANT_POS) __builtin_amdgcn_s_setprio(1);
             else __builtin_amdgcn_s_setprio(0);
         }
-        const uint32_t nsel = io.nsel[r] & ~NSEL_HEAVY;
+        const uint32_t nsel = io.nsel[r] & ~NSEL_FLAGS;
         const bool act = lane < nsel;
         /* per 12-mer state, lane i = selected 12-mer i */
         swm_sel_t my = {0, 0, 0};
@@ -1022,12 +1229,14 @@ __global__ void __launch_bounds__(64) k_plan_fetch(const uint2 *__restrict__ pl,
             sseed[k] = d.y >> 16;
         }
         if (lane == 0) spre[bn] = bpos;
-        const uint32_t nsel = io.nsel[r] & ~NSEL_HEAVY;
+        const uint32_t nsel = io.nsel[r] & ~NSEL_FLAGS;
         smeta[lane] = lane < nsel ? io.sel[(uint64_t)r * io.seln + lane].meta : 0u;
         __syncthreads();
         uint32_t *surv = io.surv + io.surv_off[r];
         const uint32_t bstart = hb.posoff;
         uint32_t nsurv = bstart;
+        uint64_t desc = 0;                  /* some chunk descends after its first entry */
+        uint2 *cdw = io.cdesc + io.bd_off[r] + hb.cstart;
         /* a chunk (<= 32 entries) per half wave, lane & 31 on its entry, so
          * a load covers two chunks with consecutive addresses and lane order
          * is entry order; 16 chunks per pass, 8 loads in flight per lane;
@@ -1049,10 +1258,26 @@ __global__ void __launch_bounds__(64) k_plan_fetch(const uint2 *__restrict__ pl,
 #pragma unroll
             for (int q = 0; q < 8; q++) {
                 const bool keep = live[q] && filter_pass(v[q].y, meta[q]);
+                desc |= __ballot(live[q] && (lane & 31u) != 0u && (v[q].y & PL_DESCENT));
                 const uint64_t m = __ballot(keep);
                 if (keep) surv[nsurv + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = adjust(v[q].x, meta[q]);
+                /* each chunk's survivor end (the chunk's descriptor is not
+                 * read again): the blocks of k_merge */
+                const uint32_t k = k0 + 2u * (uint32_t)q;
+                const uint32_t lo = (uint32_t)__popcll(m & 0xffffffffull);
+                if (lane == 0 && k < bn) cdw[k].x = nsurv + lo - bstart;
+                if (lane == 32 && k + 1 < bn) cdw[k + 1].x = nsurv + (uint32_t)__popcll(m) - bstart;
                 nsurv += (uint32_t)__popcll(m);
             }
+        }
+        if (desc) {
+            /* block_heap_sort's merge taken literally (k_merge) */
+            if (lane == 0) {
+                io.bdesc[io.bd_off[r] + hb.batch] = make_uint2(bstart, nsurv - bstart);
+                io.merge[atomicAdd(io.merge_count, 1u)] = h;
+            }
+            __syncthreads();
+            continue;
         }
         /* end of batch: sorted unique list (block_heap_sort's result) */
         const uint32_t n = nsurv - bstart;
@@ -1083,6 +1308,39 @@ __global__ void __launch_bounds__(64) k_plan_fetch(const uint2 *__restrict__ pl,
         __syncthreads();
     }
     big_flush();
+}
+
+/* Heavy batches that fetched a descending chunk (listed by k_plan_fetch,
+ * which left the batch's survivors in fetch order and each chunk's survivor
+ * end in its descriptor): block_heap_sort's merge taken literally, one
+ * workgroup per batch, the survivors staged in LDS and merged by one lane. */
+__global__ void __launch_bounds__(64) k_merge(PlanIO io)
+{
+    __shared__ uint32_t s_pos[SWM_BATCH_POS];
+    __shared__ uint32_t s_ind[2 * SWM_MAX_BLOCKS + 4];
+    __shared__ uint32_t s_hp[SWM_MAX_BLOCKS + 2], s_hb[SWM_MAX_BLOCKS + 2];
+    if (*io.ovf) return;
+    const uint32_t nm = *io.merge_count;
+    for (uint32_t i = blockIdx.x; i < nm; i += gridDim.x) {
+        const HBatch hb = io.hbatch[io.merge[i]];
+        const uint32_t r = hb.r, bn = hb.npack & 0xffu;
+        const uint2 *cd = io.cdesc + io.bd_off[r] + hb.cstart;
+        uint2 *bd = io.bdesc + io.bd_off[r] + hb.batch;
+        const uint2 b = *bd;
+        uint32_t *surv = io.surv + io.surv_off[r] + b.x;
+        for (uint32_t j = threadIdx.x; j < b.y; j += 64) s_pos[j] = surv[j];
+        for (uint32_t k = threadIdx.x; k < bn; k += 64) {
+            s_ind[2 * k] = k ? cd[k - 1].x : 0u;
+            s_ind[2 * k + 1] = cd[k].x;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const uint32_t u = block_merge(s_pos, s_ind, bn, s_hp, s_hb, surv);
+            *bd = make_uint2(b.x, u);
+            if (u) atomicAdd(&io.nuniq[r], u);
+        }
+        __syncthreads();
+    }
 }
 
 /* One wave per large batch (65 .. 2048 values), sorted in registers: R =
@@ -2787,6 +3045,8 @@ struct ScanJob {
     uint32_t cls_n;
     uint32_t *cls_heavy, *cls_heavy_count, *cls_giant, *cls_giant_count;
     uint32_t giant_pos;      /* items with more positions go to the giant list */
+    uint32_t cls_heavy_on;   /* heavy and giant lists (the work-queue form) */
+    uint32_t *cls_irreg, *cls_irreg_count;   /* items with an irregular bucket (every form) */
 };
 
 /* The SCAN_PER values of thread t in a tile (elements q * SCAN_NT + t),
@@ -2833,7 +3093,7 @@ __global__ void __launch_bounds__(SCAN_NT) k_scan_reduce(ScanJob j)
 __global__ void __launch_bounds__(SCAN_NT) k_scan_down(ScanJob j)
 {
     __shared__ uint64_t s_w[SCAN_NT / 64];
-    __shared__ uint32_t s_base[2];
+    __shared__ uint32_t s_base[3];
     __shared__ uint64_t s_v[SCAN_TILE + SCAN_TILE / 16];
     const uint32_t y = j.first + blockIdx.y;
     const uint32_t tid = threadIdx.x;
@@ -2863,7 +3123,7 @@ __global__ void __launch_bounds__(SCAN_NT) k_scan_down(ScanJob j)
     uint64_t xv[SCAN_PER];
     if (y == 0) scan_tile_load(j.in0, j.n, tile, xv);
     else scan_tile_load(j.in1, j.n, tile, xv);
-    uint32_t hm = 0, gm = 0;
+    uint32_t hm = 0, gm = 0, im = 0;
     if (cls) {
         uint32_t ns[SCAN_PER];
 #pragma unroll
@@ -2874,9 +3134,14 @@ __global__ void __launch_bounds__(SCAN_NT) k_scan_down(ScanJob j)
 #pragma unroll
         for (int q = 0; q < SCAN_PER; q++) {
             const uint64_t g = tile + (uint64_t)q * SCAN_NT + tid;
-            if (g < j.cls_n && (ns[q] & NSEL_HEAVY)) {
+            if (g >= j.cls_n) continue;
+            if (j.cls_heavy_on && (ns[q] & NSEL_HEAVY)) {
+                /* heavy items with an irregular bucket too: their batches
+                 * with a descending chunk go to k_merge */
                 if (xv[q] > j.giant_pos) gm |= 1u << q;
                 else hm |= 1u << q;
+            } else if (ns[q] & NSEL_IRREG) {
+                im |= 1u << q;
             }
         }
     }
@@ -2908,21 +3173,27 @@ __global__ void __launch_bounds__(SCAN_NT) k_scan_down(ScanJob j)
         }
     }
     if (cls) {
-        /* one atomic per list per workgroup */
-        const uint64_t c = (uint64_t)__popc(hm) | ((uint64_t)__popc(gm) << 32);
+        /* one atomic per list per workgroup; the three counts of a thread
+         * (at most SCAN_TILE each) in 21-bit fields of one scanned word */
+        const uint64_t c = (uint64_t)__popc(hm) | ((uint64_t)__popc(gm) << 21) | ((uint64_t)__popc(im) << 42);
         uint64_t ctot;
         const uint64_t cex = block_incl_scan<uint64_t, SCAN_NT>(c, s_w, &ctot) - c;
+        const uint32_t th = (uint32_t)(ctot & 0x1fffffu), tg = (uint32_t)((ctot >> 21) & 0x1fffffu),
+                       ti = (uint32_t)(ctot >> 42);
         if (tid == 0) {
-            s_base[0] = (uint32_t)ctot ? atomicAdd(j.cls_heavy_count, (uint32_t)ctot) : 0u;
-            s_base[1] = (uint32_t)(ctot >> 32) ? atomicAdd(j.cls_giant_count, (uint32_t)(ctot >> 32)) : 0u;
+            s_base[0] = th ? atomicAdd(j.cls_heavy_count, th) : 0u;
+            s_base[1] = tg ? atomicAdd(j.cls_giant_count, tg) : 0u;
+            s_base[2] = ti ? atomicAdd(j.cls_irreg_count, ti) : 0u;
         }
         __syncthreads();
-        uint32_t ph = s_base[0] + (uint32_t)cex, pg = s_base[1] + (uint32_t)(cex >> 32);
+        uint32_t ph = s_base[0] + (uint32_t)(cex & 0x1fffffu), pg = s_base[1] + (uint32_t)((cex >> 21) & 0x1fffffu),
+                 pi = s_base[2] + (uint32_t)(cex >> 42);
 #pragma unroll
         for (int q = 0; q < SCAN_PER; q++) {
             const uint32_t g = (uint32_t)(tile + (uint64_t)q * SCAN_NT + tid);
             if ((hm >> q) & 1u) j.cls_heavy[ph++] = g;
             if ((gm >> q) & 1u) j.cls_giant[pg++] = g;
+            if ((im >> q) & 1u) j.cls_irreg[pi++] = g;
         }
     }
 }
@@ -2940,7 +3211,7 @@ struct swm_ws {
     struct Buf { void *p = nullptr; size_t cap = 0; };
     enum { WORDS, SEL, NSEL, NPOS, NCHUNK, SURV_OFF, BD_OFF, SURV, BDESC, NBATCH, NUNIQ, CAND_OFF, C_VAL, C_RS, C_BI,
            C_ERR, C_LOC, C_REC, C_INFO, COUNTERS, BIG, TRACE, HEAVY, HEAVY_C, HEAVY_E, SCAN_TMP, IN_READS, IN_LENS,
-           OUT, HIST, R_WR, R_TB, R_RUNS, R_IDX, R_TOFF, R_SCAN, R_TOT, R_OUT, R_TEXT, CDESC, HBATCH, GIANT, EPART, RPART, HPART, LBINS, PERM, NBUF };
+           OUT, HIST, R_WR, R_TB, R_RUNS, R_IDX, R_TOFF, R_SCAN, R_TOT, R_OUT, R_TEXT, CDESC, HBATCH, GIANT, EPART, RPART, HPART, LBINS, PERM, IRREG, MERGE, NBUF };
     Buf b[NBUF];
     uint64_t gen = 1;
     ~swm_ws() { for (auto &x : b) if (x.p) (void)hipFree(x.p); }
@@ -2992,7 +3263,7 @@ struct swm_slot {
     /* page-locked staging, so that no copy goes through the runtime's shared
      * pageable-memory path (which serialises the slots): totals read back
      * between stages, counters, and host batches in and out */
-    uint64_t *h_small = nullptr;     /* [0..9] counters at the end; before that [8] positions, [9] chunks,
+    uint64_t *h_small = nullptr;     /* [0..11] counters at the end; before that [8] positions, [9] chunks,
                                         [10] candidates, [11..13] record totals */
     struct Pin { void *p = nullptr; size_t cap = 0; };
     Pin h_reads, h_lens, h_out;
@@ -3111,7 +3382,7 @@ struct Pipe {
  * [2] rejected windows, [3] heavy plan items, [4] their queue head,
  * [5] heavy compaction items, [6] verification order is the identity, [7] overflow, [8] heavy plan
  * batches, [9] giant plan items, [10] giant / [11] other heavy emission
- * reads */
+ * reads, [12] items with an irregular bucket, [13] heavy batches for k_merge */
 template <class Alloc>
 static int pipe_alloc_a(Alloc *ws, uint64_t n, uint32_t seln, Pipe &P)
 {
@@ -3122,10 +3393,11 @@ static int pipe_alloc_a(Alloc *ws, uint64_t n, uint32_t seln, Pipe &P)
     int rc;
     if ((rc = ws->get(swm_ws::WORDS, &P.so.words, nrs * 32)) || (rc = ws->get(swm_ws::SEL, &P.so.sel, nrs * seln)) ||
         (rc = ws->get(swm_ws::NSEL, &P.so.nsel, nrs)) || (rc = ws->get(swm_ws::NPOS, &P.so.npos, nrs + 1)) ||
-        (rc = ws->get(swm_ws::NCHUNK, &P.so.nchunk, nrs + 1)) || (rc = ws->get(swm_ws::COUNTERS, &P.counters, 10)) ||
+        (rc = ws->get(swm_ws::NCHUNK, &P.so.nchunk, nrs + 1)) || (rc = ws->get(swm_ws::COUNTERS, &P.counters, 12)) ||
         (rc = ws->get(swm_ws::SURV_OFF, &P.surv_off, nrs + 1)) || (rc = ws->get(swm_ws::BD_OFF, &P.bd_off, nrs + 1)) ||
         (rc = ws->get(swm_ws::NBATCH, &P.pio.nbatch, nrs)) || (rc = ws->get(swm_ws::NUNIQ, &P.pio.nuniq, nrs + 1)) ||
-        (rc = ws->get(swm_ws::HEAVY, &P.pio.heavy, nrs)) || (rc = ws->get(swm_ws::GIANT, &P.pio.giant, nrs)) || (rc = ws->get(swm_ws::CAND_OFF, &P.cand_off, nrs + 1)) ||
+        (rc = ws->get(swm_ws::HEAVY, &P.pio.heavy, nrs)) || (rc = ws->get(swm_ws::GIANT, &P.pio.giant, nrs)) ||
+        (rc = ws->get(swm_ws::IRREG, &P.pio.irreg, nrs)) || (rc = ws->get(swm_ws::CAND_OFF, &P.cand_off, nrs + 1)) ||
         (rc = ws->get(swm_ws::HEAVY_C, &P.heavy_c, nrs)) || (rc = ws->get(swm_ws::HEAVY_E, &P.heavy_e, 2 * n)) ||
         (rc = ws->get(swm_ws::EPART, &P.epart, 2 * n)) ||
         (rc = ws->get(swm_ws::RPART, &P.rpart, (uint64_t)blocks_for(n, 128))) ||
@@ -3147,6 +3419,8 @@ static int pipe_alloc_a(Alloc *ws, uint64_t n, uint32_t seln, Pipe &P)
     P.pio.heavy_head = P.u32c + 4;
     P.pio.hb_count = P.u32c + 8;
     P.pio.giant_count = P.u32c + 9;
+    P.pio.irreg_count = P.u32c + 12;
+    P.pio.merge_count = P.u32c + 13;
     P.pio.ovf = P.u32c + 7;
     P.cs.ovf = P.u32c + 7;
     P.cs.perm_id = P.u32c + 6;                    /* 8-byte aligned, the overflow flag next */
@@ -3162,7 +3436,8 @@ static int pipe_alloc_b(Alloc *ws, uint64_t cap_pos, uint64_t cap_chunk, Pipe &P
         (rc = ws->get(swm_ws::BDESC, &P.pio.bdesc, cap_chunk + 1)) ||
         (rc = ws->get(swm_ws::BIG, &P.pio.big, cap_chunk + 1)) ||
         (rc = ws->get(swm_ws::CDESC, &P.pio.cdesc, cap_chunk + 1)) ||
-        (rc = ws->get(swm_ws::HBATCH, &P.pio.hbatch, cap_chunk + 1)))
+        (rc = ws->get(swm_ws::HBATCH, &P.pio.hbatch, cap_chunk + 1)) ||
+        (rc = ws->get(swm_ws::MERGE, &P.pio.merge, cap_chunk + 1)))
         return rc;
     return SWM_OK;
 }
@@ -3194,6 +3469,8 @@ static int enq_a(const swm_ctx_t *ctx, Pipe &P, const char *d_reads, uint64_t st
                  uint32_t e, uint64_t cap_pos, uint64_t cap_chunk, hipStream_t st, hipEvent_t *ev)
 {
     if (ev) SWM_HIP_CHECK(hipEventRecord(ev[0], st));
+    P.so.irreg = ctx->d_irreg;
+    P.so.n_irreg = ctx->n_irreg;
     k_seed<<<blocks_for(P.n, SEED_READS), 2 * SEED_READS, 0, st>>>(d_reads, stride, d_lens, (uint32_t)P.n, ctx->k, e,
                                                                    P.seln, ctx->d_hash_offset, ctx->d_recipe, P.so);
     SWM_HIP_CHECK(hipGetLastError());
@@ -3207,11 +3484,13 @@ static int enq_a(const swm_ctx_t *ctx, Pipe &P, const char *d_reads, uint64_t st
     j.cap[0] = cap_pos; j.cap[1] = cap_chunk;
     j.tot[0] = P.counters + 3; j.tot[1] = nullptr;
     j.ovf = P.u32c + 7;
-    if (ctx->opt.work_queue) {
+    if (ctx->opt.work_queue || ctx->n_irreg) {
         j.cls_nsel = P.so.nsel;
         j.cls_n = (uint32_t)P.nrs;
+        j.cls_heavy_on = ctx->opt.work_queue;
         j.cls_heavy = P.pio.heavy; j.cls_heavy_count = P.pio.heavy_count;
         j.cls_giant = P.pio.giant; j.cls_giant_count = P.pio.giant_count;
+        j.cls_irreg = P.pio.irreg; j.cls_irreg_count = P.pio.irreg_count;
         j.giant_pos = ctx->opt.giant_pos;
     }
     scan_launch(j, 2, st);
@@ -3231,6 +3510,10 @@ static int enq_b(const swm_ctx_t *ctx, Pipe &P, uint64_t cap_cand, hipStream_t s
         k_plan_fetch<<<ctx->n_cu * 32, 64, 0, st>>>(ctx->d_pl, P.pio);
     } else {
         k_plan_plain<<<blocks_for(P.nrs, 64), 64, 0, st>>>((uint32_t)P.nrs, ctx->d_pl, P.pio);
+    }
+    if (ctx->n_irreg) {
+        k_plan_irreg<<<ctx->n_cu * 4, 64, 0, st>>>(ctx->d_pl, P.pio);
+        if (o.work_queue) k_merge<<<ctx->n_cu * 4, 64, 0, st>>>(P.pio);
     }
     k_bigsort<<<ctx->n_cu * 32, 64, 0, st>>>(P.pio.big, P.pio.big_count, P.surv_off, P.bd_off, P.pio.surv, P.pio.bdesc,
                                     P.pio.nuniq);
@@ -3319,6 +3602,7 @@ static void stats_from_counters(swm_stats_t *s, uint64_t n, const unsigned long 
     s->n_heavy_reads = u2[2] + u2[3];
     s->n_giant_items = u2[1];
     s->n_giant_reads = u2[2];
+    s->n_irregular_items = reinterpret_cast<const uint32_t *>(hc + 10)[0];
 }
 
 /* The per-read candidate histogram of the run just finished (stats). */
@@ -3392,7 +3676,7 @@ static int run_synced(swm_ctx_t *ctx, swm_slot *sl, const char *d_reads, uint64_
     if ((rc = enq_c(ctx, P, d_lens, e, d_out, ncand, st, sl->ev))) return rc;
     if (recs && (rc = records_out(ctx, sl, P, d_lens, ncand, recs))) return rc;
     turn.done();
-    SWM_HIP_CHECK(hipMemcpyAsync(hs, P.counters, 10 * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+    SWM_HIP_CHECK(hipMemcpyAsync(hs, P.counters, 12 * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
     if (h_out && n_out)
         SWM_HIP_CHECK(hipMemcpyAsync(h_out, d_out, n_out * sizeof(swm_read_result_t), hipMemcpyDeviceToHost, st));
     if ((rc = final_sync(sl))) return rc;
@@ -3549,7 +3833,7 @@ static int run_graph(swm_ctx_t *ctx, swm_slot *sl, const char *d_reads, uint64_t
         SWM_HIP_CHECK(hipEventRecord(sl->ev[5], st));
     }
     const unsigned long long *hc = reinterpret_cast<const unsigned long long *>(sl->h_small);
-    SWM_HIP_CHECK(hipMemcpyAsync(sl->h_small, P.counters, 10 * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+    SWM_HIP_CHECK(hipMemcpyAsync(sl->h_small, P.counters, 12 * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
     if (h_out && n_out)
         SWM_HIP_CHECK(hipMemcpyAsync(h_out, d_out, n_out * sizeof(swm_read_result_t), hipMemcpyDeviceToHost, st));
     if ((rc = final_sync(sl))) return rc;
@@ -3613,16 +3897,16 @@ struct ScratchArena {
 };
 
 /* What is left after the per-read arrays goes to the capacities: 60% to
- * bucket chunks (56 bytes each: descriptors, large-batch and heavy-chunk
- * lists, heavy batch records), 25% to positions (4 bytes), 15% to
+ * bucket chunks (60 bytes each: descriptors, large-batch, heavy-chunk and
+ * merge lists, heavy batch records), 25% to positions (4 bytes), 15% to
  * candidates (36 bytes: nine per-candidate arrays).  A batch over any of
  * them stands down on the device (the fixed-capacity checks of the graph
  * form) and the call reports SWM_ENOSPC. */
 static void scratch_caps(size_t left, uint64_t *cap_pos, uint64_t *cap_chunk, uint64_t *cap_cand)
 {
-    const size_t slack = 16 * 256 + 256;   /* alignment of 14 buffers and their one extra element */
+    const size_t slack = 16 * 256 + 256;   /* alignment of 15 buffers and their one extra element */
     const uint64_t r = left > slack ? left - slack : 0;
-    *cap_chunk = r * 6 / 10 / 56;
+    *cap_chunk = r * 6 / 10 / 60;
     *cap_pos = r / 4 / 4;
     *cap_cand = r * 15 / 100 / 36;
 }
@@ -3659,7 +3943,7 @@ static int run_noalloc(swm_ctx_t *ctx, const char *d_reads, uint64_t stride, con
         SWM_HIP_CHECK(hipEventRecord(sl->ev[5], st));
     }
     const unsigned long long *hc = reinterpret_cast<const unsigned long long *>(sl->h_small);
-    SWM_HIP_CHECK(hipMemcpyAsync(sl->h_small, P.counters, 10 * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+    SWM_HIP_CHECK(hipMemcpyAsync(sl->h_small, P.counters, 12 * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
     if ((rc = final_sync(sl))) return rc;
     if (reinterpret_cast<const uint32_t *>(hc + 4)[7]) return SWM_ENOSPC;   /* over a capacity: nothing written */
     stats_from_counters(&sl->stats, n, hc);
@@ -3760,15 +4044,18 @@ int swm_slot_run_host(swm_ctx_t *ctx, swm_slot *sl, const char *reads, uint64_t 
 
 /* The device keeps each bucket entry as one 8-byte (position, residual)
  * pair, so a bucket chunk costs one random cache line, not two. */
-static int upload_pl(uint2 *d, const uint32_t *pos, const uint16_t *ls, uint64_t n)
+static int upload_pl(uint2 *d, const uint32_t *pos, const uint16_t *ls, uint64_t n,
+                     const std::vector<uint64_t> &descents)
 {
     const uint64_t step = 1ull << 22;
     uint2 *h = (uint2 *)malloc((n < step ? n : step) * sizeof(uint2));
     if (!h) return SWM_ENOMEM;
     int rc = SWM_OK;
+    size_t di = 0;
     for (uint64_t b = 0; b < n && rc == SWM_OK; b += step) {
         uint64_t m = n - b < step ? n - b : step;
         for (uint64_t i = 0; i < m; i++) h[i] = make_uint2(pos[b + i], ls[b + i]);
+        for (; di < descents.size() && descents[di] < b + m; di++) h[descents[di] - b].y |= PL_DESCENT;
         if (hipMemcpy(d + b, h, m * sizeof(uint2), hipMemcpyHostToDevice) != hipSuccess) rc = SWM_EDEVICE;
     }
     free(h);
@@ -3811,21 +4098,36 @@ extern "C" int swm_ctx_create(uint32_t k, const uint32_t *hash_count, const uint
     if (!h_off) { free(rec); swm_ctx_destroy(c); return SWM_ENOMEM; }
     h_off[0] = 0;
     for (uint64_t i = 0; i < c->n_buckets; i++) h_off[i + 1] = h_off[i] + hash_count[i];
+    /* buckets whose positions are not ascending (the serial scatter's
+     * zeroed tail starts, Q23): their items are planned by k_plan_irreg */
+    std::vector<uint32_t> irr;
+    std::vector<uint64_t> descents;
+    for (uint64_t i = 0; i < c->n_buckets; i++) {
+        const uint64_t a = h_off[i], b = (uint64_t)h_off[i] + hash_count[i];
+        if (b > n_pos) break;
+        bool any = false;
+        for (uint64_t j = a + 1; j < b; j++)
+            if (positions[j] < positions[j - 1]) { descents.push_back(j); any = true; }
+        if (any) irr.push_back((uint32_t)a);
+    }
     int rc = SWM_OK;
     const size_t rec_bytes = (SWM_RECIPE_MAX_L + 1) * sizeof(swm_recipe_t);
     if (hipMalloc((void **)&c->d_hash_offset, (c->n_buckets + 1) * 4) != hipSuccess ||
         hipMalloc((void **)&c->d_pl, (n_pos + 64) * 8) != hipSuccess ||
         hipMalloc((void **)&c->d_base_2bits, (n_2bits_words + 2 * SWM_WIN_PAIRS_PAD) * 4) != hipSuccess ||
-        hipMalloc((void **)&c->d_recipe, rec_bytes) != hipSuccess)
+        hipMalloc((void **)&c->d_recipe, rec_bytes) != hipSuccess ||
+        (!irr.empty() && hipMalloc((void **)&c->d_irreg, irr.size() * 4) != hipSuccess))
         rc = SWM_ENOMEM;
     if (rc == SWM_OK) {
         if (hipMemcpy(c->d_hash_offset, h_off, (c->n_buckets + 1) * 4, hipMemcpyHostToDevice) != hipSuccess ||
             hipMemset(c->d_pl, 0, (n_pos + 64) * 8) != hipSuccess ||
-            (n_pos && upload_pl(c->d_pl, positions, pos_lseeds, n_pos) != SWM_OK) ||
+            (n_pos && upload_pl(c->d_pl, positions, pos_lseeds, n_pos, descents) != SWM_OK) ||
             hipMemset(c->d_base_2bits, 0, (n_2bits_words + 2 * SWM_WIN_PAIRS_PAD) * 4) != hipSuccess ||
             hipMemcpy(c->d_base_2bits, base_2bits, n_2bits_words * 4, hipMemcpyHostToDevice) != hipSuccess ||
-            hipMemcpy(c->d_recipe, rec, rec_bytes, hipMemcpyHostToDevice) != hipSuccess)
+            hipMemcpy(c->d_recipe, rec, rec_bytes, hipMemcpyHostToDevice) != hipSuccess ||
+            (!irr.empty() && hipMemcpy(c->d_irreg, irr.data(), irr.size() * 4, hipMemcpyHostToDevice) != hipSuccess))
             rc = SWM_EDEVICE;
+        c->n_irreg = (uint32_t)irr.size();
     }
     free(h_off);
     free(rec);
@@ -3842,6 +4144,7 @@ extern "C" void swm_ctx_destroy(swm_ctx_t *c)
     if (c->d_pl) (void)hipFree(c->d_pl);
     if (c->d_base_2bits) (void)hipFree(c->d_base_2bits);
     if (c->d_recipe) (void)hipFree(c->d_recipe);
+    if (c->d_irreg) (void)hipFree(c->d_irreg);
     delete c;
 }
 

@@ -31,16 +31,20 @@ class OrcRef(C.Structure):
                 ("n_pos", C.c_uint64), ("k", C.c_uint32)]
 
 
-_L = None
+_LIBS = {}
 
 
-def lib():
-    global _L
-    if _L is None:
-        if not os.path.exists(ORACLE_SO):
+def mutant_so(m):
+    """The oracle built with planted mistake m (ORC_MUTANT in swm_oracle.c)."""
+    return os.path.join(ROOT, "oracle", "_build", f"libswm_oracle_m{m}.so")
+
+
+def lib(path=ORACLE_SO):
+    if path not in _LIBS:
+        if not os.path.exists(path):
             import subprocess
             subprocess.run(["make", "-C", os.path.join(ROOT, "oracle")], check=True, stdout=subprocess.DEVNULL)
-        L = C.CDLL(ORACLE_SO)
+        L = C.CDLL(path)
         vp, u32, u64 = C.c_void_p, C.c_uint32, C.c_uint64
         L.orc_index_build.argtypes = [vp, u64, u32, C.POINTER(OrcIndex)]
         L.orc_index_build.restype = C.c_int
@@ -56,8 +60,8 @@ def lib():
         L.orc_reverse_read.argtypes = [vp, vp, u32]
         L.orc_myers.argtypes = [vp, vp, vp, u32, u32, u32, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
         L.orc_myers64.argtypes = L.orc_myers.argtypes
-        _L = L
-    return _L
+        _LIBS[path] = L
+    return _LIBS[path]
 
 
 def _p(a):
@@ -93,8 +97,9 @@ class OracleIndex:
 class OracleRef:
     """The mapper's in-memory view of an index (load_ref_info, mapper_mpe.c:198-314)."""
 
-    def __init__(self, k, hash_count, positions, pos_lseeds, base_2bits):
+    def __init__(self, k, hash_count, positions, pos_lseeds, base_2bits, so=ORACLE_SO):
         self.k = k
+        self.L = lib(so)
         nb = 1 << (2 * k)
         off = np.zeros(nb + 1, dtype=np.uint64)
         np.cumsum(hash_count.astype(np.uint64), out=off[1:])
@@ -108,8 +113,8 @@ class OracleRef:
                           self.base_2bits.ctypes.data, nw, 16 * nw * 4, self.positions.size, k)
 
     @classmethod
-    def from_index(cls, ix):
-        return cls(ix.k, ix.hash_count, ix.positions, ix.pos_lseeds, ix.base_2bits)
+    def from_index(cls, ix, so=ORACLE_SO):
+        return cls(ix.k, ix.hash_count, ix.positions, ix.pos_lseeds, ix.base_2bits, so=so)
 
     def map(self, reads, lens, e, threads=0):
         reads = np.ascontiguousarray(reads, dtype=np.uint8)
@@ -117,7 +122,7 @@ class OracleRef:
         n, stride = reads.shape
         hits = np.zeros(n, dtype=HIT_DTYPE)
         nv = C.c_uint64(0)
-        rc = lib().orc_map(C.byref(self.ref), _p(reads), stride, _p(lens), n, e, _p(hits), C.byref(nv), threads)
+        rc = self.L.orc_map(C.byref(self.ref), _p(reads), stride, _p(lens), n, e, _p(hits), C.byref(nv), threads)
         if rc != 0:
             raise ValueError(f"orc_map rc={rc}")
         return hits, nv.value
@@ -128,7 +133,7 @@ class OracleRef:
         n, stride = reads.shape
         hits = np.zeros(n, dtype=HIT_DTYPE)
         rp, nr, cp, ncd = C.c_void_p(), C.c_uint64(), C.c_void_p(), C.c_uint64()
-        rc = lib().orc_map_trace(C.byref(self.ref), _p(reads), stride, _p(lens), n, e, _p(hits), C.byref(rp),
+        rc = self.L.orc_map_trace(C.byref(self.ref), _p(reads), stride, _p(lens), n, e, _p(hits), C.byref(rp),
                                  C.byref(nr), C.byref(cp), C.byref(ncd))
         if rc != 0:
             raise ValueError(f"orc_map_trace rc={rc}")
@@ -138,8 +143,8 @@ class OracleRef:
             cands = np.frombuffer(C.string_at(cp, ncd.value * CAND_DTYPE.itemsize), dtype=CAND_DTYPE).copy() \
                 if ncd.value else np.zeros(0, CAND_DTYPE)
         finally:
-            lib().orc_free(rp)
-            lib().orc_free(cp)
+            self.L.orc_free(rp)
+            self.L.orc_free(cp)
         return hits, recs, cands
 
 
