@@ -112,6 +112,27 @@ void swm_index_host_free(swm_index_host_t *ix);
  * is written with one contig of the given name when contigs == NULL. */
 int swm_index_write(const swm_index_host_t *ix, const char *prefix, const char *contig_name);
 
+/* ---- FASTA/FASTQ records (replaces kseq_init / kseq_read / kseq_destroy,
+ *      kseq.h:161-237, which the reference's mapper and index builder use) ---- */
+
+typedef struct swm_seqfile swm_seqfile_t;
+typedef struct swm_seqrec {
+    char *name, *seq, *qual;            /* NUL-terminated; qual empty for FASTA */
+    size_t name_len, seq_len, qual_len;
+    size_t name_cap, seq_cap, qual_cap;
+} swm_seqrec_t;
+
+/* kseq's parse rules exactly: the name ends at the first isspace()
+ * character, sequence and quality lines lose one trailing '\r', a header at
+ * the end of the file is a record with an empty sequence.  r starts zeroed
+ * and is reused record to record.  Returns the sequence length, -1 at the
+ * end of the file, -2 for a quality string of another length, -3 when out
+ * of memory (kseq_read's codes). */
+swm_seqfile_t *swm_seqfile_open(const char *path);
+int swm_seqfile_next(swm_seqfile_t *s, swm_seqrec_t *r);
+void swm_seqfile_close(swm_seqfile_t *s);
+void swm_seqrec_free(swm_seqrec_t *r);
+
 /* ---- device-resident index (replaces load_ref_info, mapper_mpe.c:198-314) ---- */
 
 /* Loads <prefix>.* with seed length k (-s) to the current HIP device. */

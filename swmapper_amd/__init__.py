@@ -64,6 +64,12 @@ class Stats(C.Structure):
         return d
 
 
+class SeqRec(C.Structure):
+    _fields_ = [("name", C.c_void_p), ("seq", C.c_void_p), ("qual", C.c_void_p),
+                ("name_len", C.c_size_t), ("seq_len", C.c_size_t), ("qual_len", C.c_size_t),
+                ("name_cap", C.c_size_t), ("seq_cap", C.c_size_t), ("qual_cap", C.c_size_t)]
+
+
 class Records(C.Structure):
     _fields_ = [("n", C.c_uint64), ("rec", C.c_void_p), ("text_bytes", C.c_uint64), ("cigar", C.c_void_p)]
 
@@ -145,6 +151,10 @@ def lib():
                                     C.c_int),
             "swm_sam_close": ([vp], C.c_int),
             "swm_sam_set_threads": ([vp, C.c_int], C.c_int),
+            "swm_seqfile_open": ([C.c_char_p], vp),
+            "swm_seqfile_next": ([vp, C.POINTER(SeqRec)], C.c_int),
+            "swm_seqfile_close": ([vp], None),
+            "swm_seqrec_free": ([C.POINTER(SeqRec)], None),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -181,6 +191,30 @@ def synth_reads(genome, n_reads, len_min=150, len_max=150, err_min=0.025, err_ma
                                                     err_max, indel_share, seed, _ptr(reads), stride, _ptr(lens),
                                                     _ptr(tpos), _ptr(tstr)))
     return reads, lens, tpos, tstr
+
+
+# ---------------------------------------------------------------- FASTA/FASTQ
+
+def read_fastx(path):
+    """Records of a FASTA/FASTQ file as the reference's kseq_read returns
+    them: a list of (name, seq, qual) bytes, and the last return code
+    (-1 at a clean end, -2 for a quality string of another length)."""
+    L = lib()
+    f = L.swm_seqfile_open(path.encode())
+    if not f:
+        raise SwmError("swm_seqfile_open", -2)
+    rec = SeqRec()
+    out = []
+    try:
+        while True:
+            rc = L.swm_seqfile_next(f, C.byref(rec))
+            if rc < 0:
+                return out, rc
+            out.append((C.string_at(rec.name, rec.name_len), C.string_at(rec.seq, rec.seq_len),
+                        C.string_at(rec.qual, rec.qual_len)))
+    finally:
+        L.swm_seqfile_close(f)
+        L.swm_seqrec_free(C.byref(rec))
 
 
 # ---------------------------------------------------------------- index

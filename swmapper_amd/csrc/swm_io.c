@@ -13,6 +13,7 @@
  *   <p>.positions   u32 x M              (:84-90)
  *   <p>.pos_lseeds  u16 x M              (:117-121)
  */
+#include <ctype.h>
 #include <errno.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -23,9 +24,20 @@
 
 /* ---------------------------------------------------------------- FASTA/Q */
 
-/* Record reader with kseq's rules (kseq.h:197-237): the name runs to the
- * first space/tab/newline; sequence lines are concatenated, empty lines are
- * skipped, and a line starting with '>', '@' or '+' ends the sequence. */
+/* Record reader with kseq's rules (kseq_read, kseq.h:197-237, over
+ * ks_getuntil2, :99-147):
+ *   - records start at '>' or '@' (anything before the first is skipped);
+ *   - the name runs to the first isspace() character (' ', \t, \n, \v, \f,
+ *     \r); unless that was '\n' the rest of the line is the comment;
+ *   - a sequence line starting with '>', '@' or '+' ends the sequence,
+ *     empty lines are skipped, other lines are appended whole; after each
+ *     line one trailing '\r' is dropped when the sequence so far is longer
+ *     than one character (so a CR-only first line keeps its '\r');
+ *   - after '+' the rest of that line is skipped and whole quality lines
+ *     are appended (with the same '\r' rule) until there are at least as
+ *     many as sequence characters; a different count is an error (-2);
+ *   - a header with nothing after it at the end of the file is a record
+ *     with an empty sequence. */
 struct swm_seqfile {
     FILE *f;
     int last;    /* first char of the next header, already consumed */
@@ -33,6 +45,7 @@ struct swm_seqfile {
 
 swm_seqfile_t *swm_seqfile_open(const char *path)
 {
+    if (!path) return NULL;
     FILE *f = fopen(path, "rb");
     if (!f) return NULL;
     swm_seqfile_t *s = (swm_seqfile_t *)calloc(1, sizeof(*s));
@@ -60,50 +73,64 @@ static int grow(char **buf, size_t *cap, size_t need)
     return 0;
 }
 
+/* ks_getuntil2(ks, KS_SEP_LINE, str, 0, append = 1): the rest of the line
+ * appended; -1 when the file ended before any character, -3 on no memory. */
+static int line_append(FILE *f, char **buf, size_t *len, size_t *cap)
+{
+    int c, got = 0;
+    while ((c = fgetc(f)) != EOF) {
+        got = 1;
+        if (c == '\n') break;
+        if (grow(buf, cap, *len + 2)) return -3;
+        (*buf)[(*len)++] = (char)c;
+    }
+    if (!got) return -1;
+    if (*len > 1 && (*buf)[*len - 1] == '\r') --*len;
+    return 0;
+}
+
 int swm_seqfile_next(swm_seqfile_t *s, swm_seqrec_t *r)
 {
-    int c;
+    if (!s || !r) return -1;
+    int c = 0;
     if (s->last == 0) {
         while ((c = fgetc(s->f)) != EOF && c != '>' && c != '@') {}
         if (c == EOF) return -1;
         s->last = c;
     }
     r->name_len = r->seq_len = r->qual_len = 0;
-    while ((c = fgetc(s->f)) != EOF && c != ' ' && c != '\t' && c != '\n') {
+    if (grow(&r->name, &r->name_cap, 2) || grow(&r->seq, &r->seq_cap, 2) || grow(&r->qual, &r->qual_cap, 2))
+        return -3;
+    /* the name: ks_getuntil(ks, KS_SEP_SPACE, ...) */
+    int got = 0, delim = 0;
+    while ((c = fgetc(s->f)) != EOF) {
+        got = 1;
+        if (isspace(c)) { delim = c; break; }
         if (grow(&r->name, &r->name_cap, r->name_len + 2)) return -3;
         r->name[r->name_len++] = (char)c;
     }
-    if (r->name_cap == 0 && grow(&r->name, &r->name_cap, 2)) return -3;
     r->name[r->name_len] = 0;
-    if (c == EOF) { s->last = 0; return r->name_len ? -1 : -1; }
-    if (c != '\n') while ((c = fgetc(s->f)) != EOF && c != '\n') {}
-    if (grow(&r->seq, &r->seq_cap, 2)) return -3;
+    if (!got) return -1;
+    if (delim != '\n') while ((c = fgetc(s->f)) != EOF && c != '\n') {}   /* the comment */
     while ((c = fgetc(s->f)) != EOF && c != '>' && c != '+' && c != '@') {
         if (c == '\n') continue;
         if (grow(&r->seq, &r->seq_cap, r->seq_len + 2)) return -3;
         r->seq[r->seq_len++] = (char)c;
-        while ((c = fgetc(s->f)) != EOF && c != '\n') {
-            if (grow(&r->seq, &r->seq_cap, r->seq_len + 2)) return -3;
-            r->seq[r->seq_len++] = (char)c;
-        }
-        if (c == EOF) break;
+        if (line_append(s->f, &r->seq, &r->seq_len, &r->seq_cap) == -3) return -3;
     }
+    s->last = (c == '>' || c == '@') ? c : 0;
     r->seq[r->seq_len] = 0;
-    if (c == '>' || c == '@') s->last = c;
-    else s->last = 0;
+    r->qual[0] = 0;
     if (c != '+') return (int)(r->seq_len > 0x7fffffff ? 0x7fffffff : r->seq_len);
     while ((c = fgetc(s->f)) != EOF && c != '\n') {}
     if (c == EOF) return -2;
-    if (grow(&r->qual, &r->qual_cap, r->seq_len + 2)) return -3;
-    while (r->qual_len < r->seq_len && (c = fgetc(s->f)) != EOF) {
-        if (c == '\n') continue;
-        r->qual[r->qual_len++] = (char)c;
-    }
+    int q;
+    while ((q = line_append(s->f, &r->qual, &r->qual_len, &r->qual_cap)) >= 0 && r->qual_len < r->seq_len) {}
+    if (q == -3) return -3;
     r->qual[r->qual_len] = 0;
-    while ((c = fgetc(s->f)) != EOF && c != '\n') {}
     s->last = 0;
     if (r->qual_len != r->seq_len) return -2;
-    return (int)r->seq_len;
+    return (int)(r->seq_len > 0x7fffffff ? 0x7fffffff : r->seq_len);
 }
 
 void swm_seqrec_free(swm_seqrec_t *r)
