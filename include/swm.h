@@ -89,6 +89,8 @@ typedef struct swm_stats {
     /* (read, strand) items whose selected 12-mers include a bucket that the
      * reference's index holds out of ascending order */
     uint64_t n_irregular_items;
+    /* 1 when this run captured and instantiated its graph (graph form) */
+    uint64_t n_graph_captures;
 } swm_stats_t;
 
 typedef struct swm_ctx swm_ctx_t;

@@ -56,7 +56,8 @@ class Stats(C.Structure):
                 ("ms_host_enqueue", C.c_double), ("ms_host_wait", C.c_double),
                 ("cand_hist", C.c_uint64 * 24), ("max_cands", C.c_uint64),
                 ("n_heavy_items", C.c_uint64), ("n_large_batches", C.c_uint64), ("n_heavy_reads", C.c_uint64),
-                ("n_giant_items", C.c_uint64), ("n_giant_reads", C.c_uint64), ("n_irregular_items", C.c_uint64)]
+                ("n_giant_items", C.c_uint64), ("n_giant_reads", C.c_uint64), ("n_irregular_items", C.c_uint64),
+                ("n_graph_captures", C.c_uint64)]
 
     def as_dict(self):
         d = {f: getattr(self, f) for f, _ in self._fields_}

@@ -3239,7 +3239,10 @@ struct swm_ws {
     }
 };
 
-#define SLOT_GRAPHS 8
+/* captured graphs kept per slot: at least the serving loop's launch sizes
+ * (powers of two from 1024 to 2^18 reads, nine), so that after its warmup
+ * no launch captures */
+#define SLOT_GRAPHS 16
 
 /* One stream with its own workspace: batches on different slots run
  * concurrently over the shared, read-only index. */
@@ -3778,6 +3781,7 @@ static int run_graph(swm_ctx_t *ctx, swm_slot *sl, const char *d_reads, uint64_t
     const uint32_t seln = 4 * ((e + 1) < 16 ? (e + 1) : 16);
     Pipe P;
     int rc;
+    bool captured = false;
     if (!ent || ent->gen != sl->ws.gen) {
         if (!ent) {
             ent = &sl->g[0];
@@ -3806,18 +3810,27 @@ static int run_graph(swm_ctx_t *ctx, swm_slot *sl, const char *d_reads, uint64_t
             (rc = pipe_alloc_c(&sl->ws, cap_cand, P)))
             return rc;
         hipGraph_t graph = nullptr;
-        SWM_HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-        int crc = enq_a(ctx, P, d_reads, stride, d_lens, e, cap_pos, cap_chunk, st, nullptr);
-        if (!crc) crc = enq_b(ctx, P, cap_cand, st, nullptr);
-        if (!crc) crc = enq_c(ctx, P, d_lens, e, d_out, g_cand, st, nullptr);
-        const hipError_t ce = hipStreamEndCapture(st, &graph);
-        if (crc || ce != hipSuccess) {
-            if (graph) (void)hipGraphDestroy(graph);
-            return crc ? crc : SWM_EDEVICE;
+        hipError_t ie;
+        {
+            /* a capture of a server's slot holds the slots' gate: no other
+             * slot enqueues its run while a graph is captured and
+             * instantiated */
+            std::unique_lock<std::mutex> glk;
+            if (sl->gate) glk = std::unique_lock<std::mutex>(sl->gate->mu);
+            SWM_HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            int crc = enq_a(ctx, P, d_reads, stride, d_lens, e, cap_pos, cap_chunk, st, nullptr);
+            if (!crc) crc = enq_b(ctx, P, cap_cand, st, nullptr);
+            if (!crc) crc = enq_c(ctx, P, d_lens, e, d_out, g_cand, st, nullptr);
+            const hipError_t ce = hipStreamEndCapture(st, &graph);
+            if (crc || ce != hipSuccess) {
+                if (graph) (void)hipGraphDestroy(graph);
+                return crc ? crc : SWM_EDEVICE;
+            }
+            ie = hipGraphInstantiate(&ent->exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
         }
-        const hipError_t ie = hipGraphInstantiate(&ent->exec, graph, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(graph);
         if (ie != hipSuccess) { ent->exec = nullptr; return SWM_EDEVICE; }
+        captured = true;
         ent->used = true;
         ent->reads = d_reads; ent->lens = d_lens; ent->out = d_out;
         ent->n = n; ent->stride = stride; ent->e = e; ent->opt = ctx->opt; ent->gen = sl->ws.gen;
@@ -3846,6 +3859,7 @@ static int run_graph(swm_ctx_t *ctx, swm_slot *sl, const char *d_reads, uint64_t
         return run_synced(ctx, sl, d_reads, stride, d_lens, n, e, d_out, h_out, n_out);
     }
     stats_from_counters(&sl->stats, n, hc);
+    sl->stats.n_graph_captures = captured ? 1 : 0;
     float ms;
     (void)hipEventElapsedTime(&ms, sl->ev[0], sl->ev[5]);
     sl->stats.ms_total = ms;

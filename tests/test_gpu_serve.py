@@ -124,3 +124,31 @@ def test_empty_and_invalid_batches(setup):
         srv.submit(np.zeros((3, 256), np.uint8), bad)
     srv.close()
     m.close()
+
+
+def test_no_capture_after_warmup(setup):
+    """After warmup every launch size (powers of two from 1024 to 2^18 reads)
+    replays a graph captured up front: traffic of mixed sizes, merged or not,
+    captures nothing on the worker threads."""
+    # a uniform genome: per-read totals vary little from batch to batch, so
+    # no batch overflows the capacities learned at warmup (an overflow would
+    # rightly recapture)
+    g = wl.genome(600_000, seed=95)
+    pool_r, pool_l, _, _ = wl.reads(g, 30_000, len_min=50, len_max=250, err=(0.01, 0.04), seed=96)
+    m = swm.Mapper.from_index(swm.build_index(g, 12))
+    srv = swm.Server(m, e=5, stride=256, n_slots=2, graph=True, max_batch=65536, max_launch=262144)
+    _say("serve warmup")
+    srv.warmup(pool_r, pool_l)
+    rng = np.random.default_rng(94)
+    work = []
+    for n in (1000, 1500, 2100, 3000, 5000, 9000, 17000, 33000, 65000, 40000, 1024, 64000):
+        idx = rng.integers(0, pool_l.size, n)
+        work.append((pool_r[idx], pool_l[idx]))
+    _say("serve mixed sizes")
+    tickets = [srv.submit(r, l) for r, l in work]
+    for t, (r, l) in zip(tickets, work):
+        got, info = srv.wait(t)
+        assert info["stats"]["n_graph_captures"] == 0, info["launch_reads"]
+        _assert_same(got, m.map(r, l, 5), f"ticket {t}")
+    srv.close()
+    m.close()
