@@ -260,6 +260,8 @@ def main():
     ap.add_argument("--no-records", action="store_true", help="skip the records + SAM measurement")
     ap.add_argument("--set", action="append", default=[], metavar="NAME=VALUE",
                     help="set a tunable (swm_ctx_set_option by name, e.g. seed_cache=0) before timing")
+    ap.add_argument("--index-only", type=int, default=0, metavar="K",
+                    help="only build the configuration's index K times and print the build times (for profiling)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="one 256 MiB device-to-device copy first (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -287,6 +289,10 @@ def main():
     t0 = time.time()
     g = swm.synth_genome(cfg["n"], seed=GENOME_SEED, kind=cfg["kind"], frac=cfg["frac"])
     log(f"genome {cfg['n']} bp in {time.time() - t0:.1f}s")
+    if args.index_only:
+        times = [swm.build_index(g, 12).build_ms for _ in range(args.index_only)]
+        print(json.dumps({"index_build_ms": [round(t, 3) for t in times], "bases": cfg["n"]}), flush=True)
+        return
     ix = swm.build_index(g, 12)
     ix2 = swm.build_index(g, 12)      # timed again with the device warm
     build_ms = min(ix.build_ms, ix2.build_ms)

@@ -14,22 +14,22 @@
  *              the 2-bit seed words, the N test from the mask, the 12-bit
  *              residual (get_seed_from_enc_bits :280-331) from the planes;
  *              the last starts (the tail below) from the sequence itself
- *   sort       stable LSD radix sort on the code bits only (3 passes for
- *              k=12): the counting sort's bucket order with positions
- *              ascending (:343-352)
- *   k_finish64 .positions and .pos_lseeds in bucket order, and the bucket
- *              starts (store_positions' offsets :337-340) from the first
- *              key of each code
+ *   sort       stable LSD radix sort on the code bits only (3 passes of 8
+ *              bits for k=12, k_rs_*): the counting sort's bucket order
+ *              with positions ascending (:343-352); its last pass writes
+ *              .positions and .pos_lseeds in bucket order and the bucket
+ *              starts (store_positions' offsets :337-340) at each change of
+ *              code, k_rs_fix those at the first key of each tile's run
  *   k_counts   .hash_count and the END offsets from the starts
  *   k_tail     one wave replays the scatter for the last starts, whose
  *              scatter reads the zeroed words, so that slots, overflow and
  *              the END offsets of .hash_offset (Q19) match the serial build
  * Genomes whose key would not fit 64 bits (more than 2^(52-2k) sampled
- * starts) take the pair sort: key = code, value = residual << 32 | p.
+ * starts) sort code << 32 | p on the code bits instead and take the residual
+ * from the planes afterwards.
  */
 #include "swm_internal.h"
 
-#include <hipcub/hipcub.hpp>
 #include <stdlib.h>
 #include <string.h>
 
@@ -135,14 +135,11 @@ __device__ __forceinline__ uint32_t lseed_at(const uint32_t *__restrict__ b2, ui
     return fl | (fh << 6);
 }
 
-/* One thread per sampled start p = 0, 4, 8, ... <= N-k (in genome order, so
- * the plane reads are coalesced): the k-mer code and N-freeness of the
- * histogram (:253-269) as the sort key (4^k for a start with an N), and the
- * value (residual << 32 | p) that the stable sort carries into bucket
- * order. */
+/* Large genomes: one thread per sampled start p = 0, 4, 8, ... <= N-k, the
+ * key code << 32 | p with the k-mer code and N-freeness of the histogram
+ * (:253-269), 4^k for a start with an N (sorted after every valid one). */
 __global__ void __launch_bounds__(256) k_keys(const char *__restrict__ seq, uint64_t nsamp, uint32_t k,
-                                              const uint32_t *__restrict__ b2, uint32_t *__restrict__ keys,
-                                              unsigned long long *__restrict__ vals)
+                                              unsigned long long *__restrict__ keys)
 {
     uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= nsamp) return;
@@ -161,8 +158,7 @@ __global__ void __launch_bounds__(256) k_keys(const char *__restrict__ seq, uint
             code |= (uint32_t)(c < 0 ? 0 : c) << (2 * i);
         }
     }
-    keys[j] = valid ? code : (1u << (2 * k));
-    vals[j] = ((unsigned long long)lseed_at(b2, p, k) << 32) | (uint32_t)p;
+    keys[j] = ((unsigned long long)(valid ? code : (1u << (2 * k))) << 32) | (uint32_t)p;
 }
 
 #define PACK_NT 256
@@ -193,12 +189,22 @@ __device__ __forceinline__ bool start_code(const char *__restrict__ seq, const u
     return valid;
 }
 
-/* Pass 1: valid starts per tile of PACK_TILE starts. */
+/* Pass 1: valid starts per tile of PACK_TILE starts.  Also clears the
+ * radix sort's digit counts, tile counters and first look-back status. */
 __global__ void __launch_bounds__(PACK_NT) k_pack_count(const char *__restrict__ seq, uint64_t nsamp, uint32_t k,
                                                         uint64_t t0, const uint32_t *__restrict__ s2,
-                                                        const uint32_t *__restrict__ b3, uint32_t *__restrict__ tcnt)
+                                                        const uint32_t *__restrict__ b3, uint32_t *__restrict__ tcnt,
+                                                        uint32_t *__restrict__ rs_hist, uint32_t *__restrict__ rs_ctr,
+                                                        uint32_t *__restrict__ rs_status, uint64_t rs_status_n)
 {
     __shared__ uint32_t s_w[PACK_NT / 64];
+    {
+        /* grid-stride: the grid can be a single workgroup (up to 1024 starts) */
+        const uint64_t gid = (uint64_t)blockIdx.x * PACK_NT + threadIdx.x, gs = (uint64_t)gridDim.x * PACK_NT;
+        for (uint64_t i = gid; i < 4 * 256; i += gs) rs_hist[i] = 0;
+        if (gid < 4) rs_ctr[gid] = 0;
+        for (uint64_t i = gid; i < rs_status_n; i += gs) rs_status[i] = 0;
+    }
     uint32_t c = 0;
 #pragma unroll
     for (int q = 0; q < PACK_PER; q++) {
@@ -213,17 +219,27 @@ __global__ void __launch_bounds__(PACK_NT) k_pack_count(const char *__restrict__
 
 /* Pass 2: the keys of the valid starts at their rank (genome order).  A
  * tile is PACK_PER rows of PACK_NT consecutive starts, so each row's stores
- * are contiguous; one scan of four 16-bit row counters ranks all rows. */
+ * are contiguous; one scan of four 16-bit row counters ranks all rows.
+ * Workgroups take tiles grid-stride and count the radix sort's digits of
+ * every key they write (its histogram pass), one global add per digit and
+ * workgroup at the end. */
 __global__ void __launch_bounds__(PACK_NT) k_pack_write(const char *__restrict__ seq, uint64_t nsamp, uint32_t k,
                                                         uint64_t t0, const uint32_t *__restrict__ s2,
                                                         const uint32_t *__restrict__ b3,
                                                         const uint32_t *__restrict__ b2,
                                                         const uint32_t *__restrict__ toff,
                                                         const uint32_t *__restrict__ tcnt, uint32_t ntiles,
-                                                        unsigned long long *__restrict__ keys)
+                                                        unsigned long long *__restrict__ keys,
+                                                        uint32_t *__restrict__ hist, uint32_t npass)
 {
     __shared__ uint64_t s_w[PACK_NT / 64];
-    const uint64_t tile0 = (uint64_t)blockIdx.x * PACK_TILE;
+    __shared__ uint32_t s_h[PACK_NT / 64][4 * 256];   /* one histogram per wave: waves do not contend */
+    uint32_t *hw = s_h[threadIdx.x >> 6];
+    for (uint32_t i = threadIdx.x; i < (PACK_NT / 64) * 4 * 256; i += PACK_NT) (&s_h[0][0])[i] = 0;
+    __syncthreads();
+    const uint32_t hbits = 2 * k;
+    for (uint32_t tl = blockIdx.x; tl < ntiles; tl += gridDim.x) {
+    const uint64_t tile0 = (uint64_t)tl * PACK_TILE;
     uint32_t code[PACK_PER];
     uint64_t flags = 0;
 #pragma unroll
@@ -254,7 +270,7 @@ __global__ void __launch_bounds__(PACK_NT) k_pack_write(const char *__restrict__
     /* starts with an N go after all valid ones, with the largest code, so
      * the sort leaves them at the end (M = valid starts) */
     const uint64_t M = (uint64_t)toff[ntiles - 1] + tcnt[ntiles - 1];
-    const uint64_t tbase = toff[blockIdx.x];
+    const uint64_t tbase = toff[tl];
     const unsigned long long cm = (1ull << (2 * k)) - 1ull;
     uint64_t rows_before = 0;
 #pragma unroll
@@ -263,40 +279,29 @@ __global__ void __launch_bounds__(PACK_NT) k_pack_write(const char *__restrict__
         const uint64_t rank = rows_before + ((excl >> (16 * q)) & 0xffffu);   /* valid before j in the tile */
         rows_before += (tot >> (16 * q)) & 0xffffu;
         if (j >= nsamp) continue;
+        unsigned long long key;
         if ((flags >> (16 * q)) & 1u) {
             const uint64_t res = lseed_at(b2, j * SWM_SEED_STEP, k);
-            keys[tbase + rank] = (unsigned long long)code[q] | (res << (2 * k)) | (j << (2 * k + 12));
+            key = (unsigned long long)code[q] | (res << (2 * k)) | (j << (2 * k + 12));
+            keys[tbase + rank] = key;
         } else {
-            keys[M + (tile0 - tbase) + ((j - tile0) - rank)] = cm | (j << (2 * k + 12));
+            key = cm | (j << (2 * k + 12));
+            keys[M + (tile0 - tbase) + ((j - tile0) - rank)] = key;
+        }
+        for (uint32_t p = 0; p < npass; p++) {
+            const uint32_t sh = 8 * p, bits = min(8u, hbits - sh);
+            atomicAdd(&hw[p * 256 + ((uint32_t)(key >> sh) & ((1u << bits) - 1u))], 1u);
         }
     }
-}
-
-/* From the sorted 64-bit keys (code in the low 2k bits): .positions and
- * .pos_lseeds in bucket order, and the bucket starts -- the key at the
- * first index of each code sets the start of that code and of every empty
- * code below it down to the previous key's, the end closes the codes above
- * the last key with M.  Every start is written exactly once (start[nb] = M),
- * so no count pass and no scan are needed. */
-__global__ void __launch_bounds__(256) k_finish64(const uint64_t *__restrict__ mp, uint32_t k, uint64_t nb,
-                                                  const unsigned long long *__restrict__ key,
-                                                  uint32_t *__restrict__ start, uint32_t *__restrict__ positions,
-                                                  uint16_t *__restrict__ lseeds)
-{
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const uint64_t m = *mp;
-    if (i > m) return;
-    const unsigned long long cm = (1ull << (2 * k)) - 1ull;
-    uint64_t hi = nb;
-    if (i < m) {
-        const unsigned long long x = key[i];
-        positions[i] = (uint32_t)((x >> (2 * k + 12)) * SWM_SEED_STEP);
-        lseeds[i] = (uint16_t)((x >> (2 * k)) & 0xfffu);
-        hi = x & cm;
+    __syncthreads();       /* s_w is reused by the next tile's scan */
     }
-    const uint64_t lo = i > 0 ? (key[i - 1] & cm) : ~0ull;   /* ~0: from code 0 */
-    if (hi == lo) return;
-    for (uint64_t c = lo + 1; c <= hi; c++) start[c] = (uint32_t)i;      /* codes (lo, hi] start at i */
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < npass * 256; i += PACK_NT) {
+        uint32_t c = 0;
+#pragma unroll
+        for (int q = 0; q < PACK_NT / 64; q++) c += s_h[q][i];
+        if (c) atomicAdd(&hist[i], c);
+    }
 }
 
 /* Counts and END offsets from the starts. */
@@ -323,35 +328,37 @@ __global__ void k_total(const uint32_t *__restrict__ toff, const uint32_t *__res
 /* Bucket counts from the sorted keys: the thread at the end of each run of
  * one valid code gallops back to the run's start (runs average a few
  * entries; long ones cost log steps). */
-__global__ void __launch_bounds__(256) k_runs(uint64_t nsamp, uint32_t nb, const uint32_t *__restrict__ key,
+__global__ void __launch_bounds__(256) k_runs(uint64_t nsamp, uint32_t nb, const unsigned long long *__restrict__ kv,
                                               uint32_t *__restrict__ count)
 {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nsamp) return;
-    const uint32_t c = key[i];
-    if (c >= nb || (i + 1 < nsamp && key[i + 1] == c)) return;
-    uint64_t lo = i, step = 1;             /* key[lo] == c */
-    while (step <= lo && key[lo - step] == c) { lo -= step; step <<= 1; }
-    uint64_t a = step <= lo ? lo - step : 0;   /* key[a] != c or a == 0 */
-    if (key[a] == c) lo = a;
+    auto key = [&](uint64_t x) { return (uint32_t)(kv[x] >> 32); };
+    const uint32_t c = key(i);
+    if (c >= nb || (i + 1 < nsamp && key(i + 1) == c)) return;
+    uint64_t lo = i, step = 1;             /* key(lo) == c */
+    while (step <= lo && key(lo - step) == c) { lo -= step; step <<= 1; }
+    uint64_t a = step <= lo ? lo - step : 0;   /* key(a) != c or a == 0 */
+    if (key(a) == c) lo = a;
     else {
-        while (lo - a > 1) {               /* key[a] != c, key[lo] == c */
+        while (lo - a > 1) {               /* key(a) != c, key(lo) == c */
             const uint64_t mid = a + (lo - a) / 2;
-            if (key[mid] == c) lo = mid; else a = mid;
+            if (key(mid) == c) lo = mid; else a = mid;
         }
     }
     count[c] = (uint32_t)(i + 1 - lo);
 }
 
-/* positions and residuals in bucket order */
+/* positions and residuals in bucket order (the residual from the planes) */
 __global__ void __launch_bounds__(256) k_split(const uint64_t *__restrict__ mp, const unsigned long long *__restrict__ v,
+                                               const uint32_t *__restrict__ b2, uint32_t k,
                                                uint32_t *__restrict__ positions, uint16_t *__restrict__ lseeds)
 {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= *mp) return;
-    const unsigned long long x = v[i];
-    positions[i] = (uint32_t)x;
-    lseeds[i] = (uint16_t)(x >> 32);
+    const uint32_t p = (uint32_t)v[i];
+    positions[i] = p;
+    lseeds[i] = (uint16_t)lseed_at(b2, p, k);
 }
 
 /* M = start[nb] (the pair path's total). */
@@ -437,10 +444,470 @@ __global__ void __launch_bounds__(64) k_tail(const char *__restrict__ seq, uint6
     if (live && svalid) end[scode] = start[scode] + (count[scode] - t_same_s) + s_writes_s;
 }
 
+/* ------------------------------------------------------------- scans */
+
+#define SC_NT 256
+#define SC_PER 16
+#define SC_TILE (SC_NT * SC_PER)
+
+/* Exclusive scan of u32 (sums fit u32: they are index offsets) in two
+ * launches: tile sums, then each tile adds the sums before it and scans in
+ * LDS (coalesced in and out). */
+__global__ void __launch_bounds__(SC_NT) k_sc_reduce(const uint32_t *__restrict__ in, uint64_t n,
+                                                     uint32_t *__restrict__ sums)
+{
+    __shared__ uint32_t s_w[SC_NT / 64];
+    const uint64_t base = (uint64_t)blockIdx.x * SC_TILE;
+    uint32_t v = 0;
+#pragma unroll
+    for (int q = 0; q < SC_PER; q++) {
+        const uint64_t g = base + (uint64_t)q * SC_NT + threadIdx.x;
+        v += g < n ? in[g] : 0u;
+    }
+    uint32_t tot;
+    (void)block_incl_scan_u32<SC_NT>(v, s_w, &tot);
+    if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+}
+
+__global__ void __launch_bounds__(SC_NT) k_sc_down(const uint32_t *__restrict__ in, uint64_t n,
+                                                   const uint32_t *__restrict__ sums, uint32_t *__restrict__ out)
+{
+    __shared__ uint32_t s_w[SC_NT / 64];
+    __shared__ uint32_t s_v[SC_TILE + SC_TILE / 16];
+    const uint64_t base = (uint64_t)blockIdx.x * SC_TILE;
+    uint32_t pre = 0;
+    for (uint32_t b = threadIdx.x; b < blockIdx.x; b += SC_NT) pre += sums[b];
+    uint32_t tile_off;
+    (void)block_incl_scan_u32<SC_NT>(pre, s_w, &tile_off);
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < SC_PER; q++) {
+        const uint32_t e = (uint32_t)q * SC_NT + threadIdx.x;
+        const uint64_t g = base + e;
+        s_v[e + (e >> 4)] = g < n ? in[g] : 0u;
+    }
+    __syncthreads();
+    uint32_t v[SC_PER], t = 0;
+#pragma unroll
+    for (int q = 0; q < SC_PER; q++) {
+        const uint32_t e = threadIdx.x * SC_PER + (uint32_t)q;
+        v[q] = s_v[e + (e >> 4)];
+        t += v[q];
+    }
+    uint32_t tot;
+    uint32_t run = block_incl_scan_u32<SC_NT>(t, s_w, &tot) - t + tile_off;
+#pragma unroll
+    for (int q = 0; q < SC_PER; q++) {
+        const uint32_t e = threadIdx.x * SC_PER + (uint32_t)q;
+        s_v[e + (e >> 4)] = run;
+        run += v[q];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < SC_PER; q++) {
+        const uint32_t e = (uint32_t)q * SC_NT + threadIdx.x;
+        const uint64_t g = base + e;
+        if (g < n) out[g] = s_v[e + (e >> 4)];
+    }
+}
+
+/* ------------------------------------------------------ radix sort */
+
+/* Stable LSD radix sort of 64-bit keys on bits [lo, hi), 8 bits a pass.
+ * The digit starts of every pass come from one histogram of the input
+ * (k_rs_hist, or the kernel that writes the keys).  Each pass (k_rs_pass)
+ * takes tiles of 4096 keys in the order their workgroups start (a counter,
+ * so every tile's predecessors are running): it counts the tile's digits
+ * and publishes the counts at once, finds its global offsets by a decoupled
+ * look-back over the tiles before it (RS_LB statuses read per step, summed
+ * up to the first that includes every tile before it), ranks the keys in
+ * the tile and scatters them through LDS so that each digit's run of the
+ * tile is written contiguously.  Measured on 64M keys (one pass): 438 us
+ * with an 8-wide look-back step, 465 with 16, 485 with 24; 265-300 us with
+ * no waiting at all; 490 us for the wait-free form (a count pass, a scan of
+ * the tiles' counts, a scatter pass); two-level look-back over groups of
+ * 16-64 tiles 495-520 us.
+ *
+ * Stability: wave w of a tile holds keys w*1024 + r*64 + lane (r = 0..15),
+ * the keys' own order; a wave ranks its rounds in order against a private
+ * per-digit counter (lanes of one digit by a match over the digit's bits),
+ * then the waves' counters are offset in wave order. */
+#define RS_NT 256
+#define RS_PER 16
+#define RS_WAVES (RS_NT / 64)
+#define RS_TILE (RS_NT * RS_PER)
+#define RS_AGG 0x40000000u       /* status: the tile's own count */
+#define RS_PRE 0x80000000u       /* status: the count of this and every earlier tile */
+#define RS_VAL 0x3fffffffu       /* counts below 2^30 (fewer than 2^30 sampled starts) */
+#define RS_SPIN (1u << 24)       /* a look-back gives up after this many waits (and flags it) */
+#define RS_LB 8                  /* predecessors a look-back step reads at once */
+
+__global__ void __launch_bounds__(256) k_rs_hist(const unsigned long long *__restrict__ keys, uint64_t n,
+                                                 uint32_t lo, uint32_t hi, uint32_t npass,
+                                                 uint32_t *__restrict__ hist)
+{
+    __shared__ uint32_t h[4][4 * 256];
+    uint32_t *hw = h[threadIdx.x >> 6];
+    for (uint32_t i = threadIdx.x; i < 4 * 4 * 256; i += blockDim.x) (&h[0][0])[i] = 0;
+    __syncthreads();
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i0 < n; i0 += 4 * stride) {
+        unsigned long long x[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const uint64_t i = i0 + (uint64_t)u * stride;
+            x[u] = keys[i < n ? i : 0];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            if (i0 + (uint64_t)u * stride >= n) continue;
+            for (uint32_t p = 0; p < npass; p++) {
+                const uint32_t sh = lo + 8 * p, bits = min(8u, hi - sh);
+                atomicAdd(&hw[p * 256 + ((uint32_t)(x[u] >> sh) & ((1u << bits) - 1u))], 1u);
+            }
+        }
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < npass * 256; i += blockDim.x) {
+        const uint32_t c = h[0][i] + h[1][i] + h[2][i] + h[3][i];
+        if (c) atomicAdd(&hist[i], c);
+    }
+}
+
+/* Digit starts of passes p0 .. npass-1: exclusive scans of the 256 counts. */
+__global__ void __launch_bounds__(256) k_rs_bases(const uint32_t *__restrict__ hist, uint32_t p0, uint32_t npass,
+                                                  uint32_t *__restrict__ base)
+{
+    __shared__ uint32_t s_w[4];
+    for (uint32_t p = p0; p < npass; p++) {
+        const uint32_t c = hist[p * 256 + threadIdx.x];
+        uint32_t tot;
+        const uint32_t incl = block_incl_scan_u32<256>(c, s_w, &tot);
+        base[p * 256 + threadIdx.x] = incl - c;
+        __syncthreads();
+    }
+}
+
+/* What the last pass of the index sort writes instead of keys: for the
+ * sorted key at index i < M (key = code | residual << 2k | j << (2k + 12)),
+ * positions[i] = j * SWM_SEED_STEP and lseeds[i] = residual, and the bucket
+ * starts of the code boundaries inside a tile's digit run (and of the codes
+ * above the last valid key).  The boundaries at the first key of each run
+ * are left to k_rs_fix, for which each tile records per digit the code of
+ * its run's first key (rf) and the code of its last key plus one (rl, 0 for
+ * an empty run). */
+struct SortFinish {
+    const uint64_t *m;
+    uint32_t k;
+    uint64_t nb;
+    uint32_t *start, *positions;
+    uint16_t *lseeds;
+    uint32_t *rf, *rl;
+};
+
+template <bool FIN>
+__global__ void __launch_bounds__(RS_NT) k_rs_pass(const unsigned long long *__restrict__ in,
+                                                   unsigned long long *__restrict__ out, uint64_t n, uint32_t shift,
+                                                   uint32_t bits, const uint32_t *__restrict__ base,
+                                                   uint32_t *__restrict__ status, uint32_t *__restrict__ tile_ctr,
+                                                   uint32_t *__restrict__ err, SortFinish fin)
+{
+    __shared__ unsigned long long s_k[RS_TILE];
+    __shared__ uint32_t s_wc[RS_WAVES * 256];
+    __shared__ uint32_t s_dstart[256], s_gofs[256], s_cnt[256], s_w[RS_WAVES];
+    __shared__ uint32_t s_tile;
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
+    const uint32_t mask = (1u << bits) - 1u;
+    if (tid == 0) s_tile = atomicAdd(tile_ctr, 1u);
+    for (uint32_t i = tid; i < RS_WAVES * 256; i += RS_NT) s_wc[i] = 0;
+    if (tid < 256) s_cnt[tid] = 0;
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    const uint64_t t0 = (uint64_t)tile * RS_TILE + (uint64_t)w * (RS_TILE / RS_WAVES);
+    unsigned long long key[RS_PER];
+    uint32_t rank[RS_PER];
+#pragma unroll
+    for (int r = 0; r < RS_PER; r++) {
+        const uint64_t g = t0 + (uint64_t)r * 64 + lane;
+        key[r] = in[g < n ? g : 0];
+    }
+    /* the tile's digit counts first, published at once: the tiles after it
+     * find them while this one ranks its keys */
+#pragma unroll
+    for (int r = 0; r < RS_PER; r++)
+        if (t0 + (uint64_t)r * 64 + lane < n) atomicAdd(&s_cnt[(uint32_t)(key[r] >> shift) & mask], 1u);
+    __syncthreads();
+    if (tid < 256) {
+        /* thread d: the tile's count of digit d published, then the
+         * look-back for the digit's global offset -- before the ranking, so
+         * that this tile's inclusive count is out as early as possible */
+        const uint32_t d = tid, tot = s_cnt[d];
+        uint32_t *my = status + (uint64_t)tile * 256 + d;
+        __hip_atomic_store(my, (tile == 0 ? RS_PRE : RS_AGG) | tot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        uint32_t pre = 0;
+        if (tile != 0) {
+            /* RS_LB predecessors read at once, nearest first: their counts
+             * are added up to the first inclusive one; an unpublished one
+             * stops the window, which is read again from there */
+            uint32_t spins = 0;
+            int64_t j = (int64_t)tile - 1;
+            for (;;) {
+                uint32_t x[RS_LB];
+#pragma unroll
+                for (int q = 0; q < RS_LB; q++)
+                    x[q] = j - q >= 0 ? __hip_atomic_load(status + (uint64_t)(j - q) * 256 + d, __ATOMIC_RELAXED,
+                                                          __HIP_MEMORY_SCOPE_AGENT)
+                                      : RS_PRE;
+                bool done = false;
+                int used = RS_LB;
+#pragma unroll
+                for (int q = 0; q < RS_LB; q++) {
+                    if (done || used != RS_LB) continue;
+                    if ((x[q] & (RS_AGG | RS_PRE)) == 0) { used = q; continue; }
+                    pre += x[q] & RS_VAL;
+                    if (x[q] & RS_PRE) done = true;
+                }
+                if (done) break;
+                j -= used;
+                if (used < RS_LB) {
+                    if (++spins > RS_SPIN) { atomicOr(err, 1u); break; }
+                    __builtin_amdgcn_s_sleep(1);
+                }
+            }
+            __hip_atomic_store(my, RS_PRE | ((pre + tot) & RS_VAL), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        s_gofs[d] = base[d] + pre;
+    }
+    const uint64_t below = (1ull << lane) - 1ull;
+    uint32_t *wc = s_wc + w * 256;
+#pragma unroll
+    for (int r = 0; r < RS_PER; r++) {
+        const bool v = t0 + (uint64_t)r * 64 + lane < n;
+        const uint32_t d = (uint32_t)(key[r] >> shift) & mask;
+        uint64_t peers = __ballot(v);
+        for (uint32_t b = 0; b < bits; b++) {
+            const uint64_t bb = __ballot((d >> b) & 1u);
+            peers &= ((d >> b) & 1u) ? bb : ~bb;
+        }
+        const uint32_t before = wc[d];
+        rank[r] = before + (uint32_t)__popcll(peers & below);
+        if (v && (peers & below) == 0) wc[d] = before + (uint32_t)__popcll(peers);
+    }
+    __syncthreads();
+    {
+        /* thread d: the waves' offsets within digit d and the digit's start
+         * within the tile */
+        const uint32_t d = tid;
+        uint32_t tot = 0;
+        if (d < 256) {
+#pragma unroll
+            for (int q = 0; q < RS_WAVES; q++) {
+                const uint32_t c = s_wc[q * 256 + d];
+                s_wc[q * 256 + d] = tot;
+                tot += c;
+            }
+        }
+        uint32_t all;
+        const uint32_t ds = block_incl_scan_u32<RS_NT>(tot, s_w, &all) - tot;
+        if (d < 256) s_dstart[d] = ds;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < RS_PER; r++) {
+        const bool v = t0 + (uint64_t)r * 64 + lane < n;
+        const uint32_t d = (uint32_t)(key[r] >> shift) & mask;
+        if (v) s_k[s_dstart[d] + s_wc[w * 256 + d] + rank[r]] = key[r];
+    }
+    __syncthreads();
+    const uint64_t tb = (uint64_t)tile * RS_TILE;
+    const uint32_t nv = (uint32_t)(n - tb < RS_TILE ? n - tb : RS_TILE);
+    if (!FIN) {
+        for (uint32_t i = tid; i < nv; i += RS_NT) {
+            const unsigned long long x = s_k[i];
+            const uint32_t d = (uint32_t)(x >> shift) & mask;
+            out[(uint64_t)s_gofs[d] + (i - s_dstart[d])] = x;
+        }
+        return;
+    }
+    const uint64_t m = *fin.m;
+    const uint32_t cb = 2 * fin.k;
+    const unsigned long long cm = (1ull << cb) - 1ull;
+    if (tid < 256) {
+        const uint32_t c = s_cnt[tid];
+        uint32_t f = 0, l = 0;
+        if (c) {
+            f = (uint32_t)(s_k[s_dstart[tid]] & cm);
+            l = (uint32_t)(s_k[s_dstart[tid] + c - 1] & cm) + 1u;
+        }
+        fin.rf[(uint64_t)tile * 256 + tid] = f;
+        fin.rl[(uint64_t)tile * 256 + tid] = l;
+    }
+    for (uint32_t i = tid; i < nv; i += RS_NT) {
+        const unsigned long long x = s_k[i];
+        const uint32_t d = (uint32_t)(x >> shift) & mask;
+        const uint64_t gi = (uint64_t)s_gofs[d] + (i - s_dstart[d]);
+        if (gi >= m) continue;            /* the starts with an N, after every valid one */
+        fin.positions[gi] = (uint32_t)((x >> (cb + 12)) * SWM_SEED_STEP);
+        fin.lseeds[gi] = (uint16_t)((x >> cb) & 0xfffu);
+        const uint64_t code = x & cm;
+        if (i > s_dstart[d]) {
+            /* the key before it in the output is the one before it here */
+            const uint64_t prev = s_k[i - 1] & cm;
+            for (uint64_t c = prev + 1; c <= code; c++) fin.start[c] = (uint32_t)gi;
+        }
+        if (gi + 1 == m)
+            for (uint64_t c = code + 1; c <= fin.nb; c++) fin.start[c] = (uint32_t)m;
+    }
+}
+
+/* Per chunk of `chunk` tiles and digit, the largest rl (the last code plus
+ * one of the chunk's runs of that digit, 0 if none). */
+__global__ void __launch_bounds__(256) k_rs_cmax(const uint32_t *__restrict__ rl, uint32_t ntiles, uint32_t chunk,
+                                                 uint32_t *__restrict__ cmax)
+{
+    const uint32_t t0 = blockIdx.x * chunk, t1 = min(t0 + chunk, ntiles);
+    uint32_t mx = 0;
+    for (uint32_t t = t0; t < t1; t += 16) {
+        uint32_t v[16];
+#pragma unroll
+        for (int u = 0; u < 16; u++) v[u] = t + u < t1 ? rl[(uint64_t)(t + u) * 256 + threadIdx.x] : 0u;
+#pragma unroll
+        for (int u = 0; u < 16; u++) mx = max(mx, v[u]);
+    }
+    cmax[(uint64_t)blockIdx.x * 256 + threadIdx.x] = mx;
+}
+
+/* The bucket starts at the first key of each tile's digit run, after the
+ * sort's last pass.  Its status rows hold every tile's inclusive count per
+ * digit, so run (tile t, digit d) starts at g0 = base[d] + status[t - 1][d]
+ * and is empty when that equals status[t][d].  The key before it in the
+ * output is the last key of the previous non-empty run of digit d -- whose
+ * code is the largest rl of digit d over the tiles before t (codes ascend
+ * through the output) -- or, for the digit's first run, the last key of a
+ * lower digit, whose code is read back through its position.  Codes
+ * (previous, first] start at g0.  Workgroup = chunk of tiles, thread =
+ * digit, tiles in order.  With no valid start every bucket starts at 0. */
+__global__ void __launch_bounds__(256) k_rs_fix(const uint32_t *__restrict__ status, const uint32_t *__restrict__ base,
+                                                const uint32_t *__restrict__ cmax, uint32_t ntiles, uint32_t chunk,
+                                                const char *__restrict__ seq, const uint32_t *__restrict__ s2,
+                                                const uint32_t *__restrict__ b3, uint64_t t0, SortFinish fin)
+{
+    const uint64_t m = *fin.m;
+    const uint32_t d = threadIdx.x;
+    if (m == 0) {
+        for (uint64_t c = (uint64_t)blockIdx.x * 256 + d; c <= fin.nb; c += (uint64_t)gridDim.x * 256)
+            fin.start[c] = 0;
+        return;
+    }
+    uint32_t run = 0;                      /* largest rl of digit d before this tile */
+    for (uint32_t c = 0; c < blockIdx.x; c += 16) {
+        uint32_t v[16];
+#pragma unroll
+        for (int u = 0; u < 16; u++) v[u] = c + u < blockIdx.x ? cmax[(uint64_t)(c + u) * 256 + d] : 0u;
+#pragma unroll
+        for (int u = 0; u < 16; u++) run = max(run, v[u]);
+    }
+    const uint32_t ta = blockIdx.x * chunk, tz = min(ta + chunk, ntiles);
+    uint32_t excl = ta ? status[(uint64_t)(ta - 1) * 256 + d] & RS_VAL : 0u;
+    for (uint32_t tb = ta; tb < tz; tb += 16) {
+        /* 16 tiles' status and run records loaded at once, then walked */
+        uint32_t st[16], fr[16], lr[16];
+#pragma unroll
+        for (int u = 0; u < 16; u++) {
+            const bool in = tb + u < tz;
+            const uint64_t r = (uint64_t)(in ? tb + u : tb) * 256 + d;
+            st[u] = status[r] & RS_VAL;
+            fr[u] = fin.rf[r];
+            lr[u] = fin.rl[r];
+        }
+#pragma unroll
+        for (int u = 0; u < 16; u++) {
+            const uint32_t incl = tb + u < tz ? st[u] : excl;   /* past the chunk: no run */
+            if (incl != excl) {
+                const uint64_t g0 = (uint64_t)base[d] + excl;
+                if (g0 < m) {
+                    const uint64_t code = fr[u];
+                    uint64_t lo = ~0ull;   /* ~0: from code 0 */
+                    if (excl) {
+                        lo = run - 1u;
+                    } else if (g0 > 0) {
+                        uint32_t prev;
+                        (void)start_code(seq, s2, b3, fin.positions[g0 - 1], t0, fin.k, &prev);
+                        lo = prev;
+                    }
+                    for (uint64_t c = lo + 1; c <= code; c++) fin.start[c] = (uint32_t)g0;
+                }
+                run = max(run, lr[u]);
+            }
+            excl = incl;
+        }
+    }
+}
+
 inline uint32_t nblk(uint64_t n, uint32_t bs) { return (uint32_t)((n + bs - 1) / bs); }
 
+/* The sort's bookkeeping: digit counts and starts of up to four passes,
+ * per-pass tile counters and look-back status, an error flag. */
+struct RadixBufs {
+    uint32_t *hist, *base, *ctr, *status, *err;
+    uint64_t ntiles;
+};
+
+/* Sorts a[0..n) on bits [lo, hi) with b as the other buffer; *res is the
+ * buffer that holds the result.  Nothing is allocated.  counted: every
+ * pass's digit counts are in rb.hist and the look-back status and tile
+ * counters are zero (the kernels that wrote the keys did that); otherwise
+ * they are cleared and counted here.  fin: the last pass writes the index
+ * (positions, residuals, bucket starts) instead of the sorted keys. */
+static int radix_sort(unsigned long long *a, unsigned long long *b, uint64_t n, uint32_t lo, uint32_t hi,
+                      const RadixBufs &rb, int n_cu, hipStream_t st, unsigned long long **res, bool counted,
+                      const SortFinish *fin)
+{
+    const uint32_t npass = (hi - lo + 7) / 8;
+    *res = a;
+    if (n == 0 || npass == 0) return SWM_OK;
+    if (npass > 4) return SWM_EINVAL;
+    const uint64_t ntiles = (n + RS_TILE - 1) / RS_TILE;
+    if (ntiles > rb.ntiles || ntiles >= (1ull << 24)) return SWM_EINVAL;
+    if (!counted) {
+        SWM_HIP_CHECK(hipMemsetAsync(rb.ctr, 0, 4 * 4, st));
+        SWM_HIP_CHECK(hipMemsetAsync(rb.status, 0, (size_t)npass * ntiles * 256 * 4, st));
+        SWM_HIP_CHECK(hipMemsetAsync(rb.hist, 0, 4 * 256 * 4, st));
+        k_rs_hist<<<(uint32_t)n_cu * 2, 256, 0, st>>>(a, n, lo, hi, npass, rb.hist);
+    }
+    k_rs_bases<<<1, 256, 0, st>>>(rb.hist, 0, npass, rb.base);
+    SWM_HIP_CHECK(hipGetLastError());
+    const SortFinish none{};
+    unsigned long long *src = a, *dst = b;
+    for (uint32_t pss = 0; pss < npass; pss++) {
+        const uint32_t sh = lo + 8 * pss, bits = (hi - sh) < 8 ? hi - sh : 8;
+        uint32_t *stp = rb.status + (uint64_t)pss * ntiles * 256;
+        if (fin && pss + 1 == npass)
+            k_rs_pass<true><<<(uint32_t)ntiles, RS_NT, 0, st>>>(src, dst, n, sh, bits, rb.base + pss * 256, stp,
+                                                                rb.ctr + pss, rb.err, *fin);
+        else
+            k_rs_pass<false><<<(uint32_t)ntiles, RS_NT, 0, st>>>(src, dst, n, sh, bits, rb.base + pss * 256, stp,
+                                                                 rb.ctr + pss, rb.err, none);
+        SWM_HIP_CHECK(hipGetLastError());
+        unsigned long long *t = src; src = dst; dst = t;
+    }
+    *res = src;
+    return SWM_OK;
+}
+
+/* Exclusive scan of n u32 with sums[ceil(n / SC_TILE)] as scratch. */
+static int scan_u32(const uint32_t *in, uint32_t *out, uint64_t n, uint32_t *sums, hipStream_t st)
+{
+    if (n == 0) return SWM_OK;
+    const uint32_t nt = (uint32_t)((n + SC_TILE - 1) / SC_TILE);
+    k_sc_reduce<<<nt, SC_NT, 0, st>>>(in, n, sums);
+    k_sc_down<<<nt, SC_NT, 0, st>>>(in, n, sums, out);
+    SWM_HIP_CHECK(hipGetLastError());
+    return SWM_OK;
+}
+
 struct DevBufs {
-    void *p[16];
+    void *p[32];
     int n = 0;
     ~DevBufs() { for (int i = 0; i < n; i++) (void)hipFree(p[i]); }
     template <typename T>
@@ -474,39 +941,45 @@ extern "C" int swm_index_build_mem(const char *seq, uint64_t n, uint32_t k, swm_
     const uint64_t t0 = (n > 64 + (uint64_t)k) ? ((n - 64 - k) & ~(uint64_t)3) : 0;
     uint32_t jbits = 0;
     while (jbits < 64 && (nsamp - 1) >> jbits) jbits++;
-    const bool packed = 2 * k + 12 + jbits <= 64;
+    /* SWM_INDEX_LARGE_PATH=1 takes the large-genome path at any size (its tests) */
+    const char *lp = getenv("SWM_INDEX_LARGE_PATH");
+    const bool packed = 2 * k + 12 + jbits <= 64 && !(lp && lp[0] == '1');
     const uint32_t ntiles = (uint32_t)((nsamp + PACK_TILE - 1) / PACK_TILE);
     hipStream_t st;
     SWM_HIP_CHECK(hipStreamCreate(&st));
     struct SG { hipStream_t s; ~SG() { (void)hipStreamDestroy(s); } } sg{st};
     DevBufs B;
     char *d_seq;
-    uint32_t *b2, *b3, *s2, *cnt, *start, *end, *pos, *kin = nullptr, *kout = nullptr, *tcnt = nullptr, *toff = nullptr;
-    unsigned long long *vin, *vout;
+    uint32_t *b2, *b3, *s2, *cnt, *start, *end, *pos, *tcnt = nullptr, *toff = nullptr, *sums;
+    unsigned long long *vin, *vout, *sorted = nullptr;
     uint64_t *d_m;
     uint16_t *ls;
+    RadixBufs rb;
+    rb.ntiles = (nsamp + RS_TILE - 1) / RS_TILE;
+    const uint64_t nsums = ((nb + 1 > ntiles ? nb + 1 : ntiles) + SC_TILE - 1) / SC_TILE;
+    int n_cu = 256;
+    {
+        int dev = 0;
+        (void)hipGetDevice(&dev);
+        (void)hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev);
+    }
     /* every buffer up front (positions bounded by the sampled starts), so
      * the timed part has no allocation and no host round trip */
     if (!B.get(&d_seq, n + 64) || !B.get(&b2, 2 * nwords + 2 * SWM_WIN_PAIRS_PAD) ||
         !B.get(&b3, nwords + SWM_WIN_PAIRS_PAD) || !B.get(&s2, 2 * nwords + 2 * SWM_WIN_PAIRS_PAD) ||
         !B.get(&cnt, nb + 1) || !B.get(&start, nb + 1) || !B.get(&end, nb + 1) || !B.get(&vin, nsamp) ||
-        !B.get(&vout, nsamp) || !B.get(&d_m, 1) || !B.get(&pos, nsamp + 64) || !B.get(&ls, nsamp + 64))
+        !B.get(&vout, nsamp) || !B.get(&d_m, 1) || !B.get(&pos, nsamp + 64) || !B.get(&ls, nsamp + 64) ||
+        !B.get(&sums, nsums) || !B.get(&rb.hist, 4 * 256) || !B.get(&rb.base, 4 * 256) || !B.get(&rb.ctr, 4) ||
+        !B.get(&rb.status, 4 * rb.ntiles * 256) || !B.get(&rb.err, 1))
         return SWM_ENOMEM;
-    if (packed ? (!B.get(&tcnt, ntiles) || !B.get(&toff, ntiles)) : (!B.get(&kin, nsamp) || !B.get(&kout, nsamp)))
+    SWM_HIP_CHECK(hipMemsetAsync(rb.err, 0, 4, st));
+    /* the sort's last-pass run records: chunks of tiles, at most 256 chunks */
+    const uint32_t chunk = (uint32_t)(rb.ntiles <= 64 * 256 ? 64 : (rb.ntiles + 255) / 256);
+    const uint32_t nchunk = (uint32_t)((rb.ntiles + chunk - 1) / chunk);
+    uint32_t *rf = nullptr, *rl = nullptr, *cmax = nullptr;
+    if (packed && (!B.get(&tcnt, ntiles) || !B.get(&toff, ntiles) || !B.get(&rf, rb.ntiles * 256) ||
+                   !B.get(&rl, rb.ntiles * 256) || !B.get(&cmax, (uint64_t)nchunk * 256)))
         return SWM_ENOMEM;
-    size_t t1 = 0, t2 = 0, t3 = 0;
-    SWM_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, t1, cnt, start, nb + 1, st));
-    if (packed) {
-        SWM_HIP_CHECK(hipcub::DeviceRadixSort::SortKeys(nullptr, t2, vin, vout, nsamp, 0, (int)(2 * k), st));
-        SWM_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, t3, tcnt, toff, ntiles, st));
-    } else {
-        SWM_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, t2, kin, kout, vin, vout, nsamp, 0,
-                                                         (int)(2 * k + 1), st));
-    }
-    size_t tb = t1 > t2 ? t1 : t2;
-    if (t3 > tb) tb = t3;
-    void *tmp;
-    if (!B.get((char **)&tmp, tb)) return SWM_ENOMEM;
     SWM_HIP_CHECK(hipMemsetAsync(d_seq, 0, n + 64, st));
     SWM_HIP_CHECK(hipMemcpyAsync(d_seq, seq, n, hipMemcpyHostToDevice, st));
     SWM_HIP_CHECK(hipMemsetAsync(b2, 0, (2 * nwords + 2 * SWM_WIN_PAIRS_PAD) * 4, st));
@@ -523,36 +996,48 @@ extern "C" int swm_index_build_mem(const char *seq, uint64_t n, uint32_t k, swm_
     if (packed) {
         /* valid starts compacted in genome order as 64-bit keys, a sort on
          * the code bits, counts, offsets, positions and residuals */
-        k_pack_count<<<ntiles, PACK_NT, 0, st>>>(d_seq, nsamp, k, t0, s2, b3, tcnt);
-        SWM_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(tmp, t3, tcnt, toff, ntiles, st));
-        k_pack_write<<<ntiles, PACK_NT, 0, st>>>(d_seq, nsamp, k, t0, s2, b3, b2, toff, tcnt, ntiles, vin);
+        const uint32_t npass = (2 * k + 7) / 8;
+        k_pack_count<<<ntiles, PACK_NT, 0, st>>>(d_seq, nsamp, k, t0, s2, b3, tcnt, rb.hist, rb.ctr, rb.status,
+                                                 npass * rb.ntiles * 256);
+        int rc = scan_u32(tcnt, toff, ntiles, sums, st);
+        if (rc) return rc;
+        /* the keys, and the radix sort's digit counts of every pass */
+        const uint32_t pw_grid = ntiles < (uint32_t)n_cu * 16 ? ntiles : (uint32_t)n_cu * 16;
+        k_pack_write<<<pw_grid, PACK_NT, 0, st>>>(d_seq, nsamp, k, t0, s2, b3, b2, toff, tcnt, ntiles, vin, rb.hist,
+                                                  npass);
         k_total<<<1, 1, 0, st>>>(toff, tcnt, ntiles, d_m);
         SWM_HIP_CHECK(hipGetLastError());
-        SWM_HIP_CHECK(hipcub::DeviceRadixSort::SortKeys(tmp, t2, vin, vout, nsamp, 0, (int)(2 * k), st));
-        k_finish64<<<nblk(nsamp + 1, 256), 256, 0, st>>>(d_m, k, nb, vout, start, pos, ls);
+        const SortFinish fin{d_m, k, nb, start, pos, ls, rf, rl};
+        if ((rc = radix_sort(vin, vout, nsamp, 0, 2 * k, rb, n_cu, st, &sorted, true, &fin))) return rc;
+        k_rs_cmax<<<nchunk, 256, 0, st>>>(rl, (uint32_t)rb.ntiles, chunk, cmax);
+        k_rs_fix<<<nchunk, 256, 0, st>>>(rb.status + (uint64_t)(npass - 1) * rb.ntiles * 256,
+                                         rb.base + (npass - 1) * 256, cmax, (uint32_t)rb.ntiles, chunk, d_seq, s2, b3,
+                                         t0, fin);
         k_counts<<<nblk(nb + 1, 256), 256, 0, st>>>(nb, start, cnt, end);
         SWM_HIP_CHECK(hipGetLastError());
     } else {
         /* stable sort by code (the counting sort's order, positions
          * ascending), counts from the sorted runs, offsets by scan */
-        k_keys<<<nblk(nsamp, 256), 256, 0, st>>>(d_seq, nsamp, k, b2, kin, vin);
+        k_keys<<<nblk(nsamp, 256), 256, 0, st>>>(d_seq, nsamp, k, vin);
         SWM_HIP_CHECK(hipGetLastError());
-        SWM_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(tmp, t2, kin, kout, vin, vout, nsamp, 0, (int)(2 * k + 1),
-                                                         st));
-        k_runs<<<nblk(nsamp, 256), 256, 0, st>>>(nsamp, (uint32_t)nb, kout, cnt);
+        int rc = radix_sort(vin, vout, nsamp, 32, 32 + 2 * k + 1, rb, n_cu, st, &sorted, false, nullptr);
+        if (rc) return rc;
+        k_runs<<<nblk(nsamp, 256), 256, 0, st>>>(nsamp, (uint32_t)nb, sorted, cnt);
         SWM_HIP_CHECK(hipGetLastError());
-        SWM_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(tmp, t1, cnt, start, nb + 1, st));
+        if ((rc = scan_u32(cnt, start, nb + 1, sums, st))) return rc;
         k_m_from_start<<<1, 1, 0, st>>>(start, nb, d_m);
-        k_split<<<nblk(nsamp, 256), 256, 0, st>>>(d_m, vout, pos, ls);
+        k_split<<<nblk(nsamp, 256), 256, 0, st>>>(d_m, sorted, b2, k, pos, ls);
         k_end_offsets<<<nblk(nb + 1, 256), 256, 0, st>>>(nb, start, end);
     }
     k_tail<<<1, 64, 0, st>>>(d_seq, n, k, t0, b2, b3, s2, cnt, start, d_m, pos, ls, end);
     SWM_HIP_CHECK(hipGetLastError());
     SWM_HIP_CHECK(hipEventRecord(e1, st));
     uint64_t M = 0;
+    uint32_t sort_err = 0;
     SWM_HIP_CHECK(hipMemcpyAsync(&M, d_m, 8, hipMemcpyDeviceToHost, st));
+    SWM_HIP_CHECK(hipMemcpyAsync(&sort_err, rb.err, 4, hipMemcpyDeviceToHost, st));
     SWM_HIP_CHECK(hipStreamSynchronize(st));
-    if (M > nsamp) return SWM_EDEVICE;
+    if (M > nsamp || sort_err) return SWM_EDEVICE;
 
     out->k = k;
     out->n_bases = n;

@@ -200,3 +200,17 @@ def test_myers_stage_matches_oracle():
     assert np.array_equal(got_e, exp_err)
     assert np.array_equal(got_l, exp_loc)
     assert (exp_err <= e).mean() > 0.3
+
+
+@pytest.mark.parametrize("n,k", [(400_000, 12), (1_000_032, 12), (200_003, 10)])
+def test_index_build_large_genome_path(monkeypatch, n, k):
+    """The path genomes past 2^(52-2k) sampled starts take (sort of
+    code << 32 | p, counts from the runs, residuals from the planes), forced
+    at small sizes; N runs and lowercase included."""
+    g = wl.genome(n, seed=n + 1, kind=2, frac=0.3).copy()
+    g[5_000:9_000] = ord("N")
+    g[-2:] = ord("N")
+    g[20_000:20_400] = np.char.lower(g[20_000:20_400].view("S1")).view(np.uint8)
+    monkeypatch.setenv("SWM_INDEX_LARGE_PATH", "1")
+    _say("index large path", n, k)
+    _compare_index(swm.build_index(g, k), orc.OracleIndex(g, k))
