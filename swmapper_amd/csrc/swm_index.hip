@@ -7,13 +7,16 @@
  *              (get_seed/get_base, serial_index.c:146-235), including the
  *              final flush that zeroes the last word when the genome length
  *              is a multiple of the word size (:423-426)
- *   k_pack_*   the sampled starts p = 0, 4, 8, ... <= N-k whose k-mer has no
- *              N (the histogram's starts, :253-269), compacted in genome
- *              order (count, scan, write), each as one 64-bit key
- *              code | residual << 2k | (p/4) << (2k+12): the k-mer code from
- *              the 2-bit seed words, the N test from the mask, the 12-bit
- *              residual (get_seed_from_enc_bits :280-331) from the planes;
- *              the last starts (the tail below) from the sequence itself
+ *   k_pack_hist  every sort pass's digit counts from the genome words: the
+ *              sampled starts p = 0, 4, 8, ... <= N-k and whether their
+ *              k-mer has an N (the histogram's starts, :253-269)
+ *   k_pack_pass0 each start as one 64-bit key code | residual << 2k |
+ *              (p/4) << (2k+12) -- the k-mer code from the 2-bit seed words,
+ *              the N test from the mask, the 12-bit residual
+ *              (get_seed_from_enc_bits :280-331) from the planes, the last
+ *              starts (the tail below) from the sequence itself -- written
+ *              straight to its place after the first sort pass, the starts
+ *              with an N after all valid ones
  *   sort       stable LSD radix sort on the code bits only (3 passes of 8
  *              bits for k=12, k_rs_*): the counting sort's bucket order
  *              with positions ascending (:343-352); its last pass writes
@@ -32,6 +35,8 @@
 
 #include <stdlib.h>
 #include <string.h>
+
+#include <algorithm>
 
 namespace {
 
@@ -69,11 +74,14 @@ __device__ __forceinline__ uint32_t block_incl_scan_u32(uint32_t v, uint32_t *s_
     return v + off;
 }
 
+/* Also clears zero[0 .. nzero) (the sort's digit counts and tile counters). */
 __global__ void __launch_bounds__(256) k_encode(const char *__restrict__ seq, uint64_t n, uint64_t nwords,
                                                 uint32_t *__restrict__ b2, uint32_t *__restrict__ b3,
-                                                uint32_t *__restrict__ s2)
+                                                uint32_t *__restrict__ s2, uint32_t *__restrict__ zero,
+                                                uint32_t nzero)
 {
     uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (uint64_t i = w; i < nzero; i += (uint64_t)gridDim.x * blockDim.x) zero[i] = 0;
     if (w >= nwords) return;
     uint32_t lo = 0, hi = 0, nm = 0, s0 = 0, s1 = 0;
     const uint64_t p0 = w * 32;
@@ -118,15 +126,16 @@ __global__ void __launch_bounds__(256) k_encode(const char *__restrict__ seq, ui
 
 /* get_seed_from_enc_bits residual (serial_index.c:290-315) from the planes
  * as stored. */
-__device__ __forceinline__ uint32_t lseed_at(const uint32_t *__restrict__ b2, uint64_t pos, uint32_t k)
+__device__ __forceinline__ uint32_t lseed_at(const uint32_t *__restrict__ b2, uint64_t pos, uint32_t k,
+                                             uint64_t pair0 = 0)
 {
     uint64_t s_pos = pos >= 3 ? pos - 3 : 0;
-    uint64_t wi = s_pos >> 5;
+    uint64_t wi = (s_pos >> 5) - pair0;
     uint32_t sh = (uint32_t)s_pos & 31u;
     uint32_t fl = __builtin_amdgcn_alignbit(b2[2 * wi + 2], b2[2 * wi], sh) & 7u;
     uint32_t fh = __builtin_amdgcn_alignbit(b2[2 * wi + 3], b2[2 * wi + 1], sh) & 7u;
     s_pos = pos + k;
-    wi = s_pos >> 5;
+    wi = (s_pos >> 5) - pair0;
     sh = (uint32_t)s_pos & 31u;
     uint32_t bl = __builtin_amdgcn_alignbit(b2[2 * wi + 2], b2[2 * wi], sh) & 7u;
     uint32_t bh = __builtin_amdgcn_alignbit(b2[2 * wi + 3], b2[2 * wi + 1], sh) & 7u;
@@ -161,19 +170,15 @@ __global__ void __launch_bounds__(256) k_keys(const char *__restrict__ seq, uint
     keys[j] = ((unsigned long long)(valid ? code : (1u << (2 * k))) << 32) | (uint32_t)p;
 }
 
-#define PACK_NT 256
-#define PACK_PER 4
-#define PACK_TILE (PACK_NT * PACK_PER)
-
 /* The histogram's code and N test of start p: from the seed words and the
  * mask for p < t0, from the sequence for the last starts (whose words the
  * final flush may have zeroed). */
 __device__ __forceinline__ bool start_code(const char *__restrict__ seq, const uint32_t *__restrict__ s2,
                                            const uint32_t *__restrict__ b3, uint64_t p, uint64_t t0, uint32_t k,
-                                           uint32_t *code)
+                                           uint32_t *code, uint64_t s2o = 0, uint64_t b3o = 0)
 {
     if (p < t0) {
-        const uint64_t sw = p >> 4, mw = p >> 5;
+        const uint64_t sw = (p >> 4) - s2o, mw = (p >> 5) - b3o;
         const uint32_t smask = (uint32_t)((1ull << (2 * k)) - 1), fmask = (1u << k) - 1u;
         *code = __builtin_amdgcn_alignbit(s2[sw + 1], s2[sw], (uint32_t)(2 * (p & 15))) & smask;
         return (__builtin_amdgcn_alignbit(b3[mw + 1], b3[mw], (uint32_t)(p & 31)) & fmask) == 0;
@@ -189,119 +194,36 @@ __device__ __forceinline__ bool start_code(const char *__restrict__ seq, const u
     return valid;
 }
 
-/* Pass 1: valid starts per tile of PACK_TILE starts.  Also clears the
- * radix sort's digit counts, tile counters and first look-back status. */
-__global__ void __launch_bounds__(PACK_NT) k_pack_count(const char *__restrict__ seq, uint64_t nsamp, uint32_t k,
-                                                        uint64_t t0, const uint32_t *__restrict__ s2,
-                                                        const uint32_t *__restrict__ b3, uint32_t *__restrict__ tcnt,
-                                                        uint32_t *__restrict__ rs_hist, uint32_t *__restrict__ rs_ctr,
-                                                        uint32_t *__restrict__ rs_status, uint64_t rs_status_n)
-{
-    __shared__ uint32_t s_w[PACK_NT / 64];
-    {
-        /* grid-stride: the grid can be a single workgroup (up to 1024 starts) */
-        const uint64_t gid = (uint64_t)blockIdx.x * PACK_NT + threadIdx.x, gs = (uint64_t)gridDim.x * PACK_NT;
-        for (uint64_t i = gid; i < 4 * 256; i += gs) rs_hist[i] = 0;
-        if (gid < 4) rs_ctr[gid] = 0;
-        for (uint64_t i = gid; i < rs_status_n; i += gs) rs_status[i] = 0;
-    }
-    uint32_t c = 0;
-#pragma unroll
-    for (int q = 0; q < PACK_PER; q++) {
-        const uint64_t j = (uint64_t)blockIdx.x * PACK_TILE + (uint64_t)q * PACK_NT + threadIdx.x;
-        uint32_t code;
-        if (j < nsamp && start_code(seq, s2, b3, j * SWM_SEED_STEP, t0, k, &code)) c++;
-    }
-    uint32_t tot;
-    (void)block_incl_scan_u32<PACK_NT>(c, s_w, &tot);
-    if (threadIdx.x == 0) tcnt[blockIdx.x] = tot;
-}
+/* The genome words that the sampled starts j in [tb, te) (te - tb <=
+ * RS_TILE) read, staged in LDS with coalesced loads: seed words s2[tb/4 ..
+ * (te-1)/4 + 1], mask words b3[tb/8 .. (te-1)/8 + 1] and plane pairs from
+ * the residual's first word ((4 tb - 3) / 32) to its last ((4 (te-1) + k) /
+ * 32 + 1).  Every start's code, N test and residual then read LDS. */
+#define STG_S2 0
+#define STG_B3 1040
+#define STG_B2 1568
+#define STG_WORDS 2624
 
-/* Pass 2: the keys of the valid starts at their rank (genome order).  A
- * tile is PACK_PER rows of PACK_NT consecutive starts, so each row's stores
- * are contiguous; one scan of four 16-bit row counters ranks all rows.
- * Workgroups take tiles grid-stride and count the radix sort's digits of
- * every key they write (its histogram pass), one global add per digit and
- * workgroup at the end. */
-__global__ void __launch_bounds__(PACK_NT) k_pack_write(const char *__restrict__ seq, uint64_t nsamp, uint32_t k,
-                                                        uint64_t t0, const uint32_t *__restrict__ s2,
-                                                        const uint32_t *__restrict__ b3,
-                                                        const uint32_t *__restrict__ b2,
-                                                        const uint32_t *__restrict__ toff,
-                                                        const uint32_t *__restrict__ tcnt, uint32_t ntiles,
-                                                        unsigned long long *__restrict__ keys,
-                                                        uint32_t *__restrict__ hist, uint32_t npass)
+struct Stage {
+    uint64_t s2o, b3o, b2o;                /* first staged seed word, mask word, plane pair */
+};
+
+__device__ __forceinline__ Stage stage_tile(const uint32_t *__restrict__ s2, const uint32_t *__restrict__ b3,
+                                            const uint32_t *__restrict__ b2, uint64_t tb, uint64_t te, uint32_t k,
+                                            uint32_t *lds, uint32_t tid, uint32_t nt)
 {
-    __shared__ uint64_t s_w[PACK_NT / 64];
-    __shared__ uint32_t s_h[PACK_NT / 64][4 * 256];   /* one histogram per wave: waves do not contend */
-    uint32_t *hw = s_h[threadIdx.x >> 6];
-    for (uint32_t i = threadIdx.x; i < (PACK_NT / 64) * 4 * 256; i += PACK_NT) (&s_h[0][0])[i] = 0;
-    __syncthreads();
-    const uint32_t hbits = 2 * k;
-    for (uint32_t tl = blockIdx.x; tl < ntiles; tl += gridDim.x) {
-    const uint64_t tile0 = (uint64_t)tl * PACK_TILE;
-    uint32_t code[PACK_PER];
-    uint64_t flags = 0;
-#pragma unroll
-    for (int q = 0; q < PACK_PER; q++) {
-        const uint64_t j = tile0 + (uint64_t)q * PACK_NT + threadIdx.x;
-        const bool ok = j < nsamp && start_code(seq, s2, b3, j * SWM_SEED_STEP, t0, k, &code[q]);
-        flags |= (uint64_t)(ok ? 1u : 0u) << (16 * q);
-    }
-    /* inclusive scan of the packed row counters over the workgroup */
-    const uint32_t lane = __lane_id(), w = threadIdx.x >> 6;
-    uint64_t v = flags;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const uint32_t lo = (uint32_t)__shfl_up((int)(uint32_t)v, d, 64);
-        const uint32_t hi = (uint32_t)__shfl_up((int)(uint32_t)(v >> 32), d, 64);
-        if (lane >= (uint32_t)d) v += ((uint64_t)hi << 32) | lo;
-    }
-    if (lane == 63) s_w[w] = v;
-    __syncthreads();
-    uint64_t off = 0, tot = 0;
-#pragma unroll
-    for (int i = 0; i < PACK_NT / 64; i++) {
-        const uint64_t x = s_w[i];
-        if ((uint32_t)i < w) off += x;
-        tot += x;
-    }
-    const uint64_t excl = v + off - flags;
-    /* starts with an N go after all valid ones, with the largest code, so
-     * the sort leaves them at the end (M = valid starts) */
-    const uint64_t M = (uint64_t)toff[ntiles - 1] + tcnt[ntiles - 1];
-    const uint64_t tbase = toff[tl];
-    const unsigned long long cm = (1ull << (2 * k)) - 1ull;
-    uint64_t rows_before = 0;
-#pragma unroll
-    for (int q = 0; q < PACK_PER; q++) {
-        const uint64_t j = tile0 + (uint64_t)q * PACK_NT + threadIdx.x;
-        const uint64_t rank = rows_before + ((excl >> (16 * q)) & 0xffffu);   /* valid before j in the tile */
-        rows_before += (tot >> (16 * q)) & 0xffffu;
-        if (j >= nsamp) continue;
-        unsigned long long key;
-        if ((flags >> (16 * q)) & 1u) {
-            const uint64_t res = lseed_at(b2, j * SWM_SEED_STEP, k);
-            key = (unsigned long long)code[q] | (res << (2 * k)) | (j << (2 * k + 12));
-            keys[tbase + rank] = key;
-        } else {
-            key = cm | (j << (2 * k + 12));
-            keys[M + (tile0 - tbase) + ((j - tile0) - rank)] = key;
-        }
-        for (uint32_t p = 0; p < npass; p++) {
-            const uint32_t sh = 8 * p, bits = min(8u, hbits - sh);
-            atomicAdd(&hw[p * 256 + ((uint32_t)(key >> sh) & ((1u << bits) - 1u))], 1u);
-        }
-    }
-    __syncthreads();       /* s_w is reused by the next tile's scan */
-    }
-    __syncthreads();
-    for (uint32_t i = threadIdx.x; i < npass * 256; i += PACK_NT) {
-        uint32_t c = 0;
-#pragma unroll
-        for (int q = 0; q < PACK_NT / 64; q++) c += s_h[q][i];
-        if (c) atomicAdd(&hist[i], c);
-    }
+    Stage g;
+    const uint64_t p0 = tb * SWM_SEED_STEP, p1 = (te - 1) * SWM_SEED_STEP;
+    g.s2o = tb >> 2;
+    g.b3o = tb >> 3;
+    g.b2o = (p0 >= 3 ? p0 - 3 : 0) >> 5;
+    const uint32_t ns2 = (uint32_t)(((te - 1) >> 2) + 2 - g.s2o);         /* <= 1025 */
+    const uint32_t nb3 = (uint32_t)(((te - 1) >> 3) + 2 - g.b3o);         /* <= 513 */
+    const uint32_t nb2 = (uint32_t)(2 * (((p1 + k) >> 5) + 2 - g.b2o));   /* <= 1034 */
+    for (uint32_t i = tid; i < ns2; i += nt) lds[STG_S2 + i] = s2[g.s2o + i];
+    for (uint32_t i = tid; i < nb3; i += nt) lds[STG_B3 + i] = b3[g.b3o + i];
+    for (uint32_t i = tid; i < nb2; i += nt) lds[STG_B2 + i] = b2[2 * g.b2o + i];
+    return g;
 }
 
 /* Counts and END offsets from the starts. */
@@ -316,13 +238,6 @@ __global__ void __launch_bounds__(256) k_counts(uint64_t nb, const uint32_t *__r
     } else if (c == nb) {
         end[nb] = start[nb];
     }
-}
-
-/* The number of valid starts (M) from the last tile offset and count. */
-__global__ void k_total(const uint32_t *__restrict__ toff, const uint32_t *__restrict__ tcnt, uint32_t ntiles,
-                        uint64_t *__restrict__ m)
-{
-    if (threadIdx.x == 0 && blockIdx.x == 0) *m = ntiles ? (uint64_t)toff[ntiles - 1] + tcnt[ntiles - 1] : 0;
 }
 
 /* Bucket counts from the sorted keys: the thread at the end of each run of
@@ -542,9 +457,61 @@ __global__ void __launch_bounds__(SC_NT) k_sc_down(const uint32_t *__restrict__ 
 #define RS_SPIN (1u << 24)       /* a look-back gives up after this many waits (and flags it) */
 #define RS_LB 8                  /* predecessors a look-back step reads at once */
 
+/* Thread d of tile `tile` in a pass: publishes the tile's count of digit d
+ * and returns the count of digit d over every earlier tile (decoupled
+ * look-back).  RS_LB predecessors are read at once, nearest first: their
+ * counts are added up to the first inclusive one; an unpublished one stops
+ * the window, which is read again from there.  Tiles start in the order of
+ * the tile counter, so every predecessor is running or done. */
+__device__ __forceinline__ uint32_t rs_lookback(uint32_t *__restrict__ status, uint32_t tile, uint32_t d,
+                                                uint32_t tot, uint32_t *__restrict__ err)
+{
+    uint32_t *my = status + (uint64_t)tile * 256 + d;
+    __hip_atomic_store(my, (tile == 0 ? RS_PRE : RS_AGG) | tot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    uint32_t pre = 0;
+    if (tile == 0) return 0;
+    uint32_t spins = 0;
+    int64_t j = (int64_t)tile - 1;
+    for (;;) {
+        uint32_t x[RS_LB];
+#pragma unroll
+        for (int q = 0; q < RS_LB; q++)
+            x[q] = j - q >= 0 ? __hip_atomic_load(status + (uint64_t)(j - q) * 256 + d, __ATOMIC_RELAXED,
+                                                  __HIP_MEMORY_SCOPE_AGENT)
+                              : RS_PRE;
+        bool done = false;
+        int used = RS_LB;
+#pragma unroll
+        for (int q = 0; q < RS_LB; q++) {
+            if (done || used != RS_LB) continue;
+            if ((x[q] & (RS_AGG | RS_PRE)) == 0) { used = q; continue; }
+            pre += x[q] & RS_VAL;
+            if (x[q] & RS_PRE) done = true;
+        }
+        if (done) break;
+        j -= used;
+        if (used < RS_LB) {
+            if (++spins > RS_SPIN) { atomicOr(err, 1u); break; }
+            __builtin_amdgcn_s_sleep(1);
+        }
+    }
+    __hip_atomic_store(my, RS_PRE | ((pre + tot) & RS_VAL), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return pre;
+}
+
+/* The passes of a sort: pass p orders on bits [sh[p], sh[p] + bits[p]). */
+struct PassPlan {
+    uint32_t n;
+    uint32_t sh[4], bits[4];
+};
+
+__device__ __forceinline__ uint32_t pass_digit(unsigned long long x, const PassPlan &pl, uint32_t p)
+{
+    return (uint32_t)(x >> pl.sh[p]) & ((1u << pl.bits[p]) - 1u);
+}
+
 __global__ void __launch_bounds__(256) k_rs_hist(const unsigned long long *__restrict__ keys, uint64_t n,
-                                                 uint32_t lo, uint32_t hi, uint32_t npass,
-                                                 uint32_t *__restrict__ hist)
+                                                 PassPlan pl, uint32_t *__restrict__ hist)
 {
     __shared__ uint32_t h[4][4 * 256];
     uint32_t *hw = h[threadIdx.x >> 6];
@@ -561,29 +528,28 @@ __global__ void __launch_bounds__(256) k_rs_hist(const unsigned long long *__res
 #pragma unroll
         for (int u = 0; u < 4; u++) {
             if (i0 + (uint64_t)u * stride >= n) continue;
-            for (uint32_t p = 0; p < npass; p++) {
-                const uint32_t sh = lo + 8 * p, bits = min(8u, hi - sh);
-                atomicAdd(&hw[p * 256 + ((uint32_t)(x[u] >> sh) & ((1u << bits) - 1u))], 1u);
-            }
+            for (uint32_t p = 0; p < pl.n; p++) atomicAdd(&hw[p * 256 + pass_digit(x[u], pl, p)], 1u);
         }
     }
     __syncthreads();
-    for (uint32_t i = threadIdx.x; i < npass * 256; i += blockDim.x) {
+    for (uint32_t i = threadIdx.x; i < pl.n * 256; i += blockDim.x) {
         const uint32_t c = h[0][i] + h[1][i] + h[2][i] + h[3][i];
         if (c) atomicAdd(&hist[i], c);
     }
 }
 
-/* Digit starts of passes p0 .. npass-1: exclusive scans of the 256 counts. */
-__global__ void __launch_bounds__(256) k_rs_bases(const uint32_t *__restrict__ hist, uint32_t p0, uint32_t npass,
-                                                  uint32_t *__restrict__ base)
+/* Digit starts of every pass: exclusive scans of the 256 counts; total0
+ * (if given) gets pass 0's total. */
+__global__ void __launch_bounds__(256) k_rs_bases(const uint32_t *__restrict__ hist, uint32_t npass,
+                                                  uint32_t *__restrict__ base, uint64_t *__restrict__ total0)
 {
     __shared__ uint32_t s_w[4];
-    for (uint32_t p = p0; p < npass; p++) {
+    for (uint32_t p = 0; p < npass; p++) {
         const uint32_t c = hist[p * 256 + threadIdx.x];
         uint32_t tot;
         const uint32_t incl = block_incl_scan_u32<256>(c, s_w, &tot);
         base[p * 256 + threadIdx.x] = incl - c;
+        if (p == 0 && total0 && threadIdx.x == 0) *total0 = tot;
         __syncthreads();
     }
 }
@@ -638,45 +604,9 @@ __global__ void __launch_bounds__(RS_NT) k_rs_pass(const unsigned long long *__r
         if (t0 + (uint64_t)r * 64 + lane < n) atomicAdd(&s_cnt[(uint32_t)(key[r] >> shift) & mask], 1u);
     __syncthreads();
     if (tid < 256) {
-        /* thread d: the tile's count of digit d published, then the
-         * look-back for the digit's global offset -- before the ranking, so
-         * that this tile's inclusive count is out as early as possible */
-        const uint32_t d = tid, tot = s_cnt[d];
-        uint32_t *my = status + (uint64_t)tile * 256 + d;
-        __hip_atomic_store(my, (tile == 0 ? RS_PRE : RS_AGG) | tot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        uint32_t pre = 0;
-        if (tile != 0) {
-            /* RS_LB predecessors read at once, nearest first: their counts
-             * are added up to the first inclusive one; an unpublished one
-             * stops the window, which is read again from there */
-            uint32_t spins = 0;
-            int64_t j = (int64_t)tile - 1;
-            for (;;) {
-                uint32_t x[RS_LB];
-#pragma unroll
-                for (int q = 0; q < RS_LB; q++)
-                    x[q] = j - q >= 0 ? __hip_atomic_load(status + (uint64_t)(j - q) * 256 + d, __ATOMIC_RELAXED,
-                                                          __HIP_MEMORY_SCOPE_AGENT)
-                                      : RS_PRE;
-                bool done = false;
-                int used = RS_LB;
-#pragma unroll
-                for (int q = 0; q < RS_LB; q++) {
-                    if (done || used != RS_LB) continue;
-                    if ((x[q] & (RS_AGG | RS_PRE)) == 0) { used = q; continue; }
-                    pre += x[q] & RS_VAL;
-                    if (x[q] & RS_PRE) done = true;
-                }
-                if (done) break;
-                j -= used;
-                if (used < RS_LB) {
-                    if (++spins > RS_SPIN) { atomicOr(err, 1u); break; }
-                    __builtin_amdgcn_s_sleep(1);
-                }
-            }
-            __hip_atomic_store(my, RS_PRE | ((pre + tot) & RS_VAL), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-        s_gofs[d] = base[d] + pre;
+        /* before the ranking, so that this tile's inclusive count is out
+         * as early as possible */
+        s_gofs[tid] = base[tid] + rs_lookback(status, tile, tid, s_cnt[tid], err);
     }
     const uint64_t below = (1ull << lane) - 1ull;
     uint32_t *wc = s_wc + w * 256;
@@ -757,6 +687,187 @@ __global__ void __launch_bounds__(RS_NT) k_rs_pass(const unsigned long long *__r
         }
         if (gi + 1 == m)
             for (uint64_t c = code + 1; c <= fin.nb; c++) fin.start[c] = (uint32_t)m;
+    }
+}
+
+/* The packed index's digit counts before its keys exist: each sampled
+ * start's key has the start's code in its low bits, or all ones for a start
+ * with an N; these are counted for every pass of the sort, pass 0 over the
+ * valid starts only (k_pack_pass0 puts the starts with an N after all of
+ * them).  Workgroups take tiles of RS_TILE starts grid-stride, their words
+ * staged in LDS.  Also clears every pass's look-back status. */
+__global__ void __launch_bounds__(256) k_pack_hist(const char *__restrict__ seq, uint64_t nsamp, uint32_t k,
+                                                   uint64_t t0, const uint32_t *__restrict__ s2,
+                                                   const uint32_t *__restrict__ b3, const uint32_t *__restrict__ b2,
+                                                   PassPlan pl, uint32_t *__restrict__ hist,
+                                                   uint32_t *__restrict__ status, uint64_t status_n)
+{
+    __shared__ uint32_t h[4][4 * 256];     /* one histogram per wave: waves do not contend */
+    __shared__ uint32_t stg[STG_WORDS];
+    const uint32_t tid = threadIdx.x;
+    uint32_t *hw = h[tid >> 6];
+    for (uint32_t i = tid; i < 4 * 4 * 256; i += 256) (&h[0][0])[i] = 0;
+    const uint64_t gid = (uint64_t)blockIdx.x * 256 + tid, gs = (uint64_t)gridDim.x * 256;
+    for (uint64_t i = gid; i < status_n; i += gs) status[i] = 0;
+    const uint32_t cm = (uint32_t)((1ull << (2 * k)) - 1);
+    const uint64_t ntiles = (nsamp + RS_TILE - 1) / RS_TILE;
+    for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const uint64_t tb = tile * RS_TILE, te = min(tb + RS_TILE, nsamp);
+        __syncthreads();                   /* the previous tile's words are read */
+        const Stage g = stage_tile(s2, b3, b2, tb, te, k, stg, tid, 256);
+        __syncthreads();
+#pragma unroll 4
+        for (int r = 0; r < RS_TILE / 256; r++) {
+            const uint64_t j = tb + (uint64_t)r * 256 + tid;
+            if (j >= te) break;
+            uint32_t code;
+            const bool ok = start_code(seq, stg + STG_S2, stg + STG_B3, j * SWM_SEED_STEP, t0, k, &code, g.s2o,
+                                       g.b3o);
+            const uint32_t low = ok ? code : cm;
+            if (ok) atomicAdd(&hw[pass_digit(low, pl, 0)], 1u);
+            for (uint32_t p = 1; p < pl.n; p++) atomicAdd(&hw[p * 256 + pass_digit(low, pl, p)], 1u);
+        }
+    }
+    __syncthreads();
+    for (uint32_t i = tid; i < pl.n * 256; i += 256) {
+        const uint32_t c = h[0][i] + h[1][i] + h[2][i] + h[3][i];
+        if (c) atomicAdd(&hist[i], c);
+    }
+}
+
+/* The packed index's keys and the first pass of their sort in one kernel.
+ * A tile is RS_TILE consecutive sampled starts j (tiles taken in workgroup
+ * start order, as in k_rs_pass).  Each start's key -- code | residual << 2k
+ * | j << (2k + 12), or all-ones code | j << (2k + 12) for a start with an N
+ * -- is built from the genome words, ranked in the tile (valid starts by
+ * their pass-0 digit, starts with an N as one more class after them) and
+ * written where the stable pass would put the keys listed in genome order,
+ * valid ones first: a valid key at base[d] + (digit d in earlier tiles, by
+ * look-back) + its rank, a start with an N at M + (starts with an N in
+ * earlier tiles) + its rank. */
+__global__ void __launch_bounds__(RS_NT) k_pack_pass0(const char *__restrict__ seq, uint64_t nsamp, uint32_t k,
+                                                      uint64_t t0s, const uint32_t *__restrict__ s2,
+                                                      const uint32_t *__restrict__ b3, const uint32_t *__restrict__ b2,
+                                                      uint32_t bits, const uint32_t *__restrict__ base,
+                                                      const uint64_t *__restrict__ mp, uint32_t *__restrict__ status,
+                                                      uint32_t *__restrict__ tile_ctr, uint32_t *__restrict__ err,
+                                                      unsigned long long *__restrict__ out)
+{
+    constexpr uint32_t NC = 257;           /* classes: 256 digits, then the starts with an N */
+    __shared__ unsigned long long s_k[RS_TILE];
+    __shared__ uint32_t s_wc[RS_WAVES * NC];
+    __shared__ uint32_t s_dstart[256], s_gofs[256], s_cnt[NC], s_w[RS_WAVES], s_w2[RS_WAVES];
+    __shared__ uint32_t s_tile, s_nval;
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
+    const uint32_t mask = (1u << bits) - 1u;
+    if (tid == 0) s_tile = atomicAdd(tile_ctr, 1u);
+    for (uint32_t i = tid; i < RS_WAVES * NC; i += RS_NT) s_wc[i] = 0;
+    for (uint32_t i = tid; i < NC; i += RS_NT) s_cnt[i] = 0;
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    const uint64_t jw = (uint64_t)tile * RS_TILE + (uint64_t)w * (RS_TILE / RS_WAVES);
+    const uint32_t cb = 2 * k;
+    const unsigned long long cm = (1ull << cb) - 1ull;
+    /* the tile's genome words in the key buffer (free until the keys are
+     * placed, several barriers later) */
+    uint32_t *stg = reinterpret_cast<uint32_t *>(s_k);
+    const uint64_t tb0 = (uint64_t)tile * RS_TILE;
+    const Stage g = stage_tile(s2, b3, b2, tb0, min(tb0 + RS_TILE, nsamp), k, stg, tid, RS_NT);
+    __syncthreads();
+    unsigned long long key[RS_PER];
+    uint32_t rank[RS_PER];
+    uint32_t nflag = 0;                    /* bit r: row r's start has an N */
+#pragma unroll
+    for (int r = 0; r < RS_PER; r++) {
+        const uint64_t j = jw + (uint64_t)r * 64 + lane;
+        uint32_t code = 0;
+        const bool ok = j < nsamp && start_code(seq, stg + STG_S2, stg + STG_B3, j * SWM_SEED_STEP, t0s, k, &code,
+                                                g.s2o, g.b3o);
+        if (ok) {
+            key[r] = (unsigned long long)code |
+                     ((unsigned long long)lseed_at(stg + STG_B2, j * SWM_SEED_STEP, k, g.b2o) << cb) |
+                     ((unsigned long long)j << (cb + 12));
+        } else {
+            key[r] = cm | ((unsigned long long)j << (cb + 12));
+            if (j < nsamp) nflag |= 1u << r;
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < RS_PER; r++)
+        if (jw + (uint64_t)r * 64 + lane < nsamp)
+            atomicAdd(&s_cnt[((nflag >> r) & 1u) ? 256u : ((uint32_t)key[r] & mask)], 1u);
+    __syncthreads();
+    /* thread d: digit d's offset over the earlier tiles; their sum is the
+     * number of valid starts before this tile */
+    const uint32_t pre = rs_lookback(status, tile, tid, s_cnt[tid], err);
+    s_gofs[tid] = base[tid] + pre;
+    uint32_t vbefore;
+    (void)block_incl_scan_u32<RS_NT>(pre, s_w2, &vbefore);
+    const uint64_t below = (1ull << lane) - 1ull;
+    uint32_t *wc = s_wc + w * NC;
+#pragma unroll
+    for (int r = 0; r < RS_PER; r++) {
+        const bool v = jw + (uint64_t)r * 64 + lane < nsamp;
+        const bool nn = (nflag >> r) & 1u;
+        const uint32_t c = nn ? 256u : ((uint32_t)key[r] & mask);
+        const uint64_t bn = __ballot(nn);
+        uint64_t peers = __ballot(v) & (nn ? bn : ~bn);
+        for (uint32_t b = 0; b < bits; b++) {
+            const uint64_t bb = __ballot((c >> b) & 1u);
+            peers &= ((c >> b) & 1u) ? bb : ~bb;
+        }
+        const uint32_t before = wc[c];
+        rank[r] = before + (uint32_t)__popcll(peers & below);
+        if (v && (peers & below) == 0) wc[c] = before + (uint32_t)__popcll(peers);
+    }
+    __syncthreads();
+    {
+        /* thread d: the waves' offsets within class d and the class's start
+         * in the tile; thread 0 also for the class of the starts with an N,
+         * which starts after every valid key of the tile */
+        const uint32_t d = tid;
+        uint32_t tot = 0;
+#pragma unroll
+        for (int q = 0; q < RS_WAVES; q++) {
+            const uint32_t c = s_wc[q * NC + d];
+            s_wc[q * NC + d] = tot;
+            tot += c;
+        }
+        if (d == 0) {
+            uint32_t tn = 0;
+#pragma unroll
+            for (int q = 0; q < RS_WAVES; q++) {
+                const uint32_t c = s_wc[q * NC + 256];
+                s_wc[q * NC + 256] = tn;
+                tn += c;
+            }
+        }
+        uint32_t all;
+        const uint32_t ds = block_incl_scan_u32<RS_NT>(tot, s_w, &all) - tot;
+        s_dstart[d] = ds;
+        if (d == 0) s_nval = all;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < RS_PER; r++) {
+        if (jw + (uint64_t)r * 64 + lane >= nsamp) continue;
+        const bool nn = (nflag >> r) & 1u;
+        const uint32_t c = nn ? 256u : ((uint32_t)key[r] & mask);
+        s_k[(nn ? s_nval : s_dstart[c]) + s_wc[w * NC + c] + rank[r]] = key[r];
+    }
+    __syncthreads();
+    const uint64_t tb = (uint64_t)tile * RS_TILE;
+    const uint32_t nv = (uint32_t)(nsamp - tb < RS_TILE ? nsamp - tb : RS_TILE);
+    const uint32_t nval = s_nval;
+    const uint64_t nbase = *mp + (tb - vbefore);   /* starts with an N before this tile: tb - vbefore */
+    for (uint32_t i = tid; i < nv; i += RS_NT) {
+        const unsigned long long x = s_k[i];
+        if (i < nval) {
+            const uint32_t d = (uint32_t)x & mask;
+            out[(uint64_t)s_gofs[d] + (i - s_dstart[d])] = x;
+        } else {
+            out[nbase + (i - nval)] = x;
+        }
     }
 }
 
@@ -853,36 +964,59 @@ struct RadixBufs {
     uint64_t ntiles;
 };
 
-/* Sorts a[0..n) on bits [lo, hi) with b as the other buffer; *res is the
- * buffer that holds the result.  Nothing is allocated.  counted: every
- * pass's digit counts are in rb.hist and the look-back status and tile
- * counters are zero (the kernels that wrote the keys did that); otherwise
- * they are cleared and counted here.  fin: the last pass writes the index
- * (positions, residuals, bucket starts) instead of the sorted keys. */
-static int radix_sort(unsigned long long *a, unsigned long long *b, uint64_t n, uint32_t lo, uint32_t hi,
+/* 8-bit passes over bits [lo, hi). */
+static PassPlan plan_bits(uint32_t lo, uint32_t hi)
+{
+    PassPlan pl{};
+    for (uint32_t sh = lo; sh < hi && pl.n < 4; sh += 8, pl.n++) {
+        pl.sh[pl.n] = sh;
+        pl.bits[pl.n] = hi - sh < 8 ? hi - sh : 8;
+    }
+    return pl;
+}
+
+/* The packed index's sort on the 2k code bits: 8-bit passes, and for
+ * 2k <= 8 a pass 0 of no bits (k_pack_pass0 then only writes the keys)
+ * before the one pass that orders them. */
+static PassPlan plan_packed(uint32_t k)
+{
+    if (2 * k > 8) return plan_bits(0, 2 * k);
+    PassPlan pl{};
+    pl.n = 2;
+    pl.bits[1] = 2 * k;
+    return pl;
+}
+
+/* Sorts a[0..n) by the passes pl.sh/bits from pass p_first on, with b as the
+ * other buffer; *res is the buffer that holds the result.  Nothing is
+ * allocated.  counted: every pass's digit starts are in rb.base and the
+ * look-back status and tile counters are zero (the kernels that wrote the
+ * keys did that); otherwise they are cleared and counted here.  fin: the
+ * last pass writes the index (positions, residuals, bucket starts) instead
+ * of the sorted keys. */
+static int radix_sort(unsigned long long *a, unsigned long long *b, uint64_t n, const PassPlan &pl, uint32_t p_first,
                       const RadixBufs &rb, int n_cu, hipStream_t st, unsigned long long **res, bool counted,
                       const SortFinish *fin)
 {
-    const uint32_t npass = (hi - lo + 7) / 8;
     *res = a;
-    if (n == 0 || npass == 0) return SWM_OK;
-    if (npass > 4) return SWM_EINVAL;
+    if (n == 0 || pl.n == 0) return SWM_OK;
+    if (pl.n > 4 || p_first > pl.n) return SWM_EINVAL;
     const uint64_t ntiles = (n + RS_TILE - 1) / RS_TILE;
     if (ntiles > rb.ntiles || ntiles >= (1ull << 24)) return SWM_EINVAL;
     if (!counted) {
         SWM_HIP_CHECK(hipMemsetAsync(rb.ctr, 0, 4 * 4, st));
-        SWM_HIP_CHECK(hipMemsetAsync(rb.status, 0, (size_t)npass * ntiles * 256 * 4, st));
+        SWM_HIP_CHECK(hipMemsetAsync(rb.status, 0, (size_t)pl.n * ntiles * 256 * 4, st));
         SWM_HIP_CHECK(hipMemsetAsync(rb.hist, 0, 4 * 256 * 4, st));
-        k_rs_hist<<<(uint32_t)n_cu * 2, 256, 0, st>>>(a, n, lo, hi, npass, rb.hist);
+        k_rs_hist<<<(uint32_t)n_cu * 2, 256, 0, st>>>(a, n, pl, rb.hist);
+        k_rs_bases<<<1, 256, 0, st>>>(rb.hist, pl.n, rb.base, nullptr);
+        SWM_HIP_CHECK(hipGetLastError());
     }
-    k_rs_bases<<<1, 256, 0, st>>>(rb.hist, 0, npass, rb.base);
-    SWM_HIP_CHECK(hipGetLastError());
     const SortFinish none{};
     unsigned long long *src = a, *dst = b;
-    for (uint32_t pss = 0; pss < npass; pss++) {
-        const uint32_t sh = lo + 8 * pss, bits = (hi - sh) < 8 ? hi - sh : 8;
+    for (uint32_t pss = p_first; pss < pl.n; pss++) {
+        const uint32_t sh = pl.sh[pss], bits = pl.bits[pss];
         uint32_t *stp = rb.status + (uint64_t)pss * ntiles * 256;
-        if (fin && pss + 1 == npass)
+        if (fin && pss + 1 == pl.n)
             k_rs_pass<true><<<(uint32_t)ntiles, RS_NT, 0, st>>>(src, dst, n, sh, bits, rb.base + pss * 256, stp,
                                                                 rb.ctr + pss, rb.err, *fin);
         else
@@ -944,19 +1078,18 @@ extern "C" int swm_index_build_mem(const char *seq, uint64_t n, uint32_t k, swm_
     /* SWM_INDEX_LARGE_PATH=1 takes the large-genome path at any size (its tests) */
     const char *lp = getenv("SWM_INDEX_LARGE_PATH");
     const bool packed = 2 * k + 12 + jbits <= 64 && !(lp && lp[0] == '1');
-    const uint32_t ntiles = (uint32_t)((nsamp + PACK_TILE - 1) / PACK_TILE);
     hipStream_t st;
     SWM_HIP_CHECK(hipStreamCreate(&st));
     struct SG { hipStream_t s; ~SG() { (void)hipStreamDestroy(s); } } sg{st};
     DevBufs B;
     char *d_seq;
-    uint32_t *b2, *b3, *s2, *cnt, *start, *end, *pos, *tcnt = nullptr, *toff = nullptr, *sums;
+    uint32_t *b2, *b3, *s2, *cnt, *start, *end, *pos, *sums;
     unsigned long long *vin, *vout, *sorted = nullptr;
     uint64_t *d_m;
     uint16_t *ls;
     RadixBufs rb;
     rb.ntiles = (nsamp + RS_TILE - 1) / RS_TILE;
-    const uint64_t nsums = ((nb + 1 > ntiles ? nb + 1 : ntiles) + SC_TILE - 1) / SC_TILE;
+    const uint64_t nsums = (nb + 1 + SC_TILE - 1) / SC_TILE;
     int n_cu = 256;
     {
         int dev = 0;
@@ -969,16 +1102,17 @@ extern "C" int swm_index_build_mem(const char *seq, uint64_t n, uint32_t k, swm_
         !B.get(&b3, nwords + SWM_WIN_PAIRS_PAD) || !B.get(&s2, 2 * nwords + 2 * SWM_WIN_PAIRS_PAD) ||
         !B.get(&cnt, nb + 1) || !B.get(&start, nb + 1) || !B.get(&end, nb + 1) || !B.get(&vin, nsamp) ||
         !B.get(&vout, nsamp) || !B.get(&d_m, 1) || !B.get(&pos, nsamp + 64) || !B.get(&ls, nsamp + 64) ||
-        !B.get(&sums, nsums) || !B.get(&rb.hist, 4 * 256) || !B.get(&rb.base, 4 * 256) || !B.get(&rb.ctr, 4) ||
+        !B.get(&sums, nsums) || !B.get(&rb.hist, 4 * 256 + 8) || !B.get(&rb.base, 4 * 256) ||
         !B.get(&rb.status, 4 * rb.ntiles * 256) || !B.get(&rb.err, 1))
         return SWM_ENOMEM;
+    rb.ctr = rb.hist + 4 * 256;             /* cleared with the counts by k_encode */
     SWM_HIP_CHECK(hipMemsetAsync(rb.err, 0, 4, st));
     /* the sort's last-pass run records: chunks of tiles, at most 256 chunks */
     const uint32_t chunk = (uint32_t)(rb.ntiles <= 64 * 256 ? 64 : (rb.ntiles + 255) / 256);
     const uint32_t nchunk = (uint32_t)((rb.ntiles + chunk - 1) / chunk);
     uint32_t *rf = nullptr, *rl = nullptr, *cmax = nullptr;
-    if (packed && (!B.get(&tcnt, ntiles) || !B.get(&toff, ntiles) || !B.get(&rf, rb.ntiles * 256) ||
-                   !B.get(&rl, rb.ntiles * 256) || !B.get(&cmax, (uint64_t)nchunk * 256)))
+    if (packed && (!B.get(&rf, rb.ntiles * 256) || !B.get(&rl, rb.ntiles * 256) ||
+                   !B.get(&cmax, (uint64_t)nchunk * 256)))
         return SWM_ENOMEM;
     SWM_HIP_CHECK(hipMemsetAsync(d_seq, 0, n + 64, st));
     SWM_HIP_CHECK(hipMemcpyAsync(d_seq, seq, n, hipMemcpyHostToDevice, st));
@@ -991,27 +1125,26 @@ extern "C" int swm_index_build_mem(const char *seq, uint64_t n, uint32_t k, swm_
     SWM_HIP_CHECK(hipEventCreate(&e1));
     struct EG { hipEvent_t a, b; ~EG() { (void)hipEventDestroy(a); (void)hipEventDestroy(b); } } eg{e0, e1};
     SWM_HIP_CHECK(hipEventRecord(e0, st));
-    k_encode<<<nblk(nwords, 256), 256, 0, st>>>(d_seq, n, nwords, b2, b3, s2);
+    k_encode<<<nblk(nwords, 256), 256, 0, st>>>(d_seq, n, nwords, b2, b3, s2, rb.hist, 4 * 256 + 8);
     SWM_HIP_CHECK(hipGetLastError());
     if (packed) {
-        /* valid starts compacted in genome order as 64-bit keys, a sort on
-         * the code bits, counts, offsets, positions and residuals */
-        const uint32_t npass = (2 * k + 7) / 8;
-        k_pack_count<<<ntiles, PACK_NT, 0, st>>>(d_seq, nsamp, k, t0, s2, b3, tcnt, rb.hist, rb.ctr, rb.status,
-                                                 npass * rb.ntiles * 256);
-        int rc = scan_u32(tcnt, toff, ntiles, sums, st);
-        if (rc) return rc;
-        /* the keys, and the radix sort's digit counts of every pass */
-        const uint32_t pw_grid = ntiles < (uint32_t)n_cu * 16 ? ntiles : (uint32_t)n_cu * 16;
-        k_pack_write<<<pw_grid, PACK_NT, 0, st>>>(d_seq, nsamp, k, t0, s2, b3, b2, toff, tcnt, ntiles, vin, rb.hist,
-                                                  npass);
-        k_total<<<1, 1, 0, st>>>(toff, tcnt, ntiles, d_m);
+        /* every pass's digit counts from the genome words, the keys written
+         * by the first pass of their sort on the code bits, the other
+         * passes, the last of which writes positions, residuals and starts */
+        const PassPlan pl = plan_packed(k);
+        const uint32_t hgrid = (uint32_t)std::min<uint64_t>((uint64_t)n_cu * 4, rb.ntiles);
+        k_pack_hist<<<hgrid, 256, 0, st>>>(d_seq, nsamp, k, t0, s2, b3, b2, pl, rb.hist, rb.status,
+                                           (uint64_t)pl.n * rb.ntiles * 256);
+        k_rs_bases<<<1, 256, 0, st>>>(rb.hist, pl.n, rb.base, d_m);
+        k_pack_pass0<<<(uint32_t)rb.ntiles, RS_NT, 0, st>>>(d_seq, nsamp, k, t0, s2, b3, b2, pl.bits[0], rb.base,
+                                                            d_m, rb.status, rb.ctr, rb.err, vin);
         SWM_HIP_CHECK(hipGetLastError());
         const SortFinish fin{d_m, k, nb, start, pos, ls, rf, rl};
-        if ((rc = radix_sort(vin, vout, nsamp, 0, 2 * k, rb, n_cu, st, &sorted, true, &fin))) return rc;
+        int rc = radix_sort(vin, vout, nsamp, pl, 1, rb, n_cu, st, &sorted, true, &fin);
+        if (rc) return rc;
         k_rs_cmax<<<nchunk, 256, 0, st>>>(rl, (uint32_t)rb.ntiles, chunk, cmax);
-        k_rs_fix<<<nchunk, 256, 0, st>>>(rb.status + (uint64_t)(npass - 1) * rb.ntiles * 256,
-                                         rb.base + (npass - 1) * 256, cmax, (uint32_t)rb.ntiles, chunk, d_seq, s2, b3,
+        k_rs_fix<<<nchunk, 256, 0, st>>>(rb.status + (uint64_t)(pl.n - 1) * rb.ntiles * 256,
+                                         rb.base + (pl.n - 1) * 256, cmax, (uint32_t)rb.ntiles, chunk, d_seq, s2, b3,
                                          t0, fin);
         k_counts<<<nblk(nb + 1, 256), 256, 0, st>>>(nb, start, cnt, end);
         SWM_HIP_CHECK(hipGetLastError());
@@ -1020,7 +1153,8 @@ extern "C" int swm_index_build_mem(const char *seq, uint64_t n, uint32_t k, swm_
          * ascending), counts from the sorted runs, offsets by scan */
         k_keys<<<nblk(nsamp, 256), 256, 0, st>>>(d_seq, nsamp, k, vin);
         SWM_HIP_CHECK(hipGetLastError());
-        int rc = radix_sort(vin, vout, nsamp, 32, 32 + 2 * k + 1, rb, n_cu, st, &sorted, false, nullptr);
+        int rc = radix_sort(vin, vout, nsamp, plan_bits(32, 32 + 2 * k + 1), 0, rb, n_cu, st, &sorted, false,
+                            nullptr);
         if (rc) return rc;
         k_runs<<<nblk(nsamp, 256), 256, 0, st>>>(nsamp, (uint32_t)nb, sorted, cnt);
         SWM_HIP_CHECK(hipGetLastError());
