@@ -71,6 +71,7 @@ struct swm_ctx {
     swm_recipe_t *d_recipe = nullptr;   /* read-encoding recipe per read length 0..254 */
     uint32_t *d_irreg = nullptr;        /* start offsets of the buckets not in ascending order */
     uint32_t n_irreg = 0;
+    bool irr_flags = false;             /* bit 31 of d_hash_offset[bucket] flags the irregular ones */
     swm_opts opt;
     swm_slot *slot0 = nullptr;          /* the slot of the synchronous entry points */
     uint8_t *d_scratch = nullptr;       /* this device's no-allocation scratch (SWM_NOALLOC_SCRATCH_BYTES) */

@@ -192,7 +192,9 @@ struct SeedOut {
     uint32_t *nuniq;    /* [nrs + 1]: its last entry is zeroed here */
     uint32_t *lbins;    /* [LBIN_ROWS][256] candidates per read length, zeroed here */
     const uint32_t *irreg;   /* start offsets of the buckets not in ascending order, ascending */
-    uint32_t n_irreg;
+    uint32_t n_irreg;        /* 0 when the offsets carry the flag instead */
+    uint32_t off_mask;       /* the offset bits of a hash_offset entry: 0x7fffffff when bit 31 of a
+                              * bucket's start entry flags it irregular, else all ones */
 };
 
 /* Is the bucket starting at off one of the n irregular ones (sorted list)? */
@@ -405,10 +407,11 @@ __global__ void __launch_bounds__(128) k_seed(const char *__restrict__ reads, ui
             for (int q = 0; q < 4; q++) { a[q] = hash_offset[cq[q]]; b[q] = hash_offset[cq[q] + 1]; }
 #pragma unroll
             for (int q = 0; q < 4; q++) {
-                tot[m] += b[q] - a[q];
+                const uint32_t c = (b[q] & o.off_mask) - (a[q] & o.off_mask);
+                tot[m] += c;
                 if (m < SEED_CACHE_M) {
-                    s_off[(4 * m + q) * (2 * SEED_READS) + threadIdx.x] = a[q];
-                    s_cnt[(4 * m + q) * (2 * SEED_READS) + threadIdx.x] = (uint16_t)min(b[q] - a[q], 0xffffu);
+                    s_off[(4 * m + q) * (2 * SEED_READS) + threadIdx.x] = a[q];      /* with its flag */
+                    s_cnt[(4 * m + q) * (2 * SEED_READS) + threadIdx.x] = (uint16_t)min(c, 0xffffu);
                 }
             }
         }
@@ -453,7 +456,7 @@ __global__ void __launch_bounds__(128) k_seed(const char *__restrict__ reads, ui
 #pragma unroll
                 for (int q = 0; q < 4; q++) {
                     off4[q] = hash_offset[cq[q]];
-                    cnt4[q] = hash_offset[cq[q] + 1] - off4[q];
+                    cnt4[q] = (hash_offset[cq[q] + 1] & o.off_mask) - (off4[q] & o.off_mask);
                 }
             }
 #pragma unroll
@@ -467,12 +470,15 @@ __global__ void __launch_bounds__(128) k_seed(const char *__restrict__ reads, ui
                     rh |= (c >> 1) << bb;
                 }
                 swm_sel_t s;
-                s.off = off4[q];
+                s.off = off4[q] & o.off_mask;
                 s.cnt = cnt4[q];
                 s.meta = (base + (uint32_t)q) | (rl << 16) | (rh << 19) | ((uint32_t)q << 22);
                 so[4 * jm + q] = s;
                 heavy |= s.cnt > SWM_CHUNK;
-                if (o.n_irreg && s.cnt > 1) irr |= bucket_irregular(o.irreg, o.n_irreg, s.off);
+                if (s.cnt > 1) {
+                    irr |= (off4[q] & ~o.off_mask) != 0u;
+                    if (o.n_irreg) irr |= bucket_irregular(o.irreg, o.n_irreg, s.off);
+                }
                 npos += s.cnt;
                 nchunk += (s.cnt + SWM_CHUNK - 1) / SWM_CHUNK;
             }
@@ -3473,7 +3479,8 @@ static int enq_a(const swm_ctx_t *ctx, Pipe &P, const char *d_reads, uint64_t st
 {
     if (ev) SWM_HIP_CHECK(hipEventRecord(ev[0], st));
     P.so.irreg = ctx->d_irreg;
-    P.so.n_irreg = ctx->n_irreg;
+    P.so.n_irreg = ctx->irr_flags ? 0u : ctx->n_irreg;
+    P.so.off_mask = ctx->irr_flags ? 0x7fffffffu : 0xffffffffu;
     k_seed<<<blocks_for(P.n, SEED_READS), 2 * SEED_READS, 0, st>>>(d_reads, stride, d_lens, (uint32_t)P.n, ctx->k, e,
                                                                    P.seln, ctx->d_hash_offset, ctx->d_recipe, P.so);
     SWM_HIP_CHECK(hipGetLastError());
@@ -4115,15 +4122,23 @@ extern "C" int swm_ctx_create(uint32_t k, const uint32_t *hash_count, const uint
     /* buckets whose positions are not ascending (the serial scatter's
      * zeroed tail starts, Q23): their items are planned by k_plan_irreg */
     std::vector<uint32_t> irr;
-    std::vector<uint64_t> descents;
+    std::vector<uint64_t> descents, irr_bucket;
     for (uint64_t i = 0; i < c->n_buckets; i++) {
         const uint64_t a = h_off[i], b = (uint64_t)h_off[i] + hash_count[i];
         if (b > n_pos) break;
         bool any = false;
         for (uint64_t j = a + 1; j < b; j++)
             if (positions[j] < positions[j - 1]) { descents.push_back(j); any = true; }
-        if (any) irr.push_back((uint32_t)a);
+        if (any) { irr.push_back((uint32_t)a); irr_bucket.push_back(i); }
     }
+    /* with every offset below 2^31 (any index of a genome under 2^33 bases)
+     * bit 31 of an irregular bucket's start entry flags it, so the seed
+     * kernel learns it from the entry it gathers anyway */
+    bool below31 = true;
+    for (uint64_t i = 0; i <= c->n_buckets && below31; i++) below31 = h_off[i] < 0x80000000u;
+    c->irr_flags = below31 && !irr.empty();
+    if (c->irr_flags)
+        for (uint64_t i : irr_bucket) h_off[i] |= 0x80000000u;
     int rc = SWM_OK;
     const size_t rec_bytes = (SWM_RECIPE_MAX_L + 1) * sizeof(swm_recipe_t);
     if (hipMalloc((void **)&c->d_hash_offset, (c->n_buckets + 1) * 4) != hipSuccess ||
