@@ -12,6 +12,7 @@ import argparse
 import json
 import os
 import statistics
+import subprocess
 import sys
 import time
 
@@ -25,6 +26,7 @@ VALU_PEAK_OPS = 256 * 4 * 16 * 2.4e9   # CUs x SIMDs x lanes x clock = 39.3e12 i
 HBM_PEAK_GBS = 8000.0
 
 PMC_FILES = {"repeat5": "map_repeat5_1Mx150_e5_pmc_bytes.txt"}
+ISSUE_FILE = "map_valu_issue_skew_repeat5.txt"
 
 
 def pmc_traffic(config, kernel):
@@ -40,6 +42,56 @@ def pmc_traffic(config, kernel):
         if f and f[0] == kernel and len(f) >= 5:
             return {"bytes": int((float(f[3]) + float(f[4])) * 1e6), "source": "profiles/" + name}
     return None
+
+
+def valu_issue(config, kernel):
+    """A kernel's VALU issue fraction on this workload from the committed
+    rocprofv3 summary (SQ_INSTS_VALU x 4 cycles over SIMDs x the dispatch's
+    GRBM_GUI_ACTIVE cycles, all from one run), or None."""
+    path = os.path.join(ROOT, "profiles", ISSUE_FILE)
+    if not os.path.exists(path):
+        return None
+    sect = None
+    for line in open(path):
+        if line.startswith("## "):
+            sect = line[3:].split()[0]
+        f = line.split()
+        if sect == config and f and f[0] == kernel and len(f) >= 4:
+            try:
+                return {"valu_issue": float(f[2]), "source": "profiles/" + ISSUE_FILE}
+            except ValueError:
+                return None
+    return None
+
+
+def host_cpu():
+    """The host's CPU model and core counts (lscpu, run as a child and waited
+    for) and the threads this process may use."""
+    info = {"affinity_threads": len(os.sched_getaffinity(0)), "logical_cpus": os.cpu_count()}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            k, v = k.strip(), v.strip()
+            if k in ("Model name", "CPU(s)", "Socket(s)", "Core(s) per socket", "Thread(s) per core"):
+                info[k] = v
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return info
+
+
+def child_processes():
+    """Processes whose parent is this one (none should be left at exit)."""
+    me, kids = os.getpid(), []
+    for d in os.listdir("/proc"):
+        if d.isdigit():
+            try:
+                with open(f"/proc/{d}/stat") as f:
+                    if int(f.read().rsplit(")", 1)[1].split()[1]) == me:
+                        kids.append(int(d))
+            except (OSError, ValueError, IndexError):
+                pass
+    return kids
 
 
 def traffic_rate(traffic, ms):
@@ -60,6 +112,138 @@ FORMS = {
     # captured once and replayed as a hipGraph (no host round trip per batch)
     "graph": {1: 1, 2: 1},
 }
+
+
+def rooflines(swm, st_all, lens, config, form):
+    """Rooflines from a stage-timed form (HIP events around each stage on the
+    context stream): the seed lookup (HBM-bound random gathers; algorithmic
+    bytes = read bytes + 8 B per 12-mer queried, 4 per long seed, both
+    strands), the candidate plan (6 B per bucket entry fetched) and banded
+    Myers (VALU-bound, 20 int32 ops per band step)."""
+    k = 12
+    lv = lens.astype(np.int64)
+    ok = (lv + 2 <= 256) & (lv // (k + 3) <= 16)
+    seed_bytes = int(lv[ok].sum() + 2 * 4 * 8 * (lv[ok] // (k + 3)).sum())
+    roof = roof_plan = roof_myers = None
+    ms_seed = statistics.mean(s["ms_seed"] for s in st_all)
+    if ms_seed > 0:
+        traffic = pmc_traffic(config, "k_seed")
+        ach = seed_bytes / (ms_seed / 1e3) / 1e9
+        roof = {"bound": "hbm", "kernel": "k_seed", "form": form, "achieved": round(ach, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+                "traffic": traffic, "traffic_rate": traffic_rate(traffic, ms_seed),
+                "algorithmic_bytes_per_launch": seed_bytes, "avg_launch_ms": round(ms_seed, 3),
+                "note": "random 8-byte gathers into the 64 MiB offset table: each costs a whole cache line, "
+                        "so HBM traffic (rocprofv3 FETCH_SIZE/WRITE_SIZE, calibrated on a copy) is many "
+                        "times the algorithmic bytes"}
+    ms_plan = statistics.mean(s["ms_plan"] for s in st_all)
+    npos = st_all[-1]["n_positions"]
+    if ms_plan > 0 and npos:
+        ach = 6 * npos / (ms_plan / 1e3) / 1e9
+        roof_plan = {"bound": "hbm", "kernel": "k_plan_wave (+heavy, large-batch sort, scans)",
+                     "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": pmc_traffic(config, "k_plan_wave"),
+                     "algorithmic_bytes_per_launch": 6 * npos, "avg_launch_ms": round(ms_plan, 3)}
+        roof_plan["traffic_rate"] = traffic_rate(roof_plan["traffic"], ms_plan)
+    sum_l_ver = st_all[-1]["n_band_steps"]
+    ms_myers = statistics.mean(s["ms_myers"] for s in st_all)
+    if sum_l_ver and ms_myers > 0:
+        # peak: the larger of the 64-lanes-per-CU-per-clock figure and the
+        # on-box add/xor throughput probe (the probe is what the chip does)
+        measured = swm.valu_peak()
+        peak = max(VALU_PEAK_OPS, measured)
+        ach = MYERS_OPS_PER_STEP * sum_l_ver / (ms_myers / 1e3)
+        roof_myers = {"bound": "valu", "kernel": "k_myers2 (e <= 7; k_myers above)", "form": form,
+                      "achieved": round(ach / 1e12, 3), "peak": round(peak / 1e12, 2), "unit": "Tops/s (int32)",
+                      "frac": round(ach / peak, 4), "traffic": None,
+                      "peak_probe_tops": round(measured / 1e12, 2),
+                      "peak_64lane_tops": round(VALU_PEAK_OPS / 1e12, 2),
+                      "issue": valu_issue(config, "k_myers2"),
+                      "note": "achieved counts 20 algorithmic int32 ops per band step of each candidate; with "
+                              "equal read lengths two candidates share a lane (16-bit halves) and gfx950's "
+                              "3-input boolean ops fuse pairs, so frac is not capped at 1; 'issue' is the "
+                              "kernel's measured VALU instruction issue fraction (rocprofv3 counters of one "
+                              "run), which is capped at 1",
+                      "algorithmic_ops_per_launch": MYERS_OPS_PER_STEP * sum_l_ver,
+                      "avg_launch_ms": round(ms_myers, 3)}
+    return roof, roof_plan, roof_myers
+
+
+FIELDS = ("count", "pos", "strand", "dist")
+
+
+def sample_check(orc, ix, reads, lens, e, got, seconds, heavy=0):
+    """The oracle on a sample of a batch -- its first reads, as many as take
+    about `seconds` on every host core, plus the `heavy` reads with the most
+    records -- and whether the GPU's results are identical there."""
+    ref = orc.OracleRef(12, ix.hash_count, ix.positions, ix.pos_lseeds, ix.base_2bits)
+    threads = len(os.sched_getaffinity(0))
+    n = lens.size
+    probe = min(n, 2000)
+    t = time.perf_counter()
+    ref.map(reads[:probe], lens[:probe], e, threads)
+    dt = time.perf_counter() - t
+    ns = int(min(n, max(probe, probe * seconds / max(dt, 1e-3))))
+    idx = np.arange(ns)
+    if heavy:
+        idx = np.union1d(idx, np.argsort(got["count"], kind="stable")[::-1][:heavy])
+    t = time.perf_counter()
+    out, _ = ref.map(reads[idx], lens[idx], e, threads)
+    dt = time.perf_counter() - t
+    same = all(np.array_equal(out[f], got[f][idx]) for f in FIELDS)
+    what = f"first {ns}" + (f" + the {heavy} with the most records" if heavy else "")
+    return {"gpu_results_identical": bool(same),
+            "sample": f"{idx.size} reads ({what}) of {n}, oracle on {threads} threads, {dt:.1f}s"}
+
+
+def measure_config(swm, orc, name, cfg, g, ix, forms, args, log, e, tune=False, heavy=0, check_s=3.0):
+    """One configuration on the resident index: its reads, the given forms
+    timed (after the autotuner on a 100k sample if tune), rooflines from one
+    stage-timed (work-queue) step, and an oracle check on a sample."""
+    from swmapper_amd.configs import READS_SEED
+    reads, lens, _, _ = swm.synth_reads(g, cfg["reads"], len_min=cfg["lmin"], len_max=cfg["lmax"],
+                                        err_min=cfg["err"][0], err_max=cfg["err"][1], indel_share=0.2,
+                                        seed=READS_SEED, stride=256)
+    n = lens.size
+    m = swm.Mapper.from_index(ix)
+    d_reads = swm.DeviceBuffer.from_array(reads)
+    d_lens = swm.DeviceBuffer.from_array(lens)
+    d_out = swm.DeviceBuffer(n * swm.RESULT_DTYPE.itemsize).zero()
+
+    def step():
+        m.map_device(d_reads.ptr, 256, d_lens.ptr, n, e, d_out.ptr)
+        return m.stats()
+
+    tuned = m.autotune(reads[:min(n, 100_000)], lens[:min(n, 100_000)], e, reps=3) if tune else None
+    out = {"workload": cfg["desc"].rsplit("; e=", 1)[0] + f"; e={e}", "reads": n, "e": e, "tuned": tuned,
+           "forms": {}}
+    first = None
+    for f in forms:
+        elapsed, lat, st_all = time_form(m, FORMS[f], step, args.config_steps, 1)
+        res = d_out.to_array(swm.RESULT_DTYPE)
+        if first is None:
+            first = res
+        nv = st_all[-1]["n_verifications"]
+        out["forms"][f] = {"ms_per_step": round(elapsed * 1e3 / args.config_steps, 3),
+                           "verifications_per_s": round(nv * args.config_steps / elapsed, 1),
+                           "reads_per_s": round(n * args.config_steps / elapsed, 1),
+                           "identical_results": bool(np.array_equal(res, first))}
+    best = min(out["forms"], key=lambda f: out["forms"][f]["ms_per_step"])
+    out.update({"form": best, **{k: out["forms"][best][k] for k in ("ms_per_step", "verifications_per_s",
+                                                                     "reads_per_s")}})
+    _, _, st_q = time_form(m, FORMS["queue"], step, 1, 0)
+    roof, roof_plan, roof_myers = rooflines(swm, st_q, lens, name, "queue")
+    stages = {k: round(st_q[-1][k], 3) for k in ("ms_seed", "ms_plan", "ms_compact", "ms_myers", "ms_emit")}
+    out.update({"roofline": roof, "roofline_plan": roof_plan, "roofline_myers": roof_myers, "stage_ms": stages,
+                "dominant_stage": max(stages, key=stages.get), "verifications_per_step": st_q[-1]["n_verifications"],
+                "giant_items": st_q[-1].get("n_giant_items"), "giant_reads": st_q[-1].get("n_giant_reads")})
+    out.update(sample_check(orc, ix, reads, lens, e, first, check_s, heavy))
+    log(f"config {name} e={e}: {best} {out['ms_per_step']} ms/step, identical on sample: "
+        f"{out['gpu_results_identical']}")
+    m.close()
+    for b in (d_reads, d_lens, d_out):
+        b.free()
+    return out
 
 
 def time_form(m, opts, step, steps, warmup):
@@ -264,6 +448,9 @@ def main():
                     help="only build the configuration's index K times and print the build times (for profiling)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="one 256 MiB device-to-device copy first (calibrates HBM byte counters)")
+    ap.add_argument("--configs", default="small,uneven,skew",
+                    help="the other configurations measured after the headline (comma list; empty: none)")
+    ap.add_argument("--config-steps", type=int, default=2, help="timed steps per form in each other configuration")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -289,6 +476,12 @@ def main():
     t0 = time.time()
     g = swm.synth_genome(cfg["n"], seed=GENOME_SEED, kind=cfg["kind"], frac=cfg["frac"])
     log(f"genome {cfg['n']} bp in {time.time() - t0:.1f}s")
+    if args.copy_probe:
+        a = swm.DeviceBuffer(256 << 20).zero()
+        b = swm.DeviceBuffer(256 << 20)
+        swm.lib().swm_copy_d2d(swm.C.c_void_p(b.ptr), swm.C.c_void_p(a.ptr), 256 << 20)
+        a.free()
+        b.free()
     if args.index_only:
         times = [swm.build_index(g, 12).build_ms for _ in range(args.index_only)]
         print(json.dumps({"index_build_ms": [round(t, 3) for t in times], "bases": cfg["n"]}), flush=True)
@@ -301,7 +494,9 @@ def main():
     # algorithmic bytes of the build: FASTA bytes + planes (N/4) + N mask (N/8)
     # + counts and offsets (2 x 4 B x 4^k) + positions and residuals (6 B each)
     build_bytes = cfg["n"] + cfg["n"] // 4 + cfg["n"] // 8 + 2 * 4 * 4 ** 12 + 6 * int(ix.positions.size)
-    index_build = {"kernels": "k_encode, k_pack_count, scan, k_pack_write, radix sort (64-bit keys), k_finish64, k_counts, k_tail", "ms": round(build_ms, 3),
+    index_build = {"kernels": "k_encode, k_pack_hist, k_pack_pass0 (keys + sort pass 0), k_rs_pass x2 (the last "
+                              "writes positions, residuals, starts), k_rs_cmax, k_rs_fix, k_counts, k_tail",
+                   "ms": round(build_ms, 3),
                    "bound": "hbm", "algorithmic_bytes": build_bytes,
                    "achieved": round(build_bytes / (build_ms / 1e3) / 1e9, 1), "peak": HBM_PEAK_GBS,
                    "unit": "GB/s", "frac": round(build_bytes / (build_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
@@ -333,12 +528,6 @@ def main():
         m.map_device(d_reads.ptr, 256, d_lens.ptr, n, e, d_out.ptr)
         return m.stats()
 
-    if args.copy_probe:
-        a = swm.DeviceBuffer(256 << 20).zero()
-        b = swm.DeviceBuffer(256 << 20)
-        swm.lib().swm_copy_d2d(swm.C.c_void_p(b.ptr), swm.C.c_void_p(a.ptr), 256 << 20)
-        a.free()
-        b.free()
     results = {}
     ref_out = None
     for f in forms:
@@ -369,59 +558,10 @@ def main():
     verif_per_s = nv * args.steps / R["elapsed"]
     reads_per_s = n * args.steps / R["elapsed"]
 
-    # Rooflines from a stage-timed form (HIP events around each stage on the
-    # context stream).  The dominant kernel on this workload is the seed
-    # lookup: HBM-bound random gathers; its algorithmic bytes are the read
-    # bytes plus 8 B per 12-mer queried (4 per long seed, both strands).
     timed = next((f for f in ("queue", "plain") if f in results), None)
     roof = roof_plan = roof_myers = None
     if timed:
-        st_all = results[timed]["st"]
-        k = 12
-        lv = lens.astype(np.int64)
-        ok = (lv + 2 <= 256) & (lv // (k + 3) <= 16)
-        seed_bytes = int(lv[ok].sum() + 2 * 4 * 8 * (lv[ok] // (k + 3)).sum())
-        ms_seed = statistics.mean(s["ms_seed"] for s in st_all)
-        traffic = pmc_traffic(args.config, "k_seed")
-        if ms_seed > 0:
-            ach = seed_bytes / (ms_seed / 1e3) / 1e9
-            roof = {"bound": "hbm", "kernel": "k_seed", "form": timed, "achieved": round(ach, 1),
-                    "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
-                    "traffic": traffic, "traffic_rate": traffic_rate(traffic, ms_seed),
-                    "algorithmic_bytes_per_launch": seed_bytes,
-                    "avg_launch_ms": round(ms_seed, 3),
-                    "note": "random 8-byte gathers into the 64 MiB offset table: each costs a whole cache line, "
-                            "so HBM traffic (rocprofv3 FETCH_SIZE/WRITE_SIZE, calibrated on a copy) is many "
-                            "times the algorithmic bytes"}
-        # k_plan stage (bucket entries of the selected seeds: 8 B each as stored on the device,
-        # 6 B algorithmic: 4 B position + 2 B residual)
-        ms_plan = statistics.mean(s["ms_plan"] for s in st_all)
-        npos = st_all[-1]["n_positions"]
-        if ms_plan > 0 and npos:
-            ach = 6 * npos / (ms_plan / 1e3) / 1e9
-            roof_plan = {"bound": "hbm", "kernel": "k_plan_wave (+heavy, large-batch sort, scans)",
-                         "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": pmc_traffic(args.config, "k_plan_wave"),
-                         "algorithmic_bytes_per_launch": 6 * npos, "avg_launch_ms": round(ms_plan, 3)}
-            roof_plan["traffic_rate"] = traffic_rate(roof_plan["traffic"], ms_plan)
-        sum_l_ver = st_all[-1]["n_band_steps"]
-        ms_myers = statistics.mean(s["ms_myers"] for s in st_all)
-        if sum_l_ver and ms_myers > 0:
-            # peak: the larger of the 64-lanes-per-CU-per-clock figure and the
-            # on-box add/xor throughput probe (the probe is what the chip does)
-            measured = swm.valu_peak()
-            peak = max(VALU_PEAK_OPS, measured)
-            ach = MYERS_OPS_PER_STEP * sum_l_ver / (ms_myers / 1e3)
-            roof_myers = {"bound": "valu", "kernel": "k_myers2 (e <= 7; k_myers above)", "form": timed, "achieved": round(ach / 1e12, 3),
-                          "peak": round(peak / 1e12, 2), "unit": "Tops/s (int32)",
-                          "frac": round(ach / peak, 4), "traffic": None,
-                          "peak_probe_tops": round(measured / 1e12, 2),
-                          "peak_64lane_tops": round(VALU_PEAK_OPS / 1e12, 2),
-                          "note": "achieved counts 20 algorithmic int32 ops per band step; gfx950's 3-input "
-                                  "boolean (v_bitop3) lets the kernel issue ~19.5 VALU instructions per step, so "
-                                  "frac can reach 1",
-                          "algorithmic_ops_per_launch": MYERS_OPS_PER_STEP * sum_l_ver,
-                          "avg_launch_ms": round(ms_myers, 3)}
+        roof, roof_plan, roof_myers = rooflines(swm, results[timed]["st"], lens, args.config, timed)
 
     serving = None
     if args.serve_seconds > 0:
@@ -431,25 +571,87 @@ def main():
         serving = serve_bench(swm, m, g, e, args.serve_seconds, log, slots=args.serve_slots,
                               graph=args.serve_form == "graph", merge=args.serve_merge)
 
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as orc
     cpu = None
     if not args.no_cpu:
-        sys.path.insert(0, os.path.join(ROOT, "tests"))
-        import oracle_lib as orc
         ref = orc.OracleRef(12, ix.hash_count, ix.positions, ix.pos_lseeds, ix.base_2bits)
-        threads = min(16, os.cpu_count() or 1)
-        probe = min(n, 4000)
-        tp = time.perf_counter()
-        ref.map(reads[:probe], lens[:probe], e, threads)
-        dtp = time.perf_counter() - tp
-        ns = int(min(n, max(probe, probe * args.cpu_seconds / max(dtp, 1e-3))))
-        tc = time.perf_counter()
-        cpu_out, nv_cpu = ref.map(reads[:ns], lens[:ns], e, threads)
-        dtc = time.perf_counter() - tc
-        same = all(np.array_equal(cpu_out[f], ref_out[f][:ns]) for f in ("count", "pos", "strand", "dist"))
+        threads = len(os.sched_getaffinity(0))
+
+        def timed_sample(nthreads, seconds, probe):
+            tp = time.perf_counter()
+            ref.map(reads[:probe], lens[:probe], e, nthreads)
+            dtp = time.perf_counter() - tp
+            ns = int(min(n, max(probe, probe * seconds / max(dtp, 1e-3))))
+            tc = time.perf_counter()
+            out, nv_c = ref.map(reads[:ns], lens[:ns], e, nthreads)
+            return out, nv_c, ns, time.perf_counter() - tc
+
+        cpu_out, nv_cpu, ns, dtc = timed_sample(threads, args.cpu_seconds, min(n, 4000))
+        same = all(np.array_equal(cpu_out[f], ref_out[f][:ns]) for f in FIELDS)
+        one_out, nv_one, ns1, dt1 = timed_sample(1, args.cpu_seconds / 3, min(n, 500))
+        # the serial index build (serial_index.c) at the workload's size, one thread
+        ti = time.perf_counter()
+        oix = orc.OracleIndex(g, 12)
+        dti = time.perf_counter() - ti
+        ix_same = all(np.array_equal(a, ix.files()[k]) for k, a in oix.files().items())
+        del oix
         cpu = {"value": round(nv_cpu / dtc, 1), "unit": "verifications/s", "cores": threads, "kind": "port",
                "reads_per_s": round(ns / dtc, 1), "gpu_results_identical": same,
-               "sample": f"first {ns} reads of the same batch, oracle (C restatement, OpenMP dynamic,1), "
-                         f"{dtc:.1f}s"}
+               "sample": f"first {ns} reads of the same batch, oracle (C restatement, OpenMP dynamic,1) on every "
+                         f"thread of the process affinity, {dtc:.1f}s",
+               "host": host_cpu(),
+               "single_thread": {"value": round(nv_one / dt1, 1), "unit": "verifications/s",
+                                 "reads_per_s": round(ns1 / dt1, 1), "cores": 1,
+                                 "sample": f"first {ns1} reads, 1 thread, {dt1:.1f}s"},
+               "index_build": {"ms": round(dti * 1e3, 1), "threads": 1, "bases": cfg["n"],
+                               "identical_to_gpu_build": ix_same,
+                               "what": "oracle restatement of serial_index.c construct_hash_index, whole genome"}}
+        log(f"cpu baseline: {cpu}")
+    # end to end: reads from (pageable) host memory, results back to host
+    for k_, v_ in FORMS[best].items():
+        m.set_option(k_, v_)
+    m.map(reads, lens, e)
+    t_e = time.perf_counter()
+    e2e_out = m.map(reads, lens, e)
+    dt_e = time.perf_counter() - t_e
+    end_to_end = {"reads_per_s": round(n / dt_e, 1), "ms": round(dt_e * 1e3, 3),
+                  "verifications_per_s": round(nv / dt_e, 1),
+                  "identical_results": bool(np.array_equal(e2e_out, ref_out)),
+                  "entry": "swm_map_batch", "form": best,
+                  "note": "reads copied in from pageable host memory and results copied back, inside the timing"}
+    log(f"end to end: {end_to_end}")
+    del e2e_out
+    m.close()
+    for b_ in (d_reads, d_lens, d_out):
+        b_.free()
+    configs = {}
+    for name in [c for c in args.configs.split(",") if c]:
+        if name == "small":
+            c = CONFIGS["small"]
+            gs = swm.synth_genome(c["n"], seed=GENOME_SEED, kind=c["kind"], frac=c["frac"])
+            ixs = swm.build_index(gs, 12)
+            configs["small"] = measure_config(swm, orc, "small", c, gs, ixs, ["queue", "graph"], args, log, c["e"],
+                                              check_s=60.0)
+            del gs, ixs
+        elif name == "uneven":
+            c = CONFIGS["uneven"]
+            if (c["n"], c["kind"], c["frac"]) == (cfg["n"], cfg["kind"], cfg["frac"]):
+                gu, ixu = g, ix                    # the headline's genome (the same seed and shape)
+            else:
+                gu = swm.synth_genome(c["n"], seed=GENOME_SEED, kind=c["kind"], frac=c["frac"])
+                ixu = swm.build_index(gu, 12)
+            configs["uneven"] = {f"e{ee}": measure_config(swm, orc, "uneven", c, gu, ixu, ["graph"], args, log, ee,
+                                                          tune=True)
+                                 for ee in (3, 5, 8)}
+            del gu, ixu
+        elif name == "skew":
+            c = CONFIGS["skew"]
+            gk = swm.synth_genome(c["n"], seed=GENOME_SEED, kind=c["kind"], frac=c["frac"])
+            ixk = swm.build_index(gk, 12)
+            configs["skew"] = measure_config(swm, orc, "skew", c, gk, ixk, ["plain", "queue"], args, log, c["e"],
+                                             heavy=100)
+            del gk, ixk
     lat_block = {"p50": round(float(np.percentile(R["lat"], 50)), 3),
                  "p99": round(float(np.percentile(R["lat"], 99)), 3), "of": f"whole 1-batch step, form {best}"}
     if serving:
@@ -471,8 +673,10 @@ def main():
                   for f, r in results.items()},
         "config": {"workload": cfg["desc"], "name": args.config, "form": best, "reads": n, "e": e,
                    "tuned": tuned},
+        "end_to_end": end_to_end,
         "roofline": roof, "roofline_plan": roof_plan, "roofline_myers": roof_myers, "index_build": index_build,
-        "serving": serving, "records": records, "cpu_baseline": cpu,
+        "serving": serving, "records": records, "cpu_baseline": cpu, "configs": configs,
+        "child_processes_at_exit": len(child_processes()),
     }
     print(json.dumps(res), flush=True)
 
