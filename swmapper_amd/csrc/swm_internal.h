@@ -62,7 +62,8 @@ struct swm_ctx {
     int n_cu = 256;                     /* compute units: sizes the persistent work-queue grids */
     uint32_t k = 12;
     uint64_t n_buckets = 0;
-    uint32_t *d_hash_offset = nullptr;  /* exclusive scan of hash_count, 4^k + 1 */
+    uint2 *d_seed_tab = nullptr;        /* (offset, count) per k-mer, a k-mer and its reverse
+                                         * complement in one line: 2 x 4^k entries */
     uint2 *d_pl = nullptr;              /* bucket entries: (position, 12-bit residual) */
     uint64_t n_pos = 0;
     uint32_t *d_base_2bits = nullptr;   /* file words + SWM_WIN_PAIRS_PAD zero pairs */

@@ -272,6 +272,64 @@ __device__ __forceinline__ void pack_quad(uint32_t raw, uint8_t *q, uint8_t *v)
     *v = (uint8_t)ib;
 }
 
+/* ------------------------------------------------------------ seed table
+ *
+ * The offsets and counts of the k-mers (choose_seed's hash_offset gathers,
+ * mapper_cpe.c:1372-1390) are kept in a table laid out so that a k-mer and
+ * its reverse complement share one 128-byte line.  The reverse strand of a
+ * read is its reverse complement, so its k-mers are (for L = 15n exactly,
+ * for other lengths in part) the reverse complements of the forward
+ * strand's: the two threads of a read fetch each line once between them.
+ *
+ * A k-mer c is split into its inner k-2 bases (hi) and the digit d0 of its
+ * outer bases (b0 | b_{k-1} << 2).  Reverse complement maps hi to rc(hi) and
+ * d0 = (x, y) to (3 - y, 3 - x).  The pair {c, rc(c)} is stored in the
+ * 256-byte group of rep = min(hi, rc(hi)): 16 outer digits x 2 orientations
+ * (hi above rc(hi) or not) of 8-byte entries, the digits ordered so that a
+ * digit and its image lie in the same 128-byte half.  Entry (off, count):
+ * off with the irregular-bucket flag of the offset entry (bit 31), count the
+ * bucket's hits.  The table is 16 x 4^k bytes, half of it touched. */
+#define SEED_SLOT_ORDER 0x13569B74DEA2FC80ull   /* nibble d: half(d) * 8 + rank of d in its half */
+
+__host__ __device__ __forceinline__ uint32_t rc_bases(uint32_t v, uint32_t nb)
+{
+    if (nb == 0) return 0u;
+    uint32_t r = ~v;
+#if defined(__HIP_DEVICE_COMPILE__)
+    r = __builtin_bitreverse32(r);
+#else
+    r = ((r >> 1) & 0x55555555u) | ((r & 0x55555555u) << 1);
+    r = ((r >> 2) & 0x33333333u) | ((r & 0x33333333u) << 2);
+    r = ((r >> 4) & 0x0f0f0f0fu) | ((r & 0x0f0f0f0fu) << 4);
+    r = ((r >> 8) & 0x00ff00ffu) | ((r & 0x00ff00ffu) << 8);
+    r = (r >> 16) | (r << 16);
+#endif
+    /* the bit reversal also swapped the two bits of every base back */
+    r = ((r >> 1) & 0x55555555u) | ((r & 0x55555555u) << 1);
+    return r >> (32u - 2u * nb);
+}
+
+__host__ __device__ __forceinline__ uint64_t seed_slot(uint32_t c, uint32_t k)
+{
+    const uint32_t nb = k - 2u;
+    const uint32_t hi = (c >> 2) & ((1u << (2u * nb)) - 1u);
+    const uint32_t d0 = (c & 3u) | ((c >> (2u * (k - 1u))) << 2);
+    const uint32_t r = rc_bases(hi, nb);
+    const uint32_t rep = hi < r ? hi : r;
+    const uint32_t sb = (uint32_t)(SEED_SLOT_ORDER >> (4u * d0)) & 15u;
+    return (uint64_t)rep * 32u + sb * 2u + (hi > r ? 1u : 0u);
+}
+
+/* entry c of the table from the offset array (off_mask: the offset bits) */
+__global__ void __launch_bounds__(256) k_seed_table(const uint32_t *__restrict__ off, uint64_t n_buckets, uint32_t k,
+                                                    uint32_t off_mask, uint2 *__restrict__ tab)
+{
+    const uint64_t c = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+    if (c >= n_buckets) return;
+    const uint32_t a = off[c], b = off[c + 1];
+    tab[seed_slot((uint32_t)c, k)] = make_uint2(a, (b & off_mask) - (a & off_mask));
+}
+
 /* The offset and count of the 12-mers of the first SEED_CACHE_M long seeds
  * of the thread's strand are kept in LDS from the ranking gathers, so the
  * selected ones are not fetched twice; the few selected beyond them (reads
@@ -283,7 +341,7 @@ __device__ __forceinline__ void pack_quad(uint32_t raw, uint8_t *q, uint8_t *v)
 
 __global__ void __launch_bounds__(128) k_seed(const char *__restrict__ reads, uint64_t stride,
                                               const uint32_t *__restrict__ lens, uint32_t nreads, uint32_t k,
-                                              uint32_t e, uint32_t seln, const uint32_t *__restrict__ hash_offset,
+                                              uint32_t e, uint32_t seln, const uint2 *__restrict__ seed_tab,
                                               const swm_recipe_t *__restrict__ recipes, SeedOut o)
 {
     __shared__ uint8_t s_q[SEED_READS * 64], s_v[SEED_READS * 64];
@@ -391,30 +449,52 @@ __global__ void __launch_bounds__(128) k_seed(const char *__restrict__ reads, ui
 #pragma unroll
     for (int j = 0; j < 8; j++) { wo[16 + 2 * j] = pl[j]; wo[17 + 2 * j] = ph[j]; }
 
-    /* 12-mer codes and hash-offset gathers, long seed by long seed
-     * (choose_seed :1372-1390) */
+    /* 12-mer codes and seed-table gathers, long seed by long seed
+     * (choose_seed :1372-1390).  The reverse strand's 12-mer at 15m + q is
+     * the reverse complement of the forward strand's at L - 12 - 15m - q,
+     * i.e. of long seed n - 1 - m at shift 3 - q when L = 15n (nearby
+     * otherwise), so the reverse thread walks its long seeds and shifts in
+     * mirrored order: the two threads of a read request each shared line
+     * of the seed table in the same load. */
     uint32_t tot[16], idx[16];
 #pragma unroll
     for (int m = 0; m < 16; m++) {
         tot[m] = 0;
         idx[m] = (uint32_t)m;
         if ((uint32_t)m < n_long) {
-            const uint32_t base = (uint32_t)m * seed_len;
+            const uint32_t mm = strand ? n_long - 1u - (uint32_t)m : (uint32_t)m;
             uint32_t cq[4];
-            seed_codes(sq, sv, L, strand, k, base, cq);
-            uint32_t a[4], b[4];
+            seed_codes(sq, sv, L, strand, k, mm * seed_len, cq);
+            uint2 a[4];
 #pragma unroll
-            for (int q = 0; q < 4; q++) { a[q] = hash_offset[cq[q]]; b[q] = hash_offset[cq[q] + 1]; }
+            for (int q = 0; q < 4; q++) a[q] = seed_tab[seed_slot(strand ? cq[3 - q] : cq[q], k)];
 #pragma unroll
             for (int q = 0; q < 4; q++) {
-                const uint32_t c = (b[q] & o.off_mask) - (a[q] & o.off_mask);
+                const uint32_t c = a[q].y;
+                const uint32_t qq = strand ? 3u - (uint32_t)q : (uint32_t)q;
                 tot[m] += c;
-                if (m < SEED_CACHE_M) {
-                    s_off[(4 * m + q) * (2 * SEED_READS) + threadIdx.x] = a[q];      /* with its flag */
-                    s_cnt[(4 * m + q) * (2 * SEED_READS) + threadIdx.x] = (uint16_t)min(c, 0xffffu);
+                if (mm < SEED_CACHE_M) {
+                    s_off[(4 * mm + qq) * (2 * SEED_READS) + threadIdx.x] = a[q].x;    /* with its flag */
+                    s_cnt[(4 * mm + qq) * (2 * SEED_READS) + threadIdx.x] = (uint16_t)min(c, 0xffffu);
                 }
             }
         }
+    }
+    /* the reverse thread's totals back in long-seed order:
+     * tot[j] = walked[n_long - 1 - j] (a static reversal, then a shift by
+     * 16 - n_long in four select stages) */
+    if (strand) {
+        uint32_t rv[16];
+#pragma unroll
+        for (int j = 0; j < 16; j++) rv[j] = tot[15 - j];
+        const uint32_t sh = 16u - n_long;
+#pragma unroll
+        for (int b = 8; b >= 1; b >>= 1) {
+#pragma unroll
+            for (int j = 0; j < 16; j++) rv[j] = (sh & (uint32_t)b) ? (j + b < 16 ? rv[j + b] : 0u) : rv[j];
+        }
+#pragma unroll
+        for (int j = 0; j < 16; j++) tot[j] = rv[j];
     }
     /* exchange sort of the long seeds by total hits (:1391-1399, Q4) */
 #pragma unroll
@@ -455,8 +535,9 @@ __global__ void __launch_bounds__(128) k_seed(const char *__restrict__ reads, ui
                 seed_codes(sq, sv, L, strand, k, base, cq);
 #pragma unroll
                 for (int q = 0; q < 4; q++) {
-                    off4[q] = hash_offset[cq[q]];
-                    cnt4[q] = (hash_offset[cq[q] + 1] & o.off_mask) - (off4[q] & o.off_mask);
+                    const uint2 v = seed_tab[seed_slot(cq[q], k)];
+                    off4[q] = v.x;
+                    cnt4[q] = v.y;
                 }
             }
 #pragma unroll
@@ -3482,7 +3563,7 @@ static int enq_a(const swm_ctx_t *ctx, Pipe &P, const char *d_reads, uint64_t st
     P.so.n_irreg = ctx->irr_flags ? 0u : ctx->n_irreg;
     P.so.off_mask = ctx->irr_flags ? 0x7fffffffu : 0xffffffffu;
     k_seed<<<blocks_for(P.n, SEED_READS), 2 * SEED_READS, 0, st>>>(d_reads, stride, d_lens, (uint32_t)P.n, ctx->k, e,
-                                                                   P.seln, ctx->d_hash_offset, ctx->d_recipe, P.so);
+                                                                   P.seln, ctx->d_seed_tab, ctx->d_recipe, P.so);
     SWM_HIP_CHECK(hipGetLastError());
     if (ev) SWM_HIP_CHECK(hipEventRecord(ev[1], st));
     ScanJob j{};
@@ -4141,14 +4222,18 @@ extern "C" int swm_ctx_create(uint32_t k, const uint32_t *hash_count, const uint
         for (uint64_t i : irr_bucket) h_off[i] |= 0x80000000u;
     int rc = SWM_OK;
     const size_t rec_bytes = (SWM_RECIPE_MAX_L + 1) * sizeof(swm_recipe_t);
-    if (hipMalloc((void **)&c->d_hash_offset, (c->n_buckets + 1) * 4) != hipSuccess ||
+    uint32_t *d_off = nullptr;
+    const uint64_t tab_bytes = 2 * c->n_buckets * sizeof(uint2);
+    if (hipMalloc((void **)&d_off, (c->n_buckets + 1) * 4) != hipSuccess ||
+        hipMalloc((void **)&c->d_seed_tab, tab_bytes) != hipSuccess ||
         hipMalloc((void **)&c->d_pl, (n_pos + 64) * 8) != hipSuccess ||
         hipMalloc((void **)&c->d_base_2bits, (n_2bits_words + 2 * SWM_WIN_PAIRS_PAD) * 4) != hipSuccess ||
         hipMalloc((void **)&c->d_recipe, rec_bytes) != hipSuccess ||
         (!irr.empty() && hipMalloc((void **)&c->d_irreg, irr.size() * 4) != hipSuccess))
         rc = SWM_ENOMEM;
     if (rc == SWM_OK) {
-        if (hipMemcpy(c->d_hash_offset, h_off, (c->n_buckets + 1) * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        if (hipMemcpy(d_off, h_off, (c->n_buckets + 1) * 4, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemset(c->d_seed_tab, 0, tab_bytes) != hipSuccess ||
             hipMemset(c->d_pl, 0, (n_pos + 64) * 8) != hipSuccess ||
             (n_pos && upload_pl(c->d_pl, positions, pos_lseeds, n_pos, descents) != SWM_OK) ||
             hipMemset(c->d_base_2bits, 0, (n_2bits_words + 2 * SWM_WIN_PAIRS_PAD) * 4) != hipSuccess ||
@@ -4158,6 +4243,13 @@ extern "C" int swm_ctx_create(uint32_t k, const uint32_t *hash_count, const uint
             rc = SWM_EDEVICE;
         c->n_irreg = (uint32_t)irr.size();
     }
+    if (rc == SWM_OK) {
+        k_seed_table<<<(unsigned)((c->n_buckets + 255) / 256), 256>>>(d_off, c->n_buckets, k,
+                                                                      c->irr_flags ? 0x7fffffffu : 0xffffffffu,
+                                                                      c->d_seed_tab);
+        if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) rc = SWM_EDEVICE;
+    }
+    if (d_off) (void)hipFree(d_off);
     free(h_off);
     free(rec);
     if (rc != SWM_OK) { swm_ctx_destroy(c); return rc; }
@@ -4165,11 +4257,13 @@ extern "C" int swm_ctx_create(uint32_t k, const uint32_t *hash_count, const uint
     return SWM_OK;
 }
 
+extern "C" uint64_t swm_seed_slot(uint32_t code, uint32_t k) { return seed_slot(code, k); }
+
 extern "C" void swm_ctx_destroy(swm_ctx_t *c)
 {
     if (!c) return;
     swm_slot_destroy(c->slot0);
-    if (c->d_hash_offset) (void)hipFree(c->d_hash_offset);
+    if (c->d_seed_tab) (void)hipFree(c->d_seed_tab);
     if (c->d_pl) (void)hipFree(c->d_pl);
     if (c->d_base_2bits) (void)hipFree(c->d_base_2bits);
     if (c->d_recipe) (void)hipFree(c->d_recipe);
