@@ -207,6 +207,13 @@ typedef struct swm_records {
     swm_record_t *rec;
     uint64_t text_bytes;
     char *cigar;         /* the CIGAR texts back to back, no terminators */
+    /* put_cigar's output blocks (mapper_cpe.c:1437-1507): the records of a
+     * (read, strand) between two flushes (:1790, :1858, :2424) form one block;
+     * block[b] is the index in rec of block b's first record (ascending).
+     * swm_map_batch_records fills them; writers that only read rec and cigar
+     * may leave them 0 / NULL. */
+    uint64_t n_blocks;
+    uint32_t *block;
 } swm_records_t;
 
 /* swm_map_batch that also returns every record with its CIGAR text (always
@@ -215,6 +222,37 @@ typedef struct swm_records {
 int swm_map_batch_records(swm_ctx_t *ctx, const char *reads, uint64_t stride, const uint32_t *lens,
                           uint64_t n, uint32_t e, swm_read_result_t *out, swm_records_t *recs);
 void swm_records_free(swm_records_t *recs);
+
+/* ---- the reference's own output buffers (for host code that keeps
+ *      sam_format_2, mapper_mpe.c:426-687) ---- */
+
+#define SWM_REF_MAX_CIGAR 256   /* MAX_CIGAR / CIGARS_PER_BLOCK (fm_define.h:17-18) */
+#define SWM_REF_CIGAR_LEN 16    /* CIGAR_LEN / CIGAR_AVG_LEN (fm_define.h:19-20) */
+
+/* One batch's ref_pos_buffer, cigar_buffer and cigar_num (the unmapped
+ * list) as put_cigar (mapper_cpe.c:1437-1507) leaves them in the host
+ * buffers of mapper_mpe.c:791-819, with cigar_block_size = n_blocks and
+ * cigar_unmap_size = n_unmapped.  Block b: ref_pos[b * (SWM_REF_MAX_CIGAR +
+ * 2)] = cur_read (2 * read + strand), [+1] = record count | main_flag << 31,
+ * [+2 ..] = locations; cigar[b * SWM_REF_MAX_CIGAR * SWM_REF_CIGAR_LEN ..]
+ * = per record its distance byte, text-length byte and CIGAR text.
+ * unmapped[u] = 2 * read + 1 for each read with no record.  The reference
+ * orders blocks and unmapped reads by CPE timing; here blocks follow the
+ * records and unmapped reads the batch. */
+typedef struct swm_ref_blocks {
+    uint64_t n_blocks;
+    uint32_t *ref_pos;
+    char *cigar;
+    uint64_t n_unmapped;
+    uint32_t *unmapped;
+} swm_ref_blocks_t;
+
+/* From swm_map_batch_records' output (block starts included) for a batch
+ * of n_reads reads.  SWM_EINVAL when the blocks do not tile the records
+ * (one (read, strand) each, at most SWM_REF_MAX_CIGAR records) or a block's
+ * text passes the reference's 4096 bytes per block.  Host only. */
+int swm_records_to_ref_blocks(const swm_records_t *recs, uint64_t n_reads, swm_ref_blocks_t *out);
+void swm_ref_blocks_free(swm_ref_blocks_t *b);
 
 /* ---- SAM output (replaces sam_header and sam_format_2, mapper_mpe.c:315-687) ---- */
 

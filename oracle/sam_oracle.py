@@ -154,3 +154,38 @@ def sam_batch(ref_len, names, name_lens, read_names, seqs, quals, records):
         out += block_lines(ref_len, names, name_lens, read_names[i], seqs[i],
                            quals[i] if quals is not None else None, key[1], bool(r0["primary"]), blk)
     return b"".join(out)
+
+
+def _schar(b):
+    return b - 256 if b >= 128 else b
+
+
+def sam_format_2_buffers(ref_len, names, name_lens, read_names, seqs, quals, ref_pos, cigar_buffer, unmapped):
+    """sam_format_2 (mapper_mpe.c:426-687) reading the reference's own buffers:
+    unmapped = cigar_num2[0 .. cigar_unmap_size_pre) (read = entry / 2, :445-482);
+    ref_pos = ref_pos_buffer as rows of LOCATIONS = 258 words, cigar_buffer as
+    rows of CIGARS_PER_BLOCK * CIGAR_AVG_LEN = 4096 bytes (:486-492): per block
+    cur_read, cigar_num (main_flag = bit 31, count = low 28 bits, :494-495),
+    locations; per record a distance char, a length char, the text (:536-539)."""
+    out = []
+    for u in unmapped:
+        i = int(u) // 2
+        out.append(unmapped_line(read_names[i], seqs[i], quals[i] if quals is not None else None))
+    for b in range(len(ref_pos)):
+        row = ref_pos[b]
+        cur_read = int(row[0])
+        i, strand = cur_read // 2, cur_read & 1
+        num = int(row[1])
+        main_flag = num >> 31
+        num &= 0xfffffff
+        cig = bytes(cigar_buffer[b])
+        at = 0
+        blk = []
+        for j in range(num):
+            dist = _schar(cig[at])
+            clen = _schar(cig[at + 1])
+            blk.append((int(row[2 + j]), dist, cig[at + 2:at + 2 + clen]))
+            at += 2 + clen
+        out += block_lines(ref_len, names, name_lens, read_names[i], seqs[i],
+                           quals[i] if quals is not None else None, strand, bool(main_flag), blk)
+    return b"".join(out)

@@ -72,7 +72,18 @@ class SeqRec(C.Structure):
 
 
 class Records(C.Structure):
-    _fields_ = [("n", C.c_uint64), ("rec", C.c_void_p), ("text_bytes", C.c_uint64), ("cigar", C.c_void_p)]
+    _fields_ = [("n", C.c_uint64), ("rec", C.c_void_p), ("text_bytes", C.c_uint64), ("cigar", C.c_void_p),
+                ("n_blocks", C.c_uint64), ("block", C.c_void_p)]
+
+
+class RefBlocks(C.Structure):
+    _fields_ = [("n_blocks", C.c_uint64), ("ref_pos", C.c_void_p), ("cigar", C.c_void_p),
+                ("n_unmapped", C.c_uint64), ("unmapped", C.c_void_p)]
+
+
+REF_MAX_CIGAR = 256                      # SWM_REF_MAX_CIGAR
+REF_POS_STRIDE = REF_MAX_CIGAR + 2       # u32 words per block of ref_pos_buffer
+REF_TEXT_STRIDE = REF_MAX_CIGAR * 16     # bytes per block of cigar_buffer
 
 
 class ServerConfig(C.Structure):
@@ -148,6 +159,8 @@ def lib():
             "swm_host_free": ([vp], C.c_int),
             "swm_map_batch_records": ([vp, vp, u64, vp, u64, u32, vp, C.POINTER(Records)], C.c_int),
             "swm_records_free": ([C.POINTER(Records)], None),
+            "swm_records_to_ref_blocks": ([C.POINTER(Records), u64, C.POINTER(RefBlocks)], C.c_int),
+            "swm_ref_blocks_free": ([C.POINTER(RefBlocks)], None),
             "swm_sam_open": ([C.c_char_p, C.c_char_p, C.c_int, C.POINTER(vp)], C.c_int),
             "swm_sam_write_batch": ([vp, u64, C.POINTER(C.c_char_p), vp, vp, vp, u64, vp, C.POINTER(Records)],
                                     C.c_int),
@@ -312,8 +325,9 @@ class Mapper:
         _check("swm_map_batch", lib().swm_map_batch(self._h, _ptr(reads), stride, _ptr(lens), n, e, _ptr(out)))
         return out
 
-    def map_records(self, reads, lens, e=5):
-        """swm_map_batch_records: (RESULT_DTYPE[n], RECORD_DTYPE[m], CIGAR text bytes)."""
+    def map_records(self, reads, lens, e=5, blocks=False):
+        """swm_map_batch_records: (RESULT_DTYPE[n], RECORD_DTYPE[m], CIGAR text bytes), and with
+        blocks=True also the put_cigar blocks' first-record indices (uint32)."""
         reads = np.ascontiguousarray(reads, dtype=np.uint8)
         lens = np.ascontiguousarray(lens, dtype=np.uint32)
         n, stride = reads.shape
@@ -325,9 +339,11 @@ class Mapper:
             recs = np.frombuffer(C.string_at(r.rec, r.n * RECORD_DTYPE.itemsize), dtype=RECORD_DTYPE).copy() \
                 if r.n else np.zeros(0, RECORD_DTYPE)
             text = C.string_at(r.cigar, r.text_bytes) if r.text_bytes else b""
+            blk = np.frombuffer(C.string_at(r.block, r.n_blocks * 4), dtype=np.uint32).copy() \
+                if r.n_blocks else np.zeros(0, np.uint32)
         finally:
             lib().swm_records_free(C.byref(r))
-        return out, recs, text
+        return (out, recs, text, blk) if blocks else (out, recs, text)
 
     def map_device(self, d_reads, stride, d_lens, n, e, d_out):
         """Device pointers (ints), e.g. torch tensors' data_ptr()."""
@@ -371,6 +387,28 @@ class Mapper:
         return {OPTION_NAMES[i + 1]: ch[i] for i in range(N_OPTIONS)}
 
 
+def ref_blocks(recs, text, blocks, n_reads):
+    """swm_records_to_ref_blocks: the reference's ref_pos_buffer (uint32[n_blocks, 258]),
+    cigar_buffer (uint8[n_blocks, 4096]) and unmapped list (uint32) for one batch."""
+    recs = np.ascontiguousarray(recs, dtype=RECORD_DTYPE)
+    blocks = np.ascontiguousarray(blocks, dtype=np.uint32)
+    tbuf = np.frombuffer(text + b"\0", dtype=np.uint8)
+    r = Records(recs.size, recs.ctypes.data if recs.size else None, len(text), tbuf.ctypes.data,
+                blocks.size, blocks.ctypes.data if blocks.size else None)
+    b = RefBlocks()
+    _check("swm_records_to_ref_blocks", lib().swm_records_to_ref_blocks(C.byref(r), n_reads, C.byref(b)))
+    try:
+        nb, nu = b.n_blocks, b.n_unmapped
+        pos = np.frombuffer(C.string_at(b.ref_pos, nb * REF_POS_STRIDE * 4), np.uint32).reshape(nb, REF_POS_STRIDE) \
+            if nb else np.zeros((0, REF_POS_STRIDE), np.uint32)
+        cig = np.frombuffer(C.string_at(b.cigar, nb * REF_TEXT_STRIDE), np.uint8).reshape(nb, REF_TEXT_STRIDE) \
+            if nb else np.zeros((0, REF_TEXT_STRIDE), np.uint8)
+        unm = np.frombuffer(C.string_at(b.unmapped, nu * 4), np.uint32).copy() if nu else np.zeros(0, np.uint32)
+    finally:
+        lib().swm_ref_blocks_free(C.byref(b))
+    return pos.copy(), cig.copy(), unm
+
+
 def cigar_strings(recs, text):
     """The CIGAR text of each record."""
     return [text[int(o):int(o) + int(n)] for o, n in zip(recs["cigar_off"], recs["cigar_len"])]
@@ -400,7 +438,7 @@ class SamWriter:
         tbuf = np.frombuffer(text + b"\0", dtype=np.uint8)
         arr = (C.c_char_p * max(n, 1))(*names)
         nlen = np.array([len(x) for x in names], dtype=np.uint32)
-        r = Records(recs.size, recs.ctypes.data if recs.size else None, len(text), tbuf.ctypes.data)
+        r = Records(recs.size, recs.ctypes.data if recs.size else None, len(text), tbuf.ctypes.data, 0, None)
         _check("swm_sam_write_batch", lib().swm_sam_write_batch(self._h, n, arr, _ptr(nlen), _ptr(reads), _ptr(quals),
                                                                 stride, _ptr(lens), C.byref(r)))
 

@@ -1595,9 +1595,10 @@ struct CandSet {
     uint32_t *rec = nullptr;      /* record location (cigar->location) for passing lanes */
     uint32_t *info = nullptr;     /* CIGAR bytes | kind bit 16 (1 = traceback) | bit 17: record location < 0 */
     const uint32_t *ovf = nullptr;    /* capacity overflow flag (see PlanIO) */
-    /* record output (null unless records are asked for): per candidate, 1
-     * if the emission writes its record, the bytes of its CIGAR text, and the
-     * traceback's merged runs */
+    /* record output (null unless records are asked for): per candidate,
+     * WR_REC if the emission writes its record (plus WR_BLOCK when that
+     * record opens one of put_cigar's output blocks), the bytes of its CIGAR
+     * text, and the traceback's merged runs */
     uint64_t *wr = nullptr;
     uint32_t *tb = nullptr;
     uint16_t *runs = nullptr;     /* [ci * RUN_SLOT]: count, then runs in traceback order */
@@ -1620,6 +1621,11 @@ struct CandSet {
 
 #define INFO_TRACE (1u << 16)
 #define INFO_NEG (1u << 17)
+
+/* The records path's per-candidate word: one scan over it gives a written
+ * record's rank (low half) and its output block's rank (high half). */
+#define WR_REC 1ull
+#define WR_BLOCK (1ull << 32)
 
 /* (identity order, overflow) in one load: a workgroup of the Myers grid,
  * which covers every candidate, starts on one round trip, not two.  The
@@ -2643,8 +2649,9 @@ struct Emit {
     {
         if (cigar_num + 8 > 256 || cigar_len + 16 * 16 > 256 * 16) { cigar_num = 0; cigar_len = 0; }
     }
-    /* true when the candidate's record is written */
-    __device__ bool step(uint32_t bi, int32_t er, uint32_t rec, uint32_t info, uint32_t e, uint32_t strand)
+    /* 0: the candidate's record is not written; WR_REC: written; WR_REC |
+     * WR_BLOCK: written as the first record of an output block */
+    __device__ uint64_t step(uint32_t bi, int32_t er, uint32_t rec, uint32_t info, uint32_t e, uint32_t strand)
     {
         const uint32_t b = bi >> 12;
         if (b != cur_batch) {
@@ -2652,18 +2659,19 @@ struct Emit {
             cur_batch = b;
             vcount = 0;
         }
-        if (er == -2) return false;
+        if (er == -2) return 0;
         if (open && (vcount & 7u) == 0) { flush_check(); open = false; }
         open = true;
         vcount++;
-        if (er > (int32_t)e) return false;
-        if ((info & INFO_TRACE) && cigar_num > 0 && !(info & INFO_NEG) && last_loc == rec) return false;
+        if (er > (int32_t)e) return 0;
+        if ((info & INFO_TRACE) && cigar_num > 0 && !(info & INFO_NEG) && last_loc == rec) return 0;
+        const uint64_t w = cigar_num == 0 ? (WR_REC | WR_BLOCK) : WR_REC;
         cigar_num++;
         cigar_len += info & 0xffffu;
         last_loc = rec;
         if (count == 0 || (uint32_t)er < best_dist) { best_pos = rec; best_dist = (uint32_t)er; best_strand = strand; }
         count++;
-        return true;
+        return w;
     }
     __device__ void end_strand()
     {
@@ -2723,9 +2731,9 @@ __global__ void __launch_bounds__(128) k_emit(uint32_t nreads, uint32_t e, const
                 const int32_t er = c.err[ci];
                 const bool pass = er >= 0 && er <= (int32_t)e;
                 const uint32_t info = pass ? c.info[ci] : 0u;
-                const bool w = em.step(c.bi[ci], er, pass ? c.rec[ci] : 0u, info, e, strand);
+                const uint64_t w = em.step(c.bi[ci], er, pass ? c.rec[ci] : 0u, info, e, strand);
                 if (c.wr) {
-                    c.wr[ci] = w ? 1ull : 0ull;
+                    c.wr[ci] = w;
                     c.tb[ci] = w ? (info & 0xffffu) - 2u : 0u;
                 }
             }
@@ -2796,8 +2804,8 @@ __global__ void __launch_bounds__(64) k_emit_heavy(const uint32_t *__restrict__ 
             bool has_prev = false;
             uint32_t num = 0, len = 0;          /* output block counters */
             /* the open group */
-            bool g_open = false, g_seen = false, g_firstd = false;
-            uint32_t g_n = 0, g_len = 0, g_ftext = 0, g_frec = 0, g_fci = 0;
+            bool g_open = false, g_seen = false, g_firstd = false, g_wseen = false;
+            uint32_t g_n = 0, g_len = 0, g_ftext = 0, g_frec = 0, g_fci = 0, g_wci = 0;
             uint64_t g_fkey = 0;
             auto close_group = [&]() {
                 if (!g_open) return;
@@ -2806,11 +2814,15 @@ __global__ void __launch_bounds__(64) k_emit_heavy(const uint32_t *__restrict__ 
                     len = g_ftext;
                     count++;
                     if (g_fkey < best) { best = g_fkey; best_rec = g_frec; }
-                    /* the lane that stored this candidate's "not written" flag overwrites it */
+                    /* the lane that stored this candidate's "not written" flag
+                     * overwrites it; the record opens the output block */
                     if (c.wr && lane == ((g_fci - c0) & 63u)) {
-                        c.wr[g_fci] = 1ull;
+                        c.wr[g_fci] = WR_REC | WR_BLOCK;
                         c.tb[g_fci] = g_ftext - 2u;
                     }
+                } else if (num == 0 && g_wseen && c.wr && lane == ((g_wci - c0) & 63u)) {
+                    /* the group's first plainly written record opens the block */
+                    c.wr[g_wci] = WR_REC | WR_BLOCK;
                 }
                 num += g_n;
                 len += g_len;
@@ -2868,7 +2880,7 @@ __global__ void __launch_bounds__(64) k_emit_heavy(const uint32_t *__restrict__ 
                 /* plainly written lanes */
                 const bool wr = pass && !dup;
                 if (c.wr && in) {
-                    c.wr[ci] = wr ? 1ull : 0ull;
+                    c.wr[ci] = wr ? WR_REC : 0ull;
                     c.tb[ci] = wr ? (info & 0xffffu) - 2u : 0u;
                 }
                 const uint64_t am = __ballot(wr);
@@ -2899,8 +2911,11 @@ __global__ void __launch_bounds__(64) k_emit_heavy(const uint32_t *__restrict__ 
                 const uint64_t spr = pm & srm;
                 const uint32_t sf = spr ? (uint32_t)__ffsll((unsigned long long)spr) - 1u : 0u;
                 const uint64_t dm = __ballot(dup);
-                const uint32_t s_pack = (uint32_t)__popcll(am & srm) | (spr ? 0x100u : 0u) |
-                                        ((spr && ((dm >> sf) & 1ull)) ? 0x200u : 0u);
+                const uint64_t wsm = am & srm;    /* the segment's plainly written lanes */
+                const uint32_t sw = wsm ? (uint32_t)__ffsll((unsigned long long)wsm) - 1u : 0u;
+                const uint32_t s_pack = (uint32_t)__popcll(wsm) | (spr ? 0x100u : 0u) |
+                                        ((spr && ((dm >> sf) & 1ull)) ? 0x200u : 0u) |
+                                        (wsm ? 0x400u : 0u) | (sw << 16);
                 const uint32_t q_info = (uint32_t)__shfl((int)info, (int)sf, 64);
                 const uint32_t q_rec = (uint32_t)__shfl((int)rec, (int)sf, 64);
                 const uint32_t q_er = (uint32_t)__shfl((int)er, (int)sf, 64);
@@ -2908,10 +2923,11 @@ __global__ void __launch_bounds__(64) k_emit_heavy(const uint32_t *__restrict__ 
                     const uint32_t p = (uint32_t)__ffsll((unsigned long long)rest) - 1u;
                     if ((gm >> p) & 1ull) {
                         close_group();
-                        g_open = true; g_seen = false; g_firstd = false; g_n = 0; g_len = 0;
+                        g_open = true; g_seen = false; g_firstd = false; g_wseen = false; g_n = 0; g_len = 0;
                     }
                     if (!g_open) continue;
                     const uint32_t pk = rdlane(s_pack, p);
+                    if (!g_wseen && (pk & 0x400u)) { g_wseen = true; g_wci = base + (pk >> 16); }
                     g_n += pk & 0xffu;
                     g_len += rdlane(s_len, p);
                     if (!g_seen && (pk & 0x100u)) {
@@ -3039,21 +3055,27 @@ __device__ __forceinline__ uint32_t put_dec(char *o, uint32_t v)
 }
 
 /* Every written record at its rank among the written candidates (ridx, the
- * exclusive scan of wr) with its CIGAR text at toff (the scan of tb): "<L>M"
- * for a Hamming record (mapper_cpe.c:283-287), else the traceback's merged
- * runs from the last recorded to the first (:479-507).  A record is primary
- * when no candidate of its (read, strand) before it was written (main_flag,
- * :1462-1465). */
+ * exclusive scan of wr: low half) with its CIGAR text at toff (the scan of
+ * tb): "<L>M" for a Hamming record (mapper_cpe.c:283-287), else the
+ * traceback's merged runs from the last recorded to the first (:479-507).  A
+ * record is primary when no candidate of its (read, strand) before it was
+ * written (main_flag, :1462-1465).  A record that opens an output block (the
+ * first written after a put_cigar flush, :1461-1500) gives its rank to the
+ * block's entry (ridx's high half is the block's rank). */
 __global__ void __launch_bounds__(256) k_records(const uint32_t *__restrict__ ncand_p, const uint32_t *__restrict__ lens,
                                                  const uint32_t *__restrict__ cand_off, CandSet c,
                                                  const uint64_t *__restrict__ ridx, const uint32_t *__restrict__ toff,
-                                                 swm_record_t *__restrict__ out, char *__restrict__ text)
+                                                 swm_record_t *__restrict__ out, char *__restrict__ text,
+                                                 uint32_t *__restrict__ block_first)
 {
     const uint32_t ncand = *ncand_p;
     for (uint32_t ci = blockIdx.x * blockDim.x + threadIdx.x; ci < ncand; ci += gridDim.x * blockDim.x) {
-        if (!c.wr[ci]) continue;
+        const uint64_t w = c.wr[ci];
+        if (!w) continue;
         const uint32_t rs = c.rs[ci];
-        const uint64_t r = ridx[ci];
+        const uint64_t rb = ridx[ci];
+        const uint32_t r = (uint32_t)rb;
+        if (w & WR_BLOCK) block_first[rb >> 32] = r;
         const uint32_t off = toff[ci];
         char *o = text + off;
         uint32_t len = 0;
@@ -3076,7 +3098,7 @@ __global__ void __launch_bounds__(256) k_records(const uint32_t *__restrict__ nc
         x.cigar_off = off;
         x.strand = (uint8_t)(rs & 1u);
         x.dist = (uint8_t)c.err[ci];
-        x.primary = ridx[cand_off[rs]] == r ? 1 : 0;
+        x.primary = (uint32_t)ridx[cand_off[rs]] == r ? 1 : 0;
         x.cigar_len = (uint8_t)len;
         out[r] = x;
     }
@@ -3298,7 +3320,7 @@ struct swm_ws {
     struct Buf { void *p = nullptr; size_t cap = 0; };
     enum { WORDS, SEL, NSEL, NPOS, NCHUNK, SURV_OFF, BD_OFF, SURV, BDESC, NBATCH, NUNIQ, CAND_OFF, C_VAL, C_RS, C_BI,
            C_ERR, C_LOC, C_REC, C_INFO, COUNTERS, BIG, TRACE, HEAVY, HEAVY_C, HEAVY_E, SCAN_TMP, IN_READS, IN_LENS,
-           OUT, HIST, R_WR, R_TB, R_RUNS, R_IDX, R_TOFF, R_SCAN, R_TOT, R_OUT, R_TEXT, CDESC, HBATCH, GIANT, EPART, RPART, HPART, LBINS, PERM, IRREG, MERGE, NBUF };
+           OUT, HIST, R_WR, R_TB, R_RUNS, R_IDX, R_TOFF, R_SCAN, R_TOT, R_OUT, R_TEXT, R_BLOCK, CDESC, HBATCH, GIANT, EPART, RPART, HPART, LBINS, PERM, IRREG, MERGE, NBUF };
     Buf b[NBUF];
     uint64_t gen = 1;
     ~swm_ws() { for (auto &x : b) if (x.p) (void)hipFree(x.p); }
@@ -3817,35 +3839,37 @@ static int records_out(const swm_ctx_t *ctx, swm_slot *sl, Pipe &P, const uint32
     uint64_t *hs = sl->h_small;
     SWM_HIP_CHECK(hipMemcpyAsync(hs + 11, tot, 3 * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
     SWM_HIP_CHECK(hipStreamSynchronize(st));
-    const uint64_t nrec = hs[11], nbytes = hs[12];
+    const uint64_t nrec = (uint32_t)hs[11], nblk = hs[11] >> 32, nbytes = hs[12];
     if ((uint32_t)hs[13]) return SWM_ENOMEM;           /* CIGAR text beyond 4 GiB in one batch */
     if (nrec == 0) return SWM_OK;
     swm_record_t *d_rec;
     char *d_text;
-    if ((rc = ws->get(swm_ws::R_OUT, &d_rec, nrec)) || (rc = ws->get(swm_ws::R_TEXT, &d_text, nbytes + 1))) return rc;
+    uint32_t *d_blk;
+    if ((rc = ws->get(swm_ws::R_OUT, &d_rec, nrec)) || (rc = ws->get(swm_ws::R_TEXT, &d_text, nbytes + 1)) ||
+        (rc = ws->get(swm_ws::R_BLOCK, &d_blk, nblk)))
+        return rc;
     const uint32_t grid = std::min<uint32_t>(blocks_for(ncand, 256), (uint32_t)ctx->n_cu * 16);
-    k_records<<<grid, 256, 0, st>>>(P.cand_off + P.nrs, d_lens, P.cand_off, P.cs, ridx, toff, d_rec, d_text);
+    k_records<<<grid, 256, 0, st>>>(P.cand_off + P.nrs, d_lens, P.cand_off, P.cs, ridx, toff, d_rec, d_text, d_blk);
     SWM_HIP_CHECK(hipGetLastError());
     recs->rec = (swm_record_t *)malloc(nrec * sizeof(swm_record_t));
     recs->cigar = (char *)malloc(nbytes + 1);
-    if (!recs->rec || !recs->cigar) {
-        free(recs->rec);
-        free(recs->cigar);
-        memset(recs, 0, sizeof(*recs));
+    recs->block = (uint32_t *)malloc(nblk * sizeof(uint32_t));
+    if (!recs->rec || !recs->cigar || !recs->block) {
+        swm_records_free(recs);
         (void)hipStreamSynchronize(st);
         return SWM_ENOMEM;
     }
     if (hipMemcpyAsync(recs->rec, d_rec, nrec * sizeof(swm_record_t), hipMemcpyDeviceToHost, st) != hipSuccess ||
         hipMemcpyAsync(recs->cigar, d_text, nbytes, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipMemcpyAsync(recs->block, d_blk, nblk * sizeof(uint32_t), hipMemcpyDeviceToHost, st) != hipSuccess ||
         hipStreamSynchronize(st) != hipSuccess) {
-        free(recs->rec);
-        free(recs->cigar);
-        memset(recs, 0, sizeof(*recs));
+        swm_records_free(recs);
         return SWM_EDEVICE;
     }
     recs->cigar[nbytes] = 0;
     recs->n = nrec;
     recs->text_bytes = nbytes;
+    recs->n_blocks = nblk;
     return SWM_OK;
 }
 
@@ -4412,6 +4436,7 @@ extern "C" void swm_records_free(swm_records_t *recs)
     if (!recs) return;
     free(recs->rec);
     free(recs->cigar);
+    free(recs->block);
     memset(recs, 0, sizeof(*recs));
 }
 

@@ -475,3 +475,85 @@ int swm_sam_close(swm_sam_t *w)
     free_writer(w);
     return rc;
 }
+
+/* ---- the reference's output buffers ---- */
+
+/* put_cigar (mapper_cpe.c:1437-1507) fills, per batch and block b:
+ *   ref_pos_buffer[b * (MAX_CIGAR + 2)]      cur_read = 2 * read + strand
+ *                  [.. + 1]                  cigar_num | main_flag << 31
+ *                  [.. + 2 .. + 2 + num)     the records' locations
+ *   cigar_buffer[b * MAX_CIGAR * CIGAR_LEN]  per record: distance byte,
+ *                                            text-length byte, text
+ * and, for a read with no record on either strand (cigar_size == 0 at the
+ * reverse strand's last flush, :1447-1460), cur_read of that reverse strand
+ * into cigar_num[].  sam_format_2 (mapper_mpe.c:426-687) reads them with
+ * the same strides (LOCATIONS = 258, CIGARS_PER_BLOCK * CIGAR_AVG_LEN =
+ * 4096).  Blocks come in record order, unmapped reads in batch order. */
+int swm_records_to_ref_blocks(const swm_records_t *recs, uint64_t n_reads, swm_ref_blocks_t *out)
+{
+    if (!recs || !out || (recs->n && (!recs->rec || !recs->cigar || !recs->block || !recs->n_blocks)))
+        return SWM_EINVAL;
+    memset(out, 0, sizeof(*out));
+    const uint64_t nb = recs->n ? recs->n_blocks : 0;
+    const uint64_t pos_stride = SWM_REF_MAX_CIGAR + 2, text_stride = (uint64_t)SWM_REF_MAX_CIGAR * SWM_REF_CIGAR_LEN;
+    /* the blocks must tile the records: ascending starts from record 0, each
+     * within one (read, strand) and within put_cigar's limits */
+    for (uint64_t b = 0; b < nb; b++) {
+        const uint64_t r0 = recs->block[b], r1 = b + 1 < nb ? recs->block[b + 1] : recs->n;
+        if ((b == 0 && r0 != 0) || r1 <= r0 || r1 > recs->n || r1 - r0 > SWM_REF_MAX_CIGAR) return SWM_EINVAL;
+        uint64_t bytes = 0;
+        for (uint64_t r = r0; r < r1; r++) {
+            const swm_record_t *x = &recs->rec[r];
+            if (x->read >= n_reads || x->read != recs->rec[r0].read || x->strand != recs->rec[r0].strand ||
+                (uint64_t)x->cigar_off + x->cigar_len > recs->text_bytes || (x->primary && r != r0))
+                return SWM_EINVAL;
+            bytes += 2u + x->cigar_len;
+        }
+        /* a block past 4096 bytes overruns put_cigar's buffer in the reference */
+        if (bytes > text_stride) return SWM_EINVAL;
+    }
+    uint8_t *mapped = (uint8_t *)calloc(n_reads ? n_reads : 1, 1);
+    if (!mapped) return SWM_ENOMEM;
+    for (uint64_t r = 0; r < recs->n; r++) mapped[recs->rec[r].read] = 1;
+    uint64_t nu = 0;
+    for (uint64_t i = 0; i < n_reads; i++) nu += !mapped[i];
+    out->ref_pos = (uint32_t *)calloc(nb ? nb * pos_stride : 1, sizeof(uint32_t));
+    out->cigar = (char *)calloc(nb ? nb * text_stride : 1, 1);
+    out->unmapped = (uint32_t *)malloc((nu ? nu : 1) * sizeof(uint32_t));
+    if (!out->ref_pos || !out->cigar || !out->unmapped) {
+        free(mapped);
+        swm_ref_blocks_free(out);
+        return SWM_ENOMEM;
+    }
+    for (uint64_t i = 0, u = 0; i < n_reads; i++)
+        if (!mapped[i]) out->unmapped[u++] = (uint32_t)(2 * i + 1);
+    for (uint64_t b = 0; b < nb; b++) {
+        const uint64_t r0 = recs->block[b], r1 = b + 1 < nb ? recs->block[b + 1] : recs->n;
+        const swm_record_t *h = &recs->rec[r0];
+        uint32_t *p = out->ref_pos + b * pos_stride;
+        char *t = out->cigar + b * text_stride;
+        p[0] = 2u * h->read + h->strand;
+        p[1] = (uint32_t)(r1 - r0) | (h->primary ? 1u << 31 : 0u);
+        for (uint64_t r = r0; r < r1; r++) {
+            const swm_record_t *x = &recs->rec[r];
+            p[2 + (r - r0)] = x->pos;
+            *t++ = (char)x->dist;
+            *t++ = (char)x->cigar_len;
+            memcpy(t, recs->cigar + x->cigar_off, x->cigar_len);
+            t += x->cigar_len;
+        }
+    }
+    free(mapped);
+    out->n_blocks = nb;
+    out->n_unmapped = nu;
+    return SWM_OK;
+}
+
+void swm_ref_blocks_free(swm_ref_blocks_t *b)
+{
+    if (!b) return;
+    free(b->ref_pos);
+    free(b->cigar);
+    free(b->unmapped);
+    memset(b, 0, sizeof(*b));
+}

@@ -171,3 +171,12 @@ def myers(read_words, win_lo, win_hi, L, ref_len, e, width=32):
     f(_p(np.ascontiguousarray(read_words, np.uint32)), _p(np.ascontiguousarray(win_lo, np.uint32)),
       _p(np.ascontiguousarray(win_hi, np.uint32)), L, ref_len, e, C.byref(err), C.byref(loc))
     return err.value, loc.value
+
+
+def block_starts(orecs):
+    """Indices of the oracle records that open a put_cigar block: a new
+    (read, strand) or a new flush block number within it."""
+    if orecs.size == 0:
+        return np.zeros(0, np.uint32)
+    key = (orecs["read"].astype(np.int64) * 2 + orecs["strand"]) * 65536 + orecs["block"]
+    return np.flatnonzero(np.r_[True, key[1:] != key[:-1]]).astype(np.uint32)

@@ -79,12 +79,50 @@ def test_records_match_oracle(work, name, e, si):
     m = swm.Mapper.from_index(swm.build_index(g, 12))
     for k, v in opts.items():
         m.set_option(k, v)
-    res, recs, text = m.map_records(r, l, e)
+    res, recs, text, blocks = m.map_records(r, l, e, blocks=True)
     _assert_results(res, hits, "results")
     _assert_records(recs, text, orecs, f"{name} e={e}")
+    ob = orc.block_starts(orecs)
+    assert blocks.size == ob.size and (blocks == ob).all(), \
+        f"put_cigar blocks: {blocks.size} vs oracle {ob.size}, first difference at " \
+        f"{np.flatnonzero(blocks[:min(blocks.size, ob.size)] != ob[:min(blocks.size, ob.size)])[:3]}"
     assert m.stats()["n_records"] == recs.size
     _assert_results(m.map(r, l, e), hits, "map after records")
     m.close()
+
+
+@pytest.mark.parametrize("si", range(len(SETTINGS)))
+def test_reference_buffers_from_device_records(work, si):
+    """The reference's ref_pos_buffer / cigar_buffer / unmapped list built
+    from the device's records and blocks equal those built from the oracle's,
+    byte for byte, and read back by sam_format_2's buffer walk give the
+    record-level SAM restatement's lines (tandem reads flush many blocks)."""
+    g, r, l = work["tandem"]
+    opts = SETTINGS[si]
+    _say("reference buffers", {swm.OPTION_NAMES[k]: v for k, v in opts.items()})
+    ref = orc.OracleRef.from_index(orc.OracleIndex(g, 12))
+    hits, orecs, _ = ref.trace(r, l, 5)
+    m = swm.Mapper.from_index(swm.build_index(g, 12))
+    for k, v in opts.items():
+        m.set_option(k, v)
+    _, recs, text, blocks = m.map_records(r, l, 5, blocks=True)
+    m.close()
+    pos, cig, unm = swm.ref_blocks(recs, text, blocks, l.size)
+    orecs_p = np.zeros(orecs.size, swm.RECORD_DTYPE)
+    otext = bytearray()
+    for i, x in enumerate(orecs):
+        c = bytes(x["cigar"])
+        orecs_p[i] = (x["read"], x["loc"], len(otext), x["strand"], x["dist"], x["primary"], len(c))
+        otext += c
+    epos, ecig, eunm = swm.ref_blocks(orecs_p, bytes(otext), orc.block_starts(orecs), l.size)
+    assert pos.shape[0] > len(set(zip(orecs["read"].tolist(), orecs["strand"].tolist())))   # mid-strand flushes
+    assert np.array_equal(pos, epos) and np.array_equal(cig, ecig) and np.array_equal(unm, eunm)
+    n = l.size
+    names = [b"q%d" % i for i in range(n)]
+    seqs = [bytes(r[i, :l[i]]) for i in range(n)]
+    ref_len, cn, cnl = [0, 400_000], [b"chr1"], [4]
+    assert so.sam_format_2_buffers(ref_len, cn, cnl, names, seqs, None, pos, cig, unm) == \
+        so.sam_batch(ref_len, cn, cnl, names, seqs, None, orecs)
 
 
 def test_records_empty_and_unmapped():
