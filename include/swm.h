@@ -366,12 +366,14 @@ int swm_myers_device(swm_ctx_t *ctx, const uint32_t *d_read_words, const uint32_
                      const uint32_t *d_ref_len, uint64_t n, uint32_t e,
                      int32_t *d_err, int32_t *d_loc);
 
-/* Entry index of k-mer `code` (k = 2..14) in the device seed table: the
- * table holds (offset, count) of every k-mer with a k-mer and its reverse
- * complement in one 128-byte line (two 8-byte entries of one 256-byte group
- * half).  Host function, for layout checks; no counterpart in the reference,
- * whose hash_offset array (mapper_mpe.c:250-255) is indexed by code. */
-uint64_t swm_seed_slot(uint32_t code, uint32_t k);
+/* Entry index of k-mer `code` (k = 2..14) in the device seed table, in
+ * role 0 (keyed by its last k-1 bases) or 1 (by its first k-1 bases): the
+ * table holds (offset, count) of every k-mer twice, so that two consecutive
+ * k-mers of a read and the reverse complements of both share one 128-byte
+ * line (16 entries of 8 bytes).  Host function, for layout checks; no
+ * counterpart in the reference, whose hash_offset array
+ * (mapper_mpe.c:250-255) is indexed by code. */
+uint64_t swm_seed_slot(uint32_t code, uint32_t k, uint32_t role);
 
 /* Measured int32 VALU throughput of the current device (lane ops/s), the
  * peak the Myers roofline is quoted against.  No counterpart in the

@@ -137,7 +137,7 @@ def lib():
             "swm_synth_genome": ([vp, u64, u64, C.c_int, C.c_double], C.c_int),
             "swm_device_count": ([C.POINTER(C.c_int)], C.c_int),
             "swm_valu_probe": ([C.POINTER(C.c_double)], C.c_int),
-            "swm_seed_slot": ([u32, u32], u64),
+            "swm_seed_slot": ([u32, u32, u32], u64),
             "swm_dev_alloc": ([C.POINTER(vp), u64], C.c_int),
             "swm_dev_free": ([vp], C.c_int),
             "swm_copy_h2d": ([vp, vp, u64], C.c_int),

@@ -62,8 +62,9 @@ struct swm_ctx {
     int n_cu = 256;                     /* compute units: sizes the persistent work-queue grids */
     uint32_t k = 12;
     uint64_t n_buckets = 0;
-    uint2 *d_seed_tab = nullptr;        /* (offset, count) per k-mer, a k-mer and its reverse
-                                         * complement in one line: 2 x 4^k entries */
+    uint2 *d_seed_tab = nullptr;        /* (offset, count) of every k-mer in two roles: two
+                                         * consecutive k-mers and their reverse complements in
+                                         * one line (8 or 16 x 4^(k-1) entries) */
     uint2 *d_pl = nullptr;              /* bucket entries: (position, 12-bit residual) */
     uint64_t n_pos = 0;
     uint32_t *d_base_2bits = nullptr;   /* file words + SWM_WIN_PAIRS_PAD zero pairs */

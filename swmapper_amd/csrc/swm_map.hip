@@ -170,6 +170,7 @@ __device__ __forceinline__ T block_incl_scan(T v, T *s_w, T *total)
 /* ------------------------------------------------------------------ k_seed */
 
 #define SEED_READS 64        /* reads per workgroup (2 threads each) */
+#define SEED_ROW 68u         /* LDS bytes per staged read (64 used) */
 
 #define NSEL_HEAVY 0x80000000u
 /* some selected 12-mer's bucket is not in ascending order: the item is
@@ -275,22 +276,27 @@ __device__ __forceinline__ void pack_quad(uint32_t raw, uint8_t *q, uint8_t *v)
 /* ------------------------------------------------------------ seed table
  *
  * The offsets and counts of the k-mers (choose_seed's hash_offset gathers,
- * mapper_cpe.c:1372-1390) are kept in a table laid out so that a k-mer and
- * its reverse complement share one 128-byte line.  The reverse strand of a
+ * mapper_cpe.c:1372-1390) are kept in a table laid out so that one 128-byte
+ * line serves four of a read's lookups.  A long seed's four k-mers c0..c3
+ * (shifts q = 0..3) are consecutive: c_q without its first base is c_{q+1}
+ * without its last, one shared (k-1)-mer s.  And the reverse strand of a
  * read is its reverse complement, so its k-mers are (for L = 15n exactly,
  * for other lengths in part) the reverse complements of the forward
- * strand's: the two threads of a read fetch each line once between them.
+ * strand's, whose shared (k-1)-mers are rc(s).
  *
- * A k-mer c is split into its inner k-2 bases (hi) and the digit d0 of its
- * outer bases (b0 | b_{k-1} << 2).  Reverse complement maps hi to rc(hi) and
- * d0 = (x, y) to (3 - y, 3 - x).  The pair {c, rc(c)} is stored in the
- * 256-byte group of rep = min(hi, rc(hi)): 16 outer digits x 2 orientations
- * (hi above rc(hi) or not) of 8-byte entries, the digits ordered so that a
- * digit and its image lie in the same 128-byte half.  Entry (off, count):
- * off with the irregular-bucket flag of the offset entry (bit 31), count the
- * bucket's hits.  The table is 16 x 4^k bytes, half of it touched. */
-#define SEED_SLOT_ORDER 0x13569B74DEA2FC80ull   /* nibble d: half(d) * 8 + rank of d in its half */
-
+ * So every k-mer c is stored twice: in the role "x s" (s = its last k-1
+ * bases, x its first base; looked up at even q) and in the role "s y" (s =
+ * its first k-1 bases, y its last; odd q), both in the line of the pair
+ * {s, rc(s)}: 2 orientations x 2 roles x 4 extension bases = 16 entries of
+ * 8 bytes.  The pair (c0, c1) then shares a line, (c2, c3) another, and the
+ * reverse strand's (rc(c3), rc(c2)) and (rc(c1), rc(c0)) the same two:
+ * the two threads of a read fetch two lines per long seed between them.
+ * Lines are indexed by the pair's representative: for odd k-1 the member
+ * whose middle base is A or C (reverse complement maps the middle base to
+ * its complement), the middle base's high bit dropped -- dense, 4^(k-1) / 2
+ * lines; for even k-1 min(s, rc(s)), 4^(k-1) lines of which about half are
+ * touched.  Entry (off, count): off with the irregular-bucket flag of the
+ * offset entry (bit 31), count the bucket's hits. */
 __host__ __device__ __forceinline__ uint32_t rc_bases(uint32_t v, uint32_t nb)
 {
     if (nb == 0) return 0u;
@@ -309,25 +315,44 @@ __host__ __device__ __forceinline__ uint32_t rc_bases(uint32_t v, uint32_t nb)
     return r >> (32u - 2u * nb);
 }
 
-__host__ __device__ __forceinline__ uint64_t seed_slot(uint32_t c, uint32_t k)
+/* the slot of k-mer c in role 0 ("x s") or 1 ("s y") */
+__host__ __device__ __forceinline__ uint64_t seed_slot(uint32_t c, uint32_t k, uint32_t role)
 {
-    const uint32_t nb = k - 2u;
-    const uint32_t hi = (c >> 2) & ((1u << (2u * nb)) - 1u);
-    const uint32_t d0 = (c & 3u) | ((c >> (2u * (k - 1u))) << 2);
-    const uint32_t r = rc_bases(hi, nb);
-    const uint32_t rep = hi < r ? hi : r;
-    const uint32_t sb = (uint32_t)(SEED_SLOT_ORDER >> (4u * d0)) & 15u;
-    return (uint64_t)rep * 32u + sb * 2u + (hi > r ? 1u : 0u);
+    const uint32_t n = k - 1u;
+    const uint32_t sm = (1u << (2u * n)) - 1u;
+    const uint32_t sv = role ? (c & sm) : (c >> 2);
+    const uint32_t ext = role ? (c >> (2u * n)) : (c & 3u);
+    const uint32_t r = rc_bases(sv, n);
+    uint32_t line, o;
+    if (n & 1u) {
+        const uint32_t h = n >> 1;
+        o = ((sv >> (2u * h)) & 3u) >= 2u ? 1u : 0u;
+        const uint32_t rep = o ? r : sv;
+        const uint32_t lo = rep & ((1u << (2u * h)) - 1u);
+        line = lo | (((rep >> (2u * h)) & 1u) << (2u * h)) | ((rep >> (2u * h + 2u)) << (2u * h + 1u));
+    } else {
+        o = sv > r ? 1u : 0u;
+        line = o ? r : sv;
+    }
+    return (uint64_t)line * 16u + o * 8u + role * 4u + ext;
 }
 
-/* entry c of the table from the offset array (off_mask: the offset bits) */
+/* entries of the table: 8 per (k-1)-mer when k-1 is odd, 16 when even */
+__host__ __device__ __forceinline__ uint64_t seed_table_entries(uint32_t k)
+{
+    return ((k - 1u) & 1u ? 8ull : 16ull) << (2u * (k - 1u));
+}
+
+/* both entries of k-mer c from the offset array (off_mask: the offset bits) */
 __global__ void __launch_bounds__(256) k_seed_table(const uint32_t *__restrict__ off, uint64_t n_buckets, uint32_t k,
                                                     uint32_t off_mask, uint2 *__restrict__ tab)
 {
     const uint64_t c = (uint64_t)blockIdx.x * 256u + threadIdx.x;
     if (c >= n_buckets) return;
     const uint32_t a = off[c], b = off[c + 1];
-    tab[seed_slot((uint32_t)c, k)] = make_uint2(a, (b & off_mask) - (a & off_mask));
+    const uint2 v = make_uint2(a, (b & off_mask) - (a & off_mask));
+    tab[seed_slot((uint32_t)c, k, 0u)] = v;
+    tab[seed_slot((uint32_t)c, k, 1u)] = v;
 }
 
 /* The offset and count of the 12-mers of the first SEED_CACHE_M long seeds
@@ -338,13 +363,17 @@ __global__ void __launch_bounds__(256) k_seed_table(const uint32_t *__restrict__
  * SIMD issue gathers: the kernel is bound by the rate of random line fetches
  * (one per lookup), and that rate grows with the waves in flight. */
 #define SEED_CACHE_M 10
+#define SEED_CHUNK 8
 
 __global__ void __launch_bounds__(128) k_seed(const char *__restrict__ reads, uint64_t stride,
                                               const uint32_t *__restrict__ lens, uint32_t nreads, uint32_t k,
                                               uint32_t e, uint32_t seln, const uint2 *__restrict__ seed_tab,
                                               const swm_recipe_t *__restrict__ recipes, SeedOut o)
 {
-    __shared__ uint8_t s_q[SEED_READS * 64], s_v[SEED_READS * 64];
+    /* one row per read, SEED_ROW bytes apart: 17 words, so the rows of a
+     * wave's 32 reads start in distinct banks (64-byte rows had put every
+     * fourth read's bases in one bank) */
+    __shared__ uint8_t s_q[SEED_READS * SEED_ROW], s_v[SEED_READS * SEED_ROW];
     __shared__ uint32_t s_off[4 * SEED_CACHE_M * 2 * SEED_READS];
     __shared__ uint16_t s_cnt[4 * SEED_CACHE_M * 2 * SEED_READS];
     if (blockIdx.x == 0) {
@@ -385,7 +414,8 @@ __global__ void __launch_bounds__(128) k_seed(const char *__restrict__ reads, ui
                 for (int u = 0; u < 8; u++) {
                     const uint32_t qd = q0 + (uint32_t)u * blockDim.x;
                     const uint32_t kq = qd & 63u;
-                    if (qd < nq) pack_quad(4u * kq < stride ? raw[u] : 0u, &s_q[qd], &s_v[qd]);
+                    const uint32_t at = (qd >> 6) * SEED_ROW + kq;
+                    if (qd < nq) pack_quad(4u * kq < stride ? raw[u] : 0u, &s_q[at], &s_v[at]);
                 }
             }
         }
@@ -399,7 +429,7 @@ __global__ void __launch_bounds__(128) k_seed(const char *__restrict__ reads, ui
                 for (uint32_t j = 0; j < 4; j++)
                     if (4u * kq + j < stride && at + j < nbytes) raw |= (uint32_t)(uint8_t)src[at + j] << (8 * j);
             }
-            pack_quad(raw, &s_q[qd], &s_v[qd]);
+            pack_quad(raw, &s_q[lr * SEED_ROW + kq], &s_v[lr * SEED_ROW + kq]);
         }
     }
     __syncthreads();
@@ -408,7 +438,7 @@ __global__ void __launch_bounds__(128) k_seed(const char *__restrict__ reads, ui
     const int strand = (int)(threadIdx.x & 1u);
     const uint32_t read = r0 + lr;
     const uint32_t r = 2 * read + (uint32_t)strand;
-    const uint8_t *sq = s_q + lr * 64u, *sv = s_v + lr * 64u;
+    const uint8_t *sq = s_q + lr * SEED_ROW, *sv = s_v + lr * SEED_ROW;
     uint32_t L = lens[read];
     const uint32_t seed_len = k + SWM_SEED_STEP - 1;
     /* reads that do not fit the reference's fixed tables map to nothing:
@@ -455,27 +485,53 @@ __global__ void __launch_bounds__(128) k_seed(const char *__restrict__ reads, ui
      * i.e. of long seed n - 1 - m at shift 3 - q when L = 15n (nearby
      * otherwise), so the reverse thread walks its long seeds and shifts in
      * mirrored order: the two threads of a read request each shared line
-     * of the seed table in the same load. */
+     * of the seed table in the same load.  Shift q is looked up in role
+     * q & 1, so shifts (0, 1) and (2, 3) share a line. */
     uint32_t tot[16], idx[16];
 #pragma unroll
     for (int m = 0; m < 16; m++) {
         tot[m] = 0;
         idx[m] = (uint32_t)m;
-        if ((uint32_t)m < n_long) {
-            const uint32_t mm = strand ? n_long - 1u - (uint32_t)m : (uint32_t)m;
-            uint32_t cq[4];
-            seed_codes(sq, sv, L, strand, k, mm * seed_len, cq);
-            uint2 a[4];
+    }
+    /* SEED_CHUNK long seeds' gathers are issued before any is used: the
+     * workgroup's LDS holds two waves per SIMD, so a thread's own loads in
+     * flight are what hides the memory latency (one long seed at a time had
+     * put a full round trip between every four gathers) */
 #pragma unroll
-            for (int q = 0; q < 4; q++) a[q] = seed_tab[seed_slot(strand ? cq[3 - q] : cq[q], k)];
+    for (int m0 = 0; m0 < 16; m0 += SEED_CHUNK) {
+        if ((uint32_t)m0 < n_long) {
+            uint2 a[SEED_CHUNK][4];
 #pragma unroll
-            for (int q = 0; q < 4; q++) {
-                const uint32_t c = a[q].y;
-                const uint32_t qq = strand ? 3u - (uint32_t)q : (uint32_t)q;
-                tot[m] += c;
-                if (mm < SEED_CACHE_M) {
-                    s_off[(4 * mm + qq) * (2 * SEED_READS) + threadIdx.x] = a[q].x;    /* with its flag */
-                    s_cnt[(4 * mm + qq) * (2 * SEED_READS) + threadIdx.x] = (uint16_t)min(c, 0xffffu);
+            for (int u = 0; u < SEED_CHUNK; u++) {
+                const int m = m0 + u;
+#pragma unroll
+                for (int q = 0; q < 4; q++) a[u][q] = make_uint2(0u, 0u);
+                if (m < 16 && (uint32_t)m < n_long) {
+                    const uint32_t mm = strand ? n_long - 1u - (uint32_t)m : (uint32_t)m;
+                    uint32_t cq[4];
+                    seed_codes(sq, sv, L, strand, k, mm * seed_len, cq);
+#pragma unroll
+                    for (int q = 0; q < 4; q++) {
+                        const int j = strand ? 3 - q : q;
+                        a[u][q] = seed_tab[seed_slot(cq[j], k, (uint32_t)j & 1u)];
+                    }
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < SEED_CHUNK; u++) {
+                const int m = m0 + u;
+                if (m < 16 && (uint32_t)m < n_long) {
+                    const uint32_t mm = strand ? n_long - 1u - (uint32_t)m : (uint32_t)m;
+#pragma unroll
+                    for (int q = 0; q < 4; q++) {
+                        const uint32_t c = a[u][q].y;
+                        const uint32_t qq = strand ? 3u - (uint32_t)q : (uint32_t)q;
+                        tot[m] += c;
+                        if (mm < SEED_CACHE_M) {
+                            s_off[(4 * mm + qq) * (2 * SEED_READS) + threadIdx.x] = a[u][q].x;    /* with its flag */
+                            s_cnt[(4 * mm + qq) * (2 * SEED_READS) + threadIdx.x] = (uint16_t)min(c, 0xffffu);
+                        }
+                    }
                 }
             }
         }
@@ -535,7 +591,7 @@ __global__ void __launch_bounds__(128) k_seed(const char *__restrict__ reads, ui
                 seed_codes(sq, sv, L, strand, k, base, cq);
 #pragma unroll
                 for (int q = 0; q < 4; q++) {
-                    const uint2 v = seed_tab[seed_slot(cq[q], k)];
+                    const uint2 v = seed_tab[seed_slot(cq[q], k, (uint32_t)q & 1u)];
                     off4[q] = v.x;
                     cnt4[q] = v.y;
                 }
@@ -4247,7 +4303,7 @@ extern "C" int swm_ctx_create(uint32_t k, const uint32_t *hash_count, const uint
     int rc = SWM_OK;
     const size_t rec_bytes = (SWM_RECIPE_MAX_L + 1) * sizeof(swm_recipe_t);
     uint32_t *d_off = nullptr;
-    const uint64_t tab_bytes = 2 * c->n_buckets * sizeof(uint2);
+    const uint64_t tab_bytes = seed_table_entries(k) * sizeof(uint2);
     if (hipMalloc((void **)&d_off, (c->n_buckets + 1) * 4) != hipSuccess ||
         hipMalloc((void **)&c->d_seed_tab, tab_bytes) != hipSuccess ||
         hipMalloc((void **)&c->d_pl, (n_pos + 64) * 8) != hipSuccess ||
@@ -4281,7 +4337,7 @@ extern "C" int swm_ctx_create(uint32_t k, const uint32_t *hash_count, const uint
     return SWM_OK;
 }
 
-extern "C" uint64_t swm_seed_slot(uint32_t code, uint32_t k) { return seed_slot(code, k); }
+extern "C" uint64_t swm_seed_slot(uint32_t code, uint32_t k, uint32_t role) { return seed_slot(code, k, role & 1u); }
 
 extern "C" void swm_ctx_destroy(swm_ctx_t *c)
 {
