@@ -55,6 +55,7 @@ struct swm_opts {
     uint32_t myers_packed = 1;      /* e <= 7: two candidates per lane in 16-bit halves */
     uint32_t giant_pos = 16384;     /* heavy plan items with more positions are taken first */
     uint32_t giant_cands = 16384;   /* heavy emission reads with more candidates are taken first */
+    uint32_t seed_generic = 0;      /* 1: the seed kernel for any k even where a fixed-k one exists */
 };
 
 struct swm_ctx {

@@ -355,27 +355,12 @@ __global__ void __launch_bounds__(256) k_seed_table(const uint32_t *__restrict__
     tab[seed_slot((uint32_t)c, k, 1u)] = v;
 }
 
-/* The offset and count of the 12-mers of the first SEED_CACHE_M long seeds
- * of the thread's strand are kept in LDS from the ranking gathers, so the
- * selected ones are not fetched twice; the few selected beyond them (reads
- * of 165 bases and more) and counts of 65535 or more are gathered again.
- * With the reads packed, the workgroup holds 38 KB of LDS, so two waves per
- * SIMD issue gathers: the kernel is bound by the rate of random line fetches
- * (one per lookup), and that rate grows with the waves in flight. */
-#define SEED_CACHE_M 10
-#define SEED_CHUNK 8
-
-__global__ void __launch_bounds__(128) k_seed(const char *__restrict__ reads, uint64_t stride,
-                                              const uint32_t *__restrict__ lens, uint32_t nreads, uint32_t k,
-                                              uint32_t e, uint32_t seln, const uint2 *__restrict__ seed_tab,
-                                              const swm_recipe_t *__restrict__ recipes, SeedOut o)
+/* Work of every seed kernel before its per-thread part: workgroup 0 zeroes
+ * the run's counters and the sentinels; the workgroup's reads are staged in
+ * LDS packed, one SEED_ROW row each.  Returns the workgroup's read count. */
+__device__ __forceinline__ uint32_t seed_stage(const char *__restrict__ reads, uint64_t stride, uint32_t nreads,
+                                               SeedOut &o, uint8_t *s_q, uint8_t *s_v)
 {
-    /* one row per read, SEED_ROW bytes apart: 17 words, so the rows of a
-     * wave's 32 reads start in distinct banks (64-byte rows had put every
-     * fourth read's bases in one bank) */
-    __shared__ uint8_t s_q[SEED_READS * SEED_ROW], s_v[SEED_READS * SEED_ROW];
-    __shared__ uint32_t s_off[4 * SEED_CACHE_M * 2 * SEED_READS];
-    __shared__ uint16_t s_cnt[4 * SEED_CACHE_M * 2 * SEED_READS];
     if (blockIdx.x == 0) {
         /* the run's counters and the sentinel entries the scans read */
         if (threadIdx.x < 12) o.counters[threadIdx.x] = 0;
@@ -432,6 +417,32 @@ __global__ void __launch_bounds__(128) k_seed(const char *__restrict__ reads, ui
             pack_quad(raw, &s_q[lr * SEED_ROW + kq], &s_v[lr * SEED_ROW + kq]);
         }
     }
+    return nr;
+}
+
+/* The offset and count of the 12-mers of the first SEED_CACHE_M long seeds
+ * of the thread's strand are kept in LDS from the ranking gathers, so the
+ * selected ones are not fetched twice; the few selected beyond them (reads
+ * of 165 bases and more) and counts of 65535 or more are gathered again.
+ * With the reads packed, the workgroup holds 38 KB of LDS, so two waves per
+ * SIMD issue gathers: the kernel is bound by the rate of random line fetches
+ * (one per lookup), and that rate grows with the waves in flight. */
+#define SEED_CACHE_M 10
+#define SEED_CHUNK 8
+
+__global__ void __launch_bounds__(128) k_seed(const char *__restrict__ reads, uint64_t stride,
+                                              const uint32_t *__restrict__ lens, uint32_t nreads, uint32_t k,
+                                              uint32_t e, uint32_t seln, const uint2 *__restrict__ seed_tab,
+                                              const swm_recipe_t *__restrict__ recipes, SeedOut o)
+{
+    /* one row per read, SEED_ROW bytes apart: 17 words, so the rows of a
+     * wave's 32 reads start in distinct banks (64-byte rows had put every
+     * fourth read's bases in one bank) */
+    __shared__ uint8_t s_q[SEED_READS * SEED_ROW], s_v[SEED_READS * SEED_ROW];
+    __shared__ uint32_t s_off[4 * SEED_CACHE_M * 2 * SEED_READS];
+    __shared__ uint16_t s_cnt[4 * SEED_CACHE_M * 2 * SEED_READS];
+    const uint32_t r0 = blockIdx.x * SEED_READS;
+    const uint32_t nr = seed_stage(reads, stride, nreads, o, s_q, s_v);
     __syncthreads();
     const uint32_t lr = threadIdx.x >> 1;
     if (lr >= nr) return;
@@ -626,6 +637,302 @@ __global__ void __launch_bounds__(128) k_seed(const char *__restrict__ reads, ui
     o.nsel[r] = n_sel_long * SWM_SEED_STEP | (heavy ? NSEL_HEAVY : 0u) | (irr ? NSEL_IRREG : 0u);
     o.npos[r] = npos;
     o.nchunk[r] = nchunk + 1;   /* batches <= chunks + 1 (an empty first batch) */
+}
+
+/* ---- the seed kernel for a seed length fixed at compile time ----
+ *
+ * The same results as k_seed, with the read held in registers: each thread
+ * loads its read's staged row once (word loads), orients it (the reverse
+ * strand's words are the row reversed, complemented and shifted by its
+ * length), and takes every base position from compile-time bit offsets --
+ * k_seed reads two LDS bytes per base visited (about 900 LDS instructions
+ * per wave), which made LDS the kernel's bottleneck. */
+
+/* 2-bit fields of w in reverse order */
+__device__ __forceinline__ uint32_t rev_fields(uint32_t w)
+{
+    const uint32_t r = __builtin_bitreverse32(w);
+    return ((r >> 1) & 0x55555555u) | ((r & 0x55555555u) << 1);
+}
+
+/* the staged flag word of 16 bases (flags in bits 0-3 of each byte) as a
+ * 2-bit mask per base */
+__device__ __forceinline__ uint32_t flag_mask2(uint32_t x)
+{
+    uint32_t f = (x & 0xfu) | ((x >> 4) & 0xf0u) | ((x >> 8) & 0xf00u) | ((x >> 12) & 0xf000u);
+    f = (f | (f << 8)) & 0x00ff00ffu;
+    f = (f | (f << 4)) & 0x0f0f0f0fu;
+    f = (f | (f << 2)) & 0x33333333u;
+    f = (f | (f << 1)) & 0x55555555u;
+    return f * 3u;
+}
+
+/* the even bits of x gathered into its low 16 bits */
+__device__ __forceinline__ uint32_t even_bits(uint32_t x)
+{
+    x &= 0x55555555u;
+    x = (x | (x >> 1)) & 0x33333333u;
+    x = (x | (x >> 2)) & 0x0f0f0f0fu;
+    x = (x | (x >> 4)) & 0x00ff00ffu;
+    return (x | (x >> 8)) & 0x0000ffffu;
+}
+
+/* a[j] = a[j + sh] (0 past the end) for a per-lane shift sh < 32, in five
+ * select stages */
+template <int N, typename T>
+__device__ __forceinline__ void shift_down(T (&a)[N], uint32_t sh)
+{
+#pragma unroll
+    for (int b = 1; b < N; b <<= 1) {
+        const bool on = (sh & (uint32_t)b) != 0u;
+#pragma unroll
+        for (int j = 0; j < N; j++) a[j] = on ? (j + b < N ? a[j + b] : (T)0) : a[j];
+    }
+    if (sh >= (uint32_t)N) {
+#pragma unroll
+        for (int j = 0; j < N; j++) a[j] = (T)0;
+    }
+}
+
+template <uint32_t K>
+__global__ void __launch_bounds__(128) k_seed_fixed(const char *__restrict__ reads, uint64_t stride,
+                                                    const uint32_t *__restrict__ lens, uint32_t nreads,
+                                                    uint32_t e, uint32_t seln, const uint2 *__restrict__ seed_tab,
+                                                    const swm_recipe_t *__restrict__ recipes, SeedOut o)
+{
+    constexpr uint32_t SL = K + SWM_SEED_STEP - 1;          /* long seed length */
+    static_assert(2 * SL <= 32, "a long seed's codes must fit one word");
+    constexpr uint32_t SMASK = (uint32_t)((1ull << (2 * SL)) - 1ull);
+    constexpr uint32_t KMASK = (1u << (2 * K)) - 1u;
+    /* the value of a non-ACGT base at place t of a long seed (seed_val, :1260-1300) */
+    constexpr uint32_t SUBST = [] {
+        uint32_t v = 0;
+        for (uint32_t t = 0; t < SL; t++) v |= (t < K ? (t & 3u) : 3u) << (2 * t);
+        return v;
+    }();
+    __shared__ __align__(16) uint8_t s_q[SEED_READS * SEED_ROW];
+    __shared__ __align__(16) uint8_t s_v[SEED_READS * SEED_ROW];
+    __shared__ uint32_t s_off[4 * SEED_CACHE_M * 2 * SEED_READS];
+    __shared__ uint16_t s_cnt[4 * SEED_CACHE_M * 2 * SEED_READS];
+    const uint32_t r0 = blockIdx.x * SEED_READS;
+    const uint32_t nr = seed_stage(reads, stride, nreads, o, s_q, s_v);
+    __syncthreads();
+    const uint32_t lr = threadIdx.x >> 1;
+    if (lr >= nr) return;
+    const int strand = (int)(threadIdx.x & 1u);
+    const uint32_t read = r0 + lr;
+    const uint32_t r = 2 * read + (uint32_t)strand;
+    uint32_t L = lens[read];
+    if ((uint64_t)L + 2 > stride || L > SWM_READ_MAX || L / SL > 16) L = 0;
+    const uint32_t n_long = L / SL;
+
+    /* the read's codes and non-ACGT masks, oriented to the thread's strand */
+    uint32_t cw[16], iv[16];
+    {
+        const uint32_t *rq = reinterpret_cast<const uint32_t *>(s_q + lr * SEED_ROW);
+        const uint32_t *rv = reinterpret_cast<const uint32_t *>(s_v + lr * SEED_ROW);
+#pragma unroll
+        for (int j = 0; j < 16; j++) {
+            cw[j] = rq[j];
+            iv[j] = flag_mask2(rv[j]);
+        }
+    }
+    if (strand) {
+        /* base t of the reverse strand is the complement of base L - 1 - t:
+         * the 512-bit rows reversed field by field, then moved down by
+         * 512 - 2L bits */
+        uint32_t tc[16], ti[16];
+#pragma unroll
+        for (int j = 0; j < 16; j++) {
+            tc[j] = rev_fields(~cw[15 - j]);
+            ti[j] = __builtin_bitreverse32(iv[15 - j]);
+        }
+        const uint32_t sh = 512u - 2u * L;
+        shift_down(tc, sh >> 5);
+        shift_down(ti, sh >> 5);
+#pragma unroll
+        for (int j = 0; j < 16; j++) {
+            cw[j] = __builtin_amdgcn_alignbit(j < 15 ? tc[j + 1] : 0u, tc[j], sh & 31u);
+            iv[j] = __builtin_amdgcn_alignbit(j < 15 ? ti[j + 1] : 0u, ti[j], sh & 31u);
+        }
+    }
+
+    /* plain packing (non-ACGT as 1, Q15; zeros past L), bit planes, the
+     * length's store recipe (Q3) */
+    const swm_recipe_t *rec = recipes + L;
+    const uint32_t n_planes = rec->n_planes;
+    uint32_t pw[18];
+#pragma unroll
+    for (int j = 0; j < 16; j++) {
+        const int rem = (int)L - 16 * j;
+        const uint32_t lm = rem >= 16 ? 0xffffffffu : (rem <= 0 ? 0u : (1u << (2 * rem)) - 1u);
+        pw[j] = ((cw[j] & ~iv[j]) | (0x55555555u & iv[j])) & lm;
+    }
+    pw[16] = pw[17] = 0;
+    uint32_t *wo = o.words + (uint64_t)r * 32;
+#pragma unroll
+    for (int j = 0; j < 16; j++)
+        wo[j] = (pw[j] & rec->mask[j][0]) | (pw[j + 1] & rec->mask[j][1]) | (pw[j + 2] & rec->mask[j][2]);
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+        const int rem = (int)n_planes - 32 * j;
+        const uint32_t pm = rem >= 32 ? 0xffffffffu : (rem <= 0 ? 0u : (1u << rem) - 1u);
+        const uint32_t lo = pw[2 * j], hi = pw[2 * j + 1];
+        wo[16 + 2 * j] = (even_bits(lo) | (even_bits(hi) << 16)) & pm;
+        wo[17 + 2 * j] = (even_bits(lo >> 1) | (even_bits(hi >> 1) << 16)) & pm;
+    }
+
+    /* per long seed m: its seed string (the 15 bases as the codes' source,
+     * get_seed_from_reads' non-ACGT values) and its residual bits (the
+     * bases outside the 12-mer at each shift, set_read_long_seeds
+     * :2467-2496, non-ACGT as 1) */
+    uint32_t sstr[16], res[16];
+#pragma unroll
+    for (int m = 0; m < 16; m++) {
+        const uint32_t p2 = 2u * SL * (uint32_t)m;          /* bit offset of the long seed */
+        const uint32_t wi = p2 >> 5, bs = p2 & 31u;
+        uint32_t raw = 0, inv = 0;
+        if (wi < 16) {
+            raw = __builtin_amdgcn_alignbit(wi + 1 < 16 ? cw[wi + 1] : 0u, cw[wi], bs) & SMASK;
+            inv = __builtin_amdgcn_alignbit(wi + 1 < 16 ? iv[wi + 1] : 0u, iv[wi], bs) & SMASK;
+        }
+        sstr[m] = (raw & ~inv) | (SUBST & inv);
+        const uint32_t code1 = (raw & ~inv) | (0x55555555u & inv);   /* as read_base(...).code */
+        const uint32_t plo = even_bits(code1), phi = even_bits(code1 >> 1);
+        uint32_t rr = 0;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const uint32_t keep = (1u << q) - 1u;           /* bases u = bb < q, then K + bb */
+            const uint32_t rl = (plo & keep) | ((plo >> K) & 7u & ~keep);
+            const uint32_t rh = (phi & keep) | ((phi >> K) & 7u & ~keep);
+            rr |= (rl | (rh << 3)) << (6 * q);
+        }
+        res[m] = rr;
+    }
+    /* the reverse thread walks its long seeds in mirrored order (see k_seed):
+     * walk[m] = sstr[n_long - 1 - m] */
+    uint32_t walk[16];
+#pragma unroll
+    for (int m = 0; m < 16; m++) walk[m] = sstr[m];
+    if (strand) {
+#pragma unroll
+        for (int m = 0; m < 16; m++) walk[m] = sstr[15 - m];
+        shift_down(walk, 16u - n_long);
+    }
+
+    uint32_t tot[16], pk[16];
+#pragma unroll
+    for (int m = 0; m < 16; m++) {
+        tot[m] = 0;
+        pk[m] = (uint32_t)m | (res[m] << 4);                /* seed index | residual bits */
+    }
+#pragma unroll
+    for (int m0 = 0; m0 < 16; m0 += SEED_CHUNK) {
+        if ((uint32_t)m0 < n_long) {
+            uint2 a[SEED_CHUNK][4];
+#pragma unroll
+            for (int u = 0; u < SEED_CHUNK; u++) {
+                const int m = m0 + u;
+#pragma unroll
+                for (int q = 0; q < 4; q++) a[u][q] = make_uint2(0u, 0u);
+                if (m < 16 && (uint32_t)m < n_long) {
+#pragma unroll
+                    for (int q = 0; q < 4; q++) {
+                        const int j = strand ? 3 - q : q;
+                        const uint32_t c = (walk[m] >> (2 * j)) & KMASK;
+                        a[u][q] = seed_tab[seed_slot(c, K, (uint32_t)j & 1u)];
+                    }
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < SEED_CHUNK; u++) {
+                const int m = m0 + u;
+                if (m < 16 && (uint32_t)m < n_long) {
+                    const uint32_t mm = strand ? n_long - 1u - (uint32_t)m : (uint32_t)m;
+#pragma unroll
+                    for (int q = 0; q < 4; q++) {
+                        const uint32_t c = a[u][q].y;
+                        const uint32_t qq = strand ? 3u - (uint32_t)q : (uint32_t)q;
+                        tot[m] += c;
+                        if (mm < SEED_CACHE_M) {
+                            s_off[(4 * mm + qq) * (2 * SEED_READS) + threadIdx.x] = a[u][q].x;
+                            s_cnt[(4 * mm + qq) * (2 * SEED_READS) + threadIdx.x] = (uint16_t)min(c, 0xffffu);
+                        }
+                    }
+                }
+            }
+        }
+    }
+    if (strand) {
+        uint32_t rv[16];
+#pragma unroll
+        for (int j = 0; j < 16; j++) rv[j] = tot[15 - j];
+        shift_down(rv, 16u - n_long);
+#pragma unroll
+        for (int j = 0; j < 16; j++) tot[j] = rv[j];
+    }
+    /* exchange sort of the long seeds by total hits (:1391-1399, Q4), the
+     * seed strings and residual bits carried along */
+#pragma unroll
+    for (int j = 0; j < 16; j++) {
+#pragma unroll
+        for (int q = j + 1; q < 16; q++) {
+            if ((uint32_t)q < n_long && tot[j] > tot[q]) {
+                uint32_t t0 = tot[j]; tot[j] = tot[q]; tot[q] = t0;
+                uint32_t t1 = pk[j]; pk[j] = pk[q]; pk[q] = t1;
+                uint32_t t2 = sstr[j]; sstr[j] = sstr[q]; sstr[q] = t2;
+            }
+        }
+    }
+    const uint32_t n_sel_long = (e + 1 < n_long) ? e + 1 : n_long;
+    uint64_t npos = 0;
+    uint32_t nchunk = 0;
+    bool heavy = false, irr = false;
+    swm_sel_t *so = o.sel + (uint64_t)r * seln;
+#pragma unroll
+    for (int jm = 0; jm < 16; jm++) {
+        if ((uint32_t)jm < n_sel_long) {
+            const uint32_t m = pk[jm] & 15u;
+            const uint32_t base = m * SL;
+            uint32_t off4[4], cnt4[4];
+            bool again = m >= SEED_CACHE_M;
+            if (!again) {
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    off4[q] = s_off[(4 * m + q) * (2 * SEED_READS) + threadIdx.x];
+                    cnt4[q] = s_cnt[(4 * m + q) * (2 * SEED_READS) + threadIdx.x];
+                    again |= cnt4[q] == 0xffffu;
+                }
+            }
+            if (again) {
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const uint2 v = seed_tab[seed_slot((sstr[jm] >> (2 * q)) & KMASK, K, (uint32_t)q & 1u)];
+                    off4[q] = v.x;
+                    cnt4[q] = v.y;
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const uint32_t rb = (pk[jm] >> (4 + 6 * q)) & 63u;
+                swm_sel_t sl;
+                sl.off = off4[q] & o.off_mask;
+                sl.cnt = cnt4[q];
+                sl.meta = (base + (uint32_t)q) | ((rb & 7u) << 16) | ((rb >> 3) << 19) | ((uint32_t)q << 22);
+                so[4 * jm + q] = sl;
+                heavy |= sl.cnt > SWM_CHUNK;
+                if (sl.cnt > 1) {
+                    irr |= (off4[q] & ~o.off_mask) != 0u;
+                    if (o.n_irreg) irr |= bucket_irregular(o.irreg, o.n_irreg, sl.off);
+                }
+                npos += sl.cnt;
+                nchunk += (sl.cnt + SWM_CHUNK - 1) / SWM_CHUNK;
+            }
+        }
+    }
+    o.nsel[r] = n_sel_long * SWM_SEED_STEP | (heavy ? NSEL_HEAVY : 0u) | (irr ? NSEL_IRREG : 0u);
+    o.npos[r] = npos;
+    o.nchunk[r] = nchunk + 1;
 }
 
 /* ------------------------------------------------------------ plan kernels */
@@ -3640,8 +3947,12 @@ static int enq_a(const swm_ctx_t *ctx, Pipe &P, const char *d_reads, uint64_t st
     P.so.irreg = ctx->d_irreg;
     P.so.n_irreg = ctx->irr_flags ? 0u : ctx->n_irreg;
     P.so.off_mask = ctx->irr_flags ? 0x7fffffffu : 0xffffffffu;
-    k_seed<<<blocks_for(P.n, SEED_READS), 2 * SEED_READS, 0, st>>>(d_reads, stride, d_lens, (uint32_t)P.n, ctx->k, e,
-                                                                   P.seln, ctx->d_seed_tab, ctx->d_recipe, P.so);
+    if (ctx->k == 12 && !ctx->opt.seed_generic)
+        k_seed_fixed<12><<<blocks_for(P.n, SEED_READS), 2 * SEED_READS, 0, st>>>(
+            d_reads, stride, d_lens, (uint32_t)P.n, e, P.seln, ctx->d_seed_tab, ctx->d_recipe, P.so);
+    else
+        k_seed<<<blocks_for(P.n, SEED_READS), 2 * SEED_READS, 0, st>>>(d_reads, stride, d_lens, (uint32_t)P.n, ctx->k,
+                                                                       e, P.seln, ctx->d_seed_tab, ctx->d_recipe, P.so);
     SWM_HIP_CHECK(hipGetLastError());
     if (ev) SWM_HIP_CHECK(hipEventRecord(ev[1], st));
     ScanJob j{};
@@ -4376,6 +4687,7 @@ extern "C" int swm_ctx_set_option(swm_ctx_t *ctx, int opt, int64_t value)
     case SWM_OPT_MYERS_PACKED: if (value != 0 && value != 1) return SWM_EINVAL; o.myers_packed = (uint32_t)value; break;
     case SWM_OPT_GIANT_POS: if (value < 1 || value > (1 << 24)) return SWM_EINVAL; o.giant_pos = (uint32_t)value; break;
     case SWM_OPT_GIANT_CANDS: if (value < 1 || value > (1 << 24)) return SWM_EINVAL; o.giant_cands = (uint32_t)value; break;
+    case SWM_OPT_SEED_GENERIC: if (value != 0 && value != 1) return SWM_EINVAL; o.seed_generic = (uint32_t)value; break;
     default: return SWM_EINVAL;
     }
     return SWM_OK;
@@ -4396,6 +4708,7 @@ extern "C" int swm_ctx_get_option(const swm_ctx_t *ctx, int opt, int64_t *value)
     case SWM_OPT_MYERS_PACKED: *value = o.myers_packed; break;
     case SWM_OPT_GIANT_POS: *value = o.giant_pos; break;
     case SWM_OPT_GIANT_CANDS: *value = o.giant_cands; break;
+    case SWM_OPT_SEED_GENERIC: *value = o.seed_generic; break;
     default: return SWM_EINVAL;
     }
     return SWM_OK;

@@ -157,6 +157,43 @@ def test_map_genome_ends_and_n():
     _assert_same(got, exp)
 
 
+@pytest.mark.parametrize("kind", [swm.GENOME_UNIFORM, swm.GENOME_TANDEM])
+def test_seed_kernels_agree(kind):
+    """The fixed-k seed kernel (k = 12, the read in registers) and the
+    generic one (any k) give the oracle's results on reads of every length
+    with non-ACGT bytes at random places -- inside a long seed's first 12
+    bases (valued by place) and among its last three (valued 3), on both
+    strands."""
+    g = wl.genome(300_000, seed=71, kind=kind, frac=0.3)
+    r, l, _, _ = wl.reads(g, 4000, len_min=15, len_max=254, err=(0.0, 0.03), seed=72)
+    rng = np.random.default_rng(73)
+    for i in range(0, l.size, 2):
+        for _ in range(int(rng.integers(1, 6))):
+            r[i, int(rng.integers(0, l[i]))] = (ord("N"), ord("n"), ord("R"), ord("a"))[int(rng.integers(0, 4))]
+    _say("seed kernels agree", kind)
+    ix = swm.build_index(g, 12)
+    m = swm.Mapper.from_index(ix)
+    got = m.map(r, l, 5)
+    m.set_option(swm.OPT_SEED_GENERIC, 1)
+    got_generic = m.map(r, l, 5)
+    m.close()
+    exp, _ = orc.OracleRef.from_index(orc.OracleIndex(g, 12)).map(r, l, 5)
+    _assert_same(got, exp)
+    _assert_same(got_generic, exp)
+
+
+@pytest.mark.parametrize("k", [10, 13])
+def test_map_other_seed_lengths(k):
+    """-s other than 12: the generic seed kernel against the oracle."""
+    g = wl.genome(200_000, seed=80 + k, kind=swm.GENOME_REPEAT_FAMILIES, frac=0.2)
+    # at most 16 long seeds of k + 3 bases per read (the reference's tables, Q24)
+    r, l, _, _ = wl.reads(g, 2000, len_min=40, len_max=min(250, 16 * (k + 3)), err=(0.0, 0.03), seed=81 + k)
+    _say("map seed length", k)
+    got, exp, st, nv = _map_both(g, r, l, 5, k=k)
+    _assert_same(got, exp)
+    assert st["n_verifications"] == nv
+
+
 def test_myers_stage_matches_oracle():
     rng = np.random.default_rng(3)
     g = wl.genome(20_000, seed=61)
