@@ -1201,98 +1201,138 @@ __global__ void __launch_bounds__(64) k_plan_irreg(const uint2 *__restrict__ pl,
     __shared__ uint32_t s_ind[2 * SWM_MAX_BLOCKS + 4];
     __shared__ uint32_t s_hp[SWM_MAX_BLOCKS + 2], s_hb[SWM_MAX_BLOCKS + 2];
     __shared__ uint32_t s_loc[68], s_sid[68], s_pc[64];
-    if (*io.ovf || threadIdx.x != 0) return;
+    __shared__ uint2 s_stage[64 * SWM_CHUNK];      /* each 12-mer's first chunk, staged by the wave */
+    if (*io.ovf) return;
+    const uint32_t lane = threadIdx.x;
     const uint32_t nq = *io.irreg_count;
     for (uint32_t qi = blockIdx.x; qi < nq; qi += gridDim.x) {
         const uint32_t r = io.irreg[qi];
         const uint32_t nsel = io.nsel[r] & ~NSEL_FLAGS;
         const swm_sel_t *sel = io.sel + (uint64_t)r * io.seln;
-        uint32_t *surv = io.surv + io.surv_off[r];
-        uint2 *bd = io.bdesc + io.bd_off[r];
-        uint32_t hsize = 0, nsurv = 0, batch = 0, pos_size = 0, blocks = 0, nb = 0;
-        bool first_pending = true;                          /* Q7 */
-        auto take = [&](uint32_t g, uint32_t len, uint32_t meta) {
-            const uint32_t b0 = nb;
-            uint2 v[SWM_CHUNK];
+        /* the first chunk of every selected 12-mer (all of phase 1's
+         * fetches, and the heap keys of the longer ones) fetched by the whole
+         * wave, eight 12-mers' loads in flight at once: lane 0 then replays
+         * the state machine from LDS, where one fetch per chunk had put a
+         * memory round trip on its serial path for each 12-mer */
+        {
+            const swm_sel_t mine = lane < nsel ? sel[lane] : swm_sel_t{0, 0, 0};
+            for (uint32_t i0 = 0; i0 < nsel; i0 += 8) {
+                uint2 v[8];
 #pragma unroll
-            for (int j = 0; j < SWM_CHUNK; j++) v[j] = pl[(uint32_t)j < len ? g + (uint32_t)j : g];   /* all in flight */
+                for (uint32_t u = 0; u < 8; u++) {
+                    const uint32_t i = i0 + u;
+                    v[u] = make_uint2(0u, 0u);
+                    if (i < nsel) {
+                        const uint32_t cnt = (uint32_t)__builtin_amdgcn_readlane((int)mine.cnt, (int)i), off = (uint32_t)__builtin_amdgcn_readlane((int)mine.off, (int)i);
+                        if (lane < min(cnt, (uint32_t)SWM_CHUNK)) v[u] = pl[off + lane];
+                    }
+                }
 #pragma unroll
-            for (int j = 0; j < SWM_CHUNK; j++) {
-                if ((uint32_t)j < len) {
-                    if (first_pending) first_pending = false;
-                    else if (filter_pass(v[j].y, meta)) s_pos[nb++] = adjust(v[j].x, meta);
+                for (uint32_t u = 0; u < 8; u++) {
+                    const uint32_t i = i0 + u;
+                    if (i < nsel && lane < min((uint32_t)__builtin_amdgcn_readlane((int)mine.cnt, (int)i), (uint32_t)SWM_CHUNK)) s_stage[i * SWM_CHUNK + lane] = v[u];
                 }
             }
-            s_ind[2 * blocks] = b0;
-            s_ind[2 * blocks + 1] = nb;
-            blocks++;
-        };
-        auto heap_insert = [&](uint32_t loc, uint32_t ind) {    /* :1919-1935 */
-            hsize++;
-            s_loc[hsize] = loc; s_sid[hsize] = ind;
-            uint32_t i = hsize;
-            while (i > 1) {
-                const uint32_t ti = i / 2;
-                if (s_loc[ti] > s_loc[i]) {
-                    uint32_t t = s_loc[ti]; s_loc[ti] = s_loc[i]; s_loc[i] = t;
-                    t = s_sid[ti]; s_sid[ti] = s_sid[i]; s_sid[i] = t;
-                } else break;
-                i = ti;
-            }
-        };
-        auto heap_pop_insert = [&](uint32_t loc, bool del) {    /* :1936-1962 (Q6) */
-            if (del) { s_loc[1] = s_loc[hsize]; s_sid[1] = s_sid[hsize]; hsize--; }
-            else s_loc[1] = loc;
-            uint32_t i = 1;
-            while (i * 2 < hsize) {
-                uint32_t ti = i * 2;
-                if (i * 2 + 1 <= hsize && s_loc[i * 2] > s_loc[i * 2 + 1]) ti = i * 2 + 1;
-                if (s_loc[ti] < s_loc[i]) {
-                    uint32_t t = s_loc[ti]; s_loc[ti] = s_loc[i]; s_loc[i] = t;
-                    t = s_sid[ti]; s_sid[ti] = s_sid[i]; s_sid[i] = t;
-                } else break;
-                i = ti;
-            }
-        };
-        for (uint32_t i = 0; i < nsel; i++) {                   /* phase 1 (:2194-2236) */
-            const swm_sel_t sv = sel[i];
-            const uint32_t get = sv.cnt > SWM_CHUNK ? SWM_CHUNK : sv.cnt;
-            if (get > 0) take(sv.off, get, sv.meta);
-            pos_size += get;
-            s_pc[i] = get;
         }
-        for (uint32_t i = 0; i < nsel; i++) {                   /* :2244-2252 */
-            const swm_sel_t sv = sel[i];
-            if (s_pc[i] != sv.cnt) heap_insert(pl[sv.off + s_pc[i] - 1].x, i);
-        }
-        for (;;) {                                              /* :2261-2419 */
-            if (hsize > 0 && pos_size + SWM_CHUNK <= SWM_BATCH_POS && blocks < SWM_MAX_BLOCKS) {
-                const uint32_t i = s_sid[1];
+        __syncthreads();
+        if (lane == 0) {
+            uint32_t *surv = io.surv + io.surv_off[r];
+            uint2 *bd = io.bdesc + io.bd_off[r];
+            uint32_t hsize = 0, nsurv = 0, batch = 0, pos_size = 0, blocks = 0, nb = 0;
+            bool first_pending = true;                          /* Q7 */
+            auto take = [&](uint32_t g, uint32_t len, uint32_t meta) {
+                const uint32_t b0 = nb;
+                uint2 v[SWM_CHUNK];
+#pragma unroll
+                for (int j = 0; j < SWM_CHUNK; j++) v[j] = pl[(uint32_t)j < len ? g + (uint32_t)j : g];   /* all in flight */
+#pragma unroll
+                for (int j = 0; j < SWM_CHUNK; j++) {
+                    if ((uint32_t)j < len) {
+                        if (first_pending) first_pending = false;
+                        else if (filter_pass(v[j].y, meta)) s_pos[nb++] = adjust(v[j].x, meta);
+                    }
+                }
+                s_ind[2 * blocks] = b0;
+                s_ind[2 * blocks + 1] = nb;
+                blocks++;
+            };
+            auto heap_insert = [&](uint32_t loc, uint32_t ind) {    /* :1919-1935 */
+                hsize++;
+                s_loc[hsize] = loc; s_sid[hsize] = ind;
+                uint32_t i = hsize;
+                while (i > 1) {
+                    const uint32_t ti = i / 2;
+                    if (s_loc[ti] > s_loc[i]) {
+                        uint32_t t = s_loc[ti]; s_loc[ti] = s_loc[i]; s_loc[i] = t;
+                        t = s_sid[ti]; s_sid[ti] = s_sid[i]; s_sid[i] = t;
+                    } else break;
+                    i = ti;
+                }
+            };
+            auto heap_pop_insert = [&](uint32_t loc, bool del) {    /* :1936-1962 (Q6) */
+                if (del) { s_loc[1] = s_loc[hsize]; s_sid[1] = s_sid[hsize]; hsize--; }
+                else s_loc[1] = loc;
+                uint32_t i = 1;
+                while (i * 2 < hsize) {
+                    uint32_t ti = i * 2;
+                    if (i * 2 + 1 <= hsize && s_loc[i * 2] > s_loc[i * 2 + 1]) ti = i * 2 + 1;
+                    if (s_loc[ti] < s_loc[i]) {
+                        uint32_t t = s_loc[ti]; s_loc[ti] = s_loc[i]; s_loc[i] = t;
+                        t = s_sid[ti]; s_sid[ti] = s_sid[i]; s_sid[i] = t;
+                    } else break;
+                    i = ti;
+                }
+            };
+            for (uint32_t i = 0; i < nsel; i++) {                   /* phase 1 (:2194-2236), from LDS */
                 const swm_sel_t sv = sel[i];
-                const uint32_t rem = sv.cnt - s_pc[i];
-                const uint32_t get = rem > SWM_CHUNK ? SWM_CHUNK : rem;
-                take(sv.off + s_pc[i], get, sv.meta);
+                const uint32_t get = sv.cnt > SWM_CHUNK ? SWM_CHUNK : sv.cnt;
+                if (get > 0) {
+                    const uint32_t b0 = nb;
+                    for (uint32_t j = 0; j < get; j++) {
+                        const uint2 v = s_stage[i * SWM_CHUNK + j];
+                        if (first_pending) first_pending = false;
+                        else if (filter_pass(v.y, sv.meta)) s_pos[nb++] = adjust(v.x, sv.meta);
+                    }
+                    s_ind[2 * blocks] = b0;
+                    s_ind[2 * blocks + 1] = nb;
+                    blocks++;
+                }
                 pos_size += get;
-                s_pc[i] += get;
-                heap_pop_insert(pl[sv.off + s_pc[i] - 1].x, s_pc[i] == sv.cnt);
+                s_pc[i] = get;
             }
-            if (pos_size + SWM_CHUNK > SWM_BATCH_POS || hsize == 0 || blocks >= SWM_MAX_BLOCKS) {
-                const uint32_t u = block_merge(s_pos, s_ind, blocks, s_hp, s_hb, surv + nsurv);
-                bd[batch] = make_uint2(nsurv, u);
-                nsurv += u;
-                batch++;
-                pos_size = 0;
-                blocks = 0;
-                nb = 0;
-                first_pending = true;
-                if (hsize == 0) break;
+            for (uint32_t i = 0; i < nsel; i++) {                   /* :2244-2252 */
+                const swm_sel_t sv = sel[i];
+                if (s_pc[i] != sv.cnt) heap_insert(s_stage[i * SWM_CHUNK + s_pc[i] - 1].x, i);
             }
+            for (;;) {                                              /* :2261-2419 */
+                if (hsize > 0 && pos_size + SWM_CHUNK <= SWM_BATCH_POS && blocks < SWM_MAX_BLOCKS) {
+                    const uint32_t i = s_sid[1];
+                    const swm_sel_t sv = sel[i];
+                    const uint32_t rem = sv.cnt - s_pc[i];
+                    const uint32_t get = rem > SWM_CHUNK ? SWM_CHUNK : rem;
+                    take(sv.off + s_pc[i], get, sv.meta);
+                    pos_size += get;
+                    s_pc[i] += get;
+                    heap_pop_insert(pl[sv.off + s_pc[i] - 1].x, s_pc[i] == sv.cnt);
+                }
+                if (pos_size + SWM_CHUNK > SWM_BATCH_POS || hsize == 0 || blocks >= SWM_MAX_BLOCKS) {
+                    const uint32_t u = block_merge(s_pos, s_ind, blocks, s_hp, s_hb, surv + nsurv);
+                    bd[batch] = make_uint2(nsurv, u);
+                    nsurv += u;
+                    batch++;
+                    pos_size = 0;
+                    blocks = 0;
+                    nb = 0;
+                    first_pending = true;
+                    if (hsize == 0) break;
+                }
+            }
+            io.nbatch[r] = batch;
+            io.nuniq[r] = nsurv;
         }
-        io.nbatch[r] = batch;
-        io.nuniq[r] = nsurv;
+        __syncthreads();
     }
 }
-
 /* Lane reads (v_readlane) and writes (a compare + select) at a wave-uniform
  * lane index.  The seed heap and the per-seed state of k_plan_heavy live one
  * entry per lane, so its replay is scalar code with no LDS round trip per
@@ -1326,6 +1366,8 @@ __device__ __forceinline__ uint32_t sel_gt(uint32_t a, uint32_t b, uint32_t x, u
  * the 15-mer filter and read-start adjustment in entry order with the first
  * entry dropped (Q7), then the sorted unique list of the one batch (<= 64
  * survivors sorted in the wave by rank, larger ones left to k_bigsort). */
+#define PW_ROUNDS 4
+
 __global__ void __launch_bounds__(64) k_plan_wave(uint32_t nrs, const uint2 *__restrict__ pl, PlanIO io)
 {
     __shared__ uint32_t s_off[64], s_pre[64], s_meta[64], s_surv[64];
@@ -1361,27 +1403,45 @@ __global__ void __launch_bounds__(64) k_plan_wave(uint32_t nrs, const uint2 *__r
             __syncthreads();
             uint32_t *surv = io.surv + io.surv_off[r];
             uint32_t nsurv = 0;
-            for (uint32_t e0 = 0; e0 < total; e0 += 64u) {
-                const uint32_t ent = e0 + lane;
-                const bool in = ent < total;
-                /* the 12-mer of the entry: the last k with s_pre[k] <= ent
-                 * (a 12-mer with no entries shares its start with the next) */
-                uint32_t lo = 0, hi = 64;
-                while (hi - lo > 1) {
-                    const uint32_t mid = (lo + hi) >> 1;
-                    if (s_pre[mid] <= ent) lo = mid; else hi = mid;
+            /* PW_ROUNDS entries per lane at once: an item's bucket entries
+             * (about 100 on a uniform genome) come back in one round trip */
+            for (uint32_t e0 = 0; e0 < total; e0 += 64u * PW_ROUNDS) {
+                uint32_t lo[PW_ROUNDS], meta[PW_ROUNDS];
+                uint2 v[PW_ROUNDS];
+                /* the 12-mer of each entry: the last k with s_pre[k] <= ent
+                 * (a 12-mer with no entries shares its start with the next);
+                 * the rounds' searches interleaved */
+#pragma unroll
+                for (int j = 0; j < PW_ROUNDS; j++) lo[j] = 0;
+#pragma unroll
+                for (uint32_t half = 32; half >= 1; half >>= 1) {
+#pragma unroll
+                    for (int j = 0; j < PW_ROUNDS; j++) {
+                        const uint32_t ent = e0 + 64u * (uint32_t)j + lane;
+                        if (s_pre[lo[j] + half] <= ent) lo[j] += half;
+                    }
                 }
-                const uint32_t meta = s_meta[lo];
-                const uint2 v = pl[in ? s_off[lo] + (ent - s_pre[lo]) : s_off[0]];
-                const bool keep = in && ent > 0 && filter_pass(v.y, meta);
-                const uint64_t m = __ballot(keep);
-                if (keep) {
-                    const uint32_t at = nsurv + (uint32_t)__popcll(m & below);
-                    const uint32_t val = adjust(v.x, meta);
-                    surv[at] = val;
-                    if (at < 64u) s_surv[at] = val;
+#pragma unroll
+                for (int j = 0; j < PW_ROUNDS; j++) {
+                    const uint32_t ent = e0 + 64u * (uint32_t)j + lane;
+                    meta[j] = s_meta[lo[j]];
+                    v[j] = make_uint2(0u, 0u);
+                    if (e0 + 64u * (uint32_t)j < total)       /* wave-uniform */
+                        v[j] = pl[ent < total ? s_off[lo[j]] + (ent - s_pre[lo[j]]) : s_off[0]];
                 }
-                nsurv += (uint32_t)__popcll(m);
+#pragma unroll
+                for (int j = 0; j < PW_ROUNDS; j++) {
+                    const uint32_t ent = e0 + 64u * (uint32_t)j + lane;
+                    const bool keep = ent < total && ent > 0 && filter_pass(v[j].y, meta[j]);
+                    const uint64_t m = __ballot(keep);
+                    if (keep) {
+                        const uint32_t at = nsurv + (uint32_t)__popcll(m & below);
+                        const uint32_t val = adjust(v[j].x, meta[j]);
+                        surv[at] = val;
+                        if (at < 64u) s_surv[at] = val;
+                    }
+                    nsurv += (uint32_t)__popcll(m);
+                }
             }
             __syncthreads();
             uint2 *bd = io.bdesc + io.bd_off[r];
