@@ -970,6 +970,7 @@ struct PlanIO {
     uint32_t *irreg_count;
     uint32_t *merge;            /* heavy batches with a descending chunk (hbatch indices), for k_merge */
     uint32_t *merge_count;
+    unsigned long long *light_pos;  /* bucket entries fetched by k_plan_wave */
 };
 
 /* Bit 31 of a device bucket entry's second word: its position is smaller
@@ -1378,6 +1379,7 @@ __global__ void __launch_bounds__(64) k_plan_wave(uint32_t nrs, const uint2 *__r
      * one is processed (its loads would otherwise head a dependent chain) */
     uint32_t r = blockIdx.x;
     uint32_t nsw_n = 0;
+    uint64_t fetched = 0;                       /* bucket entries this wave fetched */
     swm_sel_t my_n = {0, 0, 0};
     if (r < nrs) {
         nsw_n = io.nsel[r];
@@ -1397,6 +1399,7 @@ __global__ void __launch_bounds__(64) k_plan_wave(uint32_t nrs, const uint2 *__r
             if (lane >= nsel) my = swm_sel_t{0, 0, 0};       /* slots past nsel are not written */
             const uint32_t incl = wave_incl_scan_dpp(my.cnt);
             const uint32_t total = rl(incl, 63);
+            fetched += total;
             s_off[lane] = my.off;
             s_pre[lane] = incl - my.cnt;        /* lanes >= nsel: total */
             s_meta[lane] = my.meta;
@@ -1473,6 +1476,7 @@ __global__ void __launch_bounds__(64) k_plan_wave(uint32_t nrs, const uint2 *__r
             __syncthreads();
         }
     }
+    if (lane == 0 && fetched) atomicAdd(io.light_pos, (unsigned long long)fetched);
 }
 
 /* Heavy (read, strand): a device work queue drained by persistent
@@ -3794,7 +3798,7 @@ struct swm_slot {
     };
     GraphEnt g[SLOT_GRAPHS];
     uint64_t tick = 0;
-    hipEvent_t ev[6] = {};
+    hipEvent_t ev[8] = {};      /* stage bounds [0..5]; k_plan_wave [6], [7] */
     /* page-locked staging, so that no copy goes through the runtime's shared
      * pageable-memory path (which serialises the slots): totals read back
      * between stages, counters, and host batches in and out */
@@ -3956,6 +3960,7 @@ static int pipe_alloc_a(Alloc *ws, uint64_t n, uint32_t seln, Pipe &P)
     P.pio.giant_count = P.u32c + 9;
     P.pio.irreg_count = P.u32c + 12;
     P.pio.merge_count = P.u32c + 13;
+    P.pio.light_pos = P.counters + 11;
     P.pio.ovf = P.u32c + 7;
     P.cs.ovf = P.u32c + 7;
     P.cs.perm_id = P.u32c + 6;                    /* 8-byte aligned, the overflow flag next */
@@ -4044,8 +4049,10 @@ static int enq_b(const swm_ctx_t *ctx, Pipe &P, uint64_t cap_cand, hipStream_t s
 {
     const swm_opts &o = ctx->opt;
     if (o.work_queue) {
+        if (ev) SWM_HIP_CHECK(hipEventRecord(ev[6], st));
         k_plan_wave<<<(uint32_t)std::min<uint64_t>(P.nrs ? P.nrs : 1, (uint64_t)ctx->n_cu * 32), 64, 0, st>>>(
             (uint32_t)P.nrs, ctx->d_pl, P.pio);
+        if (ev) SWM_HIP_CHECK(hipEventRecord(ev[7], st));
         k_plan_heavy<<<ctx->n_cu * o.plan_heavy_waves, 64, 0, st>>>(ctx->d_pl, P.pio);
         k_plan_fetch<<<ctx->n_cu * 32, 64, 0, st>>>(ctx->d_pl, P.pio);
     } else {
@@ -4143,6 +4150,7 @@ static void stats_from_counters(swm_stats_t *s, uint64_t n, const unsigned long 
     s->n_giant_items = u2[1];
     s->n_giant_reads = u2[2];
     s->n_irregular_items = reinterpret_cast<const uint32_t *>(hc + 10)[0];
+    s->n_light_positions = hc[11];
 }
 
 /* The per-read candidate histogram of the run just finished (stats). */
@@ -4230,6 +4238,8 @@ static int run_synced(swm_ctx_t *ctx, swm_slot *sl, const char *d_reads, uint64_
     (void)hipEventElapsedTime(&ms, sl->ev[3], sl->ev[4]); sl->stats.ms_myers = ms;
     (void)hipEventElapsedTime(&ms, sl->ev[4], sl->ev[5]); sl->stats.ms_emit = ms;
     (void)hipEventElapsedTime(&ms, sl->ev[0], sl->ev[5]); sl->stats.ms_total = ms;
+    sl->stats.ms_plan_wave = 0;
+    if (ctx->opt.work_queue) { (void)hipEventElapsedTime(&ms, sl->ev[6], sl->ev[7]); sl->stats.ms_plan_wave = ms; }
     return SWM_OK;
 }
 
