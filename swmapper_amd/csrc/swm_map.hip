@@ -343,16 +343,29 @@ __host__ __device__ __forceinline__ uint64_t seed_table_entries(uint32_t k)
     return ((k - 1u) & 1u ? 8ull : 16ull) << (2u * (k - 1u));
 }
 
-/* both entries of k-mer c from the offset array (off_mask: the offset bits) */
-__global__ void __launch_bounds__(256) k_seed_table(const uint32_t *__restrict__ off, uint64_t n_buckets, uint32_t k,
-                                                    uint32_t off_mask, uint2 *__restrict__ tab)
+/* both entries of k-mer c: its bucket's start in the device bucket array
+ * (with the irregular-bucket flag) and its hit count from the file's offsets */
+__global__ void __launch_bounds__(256) k_seed_table(const uint32_t *__restrict__ off, const uint32_t *__restrict__ dev_off,
+                                                    uint64_t n_buckets, uint32_t k, uint2 *__restrict__ tab)
 {
     const uint64_t c = (uint64_t)blockIdx.x * 256u + threadIdx.x;
     if (c >= n_buckets) return;
-    const uint32_t a = off[c], b = off[c + 1];
-    const uint2 v = make_uint2(a, (b & off_mask) - (a & off_mask));
+    const uint2 v = make_uint2(dev_off[c], off[c + 1] - off[c]);
     tab[seed_slot((uint32_t)c, k, 0u)] = v;
     tab[seed_slot((uint32_t)c, k, 1u)] = v;
+}
+
+/* The device bucket array: bucket c copied from the file's order (src at
+ * off[c]) to its place next to its reverse complement's (dst at dev_off[c]),
+ * entries past n_pos on either side left out. */
+__global__ void __launch_bounds__(256) k_pl_permute(const uint32_t *__restrict__ off, const uint32_t *__restrict__ dev_off,
+                                                    uint64_t n_buckets, uint64_t n_pos, const uint2 *__restrict__ src,
+                                                    uint2 *__restrict__ dst)
+{
+    const uint64_t c = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+    if (c >= n_buckets) return;
+    const uint64_t a = off[c], cnt = (uint32_t)(off[c + 1] - off[c]), d = dev_off[c] & 0x7fffffffu;
+    for (uint64_t j = 0; j < cnt && a + j < n_pos && d + j < n_pos; j++) dst[d + j] = src[a + j];
 }
 
 /* Work of every seed kernel before its per-thread part: workgroup 0 zeroes
@@ -1377,7 +1390,11 @@ __global__ void __launch_bounds__(64) k_plan_wave(uint32_t nrs, const uint2 *__r
     const uint64_t below = (1ull << lane) - 1ull;
     /* the next item's selection count and selections are loaded while this
      * one is processed (its loads would otherwise head a dependent chain) */
-    uint32_t r = blockIdx.x;
+    /* a wave takes both strands of a read, one after the other: their
+     * selected 12-mers are mostly reverse complements of each other, whose
+     * buckets lie side by side, so the second strand finds the lines the
+     * first one fetched in its XCD's L2 */
+    uint32_t r = 2u * blockIdx.x;
     uint32_t nsw_n = 0;
     uint64_t fetched = 0;                       /* bucket entries this wave fetched */
     swm_sel_t my_n = {0, 0, 0};
@@ -1385,10 +1402,10 @@ __global__ void __launch_bounds__(64) k_plan_wave(uint32_t nrs, const uint2 *__r
         nsw_n = io.nsel[r];
         if (lane < io.seln) my_n = io.sel[(uint64_t)r * io.seln + lane];
     }
-    for (; r < nrs; r += gridDim.x) {
+    for (; r < nrs; r = (r & 1u) ? r + 2u * gridDim.x - 1u : r + 1u) {
         const uint32_t nsw = nsw_n;
         swm_sel_t my = my_n;
-        const uint32_t rn = r + gridDim.x;
+        const uint32_t rn = (r & 1u) ? r + 2u * gridDim.x - 1u : r + 1u;
         if (rn < nrs) {
             nsw_n = io.nsel[rn];
             if (lane < io.seln) my_n = io.sel[(uint64_t)rn * io.seln + lane];
@@ -4050,7 +4067,7 @@ static int enq_b(const swm_ctx_t *ctx, Pipe &P, uint64_t cap_cand, hipStream_t s
     const swm_opts &o = ctx->opt;
     if (o.work_queue) {
         if (ev) SWM_HIP_CHECK(hipEventRecord(ev[6], st));
-        k_plan_wave<<<(uint32_t)std::min<uint64_t>(P.nrs ? P.nrs : 1, (uint64_t)ctx->n_cu * 32), 64, 0, st>>>(
+        k_plan_wave<<<(uint32_t)std::min<uint64_t>(P.n ? P.n : 1, (uint64_t)ctx->n_cu * 32), 64, 0, st>>>(
             (uint32_t)P.nrs, ctx->d_pl, P.pio);
         if (ev) SWM_HIP_CHECK(hipEventRecord(ev[7], st));
         k_plan_heavy<<<ctx->n_cu * o.plan_heavy_waves, 64, 0, st>>>(ctx->d_pl, P.pio);
@@ -4679,13 +4696,35 @@ extern "C" int swm_ctx_create(uint32_t k, const uint32_t *hash_count, const uint
     bool below31 = true;
     for (uint64_t i = 0; i <= c->n_buckets && below31; i++) below31 = h_off[i] < 0x80000000u;
     c->irr_flags = below31 && !irr.empty();
+    /* the device keeps each k-mer's bucket next to its reverse complement's:
+     * the two strands of a read select mostly the same long seeds, whose
+     * 12-mers on the reverse strand are the forward ones' reverse
+     * complements, so both fetch the same lines */
+    uint32_t *d_order = (uint32_t *)malloc(c->n_buckets * sizeof(uint32_t));
+    if (!d_order) { free(h_off); free(rec); swm_ctx_destroy(c); return SWM_ENOMEM; }
+    {
+        uint32_t acc = 0;
+        for (uint64_t i = 0; i < c->n_buckets; i++) {
+            const uint32_t rci = rc_bases((uint32_t)i, k);
+            if (rci < i) continue;
+            d_order[i] = acc;
+            acc += hash_count[i];
+            if (rci != i) { d_order[rci] = acc; acc += hash_count[rci]; }
+        }
+    }
+    for (uint32_t &x : irr) x = 0;
+    for (size_t q = 0; q < irr_bucket.size(); q++) irr[q] = d_order[irr_bucket[q]];
+    std::sort(irr.begin(), irr.end());
     if (c->irr_flags)
-        for (uint64_t i : irr_bucket) h_off[i] |= 0x80000000u;
+        for (uint64_t i : irr_bucket) d_order[i] |= 0x80000000u;
     int rc = SWM_OK;
     const size_t rec_bytes = (SWM_RECIPE_MAX_L + 1) * sizeof(swm_recipe_t);
-    uint32_t *d_off = nullptr;
+    uint32_t *d_off = nullptr, *d_dev_off = nullptr;
+    uint2 *d_pl_file = nullptr;
     const uint64_t tab_bytes = seed_table_entries(k) * sizeof(uint2);
     if (hipMalloc((void **)&d_off, (c->n_buckets + 1) * 4) != hipSuccess ||
+        hipMalloc((void **)&d_dev_off, c->n_buckets * 4) != hipSuccess ||
+        hipMalloc((void **)&d_pl_file, (n_pos + 1) * 8) != hipSuccess ||
         hipMalloc((void **)&c->d_seed_tab, tab_bytes) != hipSuccess ||
         hipMalloc((void **)&c->d_pl, (n_pos + 64) * 8) != hipSuccess ||
         hipMalloc((void **)&c->d_base_2bits, (n_2bits_words + 2 * SWM_WIN_PAIRS_PAD) * 4) != hipSuccess ||
@@ -4694,9 +4733,10 @@ extern "C" int swm_ctx_create(uint32_t k, const uint32_t *hash_count, const uint
         rc = SWM_ENOMEM;
     if (rc == SWM_OK) {
         if (hipMemcpy(d_off, h_off, (c->n_buckets + 1) * 4, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(d_dev_off, d_order, c->n_buckets * 4, hipMemcpyHostToDevice) != hipSuccess ||
             hipMemset(c->d_seed_tab, 0, tab_bytes) != hipSuccess ||
             hipMemset(c->d_pl, 0, (n_pos + 64) * 8) != hipSuccess ||
-            (n_pos && upload_pl(c->d_pl, positions, pos_lseeds, n_pos, descents) != SWM_OK) ||
+            (n_pos && upload_pl(d_pl_file, positions, pos_lseeds, n_pos, descents) != SWM_OK) ||
             hipMemset(c->d_base_2bits, 0, (n_2bits_words + 2 * SWM_WIN_PAIRS_PAD) * 4) != hipSuccess ||
             hipMemcpy(c->d_base_2bits, base_2bits, n_2bits_words * 4, hipMemcpyHostToDevice) != hipSuccess ||
             hipMemcpy(c->d_recipe, rec, rec_bytes, hipMemcpyHostToDevice) != hipSuccess ||
@@ -4705,12 +4745,15 @@ extern "C" int swm_ctx_create(uint32_t k, const uint32_t *hash_count, const uint
         c->n_irreg = (uint32_t)irr.size();
     }
     if (rc == SWM_OK) {
-        k_seed_table<<<(unsigned)((c->n_buckets + 255) / 256), 256>>>(d_off, c->n_buckets, k,
-                                                                      c->irr_flags ? 0x7fffffffu : 0xffffffffu,
-                                                                      c->d_seed_tab);
+        const unsigned nb = (unsigned)((c->n_buckets + 255) / 256);
+        k_seed_table<<<nb, 256>>>(d_off, d_dev_off, c->n_buckets, k, c->d_seed_tab);
+        if (n_pos) k_pl_permute<<<nb, 256>>>(d_off, d_dev_off, c->n_buckets, n_pos, d_pl_file, c->d_pl);
         if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) rc = SWM_EDEVICE;
     }
     if (d_off) (void)hipFree(d_off);
+    if (d_dev_off) (void)hipFree(d_dev_off);
+    if (d_pl_file) (void)hipFree(d_pl_file);
+    free(d_order);
     free(h_off);
     free(rec);
     if (rc != SWM_OK) { swm_ctx_destroy(c); return rc; }
