@@ -3221,6 +3221,7 @@ struct EmitPart {
  * two strands of a read are separate work items -- slot 2h + strand, the
  * giant reads' first -- and a giant read's serial walk is split in two;
  * k_emit_combine then forms the read's result from the two parts. */
+template <bool REC>   /* REC: the records path (c.wr set): also mark written records and block starts */
 __global__ void __launch_bounds__(64) k_emit_heavy(const uint32_t *__restrict__ heavy,
                                                    const uint32_t *__restrict__ heavy_count, uint32_t nreads, uint32_t e,
                                                    const uint32_t *__restrict__ cand_off, CandSet c,
@@ -3260,11 +3261,11 @@ __global__ void __launch_bounds__(64) k_emit_heavy(const uint32_t *__restrict__ 
                     if (g_fkey < best) { best = g_fkey; best_rec = g_frec; }
                     /* the lane that stored this candidate's "not written" flag
                      * overwrites it; the record opens the output block */
-                    if (c.wr && lane == ((g_fci - c0) & 63u)) {
+                    if (REC && lane == ((g_fci - c0) & 63u)) {
                         c.wr[g_fci] = WR_REC | WR_BLOCK;
                         c.tb[g_fci] = g_ftext - 2u;
                     }
-                } else if (num == 0 && g_wseen && c.wr && lane == ((g_wci - c0) & 63u)) {
+                } else if (REC && num == 0 && g_wseen && lane == ((g_wci - c0) & 63u)) {
                     /* the group's first plainly written record opens the block */
                     c.wr[g_wci] = WR_REC | WR_BLOCK;
                 }
@@ -3323,7 +3324,7 @@ __global__ void __launch_bounds__(64) k_emit_heavy(const uint32_t *__restrict__ 
                 const uint64_t gm = __ballot(valid && (v & 7u) == 0);
                 /* plainly written lanes */
                 const bool wr = pass && !dup;
-                if (c.wr && in) {
+                if (REC && in) {
                     c.wr[ci] = wr ? WR_REC : 0ull;
                     c.tb[ci] = wr ? (info & 0xffffu) - 2u : 0u;
                 }
@@ -3356,10 +3357,12 @@ __global__ void __launch_bounds__(64) k_emit_heavy(const uint32_t *__restrict__ 
                 const uint32_t sf = spr ? (uint32_t)__ffsll((unsigned long long)spr) - 1u : 0u;
                 const uint64_t dm = __ballot(dup);
                 const uint64_t wsm = am & srm;    /* the segment's plainly written lanes */
-                const uint32_t sw = wsm ? (uint32_t)__ffsll((unsigned long long)wsm) - 1u : 0u;
-                const uint32_t s_pack = (uint32_t)__popcll(wsm) | (spr ? 0x100u : 0u) |
-                                        ((spr && ((dm >> sf) & 1ull)) ? 0x200u : 0u) |
-                                        (wsm ? 0x400u : 0u) | (sw << 16);
+                uint32_t s_pack = (uint32_t)__popcll(wsm) | (spr ? 0x100u : 0u) |
+                                  ((spr && ((dm >> sf) & 1ull)) ? 0x200u : 0u);
+                if (REC) {
+                    const uint32_t sw = wsm ? (uint32_t)__ffsll((unsigned long long)wsm) - 1u : 0u;
+                    s_pack |= (wsm ? 0x400u : 0u) | (sw << 16);
+                }
                 const uint32_t q_info = (uint32_t)__shfl((int)info, (int)sf, 64);
                 const uint32_t q_rec = (uint32_t)__shfl((int)rec, (int)sf, 64);
                 const uint32_t q_er = (uint32_t)__shfl((int)er, (int)sf, 64);
@@ -3371,7 +3374,7 @@ __global__ void __launch_bounds__(64) k_emit_heavy(const uint32_t *__restrict__ 
                     }
                     if (!g_open) continue;
                     const uint32_t pk = rdlane(s_pack, p);
-                    if (!g_wseen && (pk & 0x400u)) { g_wseen = true; g_wci = base + (pk >> 16); }
+                    if (REC && !g_wseen && (pk & 0x400u)) { g_wseen = true; g_wci = base + (pk >> 16); }
                     g_n += pk & 0xffu;
                     g_len += rdlane(s_len, p);
                     if (!g_seen && (pk & 0x100u)) {
@@ -4142,8 +4145,8 @@ static int enq_c(const swm_ctx_t *ctx, Pipe &P, const uint32_t *d_lens, uint32_t
     k_emit<<<blocks_for(P.n, 128), 128, 0, st>>>((uint32_t)P.n, e, P.cand_off, P.cs, d_out, P.rpart,
                                                  heavy_cands, o.giant_cands, P.heavy_e, P.u32c + 10);
     if (o.work_queue)
-        k_emit_heavy<<<ctx->n_cu * o.emit_heavy_waves, 64, 0, st>>>(P.heavy_e, P.u32c + 10, (uint32_t)P.n, e,
-                                                                   P.cand_off, P.cs, P.epart, P.counters + 1);
+        (P.cs.wr ? k_emit_heavy<true> : k_emit_heavy<false>)<<<ctx->n_cu * o.emit_heavy_waves, 64, 0, st>>>(
+            P.heavy_e, P.u32c + 10, (uint32_t)P.n, e, P.cand_off, P.cs, P.epart, P.counters + 1);
     k_emit_combine<<<ctx->n_cu * 2, 256, 0, st>>>(P.heavy_e, P.u32c + 10, (uint32_t)P.n, P.epart, d_out, P.rpart,
                                                   blocks_for(P.n, 128), P.counters + 1, P.hpart, ham_grid,
                                                   P.counters + 2, P.u32c + 2);
