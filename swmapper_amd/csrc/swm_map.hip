@@ -1470,12 +1470,17 @@ __global__ void __launch_bounds__(64) k_plan_wave(uint32_t nrs, const uint2 *__r
                  * distinct smaller values */
                 const bool have = lane < nsurv;
                 const uint32_t v = have ? s_surv[lane] : 0xffffffffu;
-                bool first = have;
-                for (uint32_t q = 0; q < nsurv; q++) {
-                    const uint32_t w = (uint32_t)__builtin_amdgcn_readlane((int)v, (int)q);
-                    if (w == v && q < lane) first = false;
+                /* the first lane of each distinct value, one pass per value
+                 * (the survivors of a mapped read are mostly copies of its
+                 * true start, one per selected 12-mer) */
+                uint64_t fm = 0;
+                for (uint64_t left = __ballot(have); left;) {
+                    const uint32_t f = (uint32_t)__ffsll((unsigned long long)left) - 1u;
+                    const uint32_t w = (uint32_t)__builtin_amdgcn_readlane((int)v, (int)f);
+                    fm |= 1ull << f;
+                    left &= ~__ballot(have && v == w);
                 }
-                const uint64_t fm = __ballot(first);
+                const bool first = (fm >> lane) & 1ull;
                 uint32_t rank = 0;
                 for (uint64_t f = fm; f; f &= f - 1ull) {
                     const uint32_t q = (uint32_t)__ffsll((unsigned long long)f) - 1u;
