@@ -1380,7 +1380,7 @@ __device__ __forceinline__ uint32_t sel_gt(uint32_t a, uint32_t b, uint32_t x, u
  * the 15-mer filter and read-start adjustment in entry order with the first
  * entry dropped (Q7), then the sorted unique list of the one batch (<= 64
  * survivors sorted in the wave by rank, larger ones left to k_bigsort). */
-#define PW_ROUNDS 4
+#define PW_ROUNDS 2
 
 __global__ void __launch_bounds__(64) k_plan_wave(uint32_t nrs, const uint2 *__restrict__ pl, PlanIO io)
 {
