@@ -441,6 +441,9 @@ __device__ __forceinline__ uint32_t seed_stage(const char *__restrict__ reads, u
  * SIMD issue gathers: the kernel is bound by the rate of random line fetches
  * (one per lookup), and that rate grows with the waves in flight. */
 #define SEED_CACHE_M 10
+#ifndef SEED_FIXED_CACHE_M
+#define SEED_FIXED_CACHE_M 10
+#endif
 #define SEED_CHUNK 8
 
 __global__ void __launch_bounds__(128) k_seed(const char *__restrict__ reads, uint64_t stride,
@@ -725,8 +728,9 @@ __global__ void __launch_bounds__(128) k_seed_fixed(const char *__restrict__ rea
     }();
     __shared__ __align__(16) uint8_t s_q[SEED_READS * SEED_ROW];
     __shared__ __align__(16) uint8_t s_v[SEED_READS * SEED_ROW];
-    __shared__ uint32_t s_off[4 * SEED_CACHE_M * 2 * SEED_READS];
-    __shared__ uint16_t s_cnt[4 * SEED_CACHE_M * 2 * SEED_READS];
+    constexpr uint32_t CM = SEED_FIXED_CACHE_M;
+    __shared__ uint32_t s_off[4 * (CM ? CM : 1) * 2 * SEED_READS];
+    __shared__ uint16_t s_cnt[4 * (CM ? CM : 1) * 2 * SEED_READS];
     const uint32_t r0 = blockIdx.x * SEED_READS;
     const uint32_t nr = seed_stage(reads, stride, nreads, o, s_q, s_v);
     __syncthreads();
@@ -867,7 +871,7 @@ __global__ void __launch_bounds__(128) k_seed_fixed(const char *__restrict__ rea
                         const uint32_t c = a[u][q].y;
                         const uint32_t qq = strand ? 3u - (uint32_t)q : (uint32_t)q;
                         tot[m] += c;
-                        if (mm < SEED_CACHE_M) {
+                        if (mm < CM) {
                             s_off[(4 * mm + qq) * (2 * SEED_READS) + threadIdx.x] = a[u][q].x;
                             s_cnt[(4 * mm + qq) * (2 * SEED_READS) + threadIdx.x] = (uint16_t)min(c, 0xffffu);
                         }
@@ -908,7 +912,7 @@ __global__ void __launch_bounds__(128) k_seed_fixed(const char *__restrict__ rea
             const uint32_t m = pk[jm] & 15u;
             const uint32_t base = m * SL;
             uint32_t off4[4], cnt4[4];
-            bool again = m >= SEED_CACHE_M;
+            bool again = m >= CM;
             if (!again) {
 #pragma unroll
                 for (int q = 0; q < 4; q++) {
