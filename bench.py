@@ -130,9 +130,10 @@ def rooflines(swm, st_all, lens, config, form):
     roof_seed = roof_wave = roof_plan = roof_myers = None
     ms_seed = statistics.mean(s["ms_seed"] for s in st_all)
     if ms_seed > 0:
-        traffic = pmc_traffic(config, "k_seed")
+        traffic = pmc_traffic(config, "k_seed_fixed<12u>") or pmc_traffic(config, "k_seed")
         ach = seed_bytes / (ms_seed / 1e3) / 1e9
-        roof_seed = {"bound": "hbm", "kernel": "k_seed", "form": form, "achieved": round(ach, 1),
+        roof_seed = {"bound": "hbm", "kernel": "k_seed_fixed<12> (k_seed for other k)", "form": form,
+                     "achieved": round(ach, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "traffic_rate": traffic_rate(traffic, ms_seed),
                      "algorithmic_bytes_per_launch": seed_bytes, "avg_launch_ms": round(ms_seed, 3),
