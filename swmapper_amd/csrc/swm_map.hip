@@ -337,6 +337,14 @@ __host__ __device__ __forceinline__ uint64_t seed_slot(uint32_t c, uint32_t k, u
     return (uint64_t)line * 16u + o * 8u + role * 4u + ext;
 }
 
+/* the slots of two consecutive k-mers ca (role 0) and cb (role 1), which
+ * share their (k-1)-mer: one line, its place computed once */
+__host__ __device__ __forceinline__ uint2 seed_pair_slots(uint32_t ca, uint32_t cb, uint32_t k)
+{
+    const uint64_t a = seed_slot(ca, k, 0u);
+    return make_uint2((uint32_t)a, (uint32_t)((a & ~15ull) + (a & 8u) + 4u + (cb >> (2u * (k - 1u)))));
+}
+
 /* entries of the table: 8 per (k-1)-mer when k-1 is odd, 16 when even */
 __host__ __device__ __forceinline__ uint64_t seed_table_entries(uint32_t k)
 {
@@ -853,12 +861,14 @@ __global__ void __launch_bounds__(128) k_seed_fixed(const char *__restrict__ rea
 #pragma unroll
                 for (int q = 0; q < 4; q++) a[u][q] = make_uint2(0u, 0u);
                 if (m < 16 && (uint32_t)m < n_long) {
+                    /* shifts (0, 1) and (2, 3) share their 11-mer and line */
+                    const uint32_t w = walk[m];
+                    const uint2 p01 = seed_pair_slots(w & KMASK, (w >> 2) & KMASK, K);
+                    const uint2 p23 = seed_pair_slots((w >> 4) & KMASK, (w >> 6) & KMASK, K);
+                    const uint32_t sl[4] = {strand ? p23.y : p01.x, strand ? p23.x : p01.y,
+                                            strand ? p01.y : p23.x, strand ? p01.x : p23.y};
 #pragma unroll
-                    for (int q = 0; q < 4; q++) {
-                        const int j = strand ? 3 - q : q;
-                        const uint32_t c = (walk[m] >> (2 * j)) & KMASK;
-                        a[u][q] = seed_tab[seed_slot(c, K, (uint32_t)j & 1u)];
-                    }
+                    for (int q = 0; q < 4; q++) a[u][q] = seed_tab[sl[q]];
                 }
             }
 #pragma unroll
@@ -889,19 +899,23 @@ __global__ void __launch_bounds__(128) k_seed_fixed(const char *__restrict__ rea
         for (int j = 0; j < 16; j++) tot[j] = rv[j];
     }
     /* exchange sort of the long seeds by total hits (:1391-1399, Q4), the
-     * seed strings and residual bits carried along */
+     * seed strings and residual bits carried along.  Pass j settles place j
+     * and later passes touch only places after it, so the passes past the
+     * selected seeds are left out. */
+    const uint32_t n_sel_long = (e + 1 < n_long) ? e + 1 : n_long;
 #pragma unroll
     for (int j = 0; j < 16; j++) {
+        if ((uint32_t)j < n_sel_long) {
 #pragma unroll
-        for (int q = j + 1; q < 16; q++) {
-            if ((uint32_t)q < n_long && tot[j] > tot[q]) {
-                uint32_t t0 = tot[j]; tot[j] = tot[q]; tot[q] = t0;
-                uint32_t t1 = pk[j]; pk[j] = pk[q]; pk[q] = t1;
-                uint32_t t2 = sstr[j]; sstr[j] = sstr[q]; sstr[q] = t2;
+            for (int q = j + 1; q < 16; q++) {
+                if ((uint32_t)q < n_long && tot[j] > tot[q]) {
+                    uint32_t t0 = tot[j]; tot[j] = tot[q]; tot[q] = t0;
+                    uint32_t t1 = pk[j]; pk[j] = pk[q]; pk[q] = t1;
+                    uint32_t t2 = sstr[j]; sstr[j] = sstr[q]; sstr[q] = t2;
+                }
             }
         }
     }
-    const uint32_t n_sel_long = (e + 1 < n_long) ? e + 1 : n_long;
     uint64_t npos = 0;
     uint32_t nchunk = 0;
     bool heavy = false, irr = false;
