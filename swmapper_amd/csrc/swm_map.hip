@@ -1519,6 +1519,130 @@ __global__ void __launch_bounds__(64) k_plan_wave(uint32_t nrs, const uint2 *__r
     if (lane == 0 && fetched) atomicAdd(io.light_pos, (unsigned long long)fetched);
 }
 
+/* Light items, both strands of a read in one wave (when a strand selects
+ * at most 32 12-mers, e <= 7): lanes 0-31 hold the forward strand's
+ * selections and lanes 32-63 the reverse one's, so one prefix sum lays
+ * both strands' entries end to end and one batch of loads fetches them.
+ * The strands select mostly the same long seeds, whose 12-mers are reverse
+ * complements of each other, and each bucket lies beside its reverse
+ * complement's, so the two strands' requests for a line reach the L2
+ * together and cost it one memory read -- fetched one strand after the
+ * other they cost two (the line is gone from the L2 by then).  Filter,
+ * adjust, compaction and the one-batch sort/unique are k_plan_wave's, per
+ * strand; a strand that is heavy or irregular is left to its kernels. */
+#define PP_ROUNDS 3
+
+__global__ void __launch_bounds__(64) k_plan_pair(uint32_t nreads, const uint2 *__restrict__ pl, PlanIO io)
+{
+    __shared__ uint32_t s_off[64], s_pre[64], s_meta[64], s_surv[2][64];
+    if (*io.ovf) return;
+    const uint32_t lane = threadIdx.x;
+    const uint32_t half = lane >> 5, li = lane & 31u;
+    const uint64_t below = (1ull << lane) - 1ull;
+    uint64_t fetched = 0;
+    uint32_t nsw_n = 0;
+    swm_sel_t my_n = {0, 0, 0};
+    /* the next read's selection counts and selections are loaded while this
+     * one is processed */
+    auto prefetch = [&](uint32_t rd) {
+        const uint32_t r = 2u * rd + half;
+        nsw_n = io.nsel[r];
+        my_n = li < io.seln ? io.sel[(uint64_t)r * io.seln + li] : swm_sel_t{0, 0, 0};
+    };
+    if (blockIdx.x < nreads) prefetch(blockIdx.x);
+    for (uint32_t rd = blockIdx.x; rd < nreads; rd += gridDim.x) {
+        const uint32_t nsw = nsw_n;
+        swm_sel_t my = my_n;
+        if (rd + gridDim.x < nreads) prefetch(rd + gridDim.x);
+        const bool mine = (nsw & NSEL_FLAGS) == 0;        /* heavy / irregular: another kernel's */
+        const bool light0 = rl(mine ? 1u : 0u, 0) != 0u, light1 = rl(mine ? 1u : 0u, 32) != 0u;
+        if (!light0 && !light1) continue;
+        if (!mine || li >= nsw) my = swm_sel_t{0, 0, 0};
+        const uint32_t incl = wave_incl_scan_dpp(my.cnt);
+        const uint32_t total = rl(incl, 63), total0 = rl(incl, 31);
+        fetched += total;
+        s_off[lane] = my.off;
+        s_pre[lane] = incl - my.cnt;
+        s_meta[lane] = my.meta;
+        __syncthreads();
+        uint32_t *surv0 = io.surv + io.surv_off[2u * rd], *surv1 = io.surv + io.surv_off[2u * rd + 1u];
+        uint32_t ns0 = 0, ns1 = 0;
+        for (uint32_t e0 = 0; e0 < total; e0 += 64u * PP_ROUNDS) {
+            uint32_t lo[PP_ROUNDS], meta[PP_ROUNDS];
+            uint2 v[PP_ROUNDS];
+#pragma unroll
+            for (int j = 0; j < PP_ROUNDS; j++) lo[j] = 0;
+#pragma unroll
+            for (uint32_t hb = 32; hb >= 1; hb >>= 1) {
+#pragma unroll
+                for (int j = 0; j < PP_ROUNDS; j++) {
+                    const uint32_t ent = e0 + 64u * (uint32_t)j + lane;
+                    if (s_pre[lo[j] + hb] <= ent) lo[j] += hb;
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < PP_ROUNDS; j++) {
+                const uint32_t ent = e0 + 64u * (uint32_t)j + lane;
+                meta[j] = s_meta[lo[j]];
+                v[j] = make_uint2(0u, 0u);
+                if (e0 + 64u * (uint32_t)j < total)       /* wave-uniform */
+                    v[j] = pl[ent < total ? s_off[lo[j]] + (ent - s_pre[lo[j]]) : s_off[0]];
+            }
+#pragma unroll
+            for (int j = 0; j < PP_ROUNDS; j++) {
+                const uint32_t ent = e0 + 64u * (uint32_t)j + lane;
+                const bool st1 = ent >= total0;
+                /* each strand's first fetched entry is dropped (Q7) */
+                const bool keep = ent < total && ent != (st1 ? total0 : 0u) && filter_pass(v[j].y, meta[j]);
+                const uint64_t m = __ballot(keep), m1 = __ballot(keep && st1), m0 = m & ~m1;
+                if (keep) {
+                    const uint32_t at = st1 ? ns1 + (uint32_t)__popcll(m1 & below) : ns0 + (uint32_t)__popcll(m0 & below);
+                    const uint32_t val = adjust(v[j].x, meta[j]);
+                    (st1 ? surv1 : surv0)[at] = val;
+                    if (at < 64u) s_surv[st1 ? 1 : 0][at] = val;
+                }
+                ns0 += (uint32_t)__popcll(m0);
+                ns1 += (uint32_t)__popcll(m1);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (uint32_t h = 0; h < 2; h++) {
+            if (!(h ? light1 : light0)) continue;
+            const uint32_t r = 2u * rd + h;
+            const uint32_t nsurv = h ? ns1 : ns0;
+            uint32_t *surv = h ? surv1 : surv0;
+            uint2 *bd = io.bdesc + io.bd_off[r];
+            if (nsurv <= 64u) {
+                const bool have = lane < nsurv;
+                const uint32_t v = have ? s_surv[h][lane] : 0xffffffffu;
+                uint64_t fm = 0;
+                for (uint64_t left = __ballot(have); left;) {
+                    const uint32_t f = (uint32_t)__ffsll((unsigned long long)left) - 1u;
+                    const uint32_t w = (uint32_t)__builtin_amdgcn_readlane((int)v, (int)f);
+                    fm |= 1ull << f;
+                    left &= ~__ballot(have && v == w);
+                }
+                uint32_t rank = 0;
+                for (uint64_t f = fm; f; f &= f - 1ull) {
+                    const uint32_t q = (uint32_t)__ffsll((unsigned long long)f) - 1u;
+                    rank += (uint32_t)__builtin_amdgcn_readlane((int)v, (int)q) < v ? 1u : 0u;
+                }
+                if ((fm >> lane) & 1ull) surv[rank] = v;
+                const uint32_t u = (uint32_t)__popcll(fm);
+                if (lane == 0) { bd[0] = make_uint2(0, u); io.nuniq[r] = u; }
+            } else if (lane == 0) {
+                bd[0] = make_uint2(0, nsurv);
+                io.big[atomicAdd(io.big_count, 1u)] = make_uint2(r, 0);
+                io.nuniq[r] = nsurv;
+            }
+            if (lane == 0) io.nbatch[r] = 1;
+        }
+        __syncthreads();
+    }
+    if (lane == 0 && fetched) atomicAdd(io.light_pos, (unsigned long long)fetched);
+}
+
 /* Heavy (read, strand): a device work queue drained by persistent
  * single-wave workgroups.  alignment_rmdup's batch composition -- which
  * 32-position chunks, in what heap order, where each batch ends -- depends
@@ -4093,8 +4217,12 @@ static int enq_b(const swm_ctx_t *ctx, Pipe &P, uint64_t cap_cand, hipStream_t s
     const swm_opts &o = ctx->opt;
     if (o.work_queue) {
         if (ev) SWM_HIP_CHECK(hipEventRecord(ev[6], st));
-        k_plan_wave<<<(uint32_t)std::min<uint64_t>(P.n ? P.n : 1, (uint64_t)ctx->n_cu * 32), 64, 0, st>>>(
-            (uint32_t)P.nrs, ctx->d_pl, P.pio);
+        if (P.seln <= 32)
+            k_plan_pair<<<(uint32_t)std::min<uint64_t>(P.n ? P.n : 1, (uint64_t)ctx->n_cu * 32), 64, 0, st>>>(
+                (uint32_t)P.n, ctx->d_pl, P.pio);
+        else
+            k_plan_wave<<<(uint32_t)std::min<uint64_t>(P.n ? P.n : 1, (uint64_t)ctx->n_cu * 32), 64, 0, st>>>(
+                (uint32_t)P.nrs, ctx->d_pl, P.pio);
         if (ev) SWM_HIP_CHECK(hipEventRecord(ev[7], st));
         k_plan_heavy<<<ctx->n_cu * o.plan_heavy_waves, 64, 0, st>>>(ctx->d_pl, P.pio);
         k_plan_fetch<<<ctx->n_cu * 32, 64, 0, st>>>(ctx->d_pl, P.pio);
