@@ -21,16 +21,20 @@
 #define SWM_MAX_BLOCKS 128     /* block_size < 256 (mapper_cpe.c:2265,2356) */
 #define SWM_WIN_PAIRS_PAD 64   /* zero word pairs after the genome (windows past the end read 0) */
 
-/* Per (read, strand) seed selection entry: packed metadata.
- *   bits  0..15  adjust offset 15*m + q (adjust_ref_pos_rmdup, mapper_cpe.c:1901)
- *   bits 16..18  residual, low plane  (set_read_long_seeds, mapper_cpe.c:2489)
- *   bits 19..21  residual, high plane (mapper_cpe.c:2494)
- *   bits 22..23  q = seed_index % 4 (long_seeds_filter shift, mapper_cpe.c:2101) */
+/* Per (read, strand) seed selection entry, 8 bytes: the selected 12-mer's
+ * bucket start in the device bucket array, and cm = count << 16 | meta.
+ * meta:
+ *   bits  0..7   adjust offset 15*m + q (< the read length; adjust_ref_pos_rmdup, mapper_cpe.c:1901)
+ *   bits  8..10  residual, low plane  (set_read_long_seeds, mapper_cpe.c:2489)
+ *   bits 11..13  residual, high plane (mapper_cpe.c:2494)
+ *   bits 14..15  q = seed_index % 4 (long_seeds_filter shift, mapper_cpe.c:2101)
+ * A count of 0xffff or more is stored as 0xffff and in full in the run's
+ * big-count array at the same index (written only for those). */
 struct swm_sel_t {
     uint32_t off;
-    uint32_t cnt;
-    uint32_t meta;
+    uint32_t cm;
 };
+#define SEL_BIG 0xffffu
 
 struct swm_ws;    /* device workspace of one slot */
 struct swm_slot;  /* stream + workspace + captured graphs */

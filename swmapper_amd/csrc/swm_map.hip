@@ -186,6 +186,7 @@ __device__ __forceinline__ T block_incl_scan(T v, T *s_w, T *total)
 struct SeedOut {
     uint32_t *words;    /* [nrs][32]: 16 interleaved 2-bit words, 8 plane pairs */
     swm_sel_t *sel;     /* [nrs][seln] */
+    uint32_t *selc;     /* [nrs][seln]: counts of SEL_BIG or more, written only for those */
     uint32_t *nsel;     /* [nrs] selected 12-mers (4 per selected long seed) */
     uint64_t *npos;     /* [nrs + 1] positions over the selected seeds */
     uint32_t *nchunk;   /* [nrs + 1] sum of ceil(cnt / 32), + 1 */
@@ -641,18 +642,17 @@ __global__ void __launch_bounds__(128) k_seed(const char *__restrict__ reads, ui
                     rl |= (c & 1u) << bb;
                     rh |= (c >> 1) << bb;
                 }
-                swm_sel_t s;
-                s.off = off4[q] & o.off_mask;
-                s.cnt = cnt4[q];
-                s.meta = (base + (uint32_t)q) | (rl << 16) | (rh << 19) | ((uint32_t)q << 22);
-                so[4 * jm + q] = s;
-                heavy |= s.cnt > SWM_CHUNK;
-                if (s.cnt > 1) {
+                const uint32_t off = off4[q] & o.off_mask, cnt = cnt4[q];
+                const uint32_t meta = (base + (uint32_t)q) | (rl << 8) | (rh << 11) | ((uint32_t)q << 14);
+                so[4 * jm + q] = swm_sel_t{off, (min(cnt, SEL_BIG) << 16) | meta};
+                if (cnt >= SEL_BIG) o.selc[(uint64_t)r * seln + 4 * jm + q] = cnt;
+                heavy |= cnt > SWM_CHUNK;
+                if (cnt > 1) {
                     irr |= (off4[q] & ~o.off_mask) != 0u;
-                    if (o.n_irreg) irr |= bucket_irregular(o.irreg, o.n_irreg, s.off);
+                    if (o.n_irreg) irr |= bucket_irregular(o.irreg, o.n_irreg, off);
                 }
-                npos += s.cnt;
-                nchunk += (s.cnt + SWM_CHUNK - 1) / SWM_CHUNK;
+                npos += cnt;
+                nchunk += (cnt + SWM_CHUNK - 1) / SWM_CHUNK;
             }
         }
     }
@@ -946,18 +946,17 @@ __global__ void __launch_bounds__(128) k_seed_fixed(const char *__restrict__ rea
 #pragma unroll
             for (int q = 0; q < 4; q++) {
                 const uint32_t rb = (pk[jm] >> (4 + 6 * q)) & 63u;
-                swm_sel_t sl;
-                sl.off = off4[q] & o.off_mask;
-                sl.cnt = cnt4[q];
-                sl.meta = (base + (uint32_t)q) | ((rb & 7u) << 16) | ((rb >> 3) << 19) | ((uint32_t)q << 22);
-                so[4 * jm + q] = sl;
-                heavy |= sl.cnt > SWM_CHUNK;
-                if (sl.cnt > 1) {
+                const uint32_t off = off4[q] & o.off_mask, cnt = cnt4[q];
+                const uint32_t meta = (base + (uint32_t)q) | (rb << 8) | ((uint32_t)q << 14);
+                so[4 * jm + q] = swm_sel_t{off, (min(cnt, SEL_BIG) << 16) | meta};
+                if (cnt >= SEL_BIG) o.selc[(uint64_t)r * seln + 4 * jm + q] = cnt;
+                heavy |= cnt > SWM_CHUNK;
+                if (cnt > 1) {
                     irr |= (off4[q] & ~o.off_mask) != 0u;
-                    if (o.n_irreg) irr |= bucket_irregular(o.irreg, o.n_irreg, sl.off);
+                    if (o.n_irreg) irr |= bucket_irregular(o.irreg, o.n_irreg, off);
                 }
-                npos += sl.cnt;
-                nchunk += (sl.cnt + SWM_CHUNK - 1) / SWM_CHUNK;
+                npos += cnt;
+                nchunk += (cnt + SWM_CHUNK - 1) / SWM_CHUNK;
             }
         }
     }
@@ -978,6 +977,7 @@ __global__ void __launch_bounds__(128) k_seed_fixed(const char *__restrict__ rea
 
 struct PlanIO {
     const swm_sel_t *sel;
+    const uint32_t *selc;       /* full counts of SEL_BIG or more */
     uint32_t seln;
     const uint32_t *nsel;
     const uint64_t *surv_off;   /* exclusive scan of npos */
@@ -1016,16 +1016,36 @@ struct HBatch {
     uint32_t posoff, pad0, pad1, pad2;
 };
 
+/* A selection's metadata and count (the full count from the big-count
+ * array for SEL_BIG or more; idx = the selection's index in the run). */
+__device__ __forceinline__ uint32_t sel_meta(swm_sel_t s) { return s.cm & 0xffffu; }
+__device__ __forceinline__ uint32_t sel_cnt(swm_sel_t s, const uint32_t *__restrict__ selc, uint64_t idx)
+{
+    const uint32_t c = s.cm >> 16;
+    return c == SEL_BIG ? selc[idx] : c;
+}
+
+/* selection i of item r, decoded */
+struct SelV {
+    uint32_t off, cnt, meta;
+};
+__device__ __forceinline__ SelV ld_sel(const PlanIO &io, uint32_t r, uint32_t i)
+{
+    const uint64_t idx = (uint64_t)r * io.seln + i;
+    const swm_sel_t s = io.sel[idx];
+    return SelV{s.off, sel_cnt(s, io.selc, idx), sel_meta(s)};
+}
+
 /* 15-mer filter of one bucket entry (long_seeds_filter :2106-2117) and the
  * read-start adjustment (adjust_ref_pos_rmdup :1903-1905). */
 __device__ __forceinline__ bool filter_pass(uint32_t ls, uint32_t meta)
 {
-    const uint32_t rl = (meta >> 16) & 7u, rh = (meta >> 19) & 7u, q = (meta >> 22) & 3u;
+    const uint32_t rl = (meta >> 8) & 7u, rh = (meta >> 11) & 7u, q = (meta >> 14) & 3u;
     return ((((ls >> (3 - q)) ^ rl) | ((ls >> (9 - q)) ^ rh)) & 7u) == 0;
 }
 __device__ __forceinline__ uint32_t adjust(uint32_t p, uint32_t meta)
 {
-    const uint32_t bo = meta & 0xffffu;
+    const uint32_t bo = meta & 0xffu;
     return p > bo ? p - bo : 0u;
 }
 
@@ -1039,7 +1059,6 @@ __global__ void __launch_bounds__(64) k_plan_plain(uint32_t nrs, const uint2 *__
     if (r >= nrs) return;
     if (io.nsel[r] & NSEL_IRREG) return;       /* k_plan_irreg's */
     const uint32_t nsel = io.nsel[r] & ~NSEL_FLAGS;
-    const swm_sel_t *sel = io.sel + (uint64_t)r * io.seln;
     uint32_t *surv = io.surv + io.surv_off[r];
     uint2 *bd = io.bdesc + io.bd_off[r];
 
@@ -1093,7 +1112,7 @@ __global__ void __launch_bounds__(64) k_plan_plain(uint32_t nrs, const uint2 *__
 
     bool heavy = false;
     for (uint32_t i = 0; i < nsel; i++) {                     /* phase 1 (:2194-2236) */
-        const swm_sel_t s = sel[i];
+        const SelV s = ld_sel(io, r, i);
         const uint32_t get = s.cnt > SWM_CHUNK ? SWM_CHUNK : s.cnt;
         if (get > 0) { take(s.off, get, s.meta); blocks++; }
         pos_size += get;
@@ -1102,14 +1121,14 @@ __global__ void __launch_bounds__(64) k_plan_plain(uint32_t nrs, const uint2 *__
     }
     if (heavy) {
         for (uint32_t i = 0; i < nsel; i++) {                 /* :2244-2252 */
-            const swm_sel_t s = sel[i];
+            const SelV s = ld_sel(io, r, i);
             if (pcount[i] != s.cnt) heap_insert(pl[s.off + pcount[i] - 1].x, i);
         }
     }
     for (;;) {                                                /* :2261-2419 */
         if (hsize > 0 && pos_size + SWM_CHUNK <= SWM_BATCH_POS && blocks < SWM_MAX_BLOCKS) {
             const uint32_t i = hind[1];
-            const swm_sel_t s = sel[i];
+            const SelV s = ld_sel(io, r, i);
             const uint32_t rem = s.cnt - pcount[i];
             const uint32_t get = rem > SWM_CHUNK ? SWM_CHUNK : rem;
             take(s.off + pcount[i], get, s.meta);
@@ -1240,14 +1259,13 @@ __global__ void __launch_bounds__(64) k_plan_irreg(const uint2 *__restrict__ pl,
     for (uint32_t qi = blockIdx.x; qi < nq; qi += gridDim.x) {
         const uint32_t r = io.irreg[qi];
         const uint32_t nsel = io.nsel[r] & ~NSEL_FLAGS;
-        const swm_sel_t *sel = io.sel + (uint64_t)r * io.seln;
         /* the first chunk of every selected 12-mer (all of phase 1's
          * fetches, and the heap keys of the longer ones) fetched by the whole
          * wave, eight 12-mers' loads in flight at once: lane 0 then replays
          * the state machine from LDS, where one fetch per chunk had put a
          * memory round trip on its serial path for each 12-mer */
         {
-            const swm_sel_t mine = lane < nsel ? sel[lane] : swm_sel_t{0, 0, 0};
+            const SelV mine = lane < nsel ? ld_sel(io, r, lane) : SelV{0, 0, 0};
             for (uint32_t i0 = 0; i0 < nsel; i0 += 8) {
                 uint2 v[8];
 #pragma unroll
@@ -1316,7 +1334,7 @@ __global__ void __launch_bounds__(64) k_plan_irreg(const uint2 *__restrict__ pl,
                 }
             };
             for (uint32_t i = 0; i < nsel; i++) {                   /* phase 1 (:2194-2236), from LDS */
-                const swm_sel_t sv = sel[i];
+                const SelV sv = ld_sel(io, r, i);
                 const uint32_t get = sv.cnt > SWM_CHUNK ? SWM_CHUNK : sv.cnt;
                 if (get > 0) {
                     const uint32_t b0 = nb;
@@ -1333,13 +1351,13 @@ __global__ void __launch_bounds__(64) k_plan_irreg(const uint2 *__restrict__ pl,
                 s_pc[i] = get;
             }
             for (uint32_t i = 0; i < nsel; i++) {                   /* :2244-2252 */
-                const swm_sel_t sv = sel[i];
+                const SelV sv = ld_sel(io, r, i);
                 if (s_pc[i] != sv.cnt) heap_insert(s_stage[i * SWM_CHUNK + s_pc[i] - 1].x, i);
             }
             for (;;) {                                              /* :2261-2419 */
                 if (hsize > 0 && pos_size + SWM_CHUNK <= SWM_BATCH_POS && blocks < SWM_MAX_BLOCKS) {
                     const uint32_t i = s_sid[1];
-                    const swm_sel_t sv = sel[i];
+                    const SelV sv = ld_sel(io, r, i);
                     const uint32_t rem = sv.cnt - s_pc[i];
                     const uint32_t get = rem > SWM_CHUNK ? SWM_CHUNK : rem;
                     take(sv.off + s_pc[i], get, sv.meta);
@@ -1415,7 +1433,7 @@ __global__ void __launch_bounds__(64) k_plan_wave(uint32_t nrs, const uint2 *__r
     uint32_t r = 2u * blockIdx.x;
     uint32_t nsw_n = 0;
     uint64_t fetched = 0;                       /* bucket entries this wave fetched */
-    swm_sel_t my_n = {0, 0, 0};
+    swm_sel_t my_n = {0, 0};
     if (r < nrs) {
         nsw_n = io.nsel[r];
         if (lane < io.seln) my_n = io.sel[(uint64_t)r * io.seln + lane];
@@ -1431,13 +1449,14 @@ __global__ void __launch_bounds__(64) k_plan_wave(uint32_t nrs, const uint2 *__r
         {
             if (nsw & NSEL_FLAGS) continue;      /* heavy: k_plan_heavy's; irregular: k_plan_irreg's */
             const uint32_t nsel = nsw;
-            if (lane >= nsel) my = swm_sel_t{0, 0, 0};       /* slots past nsel are not written */
-            const uint32_t incl = wave_incl_scan_dpp(my.cnt);
+            if (lane >= nsel) my = swm_sel_t{0, 0};          /* slots past nsel are not written */
+            const uint32_t cnt = my.cm >> 16;                /* light: every count is at most a chunk */
+            const uint32_t incl = wave_incl_scan_dpp(cnt);
             const uint32_t total = rl(incl, 63);
             fetched += total;
             s_off[lane] = my.off;
-            s_pre[lane] = incl - my.cnt;        /* lanes >= nsel: total */
-            s_meta[lane] = my.meta;
+            s_pre[lane] = incl - cnt;           /* lanes >= nsel: total */
+            s_meta[lane] = sel_meta(my);
             __syncthreads();
             uint32_t *surv = io.surv + io.surv_off[r];
             uint32_t nsurv = 0;
@@ -1541,13 +1560,13 @@ __global__ void __launch_bounds__(64) k_plan_pair(uint32_t nreads, const uint2 *
     const uint64_t below = (1ull << lane) - 1ull;
     uint64_t fetched = 0;
     uint32_t nsw_n = 0;
-    swm_sel_t my_n = {0, 0, 0};
+    swm_sel_t my_n = {0, 0};
     /* the next read's selection counts and selections are loaded while this
      * one is processed */
     auto prefetch = [&](uint32_t rd) {
         const uint32_t r = 2u * rd + half;
         nsw_n = io.nsel[r];
-        my_n = li < io.seln ? io.sel[(uint64_t)r * io.seln + li] : swm_sel_t{0, 0, 0};
+        my_n = li < io.seln ? io.sel[(uint64_t)r * io.seln + li] : swm_sel_t{0, 0};
     };
     if (blockIdx.x < nreads) prefetch(blockIdx.x);
     for (uint32_t rd = blockIdx.x; rd < nreads; rd += gridDim.x) {
@@ -1557,13 +1576,14 @@ __global__ void __launch_bounds__(64) k_plan_pair(uint32_t nreads, const uint2 *
         const bool mine = (nsw & NSEL_FLAGS) == 0;        /* heavy / irregular: another kernel's */
         const bool light0 = rl(mine ? 1u : 0u, 0) != 0u, light1 = rl(mine ? 1u : 0u, 32) != 0u;
         if (!light0 && !light1) continue;
-        if (!mine || li >= nsw) my = swm_sel_t{0, 0, 0};
-        const uint32_t incl = wave_incl_scan_dpp(my.cnt);
+        if (!mine || li >= nsw) my = swm_sel_t{0, 0};
+        const uint32_t cnt = my.cm >> 16;                    /* light: every count is at most a chunk */
+        const uint32_t incl = wave_incl_scan_dpp(cnt);
         const uint32_t total = rl(incl, 63), total0 = rl(incl, 31);
         fetched += total;
         s_off[lane] = my.off;
-        s_pre[lane] = incl - my.cnt;
-        s_meta[lane] = my.meta;
+        s_pre[lane] = incl - cnt;
+        s_meta[lane] = sel_meta(my);
         __syncthreads();
         uint32_t *surv0 = io.surv + io.surv_off[2u * rd], *surv1 = io.surv + io.surv_off[2u * rd + 1u];
         uint32_t ns0 = 0, ns1 = 0;
@@ -1710,8 +1730,8 @@ __global__ void __launch_bounds__(64) k_plan_heavy(const uint2 *__restrict__ pl,
         const uint32_t nsel = io.nsel[r] & ~NSEL_FLAGS;
         const bool act = lane < nsel;
         /* per 12-mer state, lane i = selected 12-mer i */
-        swm_sel_t my = {0, 0, 0};
-        if (act) my = io.sel[(uint64_t)r * io.seln + lane];
+        SelV my = {0, 0, 0};
+        if (act) my = ld_sel(io, r, lane);
         const uint32_t cnt_v = my.cnt, off_v = my.off;
         uint32_t spc_v = min(cnt_v, (uint32_t)SWM_CHUNK);     /* positions fetched */
         /* heap key after fetching chunk c: the raw last position of that
@@ -1908,7 +1928,7 @@ __global__ void __launch_bounds__(64) k_plan_fetch(const uint2 *__restrict__ pl,
         }
         if (lane == 0) spre[bn] = bpos;
         const uint32_t nsel = io.nsel[r] & ~NSEL_FLAGS;
-        smeta[lane] = lane < nsel ? io.sel[(uint64_t)r * io.seln + lane].meta : 0u;
+        smeta[lane] = lane < nsel ? sel_meta(io.sel[(uint64_t)r * io.seln + lane]) : 0u;
         __syncthreads();
         uint32_t *surv = io.surv + io.surv_off[r];
         const uint32_t bstart = hb.posoff;
@@ -3912,7 +3932,7 @@ inline uint32_t blocks_for(uint64_t n, uint32_t bs) { return (uint32_t)((n + bs 
  * retires the graphs captured over the old addresses. */
 struct swm_ws {
     struct Buf { void *p = nullptr; size_t cap = 0; };
-    enum { WORDS, SEL, NSEL, NPOS, NCHUNK, SURV_OFF, BD_OFF, SURV, BDESC, NBATCH, NUNIQ, CAND_OFF, C_VAL, C_RS, C_BI,
+    enum { WORDS, SEL, SELC, NSEL, NPOS, NCHUNK, SURV_OFF, BD_OFF, SURV, BDESC, NBATCH, NUNIQ, CAND_OFF, C_VAL, C_RS, C_BI,
            C_ERR, C_LOC, C_REC, C_INFO, COUNTERS, BIG, TRACE, HEAVY, HEAVY_C, HEAVY_E, SCAN_TMP, IN_READS, IN_LENS,
            OUT, HIST, R_WR, R_TB, R_RUNS, R_IDX, R_TOFF, R_SCAN, R_TOT, R_OUT, R_TEXT, R_BLOCK, CDESC, HBATCH, GIANT, EPART, RPART, HPART, LBINS, PERM, IRREG, MERGE, NBUF };
     Buf b[NBUF];
@@ -4098,6 +4118,7 @@ static int pipe_alloc_a(Alloc *ws, uint64_t n, uint32_t seln, Pipe &P)
     P.seln = seln;
     int rc;
     if ((rc = ws->get(swm_ws::WORDS, &P.so.words, nrs * 32)) || (rc = ws->get(swm_ws::SEL, &P.so.sel, nrs * seln)) ||
+        (rc = ws->get(swm_ws::SELC, &P.so.selc, nrs * seln)) ||
         (rc = ws->get(swm_ws::NSEL, &P.so.nsel, nrs)) || (rc = ws->get(swm_ws::NPOS, &P.so.npos, nrs + 1)) ||
         (rc = ws->get(swm_ws::NCHUNK, &P.so.nchunk, nrs + 1)) || (rc = ws->get(swm_ws::COUNTERS, &P.counters, 12)) ||
         (rc = ws->get(swm_ws::SURV_OFF, &P.surv_off, nrs + 1)) || (rc = ws->get(swm_ws::BD_OFF, &P.bd_off, nrs + 1)) ||
@@ -4116,6 +4137,7 @@ static int pipe_alloc_a(Alloc *ws, uint64_t n, uint32_t seln, Pipe &P)
     P.so.nuniq = P.pio.nuniq;
     P.u32c = reinterpret_cast<uint32_t *>(P.counters + 4);
     P.pio.sel = P.so.sel;
+    P.pio.selc = P.so.selc;
     P.pio.seln = seln;
     P.pio.nsel = P.so.nsel;
     P.pio.surv_off = P.surv_off;
