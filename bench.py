@@ -144,14 +144,15 @@ def rooflines(swm, st_all, lens, config, form):
     ms_wave = statistics.mean(s.get("ms_plan_wave", 0.0) for s in st_all)
     lpos = st_all[-1].get("n_light_positions", 0)
     if ms_wave > 0 and lpos:
-        traffic = pmc_traffic(config, "k_plan_wave")
+        traffic = pmc_traffic(config, "k_plan_pair") or pmc_traffic(config, "k_plan_wave")
         ach = 6 * lpos / (ms_wave / 1e3) / 1e9
-        roof_wave = {"bound": "hbm", "kernel": "k_plan_wave", "form": form, "achieved": round(ach, 1),
+        roof_wave = {"bound": "hbm", "kernel": "k_plan_pair (k_plan_wave for e > 7)", "form": form,
+                     "achieved": round(ach, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "traffic_rate": traffic_rate(traffic, ms_wave),
                      "algorithmic_bytes_per_launch": 6 * lpos, "avg_launch_ms": round(ms_wave, 3),
                      "note": "random bucket fetches (about 4 entries of 8 bytes each) into the 512 MB "
-                             "bucket array: each costs a whole line"}
+                             "bucket array, both strands of a read in one load: each still costs a whole line"}
     ms_plan = statistics.mean(s["ms_plan"] for s in st_all)
     npos = st_all[-1]["n_positions"]
     if ms_plan > 0 and npos:
