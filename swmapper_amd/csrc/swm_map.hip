@@ -1550,40 +1550,77 @@ __global__ void __launch_bounds__(64) k_plan_wave(uint32_t nrs, const uint2 *__r
  * adjust, compaction and the one-batch sort/unique are k_plan_wave's, per
  * strand; a strand that is heavy or irregular is left to its kernels. */
 #define PP_ROUNDS 3
+#ifndef PLAN_PAIR_ALL_E
+#define PLAN_PAIR_ALL_E 1
+#endif
 
+template <uint32_t W>   /* selection slots per strand: 32 (e <= 7) or 64 (any e) */
 __global__ void __launch_bounds__(64) k_plan_pair(uint32_t nreads, const uint2 *__restrict__ pl, PlanIO io)
 {
-    __shared__ uint32_t s_off[64], s_pre[64], s_meta[64], s_surv[2][64];
+    /* slots: W = 32, slot s = lane (lanes 0-31 forward, 32-63 reverse);
+     * W = 64, slot lane holds the forward selection `lane` and slot 64 + lane
+     * the reverse one's */
+    constexpr uint32_t NS = 2 * W;
+    __shared__ uint32_t s_off[NS], s_pre[NS], s_meta[NS], s_surv[2][64];
     if (*io.ovf) return;
     const uint32_t lane = threadIdx.x;
     const uint32_t half = lane >> 5, li = lane & 31u;
     const uint64_t below = (1ull << lane) - 1ull;
     uint64_t fetched = 0;
-    uint32_t nsw_n = 0;
-    swm_sel_t my_n = {0, 0};
+    uint32_t nsw_n = 0, nsw_nb = 0;
+    swm_sel_t my_n = {0, 0}, my_nb = {0, 0};
     /* the next read's selection counts and selections are loaded while this
      * one is processed */
     auto prefetch = [&](uint32_t rd) {
-        const uint32_t r = 2u * rd + half;
-        nsw_n = io.nsel[r];
-        my_n = li < io.seln ? io.sel[(uint64_t)r * io.seln + li] : swm_sel_t{0, 0};
+        if (W == 32) {
+            const uint32_t r = 2u * rd + half;
+            nsw_n = io.nsel[r];
+            my_n = li < io.seln ? io.sel[(uint64_t)r * io.seln + li] : swm_sel_t{0, 0};
+        } else {
+            nsw_n = io.nsel[2u * rd];
+            nsw_nb = io.nsel[2u * rd + 1u];
+            my_n = lane < io.seln ? io.sel[(uint64_t)(2u * rd) * io.seln + lane] : swm_sel_t{0, 0};
+            my_nb = lane < io.seln ? io.sel[(uint64_t)(2u * rd + 1u) * io.seln + lane] : swm_sel_t{0, 0};
+        }
     };
     if (blockIdx.x < nreads) prefetch(blockIdx.x);
     for (uint32_t rd = blockIdx.x; rd < nreads; rd += gridDim.x) {
-        const uint32_t nsw = nsw_n;
-        swm_sel_t my = my_n;
+        const uint32_t nsw = nsw_n, nswb = nsw_nb;
+        swm_sel_t my = my_n, myb = my_nb;
         if (rd + gridDim.x < nreads) prefetch(rd + gridDim.x);
-        const bool mine = (nsw & NSEL_FLAGS) == 0;        /* heavy / irregular: another kernel's */
-        const bool light0 = rl(mine ? 1u : 0u, 0) != 0u, light1 = rl(mine ? 1u : 0u, 32) != 0u;
-        if (!light0 && !light1) continue;
-        if (!mine || li >= nsw) my = swm_sel_t{0, 0};
-        const uint32_t cnt = my.cm >> 16;                    /* light: every count is at most a chunk */
-        const uint32_t incl = wave_incl_scan_dpp(cnt);
-        const uint32_t total = rl(incl, 63), total0 = rl(incl, 31);
+        bool light0, light1;
+        uint32_t total, total0;
+        if (W == 32) {
+            const bool mine = (nsw & NSEL_FLAGS) == 0;      /* heavy / irregular: another kernel's */
+            light0 = rl(mine ? 1u : 0u, 0) != 0u;
+            light1 = rl(mine ? 1u : 0u, 32) != 0u;
+            if (!light0 && !light1) continue;
+            if (!mine || li >= nsw) my = swm_sel_t{0, 0};
+            const uint32_t cnt = my.cm >> 16;               /* light: every count is at most a chunk */
+            const uint32_t incl = wave_incl_scan_dpp(cnt);
+            total = rl(incl, 63);
+            total0 = rl(incl, 31);
+            s_off[lane] = my.off;
+            s_pre[lane] = incl - cnt;
+            s_meta[lane] = sel_meta(my);
+        } else {
+            light0 = (nsw & NSEL_FLAGS) == 0;
+            light1 = (nswb & NSEL_FLAGS) == 0;
+            if (!light0 && !light1) continue;
+            if (!light0 || lane >= nsw) my = swm_sel_t{0, 0};
+            if (!light1 || lane >= nswb) myb = swm_sel_t{0, 0};
+            const uint32_t cnt = my.cm >> 16, cntb = myb.cm >> 16;
+            const uint32_t incl = wave_incl_scan_dpp(cnt), inclb = wave_incl_scan_dpp(cntb);
+            total0 = rl(incl, 63);
+            total = total0 + rl(inclb, 63);
+            s_off[lane] = my.off;
+            s_pre[lane] = incl - cnt;
+            s_meta[lane] = sel_meta(my);
+            s_off[64 + lane] = myb.off;
+            s_pre[64 + lane] = total0 + inclb - cntb;
+            s_meta[64 + lane] = sel_meta(myb);
+        }
         fetched += total;
-        s_off[lane] = my.off;
-        s_pre[lane] = incl - cnt;
-        s_meta[lane] = sel_meta(my);
         __syncthreads();
         uint32_t *surv0 = io.surv + io.surv_off[2u * rd], *surv1 = io.surv + io.surv_off[2u * rd + 1u];
         uint32_t ns0 = 0, ns1 = 0;
@@ -1593,7 +1630,7 @@ __global__ void __launch_bounds__(64) k_plan_pair(uint32_t nreads, const uint2 *
 #pragma unroll
             for (int j = 0; j < PP_ROUNDS; j++) lo[j] = 0;
 #pragma unroll
-            for (uint32_t hb = 32; hb >= 1; hb >>= 1) {
+            for (uint32_t hb = NS / 2; hb >= 1; hb >>= 1) {
 #pragma unroll
                 for (int j = 0; j < PP_ROUNDS; j++) {
                     const uint32_t ent = e0 + 64u * (uint32_t)j + lane;
@@ -4239,12 +4276,13 @@ static int enq_b(const swm_ctx_t *ctx, Pipe &P, uint64_t cap_cand, hipStream_t s
     const swm_opts &o = ctx->opt;
     if (o.work_queue) {
         if (ev) SWM_HIP_CHECK(hipEventRecord(ev[6], st));
+        const uint32_t pgrid = (uint32_t)std::min<uint64_t>(P.n ? P.n : 1, (uint64_t)ctx->n_cu * 32);
         if (P.seln <= 32)
-            k_plan_pair<<<(uint32_t)std::min<uint64_t>(P.n ? P.n : 1, (uint64_t)ctx->n_cu * 32), 64, 0, st>>>(
-                (uint32_t)P.n, ctx->d_pl, P.pio);
+            k_plan_pair<32><<<pgrid, 64, 0, st>>>((uint32_t)P.n, ctx->d_pl, P.pio);
+        else if (!PLAN_PAIR_ALL_E)
+            k_plan_wave<<<pgrid, 64, 0, st>>>((uint32_t)P.nrs, ctx->d_pl, P.pio);
         else
-            k_plan_wave<<<(uint32_t)std::min<uint64_t>(P.n ? P.n : 1, (uint64_t)ctx->n_cu * 32), 64, 0, st>>>(
-                (uint32_t)P.nrs, ctx->d_pl, P.pio);
+            k_plan_pair<64><<<pgrid, 64, 0, st>>>((uint32_t)P.n, ctx->d_pl, P.pio);
         if (ev) SWM_HIP_CHECK(hipEventRecord(ev[7], st));
         k_plan_heavy<<<ctx->n_cu * o.plan_heavy_waves, 64, 0, st>>>(ctx->d_pl, P.pio);
         k_plan_fetch<<<ctx->n_cu * 32, 64, 0, st>>>(ctx->d_pl, P.pio);
