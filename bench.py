@@ -26,6 +26,7 @@ VALU_PEAK_OPS = 256 * 4 * 16 * 2.4e9   # CUs x SIMDs x lanes x clock = 39.3e12 i
 HBM_PEAK_GBS = 8000.0
 
 PMC_FILES = {"repeat5": "map_repeat5_1Mx150_e5_pmc_bytes.txt"}
+REQ_FILES = {"repeat5": "map_repeat5_1Mx150_e5_pmc_requests.txt"}
 ISSUE_FILE = "map_valu_issue_skew_repeat5.txt"
 
 
@@ -42,6 +43,34 @@ def pmc_traffic(config, kernel):
         if f and f[0] == kernel and len(f) >= 5:
             return {"bytes": int((float(f[3]) + float(f[4])) * 1e6), "source": "profiles/" + name}
     return None
+
+
+def pmc_line_reads(config, kernel):
+    """Memory read requests (128-byte lines) per launch of a kernel from the
+    committed rocprofv3 TCC_EA0_RDREQ summary of this workload, or None."""
+    name = REQ_FILES.get(config)
+    path = os.path.join(ROOT, "profiles", name) if name else None
+    if not path or not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = line.split()
+        if f and f[0] == kernel and len(f) >= 3:
+            try:
+                return int(f[1])
+            except ValueError:
+                return None
+    return None
+
+
+def gather_ceiling(line_reads, ms, probe):
+    """A gather-bound kernel against the measured random-gather rate: its
+    line reads per second over the probe's (one line per 8-byte gather)."""
+    if not line_reads or not ms or not probe:
+        return None
+    rate = line_reads / (ms / 1e3)
+    return {"line_reads": line_reads, "line_reads_per_s": round(rate / 1e9, 2) * 1e9,
+            "probe_gathers_per_s": round(probe / 1e9, 2) * 1e9, "frac": round(rate / probe, 4),
+            "unit": "line reads/s", "source": "profiles/" + REQ_FILES.get("repeat5", "")}
 
 
 def valu_issue(config, kernel):
@@ -128,6 +157,9 @@ def rooflines(swm, st_all, lens, config, form):
     ok = (lv + 2 <= 256) & (lv // (k + 3) <= 16)
     seed_bytes = int(lv[ok].sum() + 2 * 4 * 8 * (lv[ok] // (k + 3)).sum())
     roof_seed = roof_wave = roof_plan = roof_myers = None
+    # the chip's random 8-byte gather rate over an array the size of the
+    # bucket array: the ceiling of the two gather-bound kernels
+    probe = swm.gather_peak(512 << 20) if config in REQ_FILES else None
     ms_seed = statistics.mean(s["ms_seed"] for s in st_all)
     if ms_seed > 0:
         traffic = pmc_traffic(config, "k_seed_fixed<12u>") or pmc_traffic(config, "k_seed")
@@ -137,6 +169,7 @@ def rooflines(swm, st_all, lens, config, form):
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "traffic_rate": traffic_rate(traffic, ms_seed),
                      "algorithmic_bytes_per_launch": seed_bytes, "avg_launch_ms": round(ms_seed, 3),
+                     "gather": gather_ceiling(pmc_line_reads(config, "k_seed_fixed<12u>"), ms_seed, probe),
                      "note": "random 8-byte gathers into the seed table: a 128-byte line serves two "
                              "consecutive 12-mers of a read and their reverse complements, so HBM traffic "
                              "(rocprofv3 FETCH_SIZE/WRITE_SIZE, calibrated on a copy) is still several "
@@ -151,6 +184,7 @@ def rooflines(swm, st_all, lens, config, form):
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "traffic_rate": traffic_rate(traffic, ms_wave),
                      "algorithmic_bytes_per_launch": 6 * lpos, "avg_launch_ms": round(ms_wave, 3),
+                     "gather": gather_ceiling(pmc_line_reads(config, "k_plan_pair"), ms_wave, probe),
                      "note": "random bucket fetches (about 4 entries of 8 bytes each) into the 512 MB "
                              "bucket array, both strands of a read in one load: each still costs a whole line"}
     ms_plan = statistics.mean(s["ms_plan"] for s in st_all)

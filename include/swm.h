@@ -385,6 +385,12 @@ uint64_t swm_seed_slot(uint32_t code, uint32_t k, uint32_t role);
  * reference. */
 int swm_valu_probe(double *ops_per_s);
 
+/* Measured rate of random 8-byte gathers (one 128-byte line read each) over
+ * a device array of array_bytes, in gathers per second: the ceiling the
+ * gather-bound seed and bucket kernels are quoted against, beside the HBM
+ * byte peak.  No counterpart in the reference. */
+int swm_gather_probe(uint64_t array_bytes, double *gathers_per_s);
+
 /* ---- device memory plumbing for callers without their own HIP setup
  *      (no counterpart in the reference, whose host/CPE memory is shared) ---- */
 int swm_device_count(int *n);

@@ -139,6 +139,7 @@ def lib():
             "swm_synth_genome": ([vp, u64, u64, C.c_int, C.c_double], C.c_int),
             "swm_device_count": ([C.POINTER(C.c_int)], C.c_int),
             "swm_valu_probe": ([C.POINTER(C.c_double)], C.c_int),
+            "swm_gather_probe": ([C.c_uint64, C.POINTER(C.c_double)], C.c_int),
             "swm_seed_slot": ([u32, u32, u32], u64),
             "swm_dev_alloc": ([C.POINTER(vp), u64], C.c_int),
             "swm_dev_free": ([vp], C.c_int),
@@ -520,6 +521,14 @@ def valu_peak():
     """Measured int32 VALU lane-ops/s of the device."""
     v = C.c_double(0)
     _check("swm_valu_probe", lib().swm_valu_probe(C.byref(v)))
+    return v.value
+
+
+def gather_peak(array_bytes=512 << 20):
+    """Measured random 8-byte gathers per second (one line read each) over a
+    device array of array_bytes."""
+    v = C.c_double(0)
+    _check("swm_gather_probe", lib().swm_gather_probe(int(array_bytes), C.byref(v)))
     return v.value
 
 

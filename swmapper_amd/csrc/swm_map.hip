@@ -5269,3 +5269,69 @@ extern "C" int swm_valu_probe(double *ops_per_s)
     (void)hipFree(d);
     return rc;
 }
+
+/* ---------------------------------------------------- gather rate probe */
+
+/* Random 8-byte gathers, GP_PER per thread, all issued before any is used:
+ * entry i of gather j is a hash of (thread, j) modulo n, so the lanes of a
+ * wave (and the gathers of a thread) fall on distinct 128-byte lines and
+ * every gather costs one line read. */
+#define GP_PER 16
+__device__ __forceinline__ uint32_t gp_hash(uint32_t x)
+{
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    return x ^ (x >> 16);
+}
+__global__ void __launch_bounds__(256) k_gather_probe(const uint2 *__restrict__ a, uint64_t n, uint32_t seed,
+                                                      uint32_t *__restrict__ out)
+{
+    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    uint2 v[GP_PER];
+#pragma unroll
+    for (int j = 0; j < GP_PER; j++) {
+        const uint64_t h = ((uint64_t)gp_hash(t * GP_PER + (uint32_t)j + seed) << 6) | (gp_hash(t ^ ((uint32_t)j * 977u)) & 63u);
+        v[j] = a[h % n];
+    }
+    uint32_t acc = 0;
+#pragma unroll
+    for (int j = 0; j < GP_PER; j++) acc ^= v[j].x + v[j].y;
+    if (acc == 0x9e3779b9u) out[t] = acc;   /* keeps the loads live */
+}
+
+/* No counterpart in the reference: the chip's rate of random 8-byte gathers
+ * over an array of array_bytes (one line read each), in gathers per second
+ * -- the ceiling of the gather-bound seed and bucket kernels, whose line
+ * reads per launch rocprofv3 counts.  The best of three timed launches of
+ * 64M gathers. */
+extern "C" int swm_gather_probe(uint64_t array_bytes, double *gathers_per_s)
+{
+    if (!gathers_per_s || array_bytes < (1u << 20)) return SWM_EINVAL;
+    const uint64_t n = array_bytes / 8;
+    const uint32_t threads = 1u << 22;
+    uint2 *a = nullptr;
+    uint32_t *d = nullptr;
+    if (hipMalloc((void **)&a, n * 8) != hipSuccess) return SWM_ENOMEM;
+    if (hipMalloc((void **)&d, (uint64_t)threads * 4) != hipSuccess) { (void)hipFree(a); return SWM_ENOMEM; }
+    int rc = hipMemset(a, 0x5a, n * 8) == hipSuccess ? SWM_OK : SWM_EDEVICE;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (rc == SWM_OK && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)) rc = SWM_EDEVICE;
+    float best = 0.0f;
+    for (int rep = 0; rc == SWM_OK && rep < 4; rep++) {
+        (void)hipEventRecord(e0, 0);
+        k_gather_probe<<<threads / 256u, 256>>>(a, n, 0x1234u + (uint32_t)rep, d);
+        (void)hipEventRecord(e1, 0);
+        if (hipEventSynchronize(e1) != hipSuccess) { rc = SWM_EDEVICE; break; }
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        if (rep && (best == 0.0f || ms < best)) best = ms;      /* the first launch warms up */
+    }
+    if (rc == SWM_OK) *gathers_per_s = (double)threads * GP_PER / (best * 1e-3);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(d);
+    (void)hipFree(a);
+    return rc;
+}
