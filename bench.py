@@ -145,13 +145,13 @@ FORMS = {
 
 def rooflines(swm, st_all, lens, config, form):
     """Rooflines from a stage-timed form (HIP events around each stage, and
-    around k_plan_wave, on the context stream): the seed lookup (k_seed,
+    around k_plan_pair, on the context stream): the seed lookup (k_seed,
     HBM-bound random gathers; algorithmic bytes = read bytes + 8 B per
     12-mer queried, 4 per long seed, both strands), the light-item plan
-    kernel (k_plan_wave, HBM-bound random bucket fetches; 6 B per bucket
+    kernel (k_plan_pair, HBM-bound random bucket fetches; 6 B per bucket
     entry it fetched), the whole plan stage, and banded Myers (VALU-bound,
     20 int32 ops per band step).  `roofline` is the one of k_seed and
-    k_plan_wave with the longer launch: the dominant kernel."""
+    k_plan_pair with the longer launch: the dominant kernel."""
     k = 12
     lv = lens.astype(np.int64)
     ok = (lv + 2 <= 256) & (lv // (k + 3) <= 16)
@@ -177,9 +177,9 @@ def rooflines(swm, st_all, lens, config, form):
     ms_wave = statistics.mean(s.get("ms_plan_wave", 0.0) for s in st_all)
     lpos = st_all[-1].get("n_light_positions", 0)
     if ms_wave > 0 and lpos:
-        traffic = pmc_traffic(config, "k_plan_pair") or pmc_traffic(config, "k_plan_wave")
+        traffic = pmc_traffic(config, "k_plan_pair")
         ach = 6 * lpos / (ms_wave / 1e3) / 1e9
-        roof_wave = {"bound": "hbm", "kernel": "k_plan_pair (k_plan_wave for e > 7)", "form": form,
+        roof_wave = {"bound": "hbm", "kernel": "k_plan_pair (both strands of a read per wave)", "form": form,
                      "achieved": round(ach, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "traffic_rate": traffic_rate(traffic, ms_wave),
@@ -191,7 +191,7 @@ def rooflines(swm, st_all, lens, config, form):
     npos = st_all[-1]["n_positions"]
     if ms_plan > 0 and npos:
         ach = 6 * npos / (ms_plan / 1e3) / 1e9
-        roof_plan = {"bound": "hbm", "kernel": "plan stage (k_plan_wave, heavy plan kernels, large-batch sort)",
+        roof_plan = {"bound": "hbm", "kernel": "plan stage (k_plan_pair, heavy plan kernels, large-batch sort)",
                      "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(ach / HBM_PEAK_GBS, 4), "algorithmic_bytes_per_launch": 6 * npos,
                      "avg_launch_ms": round(ms_plan, 3)}

@@ -91,7 +91,7 @@ typedef struct swm_stats {
     uint64_t n_irregular_items;
     /* 1 when this run captured and instantiated its graph (graph form) */
     uint64_t n_graph_captures;
-    /* bucket entries fetched by the light-item plan kernel (k_plan_wave), and
+    /* bucket entries fetched by the light-item plan kernel (k_plan_pair), and
      * its own time (work-queue form, stage-timed runs; else 0) */
     uint64_t n_light_positions;
     double ms_plan_wave;

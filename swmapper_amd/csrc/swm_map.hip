@@ -5,12 +5,13 @@
  * The path replaces find_positions (mapper_cpe.c:2592-2663) behind
  * swm_map_batch*.  One launch per stage on the context's stream:
  *
- *   k_seed     64 reads per workgroup staged in LDS, one thread per
- *              (read, strand): read encoding (applied from a per-length
+ *   k_seed_fixed<12> (k_seed for other k)  64 reads per workgroup
+ *              staged in LDS, one thread per (read, strand): read encoding
+ *              (applied from a per-length
  *              recipe), 12-mer hash-offset gathers, long-seed ranking,
  *              residuals (mapper_cpe.c:1246-1401, 2451-2499)
- *   k_plan_wave  one wave per light (read, strand): the one batch of
- *              alignment_rmdup (mapper_cpe.c:2124-2426) -- the selected
+ *   k_plan_pair  one wave per read, both strands' light items: the one batch
+ *              of alignment_rmdup (mapper_cpe.c:2124-2426) -- the selected
  *              12-mers' positions fetched 64 lanes at a time, filtered by the
  *              15-mer test (:2090-2123), shifted to read starts
  *              (:1893-1917), sorted and uniqued (block_heap_sort's result,
@@ -1001,7 +1002,7 @@ struct PlanIO {
     uint32_t *irreg_count;
     uint32_t *merge;            /* heavy batches with a descending chunk (hbatch indices), for k_merge */
     uint32_t *merge_count;
-    unsigned long long *light_pos;  /* bucket entries fetched by k_plan_wave */
+    unsigned long long *light_pos;  /* bucket entries fetched by k_plan_pair */
 };
 
 /* Bit 31 of a device bucket entry's second word: its position is smaller
@@ -1404,155 +1405,27 @@ __device__ __forceinline__ uint32_t sel_gt(uint32_t a, uint32_t b, uint32_t x, u
     return r;
 }
 
-/* Light items, one wave per item: the selected 12-mers' entries are laid
- * end to end (a prefix sum of the counts) and fetched 64 at a time, lane j
- * on entry e0 + j, so the lanes of one chunk read consecutive addresses and
- * one line costs one request -- one thread per item issued a request per
- * entry (and duplicates of the last one), 2.7 per line.  Waves take items
- * one at a time (grid-stride), so a small batch is spread over all waves
- * instead of 64 items in a row per wave; the scan of the position counts
- * appends the heavy items beforehand, one atomic per list per workgroup.
- * Results as k_plan:
- * the 15-mer filter and read-start adjustment in entry order with the first
- * entry dropped (Q7), then the sorted unique list of the one batch (<= 64
- * survivors sorted in the wave by rank, larger ones left to k_bigsort). */
-#define PW_ROUNDS 2
-
-__global__ void __launch_bounds__(64) k_plan_wave(uint32_t nrs, const uint2 *__restrict__ pl, PlanIO io)
-{
-    __shared__ uint32_t s_off[64], s_pre[64], s_meta[64], s_surv[64];
-    if (*io.ovf) return;
-    const uint32_t lane = threadIdx.x;
-    const uint64_t below = (1ull << lane) - 1ull;
-    /* the next item's selection count and selections are loaded while this
-     * one is processed (its loads would otherwise head a dependent chain) */
-    /* a wave takes both strands of a read, one after the other: their
-     * selected 12-mers are mostly reverse complements of each other, whose
-     * buckets lie side by side, so the second strand finds the lines the
-     * first one fetched in its XCD's L2 */
-    uint32_t r = 2u * blockIdx.x;
-    uint32_t nsw_n = 0;
-    uint64_t fetched = 0;                       /* bucket entries this wave fetched */
-    swm_sel_t my_n = {0, 0};
-    if (r < nrs) {
-        nsw_n = io.nsel[r];
-        if (lane < io.seln) my_n = io.sel[(uint64_t)r * io.seln + lane];
-    }
-    for (; r < nrs; r = (r & 1u) ? r + 2u * gridDim.x - 1u : r + 1u) {
-        const uint32_t nsw = nsw_n;
-        swm_sel_t my = my_n;
-        const uint32_t rn = (r & 1u) ? r + 2u * gridDim.x - 1u : r + 1u;
-        if (rn < nrs) {
-            nsw_n = io.nsel[rn];
-            if (lane < io.seln) my_n = io.sel[(uint64_t)rn * io.seln + lane];
-        }
-        {
-            if (nsw & NSEL_FLAGS) continue;      /* heavy: k_plan_heavy's; irregular: k_plan_irreg's */
-            const uint32_t nsel = nsw;
-            if (lane >= nsel) my = swm_sel_t{0, 0};          /* slots past nsel are not written */
-            const uint32_t cnt = my.cm >> 16;                /* light: every count is at most a chunk */
-            const uint32_t incl = wave_incl_scan_dpp(cnt);
-            const uint32_t total = rl(incl, 63);
-            fetched += total;
-            s_off[lane] = my.off;
-            s_pre[lane] = incl - cnt;           /* lanes >= nsel: total */
-            s_meta[lane] = sel_meta(my);
-            __syncthreads();
-            uint32_t *surv = io.surv + io.surv_off[r];
-            uint32_t nsurv = 0;
-            /* PW_ROUNDS entries per lane at once: an item's bucket entries
-             * (about 100 on a uniform genome) come back in one round trip */
-            for (uint32_t e0 = 0; e0 < total; e0 += 64u * PW_ROUNDS) {
-                uint32_t lo[PW_ROUNDS], meta[PW_ROUNDS];
-                uint2 v[PW_ROUNDS];
-                /* the 12-mer of each entry: the last k with s_pre[k] <= ent
-                 * (a 12-mer with no entries shares its start with the next);
-                 * the rounds' searches interleaved */
-#pragma unroll
-                for (int j = 0; j < PW_ROUNDS; j++) lo[j] = 0;
-#pragma unroll
-                for (uint32_t half = 32; half >= 1; half >>= 1) {
-#pragma unroll
-                    for (int j = 0; j < PW_ROUNDS; j++) {
-                        const uint32_t ent = e0 + 64u * (uint32_t)j + lane;
-                        if (s_pre[lo[j] + half] <= ent) lo[j] += half;
-                    }
-                }
-#pragma unroll
-                for (int j = 0; j < PW_ROUNDS; j++) {
-                    const uint32_t ent = e0 + 64u * (uint32_t)j + lane;
-                    meta[j] = s_meta[lo[j]];
-                    v[j] = make_uint2(0u, 0u);
-                    if (e0 + 64u * (uint32_t)j < total)       /* wave-uniform */
-                        v[j] = pl[ent < total ? s_off[lo[j]] + (ent - s_pre[lo[j]]) : s_off[0]];
-                }
-#pragma unroll
-                for (int j = 0; j < PW_ROUNDS; j++) {
-                    const uint32_t ent = e0 + 64u * (uint32_t)j + lane;
-                    const bool keep = ent < total && ent > 0 && filter_pass(v[j].y, meta[j]);
-                    const uint64_t m = __ballot(keep);
-                    if (keep) {
-                        const uint32_t at = nsurv + (uint32_t)__popcll(m & below);
-                        const uint32_t val = adjust(v[j].x, meta[j]);
-                        surv[at] = val;
-                        if (at < 64u) s_surv[at] = val;
-                    }
-                    nsurv += (uint32_t)__popcll(m);
-                }
-            }
-            __syncthreads();
-            uint2 *bd = io.bdesc + io.bd_off[r];
-            if (nsurv <= 64u) {
-                /* sorted unique by rank: a value's place is the number of
-                 * distinct smaller values */
-                const bool have = lane < nsurv;
-                const uint32_t v = have ? s_surv[lane] : 0xffffffffu;
-                /* the first lane of each distinct value, one pass per value
-                 * (the survivors of a mapped read are mostly copies of its
-                 * true start, one per selected 12-mer) */
-                uint64_t fm = 0;
-                for (uint64_t left = __ballot(have); left;) {
-                    const uint32_t f = (uint32_t)__ffsll((unsigned long long)left) - 1u;
-                    const uint32_t w = (uint32_t)__builtin_amdgcn_readlane((int)v, (int)f);
-                    fm |= 1ull << f;
-                    left &= ~__ballot(have && v == w);
-                }
-                const bool first = (fm >> lane) & 1ull;
-                uint32_t rank = 0;
-                for (uint64_t f = fm; f; f &= f - 1ull) {
-                    const uint32_t q = (uint32_t)__ffsll((unsigned long long)f) - 1u;
-                    rank += (uint32_t)__builtin_amdgcn_readlane((int)v, (int)q) < v ? 1u : 0u;
-                }
-                if (first) surv[rank] = v;
-                const uint32_t u = (uint32_t)__popcll(fm);
-                if (lane == 0) { bd[0] = make_uint2(0, u); io.nuniq[r] = u; }
-            } else if (lane == 0) {
-                bd[0] = make_uint2(0, nsurv);
-                io.big[atomicAdd(io.big_count, 1u)] = make_uint2(r, 0);
-                io.nuniq[r] = nsurv;
-            }
-            if (lane == 0) io.nbatch[r] = 1;
-            __syncthreads();
-        }
-    }
-    if (lane == 0 && fetched) atomicAdd(io.light_pos, (unsigned long long)fetched);
-}
-
-/* Light items, both strands of a read in one wave (when a strand selects
- * at most 32 12-mers, e <= 7): lanes 0-31 hold the forward strand's
- * selections and lanes 32-63 the reverse one's, so one prefix sum lays
- * both strands' entries end to end and one batch of loads fetches them.
- * The strands select mostly the same long seeds, whose 12-mers are reverse
- * complements of each other, and each bucket lies beside its reverse
- * complement's, so the two strands' requests for a line reach the L2
- * together and cost it one memory read -- fetched one strand after the
- * other they cost two (the line is gone from the L2 by then).  Filter,
- * adjust, compaction and the one-batch sort/unique are k_plan_wave's, per
- * strand; a strand that is heavy or irregular is left to its kernels. */
+/* Light items (every selected 12-mer with at most one chunk of hits), both
+ * strands of a read in one wave.  A strand's selected 12-mers' entries are
+ * laid end to end (a prefix sum of the counts) and fetched 64 at a time,
+ * lane j on entry e0 + j, each lane finding its 12-mer by binary search
+ * over the prefix sums: the lanes of one round read consecutive addresses
+ * and one line costs one request.  W = 32 (e <= 7): lanes 0-31 hold the
+ * forward strand's selections and lanes 32-63 the reverse one's, so one
+ * prefix sum lays both strands' entries end to end and one batch of loads
+ * fetches them; W = 64 (any e): each lane holds both strands' selection of
+ * its index, two scans.  The strands select mostly the same long seeds,
+ * whose 12-mers are reverse complements of each other, and each bucket lies
+ * beside its reverse complement's, so the two strands' requests for a line
+ * reach the L2 together and cost it one memory read -- fetched one strand
+ * after the other they cost two (the line is gone from the L2 by then).
+ * Results as k_plan: per strand, the 15-mer filter and read-start
+ * adjustment in entry order with the first entry dropped (Q7), then the
+ * sorted unique list of the one batch (<= 64 survivors sorted in the wave
+ * by rank, larger ones left to k_bigsort).  Waves take reads one at a time
+ * (grid-stride), the next read's selections loaded while this one's are
+ * processed; a strand that is heavy or irregular is left to its kernels. */
 #define PP_ROUNDS 3
-#ifndef PLAN_PAIR_ALL_E
-#define PLAN_PAIR_ALL_E 1
-#endif
 
 template <uint32_t W>   /* selection slots per strand: 32 (e <= 7) or 64 (any e) */
 __global__ void __launch_bounds__(64) k_plan_pair(uint32_t nreads, const uint2 *__restrict__ pl, PlanIO io)
@@ -4022,7 +3895,7 @@ struct swm_slot {
     };
     GraphEnt g[SLOT_GRAPHS];
     uint64_t tick = 0;
-    hipEvent_t ev[8] = {};      /* stage bounds [0..5]; k_plan_wave [6], [7] */
+    hipEvent_t ev[8] = {};      /* stage bounds [0..5]; k_plan_pair [6], [7] */
     /* page-locked staging, so that no copy goes through the runtime's shared
      * pageable-memory path (which serialises the slots): totals read back
      * between stages, counters, and host batches in and out */
@@ -4277,10 +4150,8 @@ static int enq_b(const swm_ctx_t *ctx, Pipe &P, uint64_t cap_cand, hipStream_t s
     if (o.work_queue) {
         if (ev) SWM_HIP_CHECK(hipEventRecord(ev[6], st));
         const uint32_t pgrid = (uint32_t)std::min<uint64_t>(P.n ? P.n : 1, (uint64_t)ctx->n_cu * 32);
-        if (P.seln <= 32)
+        if (P.seln <= 32)       /* seln <= 64 for every e */
             k_plan_pair<32><<<pgrid, 64, 0, st>>>((uint32_t)P.n, ctx->d_pl, P.pio);
-        else if (!PLAN_PAIR_ALL_E)
-            k_plan_wave<<<pgrid, 64, 0, st>>>((uint32_t)P.nrs, ctx->d_pl, P.pio);
         else
             k_plan_pair<64><<<pgrid, 64, 0, st>>>((uint32_t)P.n, ctx->d_pl, P.pio);
         if (ev) SWM_HIP_CHECK(hipEventRecord(ev[7], st));
