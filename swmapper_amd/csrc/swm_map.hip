@@ -2322,6 +2322,7 @@ __global__ void __launch_bounds__(256) k_compact(uint32_t nrs, const uint64_t *_
 {
     __shared__ uint32_t s_tmp[2], s_w[4];
     __shared__ uint32_t s_bpre[257], s_bdo[256], s_o[256];   /* per item: first slot, descriptors, next output */
+    __shared__ uint32_t s_cn[256];                            /* per item: its candidate count */
     __shared__ uint64_t s_so[256];                            /* per item: first survivor */
     __shared__ uint32_t s_it[CPT_SLOTS], s_pre[CPT_SLOTS + 1], s_ob[CPT_SLOTS], s_bj[CPT_SLOTS];
     __shared__ uint64_t s_src[CPT_SLOTS];
@@ -2353,6 +2354,7 @@ __global__ void __launch_bounds__(256) k_compact(uint32_t nrs, const uint64_t *_
     s_bpre[tid] = bincl - nb;
     if (tid == 0) s_bpre[256] = tb;
     s_o[tid] = co;
+    s_cn[tid] = cn;
     s_bdo[tid] = bdo;
     s_so[tid] = so;
     __syncthreads();
@@ -2374,7 +2376,9 @@ __global__ void __launch_bounds__(256) k_compact(uint32_t nrs, const uint64_t *_
                     else hi = mid - 1;
                 }
                 const uint32_t b = slot - s_bpre[lo];
-                const uint2 d = bdesc[s_bdo[lo] + b];
+                /* an item's only batch starts its survivors and holds all
+                 * its candidates: no descriptor load */
+                const uint2 d = s_bpre[lo + 1] - s_bpre[lo] == 1u ? make_uint2(0u, s_cn[lo]) : bdesc[s_bdo[lo] + b];
                 s_it[k] = lo;
                 s_src[k] = s_so[lo] + d.x;
                 s_bj[k] = b << 12;
