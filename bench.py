@@ -174,7 +174,7 @@ def rooflines(swm, st_all, lens, config, form):
                              "consecutive 12-mers of a read and their reverse complements, so HBM traffic "
                              "(rocprofv3 FETCH_SIZE/WRITE_SIZE, calibrated on a copy) is still several "
                              "times the algorithmic bytes"}
-    ms_wave = statistics.mean(s.get("ms_plan_wave", 0.0) for s in st_all)
+    ms_wave = statistics.mean(s.get("ms_plan_light", 0.0) for s in st_all)
     lpos = st_all[-1].get("n_light_positions", 0)
     if ms_wave > 0 and lpos:
         traffic = pmc_traffic(config, "k_plan_pair")
@@ -218,7 +218,7 @@ def rooflines(swm, st_all, lens, config, form):
                       "avg_launch_ms": round(ms_myers, 3)}
     cands = [x for x in (roof_seed, roof_wave) if x]
     roof = max(cands, key=lambda x: x["avg_launch_ms"]) if cands else None
-    return {"roofline": roof, "roofline_seed": roof_seed, "roofline_plan_wave": roof_wave,
+    return {"roofline": roof, "roofline_seed": roof_seed, "roofline_plan_light": roof_wave,
             "roofline_plan": roof_plan, "roofline_myers": roof_myers}
 
 

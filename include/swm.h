@@ -94,7 +94,7 @@ typedef struct swm_stats {
     /* bucket entries fetched by the light-item plan kernel (k_plan_pair), and
      * its own time (work-queue form, stage-timed runs; else 0) */
     uint64_t n_light_positions;
-    double ms_plan_wave;
+    double ms_plan_light;
 } swm_stats_t;
 
 typedef struct swm_ctx swm_ctx_t;

@@ -57,7 +57,7 @@ class Stats(C.Structure):
                 ("cand_hist", C.c_uint64 * 24), ("max_cands", C.c_uint64),
                 ("n_heavy_items", C.c_uint64), ("n_large_batches", C.c_uint64), ("n_heavy_reads", C.c_uint64),
                 ("n_giant_items", C.c_uint64), ("n_giant_reads", C.c_uint64), ("n_irregular_items", C.c_uint64),
-                ("n_graph_captures", C.c_uint64), ("n_light_positions", C.c_uint64), ("ms_plan_wave", C.c_double)]
+                ("n_graph_captures", C.c_uint64), ("n_light_positions", C.c_uint64), ("ms_plan_light", C.c_double)]
 
     def as_dict(self):
         d = {f: getattr(self, f) for f, _ in self._fields_}

@@ -4344,8 +4344,8 @@ static int run_synced(swm_ctx_t *ctx, swm_slot *sl, const char *d_reads, uint64_
     (void)hipEventElapsedTime(&ms, sl->ev[3], sl->ev[4]); sl->stats.ms_myers = ms;
     (void)hipEventElapsedTime(&ms, sl->ev[4], sl->ev[5]); sl->stats.ms_emit = ms;
     (void)hipEventElapsedTime(&ms, sl->ev[0], sl->ev[5]); sl->stats.ms_total = ms;
-    sl->stats.ms_plan_wave = 0;
-    if (ctx->opt.work_queue) { (void)hipEventElapsedTime(&ms, sl->ev[6], sl->ev[7]); sl->stats.ms_plan_wave = ms; }
+    sl->stats.ms_plan_light = 0;
+    if (ctx->opt.work_queue) { (void)hipEventElapsedTime(&ms, sl->ev[6], sl->ev[7]); sl->stats.ms_plan_light = ms; }
     return SWM_OK;
 }
 
