@@ -135,11 +135,15 @@ FORMS = {
     # one launch per stage, one thread per (read, strand) for every item
     "plain": {1: 0, 2: 0},
     # heavy items through device work queues (persistent single-wave plan
-    # kernel, cooperative compaction and emission); stage-timed
-    "queue": {1: 1, 2: 0},
+    # kernel, cooperative compaction and emission); stage-timed, one stream
+    # (the heavy-item plan kernels after the light-item one), so that every
+    # stage and the roofline kernels are timed running alone
+    "queue": {1: 1, 2: 0, 12: 0},
     # the work-queue pipeline with device-side counts and fixed capacities,
-    # captured once and replayed as a hipGraph (no host round trip per batch)
-    "graph": {1: 1, 2: 1},
+    # captured once and replayed as a hipGraph (no host round trip per batch);
+    # the heavy-item plan kernels on the slot's second stream beside the
+    # light-item one
+    "graph": {1: 1, 2: 1, 12: 1},
 }
 
 

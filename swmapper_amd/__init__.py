@@ -104,12 +104,14 @@ OPT_HISTOGRAM = 7
 OPT_MYERS_PACKED = 8
 OPT_GIANT_POS, OPT_GIANT_CANDS = 9, 10
 OPT_SEED_GENERIC = 11
+OPT_PLAN_OVERLAP, OPT_PLAN_LIGHT_WAVES = 12, 13
 OPTION_NAMES = {OPT_WORK_QUEUE: "work_queue", OPT_GRAPH: "graph", OPT_HEAVY_CANDS: "heavy_cands",
                 OPT_MYERS_BLOCK: "myers_block", OPT_PLAN_HEAVY_WAVES: "plan_heavy_waves",
                 OPT_EMIT_HEAVY_WAVES: "emit_heavy_waves", OPT_HISTOGRAM: "histogram",
                 OPT_MYERS_PACKED: "myers_packed", OPT_GIANT_POS: "giant_pos", OPT_GIANT_CANDS: "giant_cands",
-                OPT_SEED_GENERIC: "seed_generic"}
-N_OPTIONS = 11
+                OPT_SEED_GENERIC: "seed_generic", OPT_PLAN_OVERLAP: "plan_overlap",
+                OPT_PLAN_LIGHT_WAVES: "plan_light_waves"}
+N_OPTIONS = 13
 
 
 _lib_handle = None

@@ -3885,6 +3885,11 @@ struct swm_ws {
  * concurrently over the shared, read-only index. */
 struct swm_slot {
     hipStream_t stream = nullptr;
+    /* second stream of the slot: the heavy and irregular plan kernels run on
+     * it beside the light-item kernel (forked and joined with events, also
+     * inside a captured graph) */
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     swm_ws ws;
     swm_stats_t stats{};
     /* largest observed per-read totals: they size the fixed capacities */
@@ -3918,10 +3923,14 @@ struct swm_slot {
     {
         if (ev_gate) (void)hipEventDestroy(ev_gate);
         if (stream) (void)hipStreamSynchronize(stream);
+        if (side) (void)hipStreamSynchronize(side);
+        if (ev_fork) (void)hipEventDestroy(ev_fork);
+        if (ev_join) (void)hipEventDestroy(ev_join);
         for (auto &x : g) if (x.exec) (void)hipGraphExecDestroy(x.exec);
         for (auto &x : ev) if (x) (void)hipEventDestroy(x);
         for (Pin *x : {&h_reads, &h_lens, &h_out}) if (x->p) (void)hipHostFree(x->p);
         if (h_small) (void)hipHostFree(h_small);
+        if (side) (void)hipStreamDestroy(side);
         if (stream) (void)hipStreamDestroy(stream);
     }
     int pin(Pin &x, size_t bytes)
@@ -3942,8 +3951,10 @@ swm_slot *swm_slot_create(swm_ctx_t *ctx)
     (void)ctx;
     swm_slot *s = new (std::nothrow) swm_slot();
     if (!s) return nullptr;
-    bool ok = hipStreamCreate(&s->stream) == hipSuccess;
+    bool ok = hipStreamCreate(&s->stream) == hipSuccess && hipStreamCreate(&s->side) == hipSuccess;
     for (auto &x : s->ev) ok = ok && hipEventCreate(&x) == hipSuccess;
+    ok = ok && hipEventCreateWithFlags(&s->ev_fork, hipEventDisableTiming) == hipSuccess;
+    ok = ok && hipEventCreateWithFlags(&s->ev_join, hipEventDisableTiming) == hipSuccess;
     ok = ok && hipEventCreateWithFlags(&s->ev_gate, hipEventDisableTiming) == hipSuccess;
     ok = ok && hipHostMalloc((void **)&s->h_small, 16 * sizeof(uint64_t), hipHostMallocDefault) == hipSuccess;
     if (!ok) { delete s; return nullptr; }
@@ -4147,26 +4158,43 @@ static int enq_a(const swm_ctx_t *ctx, Pipe &P, const char *d_reads, uint64_t st
 }
 
 /* Stage B: batch plans, large-batch sort, candidate offsets with their
- * capacity check. */
-static int enq_b(const swm_ctx_t *ctx, Pipe &P, uint64_t cap_cand, hipStream_t st, hipEvent_t *ev)
+ * capacity check.  In the work-queue form with plan_overlap the heavy and
+ * irregular items' kernels (latency-bound chains) run on the slot's second
+ * stream while the light-item kernel runs on the first: they take disjoint
+ * items and meet only in the large-batch list's atomic counter. */
+static int enq_b(const swm_ctx_t *ctx, swm_slot *sl, Pipe &P, uint64_t cap_cand, hipStream_t st, hipEvent_t *ev)
 {
     const swm_opts &o = ctx->opt;
     if (o.work_queue) {
+        const bool fork = o.plan_overlap != 0;
+        /* heavy items: seed-heap replay, batch fetch; irregular items and
+         * the merges of descending chunks */
+        auto heavy_chain = [&](hipStream_t hs) {
+            k_plan_heavy<<<ctx->n_cu * o.plan_heavy_waves, 64, 0, hs>>>(ctx->d_pl, P.pio);
+            k_plan_fetch<<<ctx->n_cu * 32, 64, 0, hs>>>(ctx->d_pl, P.pio);
+            if (ctx->n_irreg) {
+                k_plan_irreg<<<ctx->n_cu * 4, 64, 0, hs>>>(ctx->d_pl, P.pio);
+                k_merge<<<ctx->n_cu * 4, 64, 0, hs>>>(P.pio);
+            }
+        };
+        if (fork) {
+            SWM_HIP_CHECK(hipEventRecord(sl->ev_fork, st));
+            SWM_HIP_CHECK(hipStreamWaitEvent(sl->side, sl->ev_fork, 0));
+            heavy_chain(sl->side);
+            SWM_HIP_CHECK(hipEventRecord(sl->ev_join, sl->side));
+        }
         if (ev) SWM_HIP_CHECK(hipEventRecord(ev[6], st));
-        const uint32_t pgrid = (uint32_t)std::min<uint64_t>(P.n ? P.n : 1, (uint64_t)ctx->n_cu * 32);
+        const uint32_t pgrid = (uint32_t)std::min<uint64_t>(P.n ? P.n : 1, (uint64_t)ctx->n_cu * o.plan_light_waves);
         if (P.seln <= 32)       /* seln <= 64 for every e */
             k_plan_pair<32><<<pgrid, 64, 0, st>>>((uint32_t)P.n, ctx->d_pl, P.pio);
         else
             k_plan_pair<64><<<pgrid, 64, 0, st>>>((uint32_t)P.n, ctx->d_pl, P.pio);
         if (ev) SWM_HIP_CHECK(hipEventRecord(ev[7], st));
-        k_plan_heavy<<<ctx->n_cu * o.plan_heavy_waves, 64, 0, st>>>(ctx->d_pl, P.pio);
-        k_plan_fetch<<<ctx->n_cu * 32, 64, 0, st>>>(ctx->d_pl, P.pio);
+        if (fork) SWM_HIP_CHECK(hipStreamWaitEvent(st, sl->ev_join, 0));
+        else heavy_chain(st);
     } else {
         k_plan_plain<<<blocks_for(P.nrs, 64), 64, 0, st>>>((uint32_t)P.nrs, ctx->d_pl, P.pio);
-    }
-    if (ctx->n_irreg) {
-        k_plan_irreg<<<ctx->n_cu * 4, 64, 0, st>>>(ctx->d_pl, P.pio);
-        if (o.work_queue) k_merge<<<ctx->n_cu * 4, 64, 0, st>>>(P.pio);
+        if (ctx->n_irreg) k_plan_irreg<<<ctx->n_cu * 4, 64, 0, st>>>(ctx->d_pl, P.pio);
     }
     k_bigsort<<<ctx->n_cu * 32, 64, 0, st>>>(P.pio.big, P.pio.big_count, P.surv_off, P.bd_off, P.pio.surv, P.pio.bdesc,
                                     P.pio.nuniq);
@@ -4317,7 +4345,7 @@ static int run_synced(swm_ctx_t *ctx, swm_slot *sl, const char *d_reads, uint64_
     SWM_HIP_CHECK(hipStreamSynchronize(st));
     const uint64_t tot_pos = hs[8], tot_chunk = (uint32_t)hs[9];
     if ((rc = pipe_alloc_b(ws, tot_pos, tot_chunk, P))) return rc;
-    if ((rc = enq_b(ctx, P, ~0ull, st, sl->ev))) return rc;
+    if ((rc = enq_b(ctx, sl, P, ~0ull, st, sl->ev))) return rc;
     hs[10] = 0;
     SWM_HIP_CHECK(hipMemcpyAsync(hs + 10, P.cand_off + P.nrs, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
     SWM_HIP_CHECK(hipStreamSynchronize(st));
@@ -4474,7 +4502,7 @@ static int run_graph(swm_ctx_t *ctx, swm_slot *sl, const char *d_reads, uint64_t
             if (sl->gate) glk = std::unique_lock<std::mutex>(sl->gate->mu);
             SWM_HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
             int crc = enq_a(ctx, P, d_reads, stride, d_lens, e, cap_pos, cap_chunk, st, nullptr);
-            if (!crc) crc = enq_b(ctx, P, cap_cand, st, nullptr);
+            if (!crc) crc = enq_b(ctx, sl, P, cap_cand, st, nullptr);
             if (!crc) crc = enq_c(ctx, P, d_lens, e, d_out, g_cand, st, nullptr);
             const hipError_t ce = hipStreamEndCapture(st, &graph);
             if (crc || ce != hipSuccess) {
@@ -4607,7 +4635,7 @@ static int run_noalloc(swm_ctx_t *ctx, const char *d_reads, uint64_t stride, con
         GateTurn turn(sl);
         SWM_HIP_CHECK(hipEventRecord(sl->ev[0], st));
         if ((rc = enq_a(ctx, P, d_reads, stride, d_lens, e, cap_pos, cap_chunk, st, nullptr)) ||
-            (rc = enq_b(ctx, P, cap_cand, st, nullptr)) || (rc = enq_c(ctx, P, d_lens, e, d_out, cap_cand, st, nullptr)))
+            (rc = enq_b(ctx, sl, P, cap_cand, st, nullptr)) || (rc = enq_c(ctx, P, d_lens, e, d_out, cap_cand, st, nullptr)))
             return rc;
         SWM_HIP_CHECK(hipEventRecord(sl->ev[5], st));
     }
@@ -4890,6 +4918,8 @@ extern "C" int swm_ctx_set_option(swm_ctx_t *ctx, int opt, int64_t value)
     case SWM_OPT_GIANT_POS: if (value < 1 || value > (1 << 24)) return SWM_EINVAL; o.giant_pos = (uint32_t)value; break;
     case SWM_OPT_GIANT_CANDS: if (value < 1 || value > (1 << 24)) return SWM_EINVAL; o.giant_cands = (uint32_t)value; break;
     case SWM_OPT_SEED_GENERIC: if (value != 0 && value != 1) return SWM_EINVAL; o.seed_generic = (uint32_t)value; break;
+    case SWM_OPT_PLAN_OVERLAP: if (value != 0 && value != 1) return SWM_EINVAL; o.plan_overlap = (uint32_t)value; break;
+    case SWM_OPT_PLAN_LIGHT_WAVES: if (value < 8 || value > 64) return SWM_EINVAL; o.plan_light_waves = (uint32_t)value; break;
     default: return SWM_EINVAL;
     }
     return SWM_OK;
@@ -4911,6 +4941,8 @@ extern "C" int swm_ctx_get_option(const swm_ctx_t *ctx, int opt, int64_t *value)
     case SWM_OPT_GIANT_POS: *value = o.giant_pos; break;
     case SWM_OPT_GIANT_CANDS: *value = o.giant_cands; break;
     case SWM_OPT_SEED_GENERIC: *value = o.seed_generic; break;
+    case SWM_OPT_PLAN_OVERLAP: *value = o.plan_overlap; break;
+    case SWM_OPT_PLAN_LIGHT_WAVES: *value = o.plan_light_waves; break;
     default: return SWM_EINVAL;
     }
     return SWM_OK;

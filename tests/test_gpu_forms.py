@@ -50,6 +50,9 @@ SETTINGS = [
     {swm.OPT_GRAPH: 1, swm.OPT_WORK_QUEUE: 0},
     {swm.OPT_MYERS_PACKED: 0},
     {swm.OPT_MYERS_PACKED: 1, swm.OPT_MYERS_BLOCK: 64},
+    {swm.OPT_PLAN_OVERLAP: 0},                     # heavy plan kernels after the light-item one, one stream
+    {swm.OPT_PLAN_LIGHT_WAVES: 64, swm.OPT_PLAN_HEAVY_WAVES: 8},
+    {swm.OPT_GRAPH: 1, swm.OPT_PLAN_LIGHT_WAVES: 8},   # the fork and join inside a captured graph
 ]
 
 
