@@ -3011,7 +3011,7 @@ __device__ __forceinline__ void close_run(char c, int n, uint32_t &ms_acc, uint3
 /* RUNS: also store the merged runs (records path); a separate instance, so
  * the plain pipeline keeps the smaller register footprint. */
 template <bool RUNS>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_trace(const uint32_t *__restrict__ trace, const uint32_t *__restrict__ trace_count,
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) k_trace(const uint32_t *__restrict__ trace, const uint32_t *__restrict__ trace_count,
                                               const uint32_t *__restrict__ lens, const uint32_t *__restrict__ words,
                                               const uint32_t *__restrict__ b2, uint64_t ref_length, uint32_t e,
                                               CandSet c)
@@ -3053,7 +3053,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
          * the previous high ones; the last prefetch reads one word past the
          * read's window, inside the genome's zero padding) */
         uint32_t vp = 0, vn = 0;
-        uint32_t d0r[32], hpr[32];
+        uint32_t d0r[32], hpr[32];      /* a chunk's D0 and HP (read by the backward pass) */
         uint32_t fr0 = w[0], fr1 = w[1];
         uint32_t fl0 = genome_word(b2, (int64_t)st, 0), fh0 = genome_word(b2, (int64_t)st, 1);
         uint32_t fl1 = genome_word(b2, (int64_t)st + 32, 0), fh1 = genome_word(b2, (int64_t)st + 32, 1);
@@ -3088,13 +3088,29 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
         uint16_t *runs = RUNS ? c.runs + (uint64_t)ci * RUN_SLOT : nullptr;
         for (int cc = (int)nch - 1; cc >= 0 && !done && i >= 0; cc--) {
             const uint32_t steps = ((uint32_t)cc == nfull) ? rem : 32u;
-            vp = ck[2 * cc];
-            vn = ck[2 * cc + 1];
             const uint32_t r0 = w[2 * cc], r1 = w[2 * cc + 1];
             const int64_t g0 = (int64_t)st + 32 * (int64_t)cc;
             const uint32_t wml = genome_word(b2, g0 - 32, 0), wmh = genome_word(b2, g0 - 32, 1);
             const uint32_t l0 = genome_word(b2, g0, 0), l1 = genome_word(b2, g0 + 32, 0);
             const uint32_t h0 = genome_word(b2, g0, 1), h1 = genome_word(b2, g0 + 32, 1);
+            /* The walk's three tests at a step s of this chunk are bits of
+             * three masks over the chunk's 32 steps, all set by the search
+             * site ss alone: D0 and HP at bit ss (0 outside the word), and
+             * whether the window base at match_site (REF_LBIT/REF_HBIT,
+             * :374-378) equals the read base -- match_site - i equals ss
+             * (a match or substitution moves both, an insertion moves i
+             * and ss, a deletion match_site and ss), so those window bases
+             * are the 32 starting at chunk offset ss.  ss moves only on an
+             * indel; then the chunk's steps are recomputed from its
+             * checkpoint for the new masks.  A run of matches (D0 set,
+             * bases equal) is taken in one step: each adds a base to the
+             * pending "M" run and nothing else. */
+            const uint32_t rdl = even_bits(r0) | (even_bits(r1) << 16);
+            const uint32_t rdh = even_bits(r0 >> 1) | (even_bits(r1 >> 1) << 16);
+            const bool far = e > 10u;                   /* ss can pass 31 */
+            const uint32_t l2 = far ? genome_word(b2, g0 + 64, 0) : 0u, h2 = far ? genome_word(b2, g0 + 64, 1) : 0u;
+            vp = ck[2 * cc];
+            vn = ck[2 * cc + 1];
 #pragma unroll
             for (int s = 0; s < 32; s++) {
                 if ((uint32_t)s < steps) {
@@ -3102,33 +3118,61 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
                     else TR_STEP(s, r1)
                 }
             }
-            /* window base at match_site (REF_LBIT/REF_HBIT, :374-378) */
-            auto refb = [&](int ms) -> uint32_t {
-                const int wi = ms >> 5, sh = ms & 31;
-                uint32_t lo, hi;
-                if (wi == cc) { lo = l0; hi = h0; }
-                else if (wi == cc + 1) { lo = l1; hi = h1; }
-                else if (wi == cc - 1) { lo = wml; hi = wmh; }
-                else { lo = genome_word(b2, (int64_t)st + 32 * (int64_t)wi, 0); hi = genome_word(b2, (int64_t)st + 32 * (int64_t)wi, 1); }
-                return ((lo >> sh) & 1u) | (((hi >> sh) & 1u) << 1);
-            };
+            /* Lanes walk until their search site moves or they leave the
+             * chunk; then the wave rebuilds the masks of every lane that
+             * moved at once (a rebuild is the wave's work, so one per
+             * indel step of the slowest lane, not one per lane's). */
+            uint32_t dm = 0, hm = 0, em = 0;
+            bool stale = true, walking = !done && i >= 32 * cc;
+            while (walking) {
+                if (stale) {
+                    stale = false;
+                    dm = 0;
+                    hm = 0;
+                    if ((uint32_t)search_site < 32u) {
+                        const uint32_t sh = (uint32_t)search_site;
 #pragma unroll
-            for (int s = 31; s >= 0; s--) {
-                const int t = 32 * cc + s;
-                const uint32_t qb = ((s < 16 ? r0 : r1) >> (2 * (s & 15))) & 3u;
-                while (!done && i == t) {
-                    if (!first && sum_err == error) { done = true; break; }
-                    const uint32_t bd = (search_site < 0 || search_site >= 32) ? 0u : ((d0r[s] >> search_site) & 1u);
-                    const uint32_t bh = (search_site < 0 || search_site >= 32) ? 0u : ((hpr[s] >> search_site) & 1u);
-                    const uint32_t rb = refb(match_site);
+                        for (int k = 0; k < 32; k++) {
+                            dm |= ((d0r[k] >> sh) & 1u) << k;
+                            hm |= ((hpr[k] >> sh) & 1u) << k;
+                        }
+                    }
+                    uint32_t fl, fh;
+                    if (search_site < -32 || search_site >= 64) {
+                        fl = genome_word(b2, g0 + search_site, 0);
+                        fh = genome_word(b2, g0 + search_site, 1);
+                    } else {
+                        const bool lo = search_site < 0, hi = search_site >= 32;
+                        const uint32_t sh = (uint32_t)(search_site & 31);
+                        fl = __builtin_amdgcn_alignbit(lo ? l0 : (hi ? l2 : l1), lo ? wml : (hi ? l1 : l0), sh);
+                        fh = __builtin_amdgcn_alignbit(lo ? h0 : (hi ? h2 : h1), lo ? wmh : (hi ? h1 : h0), sh);
+                    }
+                    em = ~((fl ^ rdl) | (fh ^ rdh));
+                }
+                while (!stale) {
+                    if (i < 32 * cc) { walking = false; break; }
+                    if (!first && sum_err == error) { done = true; walking = false; break; }
+                    const uint32_t sj = (uint32_t)(i - 32 * cc);
+                    if (!first) {
+                        /* matches from step sj down */
+                        const uint32_t n = min((uint32_t)__clz(~(dm & em) << (31u - sj)), sj + 1u);
+                        if (n) {
+                            if (pre_char != 'M') { close_run(pre_char, pre_size, ms_acc, text, runs, nr); pre_char = 'M'; pre_size = 0; }
+                            pre_size += (int)n;
+                            i -= (int)n;
+                            match_site -= (int)n;
+                            continue;
+                        }
+                    }
+                    const bool bd = (dm >> sj) & 1u, bh = (hm >> sj) & 1u, same = (em >> sj) & 1u;
                     /* a substitution closes like a match (both add to the
                      * pending "<n>M" group), so it takes the match's run
                      * label: only indels start a new run */
                     char nc;
-                    if (bd && rb == qb) { i--; match_site--; nc = 'M'; }
-                    else if (!bd && rb != qb) { i--; match_site--; sum_err++; nc = 'M'; }
-                    else if (bh) { i--; search_site++; sum_err++; start_location++; nc = first ? 'M' : 'I'; }
-                    else { search_site--; match_site--; sum_err++; start_location--; nc = 'D'; }
+                    if (bd && same) { i--; match_site--; nc = 'M'; }
+                    else if (!bd && !same) { i--; match_site--; sum_err++; nc = 'M'; }
+                    else if (bh) { i--; search_site++; sum_err++; start_location++; nc = first ? 'M' : 'I'; stale = true; }
+                    else { search_site--; match_site--; sum_err++; start_location--; nc = 'D'; stale = true; }
                     if (first) { pre_char = nc; pre_size = 1; first = false; }
                     else if (nc != pre_char) { close_run(pre_char, pre_size, ms_acc, text, runs, nr); pre_char = nc; pre_size = 1; }
                     else pre_size++;
