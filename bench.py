@@ -181,14 +181,14 @@ def rooflines(swm, st_all, lens, config, form):
     ms_wave = statistics.mean(s.get("ms_plan_light", 0.0) for s in st_all)
     lpos = st_all[-1].get("n_light_positions", 0)
     if ms_wave > 0 and lpos:
-        traffic = pmc_traffic(config, "k_plan_pair")
+        traffic = pmc_traffic(config, "k_plan_pair<32u>") or pmc_traffic(config, "k_plan_pair")
         ach = 6 * lpos / (ms_wave / 1e3) / 1e9
         roof_wave = {"bound": "hbm", "kernel": "k_plan_pair (both strands of a read per wave)", "form": form,
                      "achieved": round(ach, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "traffic_rate": traffic_rate(traffic, ms_wave),
                      "algorithmic_bytes_per_launch": 6 * lpos, "avg_launch_ms": round(ms_wave, 3),
-                     "gather": gather_ceiling(pmc_line_reads(config, "k_plan_pair"), ms_wave, probe),
+                     "gather": gather_ceiling(pmc_line_reads(config, "k_plan_pair<32u>") or pmc_line_reads(config, "k_plan_pair"), ms_wave, probe),
                      "note": "random bucket fetches (about 4 entries of 8 bytes each) into the 512 MB "
                              "bucket array, both strands of a read in one load: each still costs a whole line"}
     ms_plan = statistics.mean(s["ms_plan"] for s in st_all)
