@@ -22,9 +22,9 @@
  *   k_compact  256 (read, strand) items per workgroup, copied flat: dense
  *              candidate list (k_compact_heavy: one heavy item per workgroup)
  *   k_myers2   two candidates per lane (e <= 7) / k_myers one: window
- *              fetch, banded Myers (semantics mapper_cpe.c:519-660 == :668-976)
- *   k_hamming  one lane per passing candidate: the Hamming check of
- *              align_reads (:252-287)
+ *              fetch, banded Myers (semantics mapper_cpe.c:519-660 == :668-976),
+ *              then for each passing candidate the Hamming check of
+ *              align_reads (:252-287) on the same window words
  *   k_trace    one lane per passing candidate that failed the Hamming check:
  *              the traceback of align_reads (:288-509)
  *   k_emit     one thread per read (k_emit_heavy one wave per heavy read):
@@ -2613,62 +2613,87 @@ __device__ __forceinline__ void load_window(const uint32_t *__restrict__ b2, uin
 
 /* The candidate count is read on the device (the last entry of the candidate
  * offsets), so a fixed grid serves any batch: grid-stride loop. */
-template <bool PERM>
-__device__ __forceinline__ void myers_loop(uint32_t ncand, const uint32_t *__restrict__ perm,
-                                           const uint32_t *__restrict__ lens, const uint32_t *__restrict__ words,
-                                           const uint32_t *__restrict__ b2, uint64_t ref_length, uint32_t e,
-                                           const CandSet &c)
-{
-    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < ncand; p += gridDim.x * blockDim.x) {
-        const uint32_t i = PERM ? perm[p] : p;    /* place p of the verification order */
-        const uint32_t rs = c.rs[i];
-        const uint32_t L = lens[rs >> 1];
-        uint32_t ref_len;
-        const uint32_t st = pos_start(ref_length, e, c.val[i], L, &ref_len);
-        if (st == 0xffffffffu) {
-            c.err[i] = -2;
-            c.loc[i] = -1;
-        } else {
-            uint32_t wl[SWM_NW], wh[SWM_NW];
-            load_window(b2, st, L, wl, wh);
-            const uint32_t *w = words + (uint64_t)rs * 32;
-            int32_t er, lc;
-            myers_lane<false>(w, L, e, ref_len, wl, wh, &er, &lc, nullptr, nullptr);
-            c.err[i] = er;
-            c.loc[i] = lc;
+/* Per workgroup, what verification hands on beside each candidate's
+ * distance: the candidates that need the traceback (appended to the global
+ * list once per round of the workgroup: one global atomic, the entries
+ * placed through LDS), and the band steps (read length per verified
+ * window) and rejected windows, summed and added once at the end (same-
+ * address atomics from every wave would serialise at one L2 channel).
+ * Every thread of the workgroup calls round() and finish(). */
+struct VerifyTally {
+    uint32_t *list, *count;
+    unsigned long long *n_steps;
+    uint32_t *n_rej;
+    uint32_t *s_list, *s_n, *s_base, *s_rej;
+    unsigned long long *s_steps;
+    unsigned long long steps;
+    uint32_t rej;
+    __device__ void start()
+    {
+        steps = 0;
+        rej = 0;
+        if (threadIdx.x == 0) { *s_n = 0; *s_steps = 0; *s_rej = 0; }
+        __syncthreads();
+    }
+    /* a and b: this thread's traceback candidates of the round (~0u: none) */
+    __device__ void round(uint32_t a, uint32_t b)
+    {
+        const uint32_t lane = __lane_id();
+        const uint64_t below = (1ull << lane) - 1ull;
+        const uint64_t ma = __ballot(a != ~0u), mb = __ballot(b != ~0u);
+        uint32_t wb = 0;
+        if (lane == 0 && (ma | mb)) wb = atomicAdd(s_n, (uint32_t)(__popcll(ma) + __popcll(mb)));
+        wb = (uint32_t)__shfl((int)wb, 0, 64);
+        if (a != ~0u) s_list[wb + (uint32_t)__popcll(ma & below)] = a;
+        if (b != ~0u) s_list[wb + (uint32_t)__popcll(ma) + (uint32_t)__popcll(mb & below)] = b;
+        __syncthreads();
+        const uint32_t n = *s_n;
+        if (n) {
+            if (threadIdx.x == 0) *s_base = atomicAdd(count, n);
+            __syncthreads();
+            const uint32_t base = *s_base;
+            for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) list[base + k] = s_list[k];
+            __syncthreads();
+            if (threadIdx.x == 0) *s_n = 0;
+            __syncthreads();
         }
     }
-}
+    __device__ void finish()
+    {
+#pragma unroll
+        for (int d = 32; d > 0; d >>= 1) {
+            const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)steps, d, 64);
+            const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(steps >> 32), d, 64);
+            steps += ((unsigned long long)hi << 32) | lo;
+            rej += (uint32_t)__shfl_xor((int)rej, d, 64);
+        }
+        if (__lane_id() == 0) {
+            if (steps) atomicAdd(s_steps, steps);
+            if (rej) atomicAdd(s_rej, rej);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            if (*s_steps) atomicAdd(n_steps, *s_steps);
+            if (*s_rej) atomicAdd(n_rej, *s_rej);
+        }
+    }
+};
+#define VT_LIST 512    /* traceback entries per round: two per thread of a workgroup of at most 256 */
 
-__global__ void __launch_bounds__(256) k_myers(const uint32_t *__restrict__ ncand_p, const uint32_t *__restrict__ lens,
-                                               const uint32_t *__restrict__ words, const uint32_t *__restrict__ b2,
-                                               uint64_t ref_length, uint32_t e, CandSet c)
+/* Hamming check of align_reads (:252-287) for a verified candidate with
+ * err <= e: the read against the window at the implied start loc - L + 1.
+ * Equal counts give a "<L>M" record; the rest go to the traceback (returns
+ * true).  The window is loaded again (the verification rotated its words
+ * out of the registers; its lines are still in the cache). */
+__device__ __forceinline__ bool ham_window(uint32_t i, int32_t er, int32_t lc, uint32_t st, uint32_t L,
+                                           const uint32_t *__restrict__ w, const uint32_t *__restrict__ b2,
+                                           const CandSet &c)
 {
-    const uint2 fl = myers_flags(c);
-    if (myers_stop(fl)) return;
-    const uint32_t ncand = *ncand_p;
-    if (c.perm && !fl.x) myers_loop<true>(ncand, c.perm, lens, words, b2, ref_length, e, c);
-    else myers_loop<false>(ncand, nullptr, lens, words, b2, ref_length, e, c);
-}
-
-/* Hamming check of align_reads (:252-287) for one lane with err <= e: the
- * read against the window at the implied start loc - L + 1.  Equal counts
- * give a "<L>M" record; the rest go to the traceback (returns true). */
-__device__ __forceinline__ bool hamming_one(uint32_t i, int32_t er, const uint32_t *__restrict__ lens,
-                                            const uint32_t *__restrict__ words, const uint32_t *__restrict__ b2,
-                                            uint64_t ref_length, uint32_t e, const CandSet &c)
-{
-    const int32_t lc = c.loc[i];
-    const uint32_t rs = c.rs[i];
-    const uint32_t L = lens[rs >> 1];
-    uint32_t ref_len;
-    const uint32_t st = pos_start(ref_length, e, c.val[i], L, &ref_len);
-    const uint32_t *w = words + (uint64_t)rs * 32;
     uint32_t wl[SWM_NW], wh[SWM_NW];
     load_window(b2, st, L, wl, wh);
     const int sl = lc - (int)L + 1;
-    /* branch-free over all 8 words (a mask of 0 past the read), so every
-     * window and read load is issued up front instead of one per branch */
+    /* branch-free over all words (a mask of 0 past the read), so every
+     * read plane load is issued up front */
     int tmp = 0;
 #pragma unroll
     for (int ii = 0; ii < SWM_NW - 1; ii++) {
@@ -2687,6 +2712,65 @@ __device__ __forceinline__ bool hamming_one(uint32_t i, int32_t er, const uint32
     c.rec[i] = 0;
     c.info[i] = INFO_TRACE;
     return true;
+}
+
+template <bool PERM>
+__device__ __forceinline__ void myers_loop(uint32_t ncand, const uint32_t *__restrict__ perm,
+                                           const uint32_t *__restrict__ lens, const uint32_t *__restrict__ words,
+                                           const uint32_t *__restrict__ b2, uint64_t ref_length, uint32_t e,
+                                           const CandSet &c, VerifyTally &vt)
+{
+    for (uint32_t p0 = blockIdx.x * blockDim.x; p0 < ncand; p0 += gridDim.x * blockDim.x) {
+        const uint32_t p = p0 + threadIdx.x;
+        uint32_t tq = ~0u;
+        if (p < ncand) {
+            const uint32_t i = PERM ? perm[p] : p;    /* place p of the verification order */
+            const uint32_t rs = c.rs[i];
+            const uint32_t L = lens[rs >> 1];
+            uint32_t ref_len;
+            const uint32_t st = pos_start(ref_length, e, c.val[i], L, &ref_len);
+            if (st == 0xffffffffu) {
+                c.err[i] = -2;
+                c.loc[i] = -1;
+                vt.rej++;                                /* pos_start rejected the window: not a lane */
+            } else {
+                uint32_t wl[SWM_NW], wh[SWM_NW];
+                load_window(b2, st, L, wl, wh);
+                const uint32_t *w = words + (uint64_t)rs * 32;
+                int32_t er, lc;
+                myers_lane<false>(w, L, e, ref_len, wl, wh, &er, &lc, nullptr, nullptr);
+                c.err[i] = er;
+                c.loc[i] = lc;
+                vt.steps += L;
+                if (er <= (int32_t)e && ham_window(i, er, lc, st, L, w, b2, c)) tq = i;
+            }
+        }
+        vt.round(tq, ~0u);
+    }
+}
+
+/* The verification kernels also run the Hamming check of every passing
+ * candidate (its window is in registers), list the traceback candidates and
+ * count band steps and rejected windows (the run's statistics). */
+#define VT_ARGS uint32_t *__restrict__ trace, uint32_t *__restrict__ trace_count, \
+                unsigned long long *__restrict__ n_steps, uint32_t *__restrict__ n_rej
+#define VT_SETUP                                                                           \
+    __shared__ uint32_t s_list[VT_LIST], s_n, s_base, s_rej;                               \
+    __shared__ unsigned long long s_steps;                                                 \
+    VerifyTally vt{trace, trace_count, n_steps, n_rej, s_list, &s_n, &s_base, &s_rej, &s_steps, 0, 0}; \
+    vt.start();
+
+__global__ void __launch_bounds__(256) k_myers(const uint32_t *__restrict__ ncand_p, const uint32_t *__restrict__ lens,
+                                               const uint32_t *__restrict__ words, const uint32_t *__restrict__ b2,
+                                               uint64_t ref_length, uint32_t e, CandSet c, VT_ARGS)
+{
+    const uint2 fl = myers_flags(c);
+    if (myers_stop(fl)) return;
+    const uint32_t ncand = *ncand_p;
+    VT_SETUP
+    if (c.perm && !fl.x) myers_loop<true>(ncand, c.perm, lens, words, b2, ref_length, e, c, vt);
+    else myers_loop<false>(ncand, nullptr, lens, words, b2, ref_length, e, c, vt);
+    vt.finish();
 }
 
 /* Two candidates per lane for e <= 7: the band (2e+1 <= 15 bits) fits a
@@ -2801,16 +2885,18 @@ __device__ __forceinline__ void myers_pair(const uint32_t *__restrict__ rwa, con
     myers_end_scan(vp >> 16, vn >> 16, (int)L - (int)(ones >> 16), L, e, rlb, eb, lb);
 }
 
-__device__ __forceinline__ void myers_one(uint32_t i, uint32_t st, uint32_t L, uint32_t ref_len,
+__device__ __forceinline__ bool myers_one(uint32_t i, uint32_t st, uint32_t L, uint32_t ref_len,
                                           const uint32_t *__restrict__ words, const uint32_t *__restrict__ b2,
                                           uint32_t e, const CandSet &c)
 {
     uint32_t wl[SWM_NW], wh[SWM_NW];
     load_window(b2, st, L, wl, wh);
+    const uint32_t *w = words + (uint64_t)c.rs[i] * 32;
     int32_t er, lc;
-    myers_lane<false>(words + (uint64_t)c.rs[i] * 32, L, e, ref_len, wl, wh, &er, &lc, nullptr, nullptr);
+    myers_lane<false>(w, L, e, ref_len, wl, wh, &er, &lc, nullptr, nullptr);
     c.err[i] = er;
     c.loc[i] = lc;
+    return er <= (int32_t)e && ham_window(i, er, lc, st, L, w, b2, c);
 }
 
 /* The pair loop, once per order: with the identity a pair's loads do not
@@ -2820,141 +2906,64 @@ template <bool PERM>
 __device__ __forceinline__ void myers2_loop(uint32_t ncand, const uint32_t *__restrict__ perm,
                                             const uint32_t *__restrict__ lens, const uint32_t *__restrict__ words,
                                             const uint32_t *__restrict__ b2, uint64_t ref_length, uint32_t e,
-                                            const CandSet &c)
+                                            const CandSet &c, VerifyTally &vt)
 {
     const uint32_t npair = (ncand + 1) / 2;
-    for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < npair; g += gridDim.x * blockDim.x) {
-        /* places 2g and 2g + 1 of the verification order */
-        const bool has1 = 2 * g + 1 < ncand;
-        const uint32_t i0 = PERM ? perm[2 * g] : 2 * g;
-        const uint32_t i1 = PERM ? (has1 ? perm[2 * g + 1] : 0u) : 2 * g + 1;
-        const uint32_t rs0 = c.rs[i0], rs1 = has1 ? c.rs[i1] : rs0;
-        const uint32_t L0 = lens[rs0 >> 1], L1 = lens[rs1 >> 1];
-        uint32_t rl0, rl1 = 0;
-        const uint32_t st0 = pos_start(ref_length, e, c.val[i0], L0, &rl0);
-        const uint32_t st1 = has1 ? pos_start(ref_length, e, c.val[i1], L1, &rl1) : 0xffffffffu;
-        const bool v0 = st0 != 0xffffffffu, v1 = st1 != 0xffffffffu;
-        if (!v0) { c.err[i0] = -2; c.loc[i0] = -1; }
-        if (has1 && !v1) { c.err[i1] = -2; c.loc[i1] = -1; }
-        const bool packed = v0 && v1 && L0 == L1;
-        /* the shared-read variant when every packed pair of the wave is one
-         * (read, strand): wave-uniform, so no lane runs both */
-        const bool shared = __all(!packed || rs0 == rs1);
-        if (packed) {
-            uint32_t wla[SWM_NW], wha[SWM_NW], wlb[SWM_NW], whb[SWM_NW];
-            load_window(b2, st0, L0, wla, wha);
-            load_window(b2, st1, L1, wlb, whb);
-            int32_t e0, l0, e1, l1;
-            if (shared)
-                myers_pair<true>(words + (uint64_t)rs0 * 32, words + (uint64_t)rs1 * 32, L0, e, rl0, rl1, wla, wha,
-                                 wlb, whb, &e0, &l0, &e1, &l1);
-            else
-                myers_pair<false>(words + (uint64_t)rs0 * 32, words + (uint64_t)rs1 * 32, L0, e, rl0, rl1, wla, wha,
-                                  wlb, whb, &e0, &l0, &e1, &l1);
-            c.err[i0] = e0; c.loc[i0] = l0;
-            c.err[i1] = e1; c.loc[i1] = l1;
-        } else {
-            if (v0) myers_one(i0, st0, L0, rl0, words, b2, e, c);
-            if (v1) myers_one(i1, st1, L1, rl1, words, b2, e, c);
+    for (uint32_t g0 = blockIdx.x * blockDim.x; g0 < npair; g0 += gridDim.x * blockDim.x) {
+        const uint32_t g = g0 + threadIdx.x;
+        uint32_t tq0 = ~0u, tq1 = ~0u;
+        if (g < npair) {
+            /* places 2g and 2g + 1 of the verification order */
+            const bool has1 = 2 * g + 1 < ncand;
+            const uint32_t i0 = PERM ? perm[2 * g] : 2 * g;
+            const uint32_t i1 = PERM ? (has1 ? perm[2 * g + 1] : 0u) : 2 * g + 1;
+            const uint32_t rs0 = c.rs[i0], rs1 = has1 ? c.rs[i1] : rs0;
+            const uint32_t L0 = lens[rs0 >> 1], L1 = lens[rs1 >> 1];
+            uint32_t rl0, rl1 = 0;
+            const uint32_t st0 = pos_start(ref_length, e, c.val[i0], L0, &rl0);
+            const uint32_t st1 = has1 ? pos_start(ref_length, e, c.val[i1], L1, &rl1) : 0xffffffffu;
+            const bool v0 = st0 != 0xffffffffu, v1 = st1 != 0xffffffffu;
+            if (!v0) { c.err[i0] = -2; c.loc[i0] = -1; vt.rej++; }
+            if (has1 && !v1) { c.err[i1] = -2; c.loc[i1] = -1; vt.rej++; }
+            vt.steps += (v0 ? L0 : 0u) + (v1 ? L1 : 0u);
+            const bool packed = v0 && v1 && L0 == L1;
+            /* the shared-read variant when every packed pair of the wave is
+             * one (read, strand): wave-uniform, so no lane runs both */
+            const bool shared = __all(!packed || rs0 == rs1);
+            if (packed) {
+                uint32_t wla[SWM_NW], wha[SWM_NW], wlb[SWM_NW], whb[SWM_NW];
+                load_window(b2, st0, L0, wla, wha);
+                load_window(b2, st1, L1, wlb, whb);
+                const uint32_t *w0 = words + (uint64_t)rs0 * 32, *w1 = words + (uint64_t)rs1 * 32;
+                int32_t e0, l0, e1, l1;
+                if (shared)
+                    myers_pair<true>(w0, w1, L0, e, rl0, rl1, wla, wha, wlb, whb, &e0, &l0, &e1, &l1);
+                else
+                    myers_pair<false>(w0, w1, L0, e, rl0, rl1, wla, wha, wlb, whb, &e0, &l0, &e1, &l1);
+                c.err[i0] = e0; c.loc[i0] = l0;
+                c.err[i1] = e1; c.loc[i1] = l1;
+                if (e0 <= (int32_t)e && ham_window(i0, e0, l0, st0, L0, w0, b2, c)) tq0 = i0;
+                if (e1 <= (int32_t)e && ham_window(i1, e1, l1, st1, L1, w1, b2, c)) tq1 = i1;
+            } else {
+                if (v0 && myers_one(i0, st0, L0, rl0, words, b2, e, c)) tq0 = i0;
+                if (v1 && myers_one(i1, st1, L1, rl1, words, b2, e, c)) tq1 = i1;
+            }
         }
+        vt.round(tq0, tq1);
     }
 }
 
 __global__ void __launch_bounds__(256) k_myers2(const uint32_t *__restrict__ ncand_p, const uint32_t *__restrict__ lens,
                                                 const uint32_t *__restrict__ words, const uint32_t *__restrict__ b2,
-                                                uint64_t ref_length, uint32_t e, CandSet c)
+                                                uint64_t ref_length, uint32_t e, CandSet c, VT_ARGS)
 {
     const uint2 fl = myers_flags(c);
     if (myers_stop(fl)) return;
     const uint32_t ncand = *ncand_p;
-    if (c.perm && !fl.x) myers2_loop<true>(ncand, c.perm, lens, words, b2, ref_length, e, c);
-    else myers2_loop<false>(ncand, nullptr, lens, words, b2, ref_length, e, c);
-}
-
-/* Per candidate: the Hamming check of passing lanes, the traceback list, the
- * count of rejected windows; the band steps as read length x candidates per
- * read less the rejected windows' lengths (only err is read for a window
- * that is not rejected and did not pass), per workgroup.  A
- * fixed grid loops over the candidates (block-uniform trip count); counts
- * stay in registers and the traceback list in an LDS buffer flushed to the
- * global list when nearly full, so a workgroup issues a handful of global
- * atomics in its lifetime instead of a few per 256 candidates (same-address
- * atomics serialise at one L2 channel). */
-#define TRACE_BUF 2048
-#define HAM_GRID_MAX 4096u   /* k_hamming workgroups at most (their count slots) */
-
-__global__ void __launch_bounds__(256) k_hamming(const uint32_t *__restrict__ ncand_p, const uint32_t *__restrict__ lens,
-                                                 const uint32_t *__restrict__ words, const uint32_t *__restrict__ b2,
-                                                 uint64_t ref_length, uint32_t e, CandSet c,
-                                                 uint32_t *__restrict__ trace, uint32_t *__restrict__ trace_count,
-                                                 unsigned long long *__restrict__ hpart,
-                                                 uint32_t nreads, const uint32_t *__restrict__ cand_off)
-{
-    __shared__ uint32_t s_list[TRACE_BUF];
-    __shared__ uint32_t s_n, s_base, s_rej;
-    __shared__ unsigned long long s_steps;
-    if (*c.ovf) return;
-    const uint32_t ncand = *ncand_p;
-    const uint32_t tid = threadIdx.x, lane = __lane_id();
-    const uint64_t below = (1ull << lane) - 1ull;
-    if (tid == 0) { s_n = 0; s_rej = 0; s_steps = 0; }
-    __syncthreads();
-    uint64_t my_steps = 0;
-    uint32_t my_rej = 0;
-    auto flush = [&]() {                       /* block-uniform */
-        const uint32_t n = s_n;
-        if (n) {
-            if (tid == 0) s_base = atomicAdd(trace_count, n);
-            __syncthreads();
-            for (uint32_t k = tid; k < n; k += blockDim.x) trace[s_base + k] = s_list[k];
-            __syncthreads();
-            if (tid == 0) s_n = 0;
-        }
-        __syncthreads();
-    };
-    for (uint32_t b0 = blockIdx.x * blockDim.x; b0 < ncand; b0 += gridDim.x * blockDim.x) {
-        const uint32_t i = b0 + tid;
-        bool to_trace = false;
-        if (i < ncand) {
-            const int32_t er = c.err[i];
-            if (er == -2) {
-                my_rej++;                          /* pos_start rejected the window: not a lane */
-                my_steps -= lens[c.rs[i] >> 1];    /* taken off the per-read total below */
-            } else if (er <= (int32_t)e) {
-                to_trace = hamming_one(i, er, lens, words, b2, ref_length, e, c);
-            }
-        }
-        const uint64_t m = __ballot(to_trace);
-        uint32_t woff = 0;
-        if (lane == 0 && m) woff = atomicAdd(&s_n, (uint32_t)__popcll(m));
-        woff = (uint32_t)__shfl((int)woff, 0, 64);
-        if (to_trace) s_list[woff + (uint32_t)__popcll(m & below)] = i;
-        __syncthreads();
-        if (s_n > TRACE_BUF - 256) flush();
-    }
-    flush();
-    /* band steps: read length x the read's candidates, less the rejected
-     * windows above (u64 arithmetic mod 2^64, exact in the sum) */
-    for (uint32_t rd = blockIdx.x * blockDim.x + tid; rd < nreads; rd += gridDim.x * blockDim.x)
-        my_steps += (uint64_t)lens[rd] * (cand_off[2 * rd + 2] - cand_off[2 * rd]);
-    /* counts: wave sums, then one global atomic each per workgroup */
-#pragma unroll
-    for (int d = 32; d > 0; d >>= 1) {
-        const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)my_steps, d, 64);
-        const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(my_steps >> 32), d, 64);
-        my_steps += ((uint64_t)hi << 32) | lo;
-        my_rej += (uint32_t)__shfl_xor((int)my_rej, d, 64);
-    }
-    if (lane == 0) {
-        if (my_steps) atomicAdd(&s_steps, (unsigned long long)my_steps);
-        if (my_rej) atomicAdd(&s_rej, my_rej);
-    }
-    __syncthreads();
-    /* the workgroup's counts to slots of its own, added up by k_emit_combine */
-    if (tid == 0) {
-        hpart[2 * blockIdx.x] = s_steps;
-        hpart[2 * blockIdx.x + 1] = s_rej;
-    }
+    VT_SETUP
+    if (c.perm && !fl.x) myers2_loop<true>(ncand, c.perm, lens, words, b2, ref_length, e, c, vt);
+    else myers2_loop<false>(ncand, nullptr, lens, words, b2, ref_length, e, c, vt);
+    vt.finish();
 }
 
 /* Stage entry: explicit candidates. */
@@ -3565,35 +3574,24 @@ __global__ void __launch_bounds__(64) k_emit_heavy(const uint32_t *__restrict__ 
 
 /* A heavy read's result from its strands' parts: the counts add up and the
  * best record is the smaller key (distance, strand, order).  Workgroups
- * 0..7 also add up k_emit's per-workgroup record counts and k_hamming's
- * per-workgroup band steps and rejected windows. */
+ * 0..7 also add up k_emit's per-workgroup record counts. */
 __global__ void __launch_bounds__(256) k_emit_combine(const uint32_t *__restrict__ heavy,
                                                      const uint32_t *__restrict__ heavy_count, uint32_t nreads,
                                                      const EmitPart *__restrict__ part,
                                                      swm_read_result_t *__restrict__ out,
                                                      const uint32_t *__restrict__ rpart, uint32_t nrpart,
-                                                     unsigned long long *n_records,
-                                                     const unsigned long long *__restrict__ hpart, uint32_t nhpart,
-                                                     unsigned long long *n_steps, uint32_t *n_rejected)
+                                                     unsigned long long *n_records)
 {
     if (blockIdx.x < 8) {
-        __shared__ unsigned long long s_sum[3];
-        if (threadIdx.x < 3) s_sum[threadIdx.x] = 0;
+        __shared__ unsigned long long s_sum[1];
+        if (threadIdx.x == 0) s_sum[0] = 0;
         __syncthreads();
-        unsigned long long t = 0, ts = 0, tr = 0;
+        unsigned long long t = 0;
         for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < nrpart; k += 8 * blockDim.x) t += rpart[k];
-        for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < nhpart; k += 8 * blockDim.x) {
-            ts += hpart[2 * k];
-            tr += hpart[2 * k + 1];
-        }
         if (t) atomicAdd(&s_sum[0], t);
-        if (ts) atomicAdd(&s_sum[1], ts);
-        if (tr) atomicAdd(&s_sum[2], tr);
         __syncthreads();
         if (threadIdx.x == 0) {
             if (s_sum[0]) atomicAdd(n_records, s_sum[0]);
-            if (s_sum[1]) atomicAdd(n_steps, s_sum[1]);
-            if (s_sum[2]) atomicAdd(n_rejected, (uint32_t)s_sum[2]);
         }
     }
     const uint32_t ng = heavy_count[0], nh = ng + heavy_count[1];
@@ -3892,7 +3890,7 @@ struct swm_ws {
     struct Buf { void *p = nullptr; size_t cap = 0; };
     enum { WORDS, SEL, SELC, NSEL, NPOS, NCHUNK, SURV_OFF, BD_OFF, SURV, BDESC, NBATCH, NUNIQ, CAND_OFF, C_VAL, C_RS, C_BI,
            C_ERR, C_LOC, C_REC, C_INFO, COUNTERS, BIG, TRACE, HEAVY, HEAVY_C, HEAVY_E, SCAN_TMP, IN_READS, IN_LENS,
-           OUT, HIST, R_WR, R_TB, R_RUNS, R_IDX, R_TOFF, R_SCAN, R_TOT, R_OUT, R_TEXT, R_BLOCK, CDESC, HBATCH, GIANT, EPART, RPART, HPART, LBINS, PERM, IRREG, MERGE, NBUF };
+           OUT, HIST, R_WR, R_TB, R_RUNS, R_IDX, R_TOFF, R_SCAN, R_TOT, R_OUT, R_TEXT, R_BLOCK, CDESC, HBATCH, GIANT, EPART, RPART, LBINS, PERM, IRREG, MERGE, NBUF };
     Buf b[NBUF];
     uint64_t gen = 1;
     ~swm_ws() { for (auto &x : b) if (x.p) (void)hipFree(x.p); }
@@ -4064,7 +4062,6 @@ struct Pipe {
     uint32_t *bd_off = nullptr, *cand_off = nullptr, *trace = nullptr, *heavy_c = nullptr, *heavy_e = nullptr;
     EmitPart *epart = nullptr;
     uint32_t *rpart = nullptr;
-    unsigned long long *hpart = nullptr;
     unsigned long long *counters = nullptr;
     uint32_t *u32c = nullptr;
     uint64_t *scan_tmp = nullptr;
@@ -4097,7 +4094,6 @@ static int pipe_alloc_a(Alloc *ws, uint64_t n, uint32_t seln, Pipe &P)
         (rc = ws->get(swm_ws::HEAVY_C, &P.heavy_c, nrs)) || (rc = ws->get(swm_ws::HEAVY_E, &P.heavy_e, 2 * n)) ||
         (rc = ws->get(swm_ws::EPART, &P.epart, 2 * n)) ||
         (rc = ws->get(swm_ws::RPART, &P.rpart, (uint64_t)blocks_for(n, 128))) ||
-        (rc = ws->get(swm_ws::HPART, &P.hpart, 2 * (uint64_t)HAM_GRID_MAX)) ||
         (rc = ws->get(swm_ws::LBINS, &P.lbins, (LBIN_ROWS + 1) * 256u)) ||
         (rc = ws->get(swm_ws::SCAN_TMP, &P.scan_tmp, 2 * (uint64_t)blocks_for(nrs + 1, SCAN_TILE))))
         return rc;
@@ -4284,19 +4280,16 @@ static int enq_c(const swm_ctx_t *ctx, Pipe &P, const uint32_t *d_lens, uint32_t
     if (o.myers_packed && e <= SWM_MAX_E_PACKED)
         k_myers2<<<blocks_for((grid_cand ? grid_cand : 1) / 2 + 1, mb), mb, 0, st>>>(ncand, d_lens, P.so.words,
                                                                                     ctx->d_base_2bits,
-                                                                                    ctx->ref_length, e, P.cs);
+                                                                                    ctx->ref_length, e, P.cs, P.trace,
+                                                                                    P.u32c + 1, P.counters + 2,
+                                                                                    P.u32c + 2);
     else
         k_myers<<<blocks_for(grid_cand ? grid_cand : 1, mb), mb, 0, st>>>(ncand, d_lens, P.so.words,
                                                                           ctx->d_base_2bits, ctx->ref_length, e,
-                                                                          P.cs);
+                                                                          P.cs, P.trace, P.u32c + 1, P.counters + 2,
+                                                                          P.u32c + 2);
     SWM_HIP_CHECK(hipGetLastError());
     if (ev) SWM_HIP_CHECK(hipEventRecord(ev[4], st));
-    const uint32_t ham_grid = std::min<uint32_t>(std::min<uint32_t>(blocks_for(grid_cand ? grid_cand : 1, 256),
-                                                                     (uint32_t)ctx->n_cu * 8), HAM_GRID_MAX);
-    k_hamming<<<ham_grid, 256, 0, st>>>(ncand, d_lens, P.so.words,
-                                                                          ctx->d_base_2bits, ctx->ref_length, e, P.cs,
-                                                                          P.trace, P.u32c + 1, P.hpart,
-                                                                          (uint32_t)P.n, P.cand_off);
     /* persistent: 8 waves per SIMD, the traceback chains are latency-bound */
     (P.cs.runs ? k_trace<true> : k_trace<false>)<<<ctx->n_cu * 32, 64, 0, st>>>(P.trace, P.u32c + 1, d_lens, P.so.words, ctx->d_base_2bits, ctx->ref_length, e,
                                           P.cs);
@@ -4306,8 +4299,7 @@ static int enq_c(const swm_ctx_t *ctx, Pipe &P, const uint32_t *d_lens, uint32_t
         (P.cs.wr ? k_emit_heavy<true> : k_emit_heavy<false>)<<<ctx->n_cu * o.emit_heavy_waves, 64, 0, st>>>(
             P.heavy_e, P.u32c + 10, (uint32_t)P.n, e, P.cand_off, P.cs, P.epart, P.counters + 1);
     k_emit_combine<<<ctx->n_cu * 2, 256, 0, st>>>(P.heavy_e, P.u32c + 10, (uint32_t)P.n, P.epart, d_out, P.rpart,
-                                                  blocks_for(P.n, 128), P.counters + 1, P.hpart, ham_grid,
-                                                  P.counters + 2, P.u32c + 2);
+                                                  blocks_for(P.n, 128), P.counters + 1);
     SWM_HIP_CHECK(hipGetLastError());
     if (ev) SWM_HIP_CHECK(hipEventRecord(ev[5], st));
     return SWM_OK;
