@@ -213,7 +213,9 @@ def rooflines(swm, st_all, lens, config, form):
                       "peak_probe_tops": round(measured / 1e12, 2),
                       "peak_64lane_tops": round(VALU_PEAK_OPS / 1e12, 2),
                       "issue": valu_issue(config, "k_myers2"),
-                      "note": "achieved counts 20 algorithmic int32 ops per band step of each candidate; with "
+                      "note": "the kernel also runs the Hamming check of each passing candidate (its window "
+                              "reloaded from the cache), which the count leaves out; "
+                              "achieved counts 20 algorithmic int32 ops per band step of each candidate; with "
                               "equal read lengths two candidates share a lane (16-bit halves) and gfx950's "
                               "3-input boolean ops fuse pairs, so frac is not capped at 1; 'issue' is the "
                               "kernel's measured VALU instruction issue fraction (rocprofv3 counters of one "
