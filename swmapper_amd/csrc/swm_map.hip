@@ -3110,10 +3110,10 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) k_
              * (a match or substitution moves both, an insertion moves i
              * and ss, a deletion match_site and ss), so those window bases
              * are the 32 starting at chunk offset ss.  ss moves only on an
-             * indel; then the chunk's steps are recomputed from its
-             * checkpoint for the new masks.  A run of matches (D0 set,
-             * bases equal) is taken in one step: each adds a base to the
-             * pending "M" run and nothing else. */
+             * indel; then the masks are rebuilt from the chunk's D0/HP
+             * registers.  A run of matches (D0 set, bases equal) is taken
+             * in one step: each adds a base to the pending "M" run and
+             * nothing else. */
             const uint32_t rdl = even_bits(r0) | (even_bits(r1) << 16);
             const uint32_t rdh = even_bits(r0 >> 1) | (even_bits(r1 >> 1) << 16);
             const bool far = e > 10u;                   /* ss can pass 31 */
