@@ -145,3 +145,24 @@ def test_autotune_keeps_results(tandem):
     got = m.map(r, l, 5)
     _assert_same(got, exp[5][0], "autotuned")
     m.close()
+
+
+def test_option_ranges(tandem):
+    """Every option's documented range: values outside it are refused
+    (SWM_EINVAL) and leave the option as it was; the defaults read back."""
+    gi, _, _, _ = tandem
+    m = swm.Mapper.from_index(gi)
+    defaults = m.options()
+    assert defaults["plan_overlap"] == 1 and defaults["plan_light_waves"] == 32, defaults
+    bad = {swm.OPT_PLAN_OVERLAP: [2, -1], swm.OPT_PLAN_LIGHT_WAVES: [0, 7, 65],
+           swm.OPT_PLAN_HEAVY_WAVES: [0, 33], swm.OPT_SEED_GENERIC: [2]}
+    for opt, vals in bad.items():
+        for v in vals:
+            with pytest.raises(swm.SwmError) as ex:
+                m.set_option(opt, v)
+            assert ex.value.rc == -1, (swm.OPTION_NAMES[opt], v)
+    assert m.options() == defaults
+    for opt, v in ((swm.OPT_PLAN_OVERLAP, 0), (swm.OPT_PLAN_LIGHT_WAVES, 8), (swm.OPT_PLAN_LIGHT_WAVES, 64)):
+        m.set_option(opt, v)
+        assert m.get_option(opt) == v
+    m.close()
