@@ -3279,6 +3279,7 @@ __device__ __forceinline__ void record_partial(uint32_t count, uint32_t *part, u
     if (threadIdx.x == 0) part[blockIdx.x] = *s_sum;
 }
 
+#define EMIT_LOAD 4
 __global__ void __launch_bounds__(128) k_emit(uint32_t nreads, uint32_t e, const uint32_t *__restrict__ cand_off,
                                               CandSet c, swm_read_result_t *__restrict__ out,
                                               uint32_t *__restrict__ rpart, uint32_t heavy_cands,
@@ -3305,14 +3306,33 @@ __global__ void __launch_bounds__(128) k_emit(uint32_t nreads, uint32_t e, const
             const uint32_t rs = 2 * rd + strand;
             em.start_strand();
             const uint32_t cend = cand_off[rs + 1];
-            for (uint32_t ci = cand_off[rs]; ci < cend; ci++) {
-                const int32_t er = c.err[ci];
-                const bool pass = er >= 0 && er <= (int32_t)e;
-                const uint32_t info = pass ? c.info[ci] : 0u;
-                const uint64_t w = em.step(c.bi[ci], er, pass ? c.rec[ci] : 0u, info, e, strand);
-                if (c.wr) {
-                    c.wr[ci] = w;
-                    c.tb[ci] = w ? (info & 0xffffu) - 2u : 0u;
+            /* four candidates' fields loaded together (a read has about
+             * four), then stepped through in order */
+            for (uint32_t c0 = cand_off[rs]; c0 < cend; c0 += EMIT_LOAD) {
+                int32_t erq[EMIT_LOAD];
+                uint32_t biq[EMIT_LOAD], recq[EMIT_LOAD], infoq[EMIT_LOAD];
+#pragma unroll
+                for (int k = 0; k < EMIT_LOAD; k++) {
+                    const uint32_t ci = c0 + (uint32_t)k;
+                    const bool in = ci < cend;
+                    erq[k] = in ? c.err[ci] : -1;
+                    biq[k] = in ? c.bi[ci] : 0u;
+                    recq[k] = in ? c.rec[ci] : 0u;
+                    infoq[k] = in ? c.info[ci] : 0u;
+                }
+#pragma unroll
+                for (int k = 0; k < EMIT_LOAD; k++) {
+                    const uint32_t ci = c0 + (uint32_t)k;
+                    if (ci < cend) {
+                        const int32_t er = erq[k];
+                        const bool pass = er >= 0 && er <= (int32_t)e;
+                        const uint32_t info = pass ? infoq[k] : 0u;
+                        const uint64_t w = em.step(biq[k], er, pass ? recq[k] : 0u, info, e, strand);
+                        if (c.wr) {
+                            c.wr[ci] = w;
+                            c.tb[ci] = w ? (info & 0xffffu) - 2u : 0u;
+                        }
+                    }
                 }
             }
             em.end_strand();
