@@ -719,6 +719,40 @@ __device__ __forceinline__ void shift_down(T (&a)[N], uint32_t sh)
     }
 }
 
+/* Store half h (words 16 h .. 16 h + 15) of a wave's 64 rows of 32 words,
+ * lane l's row at base + 32 l words, through 2 KB of LDS (stage, 16-byte
+ * aligned): 32 rows at a time, each row's four 16-byte pieces rotated by a
+ * quarter of its row index so that the writing lanes hit distinct banks,
+ * then read back in memory order, each store instruction covering 16 rows'
+ * halves where a store per lane would touch 64 lines. */
+__device__ __forceinline__ void store_half_rows(uint32_t *base, uint32_t h, const uint32_t (&v)[16], uint8_t *stage)
+{
+    const uint32_t lane = threadIdx.x & 63u;
+    uint4 *st4 = reinterpret_cast<uint4 *>(stage);
+    uint4 *g4 = reinterpret_cast<uint4 *>(base);
+#pragma unroll
+    for (uint32_t k = 0; k < 2; k++) {
+        if ((lane >> 5) == k) {
+            const uint32_t row = lane & 31u;
+#pragma unroll
+            for (uint32_t j = 0; j < 4; j++)
+                st4[row * 4 + ((j + (row >> 2)) & 3u)] = make_uint4(v[4 * j], v[4 * j + 1], v[4 * j + 2], v[4 * j + 3]);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        /* pieces lane and 64 + lane: piece c & 3 of row 32 k + (c >> 2) */
+        const uint32_t r0 = lane >> 2, r1 = 16u + (lane >> 2), pc = lane & 3u;
+        const uint4 a0 = st4[r0 * 4 + ((pc + (r0 >> 2)) & 3u)];
+        const uint4 a1 = st4[r1 * 4 + ((pc + (r1 >> 2)) & 3u)];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        g4[(32 * k + r0) * 8 + 4 * h + pc] = a0;
+        g4[(32 * k + r1) * 8 + 4 * h + pc] = a1;
+    }
+}
+
 template <uint32_t K>
 __global__ void __launch_bounds__(128) k_seed_fixed(const char *__restrict__ reads, uint64_t stride,
                                                     const uint32_t *__restrict__ lens, uint32_t nreads,
@@ -795,17 +829,44 @@ __global__ void __launch_bounds__(128) k_seed_fixed(const char *__restrict__ rea
         pw[j] = ((cw[j] & ~iv[j]) | (0x55555555u & iv[j])) & lm;
     }
     pw[16] = pw[17] = 0;
-    uint32_t *wo = o.words + (uint64_t)r * 32;
+    /* the wave's 64 rows are consecutive (r = 2 read + strand): a full wave
+     * stages them through its own LDS rows, whose read codes are in
+     * registers by now; a partial wave (the last workgroup's) writes its
+     * rows directly */
+    const bool full_wave = __ballot(1) == ~0ull;
+    uint32_t *wrow = o.words + (uint64_t)r * 32;
+    uint32_t *wbase = o.words + (uint64_t)(r & ~63u) * 32;
+    uint8_t *stage = s_q + (threadIdx.x >> 6) * 32u * SEED_ROW;
+    {
+        uint32_t wv[16];
 #pragma unroll
-    for (int j = 0; j < 16; j++)
-        wo[j] = (pw[j] & rec->mask[j][0]) | (pw[j + 1] & rec->mask[j][1]) | (pw[j + 2] & rec->mask[j][2]);
+        for (int j = 0; j < 16; j++)
+            wv[j] = (pw[j] & rec->mask[j][0]) | (pw[j + 1] & rec->mask[j][1]) | (pw[j + 2] & rec->mask[j][2]);
+        if (full_wave)
+            store_half_rows(wbase, 0, wv, stage);
+        else {
 #pragma unroll
-    for (int j = 0; j < 8; j++) {
-        const int rem = (int)n_planes - 32 * j;
-        const uint32_t pm = rem >= 32 ? 0xffffffffu : (rem <= 0 ? 0u : (1u << rem) - 1u);
-        const uint32_t lo = pw[2 * j], hi = pw[2 * j + 1];
-        wo[16 + 2 * j] = (even_bits(lo) | (even_bits(hi) << 16)) & pm;
-        wo[17 + 2 * j] = (even_bits(lo >> 1) | (even_bits(hi >> 1) << 16)) & pm;
+            for (int j = 0; j < 4; j++)
+                reinterpret_cast<uint4 *>(wrow)[j] = make_uint4(wv[4 * j], wv[4 * j + 1], wv[4 * j + 2], wv[4 * j + 3]);
+        }
+    }
+    {
+        uint32_t wv[16];
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const int rem = (int)n_planes - 32 * j;
+            const uint32_t pm = rem >= 32 ? 0xffffffffu : (rem <= 0 ? 0u : (1u << rem) - 1u);
+            const uint32_t lo = pw[2 * j], hi = pw[2 * j + 1];
+            wv[2 * j] = (even_bits(lo) | (even_bits(hi) << 16)) & pm;
+            wv[2 * j + 1] = (even_bits(lo >> 1) | (even_bits(hi >> 1) << 16)) & pm;
+        }
+        if (full_wave)
+            store_half_rows(wbase, 1, wv, stage);
+        else {
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+                reinterpret_cast<uint4 *>(wrow)[4 + j] = make_uint4(wv[4 * j], wv[4 * j + 1], wv[4 * j + 2], wv[4 * j + 3]);
+        }
     }
 
     /* per long seed m: its seed string (the 15 bases as the codes' source,
@@ -944,12 +1005,14 @@ __global__ void __launch_bounds__(128) k_seed_fixed(const char *__restrict__ rea
                     cnt4[q] = v.y;
                 }
             }
+            uint32_t sv[8];
 #pragma unroll
             for (int q = 0; q < 4; q++) {
                 const uint32_t rb = (pk[jm] >> (4 + 6 * q)) & 63u;
                 const uint32_t off = off4[q] & o.off_mask, cnt = cnt4[q];
                 const uint32_t meta = (base + (uint32_t)q) | (rb << 8) | ((uint32_t)q << 14);
-                so[4 * jm + q] = swm_sel_t{off, (min(cnt, SEL_BIG) << 16) | meta};
+                sv[2 * q] = off;
+                sv[2 * q + 1] = (min(cnt, SEL_BIG) << 16) | meta;
                 if (cnt >= SEL_BIG) o.selc[(uint64_t)r * seln + 4 * jm + q] = cnt;
                 heavy |= cnt > SWM_CHUNK;
                 if (cnt > 1) {
@@ -959,6 +1022,11 @@ __global__ void __launch_bounds__(128) k_seed_fixed(const char *__restrict__ rea
                 npos += cnt;
                 nchunk += (cnt + SWM_CHUNK - 1) / SWM_CHUNK;
             }
+            /* a long seed's four selections are 32 bytes at a 32-byte
+             * boundary (rows of 4 (e + 1) selections): two 16-byte stores */
+            uint4 *so4 = reinterpret_cast<uint4 *>(so + 4 * jm);
+            so4[0] = make_uint4(sv[0], sv[1], sv[2], sv[3]);
+            so4[1] = make_uint4(sv[4], sv[5], sv[6], sv[7]);
         }
     }
     o.nsel[r] = n_sel_long * SWM_SEED_STEP | (heavy ? NSEL_HEAVY : 0u) | (irr ? NSEL_IRREG : 0u);
