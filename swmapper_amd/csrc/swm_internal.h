@@ -74,6 +74,7 @@ struct swm_ctx {
                                          * one line (8 or 16 x 4^(k-1) entries) */
     uint2 *d_pl = nullptr;              /* bucket entries: (position, 12-bit residual) */
     uint64_t n_pos = 0;
+    uint64_t n_dev_pos = 0;     /* entries of the device bucket array (pairs packed into lines) */
     uint32_t *d_base_2bits = nullptr;   /* file words + SWM_WIN_PAIRS_PAD zero pairs */
     uint64_t n_2bits_words = 0;
     uint64_t ref_length = 0;            /* 16 * bytes(.base_2bits) (mapper_mpe.c:288) */

@@ -366,16 +366,16 @@ __global__ void __launch_bounds__(256) k_seed_table(const uint32_t *__restrict__
 }
 
 /* The device bucket array: bucket c copied from the file's order (src at
- * off[c]) to its place next to its reverse complement's (dst at dev_off[c]),
- * entries past n_pos on either side left out. */
+ * off[c], n_pos entries) to its place next to its reverse complement's (dst
+ * at dev_off[c], n_dev entries), entries past either end left out. */
 __global__ void __launch_bounds__(256) k_pl_permute(const uint32_t *__restrict__ off, const uint32_t *__restrict__ dev_off,
-                                                    uint64_t n_buckets, uint64_t n_pos, const uint2 *__restrict__ src,
-                                                    uint2 *__restrict__ dst)
+                                                    uint64_t n_buckets, uint64_t n_pos, uint64_t n_dev,
+                                                    const uint2 *__restrict__ src, uint2 *__restrict__ dst)
 {
     const uint64_t c = (uint64_t)blockIdx.x * 256u + threadIdx.x;
     if (c >= n_buckets) return;
     const uint64_t a = off[c], cnt = (uint32_t)(off[c + 1] - off[c]), d = dev_off[c] & 0x7fffffffu;
-    for (uint64_t j = 0; j < cnt && a + j < n_pos && d + j < n_pos; j++) dst[d + j] = src[a + j];
+    for (uint64_t j = 0; j < cnt && a + j < n_pos && d + j < n_dev; j++) dst[d + j] = src[a + j];
 }
 
 /* Work of every seed kernel before its per-thread part: workgroup 0 zeroes
@@ -4865,6 +4865,67 @@ int swm_slot_run_host(swm_ctx_t *ctx, swm_slot *sl, const char *reads, uint64_t 
 
 /* The device keeps each bucket entry as one 8-byte (position, residual)
  * pair, so a bucket chunk costs one random cache line, not two. */
+/* Places of the buckets in the device array (d_order[c], in entries of 8
+ * bytes): each k-mer's bucket directly before its reverse complement's (a
+ * pair; a reverse-palindromic k-mer's bucket alone), every pair of at most 16
+ * entries inside one 128-byte line, so that a fetch of a pair reads one line
+ * where pairs laid back to back straddle two about a third of the time.
+ * Best fit decreasing: pairs over 16 entries start on a line of their own,
+ * then the others, largest first, go into the fullest line with room for
+ * them.  Empty pairs point at entry 0.  Returns the array's length in
+ * entries, or 0 when it would reach `limit` (the offsets must stay below). */
+static uint64_t pair_layout(const uint32_t *hash_count, uint64_t n_buckets, uint32_t k, uint32_t *d_order,
+                            uint64_t limit)
+{
+    std::vector<uint32_t> by_size[17];
+    std::vector<uint32_t> big;
+    for (uint64_t i = 0; i < n_buckets; i++) {
+        const uint32_t rci = rc_bases((uint32_t)i, k);
+        if (rci < i) continue;
+        const uint64_t n = (uint64_t)hash_count[i] + (rci != i ? hash_count[rci] : 0u);
+        if (n == 0) {
+            d_order[i] = 0;
+            d_order[rci] = 0;
+        } else if (n > 16) {
+            big.push_back((uint32_t)i);
+        } else {
+            by_size[n].push_back((uint32_t)i);
+        }
+    }
+    auto place = [&](uint32_t i, uint64_t at) {
+        const uint32_t rci = rc_bases(i, k);
+        d_order[i] = (uint32_t)at;
+        if (rci != i) d_order[rci] = (uint32_t)(at + hash_count[i]);
+    };
+    std::vector<uint64_t> open[16];     /* open[f]: lines with f free entries at their end */
+    uint64_t lines = 0;
+    for (uint32_t i : big) {
+        const uint32_t rci = rc_bases(i, k);
+        const uint64_t n = (uint64_t)hash_count[i] + (rci != i ? hash_count[rci] : 0u);
+        if ((lines + (n + 15) / 16) * 16 >= limit) return 0;
+        place(i, lines * 16);
+        lines += (n + 15) / 16;
+        if (n % 16) open[16 - n % 16].push_back(lines - 1);
+    }
+    for (uint32_t n = 16; n >= 1; n--) {
+        for (uint32_t i : by_size[n]) {
+            uint32_t f = n;
+            while (f < 16 && open[f].empty()) f++;
+            uint64_t line;
+            if (f < 16) {
+                line = open[f].back();
+                open[f].pop_back();
+            } else {
+                if ((lines + 1) * 16 >= limit) return 0;
+                line = lines++;
+            }
+            place(i, line * 16 + (16 - f));
+            if (f > n) open[f - n].push_back(line);
+        }
+    }
+    return lines * 16;
+}
+
 static int upload_pl(uint2 *d, const uint32_t *pos, const uint16_t *ls, uint64_t n,
                      const std::vector<uint64_t> &descents)
 {
@@ -4943,7 +5004,9 @@ extern "C" int swm_ctx_create(uint32_t k, const uint32_t *hash_count, const uint
      * complements, so both fetch the same lines */
     uint32_t *d_order = (uint32_t *)malloc(c->n_buckets * sizeof(uint32_t));
     if (!d_order) { free(h_off); free(rec); swm_ctx_destroy(c); return SWM_ENOMEM; }
-    {
+    uint64_t n_dev = pair_layout(hash_count, c->n_buckets, k, d_order, below31 ? 0x80000000ull : 0xffffff00ull);
+    if (!n_dev) {
+        /* the packed layout would not fit the offsets: pairs back to back */
         uint32_t acc = 0;
         for (uint64_t i = 0; i < c->n_buckets; i++) {
             const uint32_t rci = rc_bases((uint32_t)i, k);
@@ -4952,7 +5015,9 @@ extern "C" int swm_ctx_create(uint32_t k, const uint32_t *hash_count, const uint
             acc += hash_count[i];
             if (rci != i) { d_order[rci] = acc; acc += hash_count[rci]; }
         }
+        n_dev = n_pos;
     }
+    c->n_dev_pos = n_dev;
     for (uint32_t &x : irr) x = 0;
     for (size_t q = 0; q < irr_bucket.size(); q++) irr[q] = d_order[irr_bucket[q]];
     std::sort(irr.begin(), irr.end());
@@ -4967,7 +5032,7 @@ extern "C" int swm_ctx_create(uint32_t k, const uint32_t *hash_count, const uint
         hipMalloc((void **)&d_dev_off, c->n_buckets * 4) != hipSuccess ||
         hipMalloc((void **)&d_pl_file, (n_pos + 1) * 8) != hipSuccess ||
         hipMalloc((void **)&c->d_seed_tab, tab_bytes) != hipSuccess ||
-        hipMalloc((void **)&c->d_pl, (n_pos + 64) * 8) != hipSuccess ||
+        hipMalloc((void **)&c->d_pl, (n_dev + 64) * 8) != hipSuccess ||
         hipMalloc((void **)&c->d_base_2bits, (n_2bits_words + 2 * SWM_WIN_PAIRS_PAD) * 4) != hipSuccess ||
         hipMalloc((void **)&c->d_recipe, rec_bytes) != hipSuccess ||
         (!irr.empty() && hipMalloc((void **)&c->d_irreg, irr.size() * 4) != hipSuccess))
@@ -4976,7 +5041,7 @@ extern "C" int swm_ctx_create(uint32_t k, const uint32_t *hash_count, const uint
         if (hipMemcpy(d_off, h_off, (c->n_buckets + 1) * 4, hipMemcpyHostToDevice) != hipSuccess ||
             hipMemcpy(d_dev_off, d_order, c->n_buckets * 4, hipMemcpyHostToDevice) != hipSuccess ||
             hipMemset(c->d_seed_tab, 0, tab_bytes) != hipSuccess ||
-            hipMemset(c->d_pl, 0, (n_pos + 64) * 8) != hipSuccess ||
+            hipMemset(c->d_pl, 0, (n_dev + 64) * 8) != hipSuccess ||
             (n_pos && upload_pl(d_pl_file, positions, pos_lseeds, n_pos, descents) != SWM_OK) ||
             hipMemset(c->d_base_2bits, 0, (n_2bits_words + 2 * SWM_WIN_PAIRS_PAD) * 4) != hipSuccess ||
             hipMemcpy(c->d_base_2bits, base_2bits, n_2bits_words * 4, hipMemcpyHostToDevice) != hipSuccess ||
@@ -4988,7 +5053,7 @@ extern "C" int swm_ctx_create(uint32_t k, const uint32_t *hash_count, const uint
     if (rc == SWM_OK) {
         const unsigned nb = (unsigned)((c->n_buckets + 255) / 256);
         k_seed_table<<<nb, 256>>>(d_off, d_dev_off, c->n_buckets, k, c->d_seed_tab);
-        if (n_pos) k_pl_permute<<<nb, 256>>>(d_off, d_dev_off, c->n_buckets, n_pos, d_pl_file, c->d_pl);
+        if (n_pos) k_pl_permute<<<nb, 256>>>(d_off, d_dev_off, c->n_buckets, n_pos, n_dev, d_pl_file, c->d_pl);
         if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) rc = SWM_EDEVICE;
     }
     if (d_off) (void)hipFree(d_off);
