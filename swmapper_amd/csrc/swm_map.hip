@@ -509,12 +509,19 @@ __global__ void __launch_bounds__(128) k_seed(const char *__restrict__ reads, ui
         }
         pw[j] = v;
     }
-    uint32_t *wo = o.words + (uint64_t)r * 32;
+    /* the row in 16-byte stores (the read's LDS row is read again below, so
+     * it cannot stage the wave's rows as k_seed_fixed does) */
+    uint4 *wo = reinterpret_cast<uint4 *>(o.words + (uint64_t)r * 32);
+    {
+        uint32_t wv[16];
 #pragma unroll
-    for (int j = 0; j < 16; j++)
-        wo[j] = (pw[j] & rec->mask[j][0]) | (pw[j + 1] & rec->mask[j][1]) | (pw[j + 2] & rec->mask[j][2]);
+        for (int j = 0; j < 16; j++)
+            wv[j] = (pw[j] & rec->mask[j][0]) | (pw[j + 1] & rec->mask[j][1]) | (pw[j + 2] & rec->mask[j][2]);
 #pragma unroll
-    for (int j = 0; j < 8; j++) { wo[16 + 2 * j] = pl[j]; wo[17 + 2 * j] = ph[j]; }
+        for (int j = 0; j < 4; j++) wo[j] = make_uint4(wv[4 * j], wv[4 * j + 1], wv[4 * j + 2], wv[4 * j + 3]);
+    }
+#pragma unroll
+    for (int j = 0; j < 4; j++) wo[4 + j] = make_uint4(pl[2 * j], ph[2 * j], pl[2 * j + 1], ph[2 * j + 1]);
 
     /* 12-mer codes and seed-table gathers, long seed by long seed
      * (choose_seed :1372-1390).  The reverse strand's 12-mer at 15m + q is
@@ -1510,9 +1517,16 @@ __global__ void __launch_bounds__(64) k_plan_pair(uint32_t nreads, const uint2 *
     uint64_t fetched = 0;
     uint32_t nsw_n = 0, nsw_nb = 0;
     swm_sel_t my_n = {0, 0}, my_nb = {0, 0};
-    /* the next read's selection counts and selections are loaded while this
-     * one is processed */
+    uint64_t so_n0 = 0, so_n1 = 0;
+    uint32_t bo_n0 = 0, bo_n1 = 0;
+    /* the next read's selection counts, selections and output offsets are
+     * loaded while this one is processed: the descriptor offsets otherwise
+     * put a round trip between the survivors' dedup and its stores */
     auto prefetch = [&](uint32_t rd) {
+        so_n0 = io.surv_off[2u * rd];
+        so_n1 = io.surv_off[2u * rd + 1u];
+        bo_n0 = io.bd_off[2u * rd];
+        bo_n1 = io.bd_off[2u * rd + 1u];
         if (W == 32) {
             const uint32_t r = 2u * rd + half;
             nsw_n = io.nsel[r];
@@ -1528,6 +1542,8 @@ __global__ void __launch_bounds__(64) k_plan_pair(uint32_t nreads, const uint2 *
     for (uint32_t rd = blockIdx.x; rd < nreads; rd += gridDim.x) {
         const uint32_t nsw = nsw_n, nswb = nsw_nb;
         swm_sel_t my = my_n, myb = my_nb;
+        const uint64_t so0 = so_n0, so1 = so_n1;
+        const uint32_t bo0 = bo_n0, bo1 = bo_n1;
         if (rd + gridDim.x < nreads) prefetch(rd + gridDim.x);
         bool light0, light1;
         uint32_t total, total0;
@@ -1563,7 +1579,7 @@ __global__ void __launch_bounds__(64) k_plan_pair(uint32_t nreads, const uint2 *
         }
         fetched += total;
         __syncthreads();
-        uint32_t *surv0 = io.surv + io.surv_off[2u * rd], *surv1 = io.surv + io.surv_off[2u * rd + 1u];
+        uint32_t *surv0 = io.surv + so0, *surv1 = io.surv + so1;
         uint32_t ns0 = 0, ns1 = 0;
         for (uint32_t e0 = 0; e0 < total; e0 += 64u * PP_ROUNDS) {
             uint32_t lo[PP_ROUNDS], meta[PP_ROUNDS];
@@ -1610,7 +1626,7 @@ __global__ void __launch_bounds__(64) k_plan_pair(uint32_t nreads, const uint2 *
             const uint32_t r = 2u * rd + h;
             const uint32_t nsurv = h ? ns1 : ns0;
             uint32_t *surv = h ? surv1 : surv0;
-            uint2 *bd = io.bdesc + io.bd_off[r];
+            uint2 *bd = io.bdesc + (h ? bo1 : bo0);
             if (nsurv <= 64u) {
                 const bool have = lane < nsurv;
                 const uint32_t v = have ? s_surv[h][lane] : 0xffffffffu;
