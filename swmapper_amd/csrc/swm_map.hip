@@ -760,6 +760,35 @@ __device__ __forceinline__ void store_half_rows(uint32_t *base, uint32_t h, cons
     }
 }
 
+/* Store a wave's 64 selection rows of P 16-byte pieces (P <= 16), lane l's
+ * row at base + P l pieces, through the same 2 KB of LDS: 8 rows at a time,
+ * read back in memory order, each store instruction covering 1 KB. */
+__device__ __forceinline__ void store_sel_rows(uint4 *base, uint32_t P, const uint4 (&v)[16], uint8_t *stage)
+{
+    const uint32_t lane = threadIdx.x & 63u;
+    uint4 *st4 = reinterpret_cast<uint4 *>(stage);
+#pragma unroll 1
+    for (uint32_t k = 0; k < 8; k++) {
+        if ((lane >> 3) == k) {
+            const uint32_t row = lane & 7u;
+#pragma unroll
+            for (uint32_t j = 0; j < 16; j++)
+                if (j < P) st4[row * P + j] = v[j];
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const uint32_t n = 8 * P;                            /* pieces of the 8 rows */
+        const uint4 a0 = st4[lane < n ? lane : 0u];
+        const uint4 a1 = st4[64 + lane < n ? 64 + lane : 0u];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (lane < n) base[k * n + lane] = a0;
+        if (64 + lane < n) base[k * n + 64 + lane] = a1;
+    }
+}
+
 template <uint32_t K>
 __global__ void __launch_bounds__(128) k_seed_fixed(const char *__restrict__ reads, uint64_t stride,
                                                     const uint32_t *__restrict__ lens, uint32_t nreads,
@@ -989,6 +1018,13 @@ __global__ void __launch_bounds__(128) k_seed_fixed(const char *__restrict__ rea
     uint32_t nchunk = 0;
     bool heavy = false, irr = false;
     swm_sel_t *so = o.sel + (uint64_t)r * seln;
+    /* rows of at most 32 selections (e <= 7) of a full wave are kept in
+     * registers and staged like the words (64 consecutive rows); entries
+     * past a read's own selections are written as zeros there */
+    const bool stage_sel = full_wave && seln <= 32u;
+    uint4 selv[16];
+#pragma unroll
+    for (int j = 0; j < 16; j++) selv[j] = make_uint4(0u, 0u, 0u, 0u);
 #pragma unroll
     for (int jm = 0; jm < 16; jm++) {
         if ((uint32_t)jm < n_sel_long) {
@@ -1031,11 +1067,18 @@ __global__ void __launch_bounds__(128) k_seed_fixed(const char *__restrict__ rea
             }
             /* a long seed's four selections are 32 bytes at a 32-byte
              * boundary (rows of 4 (e + 1) selections): two 16-byte stores */
-            uint4 *so4 = reinterpret_cast<uint4 *>(so + 4 * jm);
-            so4[0] = make_uint4(sv[0], sv[1], sv[2], sv[3]);
-            so4[1] = make_uint4(sv[4], sv[5], sv[6], sv[7]);
+            if (jm < 8 && stage_sel) {
+                selv[2 * jm] = make_uint4(sv[0], sv[1], sv[2], sv[3]);
+                selv[2 * jm + 1] = make_uint4(sv[4], sv[5], sv[6], sv[7]);
+            } else {
+                uint4 *so4 = reinterpret_cast<uint4 *>(so + 4 * jm);
+                so4[0] = make_uint4(sv[0], sv[1], sv[2], sv[3]);
+                so4[1] = make_uint4(sv[4], sv[5], sv[6], sv[7]);
+            }
         }
     }
+    if (stage_sel)
+        store_sel_rows(reinterpret_cast<uint4 *>(o.sel + (uint64_t)(r & ~63u) * seln), seln / 2u, selv, stage);
     o.nsel[r] = n_sel_long * SWM_SEED_STEP | (heavy ? NSEL_HEAVY : 0u) | (irr ? NSEL_IRREG : 0u);
     o.npos[r] = npos;
     o.nchunk[r] = nchunk + 1;
