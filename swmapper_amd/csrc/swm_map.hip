@@ -1655,14 +1655,17 @@ __global__ void __launch_bounds__(64) k_plan_pair(uint32_t nreads, const uint2 *
                 if (keep) {
                     const uint32_t at = st1 ? ns1 + (uint32_t)__popcll(m1 & below) : ns0 + (uint32_t)__popcll(m0 & below);
                     const uint32_t val = adjust(v[j].x, meta[j]);
-                    (st1 ? surv1 : surv0)[at] = val;
+                    /* the first 64 of a strand stay in LDS: they reach memory
+                     * only as uniqued values, or below if the batch is large */
                     if (at < 64u) s_surv[st1 ? 1 : 0][at] = val;
+                    else (st1 ? surv1 : surv0)[at] = val;
                 }
                 ns0 += (uint32_t)__popcll(m0);
                 ns1 += (uint32_t)__popcll(m1);
             }
         }
         __syncthreads();
+        uint32_t nu[2] = {0u, 0u};
 #pragma unroll
         for (uint32_t h = 0; h < 2; h++) {
             if (!(h ? light1 : light0)) continue;
@@ -1687,13 +1690,21 @@ __global__ void __launch_bounds__(64) k_plan_pair(uint32_t nreads, const uint2 *
                 }
                 if ((fm >> lane) & 1ull) surv[rank] = v;
                 const uint32_t u = (uint32_t)__popcll(fm);
-                if (lane == 0) { bd[0] = make_uint2(0, u); io.nuniq[r] = u; }
-            } else if (lane == 0) {
-                bd[0] = make_uint2(0, nsurv);
-                io.big[atomicAdd(io.big_count, 1u)] = make_uint2(r, 0);
-                io.nuniq[r] = nsurv;
+                if (lane == 0) bd[0] = make_uint2(0, u);
+                nu[h] = u;
+            } else {
+                surv[lane] = s_surv[h][lane];
+                if (lane == 0) {
+                    bd[0] = make_uint2(0, nsurv);
+                    io.big[atomicAdd(io.big_count, 1u)] = make_uint2(r, 0);
+                }
+                nu[h] = nsurv;
             }
-            if (lane == 0) io.nbatch[r] = 1;
+        }
+        /* both strands' counts in one two-lane store each */
+        if (lane < 2u && (lane ? light1 : light0)) {
+            io.nuniq[2u * rd + lane] = lane ? nu[1] : nu[0];
+            io.nbatch[2u * rd + lane] = 1;
         }
         __syncthreads();
     }
