@@ -140,10 +140,8 @@ FORMS = {
     # stage and the roofline kernels are timed running alone
     "queue": {1: 1, 2: 0, 12: 0},
     # the work-queue pipeline with device-side counts and fixed capacities,
-    # captured once and replayed as a hipGraph (no host round trip per batch);
-    # the heavy-item plan kernels on the slot's second stream beside the
-    # light-item one
-    "graph": {1: 1, 2: 1, 12: 1},
+    # captured once and replayed as a hipGraph (no host round trip per batch)
+    "graph": {1: 1, 2: 1, 12: 0},
 }
 
 

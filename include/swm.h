@@ -293,8 +293,8 @@ int swm_sam_close(swm_sam_t *w);
 #define SWM_OPT_GIANT_POS 9         /* heavy plan items with more positions go first (1..2^24, default 16384) */
 #define SWM_OPT_GIANT_CANDS 10      /* heavy emission reads with more candidates go first (1..2^24, default 16384) */
 #define SWM_OPT_SEED_GENERIC 11     /* 1: the seed kernel for any k, also where a fixed-k one exists (k = 12); default 0 */
-#define SWM_OPT_PLAN_OVERLAP 12     /* 1 (default): work-queue form runs the heavy-item plan kernels on a second
-                                     * stream beside the light-item kernel; 0: after it */
+#define SWM_OPT_PLAN_OVERLAP 12     /* 1: work-queue form runs the heavy-item plan kernels on a second stream
+                                     * beside the light-item kernel; 0 (default): after it */
 #define SWM_OPT_PLAN_LIGHT_WAVES 13 /* waves per CU of the light-item plan kernel (8..64, default 32) */
 #define SWM_N_OPTIONS 13
 

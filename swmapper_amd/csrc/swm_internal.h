@@ -60,7 +60,7 @@ struct swm_opts {
     uint32_t giant_pos = 16384;     /* heavy plan items with more positions are taken first */
     uint32_t giant_cands = 16384;   /* heavy emission reads with more candidates are taken first */
     uint32_t seed_generic = 0;      /* 1: the seed kernel for any k even where a fixed-k one exists */
-    uint32_t plan_overlap = 1;      /* heavy-item plan kernels on the slot's second stream (work queue) */
+    uint32_t plan_overlap = 0;      /* 1: heavy-item plan kernels on the slot's second stream (work queue) */
     uint32_t plan_light_waves = 32; /* waves per CU of the light-item plan kernel */
 };
 

@@ -1121,6 +1121,7 @@ struct PlanIO {
     uint32_t *merge;            /* heavy batches with a descending chunk (hbatch indices), for k_merge */
     uint32_t *merge_count;
     unsigned long long *light_pos;  /* bucket entries fetched by k_plan_pair */
+    uint32_t heavy_cands;       /* k_compact's threshold: items with more candidates read their descriptors */
 };
 
 /* Bit 31 of a device bucket entry's second word: its position is smaller
@@ -1690,7 +1691,9 @@ __global__ void __launch_bounds__(64) k_plan_pair(uint32_t nreads, const uint2 *
                 }
                 if ((fm >> lane) & 1ull) surv[rank] = v;
                 const uint32_t u = (uint32_t)__popcll(fm);
-                if (lane == 0) bd[0] = make_uint2(0, u);
+                /* a one-batch item's descriptor is read only by
+                 * k_compact_heavy (k_compact takes it from the count) */
+                if (lane == 0 && u > io.heavy_cands) bd[0] = make_uint2(0, u);
                 nu[h] = u;
             } else {
                 surv[lane] = s_surv[h][lane];
@@ -4383,6 +4386,7 @@ static int enq_b(const swm_ctx_t *ctx, swm_slot *sl, Pipe &P, uint64_t cap_cand,
         }
         if (ev) SWM_HIP_CHECK(hipEventRecord(ev[6], st));
         const uint32_t pgrid = (uint32_t)std::min<uint64_t>(P.n ? P.n : 1, (uint64_t)ctx->n_cu * o.plan_light_waves);
+        P.pio.heavy_cands = o.heavy_cands;      /* as k_compact gets it in the work-queue form */
         if (P.seln <= 32)       /* seln <= 64 for every e */
             k_plan_pair<32><<<pgrid, 64, 0, st>>>((uint32_t)P.n, ctx->d_pl, P.pio);
         else

@@ -384,7 +384,7 @@ extern "C" int swm_autotune(swm_ctx_t *ctx, const char *reads, uint64_t stride, 
         {SWM_OPT_MYERS_BLOCK, {256, 128, 64}},
         {SWM_OPT_HEAVY_CANDS, {64, 32, 128, 256, 512}},
         {SWM_OPT_PLAN_HEAVY_WAVES, {32, 8, 16}},
-        {SWM_OPT_PLAN_OVERLAP, {1, 0}},
+        {SWM_OPT_PLAN_OVERLAP, {0, 1}},
         {SWM_OPT_PLAN_LIGHT_WAVES, {32, 64}},
         {SWM_OPT_EMIT_HEAVY_WAVES, {32, 8, 16, 64}},
     };

@@ -50,9 +50,9 @@ SETTINGS = [
     {swm.OPT_GRAPH: 1, swm.OPT_WORK_QUEUE: 0},
     {swm.OPT_MYERS_PACKED: 0},
     {swm.OPT_MYERS_PACKED: 1, swm.OPT_MYERS_BLOCK: 64},
-    {swm.OPT_PLAN_OVERLAP: 0},                     # heavy plan kernels after the light-item one, one stream
+    {swm.OPT_PLAN_OVERLAP: 1},                     # heavy plan kernels on a second stream beside the light-item one
     {swm.OPT_PLAN_LIGHT_WAVES: 64, swm.OPT_PLAN_HEAVY_WAVES: 8},
-    {swm.OPT_GRAPH: 1, swm.OPT_PLAN_LIGHT_WAVES: 8},   # the fork and join inside a captured graph
+    {swm.OPT_GRAPH: 1, swm.OPT_PLAN_OVERLAP: 1, swm.OPT_PLAN_LIGHT_WAVES: 8},   # the fork and join inside a captured graph
 ]
 
 
@@ -153,7 +153,7 @@ def test_option_ranges(tandem):
     gi, _, _, _ = tandem
     m = swm.Mapper.from_index(gi)
     defaults = m.options()
-    assert defaults["plan_overlap"] == 1 and defaults["plan_light_waves"] == 32, defaults
+    assert defaults["plan_overlap"] == 0 and defaults["plan_light_waves"] == 32, defaults
     bad = {swm.OPT_PLAN_OVERLAP: [2, -1], swm.OPT_PLAN_LIGHT_WAVES: [0, 7, 65],
            swm.OPT_PLAN_HEAVY_WAVES: [0, 33], swm.OPT_SEED_GENERIC: [2]}
     for opt, vals in bad.items():
@@ -162,7 +162,7 @@ def test_option_ranges(tandem):
                 m.set_option(opt, v)
             assert ex.value.rc == -1, (swm.OPTION_NAMES[opt], v)
     assert m.options() == defaults
-    for opt, v in ((swm.OPT_PLAN_OVERLAP, 0), (swm.OPT_PLAN_LIGHT_WAVES, 8), (swm.OPT_PLAN_LIGHT_WAVES, 64)):
+    for opt, v in ((swm.OPT_PLAN_OVERLAP, 1), (swm.OPT_PLAN_LIGHT_WAVES, 8), (swm.OPT_PLAN_LIGHT_WAVES, 64)):
         m.set_option(opt, v)
         assert m.get_option(opt) == v
     m.close()
