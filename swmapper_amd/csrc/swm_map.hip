@@ -193,6 +193,7 @@ struct SeedOut {
     uint32_t *nchunk;   /* [nrs + 1] sum of ceil(cnt / 32), + 1 */
     unsigned long long *counters;   /* the run's counters, zeroed here */
     uint32_t *nuniq;    /* [nrs + 1]: its last entry is zeroed here */
+    uint32_t *nbatch;   /* [nrs]: 1 here, the batch count of an item planned in more (k_plan_pair leaves it) */
     uint32_t *lbins;    /* [LBIN_ROWS][256] candidates per read length, zeroed here */
     const uint32_t *irreg;   /* start offsets of the buckets not in ascending order, ascending */
     uint32_t n_irreg;        /* 0 when the offsets carry the flag instead */
@@ -668,7 +669,8 @@ __global__ void __launch_bounds__(128) k_seed(const char *__restrict__ reads, ui
      * item needs the seed heap (k_plan_heavy) */
     o.nsel[r] = n_sel_long * SWM_SEED_STEP | (heavy ? NSEL_HEAVY : 0u) | (irr ? NSEL_IRREG : 0u);
     o.npos[r] = npos;
-    o.nchunk[r] = nchunk + 1;   /* batches <= chunks + 1 (an empty first batch) */
+    o.nchunk[r] = nchunk + 1;
+    o.nbatch[r] = 1;   /* batches <= chunks + 1 (an empty first batch) */
 }
 
 /* ---- the seed kernel for a seed length fixed at compile time ----
@@ -1082,6 +1084,7 @@ __global__ void __launch_bounds__(128) k_seed_fixed(const char *__restrict__ rea
     o.nsel[r] = n_sel_long * SWM_SEED_STEP | (heavy ? NSEL_HEAVY : 0u) | (irr ? NSEL_IRREG : 0u);
     o.npos[r] = npos;
     o.nchunk[r] = nchunk + 1;
+    o.nbatch[r] = 1;
 }
 
 /* ------------------------------------------------------------ plan kernels */
@@ -1704,11 +1707,9 @@ __global__ void __launch_bounds__(64) k_plan_pair(uint32_t nreads, const uint2 *
                 nu[h] = nsurv;
             }
         }
-        /* both strands' counts in one two-lane store each */
-        if (lane < 2u && (lane ? light1 : light0)) {
-            io.nuniq[2u * rd + lane] = lane ? nu[1] : nu[0];
-            io.nbatch[2u * rd + lane] = 1;
-        }
+        /* both strands' counts in one two-lane store (their batch counts,
+         * 1, were written by the seed kernel) */
+        if (lane < 2u && (lane ? light1 : light0)) io.nuniq[2u * rd + lane] = lane ? nu[1] : nu[0];
         __syncthreads();
     }
     if (lane == 0 && fetched) atomicAdd(io.light_pos, (unsigned long long)fetched);
@@ -4261,6 +4262,7 @@ static int pipe_alloc_a(Alloc *ws, uint64_t n, uint32_t seln, Pipe &P)
     P.so.counters = P.counters;
     P.so.lbins = P.lbins;
     P.so.nuniq = P.pio.nuniq;
+    P.so.nbatch = P.pio.nbatch;
     P.u32c = reinterpret_cast<uint32_t *>(P.counters + 4);
     P.pio.sel = P.so.sel;
     P.pio.selc = P.so.selc;
